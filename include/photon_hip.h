@@ -1,0 +1,238 @@
+/* photon_hip.h — C-ABI of libphoton_hip.so, the MI355X (gfx950) wavefront path tracer that stands in for
+ * PhotonEngine's integrator hot path behind its RenderDevice interface.
+ *
+ * Every entry point returns an int status (PE_OK = 0, negative on error); the message of the last error on
+ * the calling thread is available from pe_last_error(). No C++ types and no exceptions cross this boundary.
+ * All pointers are host pointers unless stated otherwise; arrays are tightly packed little-endian.
+ *
+ * Reference interface each group replaces (paths relative to the PhotonEngine source tree):
+ *   device registry        engine/render_device.h:88-94     enumerateDevices(), createDevice(id, &err)
+ *   device info/features   engine/render_device.h:37-56     DeviceInfo, DeviceFeatures
+ *   scene upload/update    engine/render_device.h:72-80     RenderDevice::setScene / updateScene / hasScene
+ *   render                 engine/render_device.h:84-85     RenderDevice::render(cam, settings, targets, first, count, cancel)
+ *   film                   core/image/film.h:26-70          Film sums / counts / lum^2 (addBatch semantics, film.cpp:119-137)
+ *   sampler                samplers/sobol_sampler.cpp:92-146 SobolSampler::startPixel/startSample/get2D/get1D
+ *   closest hit / any hit  engine/scene.cpp:215-221         Scene::intersect / Scene::intersectAny (BVH::intersect, bvh.cpp:322-446)
+ */
+#ifndef PHOTON_HIP_H
+#define PHOTON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PE_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------------------------------------- */
+#define PE_OK 0
+#define PE_ERR_INVALID (-1)      /* bad argument, shape or handle */
+#define PE_ERR_NO_DEVICE (-2)    /* no usable HIP device with that ordinal */
+#define PE_ERR_HIP (-3)          /* a HIP runtime call failed */
+#define PE_ERR_OOM (-4)          /* device memory allocation failed */
+#define PE_ERR_UNSUPPORTED (-5)  /* feature not implemented on this device */
+#define PE_ERR_CANCELLED (-6)    /* render stopped by the cancel flag (partial result is valid) */
+
+typedef struct pe_device pe_device;
+typedef struct pe_scene pe_scene;
+typedef struct pe_film pe_film;
+
+/* ---- device registry --------------------------------------------------------------------------------
+ * Mirrors DeviceInfo (render_device.h:37-48). kind is always PE_DEVICE_HIP here. */
+#define PE_DEVICE_HIP 6
+typedef struct pe_device_info {
+    char id[32];            /* "hip:0" */
+    char name[128];         /* marketing name of the GPU */
+    char arch[32];          /* "gfx950" */
+    int32_t kind;
+    int32_t compute_units;
+    uint64_t memory_bytes;
+    int32_t usable;         /* 1 when the kernels of this library can run on it */
+    char note[128];
+} pe_device_info;
+
+/* Mirrors DeviceFeatures (render_device.h:52-56). */
+typedef struct pe_device_features {
+    int32_t aovs;
+    int32_t light_passes;
+    int32_t adaptive_sampling;
+} pe_device_features;
+
+const char* pe_last_error(void);
+int pe_abi_version(void);
+int pe_device_count(int* count);                                    /* enumerateDevices() (GPU part) */
+int pe_device_info_get(int ordinal, pe_device_info* out);           /* DeviceInfo */
+int pe_device_create(int ordinal, pe_device** out);                 /* createDevice("hip:N", &err) */
+int pe_device_destroy(pe_device* dev);
+int pe_device_features_get(const pe_device* dev, pe_device_features* out);
+int pe_device_synchronize(pe_device* dev);
+
+/* ---- scene description ------------------------------------------------------------------------------
+ * A flattened engine::Scene (engine/scene.h:30-177). Shapes appear in Scene::shapes() order: a mesh is a
+ * TriangleMesh (mesh.cpp), a "standalone" shape is one heap Triangle such as the two black emissive
+ * triangles that Scene::addLight inserts for an AreaLight (scene.cpp:72-98). Lights appear in
+ * Scene::lights() order; an emissive mesh contributes one PE_LIGHT_TRIANGLE per triangle (scene.cpp:48-62). */
+
+#define PE_MAT_DISNEY 0      /* materials/disney.h DisneyMaterial */
+#define PE_MAT_LAMBERT 1     /* materials/lambertian.h Lambertian */
+#define PE_MAT_DIELECTRIC 2  /* materials/dielectric.h Dielectric */
+
+typedef struct pe_material_desc {
+    int32_t type;
+    float base_color[3];          /* Disney baseColor, Lambert albedo, Dielectric tint */
+    float emission[3];            /* Material::emitted() */
+    float metallic, roughness, specular;
+    float clearcoat, clearcoat_roughness, coat_ior;
+    float coat_tint[3];
+    float ior, specular_transmission, anisotropy;
+    float sheen, sheen_roughness;
+    float sheen_color[3];
+    int32_t has_sheen_color;
+    float diffuse_roughness, diffuse_transmission;
+    float subsurface;
+    float subsurface_radius[3];
+    float subsurface_scale, subsurface_anisotropy;
+    float edge_tint[3];
+    float thin_film_thickness, thin_film_ior;
+    /* Dielectric only (dielectric.h:43-78): Abbe number (0 = no dispersion), interior absorption given as
+     * setAbsorption(color, distance) (sigma_a = -ln(color) / distance, dielectric.cpp:123-144), thin walls. */
+    float abbe;
+    float absorption_color[3];
+    float absorption_distance;
+    int32_t thin_walled;
+    float reserved[5];
+} pe_material_desc;
+
+#define PE_SHAPE_CAMERA_INVISIBLE 1u
+
+typedef struct pe_mesh_desc {
+    const float* positions;     /* 3 * num_vertices */
+    const float* normals;       /* 3 * num_vertices, or NULL (geometric normal) */
+    const float* uvs;           /* 2 * num_vertices, or NULL */
+    const uint32_t* indices;    /* 3 * num_triangles */
+    uint32_t num_vertices;
+    uint32_t num_triangles;
+    int32_t material;           /* index into pe_scene_desc.materials */
+    int32_t emitter_base;       /* lights() index of triangle 0 (emissive mesh / area-light quad), or -1 */
+    int32_t standalone;         /* 1: heap Triangle shape (reported primIndex -1), 0: TriangleMesh */
+    uint32_t flags;             /* PE_SHAPE_* */
+} pe_mesh_desc;
+
+#define PE_LIGHT_AREA 1      /* lights/area_light.h AreaLight(position, u, v, radiance) */
+#define PE_LIGHT_TRIANGLE 2  /* lights/area_light.h TriangleLight(p0, p1, p2, radiance) */
+
+typedef struct pe_light_desc {
+    int32_t type;
+    float p0[3], p1[3], p2[3];  /* AREA: position, u, v ; TRIANGLE: vertices */
+    float radiance[3];
+    int32_t visible_to_camera;
+} pe_light_desc;
+
+/* lights/environment_light.h EnvironmentLight(image, rotation, intensity) [+ setSun]. rgb == NULL and
+ * width == 0 select the flat grey environment (0.5 * intensity). */
+typedef struct pe_env_desc {
+    const float* rgb;           /* 3 * width * height, row 0 = zenith */
+    int32_t width, height;
+    float rotation, intensity;
+    int32_t has_sun;
+    float sun_direction[3];
+    float sun_cos_max;
+    float sun_radiance[3];
+} pe_env_desc;
+
+typedef struct pe_scene_desc {
+    const pe_mesh_desc* meshes;
+    int32_t num_meshes;
+    const pe_material_desc* materials;
+    int32_t num_materials;
+    const pe_light_desc* lights;
+    int32_t num_lights;
+    const pe_env_desc* environment;   /* NULL: no environment light */
+} pe_scene_desc;
+
+#define PE_UPDATE_FULL 0         /* SceneUpdateKind::Full (render_device.h:33) */
+#define PE_UPDATE_MATERIALS 1    /* SceneUpdateKind::Materials */
+#define PE_UPDATE_LIGHTS 2       /* SceneUpdateKind::Lights */
+
+int pe_scene_upload(pe_device* dev, const pe_scene_desc* desc, pe_scene** out);
+int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc);
+int pe_scene_release(pe_scene* scene);
+/* Scene statistics: triangles, BVH nodes, device bytes held. */
+int pe_scene_stats(const pe_scene* scene, uint64_t* triangles, uint64_t* bvh_nodes, uint64_t* device_bytes);
+
+/* ---- camera / settings ------------------------------------------------------------------------------
+ * pe_camera is CameraGpuDesc (camera/camera.h:19-31) of a pinhole PerspectiveCamera. */
+typedef struct pe_camera {
+    float position[3];
+    float lower_left[3];
+    float horizontal[3];
+    float vertical[3];
+    float lens_radius;          /* must be 0: only the pinhole camera is implemented */
+} pe_camera;
+
+#define PE_FILTER_BOX 0             /* core/image/pixel_filter.h PixelFilterType */
+#define PE_FILTER_TRIANGLE 1
+#define PE_FILTER_GAUSSIAN 2
+#define PE_FILTER_BLACKMAN_HARRIS 3
+
+/* The fields of RenderSettings (engine/render_settings.h:19-47) that the integrator reads. Film size is
+ * taken from the film, not from here (renderer.cpp:396-398). */
+typedef struct pe_settings {
+    int32_t max_bounces;        /* RenderSettings::maxBounces */
+    int32_t rr_depth;           /* PathTracer russianRouletteDepth (3 in Renderer) */
+    int32_t shadow_quality;     /* RenderSettings::shadowQuality (NEE samples per vertex) */
+    int32_t filter_type;        /* PE_FILTER_* */
+    float filter_width;
+    float clamp_indirect;       /* 0 = off */
+    uint64_t seed;              /* Sobol seed; the reference renderer hard-codes 12345 */
+    int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together (>=1) */
+    int32_t reserved[7];
+} pe_settings;
+
+/* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray counters. */
+typedef struct pe_batch_result {
+    int32_t completed;
+    uint64_t samples_traced;
+    uint64_t extension_rays;
+    uint64_t shadow_rays;
+} pe_batch_result;
+
+/* ---- film (device-resident accumulation; Film::addSample semantics per sample) ---------------------- */
+int pe_film_create(pe_device* dev, int width, int height, pe_film** out);
+int pe_film_destroy(pe_film* film);
+int pe_film_clear(pe_film* film);
+/* Copies the device film into host arrays (sum_rgb: 3*w*h floats, count: w*h, sum_lum2: w*h; any may be NULL). */
+int pe_film_download(pe_film* film, float* sum_rgb, uint32_t* count, float* sum_lum2, uint64_t* rejected);
+
+/* PerspectiveCamera(position, target, up, vFovDegrees, aspect) -> CameraGpuDesc
+ * (camera/perspective_camera.cpp:11-34, camera.h:19-31). */
+int pe_camera_perspective(const float position[3], const float target[3], const float up[3], float vfov_degrees,
+                          float aspect, pe_camera* out);
+
+/* RenderDevice::render: adds samples [first_sample, first_sample + sample_count) of every pixel into the
+ * film. cancel (may be NULL) is polled between wavefront launches; a set flag ends the call with
+ * completed = 0 and a valid partial film. */
+int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
+              int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result);
+
+/* ---- kernel-level entry points (parity and measurement) ---------------------------------------------
+ * Sobol-Owen sample values: for query i, the sampler state after startPixel(px[i], py[i]),
+ * startSample(index[i]) and dim[i] earlier get1D/get2D calls; out[2i], out[2i+1] receive the u32 bit
+ * patterns x, y whose top 24 bits scaled by 2^-24 are the get2D() floats (get1D() == x). */
+int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
+                   const uint32_t* index, const uint32_t* dim, uint32_t* out);
+
+/* Closest hit for n rays (rays: 8 floats each = origin xyz, direction xyz, tmin, tmax). hits: 4 x 32-bit
+ * each = t (float), shape index (int, -1 = miss), prim index (int, -1 for standalone shapes), and the
+ * global triangle index (int). bary (optional, may be NULL): 2 floats u, v per ray. */
+int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary);
+/* Any hit (shadow rays): occluded[i] = 1 when something lies within [tmin, tmax]. */
+int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHOTON_HIP_H */

@@ -1,0 +1,206 @@
+"""ctypes bindings of the oracle libraries (test infrastructure only).
+
+liboracle.so               the C++ restatement of the reference hot path (oracle_*.cpp)
+_ref/libphoton_ref.so      the reference's own sources built with its own flags (oracle/Makefile.ref)
+_ref/libphoton_ref_strict  the same sources built without floating-point contraction
+
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ORACLE_DIR = Path(__file__).resolve().parent
+ROOT = ORACLE_DIR.parent
+sys.path.insert(0, str(ROOT / "photon-engine_amd" / "python"))
+from photon_hip import capi  # noqa: E402  (ctypes structure definitions shared with the product)
+
+_VP = C.c_void_p
+_P = C.POINTER
+_F = _P(C.c_float)
+_I = _P(C.c_int32)
+_U = _P(C.c_uint32)
+
+
+def _fp(a):
+    return capi.ptr(a, C.c_float)
+
+
+class _Lib:
+    def __init__(self, path: Path, prefix: str, sigs: dict):
+        self.path = path
+        self.lib = C.CDLL(str(path))
+        for name, (res, args) in sigs.items():
+            fn = getattr(self.lib, prefix + name)
+            fn.restype = res
+            fn.argtypes = args
+
+    def __getattr__(self, item):
+        return getattr(self.lib, item)
+
+
+_ORACLE_SIGS = {
+    "sobol_batch": (C.c_int, [C.c_uint64, C.c_size_t, _I, _I, _U, _U, _U]),
+    "reverse_bits": (C.c_uint32, [C.c_uint32]),
+    "sobol_dim1": (C.c_uint32, [C.c_uint32]),
+    "scene_build": (_VP, [_P(capi.SceneDesc)]),
+    "scene_free": (None, [_VP]),
+    "scene_triangles": (C.c_int, [_VP]),
+    "intersect": (C.c_int, [_VP, C.c_size_t, _F, _I, _F]),
+    "intersect_brute": (C.c_int, [_VP, C.c_size_t, _F, _I]),
+    "occluded": (C.c_int, [_VP, C.c_size_t, _F, _P(C.c_uint8)]),
+}
+
+_REF_SIGS = {
+    "sobol_batch": (C.c_int, [C.c_uint64, C.c_size_t, _I, _I, _U, _U, _F]),
+    "scene_build": (_VP, [_P(capi.SceneDesc)]),
+    "scene_free": (None, [_VP]),
+    "scene_counts": (C.c_int, [_VP, _I, _I]),
+    "intersect": (C.c_int, [_VP, C.c_size_t, _F, _I, _F]),
+    "occluded": (C.c_int, [_VP, C.c_size_t, _F, _P(C.c_uint8)]),
+    "camera_describe": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, _F]),
+    "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
+    "bsdf_eval": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F, _F]),
+    "bsdf_sample": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F]),
+    "trace_samples": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_float, C.c_uint64, C.c_size_t, _I, _I, _I, _F]),
+    "render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                         C.c_float, C.c_int, C.c_int, C.c_int, _F, _U, _P(C.c_double)]),
+}
+
+_cache: dict = {}
+
+
+def oracle() -> _Lib:
+    if "oracle" not in _cache:
+        path = ORACLE_DIR / "liboracle.so"
+        if not path.exists():
+            raise FileNotFoundError(f"{path} missing: run __graft_entry__.build()")
+        _cache["oracle"] = _Lib(path, "oracle_", _ORACLE_SIGS)
+    return _cache["oracle"]
+
+
+def reference(strict: bool = False) -> _Lib | None:
+    """The reference's own code (None where oracle/_ref was not built)."""
+    key = "ref_strict" if strict else "ref"
+    if key not in _cache:
+        path = ORACLE_DIR / "_ref" / ("libphoton_ref_strict.so" if strict else "libphoton_ref.so")
+        _cache[key] = _Lib(path, "ref_", _REF_SIGS) if path.exists() else None
+    return _cache[key]
+
+
+class OracleScene:
+    """The restated scene (binary BVH built as the reference builds it)."""
+
+    def __init__(self, scene):
+        desc, self._keep = scene.pack()
+        self._lib = oracle()
+        self.h = self._lib.oracle_scene_build(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.oracle_scene_free(self.h)
+            self.h = None
+
+    def intersect(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.int32)
+        bary = np.zeros((n, 2), np.float32)
+        self._lib.oracle_intersect(self.h, n, _fp(rays), capi.ptr(hits, C.c_int32), _fp(bary))
+        return hits, bary
+
+    def intersect_brute(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        hits = np.zeros((rays.shape[0], 4), np.int32)
+        self._lib.oracle_intersect_brute(self.h, rays.shape[0], _fp(rays), capi.ptr(hits, C.c_int32))
+        return hits
+
+    def occluded(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        occ = np.zeros(rays.shape[0], np.uint8)
+        self._lib.oracle_occluded(self.h, rays.shape[0], _fp(rays), capi.ptr(occ, C.c_uint8))
+        return occ
+
+
+class RefScene:
+    """engine::Scene built by the reference's own code from the same descriptor."""
+
+    def __init__(self, scene, strict: bool = False):
+        self._lib = reference(strict)
+        if self._lib is None:
+            raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
+        desc, self._keep = scene.pack()
+        self.h = self._lib.ref_scene_build(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.ref_scene_free(self.h)
+            self.h = None
+
+    def counts(self):
+        s, l = C.c_int32(), C.c_int32()
+        self._lib.ref_scene_counts(self.h, C.byref(s), C.byref(l))
+        return s.value, l.value
+
+    def intersect(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.int32)
+        geom = np.zeros((n, 9), np.float32)
+        self._lib.ref_intersect(self.h, n, _fp(rays), capi.ptr(hits, C.c_int32), _fp(geom))
+        return hits, geom
+
+    def occluded(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        occ = np.zeros(rays.shape[0], np.uint8)
+        self._lib.ref_occluded(self.h, rays.shape[0], _fp(rays), capi.ptr(occ, C.c_uint8))
+        return occ
+
+    def trace_samples(self, view, px, py, s, shadow_quality=1, filter_type=0, filter_width=1.0, seed=12345):
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self._lib.ref_trace_samples(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
+                                    view.width, view.height, view.max_bounces, shadow_quality, filter_type,
+                                    filter_width, seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
+                                    capi.ptr(s, C.c_int32), _fp(L))
+        return L
+
+    def render(self, view, first, count, threads, shadow_quality=1, filter_type=0, filter_width=1.0):
+        w, h = view.width, view.height
+        mean = np.zeros((h, w, 3), np.float32)
+        cnt = np.zeros((h, w), np.uint32)
+        secs = C.c_double()
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self._lib.ref_render(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect, w, h,
+                             view.max_bounces, shadow_quality, filter_type, filter_width, first, count, threads,
+                             _fp(mean), capi.ptr(cnt, C.c_uint32), C.byref(secs))
+        return mean, cnt, secs.value
+
+
+def sobol_oracle(seed, px, py, idx, dim):
+    px, py = np.ascontiguousarray(px, np.int32), np.ascontiguousarray(py, np.int32)
+    idx, dim = np.ascontiguousarray(idx, np.uint32), np.ascontiguousarray(dim, np.uint32)
+    out = np.zeros((px.shape[0], 2), np.uint32)
+    oracle().oracle_sobol_batch(seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
+                                capi.ptr(idx, C.c_uint32), capi.ptr(dim, C.c_uint32), capi.ptr(out, C.c_uint32))
+    return out
+
+
+def sobol_reference(seed, px, py, idx, dim):
+    lib = reference()
+    if lib is None:
+        return None
+    px, py = np.ascontiguousarray(px, np.int32), np.ascontiguousarray(py, np.int32)
+    idx, dim = np.ascontiguousarray(idx, np.uint32), np.ascontiguousarray(dim, np.uint32)
+    out = np.zeros((px.shape[0], 2), np.float32)
+    lib.ref_sobol_batch(seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
+                        capi.ptr(idx, C.c_uint32), capi.ptr(dim, C.c_uint32), _fp(out))
+    return out

@@ -1,0 +1,346 @@
+// ref_driver.cpp — test-only harness around the PhotonEngine reference sources.
+//
+// This file is part of the oracle (test infrastructure): it is compiled together with the reference's
+// own hot-path translation units, in place under the reference tree, into oracle/_ref/libphoton_ref.so
+// (recipe: oracle/Makefile.ref). Nothing in the product links it. It exposes a small C API that the
+// tests and bench.py's CPU-baseline leg call through ctypes: Sobol values, scene build from the same
+// flattened pe_scene_desc the HIP library takes, closest/any hit on ray batches, the microfacet energy
+// tables, Disney BSDF eval/sample, per-sample radiance and Renderer::renderSamples itself.
+#include "engine/renderer.h"
+#include "engine/scene.h"
+#include "camera/perspective_camera.h"
+#include "core/image/pixel_filter.h"
+#include "geometry/mesh.h"
+#include "integrators/path_tracer.h"
+#include "lights/area_light.h"
+#include "lights/environment_light.h"
+#include "materials/dielectric.h"
+#include "materials/disney.h"
+#include "materials/lambertian.h"
+#include "materials/microfacet.h"
+#include "samplers/sobol_sampler.h"
+
+#include "photon_hip.h"
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <unordered_map>
+#include <vector>
+
+using namespace photon;
+
+namespace {
+
+Color3f c3(const float* v) { return Color3f(v[0], v[1], v[2]); }
+Vec3f v3(const float* v) { return Vec3f(v[0], v[1], v[2]); }
+
+std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
+    if (d.type == PE_MAT_LAMBERT) return std::make_shared<Lambertian>(c3(d.base_color), c3(d.emission));
+    if (d.type == PE_MAT_DIELECTRIC) {
+        auto m = std::make_shared<Dielectric>(d.ior, c3(d.base_color), d.roughness);
+        if (d.abbe > 0.0f) m->setAbbe(d.abbe);
+        if (d.absorption_distance > 0.0f) m->setAbsorption(c3(d.absorption_color), d.absorption_distance);
+        m->setThinWalled(d.thin_walled != 0);
+        return m;
+    }
+    auto m = std::make_shared<DisneyMaterial>(c3(d.base_color), d.metallic, d.roughness, d.specular);
+    m->setClearCoat(d.clearcoat);
+    m->setClearCoatRoughness(d.clearcoat_roughness);
+    m->setCoatIor(d.coat_ior);
+    m->setCoatTint(c3(d.coat_tint));
+    m->setIor(d.ior);
+    m->setSpecularTransmission(d.specular_transmission);
+    m->setAnisotropy(d.anisotropy);
+    m->setSheen(d.sheen);
+    m->setSheenRoughness(d.sheen_roughness);
+    if (d.has_sheen_color) m->setSheenColor(c3(d.sheen_color));
+    m->setDiffuseRoughness(d.diffuse_roughness);
+    m->setDiffuseTransmission(d.diffuse_transmission);
+    m->setSubsurface(d.subsurface);
+    m->setSubsurfaceRadius(c3(d.subsurface_radius));
+    m->setSubsurfaceScale(d.subsurface_scale);
+    m->setSubsurfaceAnisotropy(d.subsurface_anisotropy);
+    m->setEdgeTint(c3(d.edge_tint));
+    m->setThinFilm(d.thin_film_thickness, d.thin_film_ior);
+    m->setEmission(c3(d.emission));
+    return m;
+}
+
+struct RefScene {
+    std::shared_ptr<Scene> scene;
+    std::vector<std::shared_ptr<Material>> materials;
+    std::unordered_map<const void*, int> shapeIndex;
+};
+
+bool toRay(const float* r, Ray& ray) {
+    ray = Ray(Vec3f(r[0], r[1], r[2]), Vec3f(r[3], r[4], r[5]), r[6], r[7]);
+    return true;
+}
+
+SurfaceInteraction shadingPoint(const float* n, const float* t, int frontFace) {
+    SurfaceInteraction si;
+    si.point = Vec3f(0.0f);
+    si.normal = v3(n);
+    si.ng = v3(n);
+    si.tangent = v3(t);
+    si.frontFace = frontFace != 0;
+    return si;
+}
+
+}  // namespace
+
+#define REF_API __attribute__((visibility("default")))
+
+extern "C" {
+
+// SobolSampler(seed): startPixel(px, py), startSample(index), `dim` get1D() calls, then get2D().
+REF_API int ref_sobol_batch(uint64_t seed, size_t n, const int32_t* px, const int32_t* py, const uint32_t* index,
+                    const uint32_t* dim, float* out) {
+    SobolSampler s(seed);
+    for (size_t i = 0; i < n; ++i) {
+        s.startPixel(px[i], py[i]);
+        s.startSample(static_cast<int>(index[i]));
+        for (uint32_t d = 0; d < dim[i]; ++d) (void)s.get1D();
+        const Vec2f v = s.get2D();
+        out[2 * i] = v.x;
+        out[2 * i + 1] = v.y;
+    }
+    return 0;
+}
+
+REF_API void* ref_scene_build(const pe_scene_desc* desc) {
+    Scene::setPreferOwnBvh(true);
+    auto rs = new RefScene();
+    rs->scene = std::make_shared<Scene>();
+    for (int i = 0; i < desc->num_materials; ++i) rs->materials.push_back(makeMaterial(desc->materials[i]));
+    std::vector<char> lightAdded(static_cast<size_t>(desc->num_lights), 0);
+    for (int i = 0; i < desc->num_meshes; ++i) {
+        const pe_mesh_desc& m = desc->meshes[i];
+        if (m.standalone) {
+            const int li = m.emitter_base;
+            if (li < 0 || li >= desc->num_lights || desc->lights[li].type != PE_LIGHT_AREA) continue;
+            if (lightAdded[static_cast<size_t>(li)]) continue;
+            lightAdded[static_cast<size_t>(li)] = 1;
+            const pe_light_desc& L = desc->lights[li];
+            auto area = std::make_shared<AreaLight>(v3(L.p0), v3(L.p1), v3(L.p2), c3(L.radiance));
+            area->setVisibleToCamera(L.visible_to_camera != 0);
+            rs->scene->addLight(area);
+            continue;
+        }
+        std::vector<Vec3f> pos(m.num_vertices), nrm;
+        std::vector<Vec2f> uv;
+        for (uint32_t v = 0; v < m.num_vertices; ++v) pos[v] = v3(m.positions + 3 * v);
+        if (m.normals) {
+            nrm.resize(m.num_vertices);
+            for (uint32_t v = 0; v < m.num_vertices; ++v) nrm[v] = v3(m.normals + 3 * v);
+        }
+        if (m.uvs) {
+            uv.resize(m.num_vertices);
+            for (uint32_t v = 0; v < m.num_vertices; ++v) uv[v] = Vec2f(m.uvs[2 * v], m.uvs[2 * v + 1]);
+        }
+        std::vector<uint32_t> idx(m.indices, m.indices + 3 * static_cast<size_t>(m.num_triangles));
+        rs->scene->addShape(std::make_shared<TriangleMesh>(pos, nrm, uv, idx, rs->materials[static_cast<size_t>(m.material)].get()));
+    }
+    for (auto& mat : rs->materials) rs->scene->retainMaterial(mat);
+    if (desc->environment) {
+        const pe_env_desc& e = *desc->environment;
+        std::shared_ptr<Image> img;
+        if (e.rgb && e.width > 0 && e.height > 0) {
+            img = std::make_shared<Image>(e.width, e.height);
+            for (int y = 0; y < e.height; ++y)
+                for (int x = 0; x < e.width; ++x) {
+                    const float* p = e.rgb + 3 * (static_cast<size_t>(y) * e.width + x);
+                    img->setPixel(x, y, Color3f(p[0], p[1], p[2]));
+                }
+        }
+        auto env = std::make_shared<EnvironmentLight>(img, e.rotation, e.intensity);
+        if (e.has_sun) env->setSun(v3(e.sun_direction), e.sun_cos_max, c3(e.sun_radiance));
+        rs->scene->setEnvironment(env);
+    }
+    rs->scene->buildAccelerator();
+    const auto& shapes = rs->scene->shapes();
+    for (size_t k = 0; k < shapes.size(); ++k) rs->shapeIndex[shapes[k].get()] = static_cast<int>(k);
+    return rs;
+}
+
+REF_API void ref_scene_free(void* h) { delete static_cast<RefScene*>(h); }
+
+REF_API int ref_scene_counts(void* h, int* shapes, int* lights) {
+    auto* rs = static_cast<RefScene*>(h);
+    *shapes = static_cast<int>(rs->scene->shapes().size());
+    *lights = static_cast<int>(rs->scene->lights().size());
+    return 0;
+}
+
+// hits: t, shape index (-1 miss), primIndex, 0 ; bary/extra: point xyz, ng xyz, shading normal xyz.
+REF_API int ref_intersect(void* h, size_t n, const float* rays, int32_t* hits, float* geom) {
+    auto* rs = static_cast<RefScene*>(h);
+    for (size_t i = 0; i < n; ++i) {
+        Ray ray;
+        toRay(rays + 8 * i, ray);
+        SurfaceInteraction si;
+        int32_t* o = hits + 4 * i;
+        if (rs->scene->intersect(ray, si)) {
+            float t = si.t;
+            std::memcpy(&o[0], &t, 4);
+            auto it = rs->shapeIndex.find(si.hitObject);
+            o[1] = it == rs->shapeIndex.end() ? -2 : it->second;
+            o[2] = si.primIndex;
+            o[3] = 0;
+            if (geom) {
+                float* g = geom + 9 * i;
+                g[0] = si.point.x; g[1] = si.point.y; g[2] = si.point.z;
+                g[3] = si.ng.x; g[4] = si.ng.y; g[5] = si.ng.z;
+                g[6] = si.normal.x; g[7] = si.normal.y; g[8] = si.normal.z;
+            }
+        } else {
+            const float t = -1.0f;
+            std::memcpy(&o[0], &t, 4);
+            o[1] = -1;
+            o[2] = -1;
+            o[3] = 0;
+        }
+    }
+    return 0;
+}
+
+REF_API int ref_occluded(void* h, size_t n, const float* rays, uint8_t* occ) {
+    auto* rs = static_cast<RefScene*>(h);
+    for (size_t i = 0; i < n; ++i) {
+        Ray ray;
+        toRay(rays + 8 * i, ray);
+        occ[i] = rs->scene->intersectAny(ray) ? 1 : 0;
+    }
+    return 0;
+}
+
+// PerspectiveCamera(...).describeGpu(): position, lowerLeft, horizontal, vertical.
+REF_API int ref_camera_describe(const float* pos, const float* target, const float* up, float fov, float aspect, float* out) {
+    PerspectiveCamera cam(v3(pos), v3(target), v3(up), fov, aspect);
+    const CameraGpuDesc d = cam.describeGpu();
+    const Vec3f vs[4] = {d.position, d.lowerLeft, d.horizontal, d.vertical};
+    for (int k = 0; k < 4; ++k) {
+        out[3 * k] = vs[k].x;
+        out[3 * k + 1] = vs[k].y;
+        out[3 * k + 2] = vs[k].z;
+    }
+    return 0;
+}
+
+// which: 0 ggxDirectionalAlbedo(a, b), 1 ggxSchlickAlbedo(a, b, c), 2 ggxSchlickAverageAlbedo(a, b),
+//        3 charlieSheenAlbedo(a, b)
+REF_API int ref_microfacet(int which, size_t n, const float* a, const float* b, const float* c, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = ggxDirectionalAlbedo(a[i], b[i]); break;
+            case 1: out[i] = ggxSchlickAlbedo(a[i], b[i], c[i]); break;
+            case 2: out[i] = ggxSchlickAverageAlbedo(a[i], b[i]); break;
+            case 3: out[i] = charlieSheenAlbedo(a[i], b[i]); break;
+            default: return -1;
+        }
+    }
+    return 0;
+}
+
+// Material::evalPdf for n (normal, tangent, frontFace, wo, wi) tuples; f: 3 floats, pdf: 1 float each.
+REF_API int ref_bsdf_eval(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
+                  const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+    auto mat = makeMaterial(*md);
+    for (size_t i = 0; i < n; ++i) {
+        SurfaceInteraction si = shadingPoint(normal + 3 * i, tangent + 3 * i, frontFace[i]);
+        si.material = mat.get();
+        float p = 0.0f;
+        const Color3f v = mat->evalPdf(v3(wo + 3 * i), v3(wi + 3 * i), si, p);
+        f[3 * i] = v.r; f[3 * i + 1] = v.g; f[3 * i + 2] = v.b;
+        pdf[i] = p;
+    }
+    return 0;
+}
+
+// Material::sampleFull; out per sample: ok, wi xyz, f rgb, pdf, delta, subsurface, passThrough, weight rgb (14 floats).
+REF_API int ref_bsdf_sample(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
+                    const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+    auto mat = makeMaterial(*md);
+    for (size_t i = 0; i < n; ++i) {
+        SurfaceInteraction si = shadingPoint(normal + 3 * i, tangent + 3 * i, frontFace[i]);
+        si.material = mat.get();
+        BSDFSample bs;
+        const bool ok = mat->sampleFull(v3(wo + 3 * i), si, u3[3 * i], Vec2f(u3[3 * i + 1], u3[3 * i + 2]), bs);
+        float* o = out + 14 * i;
+        o[0] = ok ? 1.0f : 0.0f;
+        o[1] = bs.wi.x; o[2] = bs.wi.y; o[3] = bs.wi.z;
+        o[4] = bs.f.r; o[5] = bs.f.g; o[6] = bs.f.b;
+        o[7] = bs.pdf;
+        o[8] = bs.delta ? 1.0f : 0.0f;
+        o[9] = bs.subsurface ? 1.0f : 0.0f;
+        o[10] = bs.passThrough ? 1.0f : 0.0f;
+        o[11] = bs.weight.r; o[12] = bs.weight.g; o[13] = bs.weight.b;
+    }
+    return 0;
+}
+
+// Per-sample radiance along the path of tracePixel (renderer.cpp:139-162) with no AOVs: the Sobol sampler
+// is cloned with `seed`, the pixel filter importance-samples the sub-pixel offset (dims 0-1), the lens
+// sample takes dims 2-3 and PathTracer::Li(maxDepth, rrDepth 3, ao 0, shadowQuality) traces the path.
+REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
+                      float aspect, int width, int height, int maxBounces, int shadowQuality, int filterType,
+                      float filterWidth, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
+                      const int32_t* sampleIndex, float* L) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    PixelFilter pf;
+    pf.type = static_cast<PixelFilterType>(filterType);
+    pf.width = filterWidth;
+    const PixelFilterSampler filter(pf);
+    PathTracer integrator(maxBounces, 3, 0.0f, shadowQuality);
+    SobolSampler sampler(seed);
+    for (size_t i = 0; i < n; ++i) {
+        sampler.startPixel(px[i], py[i]);
+        sampler.startSample(sampleIndex[i]);
+        const Vec2f offset = filter.sample(sampler.get2D());
+        const float sx = (static_cast<float>(px[i]) + 0.5f + offset.x) / static_cast<float>(width);
+        const float sy = 1.0f - (static_cast<float>(py[i]) + 0.5f + offset.y) / static_cast<float>(height);
+        const Vec2f lens = sampler.get2D();
+        Ray ray = cam.generateRay(sx, sy, lens);
+        ray.coneWidth = cam.pixelConeWidth(height);
+        ray.coneSpread = cam.pixelConeSpread(height);
+        const Color3f c = integrator.Li(ray, *rs->scene, sampler);
+        L[3 * i] = c.r; L[3 * i + 1] = c.g; L[3 * i + 2] = c.b;
+    }
+    return 0;
+}
+
+// Renderer::renderSamples into a fresh Film; writes the resolved mean (3 floats/px) and sample counts.
+// Returns the wall time in seconds of renderSamples alone through *seconds.
+REF_API int ref_render(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov, float aspect,
+               int width, int height, int maxBounces, int shadowQuality, int filterType, float filterWidth,
+               int firstSample, int count, int threads, float* mean, uint32_t* counts, double* seconds) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = count;
+    s.maxBounces = maxBounces;
+    s.shadowQuality = shadowQuality;
+    s.numThreads = threads;
+    s.filter.type = static_cast<PixelFilterType>(filterType);
+    s.filter.width = filterWidth;
+    Film film(width, height);
+    Renderer renderer;
+    const auto t0 = std::chrono::steady_clock::now();
+    const SampleBatchResult r = renderer.renderSamples(*rs->scene, cam, s, film, firstSample, count, nullptr);
+    const auto t1 = std::chrono::steady_clock::now();
+    if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * width + x;
+            const Color3f c = film.resolvedPixel(x, y);
+            if (mean) { mean[3 * k] = c.r; mean[3 * k + 1] = c.g; mean[3 * k + 2] = c.b; }
+            if (counts) counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+        }
+    return r.completed ? 0 : 1;
+}
+
+}  // extern "C"

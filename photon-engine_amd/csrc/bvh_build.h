@@ -1,0 +1,31 @@
+// bvh_build.h — host-side SAH build of the two-wide, child-boxes-in-parent BVH that the gfx950
+// traversal kernels read (layout in pe_gpu_types.h).
+#pragma once
+
+#include "pe_gpu_types.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+namespace pe {
+
+struct BvhBuildOptions {
+    int max_leaf = 4;     // largest leaf the SAH may choose
+    int max_depth = 60;   // interior depth cap (the traversal stack holds kStackDepth entries)
+    int bins = 16;        // SAH bins per axis (all three axes are tried)
+    float traversal_cost = 1.0f;
+    float triangle_cost = 1.0f;
+};
+
+struct BvhBuildResult {
+    std::vector<BvhNode> nodes;      // interior nodes, depth-first, root at 0
+    std::vector<uint32_t> order;     // leaf order -> original triangle index
+    int depth = 0;                   // deepest interior level (root = 1)
+    size_t leaves = 0;
+};
+
+// tri_lo/tri_hi: per-triangle bounds (3 floats each). Never fails for n >= 1.
+BvhBuildResult build_bvh(const float* tri_lo, const float* tri_hi, size_t n, const BvhBuildOptions& opt);
+
+}  // namespace pe

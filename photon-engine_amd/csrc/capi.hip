@@ -1,0 +1,363 @@
+// capi.hip — the C-ABI of libphoton_hip.so: device registry, scene upload, film, and the kernel-level
+// batch entry points (Sobol values, closest hit, any hit). The wavefront integrator behind pe_render
+// lives in wavefront.hip.
+#include "photon_hip.h"
+#include "bvh_build.h"
+#include "pe_internal.h"
+#include "pe_scene_host.h"
+#include "pe_sobol.h"
+#include "pe_traverse.h"
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+
+namespace pe {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+bool DevBuf::alloc(size_t bytes) {
+    release();
+    if (bytes == 0) return true;
+    void* p = nullptr;
+    const hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        set_error("hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+        return false;
+    }
+    m_ptr = p;
+    m_bytes = bytes;
+    return true;
+}
+
+bool DevBuf::upload(const void* src, size_t bytes, hipStream_t s) {
+    if (!alloc(bytes)) return false;
+    if (bytes == 0) return true;
+    const hipError_t e = hipMemcpyAsync(m_ptr, src, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMemcpyAsync failed: ") + hipGetErrorString(e));
+        return false;
+    }
+    return true;
+}
+
+void DevBuf::release() {
+    if (m_ptr) (void)hipFree(m_ptr);
+    m_ptr = nullptr;
+    m_bytes = 0;
+}
+
+// ---------------------------------------------------------------------------------------------------------
+// Kernels of the batch entry points.
+
+__global__ void __launch_bounds__(256) k_sobol_batch(uint64_t seed, size_t n, const int32_t* __restrict__ px,
+                                                     const int32_t* __restrict__ py, const uint32_t* __restrict__ index,
+                                                     const uint32_t* __restrict__ dim, uint2* __restrict__ out) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t ph = sobol_pixel_hash(seed, px[i], py[i]);
+        uint32_t xb, yb;
+        sobol_2d_bits(ph, rev_bits(index[i]), dim[i], xb, yb);
+        out[i] = make_uint2(xb, yb);
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode* __restrict__ nodes,
+                                                                 const TriAccel* __restrict__ tris,
+                                                                 const TriShade* __restrict__ shade, size_t n,
+                                                                 const float* __restrict__ rays,
+                                                                 int4* __restrict__ hits, float2* __restrict__ bary,
+                                                                 uint32_t* __restrict__ spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
+    for (size_t i = gid; i < n; i += lanes) {
+        const float* r = rays + 8 * i;
+        RayHit h;
+        const bool hit = trace_closest(nodes, tris, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), r[6], r[7], st, h,
+                                       nullptr, nullptr);
+        int4 o;
+        if (hit) {
+            const TriShade s = shade[h.tri];
+            o = make_int4(__float_as_int(h.t), s.shape, s.prim, h.tri);
+        } else {
+            o = make_int4(__float_as_int(-1.0f), -1, -1, -1);
+        }
+        hits[i] = o;
+        if (bary) bary[i] = make_float2(h.u, h.v);
+    }
+}
+
+__global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode* __restrict__ nodes,
+                                                                const TriAccel* __restrict__ tris, size_t n,
+                                                                const float* __restrict__ rays,
+                                                                uint8_t* __restrict__ occ, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
+    for (size_t i = gid; i < n; i += lanes) {
+        const float* r = rays + 8 * i;
+        occ[i] = trace_any(nodes, tris, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), r[6], r[7], st, nullptr, nullptr)
+                     ? 1
+                     : 0;
+    }
+}
+
+// Persistent grid for 256-lane blocks and the spill buffer sized for it.
+int ensure_trace_resources(pe_device* dev, int& blocks) {
+    blocks = dev->max_grid_blocks;
+    const uint32_t lanes = static_cast<uint32_t>(blocks) * kTraceBlock;
+    if (dev->spill_lanes < lanes) {
+        if (!dev->spill.alloc(static_cast<size_t>(lanes) * (kStackDepth - kLdsDepth) * sizeof(uint32_t))) return PE_ERR_OOM;
+        dev->spill_lanes = lanes;
+    }
+    return PE_OK;
+}
+
+int grid_for(pe_device* dev, size_t n) {
+    const size_t want = (n + 255) / 256;
+    return static_cast<int>(std::max<size_t>(1, std::min<size_t>(want, static_cast<size_t>(dev->max_grid_blocks))));
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+#define PE_GUARD_BEGIN try {
+#define PE_GUARD_END                                                  \
+    }                                                                 \
+    catch (const std::bad_alloc&) {                                   \
+        set_error("host allocation failed");                          \
+        return PE_ERR_OOM;                                            \
+    }                                                                 \
+    catch (const std::exception& e) {                                 \
+        set_error(std::string("internal error: ") + e.what());        \
+        return PE_ERR_INVALID;                                        \
+    }
+
+extern "C" {
+
+const char* pe_last_error(void) { return g_last_error.c_str(); }
+
+int pe_abi_version(void) { return PE_ABI_VERSION; }
+
+int pe_device_count(int* count) {
+    if (!count) {
+        set_error("pe_device_count: count is NULL");
+        return PE_ERR_INVALID;
+    }
+    int n = 0;
+    const hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) {
+        *count = 0;
+        return PE_OK;
+    }
+    if (e != hipSuccess) {
+        *count = 0;
+        set_error(std::string("hipGetDeviceCount failed: ") + hipGetErrorString(e));
+        return PE_OK;  // no usable device is not an error of the registry (enumerateDevices lists the CPU only)
+    }
+    *count = n;
+    return PE_OK;
+}
+
+int pe_device_info_get(int ordinal, pe_device_info* out) {
+    if (!out) {
+        set_error("pe_device_info_get: out is NULL");
+        return PE_ERR_INVALID;
+    }
+    int n = 0;
+    pe_device_count(&n);
+    if (ordinal < 0 || ordinal >= n) {
+        set_error("hip:" + std::to_string(ordinal) + ": no such device (" + std::to_string(n) + " visible)");
+        return PE_ERR_NO_DEVICE;
+    }
+    hipDeviceProp_t p;
+    PE_HIP_CHECK(hipGetDeviceProperties(&p, ordinal));
+    std::memset(out, 0, sizeof(*out));
+    std::snprintf(out->id, sizeof(out->id), "hip:%d", ordinal);
+    std::snprintf(out->name, sizeof(out->name), "%s", p.name);
+    std::snprintf(out->arch, sizeof(out->arch), "%s", p.gcnArchName);
+    out->kind = PE_DEVICE_HIP;
+    out->compute_units = p.multiProcessorCount;
+    out->memory_bytes = p.totalGlobalMem;
+    const bool gfx950 = std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+    out->usable = gfx950 ? 1 : 0;
+    std::snprintf(out->note, sizeof(out->note), "%s", gfx950 ? "wavefront path tracer (gfx950 code object)"
+                                                            : "no gfx950 code object for this architecture");
+    return PE_OK;
+}
+
+int pe_device_create(int ordinal, pe_device** out) {
+    PE_GUARD_BEGIN
+    if (!out) {
+        set_error("pe_device_create: out is NULL");
+        return PE_ERR_INVALID;
+    }
+    *out = nullptr;
+    pe_device_info info;
+    int rc = pe_device_info_get(ordinal, &info);
+    if (rc != PE_OK) return rc;
+    if (!info.usable) {
+        set_error(std::string(info.id) + ": " + info.note);
+        return PE_ERR_NO_DEVICE;
+    }
+    PE_HIP_CHECK(hipSetDevice(ordinal));
+    auto dev = std::make_unique<pe_device>();
+    dev->ordinal = ordinal;
+    dev->arch = info.arch;
+    dev->cu_count = info.compute_units;
+    dev->max_grid_blocks = std::max(1, dev->cu_count) * 8;
+    PE_HIP_CHECK(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+    *out = dev.release();
+    return PE_OK;
+    PE_GUARD_END
+}
+
+int pe_device_destroy(pe_device* dev) {
+    if (!dev) return PE_OK;
+    (void)hipSetDevice(dev->ordinal);
+    (void)hipStreamSynchronize(dev->stream);
+    dev->spill.release();
+    dev->scratch.release();
+    (void)hipStreamDestroy(dev->stream);
+    delete dev;
+    return PE_OK;
+}
+
+int pe_device_features_get(const pe_device* dev, pe_device_features* out) {
+    if (!dev || !out) {
+        set_error("pe_device_features_get: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    out->aovs = 0;
+    out->light_passes = 0;
+    out->adaptive_sampling = 0;
+    return PE_OK;
+}
+
+int pe_device_synchronize(pe_device* dev) {
+    if (!dev) {
+        set_error("pe_device_synchronize: NULL device");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    return PE_OK;
+}
+
+int pe_scene_upload(pe_device* dev, const pe_scene_desc* desc, pe_scene** out) {
+    PE_GUARD_BEGIN
+    if (!dev || !desc || !out) {
+        set_error("pe_scene_upload: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    *out = nullptr;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    auto sc = std::make_unique<pe_scene>();
+    sc->dev = dev;
+    const int rc = upload_scene(dev, *desc, *sc);
+    if (rc != PE_OK) return rc;
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    *out = sc.release();
+    return PE_OK;
+    PE_GUARD_END
+}
+
+int pe_scene_release(pe_scene* scene) {
+    if (!scene) return PE_OK;
+    (void)hipSetDevice(scene->dev->ordinal);
+    (void)hipStreamSynchronize(scene->dev->stream);
+    delete scene;
+    return PE_OK;
+}
+
+int pe_scene_stats(const pe_scene* scene, uint64_t* triangles, uint64_t* bvh_nodes, uint64_t* device_bytes) {
+    if (!scene) {
+        set_error("pe_scene_stats: NULL scene");
+        return PE_ERR_INVALID;
+    }
+    if (triangles) *triangles = scene->num_tris;
+    if (bvh_nodes) *bvh_nodes = scene->num_nodes;
+    if (device_bytes) *device_bytes = scene->device_bytes();
+    return PE_OK;
+}
+
+int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
+                   const uint32_t* index, const uint32_t* dim, uint32_t* out) {
+    PE_GUARD_BEGIN
+    if (!dev || (n > 0 && (!px || !py || !index || !dim || !out))) {
+        set_error("pe_sobol_batch: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    if (n == 0) return PE_OK;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    DevBuf dpx, dpy, didx, ddim, dout;
+    if (!dpx.upload(px, n * 4, dev->stream) || !dpy.upload(py, n * 4, dev->stream) ||
+        !didx.upload(index, n * 4, dev->stream) || !ddim.upload(dim, n * 4, dev->stream) || !dout.alloc(n * 8))
+        return PE_ERR_OOM;
+    hipLaunchKernelGGL(k_sobol_batch, dim3(grid_for(dev, n)), dim3(256), 0, dev->stream, seed, n, dpx.as<int32_t>(),
+                       dpy.as<int32_t>(), didx.as<uint32_t>(), ddim.as<uint32_t>(), dout.as<uint2>());
+    PE_HIP_CHECK(hipGetLastError());
+    PE_HIP_CHECK(hipMemcpyAsync(out, dout.get(), n * 8, hipMemcpyDeviceToHost, dev->stream));
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    return PE_OK;
+    PE_GUARD_END
+}
+
+int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary) {
+    PE_GUARD_BEGIN
+    if (!dev || !scene || scene->dev != dev || (n > 0 && (!rays || !hits))) {
+        set_error("pe_intersect_batch: NULL argument or scene of another device");
+        return PE_ERR_INVALID;
+    }
+    if (n == 0) return PE_OK;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    int blocks = 0;
+    int rc = ensure_trace_resources(dev, blocks);
+    if (rc != PE_OK) return rc;
+    blocks = grid_for(dev, n);
+    DevBuf drays, dhits, dbary;
+    if (!drays.upload(rays, n * 32, dev->stream) || !dhits.alloc(n * 16) || (bary && !dbary.alloc(n * 8)))
+        return PE_ERR_OOM;
+    hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
+                       scene->tris.as<TriAccel>(), scene->tri_shade.as<TriShade>(), n, drays.as<float>(),
+                       dhits.as<int4>(), bary ? dbary.as<float2>() : nullptr, dev->spill.as<uint32_t>());
+    PE_HIP_CHECK(hipGetLastError());
+    PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
+    if (bary) PE_HIP_CHECK(hipMemcpyAsync(bary, dbary.get(), n * 8, hipMemcpyDeviceToHost, dev->stream));
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    return PE_OK;
+    PE_GUARD_END
+}
+
+int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded) {
+    PE_GUARD_BEGIN
+    if (!dev || !scene || scene->dev != dev || (n > 0 && (!rays || !occluded))) {
+        set_error("pe_occluded_batch: NULL argument or scene of another device");
+        return PE_ERR_INVALID;
+    }
+    if (n == 0) return PE_OK;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    int blocks = 0;
+    int rc = ensure_trace_resources(dev, blocks);
+    if (rc != PE_OK) return rc;
+    blocks = grid_for(dev, n);
+    DevBuf drays, docc;
+    if (!drays.upload(rays, n * 32, dev->stream) || !docc.alloc(n)) return PE_ERR_OOM;
+    hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
+                       scene->tris.as<TriAccel>(), n, drays.as<float>(), docc.as<uint8_t>(), dev->spill.as<uint32_t>());
+    PE_HIP_CHECK(hipGetLastError());
+    PE_HIP_CHECK(hipMemcpyAsync(occluded, docc.get(), n, hipMemcpyDeviceToHost, dev->stream));
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    return PE_OK;
+    PE_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,95 @@
+// pe_internal.h — host-side objects behind the opaque C-ABI handles, and error plumbing.
+#pragma once
+
+#include "photon_hip.h"
+#include "pe_gpu_types.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace pe {
+
+void set_error(const std::string& msg);
+
+// Owning device allocation. Device memory is never freed while a kernel may still read it: owners
+// synchronise their stream before releasing.
+class DevBuf {
+public:
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : m_ptr(o.m_ptr), m_bytes(o.m_bytes) {
+        o.m_ptr = nullptr;
+        o.m_bytes = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            m_ptr = o.m_ptr;
+            m_bytes = o.m_bytes;
+            o.m_ptr = nullptr;
+            o.m_bytes = 0;
+        }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+
+    bool alloc(size_t bytes);  // false (and error set) on failure; 0 bytes gives a null pointer
+    bool upload(const void* src, size_t bytes, hipStream_t s);
+    void release();
+    void* get() const { return m_ptr; }
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(m_ptr);
+    }
+    size_t bytes() const { return m_bytes; }
+
+private:
+    void* m_ptr = nullptr;
+    size_t m_bytes = 0;
+};
+
+}  // namespace pe
+
+struct pe_device {
+    int ordinal = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    std::string arch;
+    pe::DevBuf spill;            // traversal stack spill, (kStackDepth - kLdsDepth) entries per lane
+    uint32_t spill_lanes = 0;    // lanes the spill buffer is sized for
+    pe::DevBuf scratch;          // small per-call staging (counters)
+    int max_grid_blocks = 0;     // persistent-grid size for 256-lane blocks
+};
+
+struct pe_scene {
+    pe_device* dev = nullptr;
+    uint64_t num_tris = 0;
+    uint64_t num_verts = 0;
+    int num_shapes = 0;
+    int bvh_depth = 0;
+    pe::DevBuf nodes, tris, tri_shade, positions, normals, uvs;
+    uint64_t num_nodes = 0;
+    uint64_t device_bytes() const {
+        return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes();
+    }
+};
+
+struct pe_film {
+    pe_device* dev = nullptr;
+    int width = 0, height = 0;
+    pe::DevBuf sum, count, lum2, rejected;
+};
+
+#define PE_HIP_CHECK(call)                                                                              \
+    do {                                                                                                \
+        hipError_t _e = (call);                                                                         \
+        if (_e != hipSuccess) {                                                                         \
+            pe::set_error(std::string(#call) + " failed: " + hipGetErrorString(_e));                    \
+            return PE_ERR_HIP;                                                                          \
+        }                                                                                               \
+    } while (0)

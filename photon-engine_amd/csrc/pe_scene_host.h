@@ -1,0 +1,11 @@
+// pe_scene_host.h — host-side packing of a pe_scene_desc into device buffers.
+#pragma once
+
+#include "pe_internal.h"
+
+namespace pe {
+
+// Validates desc, builds the BVH and per-triangle records, and uploads them on dev->stream.
+int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc);
+
+}  // namespace pe

@@ -1,0 +1,160 @@
+// wavefront.hip — film accumulation, camera set-up and the wavefront integrator behind pe_render.
+#include "photon_hip.h"
+#include "pe_internal.h"
+#include "pe_math.h"
+
+#include <cmath>
+#include <string>
+
+namespace pe {
+
+__global__ void k_zero_u32(uint32_t* __restrict__ p, size_t n) {
+    const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+    for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
+// Clears a film with a fill kernel (no memset node, so the call is safe inside a captured graph).
+int film_clear(pe_film* f) {
+    const size_t px = static_cast<size_t>(f->width) * f->height;
+    const size_t words[4] = {3 * px, px, px, 2};
+    void* bufs[4] = {f->sum.get(), f->count.get(), f->lum2.get(), f->rejected.get()};
+    for (int k = 0; k < 4; ++k) {
+        if (!bufs[k] || words[k] == 0) continue;
+        const int grid = static_cast<int>(std::min<size_t>((words[k] + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid), dim3(256), 0, f->dev->stream, static_cast<uint32_t*>(bufs[k]),
+                           words[k]);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("film clear: ") + hipGetErrorString(e));
+        return PE_ERR_HIP;
+    }
+    return PE_OK;
+}
+
+}  // namespace pe
+
+using namespace pe;
+
+extern "C" {
+
+// PerspectiveCamera constructor (camera/perspective_camera.cpp:11-34) and describeGpu (:42-54).
+int pe_camera_perspective(const float position[3], const float target[3], const float up[3], float vfov_degrees,
+                          float aspect, pe_camera* out) {
+#pragma clang fp contract(off)
+    if (!position || !target || !up || !out) {
+        set_error("pe_camera_perspective: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    const V3 pos = mk(position[0], position[1], position[2]);
+    const V3 tgt = mk(target[0], target[1], target[2]);
+    const V3 upv = mk(up[0], up[1], up[2]);
+    const float theta = vfov_degrees * (kPi / 180.0f);  // deg2rad (utils.h:44-46)
+    const float h = std::tan(theta * 0.5f);
+    const float vh = 2.0f * h;
+    const float vw = aspect * vh;
+    const V3 w = normalized(pos - tgt);
+    const V3 u = normalized(cross(upv, w));
+    const V3 v = cross(w, u);
+    const V3 hor = vw * u;
+    const V3 ver = vh * v;
+    const V3 ll = pos - hor * 0.5f - ver * 0.5f - w;
+    const V3 vs[4] = {pos, ll, hor, ver};
+    float* dst[4] = {out->position, out->lower_left, out->horizontal, out->vertical};
+    for (int k = 0; k < 4; ++k) {
+        dst[k][0] = vs[k].x;
+        dst[k][1] = vs[k].y;
+        dst[k][2] = vs[k].z;
+    }
+    out->lens_radius = 0.0f;
+    return PE_OK;
+}
+
+int pe_film_create(pe_device* dev, int width, int height, pe_film** out) {
+    if (!dev || !out || width <= 0 || height <= 0) {
+        set_error("pe_film_create: bad argument");
+        return PE_ERR_INVALID;
+    }
+    *out = nullptr;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    auto f = std::make_unique<pe_film>();
+    f->dev = dev;
+    f->width = width;
+    f->height = height;
+    const size_t px = static_cast<size_t>(width) * height;
+    if (!f->sum.alloc(px * 12) || !f->count.alloc(px * 4) || !f->lum2.alloc(px * 4) || !f->rejected.alloc(8))
+        return PE_ERR_OOM;
+    int rc = film_clear(f.get());
+    if (rc != PE_OK) return rc;
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    *out = f.release();
+    return PE_OK;
+}
+
+int pe_film_destroy(pe_film* film) {
+    if (!film) return PE_OK;
+    (void)hipSetDevice(film->dev->ordinal);
+    (void)hipStreamSynchronize(film->dev->stream);
+    delete film;
+    return PE_OK;
+}
+
+int pe_film_clear(pe_film* film) {
+    if (!film) {
+        set_error("pe_film_clear: NULL film");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipSetDevice(film->dev->ordinal));
+    int rc = film_clear(film);
+    if (rc != PE_OK) return rc;
+    PE_HIP_CHECK(hipStreamSynchronize(film->dev->stream));
+    return PE_OK;
+}
+
+int pe_film_download(pe_film* film, float* sum_rgb, uint32_t* count, float* sum_lum2, uint64_t* rejected) {
+    if (!film) {
+        set_error("pe_film_download: NULL film");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipSetDevice(film->dev->ordinal));
+    hipStream_t s = film->dev->stream;
+    const size_t px = static_cast<size_t>(film->width) * film->height;
+    if (sum_rgb) PE_HIP_CHECK(hipMemcpyAsync(sum_rgb, film->sum.get(), px * 12, hipMemcpyDeviceToHost, s));
+    if (count) PE_HIP_CHECK(hipMemcpyAsync(count, film->count.get(), px * 4, hipMemcpyDeviceToHost, s));
+    if (sum_lum2) PE_HIP_CHECK(hipMemcpyAsync(sum_lum2, film->lum2.get(), px * 4, hipMemcpyDeviceToHost, s));
+    if (rejected) PE_HIP_CHECK(hipMemcpyAsync(rejected, film->rejected.get(), 8, hipMemcpyDeviceToHost, s));
+    PE_HIP_CHECK(hipStreamSynchronize(s));
+    return PE_OK;
+}
+
+int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
+    (void)kind;
+    (void)desc;
+    if (!scene) {
+        set_error("pe_scene_update: NULL scene");
+        return PE_ERR_INVALID;
+    }
+    set_error("pe_scene_update: not implemented yet");
+    return PE_ERR_UNSUPPORTED;
+}
+
+int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
+              int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result) {
+    (void)first_sample;
+    (void)sample_count;
+    (void)cancel;
+    if (result) {
+        result->completed = 0;
+        result->samples_traced = 0;
+        result->extension_rays = 0;
+        result->shadow_rays = 0;
+    }
+    if (!dev || !scene || !camera || !settings || !film) {
+        set_error("pe_render: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    set_error("pe_render: not implemented yet");
+    return PE_ERR_UNSUPPORTED;
+}
+
+}  // extern "C"

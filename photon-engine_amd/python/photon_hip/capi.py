@@ -1,0 +1,165 @@
+"""ctypes mirror of include/photon_hip.h and the loader of libphoton_hip.so.
+
+The library is loaded from the package directory (photon-engine_amd/libphoton_hip.so). There is no
+fallback: if the library is missing or a call fails, a PhotonHipError is raised with the C-ABI's message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parents[2]
+LIB_PATH = Path(os.environ.get("PHOTON_HIP_LIB", PKG_DIR / "libphoton_hip.so"))
+
+PE_OK = 0
+PE_ERR_INVALID = -1
+PE_ERR_NO_DEVICE = -2
+PE_ERR_HIP = -3
+PE_ERR_OOM = -4
+PE_ERR_UNSUPPORTED = -5
+PE_ERR_CANCELLED = -6
+
+PE_MAT_DISNEY, PE_MAT_LAMBERT, PE_MAT_DIELECTRIC = 0, 1, 2
+PE_LIGHT_AREA, PE_LIGHT_TRIANGLE = 1, 2
+PE_SHAPE_CAMERA_INVISIBLE = 1
+PE_FILTER_BOX, PE_FILTER_TRIANGLE, PE_FILTER_GAUSSIAN, PE_FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3
+PE_UPDATE_FULL, PE_UPDATE_MATERIALS, PE_UPDATE_LIGHTS = 0, 1, 2
+
+F3 = C.c_float * 3
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("id", C.c_char * 32), ("name", C.c_char * 128), ("arch", C.c_char * 32), ("kind", C.c_int32),
+                ("compute_units", C.c_int32), ("memory_bytes", C.c_uint64), ("usable", C.c_int32),
+                ("note", C.c_char * 128)]
+
+
+class DeviceFeatures(C.Structure):
+    _fields_ = [("aovs", C.c_int32), ("light_passes", C.c_int32), ("adaptive_sampling", C.c_int32)]
+
+
+class MaterialDesc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("base_color", F3), ("emission", F3),
+        ("metallic", C.c_float), ("roughness", C.c_float), ("specular", C.c_float),
+        ("clearcoat", C.c_float), ("clearcoat_roughness", C.c_float), ("coat_ior", C.c_float),
+        ("coat_tint", F3),
+        ("ior", C.c_float), ("specular_transmission", C.c_float), ("anisotropy", C.c_float),
+        ("sheen", C.c_float), ("sheen_roughness", C.c_float), ("sheen_color", F3), ("has_sheen_color", C.c_int32),
+        ("diffuse_roughness", C.c_float), ("diffuse_transmission", C.c_float),
+        ("subsurface", C.c_float), ("subsurface_radius", F3),
+        ("subsurface_scale", C.c_float), ("subsurface_anisotropy", C.c_float),
+        ("edge_tint", F3), ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
+        ("abbe", C.c_float), ("absorption_color", F3), ("absorption_distance", C.c_float),
+        ("thin_walled", C.c_int32), ("reserved", C.c_float * 5),
+    ]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("uvs", C.POINTER(C.c_float)), ("indices", C.POINTER(C.c_uint32)),
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("material", C.c_int32),
+                ("emitter_base", C.c_int32), ("standalone", C.c_int32), ("flags", C.c_uint32)]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("p0", F3), ("p1", F3), ("p2", F3), ("radiance", F3),
+                ("visible_to_camera", C.c_int32)]
+
+
+class EnvDesc(C.Structure):
+    _fields_ = [("rgb", C.POINTER(C.c_float)), ("width", C.c_int32), ("height", C.c_int32),
+                ("rotation", C.c_float), ("intensity", C.c_float), ("has_sun", C.c_int32),
+                ("sun_direction", F3), ("sun_cos_max", C.c_float), ("sun_radiance", F3)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("meshes", C.POINTER(MeshDesc)), ("num_meshes", C.c_int32),
+                ("materials", C.POINTER(MaterialDesc)), ("num_materials", C.c_int32),
+                ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_int32),
+                ("environment", C.POINTER(EnvDesc))]
+
+
+class Camera(C.Structure):
+    _fields_ = [("position", F3), ("lower_left", F3), ("horizontal", F3), ("vertical", F3),
+                ("lens_radius", C.c_float)]
+
+
+class Settings(C.Structure):
+    _fields_ = [("max_bounces", C.c_int32), ("rr_depth", C.c_int32), ("shadow_quality", C.c_int32),
+                ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
+                ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("reserved", C.c_int32 * 7)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("completed", C.c_int32), ("samples_traced", C.c_uint64), ("extension_rays", C.c_uint64),
+                ("shadow_rays", C.c_uint64)]
+
+
+class PhotonHipError(RuntimeError):
+    def __init__(self, func: str, code: int, msg: str):
+        super().__init__(f"{func} -> {code}: {msg}")
+        self.code = code
+
+
+_VP = C.c_void_p
+_P = C.POINTER
+
+# Every symbol include/photon_hip.h declares, with its ctypes signature.
+SIGNATURES = {
+    "pe_last_error": (C.c_char_p, []),
+    "pe_abi_version": (C.c_int, []),
+    "pe_device_count": (C.c_int, [_P(C.c_int)]),
+    "pe_device_info_get": (C.c_int, [C.c_int, _P(DeviceInfo)]),
+    "pe_device_create": (C.c_int, [C.c_int, _P(_VP)]),
+    "pe_device_destroy": (C.c_int, [_VP]),
+    "pe_device_features_get": (C.c_int, [_VP, _P(DeviceFeatures)]),
+    "pe_device_synchronize": (C.c_int, [_VP]),
+    "pe_scene_upload": (C.c_int, [_VP, _P(SceneDesc), _P(_VP)]),
+    "pe_scene_update": (C.c_int, [_VP, C.c_int, _P(SceneDesc)]),
+    "pe_scene_release": (C.c_int, [_VP]),
+    "pe_scene_stats": (C.c_int, [_VP, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
+    "pe_camera_perspective": (C.c_int, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, _P(Camera)]),
+    "pe_film_create": (C.c_int, [_VP, C.c_int, C.c_int, _P(_VP)]),
+    "pe_film_destroy": (C.c_int, [_VP]),
+    "pe_film_clear": (C.c_int, [_VP]),
+    "pe_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_float), _P(C.c_uint64)]),
+    "pe_render": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _VP, C.c_int, C.c_int, _P(C.c_int32),
+                            _P(BatchResult)]),
+    "pe_sobol_batch": (C.c_int, [_VP, C.c_uint64, C.c_size_t, _P(C.c_int32), _P(C.c_int32), _P(C.c_uint32),
+                                 _P(C.c_uint32), _P(C.c_uint32)]),
+    "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
+    "pe_occluded_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_uint8)]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libphoton_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950) first")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(func: str, rc: int) -> None:
+    if rc != PE_OK:
+        raise PhotonHipError(func, rc, load().pe_last_error().decode(errors="replace"))
+
+
+def ptr(a: np.ndarray | None, ctype):
+    if a is None:
+        return C.cast(None, C.POINTER(ctype))
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,224 @@
+"""Seeded synthetic workloads.
+
+box_icosphere   closed box of 6 inward quads (12 triangles) around an icosphere (level 5: 20,480 triangles),
+                Disney principled everywhere (seeded baseColor U[0.2, 0.9], roughness 0.3, metallic 0,
+                specular 0.5), one 0.5 x 0.5 area light of radiance 10 under the ceiling; 512 x 512, 64 spp,
+                max depth 8, Russian roulette from depth 3.
+product_sphere  a UV sphere of 1024 x 1000 segments (2,045,952 triangles) displaced by 0.05 * Perlin(4p),
+                split into 16 longitude wedges, each its own Disney material (metallic in {0, 1},
+                roughness U[0.05, 0.8], clearcoat U[0, 1]); a 2048 x 1024 sky-gradient environment with a
+                0.5-degree sun disk at 1e4 x baked in; 1920 x 1080, 256 spp, max depth 12.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .scene import AreaLight, Environment, Mesh, Scene, disney
+
+
+@dataclass
+class View:
+    position: tuple
+    target: tuple
+    up: tuple
+    vfov: float
+    width: int
+    height: int
+    spp: int
+    max_bounces: int
+
+    @property
+    def aspect(self) -> float:
+        return self.width / self.height
+
+
+def icosphere(level: int, radius: float, center) -> Mesh:
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    v = np.array(verts, dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array(faces, dtype=np.int64)
+    for _ in range(level):
+        edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        key = np.sort(edges, axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        inv = inv.reshape(-1)
+        mid = v[uniq[:, 0]] + v[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        base = v.shape[0]
+        v = np.concatenate([v, mid])
+        nf = f.shape[0]
+        a = base + inv[0:nf]
+        b = base + inv[nf:2 * nf]
+        c = base + inv[2 * nf:3 * nf]
+        f = np.concatenate([np.stack([f[:, 0], a, c], 1), np.stack([f[:, 1], b, a], 1),
+                            np.stack([f[:, 2], c, b], 1), np.stack([a, b, c], 1)])
+    pos = (v * radius + np.asarray(center, dtype=np.float64)).astype(np.float32)
+    return Mesh(pos, f.astype(np.uint32), v.astype(np.float32))
+
+
+def _quad(p0, p1, p2, p3) -> Mesh:
+    return Mesh(np.array([p0, p1, p2, p3], dtype=np.float32), np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32))
+
+
+def box_icosphere(seed: int = 0, level: int = 5, width: int = 512, height: int = 512, spp: int = 64,
+                  max_bounces: int = 8) -> tuple[Scene, View]:
+    rng = np.random.default_rng(seed)
+    sc = Scene()
+    # Six inward-facing walls of [-1, 1]^3 (counter-clockwise seen from inside).
+    walls = [
+        ((-1, -1, -1), (1, -1, -1), (1, -1, 1), (-1, -1, 1)),   # floor, normal +y
+        ((-1, 1, -1), (-1, 1, 1), (1, 1, 1), (1, 1, -1)),       # ceiling, normal -y
+        ((-1, -1, -1), (-1, 1, -1), (1, 1, -1), (1, -1, -1)),   # back, normal +z
+        ((-1, -1, 1), (1, -1, 1), (1, 1, 1), (-1, 1, 1)),       # front, normal -z
+        ((-1, -1, -1), (-1, -1, 1), (-1, 1, 1), (-1, 1, -1)),   # left, normal +x
+        ((1, -1, -1), (1, 1, -1), (1, 1, 1), (1, -1, 1)),       # right, normal -x
+    ]
+    for w in walls:
+        m = sc.add_material(disney(rng.uniform(0.2, 0.9, 3), 0.0, 0.3, 0.5))
+        sc.add_shape(_quad(*w), m)
+    m = sc.add_material(disney(rng.uniform(0.2, 0.9, 3), 0.0, 0.3, 0.5))
+    sc.add_shape(icosphere(level, 0.6, (0.0, -0.4, 0.0)), m)
+    sc.add_light(AreaLight((-0.25, 0.999, -0.25), (0.5, 0.0, 0.0), (0.0, 0.0, 0.5), (10.0, 10.0, 10.0)))
+    view = View((0.0, 0.0, 0.95), (0.0, -0.2, 0.0), (0.0, 1.0, 0.0), 60.0, width, height, spp, max_bounces)
+    return sc, view
+
+
+# ---- product_sphere --------------------------------------------------------------------------------------
+
+def _perlin3(p: np.ndarray, perm: np.ndarray) -> np.ndarray:
+    """Improved Perlin noise (Perlin 2002) on points p (N, 3), permutation perm (512,)."""
+    pi = np.floor(p).astype(np.int64)
+    f = p - pi
+    pi &= 255
+    u = f * f * f * (f * (f * 6.0 - 15.0) + 10.0)
+
+    def grad(h, x, y, z):
+        h = h & 15
+        a = np.where(h < 8, x, y)
+        b = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -a, a) + np.where(h & 2, -b, b)
+
+    X, Y, Z = pi[:, 0], pi[:, 1], pi[:, 2]
+    x, y, z = f[:, 0], f[:, 1], f[:, 2]
+    A = perm[X] + Y
+    AA = perm[A] + Z
+    AB = perm[A + 1] + Z
+    B = perm[X + 1] + Y
+    BA = perm[B] + Z
+    BB = perm[B + 1] + Z
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    return lerp(u[:, 2],
+                lerp(u[:, 1], lerp(u[:, 0], grad(perm[AA], x, y, z), grad(perm[BA], x - 1, y, z)),
+                     lerp(u[:, 0], grad(perm[AB], x, y - 1, z), grad(perm[BB], x - 1, y - 1, z))),
+                lerp(u[:, 1], lerp(u[:, 0], grad(perm[AA + 1], x, y, z - 1), grad(perm[BA + 1], x - 1, y, z - 1)),
+                     lerp(u[:, 0], grad(perm[AB + 1], x, y - 1, z - 1), grad(perm[BB + 1], x - 1, y - 1, z - 1))))
+
+
+def displaced_sphere(lon: int = 1024, lat: int = 1000, parts: int = 16, amplitude: float = 0.05,
+                     seed: int = 1) -> list[Mesh]:
+    """UV sphere with 2*lon*(lat-1) triangles, radius 1 + amplitude * Perlin(4p), as `parts` longitude wedges."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(256)
+    perm = np.concatenate([perm, perm]).astype(np.int64)
+    theta = np.linspace(0.0, np.pi, lat + 1)[1:-1]                     # interior rings
+    phi = np.linspace(0.0, 2.0 * np.pi, lon + 1)                       # lon + 1 columns (seam duplicated)
+    st, ct = np.sin(theta)[:, None], np.cos(theta)[:, None]
+    ring = np.stack([st * np.cos(phi)[None, :], np.broadcast_to(ct, (lat - 1, lon + 1)), st * np.sin(phi)[None, :]], -1)
+    dirs = np.concatenate([ring.reshape(-1, 3), [[0.0, 1.0, 0.0], [0.0, -1.0, 0.0]]])
+    r = 1.0 + amplitude * _perlin3(4.0 * dirs, perm)
+    pos_all = dirs * r[:, None]
+    rows, cols = lat - 1, lon + 1
+    north, south = rows * cols, rows * cols + 1
+
+    def vid(i, j):
+        return i * cols + j
+
+    tris = []
+    for j in range(lon):
+        tris.append((north, vid(0, j + 1), vid(0, j)))
+        for i in range(rows - 1):
+            a, b, c, d = vid(i, j), vid(i, j + 1), vid(i + 1, j), vid(i + 1, j + 1)
+            tris.append((a, b, d))
+            tris.append((a, d, c))
+        tris.append((south, vid(rows - 1, j), vid(rows - 1, j + 1)))
+    tris = np.array(tris, dtype=np.int64).reshape(lon, -1, 3)
+    # Smooth vertex normals: area-weighted face normals, accumulated per vertex.
+    t = tris.reshape(-1, 3)
+    fn = np.cross(pos_all[t[:, 1]] - pos_all[t[:, 0]], pos_all[t[:, 2]] - pos_all[t[:, 0]])
+    nrm = np.zeros_like(pos_all)
+    for k in range(3):
+        np.add.at(nrm, t[:, k], fn)
+    # Seam columns share the normal of their twin.
+    seam = nrm[vid(np.arange(rows), 0)] + nrm[vid(np.arange(rows), lon)]
+    nrm[vid(np.arange(rows), 0)] = seam
+    nrm[vid(np.arange(rows), lon)] = seam
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    meshes = []
+    per = lon // parts
+    for p in range(parts):
+        pt = tris[p * per:(p + 1) * per].reshape(-1, 3)
+        used, local = np.unique(pt, return_inverse=True)
+        meshes.append(Mesh(pos_all[used].astype(np.float32), local.reshape(-1, 3).astype(np.uint32),
+                           nrm[used].astype(np.float32)))
+    return meshes
+
+
+def sky_with_sun(width: int = 2048, height: int = 1024, sun_elevation_deg: float = 45.0,
+                 sun_azimuth_deg: float = 30.0, sun_diameter_deg: float = 0.5, sun_scale: float = 1e4) -> np.ndarray:
+    """Sky gradient (zenith (0.3, 0.5, 1.0) to horizon 1.0, ground 0.2) with a sun disk baked in at 1e4 x."""
+    v = (np.arange(height) + 0.5) / height
+    u = (np.arange(width) + 0.5) / width
+    theta = v * np.pi
+    elev = np.pi / 2 - theta
+    zen = np.array([0.3, 0.5, 1.0])
+    hor = np.array([1.0, 1.0, 1.0])
+    s = np.clip(elev / (np.pi / 2), 0.0, 1.0)[:, None]
+    sky = hor[None, :] * (1 - s) + zen[None, :] * s
+    sky = np.where((elev < 0)[:, None], np.array([0.2, 0.2, 0.2])[None, :], sky)
+    img = np.repeat(sky[:, None, :], width, axis=1)
+    # Direction of every texel (environment_light.h:33-39 equirectDirection with rotation 0).
+    a = u[None, :] * 2 * np.pi - np.pi
+    st = np.sin(theta)[:, None]
+    stw = np.broadcast_to(st, (height, width))
+    ct = np.broadcast_to(np.cos(theta)[:, None], (height, width))
+    d = np.stack([stw * np.cos(a), ct, -stw * np.sin(a)], -1)
+    se, sa = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg)
+    sun = np.array([np.cos(se) * np.cos(sa), np.sin(se), np.cos(se) * np.sin(sa)])
+    cosmax = np.cos(np.radians(sun_diameter_deg / 2))
+    disk = (d @ sun) >= cosmax
+    img[disk] *= sun_scale
+    return img.astype(np.float32)
+
+
+def product_sphere(seed: int = 1, width: int = 1920, height: int = 1080, spp: int = 256, max_bounces: int = 12,
+                   lon: int = 1024, lat: int = 1000, env_width: int = 2048, env_height: int = 1024) -> tuple[Scene, View]:
+    rng = np.random.default_rng(seed + 100)
+    sc = Scene()
+    for mesh in displaced_sphere(lon, lat, 16, 0.05, seed):
+        m = sc.add_material(disney(rng.uniform(0.2, 0.9, 3), float(rng.random() < 0.5), rng.uniform(0.05, 0.8), 0.5,
+                                   clearcoat=float(rng.uniform(0.0, 1.0))))
+        sc.add_shape(mesh, m)
+    sc.set_environment(Environment(sky_with_sun(env_width, env_height)))
+    view = View((0.0, 0.35, 3.2), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, width, height, spp, max_bounces)
+    return sc, view
+
+
+def tiny_scene(seed: int = 7, n_tris: int = 64) -> Scene:
+    """A small random triangle soup (for hit parity against brute force)."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-1, 1, (n_tris, 1, 3))
+    tri = c + rng.normal(0, 0.25, (n_tris, 3, 3))
+    sc = Scene()
+    m = sc.add_material(disney((0.5, 0.5, 0.5)))
+    sc.add_shape(Mesh(tri.reshape(-1, 3).astype(np.float32), np.arange(3 * n_tris, dtype=np.uint32).reshape(-1, 3)), m)
+    return sc

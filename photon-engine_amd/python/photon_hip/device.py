@@ -1,0 +1,158 @@
+"""Thin Python handles over the C-ABI (device, scene, film), used by tests and bench.py."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import check, load, ptr
+
+
+def device_count() -> int:
+    n = C.c_int()
+    check("pe_device_count", load().pe_device_count(C.byref(n)))
+    return n.value
+
+
+def camera_perspective(position, target, up, vfov, aspect) -> capi.Camera:
+    cam = capi.Camera()
+    f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+    check("pe_camera_perspective", load().pe_camera_perspective(f3(position), f3(target), f3(up), float(vfov),
+                                                                  float(aspect), C.byref(cam)))
+    return cam
+
+
+def settings(max_bounces=8, rr_depth=3, shadow_quality=1, filter_type=capi.PE_FILTER_BOX, filter_width=1.0,
+             seed=12345, samples_per_launch=1, clamp_indirect=0.0) -> capi.Settings:
+    s = capi.Settings()
+    s.max_bounces, s.rr_depth, s.shadow_quality = max_bounces, rr_depth, shadow_quality
+    s.filter_type, s.filter_width, s.clamp_indirect = filter_type, filter_width, clamp_indirect
+    s.seed, s.samples_per_launch = seed, samples_per_launch
+    return s
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        self._lib = load()
+        h = C.c_void_p()
+        check("pe_device_create", self._lib.pe_device_create(ordinal, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.pe_device_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, ordinal: int = 0) -> capi.DeviceInfo:
+        info = capi.DeviceInfo()
+        check("pe_device_info_get", self._lib.pe_device_info_get(ordinal, C.byref(info)))
+        return info
+
+    def synchronize(self):
+        check("pe_device_synchronize", self._lib.pe_device_synchronize(self.h))
+
+    def sobol(self, seed, px, py, index, dim) -> np.ndarray:
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        index = np.ascontiguousarray(index, np.uint32)
+        dim = np.ascontiguousarray(dim, np.uint32)
+        out = np.zeros((px.shape[0], 2), np.uint32)
+        check("pe_sobol_batch", self._lib.pe_sobol_batch(self.h, seed, px.shape[0], ptr(px, C.c_int32),
+                                                         ptr(py, C.c_int32), ptr(index, C.c_uint32),
+                                                         ptr(dim, C.c_uint32), ptr(out, C.c_uint32)))
+        return out
+
+    def upload(self, scene) -> "GpuScene":
+        return GpuScene(self, scene)
+
+    def film(self, width: int, height: int) -> "GpuFilm":
+        return GpuFilm(self, width, height)
+
+
+class GpuScene:
+    def __init__(self, dev: Device, scene):
+        self.dev = dev
+        self._lib = dev._lib
+        desc, keep = scene.pack()
+        h = C.c_void_p()
+        check("pe_scene_upload", self._lib.pe_scene_upload(dev.h, C.byref(desc), C.byref(h)))
+        del keep
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.pe_scene_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def stats(self):
+        t, n, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check("pe_scene_stats", self._lib.pe_scene_stats(self.h, C.byref(t), C.byref(n), C.byref(b)))
+        return {"triangles": t.value, "bvh_nodes": n.value, "device_bytes": b.value}
+
+    def intersect(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        hits = np.zeros((n, 4), np.int32)
+        bary = np.zeros((n, 2), np.float32)
+        check("pe_intersect_batch", self._lib.pe_intersect_batch(self.dev.h, self.h, n, ptr(rays, C.c_float),
+                                                                 ptr(hits, C.c_int32), ptr(bary, C.c_float)))
+        return hits, bary
+
+    def occluded(self, rays: np.ndarray):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        occ = np.zeros(rays.shape[0], np.uint8)
+        check("pe_occluded_batch", self._lib.pe_occluded_batch(self.dev.h, self.h, rays.shape[0],
+                                                               ptr(rays, C.c_float), ptr(occ, C.c_uint8)))
+        return occ
+
+
+class GpuFilm:
+    def __init__(self, dev: Device, width: int, height: int):
+        self.dev = dev
+        self._lib = dev._lib
+        self.width, self.height = width, height
+        h = C.c_void_p()
+        check("pe_film_create", self._lib.pe_film_create(dev.h, width, height, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.pe_film_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def clear(self):
+        check("pe_film_clear", self._lib.pe_film_clear(self.h))
+
+    def download(self):
+        """Returns (sum_rgb (H, W, 3), count (H, W), sum_lum2 (H, W), rejected)."""
+        s = np.zeros((self.height, self.width, 3), np.float32)
+        c = np.zeros((self.height, self.width), np.uint32)
+        l2 = np.zeros((self.height, self.width), np.float32)
+        rej = C.c_uint64()
+        check("pe_film_download", self._lib.pe_film_download(self.h, ptr(s, C.c_float), ptr(c, C.c_uint32),
+                                                             ptr(l2, C.c_float), C.byref(rej)))
+        return s, c, l2, rej.value
+
+    def mean(self) -> np.ndarray:
+        s, c, _, _ = self.download()
+        return s / np.maximum(c, 1)[..., None]
+
+
+def render(dev: Device, scene: GpuScene, camera: capi.Camera, st: capi.Settings, film: GpuFilm, first: int,
+           count: int, cancel=None) -> capi.BatchResult:
+    res = capi.BatchResult()
+    cptr = ptr(cancel, C.c_int32) if cancel is not None else C.cast(None, C.POINTER(C.c_int32))
+    rc = load().pe_render(dev.h, scene.h, C.byref(camera), C.byref(st), film.h, first, count, cptr, C.byref(res))
+    if rc not in (capi.PE_OK, capi.PE_ERR_CANCELLED):
+        check("pe_render", rc)
+    return res

@@ -1,0 +1,212 @@
+"""Scene assembly with the reference's Scene semantics, flattened into a pe_scene_desc.
+
+Scene::addShape / addLight / setEnvironment (engine/scene.cpp:18-104) are mirrored:
+  * add_shape(mesh, material): a TriangleMesh; when the material emits, every triangle also becomes a
+    TriangleLight appended to lights() (scene.cpp:48-62), so emitter index = base + primIndex;
+  * add_light(AreaLight): appends the light, then two standalone black emissive triangles
+    (p0, p1, p2) and (p0, p2, p3) with the light's normal as vertex normals and uv (0,0),(1,0),(1,1),(0,1)
+    (scene.cpp:65-99);
+  * set_environment(EnvironmentLight).
+The result is consumed by libphoton_hip (pe_scene_upload) and by the test oracles, from the same bytes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import capi
+
+
+def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.MaterialDesc:
+    """DisneyMaterial(baseColor, metallic, roughness, specular) with the reference's defaults (disney.h:210-245)."""
+    m = capi.MaterialDesc()
+    m.type = capi.PE_MAT_DISNEY
+    m.base_color[:] = list(map(float, base_color))
+    m.metallic, m.roughness, m.specular = float(metallic), float(roughness), float(specular)
+    m.clearcoat, m.clearcoat_roughness, m.coat_ior = 0.0, 0.03, 1.5
+    m.coat_tint[:] = [1.0, 1.0, 1.0]
+    m.ior, m.specular_transmission, m.anisotropy = 1.5, 0.0, 0.0
+    m.sheen, m.sheen_roughness = 0.0, 0.3
+    m.sheen_color[:] = [1.0, 1.0, 1.0]
+    m.has_sheen_color = 0
+    m.diffuse_roughness, m.diffuse_transmission, m.subsurface = 0.0, 0.0, 0.0
+    m.subsurface_radius[:] = [1.0, 0.2, 0.1]
+    m.subsurface_scale, m.subsurface_anisotropy = 0.05, 0.0
+    m.edge_tint[:] = [1.0, 1.0, 1.0]
+    m.thin_film_thickness, m.thin_film_ior = 0.0, 1.4
+    m.absorption_color[:] = [1.0, 1.0, 1.0]
+    m.absorption_distance = 0.0
+    for k, v in kw.items():
+        cur = getattr(m, k)
+        if isinstance(cur, C.Array):
+            cur[:] = list(map(float, v))
+        else:
+            setattr(m, k, v)
+    return m
+
+
+def lambert(albedo, emission=(0.0, 0.0, 0.0)) -> capi.MaterialDesc:
+    """Lambertian(albedo, emission) (lambertian.h:16)."""
+    m = disney(albedo)
+    m.type = capi.PE_MAT_LAMBERT
+    m.emission[:] = list(map(float, emission))
+    return m
+
+
+@dataclass
+class Mesh:
+    positions: np.ndarray                 # (V, 3) float32
+    indices: np.ndarray                   # (T, 3) uint32
+    normals: np.ndarray | None = None     # (V, 3) float32
+    uvs: np.ndarray | None = None         # (V, 2) float32
+
+    def __post_init__(self):
+        self.positions = np.ascontiguousarray(self.positions, dtype=np.float32).reshape(-1, 3)
+        self.indices = np.ascontiguousarray(self.indices, dtype=np.uint32).reshape(-1, 3)
+        if self.normals is not None:
+            self.normals = np.ascontiguousarray(self.normals, dtype=np.float32).reshape(-1, 3)
+        if self.uvs is not None:
+            self.uvs = np.ascontiguousarray(self.uvs, dtype=np.float32).reshape(-1, 2)
+
+    @property
+    def num_triangles(self) -> int:
+        return int(self.indices.shape[0])
+
+
+@dataclass
+class AreaLight:
+    """AreaLight(position, u, v, radiance) (area_light.h:17-26)."""
+    position: tuple
+    u: tuple
+    v: tuple
+    radiance: tuple
+    visible_to_camera: bool = True
+
+
+@dataclass
+class Environment:
+    """EnvironmentLight(image, rotation, intensity) [+ setSun] (environment_light.h:52-62)."""
+    rgb: np.ndarray | None               # (H, W, 3) float32, row 0 = zenith; None = flat grey
+    rotation: float = 0.0
+    intensity: float = 1.0
+    sun_direction: tuple | None = None
+    sun_cos_max: float = 1.0
+    sun_radiance: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclass
+class _Shape:
+    mesh: Mesh
+    material: int
+    emitter_base: int
+    standalone: bool
+    flags: int = 0
+
+
+@dataclass
+class Scene:
+    materials: list = field(default_factory=list)
+    shapes: list = field(default_factory=list)
+    lights: list = field(default_factory=list)     # LightDesc in Scene::lights() order
+    environment: Environment | None = None
+
+    def add_material(self, m: capi.MaterialDesc) -> int:
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def add_shape(self, mesh: Mesh, material: int) -> None:
+        mat = self.materials[material]
+        emissive = any(float(x) != 0.0 for x in mat.emission)
+        base = -1
+        if emissive:
+            base = len(self.lights)
+            p = mesh.positions[mesh.indices]
+            for t in range(mesh.num_triangles):
+                L = capi.LightDesc()
+                L.type = capi.PE_LIGHT_TRIANGLE
+                L.p0[:], L.p1[:], L.p2[:] = p[t, 0].tolist(), p[t, 1].tolist(), p[t, 2].tolist()
+                L.radiance[:] = list(mat.emission)
+                L.visible_to_camera = 1
+                self.lights.append(L)
+        self.shapes.append(_Shape(mesh, material, base, False))
+
+    def add_light(self, light: AreaLight) -> None:
+        index = len(self.lights)
+        L = capi.LightDesc()
+        L.type = capi.PE_LIGHT_AREA
+        L.p0[:] = list(map(float, light.position))
+        L.p1[:] = list(map(float, light.u))
+        L.p2[:] = list(map(float, light.v))
+        L.radiance[:] = list(map(float, light.radiance))
+        L.visible_to_camera = 1 if light.visible_to_camera else 0
+        self.lights.append(L)
+        # The quad inserted into the BVH as two black emissive Lambert triangles (scene.cpp:77-98).
+        mat = self.add_material(lambert((0.0, 0.0, 0.0), light.radiance))
+        p0 = np.asarray(light.position, dtype=np.float32)
+        u = np.asarray(light.u, dtype=np.float32)
+        v = np.asarray(light.v, dtype=np.float32)
+        p1 = p0 + u
+        p2 = p1 + v
+        p3 = p0 + v
+        n = np.cross(u, v)
+        n = (n * np.float32(1.0 / np.sqrt(np.float32(np.dot(n, n))))).astype(np.float32)
+        flags = 0 if light.visible_to_camera else capi.PE_SHAPE_CAMERA_INVISIBLE
+        for tri, uvs in (((p0, p1, p2), ((0, 0), (1, 0), (1, 1))), ((p0, p2, p3), ((0, 0), (1, 1), (0, 1)))):
+            m = Mesh(np.stack(tri), np.array([[0, 1, 2]]), np.stack([n, n, n]), np.array(uvs, dtype=np.float32))
+            self.shapes.append(_Shape(m, mat, index, True, flags))
+
+    def set_environment(self, env: Environment) -> None:
+        self.environment = env
+
+    @property
+    def num_triangles(self) -> int:
+        return sum(s.mesh.num_triangles for s in self.shapes)
+
+    def pack(self):
+        """Returns (SceneDesc, keepalive). keepalive owns every buffer the descriptor points into."""
+        keep = []
+        meshes = (capi.MeshDesc * max(1, len(self.shapes)))()
+        for i, s in enumerate(self.shapes):
+            md = meshes[i]
+            md.positions = capi.ptr(s.mesh.positions, C.c_float)
+            md.normals = capi.ptr(s.mesh.normals, C.c_float)
+            md.uvs = capi.ptr(s.mesh.uvs, C.c_float)
+            md.indices = capi.ptr(s.mesh.indices, C.c_uint32)
+            md.num_vertices = s.mesh.positions.shape[0]
+            md.num_triangles = s.mesh.num_triangles
+            md.material = s.material
+            md.emitter_base = s.emitter_base
+            md.standalone = 1 if s.standalone else 0
+            md.flags = s.flags
+            keep.append(s.mesh)
+        mats = (capi.MaterialDesc * max(1, len(self.materials)))(*self.materials)
+        lights = (capi.LightDesc * max(1, len(self.lights)))(*self.lights)
+        desc = capi.SceneDesc()
+        desc.meshes = C.cast(meshes, C.POINTER(capi.MeshDesc))
+        desc.num_meshes = len(self.shapes)
+        desc.materials = C.cast(mats, C.POINTER(capi.MaterialDesc))
+        desc.num_materials = len(self.materials)
+        desc.lights = C.cast(lights, C.POINTER(capi.LightDesc))
+        desc.num_lights = len(self.lights)
+        keep += [meshes, mats, lights]
+        if self.environment is not None:
+            e = self.environment
+            env = capi.EnvDesc()
+            if e.rgb is not None:
+                rgb = np.ascontiguousarray(e.rgb, dtype=np.float32)
+                keep.append(rgb)
+                env.rgb = capi.ptr(rgb, C.c_float)
+                env.height, env.width = int(rgb.shape[0]), int(rgb.shape[1])
+            env.rotation, env.intensity = float(e.rotation), float(e.intensity)
+            if e.sun_direction is not None:
+                env.has_sun = 1
+                env.sun_direction[:] = list(map(float, e.sun_direction))
+                env.sun_cos_max = float(e.sun_cos_max)
+                env.sun_radiance[:] = list(map(float, e.sun_radiance))
+            keep.append(env)
+            desc.environment = C.pointer(env)
+        else:
+            desc.environment = C.cast(None, C.POINTER(capi.EnvDesc))
+        return desc, keep
