@@ -52,6 +52,16 @@ PE_HD V3& operator*=(V3& a, float s) {
 }
 PE_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PE_HD V3 cross(V3 a, V3 b) { return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+// Uncontracted forms: each product is rounded before the sum, as in an IEEE build without FMA. The pragma is
+// scoped to these bodies, so they stay exact wherever they are inlined.
+PE_HD float dot_exact(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+PE_HD V3 cross_exact(V3 a, V3 b) {
+#pragma clang fp contract(off)
+    return V3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
 PE_HD float length_sq(V3 a) { return dot(a, a); }
 PE_HD float length(V3 a) { return sqrtf(length_sq(a)); }
 PE_HD V3 normalized(V3 a) {

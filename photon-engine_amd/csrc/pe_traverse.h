@@ -68,23 +68,24 @@ __device__ __forceinline__ void box_test2(const BvhNode& n, V3 o, V3 inv, float 
     t1 = bn;
 }
 
-// Lean Möller–Trumbore (bvh.cpp:302-316). Returns true and t/u/v when tmin <= t <= tmax.
+// Lean Möller–Trumbore (bvh.cpp:302-316). Returns true and t/u/v when tmin <= t <= tmax. Every dot and cross
+// product is evaluated uncontracted (dot_exact / cross_exact): hipcc would otherwise fuse them into FMAs.
 __device__ __forceinline__ bool tri_test(const TriAccel& T, V3 o, V3 d, float tmin, float tmax, float& t, float& u,
                                          float& v) {
 #pragma clang fp contract(off)
     const V3 e1 = mk(T.e1.x, T.e1.y, T.e1.z);
     const V3 e2 = mk(T.e2.x, T.e2.y, T.e2.z);
-    const V3 h = cross(d, e2);
-    const float a = dot(e1, h);
+    const V3 h = cross_exact(d, e2);
+    const float a = dot_exact(e1, h);
     if (a * a <= T.v0.w) return false;
     const float f = 1.0f / a;
     const V3 s = o - mk(T.v0.x, T.v0.y, T.v0.z);
-    u = f * dot(s, h);
+    u = f * dot_exact(s, h);
     if (!(u >= 0.0f && u <= 1.0f)) return false;
-    const V3 q = cross(s, e1);
-    v = f * dot(d, q);
+    const V3 q = cross_exact(s, e1);
+    v = f * dot_exact(d, q);
     if (!(v >= 0.0f && u + v <= 1.0f)) return false;
-    t = f * dot(e2, q);
+    t = f * dot_exact(e2, q);
     return t >= tmin && t <= tmax;
 }
 

@@ -40,6 +40,7 @@ class Device:
         self.h = h
 
     def close(self):
+        """Destroys the device. Scenes and films hold a reference to it, so only call this once they are gone."""
         if getattr(self, "h", None):
             self._lib.pe_device_destroy(self.h)
             self.h = None

@@ -20,9 +20,8 @@ def dev():
     from photon_hip import Device, device_count
 
     assert device_count() > 0, "no HIP device visible"
-    d = Device(0)
-    yield d
-    d.close()
+    # Scenes and films keep a reference to their device; it is released when the last of them is.
+    yield Device(0)
 
 
 @pytest.fixture(scope="session")
