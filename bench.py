@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Throughput of the integrator hot path on one MI355X: the product-sphere workload (a seeded 2,045,952-triangle
+displaced sphere in 16 Disney parts under a 2048 x 1024 sky with a 0.5-degree sun at 1e4 x; 1920 x 1080,
+256 spp, max depth 12), rendered by the gfx950 wavefront path tracer through the C-ABI.
+
+A step is one wavefront launch of --spl samples for every pixel of the 1080p frame (default 4, so the default
+64 timed steps are exactly the workload's 256 spp). Prints one JSON line:
+  value            Mrays/s: extension + shadow rays the kernels traced, over the timed wall time
+  spp_per_s        1080p frames' worth of samples per second;  msamples_per_s: the reference's "M samples/s"
+  roofline         closest-hit kernel: algorithmic bytes per launch (ray in/out + node and triangle bytes of the
+                   measured visits) over its HIP-event time, against 8 TB/s HBM3E
+  cpu_baseline     the reference's own CPU renderer (oracle/_ref, its own compiler flags) on this host's cores,
+                   on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "photon-engine_amd" / "python"))
+
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES, TRI_BYTES, RAY_IN_BYTES, HIT_OUT_BYTES = 64, 48, 32, 16
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def kernel_times(dev) -> dict:
+    from photon_hip import capi
+
+    out = (capi.KernelStats * 8)()
+    n = C.c_int()
+    capi.check("pe_profile_read", capi.load().pe_profile_read(dev.h, out, 8, C.byref(n)))
+    return {out[k].name.decode(): {"launches": int(out[k].launches), "ms": float(out[k].total_ms)} for k in range(n.value)}
+
+
+def cpu_baseline(scene, view, rays_per_sample: float, target_s: float) -> dict | None:
+    """The reference renderer (or, where it was not built, the oracle restatement) on this host's cores."""
+    try:
+        from oracle import bindings
+    except Exception as e:  # noqa: BLE001
+        log(f"cpu baseline unavailable: {e}")
+        return None
+    cores = len(os.sched_getaffinity(0))
+    small = view
+    if bindings.reference(strict=False) is not None:
+        t = time.time()
+        ref = bindings.RefScene(scene, strict=False)
+        log(f"cpu baseline: reference scene + BVH built in {time.time() - t:.1f} s")
+        _, _, t1 = ref.render(small, 0, 1, cores)
+        count = max(1, min(64, int(target_s / max(t1, 1e-3))))
+        _, _, secs = ref.render(small, 1, count, cores)
+        samples = small.width * small.height * count
+        msps = samples / secs / 1e6
+        return {"value": msps * rays_per_sample, "unit": "Mrays/s", "cores": cores, "kind": "reference",
+                "msamples_per_s": msps,
+                "sample": (f"Renderer::renderSamples of the same 1920x1080 product-sphere scene, samples 1..{count} "
+                           f"({secs:.1f} s on {cores} threads, reference flags -O3 -ffast-math -mavx2 -mfma); "
+                           f"M samples/s x {rays_per_sample:.3f} rays/sample counted by the GPU kernels on the "
+                           f"same sample sequence")}
+    from photon_hip import camera_perspective, settings
+
+    o = bindings.OracleScene(scene)
+    cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    rows = max(1, view.height // 16)
+    _, secs, rays = o.render(cam, st, view.width, view.height, 0, 1, threads=cores, rows=(0, rows))
+    return {"value": (rays[0] + rays[1]) / secs / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"oracle restatement, 1 spp on rows 0..{rows} of the 1920x1080 frame, {cores} threads"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--spl", type=int, default=4, help="samples per pixel per step (one wavefront launch)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="reduced scene (smoke runs only; not the workload)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
+              f"a frame shards by sample range across GPUs with one film reduction, which is not implemented.",
+              file=sys.stderr)
+        return 2
+
+    from photon_hip import Device, camera_perspective, configs, render, settings
+
+    t = time.time()
+    if args.small:
+        scene, view = configs.product_sphere(width=480, height=270, lon=256, lat=200, env_width=512, env_height=256)
+        workload = "product_sphere_small (smoke only)"
+    else:
+        scene, view = configs.product_sphere()
+        workload = "product_sphere_2M_1080p"
+    log(f"scene generated in {time.time() - t:.1f} s: {scene.num_triangles} triangles")
+    dev = Device(0)
+    t = time.time()
+    gs = dev.upload(scene)
+    log(f"uploaded (BVH build + copies) in {time.time() - t:.1f} s: {gs.stats()}")
+    cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=args.spl)
+    film = dev.film(view.width, view.height)
+    pixels = view.width * view.height
+
+    first = 0
+    for _ in range(args.warmup):
+        render(dev, gs, cam, st, film, first, args.spl)
+        first += args.spl
+    # Traversal visit counts of one step (untimed, counting kernels are slower).
+    st_count = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=args.spl)
+    st_count.count_visits = 1
+    scratch = dev.film(view.width, view.height)
+    rc = render(dev, gs, cam, st_count, scratch, first, args.spl)
+    ext_nodes = rc.node_visits / max(1, rc.extension_rays)
+    ext_tris = rc.tri_visits / max(1, rc.extension_rays)
+    sh_nodes = rc.shadow_node_visits / max(1, rc.shadow_rays)
+    sh_tris = rc.shadow_tri_visits / max(1, rc.shadow_rays)
+    del scratch
+    log(f"visits per closest-hit ray: {ext_nodes:.1f} nodes, {ext_tris:.1f} triangles; "
+        f"per shadow ray: {sh_nodes:.1f} nodes, {sh_tris:.1f} triangles")
+
+    lib_prof = dev._lib.pe_profile_enable
+    dev.synchronize()
+    lib_prof(dev.h, 1)
+    ext = shadow = 0
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r = render(dev, gs, cam, st, film, first, args.spl)
+        first += args.spl
+        ext += r.extension_rays
+        shadow += r.shadow_rays
+        if (k + 1) % 16 == 0:
+            log(f"step {k + 1}/{args.steps}")
+    dev.synchronize()
+    wall = time.perf_counter() - t0
+    kt = kernel_times(dev)
+    lib_prof(dev.h, 0)
+
+    samples = pixels * args.steps * args.spl
+    mrays = (ext + shadow) / wall / 1e6
+    isect = kt.get("intersect", {"launches": 0, "ms": 0.0})
+    bytes_per_ray = RAY_IN_BYTES + HIT_OUT_BYTES + ext_nodes * NODE_BYTES + ext_tris * TRI_BYTES
+    isect_gbs = (ext * bytes_per_ray) / (isect["ms"] * 1e-3) / 1e9 if isect["ms"] > 0 else 0.0
+    dominant = max(kt.items(), key=lambda kv: kv[1]["ms"])[0] if kt else "intersect"
+    traffic = None
+    tfile = ROOT / "profiles" / "intersect_traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get("bytes_per_launch")
+        except Exception:  # noqa: BLE001
+            traffic = None
+    result = {
+        "metric": "Mrays/s",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator: displaced sphere, sky + sun environment)",
+        "config": {"workload": workload, "triangles": scene.num_triangles, "width": view.width,
+                   "height": view.height, "spp": args.steps * args.spl, "samples_per_step": args.spl,
+                   "max_depth": view.max_bounces, "rr_depth": 3, "seed": 12345, "filter": "box 1px"},
+        "spp_per_s": round(args.steps * args.spl / wall, 3),
+        "msamples_per_s": round(samples / wall / 1e6, 2),
+        "extension_rays": ext,
+        "shadow_rays": shadow,
+        "rays_per_sample": round((ext + shadow) / samples, 4),
+        "kernels_ms": {k: round(v["ms"], 2) for k, v in kt.items()},
+        "dominant_kernel": dominant,
+        "roofline": {"kernel": "intersect", "bound": "hbm", "achieved": round(isect_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(isect_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "bytes_per_ray": round(bytes_per_ray, 1), "node_visits_per_ray": round(ext_nodes, 2),
+                     "tri_visits_per_ray": round(ext_tris, 2),
+                     "launches": isect["launches"],
+                     "avg_launch_ms": round(isect["ms"] / max(1, isect["launches"]), 4)},
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cb = cpu_baseline(scene, view, (ext + shadow) / samples, args.cpu_seconds)
+        result["cpu_baseline"] = cb
+        if cb:
+            result["gpu_vs_cpu"] = round(mrays / cb["value"], 1) if cb["value"] > 0 else None
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

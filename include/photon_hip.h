@@ -188,16 +188,30 @@ typedef struct pe_settings {
     float clamp_indirect;       /* 0 = off */
     uint64_t seed;              /* Sobol seed; the reference renderer hard-codes 12345 */
     int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together (>=1) */
-    int32_t reserved[7];
+    int32_t count_visits;       /* 1: also count BVH node / triangle visits (measurement runs; slower) */
+    int32_t reserved[6];
 } pe_settings;
 
-/* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray counters. */
+/* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray and traversal counters. */
 typedef struct pe_batch_result {
     int32_t completed;
     uint64_t samples_traced;
     uint64_t extension_rays;
     uint64_t shadow_rays;
+    uint64_t node_visits;        /* closest-hit traversal, when settings.count_visits */
+    uint64_t tri_visits;
+    uint64_t shadow_node_visits; /* any-hit traversal, when settings.count_visits */
+    uint64_t shadow_tri_visits;
 } pe_batch_result;
+
+/* Per-kernel device time of the wavefront stages, from HIP events recorded on the device's stream around
+ * every launch while profiling is enabled. No counterpart in the reference (its renderer is timed as a whole,
+ * main.cpp:302-304); it backs the roofline figures. */
+typedef struct pe_kernel_stats {
+    char name[32];
+    uint64_t launches;
+    double total_ms;
+} pe_kernel_stats;
 
 /* ---- film (device-resident accumulation; Film::addSample semantics per sample) ---------------------- */
 int pe_film_create(pe_device* dev, int width, int height, pe_film** out);
@@ -216,6 +230,10 @@ int pe_camera_perspective(const float position[3], const float target[3], const 
  * completed = 0 and a valid partial film. */
 int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
               int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result);
+
+int pe_profile_enable(pe_device* dev, int enable);
+/* Accumulated times since profiling was enabled (or last read); resets them. */
+int pe_profile_read(pe_device* dev, pe_kernel_stats* out, int capacity, int* count);
 
 /* ---- kernel-level entry points (parity and measurement) ---------------------------------------------
  * Sobol-Owen sample values: for query i, the sampler state after startPixel(px[i], py[i]),

@@ -53,6 +53,13 @@ _ORACLE_SIGS = {
     "intersect": (C.c_int, [_VP, C.c_size_t, _F, _I, _F]),
     "intersect_brute": (C.c_int, [_VP, C.c_size_t, _F, _I]),
     "occluded": (C.c_int, [_VP, C.c_size_t, _F, _P(C.c_uint8)]),
+    "trace_samples": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I,
+                                _F, _P(C.c_uint64)]),
+    "render": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                         C.c_int, C.c_int, _F, _U, _F, _P(C.c_uint64), _P(C.c_double)]),
+    "bsdf_eval": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F, _F]),
+    "bsdf_sample": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F]),
+    "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
 }
 
 _REF_SIGS = {
@@ -126,6 +133,36 @@ class OracleScene:
         self._lib.oracle_occluded(self.h, rays.shape[0], _fp(rays), capi.ptr(occ, C.c_uint8))
         return occ
 
+    def trace_samples(self, camera, settings, width, height, px, py, s):
+        """Per-sample radiance (tracePixel + PathTracer::Li). Returns (L (n, 3), (extension, shadow) rays)."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        rays = (C.c_uint64 * 2)()
+        rc = self._lib.oracle_trace_samples(self.h, C.byref(camera), C.byref(settings), width, height, px.shape[0],
+                                            capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32), capi.ptr(s, C.c_int32),
+                                            _fp(L), rays)
+        if rc != 0:
+            raise RuntimeError("oracle_trace_samples failed (feature outside the restated scope)")
+        return L, (rays[0], rays[1])
+
+    def render(self, camera, settings, width, height, first, count, threads=1, rows=None, film=None):
+        """Accumulates samples [first, first+count) of every pixel (rows [r0, r1) if given) into film
+        (sum (H,W,3), count (H,W), lum2 (H,W)); returns (film, seconds, (extension, shadow) rays)."""
+        if film is None:
+            film = (np.zeros((height, width, 3), np.float32), np.zeros((height, width), np.uint32),
+                    np.zeros((height, width), np.float32))
+        r0, r1 = rows if rows is not None else (0, height)
+        rays = (C.c_uint64 * 2)()
+        secs = C.c_double()
+        rc = self._lib.oracle_render(self.h, C.byref(camera), C.byref(settings), width, height, first, count, threads,
+                                     r0, r1, _fp(film[0]), capi.ptr(film[1], C.c_uint32), _fp(film[2]), rays,
+                                     C.byref(secs))
+        if rc != 0:
+            raise RuntimeError("oracle_render failed (feature outside the restated scope)")
+        return film, secs.value, (rays[0], rays[1])
+
 
 class RefScene:
     """engine::Scene built by the reference's own code from the same descriptor."""
@@ -183,6 +220,52 @@ class RefScene:
                              view.max_bounces, shadow_quality, filter_type, filter_width, first, count, threads,
                              _fp(mean), capi.ptr(cnt, C.c_uint32), C.byref(secs))
         return mean, cnt, secs.value
+
+
+def bsdf_eval(lib_kind: str, material, normal, tangent, front, wo, wi):
+    """lib_kind 'oracle' or 'ref': Material::evalPdf on n tuples -> (f (n,3), pdf (n,))."""
+    n = normal.shape[0]
+    args = [np.ascontiguousarray(a, np.float32) for a in (normal, tangent)]
+    front = np.ascontiguousarray(front, np.int32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    if lib_kind == "oracle":
+        fn = oracle().oracle_bsdf_eval
+    else:
+        fn = reference(strict=(lib_kind == "ref_strict")).ref_bsdf_eval
+    rc = fn(C.byref(material), n, _fp(args[0]), _fp(args[1]), capi.ptr(front, C.c_int32), _fp(wo), _fp(wi), _fp(f),
+            _fp(pdf))
+    if rc != 0:
+        raise RuntimeError("bsdf_eval failed")
+    return f, pdf
+
+
+def bsdf_sample(lib_kind: str, material, normal, tangent, front, wo, u3):
+    """Material::sampleFull -> (n, 14): ok, wi, f, pdf, delta, subsurface, passThrough, weight."""
+    n = normal.shape[0]
+    normal = np.ascontiguousarray(normal, np.float32)
+    tangent = np.ascontiguousarray(tangent, np.float32)
+    front = np.ascontiguousarray(front, np.int32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    u3 = np.ascontiguousarray(u3, np.float32)
+    out = np.zeros((n, 14), np.float32)
+    fn = oracle().oracle_bsdf_sample if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).ref_bsdf_sample
+    rc = fn(C.byref(material), n, _fp(normal), _fp(tangent), capi.ptr(front, C.c_int32), _fp(wo), _fp(u3), _fp(out))
+    if rc != 0:
+        raise RuntimeError("bsdf_sample failed")
+    return out
+
+
+def microfacet(lib_kind: str, which: int, a, b, c=None):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    c = np.ascontiguousarray(c if c is not None else np.zeros_like(a), np.float32)
+    out = np.zeros_like(a)
+    fn = oracle().oracle_microfacet if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).ref_microfacet
+    fn(which, a.shape[0], _fp(a), _fp(b), _fp(c), _fp(out))
+    return out
 
 
 def sobol_oracle(seed, px, py, idx, dim):

@@ -1,0 +1,1105 @@
+// oracle_integrator.cpp — restated light sources, light selection, surface interaction, ray spawning,
+// PathTracer::Li, tracePixel and the per-pixel sample loop of Renderer::renderSamples (test
+// infrastructure only). Scope: what the box/icosphere and product-sphere workloads use — Disney and Lambert
+// surfaces, area/triangle/environment lights (with sun), exhaustive light selection, one-sample NEE + BSDF
+// MIS with the power heuristic, Russian roulette, the NaN guard and Film::addSample.
+#include "oracle.h"
+#include "oracle_scene.h"
+#include "oracle_shading.h"
+
+#include <atomic>
+#include <chrono>
+#include <cstring>
+#include <stdexcept>
+#include <thread>
+
+namespace oracle {
+
+namespace {
+
+V3 c3(const float* v) { return V3(v[0], v[1], v[2]); }
+
+// ------------------------------------------------------------------------------ float_bits.h / ray_spawn.h
+float nextFloatUp(float v) {  // float_bits.h:30-37
+    if (!finiteFloat(v) && v > 0.0f) return v;
+    if (v == -0.0f) v = 0.0f;
+    uint32_t ui;
+    std::memcpy(&ui, &v, 4);
+    if (v >= 0.0f) ++ui;
+    else --ui;
+    std::memcpy(&v, &ui, 4);
+    return v;
+}
+float nextFloatDown(float v) {  // float_bits.h:40-47
+    if (!finiteFloat(v) && v < 0.0f) return v;
+    if (v == 0.0f) v = -0.0f;
+    uint32_t ui;
+    std::memcpy(&ui, &v, 4);
+    if (v > 0.0f) --ui;
+    else ++ui;
+    std::memcpy(&v, &ui, 4);
+    return v;
+}
+float floatGamma(int n) {  // float_bits.h:51-55
+    const float eps = 5.9604645e-8f;
+    return (static_cast<float>(n) * eps) / (1.0f - static_cast<float>(n) * eps);
+}
+
+struct SI {  // geometry/surface_interaction.h:16-54
+    V3 point, normal, ng, tangent;
+    float uvx = 0, uvy = 0, t = -1.0f;
+    int material = -1, tri = -1, emitter = -1;
+    bool frontFace = true, cameraInvisible = false, hasSmoothPoint = false;
+    V3 pError, pSmooth;
+};
+
+V3 offsetRayOrigin(V3 p, V3 pError, V3 n, V3 w) {  // ray_spawn.h:25-36
+    const float d = dot(vabs(n), pError);
+    V3 offset = n * d;
+    if (dot(w, n) < 0.0f) offset = -offset;
+    V3 po = p + offset;
+    for (int i = 0; i < 3; ++i) {
+        if (offset[i] > 0.0f) po[i] = nextFloatUp(po[i]);
+        else if (offset[i] < 0.0f) po[i] = nextFloatDown(po[i]);
+    }
+    return po;
+}
+V3 spawnOrigin(const SI& si, V3 dir) {  // ray_spawn.h:40-43
+    const V3 n = lengthSq(si.ng) > 0.0f ? si.ng : si.normal;
+    return offsetRayOrigin(si.point, si.pError, n, dir);
+}
+V3 shadowOrigin(const SI& si, V3 toLight) {  // ray_spawn.h:52-68
+    const V3 n = lengthSq(si.ng) > 0.0f ? si.ng : si.normal;
+    V3 p = si.point;
+    if (si.hasSmoothPoint) {
+        const V3 shadingOut = si.frontFace ? si.normal : -si.normal;
+        const V3 ngOut = dot(n, shadingOut) >= 0.0f ? n : -n;
+        const float NL = dot(shadingOut, toLight);
+        if (NL > 0.0f) {
+            constexpr float kCutoff = 0.1f;
+            const float NgL = dot(ngOut, toLight);
+            const float amount = NL < kCutoff ? clampf(2.0f - (NgL + NL) / kCutoff, 0.0f, 1.0f)
+                                              : clampf(1.0f - NgL / kCutoff, 0.0f, 1.0f);
+            if (amount > 0.0f) p += (si.pSmooth - si.point) * amount;
+        }
+    }
+    return offsetRayOrigin(p, si.pError, n, toLight);
+}
+Ray spawnRay(const SI& si, V3 dir) {  // ray_spawn.h:71-73
+    Ray r;
+    r.o = spawnOrigin(si, dir);
+    r.d = dir;
+    r.tMin = 0.0f;
+    r.tMax = 1e30f;
+    return r;
+}
+bool spawnRayTo(const SI& si, V3 target, Ray& out) {  // ray_spawn.h:78-87
+    const V3 toTarget = target - si.point;
+    const float len = length(toTarget);
+    if (!(len > 0.0f)) return false;
+    const V3 origin = shadowOrigin(si, toTarget / len);
+    const V3 d = target - origin;
+    const float dist = length(d);
+    const float slack = std::max(dist * 1e-4f, 2e-6f * maxComp(vabs(target)));
+    if (!(dist > slack)) return false;
+    out.o = origin;
+    out.d = d / dist;
+    out.tMin = 0.0f;
+    out.tMax = dist - slack;
+    return true;
+}
+
+// triangle.cpp:54-133 (fillHit) for the winning triangle, with (t, u, v) of the closest-hit test.
+void fillHit(const Scene& s, const Ray& ray, const Hit& h, SI& si) {
+    const TriRec& tr = s.tris[static_cast<size_t>(h.tri)];
+    const V3 v0 = s.pos[tr.v[0]], v1 = s.pos[tr.v[1]], v2 = s.pos[tr.v[2]];
+    const V3 n0 = tr.hasNormals ? s.nrm[tr.v[0]] : V3(0.0f), n1 = tr.hasNormals ? s.nrm[tr.v[1]] : V3(0.0f),
+             n2 = tr.hasNormals ? s.nrm[tr.v[2]] : V3(0.0f);
+    const float u = h.u, v = h.v;
+    const V3 edge1 = v1 - v0, edge2 = v2 - v0;
+    si.t = h.t;
+    const float w = 1.0f - u - v;
+    const V3 a0 = v0 * w, a1 = v1 * u, a2 = v2 * v;
+    si.point = a0 + a1 + a2;
+    const float iso = floatGamma(32) * std::max({maxComp(vabs(v0)), maxComp(vabs(v1)), maxComp(vabs(v2)), 1e-20f});
+    si.pError = (vabs(a0) + vabs(a1) + vabs(a2)) * floatGamma(16) + V3(iso);
+    const V3 geomNormal = normalized(cross(edge1, edge2));
+    const bool smooth = lengthSq(n0) > 0.0f && lengthSq(n1) > 0.0f && lengthSq(n2) > 0.0f;
+    const V3 shadingNormal = smooth ? normalized(n0 * w + n1 * u + n2 * v) : geomNormal;
+    si.ng = geomNormal;
+    si.frontFace = dot(ray.d, shadingNormal) < 0.0f;  // surface_interaction.h:48-52
+    si.normal = si.frontFace ? shadingNormal : -shadingNormal;
+    if (lengthSq(si.ng) == 0.0f) si.ng = shadingNormal;
+    si.hasSmoothPoint = false;
+    if (smooth) {
+        const V3 p = si.point;
+        const V3 m0 = normalized(n0), m1 = normalized(n1), m2 = normalized(n2);
+        const V3 lift = m0 * (w * std::max(0.0f, dot(v0 - p, m0))) + m1 * (u * std::max(0.0f, dot(v1 - p, m1))) +
+                        m2 * (v * std::max(0.0f, dot(v2 - p, m2)));
+        if (lengthSq(lift) > 0.0f) {
+            si.pSmooth = p + lift;
+            si.hasSmoothPoint = true;
+        }
+    }
+    const float* uv = s.uv.data();
+    const float u0x = tr.hasUv ? uv[2 * tr.v[0]] : 0.0f, u0y = tr.hasUv ? uv[2 * tr.v[0] + 1] : 0.0f;
+    const float u1x = tr.hasUv ? uv[2 * tr.v[1]] : 0.0f, u1y = tr.hasUv ? uv[2 * tr.v[1] + 1] : 0.0f;
+    const float u2x = tr.hasUv ? uv[2 * tr.v[2]] : 0.0f, u2y = tr.hasUv ? uv[2 * tr.v[2] + 1] : 0.0f;
+    si.uvx = u0x * w + u1x * u + u2x * v;
+    si.uvy = u0y * w + u1y * u + u2y * v;
+    const float d1x = u1x - u0x, d1y = u1y - u0y, d2x = u2x - u0x, d2y = u2y - u0y;
+    const float r = d1x * d2y - d2x * d1y;
+    V3 tangent;
+    if (std::abs(r) < 1e-6f) {
+        const V3 n = si.normal;
+        const float sign = std::copysign(1.0f, n.z);
+        const float a = -1.0f / (sign + n.z);
+        const float b = n.x * n.y * a;
+        tangent = V3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    } else {
+        const float invR = 1.0f / r;
+        tangent = (edge1 * d2y - edge2 * d1y) * invR;
+    }
+    si.tangent = normalized(tangent);
+    si.material = tr.material;
+    si.tri = h.tri;
+    si.emitter = tr.emitter;
+    si.cameraInvisible = tr.cameraInvisible;
+}
+
+// ----------------------------------------------------------------------------------- lights
+struct LightSample {  // light.h:35-42
+    V3 wi, Li;
+    float pdf = 0.0f, distance = 0.0f;
+    bool valid() const { return pdf > 0.0f && !isBlack(Li); }
+};
+
+float receiverCosine(V3 n, V3 toLight) {  // light.h:95-98
+    const float c = dot(n, toLight);
+    return c > 0.05f ? c : 0.05f;
+}
+float areaSolidAnglePdf(float distSq, float cosAbs, float area) {  // area_light.cpp:18-21
+    if (!(cosAbs > 1e-6f) || !(area > 0.0f) || !(distSq > 0.0f)) return 0.0f;
+    return distSq / (area * cosAbs);
+}
+constexpr float kMinSolidAngle = 1e-3f;
+float safeAsin(float x) { return std::asin(clampf(x, -1.0f, 1.0f)); }
+float angleBetween(V3 a, V3 b) {  // area_light.cpp:30-33
+    if (dot(a, b) < 0.0f) return PI - 2.0f * safeAsin(length(a + b) * 0.5f);
+    return 2.0f * safeAsin(length(b - a) * 0.5f);
+}
+struct SphericalRect {  // area_light.cpp:39-45
+    Frame frame;
+    float x0 = 0, y0 = 0, z0 = 0, x1 = 0, y1 = 0, b0 = 0, b1 = 0, g2g3 = 0, solidAngle = 0.0f;
+};
+SphericalRect setupSphericalRect(V3 ref, V3 corner, V3 ex, V3 ey) {  // area_light.cpp:47-72
+    SphericalRect r;
+    const float exl = length(ex), eyl = length(ey);
+    const V3 x = ex / exl, y = ey / eyl;
+    V3 z = cross(x, y);
+    const V3 d = corner - ref;
+    r.z0 = dot(d, z);
+    if (r.z0 > 0.0f) {
+        z = -z;
+        r.z0 = -r.z0;
+    }
+    r.frame = Frame(x, y, z);
+    r.x0 = dot(d, x);
+    r.y0 = dot(d, y);
+    r.x1 = r.x0 + exl;
+    r.y1 = r.y0 + eyl;
+    if (r.z0 > -1e-7f) return r;
+    const V3 v00(r.x0, r.y0, r.z0), v01(r.x0, r.y1, r.z0), v10(r.x1, r.y0, r.z0), v11(r.x1, r.y1, r.z0);
+    const V3 n0 = normalized(cross(v00, v10)), n1 = normalized(cross(v10, v11));
+    const V3 n2 = normalized(cross(v11, v01)), n3 = normalized(cross(v01, v00));
+    const float g0 = angleBetween(-n0, n1), g1 = angleBetween(-n1, n2);
+    const float g2 = angleBetween(-n2, n3), g3 = angleBetween(-n3, n0);
+    r.solidAngle = g0 + g1 + g2 + g3 - TWO_PI;
+    r.b0 = n0.z;
+    r.b1 = n2.z;
+    r.g2g3 = g2 + g3;
+    return r;
+}
+V3 sampleSphericalRect(const SphericalRect& r, float ux, float uy) {  // area_light.cpp:75-92
+    const float au = ux * r.solidAngle - r.g2g3;
+    const float fu = (std::cos(au) * r.b0 - r.b1) / std::sin(au);
+    float cu = std::copysign(1.0f, fu) / std::sqrt(fu * fu + r.b0 * r.b0);
+    cu = clampf(cu, -0.9999999f, 0.9999999f);
+    float xu = -(cu * r.z0) / std::sqrt(std::max(0.0f, 1.0f - cu * cu));
+    xu = clampf(xu, r.x0, r.x1);
+    const float dd = std::sqrt(xu * xu + r.z0 * r.z0);
+    const float h0 = r.y0 / std::sqrt(dd * dd + r.y0 * r.y0);
+    const float h1 = r.y1 / std::sqrt(dd * dd + r.y1 * r.y1);
+    const float hv = h0 + uy * (h1 - h0);
+    const float hv2 = hv * hv;
+    const float yv = hv2 < 1.0f - 1e-6f ? hv * dd / std::sqrt(1.0f - hv2) : r.y1;
+    return r.frame.toWorld(V3(xu, clampf(yv, r.y0, r.y1), r.z0));
+}
+
+struct LightRec {
+    int type;
+    V3 p0, p1, p2, radiance;
+    // AreaLight (area_light.h:20-27): position, u, v, normal, area, rectangle flag
+    V3 normal;
+    float area = 0.0f;
+    bool rectangle = false;
+};
+
+LightRec makeLight(const pe_light_desc& d) {
+    LightRec L;
+    L.type = d.type;
+    L.p0 = c3(d.p0);
+    L.p1 = c3(d.p1);
+    L.p2 = c3(d.p2);
+    L.radiance = c3(d.radiance);
+    if (d.type == PE_LIGHT_AREA) {
+        const V3 n = cross(L.p1, L.p2);
+        L.area = length(n);
+        L.normal = normalized(n);
+        const float lu = length(L.p1), lv = length(L.p2);
+        L.rectangle = lu > 0.0f && lv > 0.0f && std::abs(dot(L.p1, L.p2)) <= 1e-4f * lu * lv;
+    } else {  // TriangleLight (area_light.cpp:195-200)
+        const V3 n = cross(L.p1 - L.p0, L.p2 - L.p0);
+        const float len = length(n);
+        L.area = 0.5f * len;
+        L.normal = len > 0.0f ? n / len : V3(0, 1, 0);
+    }
+    return L;
+}
+
+float lightImportance(const LightRec& L, V3 p, V3 n) {  // area_light.cpp:100-112 and :202-212
+    const V3 c = L.type == PE_LIGHT_AREA ? L.p0 + (L.p1 + L.p2) * 0.5f : (L.p0 + L.p1 + L.p2) * (1.0f / 3.0f);
+    const V3 d = c - p;
+    const float dist2 = lengthSq(d);
+    const float lum = std::max(0.0f, luminance(L.radiance));
+    if (!(dist2 > 0.0f)) return lum * TWO_PI;
+    const V3 w = d / std::sqrt(dist2);
+    const float cosL = std::max(std::abs(dot(w, L.normal)), 0.2f);
+    const float omega = std::min(L.area * cosL / dist2, TWO_PI);
+    return lum * omega * receiverCosine(n, w);
+}
+
+LightSample lightSampleLi(const LightRec& L, V3 ref, float ux, float uy) {
+    LightSample ls;
+    if (L.type == PE_LIGHT_AREA) {  // area_light.cpp:114-157
+        if (L.rectangle) {
+            const SphericalRect r = setupSphericalRect(ref, L.p0, L.p1, L.p2);
+            if (r.solidAngle >= kMinSolidAngle) {
+                const V3 toLight = sampleSphericalRect(r, ux, uy);
+                const float dist = length(toLight);
+                if (!(dist > 0.0f)) return ls;
+                ls.distance = dist;
+                ls.wi = toLight / dist;
+                ls.Li = L.radiance;
+                ls.pdf = 1.0f / r.solidAngle;
+                return ls;
+            }
+        }
+        const V3 pLight = L.p0 + L.p1 * ux + L.p2 * uy;
+        const V3 toLight = pLight - ref;
+        const float distSq = lengthSq(toLight);
+        ls.distance = std::sqrt(distSq);
+        if (ls.distance > 0.0f) {
+            ls.wi = toLight / ls.distance;
+            const float pdf = areaSolidAnglePdf(distSq, std::abs(dot(ls.wi, L.normal)), L.area);
+            if (pdf > 0.0f) {
+                ls.Li = L.radiance;
+                ls.pdf = pdf;
+            } else {
+                ls.Li = V3(0.0f);
+                ls.pdf = 0.0f;
+            }
+        }
+        return ls;
+    }
+    // TriangleLight::sampleLi (area_light.cpp:235-252)
+    if (!(L.area > 0.0f) || isBlack(L.radiance)) return ls;
+    float b1, b2;
+    uniformSampleTriangle(ux, uy, b1, b2);
+    const V3 pLight = L.p0 * (1.0f - b1 - b2) + L.p1 * b1 + L.p2 * b2;
+    const V3 toLight = pLight - ref;
+    const float distSq = lengthSq(toLight);
+    ls.distance = std::sqrt(distSq);
+    if (!(ls.distance > 0.0f)) return ls;
+    ls.wi = toLight / ls.distance;
+    const float pdf = areaSolidAnglePdf(distSq, std::abs(dot(ls.wi, L.normal)), L.area);
+    if (!(pdf > 0.0f)) return ls;
+    ls.Li = L.radiance;
+    ls.pdf = pdf;
+    return ls;
+}
+
+float lightPdfLiHit(const LightRec& L, V3 ref, V3 hitPoint) {
+    if (L.type == PE_LIGHT_AREA) {  // AreaLight::pdfLi (area_light.cpp:163-187)
+        const V3 d = hitPoint - L.p0;
+        const float scale = std::max(length(L.p1) + length(L.p2), 1.0f);
+        if (std::abs(dot(d, L.normal)) > 1e-3f * scale) return 0.0f;
+        const float uu = dot(L.p1, L.p1), vv = dot(L.p2, L.p2), uv = dot(L.p1, L.p2);
+        const float du = dot(d, L.p1), dv = dot(d, L.p2);
+        const float det = uu * vv - uv * uv;
+        if (!(std::abs(det) > 1e-12f * uu * vv)) return 0.0f;
+        const float s = (vv * du - uv * dv) / det;
+        const float t = (uu * dv - uv * du) / det;
+        const float eps = 1e-3f;
+        if (s < -eps || t < -eps || s > 1.0f + eps || t > 1.0f + eps) return 0.0f;
+        if (L.rectangle) {
+            const float omega = setupSphericalRect(ref, L.p0, L.p1, L.p2).solidAngle;
+            if (omega >= kMinSolidAngle) return 1.0f / omega;
+        }
+        const V3 toLight = hitPoint - ref;
+        const float distSq = lengthSq(toLight);
+        if (distSq <= 0.0f) return 0.0f;
+        const V3 wi = toLight / std::sqrt(distSq);
+        return areaSolidAnglePdf(distSq, std::abs(dot(wi, L.normal)), L.area);
+    }
+    const V3 toLight = hitPoint - ref;  // TriangleLight::pdfLiHit (area_light.cpp:226-231)
+    const float distSq = lengthSq(toLight);
+    if (!(distSq > 0.0f)) return 0.0f;
+    return areaSolidAnglePdf(distSq, std::abs(dot(toLight / std::sqrt(distSq), L.normal)), L.area);
+}
+
+// ------------------------------------------------------------------------------ environment_light.cpp
+struct Distribution {  // environment_light.h:108-115
+    float funcInt = 0.0f;
+    std::vector<float> func, marginalCdf, condCdf;
+    bool valid() const { return funcInt > 0.0f && !marginalCdf.empty(); }
+};
+
+struct Env {
+    int w = 0, h = 0;
+    const float* rgb = nullptr;
+    float rotation = 0.0f, intensity = 1.0f, meanLuminance = 0.0f;
+    V3 sunDir{0, 1, 0}, sunRadiance;
+    float sunCosMax = 1.0f, sunPdfWeight = 0.0f;
+    Distribution full, mis;
+
+    V3 pixel(int x, int y) const { return c3(rgb + 3 * (static_cast<size_t>(y) * w + x)); }
+
+    void buildCdf(Distribution& d) const {  // environment_light.cpp:149-172
+        d.marginalCdf.assign(static_cast<size_t>(h) + 1, 0.0f);
+        d.condCdf.assign(static_cast<size_t>(h) * static_cast<size_t>(w + 1), 0.0f);
+        for (int y = 0; y < h; ++y) {
+            float* row = d.condCdf.data() + static_cast<size_t>(y) * static_cast<size_t>(w + 1);
+            row[0] = 0.0f;
+            for (int x = 0; x < w; ++x) row[x + 1] = row[x] + d.func[static_cast<size_t>(y) * w + x];
+            d.marginalCdf[static_cast<size_t>(y) + 1] = d.marginalCdf[static_cast<size_t>(y)] + row[w];
+        }
+        d.funcInt = d.marginalCdf[static_cast<size_t>(h)];
+        if (!(d.funcInt > 0.0f)) {
+            d.funcInt = 0.0f;
+            d.marginalCdf.clear();
+            return;
+        }
+        const float inv = 1.0f / d.funcInt;
+        for (float& c : d.marginalCdf) c *= inv;
+        for (int y = 0; y < h; ++y) {
+            float* row = d.condCdf.data() + static_cast<size_t>(y) * static_cast<size_t>(w + 1);
+            const float rowSum = row[w];
+            if (rowSum <= 0.0f) continue;
+            const float invRow = 1.0f / rowSum;
+            for (int x = 0; x <= w; ++x) row[x] *= invRow;
+        }
+    }
+
+    void build(bool misCompensation) {  // environment_light.cpp:90-147
+        if (!rgb || w <= 0 || h <= 0) return;
+        full.func.assign(static_cast<size_t>(w) * h, 0.0f);
+        std::vector<float> sinRow(static_cast<size_t>(h));
+        double weighted = 0.0, weights = 0.0;
+        for (int y = 0; y < h; ++y) {
+            const float sinTheta = std::sin(PI * (static_cast<float>(y) + 0.5f) / static_cast<float>(h));
+            sinRow[static_cast<size_t>(y)] = sinTheta;
+            for (int x = 0; x < w; ++x) {
+                float L = luminance(pixel(x, y));
+                if (!(L > 0.0f)) L = 0.0f;
+                full.func[static_cast<size_t>(y) * w + x] = L * sinTheta;
+                weighted += static_cast<double>(L) * sinTheta;
+                weights += sinTheta;
+            }
+        }
+        buildCdf(full);
+        if (!full.valid()) return;
+        const float meanL = weights > 0.0 ? static_cast<float>(weighted / weights) : 0.0f;
+        mis.func.resize(full.func.size());
+        double misTotal = 0.0, fullTotal = 0.0;
+        for (int y = 0; y < h; ++y) {
+            const float s = sinRow[static_cast<size_t>(y)];
+            for (int x = 0; x < w; ++x) {
+                const size_t i = static_cast<size_t>(y) * w + x;
+                float L = luminance(pixel(x, y));
+                if (!(L > 0.0f)) L = 0.0f;
+                const float excess = L - meanL;
+                mis.func[i] = excess > 1e-4f * meanL ? excess * s : 0.0f;
+                misTotal += mis.func[i];
+                fullTotal += full.func[i];
+            }
+        }
+        bool comp = misCompensation && misTotal > 1e-3 * fullTotal;
+        if (comp) {
+            buildCdf(mis);
+            comp = mis.valid();
+        }
+        if (!comp) mis = full;
+    }
+
+    V3 power() const {  // environment_light.cpp:308-323
+        if (!rgb || w <= 0) return V3(0.5f) * intensity * 4.0f * PI;
+        V3 sum(0.0f);
+        float wsum = 0.0f;
+        for (int y = 0; y < h; y += 4) {
+            const float s = std::sin(PI * (static_cast<float>(y) + 0.5f) / static_cast<float>(h));
+            for (int x = 0; x < w; x += 4) {
+                sum += pixel(x, y) * s;
+                wsum += s;
+            }
+        }
+        return (wsum > 0.0f ? sum / wsum : sum) * intensity * 4.0f * PI;
+    }
+
+    void setSun(V3 direction, float cosMax, V3 radiance) {  // environment_light.cpp:168-182
+        sunDir = normalized(direction);
+        sunCosMax = clampf(cosMax, -0.999f, 0.9999999f);
+        sunRadiance = radiance * intensity;
+        const float omega = TWO_PI * (1.0f - sunCosMax);
+        const float sunPower = std::max(0.0f, luminance(sunRadiance)) * omega;
+        const float skyPower = 4.0f * PI * meanLuminance;
+        if (!(sunPower > 0.0f)) {
+            sunPdfWeight = 0.0f;
+            return;
+        }
+        sunPdfWeight = clampf(sunPower / (sunPower + skyPower), 0.15f, 0.85f);
+        if (!full.valid()) sunPdfWeight = 1.0f;
+    }
+
+    // directionToEquirect (environment_light.h:22-29)
+    void toEquirect(V3 dir, float& u, float& v) const {
+        const V3 d = normalized(dir);
+        const float theta = std::acos(clampf(d.y, -1.0f, 1.0f));
+        float phi = std::atan2(-d.z, d.x) + PI + rotation;
+        phi = std::fmod(phi, TWO_PI);
+        if (phi < 0.0f) phi += TWO_PI;
+        u = phi / TWO_PI;
+        v = theta / PI;
+    }
+    V3 equirectDirection(float u, float v) const {  // environment_light.h:33-39
+        const float theta = v * PI;
+        const float a = u * TWO_PI - PI - rotation;
+        const float st = std::sin(theta);
+        return V3(st * std::cos(a), std::cos(theta), -st * std::sin(a));
+    }
+    V3 sampleEquirect(float u, float v) const {  // environment_light.cpp:43-62
+        u -= std::floor(u);
+        v = clampf(v, 0.0f, 1.0f);
+        const float x = u * static_cast<float>(w) - 0.5f;
+        const float y = v * static_cast<float>(h) - 0.5f;
+        const int x0 = static_cast<int>(std::floor(x)), y0 = static_cast<int>(std::floor(y));
+        const float fx = x - static_cast<float>(x0), fy = y - static_cast<float>(y0);
+        auto wrapX = [this](int i) {
+            i %= w;
+            return i < 0 ? i + w : i;
+        };
+        auto clampY = [this](int i) { return std::clamp(i, 0, h - 1); };
+        const int xa = wrapX(x0), xb = wrapX(x0 + 1), ya = clampY(y0), yb = clampY(y0 + 1);
+        const V3 c0 = pixel(xa, ya) * (1.0f - fx) + pixel(xb, ya) * fx;
+        const V3 c1 = pixel(xa, yb) * (1.0f - fx) + pixel(xb, yb) * fx;
+        return c0 * (1.0f - fy) + c1 * fy;
+    }
+    V3 evalSky(V3 dir) const {  // environment_light.cpp:301-306
+        if (!rgb || w <= 0) return V3(0.5f) * intensity;
+        float u, v;
+        toEquirect(dir, u, v);
+        return sampleEquirect(u, v) * intensity;
+    }
+    V3 eval(V3 dir) const {  // environment_light.cpp:296-299
+        V3 sun(0.0f);
+        if (sunPdfWeight > 0.0f && dot(normalized(dir), sunDir) >= sunCosMax) sun = sunRadiance;
+        return evalSky(dir) + sun;
+    }
+    float solidAnglePdf(const Distribution& dist, V3 wi) const {  // environment_light.cpp:266-282
+        if (!dist.valid() || w <= 0 || h <= 0) return 0.25f * INV_PI;
+        const V3 d = normalized(wi);
+        float u, v;
+        toEquirect(d, u, v);
+        int x = std::min(static_cast<int>(u * static_cast<float>(w)), w - 1);
+        int y = std::min(static_cast<int>(v * static_cast<float>(h)), h - 1);
+        if (x < 0) x = 0;
+        if (y < 0) y = 0;
+        const float sinTheta = std::sqrt(std::max(0.0f, 1.0f - d.y * d.y));
+        if (sinTheta < 1e-4f) return 0.0f;
+        const float f = dist.func[static_cast<size_t>(y) * w + x];
+        return (f / dist.funcInt) * static_cast<float>(w * h) / (TWO_PI * PI * sinTheta);
+    }
+    float pdfLi(V3 wi) const {  // environment_light.cpp:288-294
+        if (sunPdfWeight <= 0.0f) return solidAnglePdf(mis, wi);
+        float pdf = 0.0f;
+        if (dot(normalized(wi), sunDir) >= sunCosMax) pdf += sunPdfWeight / (TWO_PI * (1.0f - sunCosMax));
+        if (sunPdfWeight < 1.0f) pdf += (1.0f - sunPdfWeight) * solidAnglePdf(mis, wi);
+        return pdf;
+    }
+    static void sampleCdf(const float* cdf, int n, float u, int& index, float& pdf, float& coord) {  // :22-36
+        const float* end = cdf + n + 1;
+        const float* it = std::upper_bound(cdf, end, u);
+        int i = static_cast<int>(it - cdf) - 1;
+        if (i < 0) i = 0;
+        if (i >= n) i = n - 1;
+        pdf = cdf[i + 1] - cdf[i];
+        float du = pdf > 0.0f ? (u - cdf[i]) / pdf : 0.0f;
+        du = clampf(du, 0.0f, 1.0f);
+        index = i;
+        coord = (static_cast<float>(i) + du) / static_cast<float>(n);
+    }
+    LightSample sampleImage(const Distribution& d, float sx, float sy) const {  // :218-237
+        LightSample ls;
+        ls.distance = INFINITY_F;
+        if (!d.valid() || w <= 0) {
+            ls.wi = uniformSampleSphere(sx, sy);
+            ls.pdf = 0.25f * INV_PI;
+            ls.Li = eval(ls.wi);
+            return ls;
+        }
+        int ri, ci;
+        float rp, rc, cp, cc;
+        sampleCdf(d.marginalCdf.data(), h, sy, ri, rp, rc);
+        sampleCdf(d.condCdf.data() + static_cast<size_t>(ri) * static_cast<size_t>(w + 1), w, sx, ci, cp, cc);
+        const float theta = rc * PI;
+        const float sinTheta = std::max(std::sin(theta), 1e-6f);
+        ls.wi = normalized(equirectDirection(cc, rc));
+        ls.pdf = (rp * cp) * static_cast<float>(w * h) / (TWO_PI * PI * sinTheta);
+        ls.Li = eval(ls.wi);
+        return ls;
+    }
+    LightSample sampleLi(float sx, float sy) const {  // :191-216
+        LightSample ls;
+        ls.distance = INFINITY_F;
+        if (sunPdfWeight > 0.0f) {
+            if (sx < sunPdfWeight) {
+                const float ux = std::min(sx / sunPdfWeight, 0.99999994f);
+                const float cosT = 1.0f - ux * (1.0f - sunCosMax);
+                const float sinT = std::sqrt(std::max(0.0f, 1.0f - cosT * cosT));
+                const float phi = TWO_PI * sy;
+                const Frame f(sunDir);
+                ls.wi = normalized(f.toWorld(V3(sinT * std::cos(phi), sinT * std::sin(phi), cosT)));
+                ls.Li = eval(ls.wi);
+                ls.pdf = pdfLi(ls.wi);
+                return ls;
+            }
+            if (sunPdfWeight >= 1.0f) return ls;
+            sx = std::min((sx - sunPdfWeight) / (1.0f - sunPdfWeight), 0.99999994f);
+        }
+        LightSample img = sampleImage(mis, sx, sy);
+        if (sunPdfWeight > 0.0f && img.pdf > 0.0f) img.pdf = pdfLi(img.wi);
+        return img;
+    }
+    float importance(V3 n) const {  // :184-189
+        float imp = PI * meanLuminance;
+        if (sunPdfWeight > 0.0f)
+            imp += std::max(0.0f, luminance(sunRadiance)) * TWO_PI * (1.0f - sunCosMax) * receiverCosine(n, sunDir);
+        return imp;
+    }
+};
+
+// ------------------------------------------------------------------------------------ light_sampler.cpp
+constexpr float kUniformShare = 0.15f;
+constexpr int kMaxExhaustive = 33;
+
+struct Integrator {
+    std::vector<LightRec> lights;
+    bool hasEnv = false;
+    Env env;
+    int count = 0;
+
+    int explicitLights() const { return static_cast<int>(lights.size()); }
+};
+
+struct LightContext {  // LightSampler::Context (light_sampler.h:39-57), exhaustive and uniform modes
+    const Integrator* owner = nullptr;
+    bool uniform = true;
+    float pmfs[kMaxExhaustive] = {};
+    int topCount = 0;
+
+    int sample(float u, float& pmf) const {  // light_sampler.cpp:196-229
+        pmf = 0.0f;
+        if (!owner) return -1;
+        const int n = owner->count;
+        if (n <= 0) return -1;
+        if (uniform) {
+            pmf = 1.0f / static_cast<float>(n);
+            return std::min(static_cast<int>(u * static_cast<float>(n)), n - 1);
+        }
+        float acc = 0.0f;
+        int k = topCount - 1;
+        for (int i = 0; i < topCount - 1; ++i) {
+            if (u < acc + pmfs[i]) {
+                k = i;
+                break;
+            }
+            acc += pmfs[i];
+        }
+        pmf = pmfs[k];
+        return k;
+    }
+    float pmf(int index) const {  // light_sampler.cpp:231-240
+        if (!owner) return 0.0f;
+        if (index < 0 || index >= owner->count) return 0.0f;
+        if (uniform) return 1.0f / static_cast<float>(owner->count);
+        return pmfs[index];
+    }
+};
+
+LightContext prepare(const Integrator& it, V3 p, V3 n) {  // light_sampler.cpp:150-194 (exhaustive)
+    LightContext c;
+    c.owner = &it;
+    if (it.count <= 1) return c;
+    if (it.count > kMaxExhaustive) throw std::runtime_error("oracle: light BVH selection is not restated yet");
+    const int count = it.count;
+    for (int i = 0; i < count; ++i) {
+        const float w = i < it.explicitLights() ? lightImportance(it.lights[static_cast<size_t>(i)], p, n) : it.env.importance(n);
+        c.pmfs[i] = (w > 0.0f && std::isfinite(w)) ? w : 0.0f;
+    }
+    c.topCount = count;
+    float sum = 0.0f;
+    for (int i = 0; i < count; ++i) sum += c.pmfs[i];
+    if (!(sum > 0.0f) || !std::isfinite(sum)) {
+        for (int i = 0; i < count; ++i) c.pmfs[i] = 1.0f / static_cast<float>(count);
+    } else {
+        const float a = (1.0f - kUniformShare) / sum, b = kUniformShare / static_cast<float>(count);
+        for (int i = 0; i < count; ++i) c.pmfs[i] = c.pmfs[i] * a + b;
+    }
+    c.uniform = false;
+    return c;
+}
+
+// ------------------------------------------------------------------------------------ PathTracer::Li
+struct Settings {
+    int maxDepth = 8, rrDepth = 3, shadowSamples = 1;
+    float clampIndirect = 0.0f;
+};
+
+struct RayCounters {
+    uint64_t extension = 0, shadow = 0;
+};
+
+bool russianRoulette(V3& throughput, Sobol& sampler) {  // path_tracer.cpp:131-137
+    const float m = std::max({throughput.x, throughput.y, throughput.z});
+    if (m >= 1.0f) return true;
+    if (!(m > 0.0f) || sampler.get1D() >= m) return false;
+    throughput = throughput / m;
+    return true;
+}
+
+bool makeShadowRay(const SI& isect, const LightSample& ls, Ray& out) {  // path_tracer.cpp:144-157
+    if (!(ls.distance < INFINITY_F) || ls.distance >= 1e29f) {
+        out.o = shadowOrigin(isect, ls.wi);
+        out.d = ls.wi;
+        out.tMin = 0.0f;
+        out.tMax = 1e30f;
+        return true;
+    }
+    return spawnRayTo(isect, isect.point + ls.wi * ls.distance, out);
+}
+
+V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& ray, Sobol& sampler, RayCounters& rc) {
+    V3 L(0.0f), throughput(1.0f);  // path_tracer.cpp:321-890
+    Ray cur = ray;
+    bool specularBounce = true;
+    float lastBsdfPdf = 0.0f;
+    V3 lastPoint = ray.o;
+    int depth = 0;
+    const int explicitLights = it.explicitLights();
+    const int lightCount = explicitLights + (it.hasEnv ? 1 : 0);
+    LightContext picker;
+    const int shadowSamples = st.shadowSamples;
+    bool cameraRay = true;
+    int diffuseBounces = 0;
+    auto accumulate = [&L](V3 c) {
+        if (!isFinite(c)) return;
+        L += c;
+    };
+    auto accumulateIndirect = [&](V3 c) {
+        if (st.clampIndirect > 0.0f) {
+            const float m = std::max({c.x, c.y, c.z});
+            if (m > st.clampIndirect) {
+                accumulate(c * (st.clampIndirect / m));
+                return;
+            }
+        }
+        accumulate(c);
+    };
+    for (;;) {
+        SI isect;
+        Hit h;
+        ++rc.extension;
+        bool hit = intersect(scene, cur, h);
+        if (hit) fillHit(scene, cur, h, isect);
+        for (int skip = 0; hit && skip < 64; ++skip) {  // :423-441 (no cutouts: opacity is 1)
+            if (!(cameraRay && isect.cameraInvisible)) break;
+            Ray next;
+            next.o = offsetRayOrigin(isect.point, isect.pError, isect.ng, cur.d);
+            next.d = cur.d;
+            next.tMin = 0.0f;
+            next.tMax = 1e30f;
+            cur = next;
+            isect = SI{};
+            ++rc.extension;
+            hit = intersect(scene, cur, h);
+            if (hit) fillHit(scene, cur, h, isect);
+        }
+        const V3 wo = -cur.d;
+        if (!hit) {  // :482-514
+            if (it.hasEnv) {
+                const V3 Le = it.env.eval(cur.d);
+                float weight = 1.0f;
+                if (!specularBounce) {
+                    const float lightPdf = picker.pmf(explicitLights) * it.env.pdfLi(cur.d);
+                    weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
+                }
+                const V3 envLocal = Le * weight;
+                if (diffuseBounces >= 2) accumulateIndirect(throughput * envLocal);
+                else accumulate(throughput * envLocal);
+            }
+            break;
+        }
+        const pe_material_desc& mat = scene.materials[static_cast<size_t>(isect.material)];
+        const V3 emitted = materialEmitted(mat);
+        if (!isBlack(emitted)) {  // :565-593
+            const int li = isect.emitter;
+            float weight = 1.0f;
+            if (!specularBounce) {
+                const float lightPdf = li >= 0 ? picker.pmf(li) * lightPdfLiHit(it.lights[static_cast<size_t>(li)], lastPoint, isect.point) : 0.0f;
+                if (lightPdf > 0.0f && lastBsdfPdf > 0.0f) weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
+            }
+            const V3 emitLocal = emitted * weight;
+            if (diffuseBounces >= 2) accumulateIndirect(throughput * emitLocal);
+            else accumulate(throughput * emitLocal);
+        }
+        if (depth >= st.maxDepth) break;  // :596-599
+        const bool isSpecular = materialIsDelta(mat);
+        ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace};
+        if (lightCount > 0 && !isSpecular) {  // NEE, :631-698
+            picker = prepare(it, isect.point, isect.normal);
+            for (int s = 0; s < shadowSamples; ++s) {
+                float lightPmf = 0.0f;
+                const int lightIndex = picker.sample(sampler.get1D(), lightPmf);
+                float ux, uy;
+                if (lightIndex < 0 || !(lightPmf > 0.0f)) {
+                    sampler.get2D(ux, uy);
+                    continue;
+                }
+                const bool fromEnv = lightIndex >= explicitLights;
+                sampler.get2D(ux, uy);
+                const LightSample ls = fromEnv ? it.env.sampleLi(ux, uy)
+                                               : lightSampleLi(it.lights[static_cast<size_t>(lightIndex)], isect.point, ux, uy);
+                if (!ls.valid()) continue;
+                Ray shadow;
+                if (!makeShadowRay(isect, ls, shadow)) continue;
+                ++rc.shadow;
+                const bool occluded = intersectAny(scene, shadow);
+                if (!occluded) {
+                    float bsdfPdf = 0.0f;
+                    const V3 f = evalPdf(mat, wo, ls.wi, sp, bsdfPdf);
+                    const float cosTheta = std::abs(dot(ls.wi, isect.normal));
+                    if (!isBlack(f) && cosTheta > 0.0f) {
+                        const float lightPdf = lightPmf * ls.pdf;
+                        const float weight = powerHeuristic(shadowSamples, lightPdf, 1, bsdfPdf);
+                        const V3 neeLocal = f * ls.Li * V3(1.0f) *
+                                            (cosTheta * weight / (lightPdf * static_cast<float>(shadowSamples)));
+                        const V3 c = throughput * neeLocal;
+                        if (diffuseBounces >= 1) accumulateIndirect(c);
+                        else accumulate(c);
+                    }
+                }
+            }
+        }
+        const float uc = sampler.get1D();  // :736-738
+        float u0, u1;
+        sampler.get2D(u0, u1);
+        BsdfSample bs;
+        if (!sampleFull(mat, wo, sp, uc, u0, u1, bs)) break;
+        if (bs.subsurface) throw std::runtime_error("oracle: subsurface random walk is not restated yet");
+        if (bs.pdf <= 0.0f || isBlack(bs.f)) break;
+        if (bs.passThrough) throw std::runtime_error("oracle: thin-wall pass-through is not restated yet");
+        const V3 wi = bs.wi;
+        const float cosTheta = std::abs(dot(wi, isect.normal));  // :814-816
+        throughput *= bs.f * (cosTheta / bs.pdf);
+        if (!isFinite(throughput)) break;
+        lastBsdfPdf = bs.pdf;
+        lastPoint = isect.point;
+        specularBounce = bs.delta;
+        if (!bs.delta) ++diffuseBounces;
+        cameraRay = cameraRay && bs.delta && dot(wi, isect.normal) < 0.0f;
+        cur = spawnRay(isect, wi);
+        ++depth;
+        if (depth > st.rrDepth) {  // :862-870
+            if (!russianRoulette(throughput, sampler)) break;
+        }
+    }
+    return L;
+}
+
+// ----------------------------------------------------------------------------- pixel_filter.cpp / camera
+struct FilterSampler {  // pixel_filter.cpp:37-87
+    int type = 0;
+    float width = 1.0f, radius = 0.5f;
+    std::vector<float> cdf;
+    FilterSampler(int t, float wIn) : type(t) {
+        width = clampf(wIn, 0.25f, 8.0f);
+        radius = 0.5f * width;
+        if (type == PE_FILTER_BOX) return;
+        constexpr int kBins = 512;
+        cdf.resize(kBins + 1);
+        cdf[0] = 0.0f;
+        const float dx = width / kBins;
+        double acc = 0.0;
+        for (int i = 0; i < kBins; ++i) {
+            acc += std::max(0.0f, evaluate(-radius + (static_cast<float>(i) + 0.5f) * dx));
+            cdf[static_cast<size_t>(i) + 1] = static_cast<float>(acc);
+        }
+        if (!(acc > 0.0)) {
+            cdf.clear();
+            type = PE_FILTER_BOX;
+            return;
+        }
+        for (float& c : cdf) c = static_cast<float>(c / acc);
+        cdf.back() = 1.0f;
+    }
+    float evaluate(float x) const {
+        const float r = radius;
+        if (std::abs(x) > r) return 0.0f;
+        switch (type) {
+            case PE_FILTER_BOX: return 1.0f;
+            case PE_FILTER_TRIANGLE: return std::max(0.0f, 1.0f - std::abs(x) / r);
+            case PE_FILTER_GAUSSIAN: {
+                const float s = width / 6.0f;
+                const float k = 1.0f / (2.0f * s * s);
+                return std::max(0.0f, std::exp(-k * x * x) - std::exp(-k * r * r));
+            }
+            case PE_FILTER_BLACKMAN_HARRIS: {
+                const float a = TWO_PI * (x / width + 0.5f);
+                return 0.35875f - 0.48829f * std::cos(a) + 0.14128f * std::cos(2.0f * a) - 0.01168f * std::cos(3.0f * a);
+            }
+        }
+        return 0.0f;
+    }
+    float sample1D(float u) const {
+        if (cdf.empty()) return (u - 0.5f) * width;
+        constexpr int kBins = 512;
+        const auto it = std::upper_bound(cdf.begin(), cdf.end(), u);
+        const int i = std::clamp(static_cast<int>(it - cdf.begin()) - 1, 0, kBins - 1);
+        const float c0 = cdf[static_cast<size_t>(i)], c1 = cdf[static_cast<size_t>(i) + 1];
+        const float t = c1 > c0 ? (u - c0) / (c1 - c0) : 0.5f;
+        return -radius + (static_cast<float>(i) + clampf(t, 0.0f, 1.0f)) * (width / kBins);
+    }
+};
+
+struct Tracer {
+    const Scene& scene;
+    const Integrator& it;
+    Settings st;
+    pe_camera cam;
+    FilterSampler filter;
+    int width, height;
+    uint64_t seed;
+
+    // tracePixel (renderer.cpp:139-162) without AOVs: filter offset (dim 0), lens sample (dim 1), Li.
+    V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc) const {
+        sampler.startPixel(x, y);
+        sampler.startSample(s);
+        float fu, fv;
+        sampler.get2D(fu, fv);
+        const float ox = filter.sample1D(fu), oy = filter.sample1D(fv);
+        const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(width);  // rasterToScreen (:37-39)
+        const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(height);
+        float lx, ly;
+        sampler.get2D(lx, ly);
+        const V3 pos = c3(cam.position), ll = c3(cam.lower_left), hor = c3(cam.horizontal), ver = c3(cam.vertical);
+        const V3 target = ll + sx * hor + sy * ver;  // perspective_camera.cpp:36-40
+        Ray ray;
+        ray.o = pos;
+        ray.d = normalized(target - pos);
+        return Li(scene, it, st, ray, sampler, rc);
+    }
+};
+
+std::shared_ptr<Integrator> integratorFor(const Scene& s) {
+    auto it = std::make_shared<Integrator>();
+    for (const auto& l : s.lights) it->lights.push_back(makeLight(l));
+    if (s.hasEnv) {
+        it->hasEnv = true;
+        Env& e = it->env;
+        e.rotation = s.env.rotation;
+        e.intensity = s.env.intensity;
+        if (!s.envRgb.empty()) {
+            e.w = s.env.width;
+            e.h = s.env.height;
+            e.rgb = s.envRgb.data();
+        }
+        e.build(true);
+        e.meanLuminance = std::max(0.0f, luminance(e.power()) / (4.0f * PI));
+        if (s.env.has_sun) e.setSun(c3(s.env.sun_direction), s.env.sun_cos_max, c3(s.env.sun_radiance));
+    }
+    it->count = static_cast<int>(it->lights.size()) + (it->hasEnv ? 1 : 0);
+    return it;
+}
+
+const Integrator& integratorOf(const Scene& s) {
+    auto* self = const_cast<Scene*>(&s);
+    if (!self->integrator) self->integrator = integratorFor(s);
+    return *static_cast<const Integrator*>(self->integrator.get());
+}
+
+}  // namespace
+
+}  // namespace oracle
+
+using namespace oracle;
+
+#define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+// Per-sample radiance (tracePixel + Li) for n (px, py, sample) triples.
+ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,
+                                    size_t n, const int32_t* px, const int32_t* py, const int32_t* s, float* L,
+                                    uint64_t* rays) {
+    try {
+        const Scene& scene = *static_cast<Scene*>(sp);
+        const Integrator& it = integratorOf(scene);
+        Settings st;
+        st.maxDepth = ps->max_bounces;
+        st.rrDepth = ps->rr_depth;
+        st.shadowSamples = std::max(1, ps->shadow_quality);
+        st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
+        Sobol sampler(ps->seed);
+        RayCounters rc;
+        for (size_t i = 0; i < n; ++i) {
+            const V3 c = tr.trace(px[i], py[i], s[i], sampler, rc);
+            L[3 * i] = c.x;
+            L[3 * i + 1] = c.y;
+            L[3 * i + 2] = c.z;
+        }
+        if (rays) {
+            rays[0] = rc.extension;
+            rays[1] = rc.shadow;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Renderer::renderSamples (renderer.cpp:392-531) minus tiles/AOVs: every pixel takes samples
+// [first, first + count) in order and Film::addSample (film.cpp:56-71) accumulates them. Rows are shared out
+// over `threads` workers. Film arrays are accumulated into (not cleared).
+ORACLE_API int oracle_render(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height, int first,
+                             int count, int threads, int row_begin, int row_end, float* sum, uint32_t* cnt,
+                             float* lum2, uint64_t* rays, double* seconds) {
+    try {
+        const Scene& scene = *static_cast<Scene*>(sp);
+        const Integrator& it = integratorOf(scene);
+        Settings st;
+        st.maxDepth = ps->max_bounces;
+        st.rrDepth = ps->rr_depth;
+        st.shadowSamples = std::max(1, ps->shadow_quality);
+        st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        const Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
+        if (row_end <= row_begin) {
+            row_begin = 0;
+            row_end = height;
+        }
+        std::atomic<int> nextRow{row_begin};
+        std::atomic<uint64_t> ext{0}, sh{0};
+        const auto t0 = std::chrono::steady_clock::now();
+        auto worker = [&]() {
+            Sobol sampler(ps->seed);
+            RayCounters rc;
+            for (int y; (y = nextRow.fetch_add(1)) < row_end;) {
+                for (int x = 0; x < width; ++x) {
+                    const size_t k = static_cast<size_t>(y) * width + x;
+                    for (int s = first; s < first + count; ++s) {
+                        const V3 c = tr.trace(x, y, s, sampler, rc);
+                        ++cnt[k];
+                        const bool ok = isFinite(c) && c.x >= 0.0f && c.y >= 0.0f && c.z >= 0.0f;
+                        if (!ok) continue;
+                        sum[3 * k] += c.x;
+                        sum[3 * k + 1] += c.y;
+                        sum[3 * k + 2] += c.z;
+                        const float l = luminance(c);
+                        lum2[k] += l * l;
+                    }
+                }
+            }
+            ext += rc.extension;
+            sh += rc.shadow;
+        };
+        std::vector<std::thread> pool;
+        for (int t = 1; t < std::max(1, threads); ++t) pool.emplace_back(worker);
+        worker();
+        for (auto& t : pool) t.join();
+        if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rays) {
+            rays[0] = ext.load();
+            rays[1] = sh.load();
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+ORACLE_API int oracle_bsdf_eval(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
+                                const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+    try {
+        for (size_t i = 0; i < n; ++i) {
+            ShadingPoint sp{V3(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]),
+                            V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0};
+            float p = 0.0f;
+            const V3 v = evalPdf(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]),
+                                 sp, p);
+            f[3 * i] = v.x;
+            f[3 * i + 1] = v.y;
+            f[3 * i + 2] = v.z;
+            pdf[i] = p;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+ORACLE_API int oracle_bsdf_sample(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
+                                  const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+    try {
+        for (size_t i = 0; i < n; ++i) {
+            ShadingPoint sp{V3(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]),
+                            V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0};
+            BsdfSample bs;
+            const bool ok = sampleFull(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), sp, u3[3 * i], u3[3 * i + 1],
+                                       u3[3 * i + 2], bs);
+            float* o = out + 14 * i;
+            o[0] = ok ? 1.0f : 0.0f;
+            o[1] = bs.wi.x; o[2] = bs.wi.y; o[3] = bs.wi.z;
+            o[4] = bs.f.x; o[5] = bs.f.y; o[6] = bs.f.z;
+            o[7] = bs.pdf;
+            o[8] = bs.delta ? 1.0f : 0.0f;
+            o[9] = bs.subsurface ? 1.0f : 0.0f;
+            o[10] = bs.passThrough ? 1.0f : 0.0f;
+            o[11] = bs.weight.x; o[12] = bs.weight.y; o[13] = bs.weight.z;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// which: 0 ggxDirectionalAlbedo, 1 ggxSchlickAlbedo, 2 ggxSchlickAverageAlbedo, 3 charlieSheenAlbedo
+ORACLE_API int oracle_microfacet(int which, size_t n, const float* a, const float* b, const float* c, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        switch (which) {
+            case 0: out[i] = ggxDirectionalAlbedo(a[i], b[i]); break;
+            case 1: out[i] = ggxSchlickAlbedo(a[i], b[i], c[i]); break;
+            case 2: out[i] = ggxSchlickAverageAlbedo(a[i], b[i]); break;
+            case 3: out[i] = charlieSheenAlbedo(a[i], b[i]); break;
+            default: return -1;
+        }
+    }
+    return 0;
+}

@@ -1,0 +1,686 @@
+// oracle_shading.cpp — restated Disney/OpenPBR principled BSDF, Lambert, GGX helpers and energy tables
+// (test infrastructure only). Each function follows the reference file:line cited beside it.
+#include "oracle_shading.h"
+
+#include <algorithm>
+#include <array>
+#include <cstring>
+#include <stdexcept>
+
+namespace oracle {
+
+// frame.h:27-38 (Duff et al. 2017)
+Frame::Frame(V3 normal) : n(normal) {
+    const float sign = std::copysign(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    s = V3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    t = V3(b, sign + n.y * n.y * a, -n.y);
+}
+
+// sampling.h:24-29
+V3 uniformSampleSphere(float u0, float u1) {
+    const float z = 1.0f - 2.0f * u0;
+    const float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
+    const float phi = TWO_PI * u1;
+    return V3(r * std::cos(phi), r * std::sin(phi), z);
+}
+
+// sampling.h:53-72 (Shirley–Chiu concentric map)
+void uniformSampleDisk(float u0, float u1, float& x, float& y) {
+    const float sx = 2.0f * u0 - 1.0f, sy = 2.0f * u1 - 1.0f;
+    if (sx == 0.0f && sy == 0.0f) {
+        x = y = 0.0f;
+        return;
+    }
+    float r, theta;
+    if (std::abs(sx) > std::abs(sy)) {
+        r = sx;
+        theta = PI_OVER_4 * (sy / sx);
+    } else {
+        r = sy;
+        theta = PI_OVER_2 - PI_OVER_4 * (sx / sy);
+    }
+    x = r * std::cos(theta);
+    y = r * std::sin(theta);
+}
+
+// sampling.h:79-83
+V3 cosineSampleHemisphere(float u0, float u1) {
+    float x, y;
+    uniformSampleDisk(u0, u1, x, y);
+    return V3(x, y, std::sqrt(std::max(0.0f, 1.0f - x * x - y * y)));
+}
+
+// sampling.h:95-98
+void uniformSampleTriangle(float u0, float u1, float& b1, float& b2) {
+    const float su0 = std::sqrt(u0);
+    b1 = 1.0f - su0;
+    b2 = u1 * su0;
+}
+
+// utils.h:163-175
+bool refractVec(V3 wi, V3 n, float eta, V3& wt) {
+    const float cosThetaI = dot(n, wi);
+    const float sin2ThetaI = std::max(0.0f, 1.0f - cosThetaI * cosThetaI);
+    const float sin2ThetaT = eta * eta * sin2ThetaI;
+    if (sin2ThetaT >= 1.0f) return false;
+    const float cosThetaT = std::sqrt(1.0f - sin2ThetaT);
+    wt = eta * (-wi) + (eta * cosThetaI - cosThetaT) * n;
+    return true;
+}
+
+// utils.h:206-232
+float fresnelDielectric(float cosThetaI, float etaI, float etaT) {
+    cosThetaI = clampf(cosThetaI, -1.0f, 1.0f);
+    if (!(cosThetaI > 0.0f)) {
+        std::swap(etaI, etaT);
+        cosThetaI = std::abs(cosThetaI);
+    }
+    const float sinThetaI = std::sqrt(std::max(0.0f, 1.0f - cosThetaI * cosThetaI));
+    const float sinThetaT = (etaI / etaT) * sinThetaI;
+    if (sinThetaT >= 1.0f) return 1.0f;
+    const float cosThetaT = std::sqrt(std::max(0.0f, 1.0f - sinThetaT * sinThetaT));
+    const float rs = (etaT * cosThetaI - etaI * cosThetaT) / (etaT * cosThetaI + etaI * cosThetaT);
+    const float rp = (etaI * cosThetaI - etaT * cosThetaT) / (etaI * cosThetaI + etaT * cosThetaT);
+    return 0.5f * (rs * rs + rp * rp);
+}
+
+// ------------------------------------------------------------------------------------ microfacet.h
+namespace {
+constexpr float GGX_MIN_ALPHA = 1e-4f;  // microfacet.h:15
+float ggxClampAlpha(float a) { return std::max(a, GGX_MIN_ALPHA); }
+float pow5(float x) {  // microfacet.h:129-132
+    const float x2 = x * x;
+    return x2 * x2 * x;
+}
+float ggxLambda(V3 w, float ax, float ay) {  // microfacet.h:43-50
+    const float z2 = w.z * w.z;
+    if (z2 <= 1e-12f) return 1e12f;
+    ax = ggxClampAlpha(ax);
+    ay = ggxClampAlpha(ay);
+    const float t2 = (ax * ax * w.x * w.x + ay * ay * w.y * w.y) / z2;
+    return 0.5f * (-1.0f + std::sqrt(1.0f + t2));
+}
+float charlieD(float cosH, float roughness) {  // microfacet.h:111-116
+    const float a = std::max(roughness * roughness, 1e-4f);
+    const float inv = 1.0f / a;
+    const float sin2 = std::max(0.0f, 1.0f - cosH * cosH);
+    return (2.0f + inv) * std::pow(sin2, 0.5f * inv) * (0.5f * INV_PI);
+}
+float charlieV(float cosI, float cosO) { return 1.0f / std::max(4.0f * (cosI + cosO - cosI * cosO), 1e-6f); }
+}  // namespace
+
+float ggxD(V3 h, float ax, float ay) {  // microfacet.h:27-33
+    if (h.z <= 0.0f) return 0.0f;
+    ax = ggxClampAlpha(ax);
+    ay = ggxClampAlpha(ay);
+    const float e = (h.x * h.x) / (ax * ax) + (h.y * h.y) / (ay * ay) + h.z * h.z;
+    return 1.0f / (PI * ax * ay * e * e);
+}
+float ggxG1(V3 w, float ax, float ay) { return 1.0f / (1.0f + ggxLambda(w, ax, ay)); }
+float ggxG(V3 wo, V3 wi, float ax, float ay) { return 1.0f / (1.0f + ggxLambda(wo, ax, ay) + ggxLambda(wi, ax, ay)); }
+float ggxVisiblePdf(V3 wo, V3 h, float ax, float ay) {  // microfacet.h:68-72
+    const float cosO = std::abs(wo.z);
+    if (cosO <= 0.0f) return 0.0f;
+    return ggxG1(wo, ax, ay) * std::abs(dot(wo, h)) * ggxD(h, ax, ay) / cosO;
+}
+V3 ggxSampleVisibleNormal(V3 wo, float ax, float ay, float u1, float u2) {  // microfacet.h:81-100
+    ax = ggxClampAlpha(ax);
+    ay = ggxClampAlpha(ay);
+    if (wo.z < 0.0f) wo = -wo;
+    const V3 vh = normalized(V3(ax * wo.x, ay * wo.y, wo.z));
+    const float lenSq = vh.x * vh.x + vh.y * vh.y;
+    const V3 t1 = lenSq > 0.0f ? V3(-vh.y, vh.x, 0.0f) / std::sqrt(lenSq) : V3(1.0f, 0.0f, 0.0f);
+    const V3 t2 = cross(vh, t1);
+    const float r = std::sqrt(u1);
+    const float phi = TWO_PI * u2;
+    const float p1 = r * std::cos(phi);
+    float p2 = r * std::sin(phi);
+    const float s = 0.5f * (1.0f + vh.z);
+    p2 = (1.0f - s) * std::sqrt(std::max(0.0f, 1.0f - p1 * p1)) + s * p2;
+    const V3 nh = t1 * p1 + t2 * p2 + vh * std::sqrt(std::max(0.0f, 1.0f - p1 * p1 - p2 * p2));
+    return normalized(V3(ax * nh.x, ay * nh.y, std::max(1e-6f, nh.z)));
+}
+
+// --------------------------------------------------------------------------------- microfacet.cpp
+namespace {
+constexpr int kN = 32;
+using Table2D = std::array<float, kN * kN>;
+using Table1D = std::array<float, kN>;
+
+float lookup2D(const Table2D& t, float mu, float alpha) {  // microfacet.cpp:20-32
+    const float x = clampf(mu, 0.0f, 1.0f) * (kN - 1);
+    const float y = clampf(alpha, 0.0f, 1.0f) * (kN - 1);
+    const int x0 = std::clamp(static_cast<int>(std::floor(x)), 0, kN - 1);
+    const int y0 = std::clamp(static_cast<int>(std::floor(y)), 0, kN - 1);
+    const int x1 = std::min(x0 + 1, kN - 1), y1 = std::min(y0 + 1, kN - 1);
+    const float fx = clampf(x - static_cast<float>(x0), 0.0f, 1.0f);
+    const float fy = clampf(y - static_cast<float>(y0), 0.0f, 1.0f);
+    auto at = [&](int ix, int iy) { return t[static_cast<size_t>(iy * kN + ix)]; };
+    const float a = at(x0, y0) * (1.0f - fx) + at(x1, y0) * fx;
+    const float b = at(x0, y1) * (1.0f - fx) + at(x1, y1) * fx;
+    return a * (1.0f - fy) + b * fy;
+}
+float lookup1D(const Table1D& t, float alpha) {  // microfacet.cpp:34-40
+    const float y = clampf(alpha, 0.0f, 1.0f) * (kN - 1);
+    const int y0 = std::clamp(static_cast<int>(std::floor(y)), 0, kN - 1);
+    const int y1 = std::min(y0 + 1, kN - 1);
+    const float fy = clampf(y - static_cast<float>(y0), 0.0f, 1.0f);
+    return t[static_cast<size_t>(y0)] * (1.0f - fy) + t[static_cast<size_t>(y1)] * fy;
+}
+float nodeMu(int im) { return std::max(static_cast<float>(im) / (kN - 1), 1e-3f); }
+
+struct GgxTables {  // microfacet.cpp:46-100
+    Table2D e{}, ea{}, eb{};
+    Table1D avg0{}, avg1{}, avg2{};
+    GgxTables() {
+        constexpr int kS = 24;
+        for (int ia = 0; ia < kN; ++ia) {
+            const float alpha = static_cast<float>(ia) / (kN - 1);
+            for (int im = 0; im < kN; ++im) {
+                const float mu = nodeMu(im);
+                const V3 wo(std::sqrt(std::max(0.0f, 1.0f - mu * mu)), 0.0f, mu);
+                const float g1 = ggxG1(wo, alpha, alpha);
+                double sum = 0.0, sumA = 0.0, sumB = 0.0;
+                for (int sy = 0; sy < kS; ++sy)
+                    for (int sx = 0; sx < kS; ++sx) {
+                        const float u1 = (static_cast<float>(sx) + 0.5f) / kS;
+                        const float u2 = (static_cast<float>(sy) + 0.5f) / kS;
+                        const V3 h = ggxSampleVisibleNormal(wo, alpha, alpha, u1, u2);
+                        const float c = dot(wo, h);
+                        const V3 wi = h * (2.0f * c) - wo;
+                        if (wi.z <= 0.0f) continue;
+                        const double w = ggxG(wo, wi, alpha, alpha) / g1;
+                        const double s5 = pow5(1.0f - clampf(c, 0.0f, 1.0f));
+                        sum += w;
+                        sumA += w * (1.0 - s5);
+                        sumB += w * s5;
+                    }
+                const size_t k = static_cast<size_t>(ia * kN + im);
+                e[k] = static_cast<float>(sum / (kS * kS));
+                ea[k] = static_cast<float>(sumA / (kS * kS));
+                eb[k] = static_cast<float>(sumB / (kS * kS));
+            }
+        }
+        constexpr int kM = 256;
+        for (int ia = 0; ia < kN; ++ia) {
+            const float alpha = static_cast<float>(ia) / (kN - 1);
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+            for (int i = 0; i < kM; ++i) {
+                const float mu = (static_cast<float>(i) + 0.5f) / kM;
+                const float E = clampf(lookup2D(e, mu, alpha), 1e-3f, 1.0f);
+                const float m = 1.0f / E - 1.0f;
+                const float A = lookup2D(ea, mu, alpha), B = lookup2D(eb, mu, alpha);
+                const double w = 2.0 * mu / kM;
+                a0 += w * B;
+                a1 += w * (A + B * m);
+                a2 += w * (A * m);
+            }
+            avg0[static_cast<size_t>(ia)] = static_cast<float>(a0);
+            avg1[static_cast<size_t>(ia)] = static_cast<float>(a1);
+            avg2[static_cast<size_t>(ia)] = static_cast<float>(a2);
+        }
+    }
+};
+const GgxTables& ggxTables() {
+    static const GgxTables t;
+    return t;
+}
+
+struct SheenTable {  // microfacet.cpp:108-133
+    Table2D e{};
+    SheenTable() {
+        constexpr int kC = 96, kP = 48;
+        for (int ir = 0; ir < kN; ++ir) {
+            const float r = std::max(static_cast<float>(ir) / (kN - 1), 0.02f);
+            for (int im = 0; im < kN; ++im) {
+                const float mu = nodeMu(im);
+                const V3 wo(std::sqrt(std::max(0.0f, 1.0f - mu * mu)), 0.0f, mu);
+                double sum = 0.0;
+                for (int i = 0; i < kC; ++i) {
+                    const float cz = (static_cast<float>(i) + 0.5f) / kC;
+                    const float s = std::sqrt(std::max(0.0f, 1.0f - cz * cz));
+                    for (int j = 0; j < kP; ++j) {
+                        const float phi = PI * (static_cast<float>(j) + 0.5f) / kP;
+                        const V3 wi(s * std::cos(phi), s * std::sin(phi), cz);
+                        const V3 h = normalized(wo + wi);
+                        sum += charlieD(h.z, r) * charlieV(cz, mu) * cz;
+                    }
+                }
+                e[static_cast<size_t>(ir * kN + im)] = static_cast<float>(sum * (1.0 / kC) * (TWO_PI / kP));
+            }
+        }
+    }
+};
+const SheenTable& sheenTable() {
+    static const SheenTable t;
+    return t;
+}
+}  // namespace
+
+float ggxDirectionalAlbedo(float cosTheta, float alpha) {  // microfacet.cpp:142-144
+    return clampf(lookup2D(ggxTables().e, cosTheta, alpha), 1e-3f, 1.0f);
+}
+float ggxSchlickAlbedo(float cosTheta, float alpha, float f0) {  // microfacet.cpp:146-151
+    const GgxTables& t = ggxTables();
+    const float E = clampf(lookup2D(t.e, cosTheta, alpha), 1e-3f, 1.0f);
+    const float ss = f0 * lookup2D(t.ea, cosTheta, alpha) + lookup2D(t.eb, cosTheta, alpha);
+    return clampf(ss * (1.0f + f0 * (1.0f / E - 1.0f)), 0.0f, 1.0f);
+}
+float ggxSchlickAverageAlbedo(float alpha, float f0) {  // microfacet.cpp:153-157
+    const GgxTables& t = ggxTables();
+    const float v = lookup1D(t.avg0, alpha) + f0 * lookup1D(t.avg1, alpha) + f0 * f0 * lookup1D(t.avg2, alpha);
+    return clampf(v, 0.0f, 1.0f);
+}
+float charlieSheenAlbedo(float cosTheta, float roughness) {  // microfacet.cpp:159-161
+    return std::max(lookup2D(sheenTable().e, cosTheta, roughness), 0.0f);
+}
+
+// ------------------------------------------------------------------------------ disney.cpp / disney.h
+namespace {
+
+// DisneyMaterial setters' clamps (disney.h:22-24, :62-126) applied to the raw descriptor.
+struct Params {
+    V3 baseColor;
+    float metallic, roughness, specular, clearCoat, clearCoatRoughness, ior, specularTransmission, coatIor;
+    V3 coatTint, sheenColor;
+    bool hasCustomSheenColor;
+    float anisotropy, sheen, sheenRoughness, diffuseRoughness, diffuseTransmission, subsurface;
+    V3 edgeTint;
+    V3 normal, tangent;
+    bool frontFace;
+};
+
+V3 c3(const float* v) { return V3(v[0], v[1], v[2]); }
+V3 clamp01(V3 c) { return V3(clampf(c.x, 0.0f, 1.0f), clampf(c.y, 0.0f, 1.0f), clampf(c.z, 0.0f, 1.0f)); }
+
+Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.cpp:543-603 (no textures, no flakes)
+    if (d.thin_film_thickness > 0.0f) throw std::runtime_error("oracle: thin-film Disney layers are not restated");
+    Params p;
+    p.baseColor = c3(d.base_color);
+    p.metallic = d.metallic;
+    p.roughness = std::max(0.001f, d.roughness);
+    p.specular = d.specular;
+    p.clearCoat = clampf(d.clearcoat, 0.0f, 1.0f);
+    p.clearCoatRoughness = std::max(0.001f, d.clearcoat_roughness);
+    p.ior = clampf(d.ior, 1.0001f, 3.0f);
+    p.specularTransmission = clampf(d.specular_transmission, 0.0f, 1.0f);
+    p.coatIor = clampf(d.coat_ior, 1.0001f, 3.0f);
+    p.coatTint = clamp01(c3(d.coat_tint));
+    p.hasCustomSheenColor = d.has_sheen_color != 0;
+    p.sheenColor = p.hasCustomSheenColor ? clamp01(c3(d.sheen_color)) : V3(1.0f);
+    p.anisotropy = clampf(d.anisotropy, 0.0f, 1.0f);
+    p.sheen = clampf(d.sheen, 0.0f, 1.0f);
+    p.sheenRoughness = clampf(d.sheen_roughness, 0.02f, 1.0f);
+    p.diffuseRoughness = clampf(d.diffuse_roughness, 0.0f, 1.0f);
+    p.diffuseTransmission = clampf(d.diffuse_transmission, 0.0f, 1.0f);
+    p.subsurface = clampf(d.subsurface, 0.0f, 1.0f);
+    p.edgeTint = clamp01(c3(d.edge_tint));
+    p.normal = sp.normal;
+    p.tangent = sp.tangent;
+    p.frontFace = sp.frontFace;
+    return p;
+}
+
+float f0FromIor(float ior) {  // disney.cpp:25-28
+    const float r = (ior - 1.0f) / (ior + 1.0f);
+    return r * r;
+}
+float dielectricWeight(const Params& p) { return clampf(2.0f * p.specular, 0.0f, 1.0f); }  // :34-36
+float dielectricF0(const Params& p) {  // :40-42
+    return clampf(f0FromIor(p.ior) * std::max(1.0f, 2.0f * p.specular), 0.0f, 1.0f);
+}
+float coatF0(const Params& p) { return clampf(f0FromIor(p.coatIor), 0.0f, 1.0f); }  // :44-46
+float schlick(float f0, float c) { return f0 + (1.0f - f0) * pow5(1.0f - clampf(c, 0.0f, 1.0f)); }  // :48-50
+float schlickF(float f0, float cosTheta) {  // fresnel.h:20-24
+    const float m = 1.0f - clampf(cosTheta, 0.0f, 1.0f);
+    const float m2 = m * m;
+    return f0 + (1.0f - f0) * m2 * m2 * m;
+}
+V3 fresnelF82Tint(V3 f0, V3 tint, float cosTheta) {  // fresnel.h:31-47
+    constexpr float muBar = 1.0f / 7.0f;
+    constexpr float oneMinus = 1.0f - muBar;
+    const float denom = muBar * oneMinus * oneMinus * oneMinus * oneMinus * oneMinus * oneMinus;
+    const float mu = clampf(cosTheta, 0.0f, 1.0f);
+    const float m = 1.0f - mu;
+    const float m6 = m * m * m * m * m * m;
+    V3 out;
+    for (int c = 0; c < 3; ++c) {
+        const float fsBar = schlickF(f0[c], muBar);
+        const float b = fsBar * (1.0f - tint[c]) / denom;
+        out[c] = clampf(schlickF(f0[c], mu) - b * mu * m6, 0.0f, 1.0f);
+    }
+    return out;
+}
+void anisoAlpha(float roughness, float anisotropy, float& ax, float& ay) {  // :55-60
+    const float aspect = std::sqrt(std::max(0.1f, 1.0f - 0.9f * clampf(anisotropy, 0.0f, 1.0f)));
+    const float a = roughness * roughness;
+    ax = a / aspect;
+    ay = a * aspect;
+}
+Frame shadingFrame(V3 nIn, V3 tangentIn) {  // :62-73
+    const V3 n = lengthSq(nIn) > 0.0f ? normalized(nIn) : V3(0, 1, 0);
+    V3 t = tangentIn - n * dot(tangentIn, n);
+    if (lengthSq(t) < 1e-8f) return Frame(n);
+    t = normalized(t);
+    V3 b = cross(n, t);
+    if (lengthSq(b) < 1e-8f) return Frame(n);
+    b = normalized(b);
+    t = normalized(cross(b, n));
+    return Frame(t, b, n);
+}
+V3 specularFresnel(const Params& p, float cosD) {  // :79-96 (thin film excluded above)
+    const float f0d = dielectricF0(p);
+    const float w = dielectricWeight(p);
+    const V3 fd(w * schlickF(f0d, cosD));
+    if (p.metallic <= 0.0f) return fd;
+    return lerpc(fd, fresnelF82Tint(p.baseColor, p.edgeTint, cosD), p.metallic);
+}
+V3 multiScatterFactor(const Params& p, float cosO, float ax, float ay) {  // :99-103
+    const float E = ggxDirectionalAlbedo(cosO, std::sqrt(ax * ay));
+    const V3 f0 = lerpc(V3(dielectricF0(p)), p.baseColor, p.metallic);
+    return V3(1.0f) + f0 * (1.0f / E - 1.0f);
+}
+struct DielectricLayer {  // :128-146
+    float alpha = 0.0f, f0 = 0.0f, weight = 1.0f, invOneMinusAvg = 1.0f;
+    explicit DielectricLayer(const Params& p) {
+        float ax, ay;
+        anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+        alpha = std::sqrt(ax * ay);
+        f0 = dielectricF0(p);
+        weight = dielectricWeight(p);
+        invOneMinusAvg = 1.0f / std::max(1e-4f, 1.0f - weight * ggxSchlickAverageAlbedo(alpha, f0));
+    }
+    float transmit(float mu) const { return std::max(0.0f, 1.0f - weight * ggxSchlickAlbedo(mu, alpha, f0)); }
+    float factor(float muO, float muI) const { return transmit(muO) * transmit(muI) * invOneMinusAvg; }
+};
+float coatTransmit(const Params& p, float mu) {  // :149-153
+    if (p.clearCoat <= 0.0f) return 1.0f;
+    const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
+    return 1.0f - p.clearCoat * ggxSchlickAlbedo(mu, alphaC, coatF0(p));
+}
+V3 coatTintFactor(const Params& p) {  // :155-157
+    return p.clearCoat > 0.0f ? lerpc(V3(1.0f), p.coatTint, p.clearCoat) : V3(1.0f);
+}
+V3 resolvedSheenColor(const Params& p) {  // :160-166
+    if (p.sheen <= 0.0f || p.metallic >= 1.0f) return V3(0.0f);
+    if (p.hasCustomSheenColor) return p.sheenColor * (p.sheen * (1.0f - p.metallic));
+    const float lum = luminance(p.baseColor);
+    const V3 tint = lum > 1e-6f ? p.baseColor * (1.0f / lum) : V3(1.0f);
+    return (V3(0.5f) + clamp01(tint) * 0.5f) * (p.sheen * (1.0f - p.metallic));
+}
+float sheenScale(const Params& p, float s, float muO, float muI) {  // :169-174
+    if (s <= 0.0f) return 1.0f;
+    const float eo = std::min(1.0f, charlieSheenAlbedo(muO, p.sheenRoughness));
+    const float ei = std::min(1.0f, charlieSheenAlbedo(muI, p.sheenRoughness));
+    return std::max(0.0f, std::min(1.0f - s * eo, 1.0f - s * ei));
+}
+V3 evalSheen(V3 color, const Params& p, float cosO, float cosI, float cosH) {  // :177-181
+    const float norm = std::max({1.0f, charlieSheenAlbedo(cosO, p.sheenRoughness), charlieSheenAlbedo(cosI, p.sheenRoughness)});
+    return color * (charlieD(cosH, p.sheenRoughness) * charlieV(cosI, cosO) / norm);
+}
+V3 evalSpecTransmission(V3 wo, V3 wi, const Params& p) {  // :185-205
+    const float stW = clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
+    if (stW <= 0.0f || wo.z <= 0.0f || wi.z >= 0.0f) return V3(0.0f);
+    const float etap = p.frontFace ? p.ior : (1.0f / p.ior);
+    V3 wm = wi * etap + wo;
+    if (lengthSq(wm) <= 1e-12f) return V3(0.0f);
+    wm = normalized(wm);
+    if (wm.z < 0.0f) wm = -wm;
+    const float woDotH = dot(wo, wm), wiDotH = dot(wi, wm);
+    if (woDotH <= 0.0f || wiDotH >= 0.0f) return V3(0.0f);
+    const float alpha = ggxClampAlpha(p.roughness * p.roughness);
+    const float F = fresnelDielectric(p.frontFace ? woDotH : -woDotH, 1.0f, p.ior);
+    const float D = ggxD(wm, alpha, alpha);
+    const float G = ggxG(wo, V3(wi.x, wi.y, -wi.z), alpha, alpha);
+    const float s = wiDotH + woDotH / etap;
+    const float denom = s * s * std::abs(wi.z * wo.z);
+    if (denom <= 1e-12f) return V3(0.0f);
+    const float ft = D * (1.0f - F) * G * std::abs(wiDotH * woDotH) / (denom * etap * etap);
+    return p.baseColor * (stW * ft);
+}
+float pdfSpecTransmission(V3 wo, V3 wi, const Params& p) {  // :207-220
+    if (wo.z <= 0.0f || wi.z >= 0.0f) return 0.0f;
+    const float etap = p.frontFace ? p.ior : (1.0f / p.ior);
+    V3 wm = wi * etap + wo;
+    if (lengthSq(wm) <= 1e-12f) return 0.0f;
+    wm = normalized(wm);
+    if (wm.z < 0.0f) wm = -wm;
+    const float woDotH = dot(wo, wm), wiDotH = dot(wi, wm);
+    if (woDotH <= 0.0f || wiDotH >= 0.0f) return 0.0f;
+    const float alpha = ggxClampAlpha(p.roughness * p.roughness);
+    const float visible = ggxVisiblePdf(wo, wm, alpha, alpha);
+    const float s = wiDotH + woDotH / etap;
+    if (s * s <= 1e-12f) return 0.0f;
+    return visible * std::abs(wiDotH) / (s * s);
+}
+float fonAlbedo(float mu, float r) {  // eon.h:27-34
+    constexpr float kFonC1 = 0.5f - 2.0f / (3.0f * PI);
+    mu = clampf(mu, 1e-4f, 1.0f);
+    const float AF = 1.0f / (1.0f + kFonC1 * r);
+    const float BF = r * AF;
+    const float Si = std::sqrt(std::max(0.0f, 1.0f - mu * mu));
+    const float G = Si * (std::acos(mu) - Si * mu) + (2.0f / 3.0f) * ((Si / mu) * (1.0f - Si * Si * Si) - Si);
+    return AF + (BF * INV_PI) * G;
+}
+V3 eonEval(V3 rho, float r, V3 wi, V3 wo) {  // eon.h:37-51
+    constexpr float kFonC1 = 0.5f - 2.0f / (3.0f * PI);
+    constexpr float kFonC2 = 2.0f / 3.0f - 28.0f / (15.0f * PI);
+    const float muI = wi.z, muO = wo.z;
+    if (r <= 0.0f) return rho * INV_PI;
+    const float s = dot(wi, wo) - muI * muO;
+    const float sOverT = s > 0.0f ? s / std::max(muI, muO) : s;
+    const float AF = 1.0f / (1.0f + kFonC1 * r);
+    const V3 fss = rho * (INV_PI * AF * (1.0f + r * sOverT));
+    const float EFo = fonAlbedo(muO, r), EFi = fonAlbedo(muI, r);
+    const float avgEF = AF * (1.0f + kFonC2 * r);
+    V3 rhoMs;
+    for (int c = 0; c < 3; ++c) rhoMs[c] = rho[c] * rho[c] * avgEF / std::max(1e-6f, 1.0f - rho[c] * (1.0f - avgEF));
+    constexpr float eps = 1e-7f;
+    const float ms = std::max(eps, 1.0f - EFo) * std::max(eps, 1.0f - EFi) / std::max(eps, 1.0f - avgEF);
+    return fss + rhoMs * (INV_PI * ms);
+}
+
+V3 evalLobes(V3 wo, V3 wi, const Params& p) {  // :223-311 (no flakes)
+    const Frame frame = shadingFrame(p.normal, p.tangent);
+    const V3 woL = frame.toLocal(wo), wiL = frame.toLocal(wi);
+    if (woL.z <= 0.0f) return V3(0.0f);
+    const float cosThetaO = woL.z;
+    const float coatO = coatTransmit(p, cosThetaO);
+    const V3 cTint = coatTintFactor(p);
+    const float stW = clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
+    const float opaqueDielectric = (1.0f - p.metallic) * (1.0f - clampf(p.specularTransmission, 0.0f, 1.0f));
+    const float trans = clampf(p.diffuseTransmission, 0.0f, 1.0f) * opaqueDielectric;
+    if (wiL.z <= 0.0f) {
+        V3 ft(0.0f);
+        if (trans > 0.0f) {
+            const float muI = -wiL.z;
+            const DielectricLayer dl(p);
+            const V3 sc = resolvedSheenColor(p);
+            const float sh = sheenScale(p, std::max({sc.x, sc.y, sc.z}), cosThetaO, muI);
+            ft += p.baseColor * (trans * INV_PI * coatO * dl.factor(cosThetaO, muI) * sh) * cTint;
+        }
+        if (stW > 0.0f) ft += evalSpecTransmission(woL, wiL, p) * coatO * cTint;
+        return ft;
+    }
+    const V3 hL = normalized(woL + wiL);
+    const float cosThetaI = wiL.z;
+    const float cosThetaD = dot(woL, hL);
+    V3 base;
+    {
+        const DielectricLayer dl(p);
+        const float diffW = opaqueDielectric * (1.0f - clampf(p.diffuseTransmission, 0.0f, 1.0f)) * (1.0f - p.subsurface);
+        const V3 fDiffuse = diffW > 0.0f ? eonEval(p.baseColor, p.diffuseRoughness, wiL, woL) *
+                                               (diffW * dl.factor(cosThetaO, cosThetaI))
+                                         : V3(0.0f);
+        float ax, ay;
+        anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+        const float D = ggxD(hL, ax, ay);
+        const float G = ggxG(woL, wiL, ax, ay);
+        V3 F = specularFresnel(p, cosThetaD);
+        if (stW > 0.0f && !p.frontFace) {
+            const float fInt = fresnelDielectric(-cosThetaD, 1.0f, p.ior);
+            F = lerpc(F, V3(fInt), stW);
+        }
+        const V3 fSpecular = (F * D * G) / (4.0f * cosThetaO * cosThetaI) * multiScatterFactor(p, cosThetaO, ax, ay);
+        base = fDiffuse + fSpecular;
+        const V3 sc = resolvedSheenColor(p);
+        if (!isBlack(sc)) {
+            const float s = sheenScale(p, std::max({sc.x, sc.y, sc.z}), cosThetaO, cosThetaI);
+            base = base * s + evalSheen(sc, p, cosThetaO, cosThetaI, hL.z);
+        }
+    }
+    if (p.clearCoat <= 0.0f) return base;
+    const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
+    const float Dc = ggxD(hL, alphaC, alphaC);
+    const float Gc = ggxG(woL, wiL, alphaC, alphaC);
+    const float f0c = coatF0(p);
+    const float Fc = schlick(f0c, cosThetaD);
+    const float msC = 1.0f + f0c * (1.0f / ggxDirectionalAlbedo(cosThetaO, alphaC) - 1.0f);
+    const V3 fCoat(p.clearCoat * Fc * Dc * Gc / (4.0f * cosThetaO * cosThetaI) * msC);
+    const V3 coatAtt = cTint * (coatO * coatTransmit(p, cosThetaI));
+    return base * coatAtt + fCoat;
+}
+
+void lobeWeights(const Params& p, float& pCoat, float& pSpec, float& pDiffRefl, float& pTrans, float& pSss,
+                 float& pSpecTrans) {  // :313-338 (no flakes)
+    pCoat = 0.25f * p.clearCoat;
+    const float rem = 1.0f - pCoat;
+    const float st = clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
+    pSpec = rem * lerpf(0.5f * (st > 0.0f ? 1.0f : dielectricWeight(p)), 1.0f, p.metallic);
+    float pRest = rem - pSpec;
+    const float pSt = pRest * st;
+    pRest -= pSt;
+    const float dTrans = clampf(p.diffuseTransmission, 0.0f, 1.0f);
+    pTrans = pRest * dTrans;
+    pDiffRefl = pRest - pTrans;
+    pSss = p.metallic < 1.0f ? pDiffRefl * p.subsurface : 0.0f;
+    pDiffRefl -= pSss;
+    pSpecTrans = pSt;
+}
+
+float pdfLobes(V3 wo, V3 wi, const Params& p) {  // :340-378 (no flakes)
+    const Frame frame = shadingFrame(p.normal, p.tangent);
+    const V3 woL = frame.toLocal(wo), wiL = frame.toLocal(wi);
+    if (woL.z <= 0.0f) return 0.0f;
+    float pCoat, pSpec, pDiffRefl, pTrans, pSss, pSpecTrans;
+    lobeWeights(p, pCoat, pSpec, pDiffRefl, pTrans, pSss, pSpecTrans);
+    if (wiL.z <= 0.0f) {
+        float pdf = 0.0f;
+        if (pTrans > 0.0f) pdf += pTrans * cosineHemispherePdf(-wiL.z);
+        if (pSpecTrans > 0.0f) pdf += pSpecTrans * pdfSpecTransmission(woL, wiL, p);
+        return pdf;
+    }
+    const V3 hL = normalized(woL + wiL);
+    const float cosThetaD = dot(woL, hL);
+    if (cosThetaD <= 0.0f) return 0.0f;
+    const float pdfDiffuse = cosineHemispherePdf(wiL.z);
+    float ax, ay;
+    anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+    const float pdfSpecular = ggxVisiblePdf(woL, hL, ax, ay) / (4.0f * cosThetaD);
+    float pdfCoat = 0.0f;
+    if (pCoat > 0.0f) {
+        const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
+        pdfCoat = ggxVisiblePdf(woL, hL, alphaC, alphaC) / (4.0f * cosThetaD);
+    }
+    return pDiffRefl * pdfDiffuse + pSpec * pdfSpecular + pCoat * pdfCoat;
+}
+
+bool disneySampleFull(const Params& p, V3 wo, float uc, float u0, float u1, BsdfSample& out) {  // :680-729
+    out.delta = out.subsurface = out.passThrough = false;
+    const Frame frame = shadingFrame(p.normal, p.tangent);
+    const V3 woL = frame.toLocal(wo);
+    if (woL.z <= 0.0f) return false;
+    float pCoat, pSpec, pDiffRefl, pTrans, pSss, pSpecTrans;
+    lobeWeights(p, pCoat, pSpec, pDiffRefl, pTrans, pSss, pSpecTrans);
+    V3 wiL;
+    if (uc < pCoat) {
+        const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
+        wiL = reflectVec(woL, ggxSampleVisibleNormal(woL, alphaC, alphaC, u0, u1));
+        if (wiL.z <= 0.0f) return false;
+    } else if (uc < pCoat + pSpec) {
+        float ax, ay;  // sampleSpecularLobe, :382-396
+        anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+        wiL = reflectVec(woL, ggxSampleVisibleNormal(woL, ax, ay, u0, u1));
+        if (!(wiL.z > 0.0f)) return false;
+    } else if (uc < pCoat + pSpec + pSpecTrans) {
+        const float alpha = ggxClampAlpha(p.roughness * p.roughness);  // sampleSpecTransLobe, :398-405
+        const V3 wm = ggxSampleVisibleNormal(woL, alpha, alpha, u0, u1);
+        const float eta = p.frontFace ? (1.0f / p.ior) : p.ior;
+        if (!refractVec(woL, wm, eta, wiL)) return false;
+        if (!(wiL.z < 0.0f)) return false;
+    } else if (uc < pCoat + pSpec + pSpecTrans + pDiffRefl) {
+        wiL = cosineSampleHemisphere(u0, u1);
+    } else if (uc < pCoat + pSpec + pSpecTrans + pDiffRefl + pSss) {
+        wiL = cosineSampleHemisphere(u0, u1);
+        wiL.z = -wiL.z;
+        const float opaqueDielectric = (1.0f - p.metallic) * (1.0f - clampf(p.specularTransmission, 0.0f, 1.0f));
+        const float trans = clampf(p.diffuseTransmission, 0.0f, 1.0f);
+        const float coatO = coatTransmit(p, woL.z);
+        const V3 sc = resolvedSheenColor(p);
+        const float sheenO = sheenScale(p, std::max({sc.x, sc.y, sc.z}), woL.z, 1.0f);
+        const float layerO = DielectricLayer(p).transmit(woL.z);
+        const float w = p.subsurface * opaqueDielectric * (1.0f - trans) * coatO * sheenO * layerO / std::max(pSss, 1e-6f);
+        out.wi = normalized(frame.toWorld(wiL));
+        out.subsurface = true;
+        out.weight = coatTintFactor(p) * w;
+        out.pdf = pSss * cosineHemispherePdf(-wiL.z);
+        out.f = out.weight * out.pdf / std::max(-wiL.z, 1e-6f);
+        return out.pdf > 0.0f;
+    } else {
+        wiL = cosineSampleHemisphere(u0, u1);
+        wiL.z = -wiL.z;
+    }
+    out.wi = normalized(frame.toWorld(wiL));
+    out.f = evalLobes(wo, out.wi, p);
+    out.pdf = pdfLobes(wo, out.wi, p);
+    return out.pdf > 0.0f;
+}
+
+}  // namespace
+
+bool materialIsDelta(const pe_material_desc& m) {
+    if (m.type == PE_MAT_DIELECTRIC) return m.thin_walled != 0 || m.roughness <= 1e-3f;  // dielectric.h:37
+    return false;
+}
+
+V3 materialEmitted(const pe_material_desc& m) { return c3(m.emission); }
+
+V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, float& pdf) {
+    if (m.type == PE_MAT_LAMBERT) {  // lambertian.cpp:38-60
+        const Frame f(sp.normal);
+        const float ci = f.toLocal(wi).z, co = f.toLocal(wo).z;
+        if (ci <= 0.0f || co <= 0.0f) {
+            pdf = 0.0f;
+            return V3(0.0f);
+        }
+        pdf = cosineHemispherePdf(ci);
+        return c3(m.base_color) * INV_PI;
+    }
+    if (m.type != PE_MAT_DISNEY) throw std::runtime_error("oracle: material type not restated");
+    const Params p = resolve(m, sp);  // disney.cpp:664-674
+    pdf = pdfLobes(wo, wi, p);
+    return evalLobes(wo, wi, p);
+}
+
+bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1, BsdfSample& out) {
+    if (m.type == PE_MAT_LAMBERT) {  // material.h:91-96 -> lambertian.cpp:8-35 (uc unused)
+        out.delta = false;
+        out.subsurface = false;
+        out.passThrough = false;
+        const Frame f(sp.normal);
+        const V3 woL = f.toLocal(wo);
+        if (woL.z <= 0.0f) return false;
+        const V3 wiL = cosineSampleHemisphere(u0, u1);
+        out.wi = normalized(f.toWorld(wiL));
+        if (wiL.z <= 0.0f) return false;
+        out.f = c3(m.base_color) * INV_PI;
+        out.pdf = cosineHemispherePdf(wiL.z);
+        return true;
+    }
+    if (m.type != PE_MAT_DISNEY) throw std::runtime_error("oracle: material type not restated");
+    return disneySampleFull(resolve(m, sp), wo, uc, u0, u1, out);
+}
+
+}  // namespace oracle

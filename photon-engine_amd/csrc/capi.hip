@@ -225,6 +225,7 @@ int pe_device_destroy(pe_device* dev) {
     if (!dev) return PE_OK;
     (void)hipSetDevice(dev->ordinal);
     (void)hipStreamSynchronize(dev->stream);
+    wavefront_release(dev);
     dev->spill.release();
     dev->scratch.release();
     (void)hipStreamDestroy(dev->stream);

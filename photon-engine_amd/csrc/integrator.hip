@@ -1,0 +1,797 @@
+// integrator.hip — the wavefront path tracer behind pe_render.
+//
+// PathTracer::Li (integrators/path_tracer.cpp:321-890) is split at its ray casts into four kernels that run
+// over every path in flight:
+//   generate   tracePixel (engine/renderer.cpp:139-162): Sobol dims 0-1, pixel-filter offset, camera ray
+//   intersect  closest hit for the compacted extension-ray queue
+//   shade      one path vertex: environment / emission with MIS, NEE (light pick, light sample, BSDF eval,
+//              shadow ray appended to the shadow queue with its contribution held back), BSDF sample,
+//              throughput, Russian roulette, next ray appended to the next queue
+//   shadow     any hit for the shadow queue: an occluded ray zeroes its held-back contribution
+//   film       Film::addSample (core/image/film.cpp:56-71) of every sample of a pixel, in sample order
+// A held-back NEE contribution enters the path's radiance at the start of the path's next shade, or in the
+// film kernel if the path ended: the radiance sum sees its terms in the reference's order.
+//
+// Queues are arrays of path ids; appends are wave-aggregated (one atomic per wave, ballot + popcount for the
+// slot). Every kernel is a persistent grid-stride loop whose trip count is read from the device counter, so a
+// render of any number of bounces issues no host synchronisation until the end of a launch.
+//
+// Path state is SoA in HBM (float4 / uint4 columns, 16-byte coalesced loads per lane).
+#include "photon_hip.h"
+#include "pe_internal.h"
+#include "pe_lights.h"
+#include "pe_shading.h"
+#include "pe_sobol.h"
+#include "pe_traverse.h"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace pe {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxShadowSamples = 4;
+
+struct PathBuffers {
+    float4* o;        // origin xyz
+    float4* d;        // direction xyz
+    float4* thr;      // throughput rgb, w = last BSDF pdf
+    float4* L;        // radiance rgb
+    float4* last_p;   // previous vertex point (MIS)
+    float4* last_n;   // previous vertex normal (MIS)
+    uint4* ids;       // pixel, sample, Sobol pixel hash, packed bits (see pack_bits)
+    float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
+    float4* hits;     // per queue slot: t, triangle (int bits), u, v
+    uint32_t* q[2];   // ray queues (path ids)
+    uint32_t* sq_id;  // shadow queue: pid * S + s
+    float4* sq_o;     // shadow ray origin xyz, w = tmax
+    float4* sq_d;     // shadow ray direction
+    uint32_t* ctr;    // [0] current queue size, [1] next queue size, [2] shadow queue size
+    unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+};
+
+// packed: dim (bits 0-15), depth (16-22), diffuse bounces (23-29), specular bounce (30), camera ray (31)
+__device__ __forceinline__ uint32_t get_dim(uint32_t w) { return w & 0xFFFFu; }
+__device__ __forceinline__ uint32_t get_depth(uint32_t w) { return (w >> 16) & 0x7Fu; }
+__device__ __forceinline__ uint32_t get_diffuse(uint32_t w) { return (w >> 23) & 0x7Fu; }
+__device__ __forceinline__ bool get_specular(uint32_t w) { return (w >> 30) & 1u; }
+__device__ __forceinline__ bool get_camera(uint32_t w) { return (w >> 31) & 1u; }
+__device__ __forceinline__ uint32_t pack_bits(uint32_t dim, uint32_t depth, uint32_t diffuse, bool spec, bool cam) {
+    return (dim & 0xFFFFu) | ((depth & 0x7Fu) << 16) | ((diffuse & 0x7Fu) << 23) | (spec ? (1u << 30) : 0u) |
+           (cam ? (1u << 31) : 0u);
+}
+
+struct LaunchParams {
+    int width, height;
+    int first_sample;
+    int spl;              // samples per pixel in this launch
+    uint32_t num_paths;   // width * height * spl
+    int max_depth, rr_depth, shadow_samples;
+    float clamp_indirect;
+    uint64_t seed;
+    int filter_box;
+    float filter_width, filter_radius;
+    const float* filter_cdf;  // 513 entries when not a box
+    V3 cam_pos, cam_ll, cam_h, cam_v;
+};
+
+// Wave-aggregated append: returns this lane's slot in the queue behind `counter`.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    const unsigned long long mask = __ballot(want);
+    if (!want) return 0;
+    const int lane = __lane_id();
+    const int leader = __ffsll(static_cast<long long>(mask)) - 1;
+    const uint32_t before = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+    base = __shfl(base, leader);
+    return base + before;
+}
+
+__device__ __forceinline__ float filter_sample(const LaunchParams& lp, float u) {  // pixel_filter.cpp:79-87
+    if (lp.filter_box) return (u - 0.5f) * lp.filter_width;
+    const float* cdf = lp.filter_cdf;
+    int lo = 0, count = 513;
+    while (count > 0) {
+        const int step = count >> 1;
+        const int it = lo + step;
+        if (!(u < cdf[it])) {
+            lo = it + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    int i = lo - 1;
+    i = i < 0 ? 0 : (i > 511 ? 511 : i);
+    const float c0 = cdf[i], c1 = cdf[i + 1];
+    const float t = c1 > c0 ? (u - c0) / (c1 - c0) : 0.5f;
+    return -lp.filter_radius + (static_cast<float>(i) + clampf(t, 0.0f, 1.0f)) * (lp.filter_width / 512.0f);
+}
+
+__device__ __forceinline__ float power_heuristic(int nf, float f_pdf, int ng, float g_pdf) {  // utils.h:108-113
+    const float f = static_cast<float>(nf) * f_pdf, g = static_cast<float>(ng) * g_pdf;
+    const float f2 = f * f;
+    return f2 / (f2 + g * g);
+}
+
+// ---------------------------------------------------------------------------------------------- kernels
+
+__global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffers pb) {
+    const uint32_t pixels = static_cast<uint32_t>(lp.width) * static_cast<uint32_t>(lp.height);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lp.num_paths; i += stride) {
+        const uint32_t pix = i % pixels;
+        const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + i / pixels;
+        const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
+        const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
+        SobolState ss{sobol_pixel_hash(lp.seed, x, y), rev_bits(sample), 0u};
+        float fu, fv, lu, lv;
+        sobol_get2d(ss, fu, fv);  // pixel filter (renderer.cpp:148)
+        sobol_get2d(ss, lu, lv);  // lens sample, unused by the pinhole camera but consumed (renderer.cpp:151)
+        const float ox = filter_sample(lp, fu), oy = filter_sample(lp, fv);
+        const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(lp.width);
+        const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
+        const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
+        const V3 dir = normalized(target - lp.cam_pos);
+        pb.o[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
+        pb.d[i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+        pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+        pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
+        pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+        pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
+        for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
+        pb.q[0][i] = i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.ctr[0] = lp.num_paths;
+        pb.ctr[1] = 0;
+        pb.ctr[2] = 0;
+    }
+}
+
+// Sum of a per-lane counter over the wave, added once per wave to a 64-bit device counter.
+__device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
+    unsigned long long x = v;
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (__lane_id() == 0 && x) atomicAdd(dst, x);
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuffers pb, int cur, uint32_t* spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
+    const uint32_t n = pb.ctr[0];
+    const uint32_t* q = pb.q[cur];
+    uint32_t nv = 0, tv = 0;
+    for (uint32_t i = gid; i < n; i += lanes) {
+        const uint32_t pid = q[i];
+        const float4 o = pb.o[pid], d = pb.d[pid];
+        RayHit h;
+        trace_closest(sc.nodes, sc.tris, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, 1e30f, st, h, kCount ? &nv : nullptr,
+                      kCount ? &tv : nullptr);
+        pb.hits[i] = make_float4(h.t, __int_as_float(h.tri), h.u, h.v);
+    }
+    if (kCount) {
+        wave_count(&pb.stats[2], nv);
+        wave_count(&pb.stats[3], tv);
+    }
+}
+
+template <bool kCount>
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, uint32_t* spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
+    const uint32_t n = pb.ctr[2];
+    uint32_t nv = 0, tv = 0;
+    for (uint32_t i = gid; i < n; i += lanes) {
+        const float4 o = pb.sq_o[i], d = pb.sq_d[i];
+        if (trace_any(sc.nodes, sc.tris, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w, st, kCount ? &nv : nullptr,
+                      kCount ? &tv : nullptr))
+            pb.pend[pb.sq_id[i]] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    if (kCount) {
+        wave_count(&pb.stats[4], nv);
+        wave_count(&pb.stats[5], tv);
+    }
+}
+
+__device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
+    if (limit > 0.0f) {
+        const float m = fmaxf(c.x, fmaxf(c.y, c.z));
+        if (m > limit) return c * (limit / m);
+    }
+    return c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
+    const uint32_t n = pb.ctr[0];
+    const uint32_t* q = pb.q[cur];
+    uint32_t* qn = pb.q[cur ^ 1];
+    const EnergyTables& T = *sc.tables;
+    const int S = lp.shadow_samples;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // All lanes of a wave run the same number of iterations so the queue appends below see converged waves.
+    const uint32_t n_round = (n + stride - 1) / stride * stride;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        const bool active = i < n;
+        bool extend = false;
+        bool alive = false;
+        uint32_t pid = 0;
+        V3 new_o, new_d;
+        // NEE shadow rays of this vertex, appended after the vertex is shaded.
+        bool want_shadow[kMaxShadowSamples] = {false, false, false, false};
+        V3 sh_o[kMaxShadowSamples], sh_d[kMaxShadowSamples];
+        float sh_t[kMaxShadowSamples];
+        if (active) {
+            pid = q[i];
+            alive = true;
+            const float4 d4 = pb.d[pid];
+            const V3 dir = mk(d4.x, d4.y, d4.z);
+            const float4 t4 = pb.thr[pid];
+            V3 thr = mk(t4.x, t4.y, t4.z);
+            float last_pdf = t4.w;
+            const float4 L4 = pb.L[pid];
+            V3 L = mk(L4.x, L4.y, L4.z);
+            const uint4 ids = pb.ids[pid];
+            SobolState ss{ids.z, rev_bits(ids.y), get_dim(ids.w)};
+            uint32_t depth = get_depth(ids.w), diffuse = get_diffuse(ids.w);
+            bool specular = get_specular(ids.w), camera = get_camera(ids.w);
+            // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
+            for (int s = 0; s < S; ++s) {
+                const size_t k = static_cast<size_t>(pid) * S + s;
+                const float4 p = pb.pend[k];
+                L += mk(p.x, p.y, p.z);
+                pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            const float4 h = pb.hits[i];
+            const int tri = __float_as_int(h.y);
+            const float4 lp4 = pb.last_p[pid], ln4 = pb.last_n[pid];
+            const V3 last_p = mk(lp4.x, lp4.y, lp4.z), last_n = mk(ln4.x, ln4.y, ln4.z);
+            if (tri < 0) {
+                // Miss: environment with MIS against NEE (path_tracer.cpp:482-514).
+                if (sc.has_env) {
+                    const V3 le = env_eval(sc.env, dir);
+                    float weight = 1.0f;
+                    if (!specular) {
+                        const Picker pk = picker_prepare(sc, last_p, last_n);
+                        const float light_pdf = picker_pmf(sc, pk, sc.num_lights, last_p, last_n) * env_pdf(sc.env, dir);
+                        weight = power_heuristic(1, last_pdf, S, light_pdf);
+                    }
+                    V3 c = thr * (le * weight);
+                    if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
+                    if (finite3(c)) L += c;
+                }
+                alive = false;
+            } else {
+                Surface si;
+                fill_hit(sc, tri, dir, h.z, h.w, si);
+                if (camera && (si.flags & kTriCameraInvisible)) {
+                    // Camera rays pass through lights hidden from the camera (path_tracer.cpp:423-441): the ray
+                    // continues from the far side, and the continuation is not a new vertex.
+                    new_o = offset_ray_origin(si.point, si.p_error, si.ng, dir);
+                    new_d = dir;
+                    extend = true;
+                } else {
+                    const GpuMaterial& m = sc.materials[si.material];
+                    if (m.flags & kMatFlagEmissive) {  // path_tracer.cpp:565-593
+                        const int li = si.emitter;
+                        float weight = 1.0f;
+                        if (!specular) {
+                            float light_pdf = 0.0f;
+                            if (li >= 0) {
+                                const Picker pk = picker_prepare(sc, last_p, last_n);
+                                light_pdf = picker_pmf(sc, pk, li, last_p, last_n) * light_pdf_hit(sc.lights[li], last_p, si.point);
+                            }
+                            if (light_pdf > 0.0f && last_pdf > 0.0f) weight = power_heuristic(1, last_pdf, S, light_pdf);
+                        }
+                        V3 c = thr * (m.emission * weight);
+                        if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
+                        if (finite3(c)) L += c;
+                    }
+                    if (static_cast<int>(depth) >= lp.max_depth) {
+                        alive = false;
+                    } else {
+                        const V3 wo = -dir;
+                        const bool is_delta = (m.flags & kMatFlagDelta) != 0;
+                        if (sc.light_count > 0 && !is_delta) {  // NEE, path_tracer.cpp:631-698
+                            const Picker pk = picker_prepare(sc, si.point, si.normal);
+                            for (int s = 0; s < S; ++s) {
+                                float pmf = 0.0f;
+                                const float u = sobol_get1d(ss);
+                                const int li = picker_sample(sc, pk, u, si.point, si.normal, pmf);
+                                float ux, uy;
+                                sobol_get2d(ss, ux, uy);
+                                if (li < 0 || !(pmf > 0.0f)) continue;
+                                const bool from_env = li >= sc.num_lights;
+                                const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
+                                                                : light_sample(sc.lights[li], si.point, ux, uy);
+                                if (!(ls.pdf > 0.0f) || is_black(ls.li)) continue;
+                                V3 so, sd;
+                                float st;
+                                if (!(ls.distance < kFloatMax) || ls.distance >= 1e29f) {
+                                    so = shadow_origin(si, ls.wi);
+                                    sd = ls.wi;
+                                    st = 1e30f;
+                                } else if (!spawn_ray_to(si, si.point + ls.wi * ls.distance, so, sd, st)) {
+                                    continue;
+                                }
+                                float bsdf_pdf = 0.0f;
+                                const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, bsdf_pdf);
+                                const float cos_t = fabsf(dot(ls.wi, si.normal));
+                                if (is_black(f) || !(cos_t > 0.0f)) continue;
+                                const float light_pdf = pmf * ls.pdf;
+                                const float weight = power_heuristic(S, light_pdf, 1, bsdf_pdf);
+                                const V3 nee = f * ls.li * (cos_t * weight / (light_pdf * static_cast<float>(S)));
+                                V3 c = thr * nee;
+                                if (diffuse >= 1) c = clamp_indirect(c, lp.clamp_indirect);
+                                if (!finite3(c)) continue;
+                                pb.pend[static_cast<size_t>(pid) * S + s] = make_float4(c.x, c.y, c.z, 0.0f);
+                                want_shadow[s] = true;
+                                sh_o[s] = so;
+                                sh_d[s] = sd;
+                                sh_t[s] = st;
+                            }
+                        }
+                        // BSDF sample (path_tracer.cpp:736-743), throughput and roulette (:814-870).
+                        const float uc = sobol_get1d(ss);
+                        float u0, u1;
+                        sobol_get2d(ss, u0, u1);
+                        BsdfSample bs;
+                        if (!bsdf_sample(m, T, si, wo, uc, u0, u1, bs) || bs.pdf <= 0.0f || is_black(bs.f)) {
+                            alive = false;
+                        } else {
+                            const float cos_t = fabsf(dot(bs.wi, si.normal));
+                            thr *= bs.f * (cos_t / bs.pdf);
+                            if (!finite3(thr)) {
+                                alive = false;
+                            } else {
+                                last_pdf = bs.pdf;
+                                pb.last_p[pid] = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
+                                pb.last_n[pid] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                                specular = bs.delta;
+                                if (!bs.delta) ++diffuse;
+                                camera = camera && bs.delta && dot(bs.wi, si.normal) < 0.0f;
+                                new_o = spawn_origin(si, bs.wi);
+                                new_d = bs.wi;
+                                ++depth;
+                                extend = true;
+                                if (static_cast<int>(depth) > lp.rr_depth) {
+                                    const float mx = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                                    if (mx < 1.0f) {
+                                        if (!(mx > 0.0f) || sobol_get1d(ss) >= mx) {
+                                            extend = false;
+                                            alive = false;
+                                        } else {
+                                            thr = thr / mx;
+                                        }
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+            pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
+            if (extend) {
+                pb.o[pid] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
+                pb.d[pid] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+                pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+                pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
+            }
+            (void)alive;
+        }
+        const uint32_t slot = wave_append(&pb.ctr[1], extend);
+        if (extend) qn[slot] = pid;
+        for (int s = 0; s < S; ++s) {
+            const uint32_t ss_slot = wave_append(&pb.ctr[2], want_shadow[s]);
+            if (want_shadow[s]) {
+                pb.sq_id[ss_slot] = pid * static_cast<uint32_t>(S) + static_cast<uint32_t>(s);
+                pb.sq_o[ss_slot] = make_float4(sh_o[s].x, sh_o[s].y, sh_o[s].z, sh_t[s]);
+                pb.sq_d[ss_slot] = make_float4(sh_d[s].x, sh_d[s].y, sh_d[s].z, 0.0f);
+            }
+        }
+    }
+}
+
+// After a bounce: the next queue becomes current, the shadow queue is emptied, ray counts accumulate.
+__global__ void k_rotate(PathBuffers pb) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        pb.stats[0] += pb.ctr[0];
+        pb.stats[1] += pb.ctr[2];
+        pb.ctr[0] = pb.ctr[1];
+        pb.ctr[1] = 0;
+        pb.ctr[2] = 0;
+    }
+}
+
+__global__ void k_zero_stats(PathBuffers pb) {
+    if (blockIdx.x == 0 && threadIdx.x < 8) pb.stats[threadIdx.x] = 0;
+}
+
+// Film::addSample for every sample of every pixel, in sample order (film.cpp:56-71).
+__global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb, float* __restrict__ sum,
+                                                 uint32_t* __restrict__ count, float* __restrict__ lum2,
+                                                 unsigned long long* __restrict__ rejected) {
+    const uint32_t pixels = static_cast<uint32_t>(lp.width) * static_cast<uint32_t>(lp.height);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const int S = lp.shadow_samples;
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += stride) {
+        float sr = sum[3 * static_cast<size_t>(p)], sg = sum[3 * static_cast<size_t>(p) + 1], sb = sum[3 * static_cast<size_t>(p) + 2];
+        uint32_t c = count[p];
+        float l2 = lum2[p];
+        uint32_t rej = 0;
+        for (int k = 0; k < lp.spl; ++k) {
+            const uint32_t pid = p + static_cast<uint32_t>(k) * pixels;
+            const float4 L4 = pb.L[pid];
+            V3 L = mk(L4.x, L4.y, L4.z);
+            for (int s = 0; s < S; ++s) {
+                const float4 e = pb.pend[static_cast<size_t>(pid) * S + s];
+                L += mk(e.x, e.y, e.z);
+            }
+            ++c;
+            const bool ok = finite3(L) && L.x >= 0.0f && L.y >= 0.0f && L.z >= 0.0f;
+            if (!ok) {
+                ++rej;
+                continue;
+            }
+            sr += L.x;
+            sg += L.y;
+            sb += L.z;
+            const float lum = luminance(L);
+            l2 += lum * lum;
+        }
+        sum[3 * static_cast<size_t>(p)] = sr;
+        sum[3 * static_cast<size_t>(p) + 1] = sg;
+        sum[3 * static_cast<size_t>(p) + 2] = sb;
+        count[p] = c;
+        lum2[p] = l2;
+        if (rej) atomicAdd(rejected, static_cast<unsigned long long>(rej));
+    }
+}
+
+// Pixel-filter inverse CDF (pixel_filter.cpp:37-63), built on the host for the non-box filters.
+std::vector<float> filter_cdf(int type, float width) {
+    const float w = std::clamp(width, 0.25f, 8.0f);
+    const float r = 0.5f * w;
+    auto evaluate = [&](float x) -> float {
+        if (std::abs(x) > r) return 0.0f;
+        switch (type) {
+            case PE_FILTER_TRIANGLE: return std::max(0.0f, 1.0f - std::abs(x) / r);
+            case PE_FILTER_GAUSSIAN: {
+                const float s = w / 6.0f;
+                const float k = 1.0f / (2.0f * s * s);
+                return std::max(0.0f, std::exp(-k * x * x) - std::exp(-k * r * r));
+            }
+            case PE_FILTER_BLACKMAN_HARRIS: {
+                const float a = kTwoPi * (x / w + 0.5f);
+                return 0.35875f - 0.48829f * std::cos(a) + 0.14128f * std::cos(2.0f * a) - 0.01168f * std::cos(3.0f * a);
+            }
+            default: return 1.0f;
+        }
+    };
+    std::vector<float> cdf(513, 0.0f);
+    const float dx = w / 512;
+    double acc = 0.0;
+    for (int i = 0; i < 512; ++i) {
+        acc += std::max(0.0f, evaluate(-r + (static_cast<float>(i) + 0.5f) * dx));
+        cdf[static_cast<size_t>(i) + 1] = static_cast<float>(acc);
+    }
+    if (!(acc > 0.0)) return {};
+    for (float& c : cdf) c = static_cast<float>(c / acc);
+    cdf.back() = 1.0f;
+    return cdf;
+}
+
+}  // namespace
+
+enum KernelKind { kKGenerate = 0, kKIntersect, kKShade, kKShadow, kKFilm, kKindCount };
+const char* const kKernelNames[kKindCount] = {"generate", "intersect", "shade", "shadow", "film"};
+
+// HIP events around each wavefront launch on the device's stream while profiling is on.
+struct Profiler {
+    bool on = false;
+    struct Rec {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    double ms[kKindCount] = {};
+    uint64_t launches[kKindCount] = {};
+
+    void begin(int kind, hipStream_t s) {
+        if (!on) return;
+        Rec r{kind, nullptr, nullptr};
+        if (hipEventCreate(&r.a) != hipSuccess || hipEventCreate(&r.b) != hipSuccess) return;
+        (void)hipEventRecord(r.a, s);
+        recs.push_back(r);
+    }
+    void end(hipStream_t s) {
+        if (!on || recs.empty()) return;
+        (void)hipEventRecord(recs.back().b, s);
+    }
+    void drain() {
+        for (Rec& r : recs) {
+            float t = 0.0f;
+            if (hipEventSynchronize(r.b) == hipSuccess && hipEventElapsedTime(&t, r.a, r.b) == hipSuccess) {
+                ms[r.kind] += t;
+                ++launches[r.kind];
+            }
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        recs.clear();
+    }
+};
+
+// Per-device wavefront buffers, grown on demand and kept between renders.
+struct Wavefront {
+    Profiler prof;
+    uint32_t capacity = 0;
+    int shadow_samples = 0;
+    DevBuf o, d, thr, L, last_p, last_n, ids, pend, hits, q0, q1, sq_id, sq_o, sq_d, ctr, stats, filter;
+    int filter_type = -1;
+    float filter_width = 0.0f;
+
+    ~Wavefront() { prof.drain(); }
+
+    int ensure(uint32_t n, int S) {
+        if (n <= capacity && S == shadow_samples) return PE_OK;
+        const size_t N = n, NS = static_cast<size_t>(n) * S;
+        if (!o.alloc(N * 16) || !d.alloc(N * 16) || !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) ||
+            !last_n.alloc(N * 16) || !ids.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) ||
+            !q0.alloc(N * 4) || !q1.alloc(N * 4) || !sq_id.alloc(NS * 4) || !sq_o.alloc(NS * 16) ||
+            !sq_d.alloc(NS * 16) || !ctr.alloc(64) || !stats.alloc(64)) {
+            capacity = 0;
+            return PE_ERR_OOM;
+        }
+        capacity = n;
+        shadow_samples = S;
+        return PE_OK;
+    }
+
+    PathBuffers view() {
+        PathBuffers pb;
+        pb.o = o.as<float4>();
+        pb.d = d.as<float4>();
+        pb.thr = thr.as<float4>();
+        pb.L = L.as<float4>();
+        pb.last_p = last_p.as<float4>();
+        pb.last_n = last_n.as<float4>();
+        pb.ids = ids.as<uint4>();
+        pb.pend = pend.as<float4>();
+        pb.hits = hits.as<float4>();
+        pb.q[0] = q0.as<uint32_t>();
+        pb.q[1] = q1.as<uint32_t>();
+        pb.sq_id = sq_id.as<uint32_t>();
+        pb.sq_o = sq_o.as<float4>();
+        pb.sq_d = sq_d.as<float4>();
+        pb.ctr = ctr.as<uint32_t>();
+        pb.stats = stats.as<unsigned long long>();
+        return pb;
+    }
+};
+
+Wavefront* wavefront_of(pe_device* dev);
+void wavefront_release(pe_device* dev);
+
+}  // namespace pe
+
+using namespace pe;
+
+namespace pe {
+
+Wavefront* wavefront_of(pe_device* dev) {
+    if (!dev->wavefront) dev->wavefront = new Wavefront();
+    return static_cast<Wavefront*>(dev->wavefront);
+}
+
+void wavefront_release(pe_device* dev) {
+    delete static_cast<Wavefront*>(dev->wavefront);
+    dev->wavefront = nullptr;
+}
+
+}  // namespace pe
+
+extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings,
+                         pe_film* film, int first_sample, int sample_count, const volatile int32_t* cancel,
+                         pe_batch_result* result) {
+    if (result) {
+        result->completed = 0;
+        result->samples_traced = 0;
+        result->extension_rays = 0;
+        result->shadow_rays = 0;
+    }
+    if (!dev || !scene || !camera || !settings || !film) {
+        set_error("pe_render: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    if (scene->dev != dev || film->dev != dev) {
+        set_error("pe_render: scene or film belongs to another device");
+        return PE_ERR_INVALID;
+    }
+    if (first_sample < 0 || sample_count < 0) {
+        set_error("pe_render: negative sample range");
+        return PE_ERR_INVALID;
+    }
+    if (camera->lens_radius != 0.0f) {
+        set_error("pe_render: only the pinhole camera (lens_radius 0) is implemented");
+        return PE_ERR_UNSUPPORTED;
+    }
+    if (settings->max_bounces < 0 || settings->max_bounces > 100) {
+        set_error("pe_render: max_bounces must be in [0, 100]");
+        return PE_ERR_INVALID;
+    }
+    if (result) result->completed = 1;
+    if (sample_count == 0) return PE_OK;
+    try {
+        PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+        Wavefront* wf = wavefront_of(dev);
+        const int S = std::clamp(settings->shadow_quality, 1, kMaxShadowSamples);  // PathTracer ctor (path_tracer.h:43)
+        const uint32_t pixels = static_cast<uint32_t>(film->width) * static_cast<uint32_t>(film->height);
+        const int spl_req = std::max(1, settings->samples_per_launch);
+        const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(uint64_t{1} << 31, uint64_t{pixels} * spl_req));
+        const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
+        int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
+        if (rc != PE_OK) return rc;
+        int blocks = 0;
+        rc = ensure_trace_resources(dev, blocks);
+        if (rc != PE_OK) return rc;
+
+        LaunchParams lp{};
+        lp.width = film->width;
+        lp.height = film->height;
+        lp.max_depth = settings->max_bounces;
+        lp.rr_depth = settings->rr_depth;
+        lp.shadow_samples = S;
+        lp.clamp_indirect = settings->clamp_indirect > 0.0f ? settings->clamp_indirect : 0.0f;
+        lp.seed = settings->seed;
+        lp.filter_width = std::clamp(settings->filter_width, 0.25f, 8.0f);
+        lp.filter_radius = 0.5f * lp.filter_width;
+        lp.filter_box = settings->filter_type == PE_FILTER_BOX ? 1 : 0;
+        if (!lp.filter_box) {
+            if (wf->filter_type != settings->filter_type || wf->filter_width != settings->filter_width) {
+                const std::vector<float> cdf = filter_cdf(settings->filter_type, settings->filter_width);
+                if (cdf.empty()) {
+                    lp.filter_box = 1;
+                } else {
+                    if (!wf->filter.upload(cdf.data(), cdf.size() * 4, dev->stream)) return PE_ERR_OOM;
+                    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+                    wf->filter_type = settings->filter_type;
+                    wf->filter_width = settings->filter_width;
+                }
+            }
+            lp.filter_cdf = wf->filter.as<float>();
+        }
+        lp.cam_pos = mk(camera->position[0], camera->position[1], camera->position[2]);
+        lp.cam_ll = mk(camera->lower_left[0], camera->lower_left[1], camera->lower_left[2]);
+        lp.cam_h = mk(camera->horizontal[0], camera->horizontal[1], camera->horizontal[2]);
+        lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
+
+        PathBuffers pb = wf->view();
+        hipStream_t s = dev->stream;
+        const int gblocks = dev->max_grid_blocks;
+        (void)blocks;
+        hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, s, pb);
+        const bool count = settings->count_visits != 0;
+        uint32_t* spill = dev->spill.as<uint32_t>();
+        const SceneView view = scene->view;
+        // One path vertex for every path in flight: closest hit, shade, shadow rays, queue rotation.
+        auto bounce = [&](int cur) {
+            wf->prof.begin(kKIntersect, s);
+            if (count) hipLaunchKernelGGL(k_intersect<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
+            else hipLaunchKernelGGL(k_intersect<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
+            wf->prof.end(s);
+            wf->prof.begin(kKShade, s);
+            hipLaunchKernelGGL(k_shade, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+            wf->prof.end(s);
+            wf->prof.begin(kKShadow, s);
+            if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+            else hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+            wf->prof.end(s);
+            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, s, pb);
+        };
+        uint64_t traced = 0;
+        bool cancelled = false;
+        for (int done = 0; done < sample_count;) {
+            if (cancel && *cancel) {
+                cancelled = true;
+                break;
+            }
+            const int spl = std::min(spl_cap, sample_count - done);
+            lp.first_sample = first_sample + done;
+            lp.spl = spl;
+            lp.num_paths = pixels * static_cast<uint32_t>(spl);
+            const int gen_blocks = static_cast<int>(std::min<uint32_t>((lp.num_paths + kBlock - 1) / kBlock, 65535u));
+            wf->prof.begin(kKGenerate, s);
+            hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, lp, pb);
+            wf->prof.end(s);
+            // max_depth + 1 vertices per path; a camera-invisible light adds a continuation without a vertex,
+            // so the loop runs on (checking the queue on the host) until every path has ended.
+            int it = 0;
+            const int planned = lp.max_depth + 1;
+            for (;;) {
+                for (; it < planned; ++it) bounce(it & 1);
+                uint32_t left = 0;
+                PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr, 4, hipMemcpyDeviceToHost, s));
+                PE_HIP_CHECK(hipStreamSynchronize(s));
+                if (left == 0) break;
+                if (it > planned + 256) {
+                    set_error("pe_render: paths still alive after max_bounces + 256 continuations");
+                    return PE_ERR_INVALID;
+                }
+                bounce(it & 1);
+                ++it;
+            }
+            const int film_blocks = static_cast<int>(std::min<uint32_t>((pixels + kBlock - 1) / kBlock, 65535u));
+            wf->prof.begin(kKFilm, s);
+            hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, film->sum.as<float>(),
+                               film->count.as<uint32_t>(), film->lum2.as<float>(),
+                               film->rejected.as<unsigned long long>());
+            wf->prof.end(s);
+            PE_HIP_CHECK(hipGetLastError());
+            done += spl;
+            traced += static_cast<uint64_t>(pixels) * spl;
+        }
+        unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+        PE_HIP_CHECK(hipMemcpyAsync(st, pb.stats, sizeof(st), hipMemcpyDeviceToHost, s));
+        PE_HIP_CHECK(hipStreamSynchronize(s));
+        if (result) {
+            result->completed = cancelled ? 0 : 1;
+            result->samples_traced = traced;
+            result->extension_rays = st[0];
+            result->shadow_rays = st[1];
+            result->node_visits = st[2];
+            result->tri_visits = st[3];
+            result->shadow_node_visits = st[4];
+            result->shadow_tri_visits = st[5];
+        }
+        if (cancelled) {
+            set_error("pe_render: cancelled");
+            return PE_ERR_CANCELLED;
+        }
+        return PE_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("pe_render: ") + e.what());
+        return PE_ERR_INVALID;
+    }
+}
+
+extern "C" int pe_profile_enable(pe_device* dev, int enable) {
+    if (!dev) {
+        set_error("pe_profile_enable: NULL device");
+        return PE_ERR_INVALID;
+    }
+    Wavefront* wf = wavefront_of(dev);
+    wf->prof.drain();
+    wf->prof.on = enable != 0;
+    return PE_OK;
+}
+
+extern "C" int pe_profile_read(pe_device* dev, pe_kernel_stats* out, int capacity, int* count) {
+    if (!dev || (capacity > 0 && !out)) {
+        set_error("pe_profile_read: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    Wavefront* wf = wavefront_of(dev);
+    wf->prof.drain();
+    const int n = std::min<int>(capacity, kKindCount);
+    for (int k = 0; k < n; ++k) {
+        std::snprintf(out[k].name, sizeof(out[k].name), "%s", kKernelNames[k]);
+        out[k].launches = wf->prof.launches[k];
+        out[k].total_ms = wf->prof.ms[k];
+        wf->prof.launches[k] = 0;
+        wf->prof.ms[k] = 0.0;
+    }
+    if (count) *count = n;
+    return PE_OK;
+}

@@ -1,0 +1,98 @@
+// pe_device_scene.h — shading-side scene data in HBM (materials, lights, environment, energy tables) and the
+// by-value view of a whole scene that every wavefront kernel receives.
+#pragma once
+
+#include "pe_gpu_types.h"
+#include "pe_math.h"
+
+namespace pe {
+
+// Microfacet energy tables (32 x 32 GGX E/Ea/Eb, their hemispherical averages, Charlie sheen albedo), built on
+// the host exactly as the reference builds them at first use (materials/microfacet.cpp:46-133).
+constexpr int kTableN = 32;
+struct EnergyTables {
+    float e[kTableN * kTableN];
+    float ea[kTableN * kTableN];
+    float eb[kTableN * kTableN];
+    float avg0[kTableN], avg1[kTableN], avg2[kTableN];
+    float sheen[kTableN * kTableN];
+};
+
+// A material with its parameters clamped as the reference's setters clamp them and every quantity that depends
+// on the material alone precomputed on the host (same operations the reference performs per shading call).
+enum : int32_t { kMatDisney = 0, kMatLambert = 1, kMatDielectric = 2 };
+struct alignas(16) GpuMaterial {
+    int32_t type;
+    uint32_t flags;  // kMatFlag*
+    float metallic, roughness;
+    V3 base_color;
+    float specular;
+    V3 emission;
+    float clearcoat;
+    V3 coat_tint;
+    float clearcoat_roughness;
+    V3 sheen_color;       // resolvedSheenColor (disney.cpp:160-166), already scaled by sheen * (1 - metallic)
+    float sheen_max;      // max component of sheen_color
+    V3 edge_tint;
+    float sheen_roughness;
+    float ior, coat_ior, specular_transmission, anisotropy;
+    float diffuse_roughness, diffuse_transmission, subsurface, pad0;
+    // derived
+    float ax, ay, alpha;             // anisoAlpha and sqrt(ax * ay) (disney.cpp:55-60, :133-135)
+    float f0d, wd;                   // dielectricF0, dielectricWeight (:34-42)
+    float inv_one_minus_avg;         // DielectricLayer::invOneMinusAvg (:139)
+    float f0c, alpha_c;              // coatF0 (:44-46), clearcoat alpha
+    float p_coat, p_spec, p_spec_trans, p_diff_refl, p_sss, p_trans;  // lobeWeights (:313-338)
+    float st_w, opaque_dielectric, trans_w, diff_w;                      // evalLobes weights (:230-237, :263)
+    V3 coat_tint_factor;             // coatTintFactor (:155-157)
+    float pad1;
+};
+
+enum : uint32_t { kMatFlagEmissive = 1u, kMatFlagDelta = 2u, kMatFlagSheen = 4u, kMatFlagCoat = 8u };
+
+// AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.
+struct alignas(16) GpuLight {
+    int32_t type;         // PE_LIGHT_AREA / PE_LIGHT_TRIANGLE
+    int32_t rectangle;    // AreaLight::m_rectangle
+    float area;
+    float lum;            // max(0, luminance(radiance))
+    V3 p0, p1, p2;        // AREA: position, u, v
+    V3 radiance;
+    V3 normal;
+    V3 center;            // importance reference point
+    float len_u, len_v;   // |u|, |v| (AREA)
+};
+
+// EnvironmentLight: image, MIS-compensated sampling distribution and the optional sun cone.
+struct EnvView {
+    const float* rgb;        // 3 * w * h
+    const float* func;       // w * h (sampling distribution of the mis-compensated image)
+    const float* marginal;   // h + 1
+    const float* cond;       // h * (w + 1)
+    int32_t w, h;
+    float func_int, rotation, intensity, mean_luminance;
+    int32_t valid;           // distribution valid
+    V3 sun_dir, sun_radiance;
+    float sun_cos_max, sun_pdf_weight;
+};
+
+enum : int32_t { kPickUniform = 0, kPickExhaustive = 1 };
+
+struct SceneView {
+    const BvhNode* nodes;
+    const TriAccel* tris;
+    const TriShade* shade;
+    const float4* pos;
+    const float4* nrm;
+    const float2* uv;
+    const GpuMaterial* materials;
+    const GpuLight* lights;
+    const EnergyTables* tables;
+    EnvView env;
+    int32_t num_lights;      // explicit lights (Scene::lights())
+    int32_t has_env;
+    int32_t light_count;     // candidates incl. the environment
+    int32_t pick_mode;       // kPick*
+};
+
+}  // namespace pe

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "photon_hip.h"
+#include "pe_device_scene.h"
 #include "pe_gpu_types.h"
 
 #include <hip/hip_runtime.h>
@@ -64,6 +65,7 @@ struct pe_device {
     uint32_t spill_lanes = 0;    // lanes the spill buffer is sized for
     pe::DevBuf scratch;          // small per-call staging (counters)
     int max_grid_blocks = 0;     // persistent-grid size for 256-lane blocks
+    void* wavefront = nullptr;   // integrator buffers (integrator.hip), grown on demand
 };
 
 struct pe_scene {
@@ -73,9 +75,13 @@ struct pe_scene {
     int num_shapes = 0;
     int bvh_depth = 0;
     pe::DevBuf nodes, tris, tri_shade, positions, normals, uvs;
+    pe::DevBuf materials, lights, tables, env_rgb, env_func, env_marginal, env_cond;
+    pe::SceneView view{};
     uint64_t num_nodes = 0;
     uint64_t device_bytes() const {
-        return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes();
+        return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes() +
+               materials.bytes() + lights.bytes() + tables.bytes() + env_rgb.bytes() + env_func.bytes() +
+               env_marginal.bytes() + env_cond.bytes();
     }
 };
 
@@ -84,6 +90,13 @@ struct pe_film {
     int width = 0, height = 0;
     pe::DevBuf sum, count, lum2, rejected;
 };
+
+namespace pe {
+// Persistent grid size and the traversal-stack spill buffer sized for it (capi.hip).
+int ensure_trace_resources(pe_device* dev, int& blocks);
+// Releases the integrator's buffers of a device (integrator.hip).
+void wavefront_release(pe_device* dev);
+}  // namespace pe
 
 #define PE_HIP_CHECK(call)                                                                              \
     do {                                                                                                \

@@ -8,4 +8,7 @@ namespace pe {
 // Validates desc, builds the BVH and per-triangle records, and uploads them on dev->stream.
 int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc);
 
+// Materials, lights, environment distributions and energy tables (scene_prep.cpp); fills sc.view.
+int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc);
+
 }  // namespace pe

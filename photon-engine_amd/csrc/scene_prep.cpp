@@ -1,0 +1,369 @@
+// scene_prep.cpp — host preparation of the shading-side scene: materials with their material-only terms
+// precomputed, light records, the environment's sampling distributions and the energy tables.
+//
+// Every quantity is computed with the reference's operations in the reference's order (cited per function)
+// and without contraction, so the device sees the same floats the reference's IEEE build computes.
+#include "pe_scene_host.h"
+#include "pe_microfacet.h"
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+namespace pe {
+
+namespace {
+
+V3 c3(const float* v) { return mk(v[0], v[1], v[2]); }
+V3 clamp01v(V3 c) { return clamp3(c, 0.0f, 1.0f); }
+
+float f0_from_ior(float ior) {  // disney.cpp:25-28
+    const float r = (ior - 1.0f) / (ior + 1.0f);
+    return r * r;
+}
+
+}  // namespace
+
+// DisneyMaterial setters (disney.h:22-126), then the per-material terms of disney.cpp.
+int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMaterial& g, std::string& err) {
+    g = GpuMaterial{};
+    g.emission = c3(d.emission);
+    g.flags = is_black(g.emission) ? 0u : kMatFlagEmissive;
+    if (d.type == PE_MAT_LAMBERT) {
+        g.type = kMatLambert;
+        g.base_color = c3(d.base_color);
+        return PE_OK;
+    }
+    if (d.type == PE_MAT_DIELECTRIC) {
+        err = "Dielectric materials are not implemented on the gfx950 path yet";
+        return PE_ERR_UNSUPPORTED;
+    }
+    if (d.type != PE_MAT_DISNEY) {
+        err = "unknown material type " + std::to_string(d.type);
+        return PE_ERR_INVALID;
+    }
+    if (d.thin_film_thickness > 0.0f) {
+        err = "Disney thin-film interference is not implemented on the gfx950 path";
+        return PE_ERR_UNSUPPORTED;
+    }
+    g.type = kMatDisney;
+    g.base_color = c3(d.base_color);
+    g.metallic = d.metallic;
+    g.roughness = std::fmax(0.001f, d.roughness);
+    g.specular = d.specular;
+    g.clearcoat = clampf(d.clearcoat, 0.0f, 1.0f);
+    g.clearcoat_roughness = std::fmax(0.001f, d.clearcoat_roughness);
+    g.ior = clampf(d.ior, 1.0001f, 3.0f);
+    g.specular_transmission = clampf(d.specular_transmission, 0.0f, 1.0f);
+    g.coat_ior = clampf(d.coat_ior, 1.0001f, 3.0f);
+    g.coat_tint = clamp01v(c3(d.coat_tint));
+    g.anisotropy = clampf(d.anisotropy, 0.0f, 1.0f);
+    const float sheen = clampf(d.sheen, 0.0f, 1.0f);
+    g.sheen_roughness = clampf(d.sheen_roughness, 0.02f, 1.0f);
+    g.diffuse_roughness = clampf(d.diffuse_roughness, 0.0f, 1.0f);
+    g.diffuse_transmission = clampf(d.diffuse_transmission, 0.0f, 1.0f);
+    g.subsurface = clampf(d.subsurface, 0.0f, 1.0f);
+    g.edge_tint = clamp01v(c3(d.edge_tint));
+    if (g.specular_transmission > 0.0f || g.subsurface > 0.0f) {
+        err = "Disney specular transmission / subsurface lobes are not implemented on the gfx950 path yet";
+        return PE_ERR_UNSUPPORTED;
+    }
+
+    // resolvedSheenColor (disney.cpp:160-166)
+    V3 sc = splat(0.0f);
+    if (!(sheen <= 0.0f || g.metallic >= 1.0f)) {
+        if (d.has_sheen_color) {
+            sc = clamp01v(c3(d.sheen_color)) * (sheen * (1.0f - g.metallic));
+        } else {
+            const float lum = luminance(g.base_color);
+            const V3 tint = lum > 1e-6f ? g.base_color * (1.0f / lum) : splat(1.0f);
+            sc = (splat(0.5f) + clamp01v(tint) * 0.5f) * (sheen * (1.0f - g.metallic));
+        }
+    }
+    g.sheen_color = sc;
+    g.sheen_max = std::max(sc.x, std::max(sc.y, sc.z));
+    if (!is_black(sc)) g.flags |= kMatFlagSheen;
+    if (g.clearcoat > 0.0f) g.flags |= kMatFlagCoat;
+
+    // anisoAlpha (disney.cpp:55-60) and DielectricLayer (:128-146)
+    const float aspect = std::sqrt(std::fmax(0.1f, 1.0f - 0.9f * clampf(g.anisotropy, 0.0f, 1.0f)));
+    const float a = g.roughness * g.roughness;
+    g.ax = a / aspect;
+    g.ay = a * aspect;
+    g.alpha = std::sqrt(g.ax * g.ay);
+    g.f0d = clampf(f0_from_ior(g.ior) * std::fmax(1.0f, 2.0f * g.specular), 0.0f, 1.0f);
+    g.wd = clampf(2.0f * g.specular, 0.0f, 1.0f);
+    g.inv_one_minus_avg = 1.0f / std::fmax(1e-4f, 1.0f - g.wd * ggx_schlick_average_albedo(T, g.alpha, g.f0d));
+    g.f0c = clampf(f0_from_ior(g.coat_ior), 0.0f, 1.0f);
+    g.alpha_c = g.clearcoat_roughness * g.clearcoat_roughness;
+    g.coat_tint_factor = g.clearcoat > 0.0f ? lerp3(splat(1.0f), g.coat_tint, g.clearcoat) : splat(1.0f);
+
+    // lobeWeights (disney.cpp:313-338), no flakes
+    const float p_coat = 0.25f * g.clearcoat;
+    const float rem = 1.0f - p_coat;
+    const float st = clampf(g.specular_transmission, 0.0f, 1.0f) * (1.0f - g.metallic);
+    const float lo = 0.5f * (st > 0.0f ? 1.0f : g.wd);
+    const float p_spec = rem * (lo + g.metallic * (1.0f - lo));  // lerp(a, b, t) = a + t * (b - a), utils.h:32
+    float p_rest = rem - p_spec;
+    const float p_st = p_rest * st;
+    p_rest -= p_st;
+    const float p_trans = p_rest * clampf(g.diffuse_transmission, 0.0f, 1.0f);
+    float p_diff = p_rest - p_trans;
+    const float p_sss = g.metallic < 1.0f ? p_diff * g.subsurface : 0.0f;
+    p_diff -= p_sss;
+    g.p_coat = p_coat;
+    g.p_spec = p_spec;
+    g.p_spec_trans = p_st;
+    g.p_diff_refl = p_diff;
+    g.p_sss = p_sss;
+    g.p_trans = p_trans;
+
+    // evalLobes weights (disney.cpp:230-237, :263)
+    g.st_w = st;
+    g.opaque_dielectric = (1.0f - g.metallic) * (1.0f - clampf(g.specular_transmission, 0.0f, 1.0f));
+    g.trans_w = clampf(g.diffuse_transmission, 0.0f, 1.0f) * g.opaque_dielectric;
+    g.diff_w = g.opaque_dielectric * (1.0f - clampf(g.diffuse_transmission, 0.0f, 1.0f)) * (1.0f - g.subsurface);
+    return PE_OK;
+}
+
+// AreaLight constructor (area_light.h:20-27) / TriangleLight constructor (area_light.cpp:195-200).
+GpuLight prepare_light(const pe_light_desc& d) {
+    GpuLight L{};
+    L.type = d.type;
+    L.p0 = c3(d.p0);
+    L.p1 = c3(d.p1);
+    L.p2 = c3(d.p2);
+    L.radiance = c3(d.radiance);
+    L.lum = std::fmax(0.0f, luminance(L.radiance));
+    if (d.type == PE_LIGHT_AREA) {
+        const V3 n = cross(L.p1, L.p2);
+        L.area = length(n);
+        L.normal = normalized(n);
+        L.len_u = length(L.p1);
+        L.len_v = length(L.p2);
+        L.rectangle = (L.len_u > 0.0f && L.len_v > 0.0f && std::fabs(dot(L.p1, L.p2)) <= 1e-4f * L.len_u * L.len_v) ? 1 : 0;
+        L.center = L.p0 + (L.p1 + L.p2) * 0.5f;  // AreaLight::importance (area_light.cpp:101)
+    } else {
+        const V3 n = cross(L.p1 - L.p0, L.p2 - L.p0);
+        const float len = length(n);
+        L.area = 0.5f * len;
+        L.normal = len > 0.0f ? n / len : mk(0.0f, 1.0f, 0.0f);
+        L.center = (L.p0 + L.p1 + L.p2) * (1.0f / 3.0f);  // TriangleLight::importance (area_light.cpp:203)
+    }
+    return L;
+}
+
+namespace {
+
+struct Distribution {
+    float func_int = 0.0f;
+    std::vector<float> func, marginal, cond;
+    bool valid() const { return func_int > 0.0f && !marginal.empty(); }
+};
+
+// EnvironmentLight::buildCdf (environment_light.cpp:149-172)
+void build_cdf(Distribution& d, int w, int h) {
+    d.marginal.assign(static_cast<size_t>(h) + 1, 0.0f);
+    d.cond.assign(static_cast<size_t>(h) * static_cast<size_t>(w + 1), 0.0f);
+    for (int y = 0; y < h; ++y) {
+        float* row = d.cond.data() + static_cast<size_t>(y) * static_cast<size_t>(w + 1);
+        row[0] = 0.0f;
+        for (int x = 0; x < w; ++x) row[x + 1] = row[x] + d.func[static_cast<size_t>(y) * w + x];
+        d.marginal[static_cast<size_t>(y) + 1] = d.marginal[static_cast<size_t>(y)] + row[w];
+    }
+    d.func_int = d.marginal[static_cast<size_t>(h)];
+    if (!(d.func_int > 0.0f)) {
+        d.func_int = 0.0f;
+        d.marginal.clear();
+        return;
+    }
+    const float inv = 1.0f / d.func_int;
+    for (float& c : d.marginal) c *= inv;
+    for (int y = 0; y < h; ++y) {
+        float* row = d.cond.data() + static_cast<size_t>(y) * static_cast<size_t>(w + 1);
+        const float row_sum = row[w];
+        if (row_sum <= 0.0f) continue;
+        const float inv_row = 1.0f / row_sum;
+        for (int x = 0; x <= w; ++x) row[x] *= inv_row;
+    }
+}
+
+}  // namespace
+
+// EnvironmentLight(image, rotation, intensity) + setSun (environment_light.cpp:64-182); the sampling
+// distribution kept on the device is the MIS-compensated one (or the full one where compensation is off).
+int prepare_environment(pe_device* dev, const pe_env_desc& e, pe_scene& sc, EnvView& ev) {
+    ev = EnvView{};
+    ev.rotation = e.rotation;
+    ev.intensity = e.intensity;
+    const bool has_image = e.rgb && e.width > 0 && e.height > 0;
+    const int w = has_image ? e.width : 0, h = has_image ? e.height : 0;
+    Distribution full, mis;
+    if (has_image) {
+        const float* rgb = e.rgb;
+        auto lum_at = [&](int x, int y) {
+            const float* p = rgb + 3 * (static_cast<size_t>(y) * w + x);
+            return luminance(mk(p[0], p[1], p[2]));
+        };
+        full.func.assign(static_cast<size_t>(w) * h, 0.0f);
+        std::vector<float> sin_row(static_cast<size_t>(h));
+        double weighted = 0.0, weights = 0.0;
+        for (int y = 0; y < h; ++y) {
+            const float sin_theta = std::sin(kPi * (static_cast<float>(y) + 0.5f) / static_cast<float>(h));
+            sin_row[static_cast<size_t>(y)] = sin_theta;
+            for (int x = 0; x < w; ++x) {
+                float L = lum_at(x, y);
+                if (!(L > 0.0f)) L = 0.0f;
+                full.func[static_cast<size_t>(y) * w + x] = L * sin_theta;
+                weighted += static_cast<double>(L) * sin_theta;
+                weights += sin_theta;
+            }
+        }
+        build_cdf(full, w, h);
+        if (full.valid()) {
+            const float mean_l = weights > 0.0 ? static_cast<float>(weighted / weights) : 0.0f;
+            mis.func.resize(full.func.size());
+            double mis_total = 0.0, full_total = 0.0;
+            for (int y = 0; y < h; ++y) {
+                const float s = sin_row[static_cast<size_t>(y)];
+                for (int x = 0; x < w; ++x) {
+                    const size_t i = static_cast<size_t>(y) * w + x;
+                    float L = lum_at(x, y);
+                    if (!(L > 0.0f)) L = 0.0f;
+                    const float excess = L - mean_l;
+                    mis.func[i] = excess > 1e-4f * mean_l ? excess * s : 0.0f;
+                    mis_total += mis.func[i];
+                    full_total += full.func[i];
+                }
+            }
+            bool comp = mis_total > 1e-3 * full_total;
+            if (comp) {
+                build_cdf(mis, w, h);
+                comp = mis.valid();
+            }
+            if (!comp) mis = full;
+        }
+        // power() (environment_light.cpp:308-323): every fourth texel in both directions.
+        V3 sum = splat(0.0f);
+        float wsum = 0.0f;
+        for (int y = 0; y < h; y += 4) {
+            const float s = std::sin(kPi * (static_cast<float>(y) + 0.5f) / static_cast<float>(h));
+            for (int x = 0; x < w; x += 4) {
+                const float* p = rgb + 3 * (static_cast<size_t>(y) * w + x);
+                sum += mk(p[0], p[1], p[2]) * s;
+                wsum += s;
+            }
+        }
+        const V3 power = (wsum > 0.0f ? sum / wsum : sum) * e.intensity * 4.0f * kPi;
+        ev.mean_luminance = std::fmax(0.0f, luminance(power) / (4.0f * kPi));
+    } else {
+        const V3 power = splat(0.5f) * e.intensity * 4.0f * kPi;
+        ev.mean_luminance = std::fmax(0.0f, luminance(power) / (4.0f * kPi));
+    }
+    ev.w = w;
+    ev.h = h;
+    ev.valid = mis.valid() ? 1 : 0;
+    ev.func_int = mis.func_int;
+    // setSun (environment_light.cpp:168-182)
+    ev.sun_dir = mk(0.0f, 1.0f, 0.0f);
+    ev.sun_cos_max = 1.0f;
+    ev.sun_pdf_weight = 0.0f;
+    if (e.has_sun) {
+        ev.sun_dir = normalized(c3(e.sun_direction));
+        ev.sun_cos_max = clampf(e.sun_cos_max, -0.999f, 0.9999999f);
+        ev.sun_radiance = c3(e.sun_radiance) * e.intensity;
+        const float omega = kTwoPi * (1.0f - ev.sun_cos_max);
+        const float sun_power = std::fmax(0.0f, luminance(ev.sun_radiance)) * omega;
+        const float sky_power = 4.0f * kPi * ev.mean_luminance;
+        if (sun_power > 0.0f) {
+            ev.sun_pdf_weight = clampf(sun_power / (sun_power + sky_power), 0.15f, 0.85f);
+            if (!full.valid()) ev.sun_pdf_weight = 1.0f;
+        }
+    }
+    hipStream_t s = dev->stream;
+    if (has_image) {
+        if (!sc.env_rgb.upload(e.rgb, 3ull * w * h * sizeof(float), s)) return PE_ERR_OOM;
+        if (mis.valid()) {
+            if (!sc.env_func.upload(mis.func.data(), mis.func.size() * sizeof(float), s) ||
+                !sc.env_marginal.upload(mis.marginal.data(), mis.marginal.size() * sizeof(float), s) ||
+                !sc.env_cond.upload(mis.cond.data(), mis.cond.size() * sizeof(float), s))
+                return PE_ERR_OOM;
+        }
+        const hipError_t er = hipStreamSynchronize(s);
+        if (er != hipSuccess) {
+            set_error(std::string("environment upload: ") + hipGetErrorString(er));
+            return PE_ERR_HIP;
+        }
+    }
+    ev.rgb = sc.env_rgb.as<float>();
+    ev.func = sc.env_func.as<float>();
+    ev.marginal = sc.env_marginal.as<float>();
+    ev.cond = sc.env_cond.as<float>();
+    return PE_OK;
+}
+
+int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
+    EnergyTables* T = new EnergyTables();
+    build_energy_tables(*T);
+    std::vector<GpuMaterial> mats(static_cast<size_t>(std::max(1, desc.num_materials)));
+    for (int i = 0; i < desc.num_materials; ++i) {
+        std::string err;
+        const int rc = prepare_material(desc.materials[i], *T, mats[static_cast<size_t>(i)], err);
+        if (rc != PE_OK) {
+            delete T;
+            set_error("material " + std::to_string(i) + ": " + err);
+            return rc;
+        }
+    }
+    std::vector<GpuLight> lights(static_cast<size_t>(std::max(1, desc.num_lights)));
+    for (int i = 0; i < desc.num_lights; ++i) {
+        const pe_light_desc& L = desc.lights[i];
+        if (L.type != PE_LIGHT_AREA && L.type != PE_LIGHT_TRIANGLE) {
+            delete T;
+            set_error("light " + std::to_string(i) + ": unknown type " + std::to_string(L.type));
+            return PE_ERR_INVALID;
+        }
+        lights[static_cast<size_t>(i)] = prepare_light(L);
+    }
+    hipStream_t s = dev->stream;
+    if (!sc.materials.upload(mats.data(), mats.size() * sizeof(GpuMaterial), s) ||
+        !sc.lights.upload(lights.data(), lights.size() * sizeof(GpuLight), s) ||
+        !sc.tables.upload(T, sizeof(EnergyTables), s)) {
+        delete T;
+        return PE_ERR_OOM;
+    }
+    const hipError_t er = hipStreamSynchronize(s);
+    delete T;
+    if (er != hipSuccess) {
+        set_error(std::string("shading upload: ") + hipGetErrorString(er));
+        return PE_ERR_HIP;
+    }
+    SceneView& v = sc.view;
+    v = SceneView{};
+    v.num_lights = desc.num_lights;
+    v.has_env = desc.environment ? 1 : 0;
+    v.light_count = desc.num_lights + v.has_env;
+    if (desc.environment) {
+        const int rc = prepare_environment(dev, *desc.environment, sc, v.env);
+        if (rc != PE_OK) return rc;
+    }
+    // LightSampler::build (light_sampler.cpp:42-74): at most 33 candidates are chosen exhaustively.
+    if (v.light_count > 33) {
+        set_error("more than 33 light candidates: the light-BVH selection is not implemented on the gfx950 path yet");
+        return PE_ERR_UNSUPPORTED;
+    }
+    v.pick_mode = v.light_count <= 1 ? kPickUniform : kPickExhaustive;
+    v.nodes = sc.nodes.as<BvhNode>();
+    v.tris = sc.tris.as<TriAccel>();
+    v.shade = sc.tri_shade.as<TriShade>();
+    v.pos = sc.positions.as<float4>();
+    v.nrm = sc.normals.as<float4>();
+    v.uv = sc.uvs.as<float2>();
+    v.materials = sc.materials.as<GpuMaterial>();
+    v.lights = sc.lights.as<GpuLight>();
+    v.tables = sc.tables.as<EnergyTables>();
+    return PE_OK;
+}
+
+}  // namespace pe

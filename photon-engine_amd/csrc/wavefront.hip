@@ -138,23 +138,4 @@ int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
     return PE_ERR_UNSUPPORTED;
 }
 
-int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
-              int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result) {
-    (void)first_sample;
-    (void)sample_count;
-    (void)cancel;
-    if (result) {
-        result->completed = 0;
-        result->samples_traced = 0;
-        result->extension_rays = 0;
-        result->shadow_rays = 0;
-    }
-    if (!dev || !scene || !camera || !settings || !film) {
-        set_error("pe_render: NULL argument");
-        return PE_ERR_INVALID;
-    }
-    set_error("pe_render: not implemented yet");
-    return PE_ERR_UNSUPPORTED;
-}
-
 }  // extern "C"

@@ -91,12 +91,18 @@ class Camera(C.Structure):
 class Settings(C.Structure):
     _fields_ = [("max_bounces", C.c_int32), ("rr_depth", C.c_int32), ("shadow_quality", C.c_int32),
                 ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
-                ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("reserved", C.c_int32 * 7)]
+                ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("count_visits", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
 
 
 class BatchResult(C.Structure):
     _fields_ = [("completed", C.c_int32), ("samples_traced", C.c_uint64), ("extension_rays", C.c_uint64),
-                ("shadow_rays", C.c_uint64)]
+                ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_visits", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_visits", C.c_uint64)]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
 
 class PhotonHipError(RuntimeError):
@@ -129,6 +135,8 @@ SIGNATURES = {
     "pe_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_float), _P(C.c_uint64)]),
     "pe_render": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _VP, C.c_int, C.c_int, _P(C.c_int32),
                             _P(BatchResult)]),
+    "pe_profile_enable": (C.c_int, [_VP, C.c_int]),
+    "pe_profile_read": (C.c_int, [_VP, _P(KernelStats), C.c_int, _P(C.c_int)]),
     "pe_sobol_batch": (C.c_int, [_VP, C.c_uint64, C.c_size_t, _P(C.c_int32), _P(C.c_int32), _P(C.c_uint32),
                                  _P(C.c_uint32), _P(C.c_uint32)]),
     "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
