@@ -88,7 +88,20 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene (smoke runs only; not the workload)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_mib > 0:
+        import torch
+
+        n = args.copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_mib, "bytes_read": n * 4, "bytes_written": n * 4}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
               f"a frame shards by sample range across GPUs with one film reduction, which is not implemented.",
