@@ -187,7 +187,8 @@ typedef struct pe_settings {
     float filter_width;
     float clamp_indirect;       /* 0 = off */
     uint64_t seed;              /* Sobol seed; the reference renderer hard-codes 12345 */
-    int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together (>=1) */
+    int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together; 0 = sized
+                                   by the library (about 2^26 paths, within a quarter of free memory) */
     int32_t count_visits;       /* 1: also count BVH node / triangle visits (measurement runs; slower) */
     int32_t reserved[6];
 } pe_settings;

@@ -287,3 +287,50 @@ def sobol_reference(seed, px, py, idx, dim):
     lib.ref_sobol_batch(seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
                         capi.ptr(idx, C.c_uint32), capi.ptr(dim, C.c_uint32), _fp(out))
     return out
+
+
+_HIPDEV_SIGS = {
+    "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
+    "ref_scene_free": (None, [_VP]),
+    "hipdev_session_render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, C.c_float, C.c_int, C.c_int, _F, _U, _I, C.c_char_p, C.c_int]),
+    "hipdev_enumerate": (C.c_int, [C.c_char_p, C.c_int]),
+}
+
+
+def hip_device_harness() -> _Lib | None:
+    """oracle/_ref/libphoton_hipdevice.so: the HipDevice adapter driven by the reference's RenderSession
+    (None where oracle/_ref was not built)."""
+    if "hipdev" not in _cache:
+        path = ORACLE_DIR / "_ref" / "libphoton_hipdevice.so"
+        _cache["hipdev"] = _Lib(path, "", _HIPDEV_SIGS) if path.exists() else None
+    return _cache["hipdev"]
+
+
+class HipDeviceSession:
+    """A reference Scene built from the descriptor, rendered by HipDevice under RenderSession (Final mode)."""
+
+    def __init__(self, scene):
+        self._lib = hip_device_harness()
+        if self._lib is None:
+            raise FileNotFoundError("oracle/_ref/libphoton_hipdevice.so was not built (the reference tree is absent)")
+        desc, self._keep = scene.pack()
+        self.h = self._lib.ref_scene_build(C.byref(desc))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._lib.ref_scene_free(self.h)
+            self.h = None
+
+    def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0):
+        """Returns (rc, mean (H,W,3), counts (H,W), batches, error text)."""
+        f3 = C.c_float * 3
+        mean = np.zeros((view.height, view.width, 3), np.float32)
+        counts = np.zeros((view.height, view.width), np.uint32)
+        batches = C.c_int32()
+        err = C.create_string_buffer(512)
+        rc = self._lib.hipdev_session_render(
+            self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
+            view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, _fp(mean),
+            capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
+        return rc, mean, counts, batches.value, err.value.decode(errors="replace")

@@ -1,0 +1,98 @@
+// hipdev_driver.cpp — test-only harness that drives photon-engine_amd/host/hip_device.cpp (the RenderDevice
+// adapter over libphoton_hip.so) through the reference's own RenderSession (engine/render_session.cpp).
+//
+// Part of the oracle (test infrastructure): compiled with the reference sources and the adapter into
+// oracle/_ref/libphoton_hipdevice.so (oracle/Makefile.ref). The scene is the reference Scene that
+// ref_scene_build (ref_driver.cpp) makes from a pe_scene_desc, so the adapter packs a genuine reference scene.
+#include "engine/render_session.h"
+#include "camera/perspective_camera.h"
+#include "hip_device.h"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+using namespace photon;
+
+std::shared_ptr<const Scene> ref_scene_shared(void* h);
+
+#define HIPDEV_API __attribute__((visibility("default")))
+
+namespace {
+void copy_error(const std::string& msg, char* err, int errlen) {
+    if (!err || errlen <= 0) return;
+    std::strncpy(err, msg.c_str(), static_cast<size_t>(errlen) - 1);
+    err[errlen - 1] = '\0';
+}
+}  // namespace
+
+extern "C" {
+
+// RenderSession(Final, targetSpp = spp) on a HipDevice for the reference scene `h`; writes the resolved mean
+// (3 floats/px), the sample counts and the number of session batches. Returns 0 on success, 1 when the device
+// refused the scene or a batch (err says why), 2 when no HIP device could be created.
+HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* camTarget, const float* camUp,
+                                     float fov, float aspect, int width, int height, int maxBounces,
+                                     int shadowQuality, int filterType, float filterWidth, int spp, int maxBatch,
+                                     float* mean, uint32_t* counts, int* batches, char* err, int errlen) {
+    std::string e;
+    std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
+    if (!dev) {
+        copy_error(e, err, errlen);
+        return 2;
+    }
+    dev->setScene(ref_scene_shared(h));
+    if (!dev->hasScene()) {
+        copy_error(dev->lastError(), err, errlen);
+        return 1;
+    }
+    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
+                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
+                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = spp;
+    s.maxBounces = maxBounces;
+    s.shadowQuality = shadowQuality;
+    s.filter.type = static_cast<PixelFilterType>(filterType);
+    s.filter.width = filterWidth;
+    SessionConfig cfg;
+    cfg.mode = RenderMode::Final;
+    cfg.targetSpp = spp;
+    cfg.maxBatch = maxBatch;
+    RenderSession session(*dev);
+    session.begin(cam, s, cfg);
+    int n = 0;
+    while (!session.finished()) {
+        if (!session.advance(nullptr)) {
+            copy_error(dev->lastError().empty() ? "RenderSession::advance returned false" : dev->lastError(), err,
+                       errlen);
+            return 1;
+        }
+        ++n;
+    }
+    if (batches) *batches = n;
+    const Film& film = session.film();
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * static_cast<size_t>(width) + static_cast<size_t>(x);
+            const Color3f c = film.resolvedPixel(x, y);
+            mean[3 * k] = c.r;
+            mean[3 * k + 1] = c.g;
+            mean[3 * k + 2] = c.b;
+            counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+        }
+    return 0;
+}
+
+// The HIP devices as the adapter lists them: writes "id|name|usable|note" lines into out.
+HIPDEV_API int hipdev_enumerate(char* out, int outlen) {
+    std::string s;
+    for (const DeviceInfo& d : enumerateHipDevices())
+        s += d.id + "|" + d.name + "|" + (d.usable ? "1" : "0") + "|" + d.note + "\n";
+    copy_error(s, out, outlen);
+    return 0;
+}
+
+}  // extern "C"

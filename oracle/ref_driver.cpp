@@ -344,3 +344,6 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
 }
 
 }  // extern "C"
+
+// The reference Scene behind a handle of ref_scene_build (for oracle/hipdev_driver.cpp, linked beside it).
+std::shared_ptr<const Scene> ref_scene_shared(void* h) { return static_cast<RefScene*>(h)->scene; }

@@ -4,7 +4,8 @@
 // The reference builds a BVH2 with 12 buckets on the largest centroid axis only (geometry/bvh.cpp:100-238).
 // Closest-hit results do not depend on the tree (only on the per-triangle arithmetic, see pe_traverse.h),
 // so this build is free to optimise for the GPU: all three axes are binned, leaves hold up to max_leaf
-// triangles (fewer, longer-lived divergent loops), and depth is capped so the traversal stack is bounded.
+// triangles, and depth is bounded so the traversal stack is: past max_depth, nodes are halved by count
+// until they fit a leaf of max_leaf_cap triangles.
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -86,7 +87,10 @@ private:
         }
         node.box = box;
         const uint32_t n = e - b;
-        if (n == 1 || depth >= m_opt.max_depth) {
+        // Past the depth cap a node becomes a leaf once it fits one (max_leaf_cap triangles); larger ones are
+        // halved by count, which adds at most log2(n / max_leaf_cap) levels.
+        const bool capped = depth >= m_opt.max_depth;
+        if (n == 1 || (capped && static_cast<int>(n) <= m_opt.max_leaf_cap)) {
             make_leaf(ni, b, e);
             return;
         }
@@ -98,7 +102,7 @@ private:
         const int nb = std::max(2, std::min(m_opt.bins, 64));
         int best_axis = -1, best_split = -1;
         float best_cost = std::numeric_limits<float>::infinity();
-        for (int axis = 0; axis < 3; ++axis) {
+        for (int axis = 0; axis < 3 && !capped; ++axis) {
             const float c0 = cbox.lo[axis], c1 = cbox.hi[axis];
             if (!(c1 > c0)) continue;
             const float scale = static_cast<float>(nb) / (c1 - c0);
@@ -141,7 +145,7 @@ private:
         const float split_cost = m_opt.traversal_cost +
                                  (parent_area > 0.0f ? m_opt.triangle_cost * best_cost / parent_area
                                                      : m_opt.triangle_cost * static_cast<float>(n));
-        if (static_cast<int>(n) <= m_opt.max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) {
+        if (!capped && static_cast<int>(n) <= m_opt.max_leaf && (best_axis < 0 || leaf_cost <= split_cost)) {
             make_leaf(ni, b, e);
             return;
         }
@@ -158,7 +162,8 @@ private:
             mid = static_cast<uint32_t>(it - m_idx.begin());
         }
         if (mid == b || mid == e) {
-            // No usable SAH plane (coincident centroids): split by count along the widest axis.
+            // No usable SAH plane (coincident centroids, or past the depth cap): split by count along the
+            // widest axis.
             int axis = 0;
             float ext = -1.0f;
             for (int k = 0; k < 3; ++k)

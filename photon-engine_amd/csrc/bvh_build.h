@@ -11,8 +11,10 @@
 namespace pe {
 
 struct BvhBuildOptions {
-    int max_leaf = 4;     // largest leaf the SAH may choose
-    int max_depth = 60;   // interior depth cap (the traversal stack holds kStackDepth entries)
+    int max_leaf = 4;       // largest leaf the SAH may choose
+    int max_depth = 60;     // SAH depth; deeper nodes are halved by count until they hold <= max_leaf_cap
+    int max_leaf_cap = 16;  // largest leaf at all (a traversal leaf item holds at most 16 triangles), so the
+                            // tree is at most max_depth + log2(n / 16) deep (< 96 entries of traversal stack)
     int bins = 16;        // SAH bins per axis (all three axes are tried)
     float traversal_cost = 1.0f;
     float triangle_cost = 1.0f;

@@ -67,47 +67,59 @@ __global__ void __launch_bounds__(256) k_sobol_batch(uint64_t seed, size_t n, co
     }
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode* __restrict__ nodes,
-                                                                 const TriAccel* __restrict__ tris,
-                                                                 const TriShade* __restrict__ shade, size_t n,
-                                                                 const float* __restrict__ rays,
-                                                                 int4* __restrict__ hits, float2* __restrict__ bary,
-                                                                 uint32_t* __restrict__ spill) {
-    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
-    const uint32_t lanes = gridDim.x * blockDim.x;
-    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
-    for (size_t i = gid; i < n; i += lanes) {
-        const float* r = rays + 8 * i;
-        RayHit h;
-        const bool hit = trace_closest(nodes, tris, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), r[6], r[7], st, h,
-                                       nullptr, nullptr);
+// Batch rays are 8 floats: origin xyz, direction xyz, tmin, tmax (read as two float4).
+struct BatchRays {
+    const float4* r;
+    __device__ __forceinline__ void fetch(uint32_t i, float4& a, float4& b) const {
+        a = r[2 * static_cast<size_t>(i)];
+        b = r[2 * static_cast<size_t>(i) + 1];
+    }
+    __device__ __forceinline__ TraceRay decode(const float4& a, const float4& b) const {
+        return TraceRay{mk(a.x, a.y, a.z), mk(a.w, b.x, b.y), b.z, b.w, 0u};
+    }
+};
+
+struct BatchClosest : BatchRays {
+    const TriShade* shade;
+    int4* hits;
+    float2* bary;
+    __device__ __forceinline__ void closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v) const {
         int4 o;
-        if (hit) {
-            const TriShade s = shade[h.tri];
-            o = make_int4(__float_as_int(h.t), s.shape, s.prim, h.tri);
+        if (tri >= 0) {
+            const TriShade s = shade[tri];
+            o = make_int4(__float_as_int(t), s.shape, s.prim, tri);
         } else {
             o = make_int4(__float_as_int(-1.0f), -1, -1, -1);
+            u = v = 0.0f;
         }
         hits[i] = o;
-        if (bary) bary[i] = make_float2(h.u, h.v);
+        if (bary) bary[i] = make_float2(u, v);
     }
+};
+
+struct BatchAny : BatchRays {
+    uint8_t* occ;
+    __device__ __forceinline__ void any(uint32_t i, const TraceRay&, bool occluded) const { occ[i] = occluded ? 1 : 0; }
+};
+
+__global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode* __restrict__ nodes,
+                                                                 const TriAccel* __restrict__ tris, BatchClosest job,
+                                                                 uint32_t n, uint32_t* __restrict__ spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
+    uint32_t nv = 0, tv = 0;
+    trace_queue<false, false>(job, n, nodes, tris, st, nv, tv);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode* __restrict__ nodes,
-                                                                const TriAccel* __restrict__ tris, size_t n,
-                                                                const float* __restrict__ rays,
-                                                                uint8_t* __restrict__ occ, uint32_t* __restrict__ spill) {
+                                                                const TriAccel* __restrict__ tris, BatchAny job,
+                                                                uint32_t n, uint32_t* __restrict__ spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
-    const uint32_t lanes = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
-    for (size_t i = gid; i < n; i += lanes) {
-        const float* r = rays + 8 * i;
-        occ[i] = trace_any(nodes, tris, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]), r[6], r[7], st, nullptr, nullptr)
-                     ? 1
-                     : 0;
-    }
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
+    uint32_t nv = 0, tv = 0;
+    trace_queue<true, false>(job, n, nodes, tris, st, nv, tv);
 }
 
 // Persistent grid for 256-lane blocks and the spill buffer sized for it.
@@ -319,6 +331,10 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
         return PE_ERR_INVALID;
     }
     if (n == 0) return PE_OK;
+    if (n >= (size_t{1} << 31)) {
+        set_error("pe_intersect_batch: at most 2^31 - 1 rays per batch");
+        return PE_ERR_INVALID;
+    }
     PE_HIP_CHECK(hipSetDevice(dev->ordinal));
     int blocks = 0;
     int rc = ensure_trace_resources(dev, blocks);
@@ -327,9 +343,13 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
     DevBuf drays, dhits, dbary;
     if (!drays.upload(rays, n * 32, dev->stream) || !dhits.alloc(n * 16) || (bary && !dbary.alloc(n * 8)))
         return PE_ERR_OOM;
+    BatchClosest job{};
+    job.r = drays.as<float4>();
+    job.shade = scene->tri_shade.as<TriShade>();
+    job.hits = dhits.as<int4>();
+    job.bary = bary ? dbary.as<float2>() : nullptr;
     hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
-                       scene->tris.as<TriAccel>(), scene->tri_shade.as<TriShade>(), n, drays.as<float>(),
-                       dhits.as<int4>(), bary ? dbary.as<float2>() : nullptr, dev->spill.as<uint32_t>());
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
     if (bary) PE_HIP_CHECK(hipMemcpyAsync(bary, dbary.get(), n * 8, hipMemcpyDeviceToHost, dev->stream));
@@ -345,6 +365,10 @@ int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* ra
         return PE_ERR_INVALID;
     }
     if (n == 0) return PE_OK;
+    if (n >= (size_t{1} << 31)) {
+        set_error("pe_occluded_batch: at most 2^31 - 1 rays per batch");
+        return PE_ERR_INVALID;
+    }
     PE_HIP_CHECK(hipSetDevice(dev->ordinal));
     int blocks = 0;
     int rc = ensure_trace_resources(dev, blocks);
@@ -352,8 +376,11 @@ int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* ra
     blocks = grid_for(dev, n);
     DevBuf drays, docc;
     if (!drays.upload(rays, n * 32, dev->stream) || !docc.alloc(n)) return PE_ERR_OOM;
+    BatchAny job{};
+    job.r = drays.as<float4>();
+    job.occ = docc.as<uint8_t>();
     hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
-                       scene->tris.as<TriAccel>(), n, drays.as<float>(), docc.as<uint8_t>(), dev->spill.as<uint32_t>());
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(occluded, docc.get(), n, hipMemcpyDeviceToHost, dev->stream));
     PE_HIP_CHECK(hipStreamSynchronize(dev->stream));

@@ -12,9 +12,11 @@
 // A held-back NEE contribution enters the path's radiance at the start of the path's next shade, or in the
 // film kernel if the path ended: the radiance sum sees its terms in the reference's order.
 //
-// Queues are arrays of path ids; appends are wave-aggregated (one atomic per wave, ballot + popcount for the
-// slot). Every kernel is a persistent grid-stride loop whose trip count is read from the device counter, so a
-// render of any number of bounces issues no host synchronisation until the end of a launch.
+// Queues hold the ray records themselves (origin | path id, direction; the shadow queue origin | tmax,
+// direction | held-back slot), so a traversal lane starts a ray with one 32-byte load and no indirection;
+// appends are wave-aggregated (one atomic per wave, ballot + popcount for the slot). Every kernel is
+// persistent and reads its trip count from the device counter, so a render of any number of bounces issues
+// no host synchronisation until the end of a launch.
 //
 // Path state is SoA in HBM (float4 / uint4 columns, 16-byte coalesced loads per lane).
 #include "photon_hip.h"
@@ -37,8 +39,8 @@ constexpr int kBlock = 256;
 constexpr int kMaxShadowSamples = 4;
 
 struct PathBuffers {
-    float4* o;        // origin xyz
-    float4* d;        // direction xyz
+    float4* ro[2];    // ray queues (ping-pong), per slot: origin xyz, w = path id (uint bits)
+    float4* rd[2];    //                                   direction xyz
     float4* thr;      // throughput rgb, w = last BSDF pdf
     float4* L;        // radiance rgb
     float4* last_p;   // previous vertex point (MIS)
@@ -46,10 +48,8 @@ struct PathBuffers {
     uint4* ids;       // pixel, sample, Sobol pixel hash, packed bits (see pack_bits)
     float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
-    uint32_t* q[2];   // ray queues (path ids)
-    uint32_t* sq_id;  // shadow queue: pid * S + s
-    float4* sq_o;     // shadow ray origin xyz, w = tmax
-    float4* sq_d;     // shadow ray direction
+    float4* sq_o;     // shadow queue: origin xyz, w = tmax
+    float4* sq_d;     //               direction xyz, w = pend index pid * S + s (uint bits)
     uint32_t* ctr;    // [0] current queue size, [1] next queue size, [2] shadow queue size
     unsigned long long* stats;  // [0] extension rays, [1] shadow rays
 };
@@ -138,15 +138,14 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
         const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
         const V3 dir = normalized(target - lp.cam_pos);
-        pb.o[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
-        pb.d[i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+        pb.ro[0][i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(i));
+        pb.rd[0][i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
         pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
         pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
-        pb.q[0][i] = i;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         pb.ctr[0] = lp.num_paths;
@@ -162,23 +161,47 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) 
     if (__lane_id() == 0 && x) atomicAdd(dst, x);
 }
 
+// Extension rays: the current ray queue, hits written per queue slot.
+struct ExtensionJob {
+    const float4* ro;
+    const float4* rd;
+    float4* hits;
+    __device__ __forceinline__ void fetch(uint32_t i, float4& a, float4& b) const {
+        a = ro[i];
+        b = rd[i];
+    }
+    __device__ __forceinline__ TraceRay decode(const float4& a, const float4& b) const {
+        return TraceRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 0.0f, 1e30f, 0u};
+    }
+    __device__ __forceinline__ void closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v) const {
+        hits[i] = make_float4(t, __int_as_float(tri), u, v);
+    }
+};
+
+// Shadow rays: an occluded ray zeroes its held-back contribution.
+struct ShadowJob {
+    const float4* so;
+    const float4* sd;
+    float4* pend;
+    __device__ __forceinline__ void fetch(uint32_t i, float4& a, float4& b) const {
+        a = so[i];
+        b = sd[i];
+    }
+    __device__ __forceinline__ TraceRay decode(const float4& a, const float4& b) const {
+        return TraceRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 0.0f, a.w, __float_as_uint(b.w)};
+    }
+    __device__ __forceinline__ void any(uint32_t, const TraceRay& r, bool occluded) const {
+        if (occluded) pend[r.tag] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+};
+
 template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuffers pb, int cur, uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
-    const uint32_t lanes = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
-    const uint32_t n = pb.ctr[0];
-    const uint32_t* q = pb.q[cur];
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    for (uint32_t i = gid; i < n; i += lanes) {
-        const uint32_t pid = q[i];
-        const float4 o = pb.o[pid], d = pb.d[pid];
-        RayHit h;
-        trace_closest(sc.nodes, sc.tris, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, 1e30f, st, h, kCount ? &nv : nullptr,
-                      kCount ? &tv : nullptr);
-        pb.hits[i] = make_float4(h.t, __int_as_float(h.tri), h.u, h.v);
-    }
+    trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, pb.ctr[0], sc.nodes, sc.tris, st, nv, tv);
     if (kCount) {
         wave_count(&pb.stats[2], nv);
         wave_count(&pb.stats[3], tv);
@@ -188,17 +211,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuf
 template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
-    const uint32_t lanes = gridDim.x * blockDim.x;
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
-    TraceStack st{lds_stack + threadIdx.x, spill + gid, lanes, 0};
-    const uint32_t n = pb.ctr[2];
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    for (uint32_t i = gid; i < n; i += lanes) {
-        const float4 o = pb.sq_o[i], d = pb.sq_d[i];
-        if (trace_any(sc.nodes, sc.tris, mk(o.x, o.y, o.z), mk(d.x, d.y, d.z), 0.0f, o.w, st, kCount ? &nv : nullptr,
-                      kCount ? &tv : nullptr))
-            pb.pend[pb.sq_id[i]] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    }
+    trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, pb.ctr[2], sc.nodes, sc.tris, st, nv, tv);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
         wave_count(&pb.stats[5], tv);
@@ -215,27 +231,56 @@ __device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
 
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
     const uint32_t n = pb.ctr[0];
-    const uint32_t* q = pb.q[cur];
-    uint32_t* qn = pb.q[cur ^ 1];
+    const float4* ro = pb.ro[cur];
+    const float4* rd = pb.rd[cur];
+    float4* ro_next = pb.ro[cur ^ 1];
+    float4* rd_next = pb.rd[cur ^ 1];
     const EnergyTables& T = *sc.tables;
     const int S = lp.shadow_samples;
     const uint32_t stride = gridDim.x * blockDim.x;
-    // All lanes of a wave run the same number of iterations so the queue appends below see converged waves.
-    const uint32_t n_round = (n + stride - 1) / stride * stride;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
-        const bool active = i < n;
+    // Each block takes 256 queue slots per round and reorders them in LDS, hits first and misses after, so a
+    // wave runs either the long surface path or the short environment path, not both with half its lanes.
+    __shared__ uint32_t s_slot[kBlock];
+    __shared__ uint32_t s_count[2][kBlock / 64];
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
+    // The loop bound is uniform over the block, so every lane takes part in the barriers and queue appends.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+        const uint32_t mine = base + threadIdx.x;
+        const bool valid = mine < n;
+        const bool hit_here = valid && __float_as_int(pb.hits[mine].y) >= 0;
+        const uint64_t hm = __ballot(hit_here), mm = __ballot(valid && !hit_here);
+        if (__lane_id() == 0) {
+            s_count[0][wave] = static_cast<uint32_t>(__popcll(hm));
+            s_count[1][wave] = static_cast<uint32_t>(__popcll(mm));
+        }
+        __syncthreads();
+        uint32_t hits_before = 0, misses_before = 0, total_hits = 0, total = 0;
+        for (uint32_t k = 0; k < kBlock / 64; ++k) {
+            total_hits += s_count[0][k];
+            total += s_count[0][k] + s_count[1][k];
+            if (k < wave) {
+                hits_before += s_count[0][k];
+                misses_before += s_count[1][k];
+            }
+        }
+        if (valid) {
+            const uint32_t pos = hit_here ? hits_before + static_cast<uint32_t>(__popcll(hm & below))
+                                          : total_hits + misses_before + static_cast<uint32_t>(__popcll(mm & below));
+            s_slot[pos] = mine;
+        }
+        __syncthreads();
+        const bool active = threadIdx.x < total;
+        const uint32_t i = active ? s_slot[threadIdx.x] : 0u;
+        __syncthreads();  // s_slot and s_count are rewritten by the next round
         bool extend = false;
         bool alive = false;
         uint32_t pid = 0;
         V3 new_o, new_d;
-        // NEE shadow rays of this vertex, appended after the vertex is shaded.
-        bool want_shadow[kMaxShadowSamples] = {false, false, false, false};
-        V3 sh_o[kMaxShadowSamples], sh_d[kMaxShadowSamples];
-        float sh_t[kMaxShadowSamples];
         if (active) {
-            pid = q[i];
+            pid = __float_as_uint(ro[i].w);
             alive = true;
-            const float4 d4 = pb.d[pid];
+            const float4 d4 = rd[i];
             const V3 dir = mk(d4.x, d4.y, d4.z);
             const float4 t4 = pb.thr[pid];
             V3 thr = mk(t4.x, t4.y, t4.z);
@@ -305,41 +350,51 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                         const bool is_delta = (m.flags & kMatFlagDelta) != 0;
                         if (sc.light_count > 0 && !is_delta) {  // NEE, path_tracer.cpp:631-698
                             const Picker pk = picker_prepare(sc, si.point, si.normal);
+                            // Each NEE sample's shadow ray is appended as soon as it is built (the append is
+                            // wave-aggregated over the lanes that reach it), so no shadow record stays live
+                            // across the loop.
                             for (int s = 0; s < S; ++s) {
                                 float pmf = 0.0f;
                                 const float u = sobol_get1d(ss);
                                 const int li = picker_sample(sc, pk, u, si.point, si.normal, pmf);
                                 float ux, uy;
                                 sobol_get2d(ss, ux, uy);
-                                if (li < 0 || !(pmf > 0.0f)) continue;
-                                const bool from_env = li >= sc.num_lights;
-                                const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
-                                                                : light_sample(sc.lights[li], si.point, ux, uy);
-                                if (!(ls.pdf > 0.0f) || is_black(ls.li)) continue;
+                                bool ok = false;
                                 V3 so, sd;
-                                float st;
-                                if (!(ls.distance < kFloatMax) || ls.distance >= 1e29f) {
-                                    so = shadow_origin(si, ls.wi);
-                                    sd = ls.wi;
-                                    st = 1e30f;
-                                } else if (!spawn_ray_to(si, si.point + ls.wi * ls.distance, so, sd, st)) {
-                                    continue;
+                                float st = 0.0f;
+                                do {
+                                    if (li < 0 || !(pmf > 0.0f)) break;
+                                    const bool from_env = li >= sc.num_lights;
+                                    const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
+                                                                    : light_sample(sc.lights[li], si.point, ux, uy);
+                                    if (!(ls.pdf > 0.0f) || is_black(ls.li)) break;
+                                    if (!(ls.distance < kFloatMax) || ls.distance >= 1e29f) {
+                                        so = shadow_origin(si, ls.wi);
+                                        sd = ls.wi;
+                                        st = 1e30f;
+                                    } else if (!spawn_ray_to(si, si.point + ls.wi * ls.distance, so, sd, st)) {
+                                        break;
+                                    }
+                                    float bsdf_pdf = 0.0f;
+                                    const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, bsdf_pdf);
+                                    const float cos_t = fabsf(dot(ls.wi, si.normal));
+                                    if (is_black(f) || !(cos_t > 0.0f)) break;
+                                    const float light_pdf = pmf * ls.pdf;
+                                    const float weight = power_heuristic(S, light_pdf, 1, bsdf_pdf);
+                                    const V3 nee = f * ls.li * (cos_t * weight / (light_pdf * static_cast<float>(S)));
+                                    V3 c = thr * nee;
+                                    if (diffuse >= 1) c = clamp_indirect(c, lp.clamp_indirect);
+                                    if (!finite3(c)) break;
+                                    pb.pend[static_cast<size_t>(pid) * S + s] = make_float4(c.x, c.y, c.z, 0.0f);
+                                    ok = true;
+                                } while (false);
+                                const uint32_t ss_slot = wave_append(&pb.ctr[2], ok);
+                                if (ok) {
+                                    pb.sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
+                                    pb.sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
+                                                                   __uint_as_float(pid * static_cast<uint32_t>(S) +
+                                                                                   static_cast<uint32_t>(s)));
                                 }
-                                float bsdf_pdf = 0.0f;
-                                const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, bsdf_pdf);
-                                const float cos_t = fabsf(dot(ls.wi, si.normal));
-                                if (is_black(f) || !(cos_t > 0.0f)) continue;
-                                const float light_pdf = pmf * ls.pdf;
-                                const float weight = power_heuristic(S, light_pdf, 1, bsdf_pdf);
-                                const V3 nee = f * ls.li * (cos_t * weight / (light_pdf * static_cast<float>(S)));
-                                V3 c = thr * nee;
-                                if (diffuse >= 1) c = clamp_indirect(c, lp.clamp_indirect);
-                                if (!finite3(c)) continue;
-                                pb.pend[static_cast<size_t>(pid) * S + s] = make_float4(c.x, c.y, c.z, 0.0f);
-                                want_shadow[s] = true;
-                                sh_o[s] = so;
-                                sh_d[s] = sd;
-                                sh_t[s] = st;
                             }
                         }
                         // BSDF sample (path_tracer.cpp:736-743), throughput and roulette (:814-870).
@@ -383,22 +438,15 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             }
             pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
             if (extend) {
-                pb.o[pid] = make_float4(new_o.x, new_o.y, new_o.z, 0.0f);
-                pb.d[pid] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
                 pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
             }
             (void)alive;
         }
         const uint32_t slot = wave_append(&pb.ctr[1], extend);
-        if (extend) qn[slot] = pid;
-        for (int s = 0; s < S; ++s) {
-            const uint32_t ss_slot = wave_append(&pb.ctr[2], want_shadow[s]);
-            if (want_shadow[s]) {
-                pb.sq_id[ss_slot] = pid * static_cast<uint32_t>(S) + static_cast<uint32_t>(s);
-                pb.sq_o[ss_slot] = make_float4(sh_o[s].x, sh_o[s].y, sh_o[s].z, sh_t[s]);
-                pb.sq_d[ss_slot] = make_float4(sh_d[s].x, sh_d[s].y, sh_d[s].z, 0.0f);
-            }
+        if (extend) {
+            ro_next[slot] = make_float4(new_o.x, new_o.y, new_o.z, __uint_as_float(pid));
+            rd_next[slot] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
         }
     }
 }
@@ -538,18 +586,21 @@ struct Wavefront {
     Profiler prof;
     uint32_t capacity = 0;
     int shadow_samples = 0;
-    DevBuf o, d, thr, L, last_p, last_n, ids, pend, hits, q0, q1, sq_id, sq_o, sq_d, ctr, stats, filter;
+    DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, pend, hits, sq_o, sq_d, ctr, stats, filter;
     int filter_type = -1;
     float filter_width = 0.0f;
 
     ~Wavefront() { prof.drain(); }
 
+    // Device bytes of the wavefront buffers per path in flight.
+    static uint64_t bytes_per_path(int S) { return 4 * 16 + 6 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
+
     int ensure(uint32_t n, int S) {
         if (n <= capacity && S == shadow_samples) return PE_OK;
         const size_t N = n, NS = static_cast<size_t>(n) * S;
-        if (!o.alloc(N * 16) || !d.alloc(N * 16) || !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) ||
-            !last_n.alloc(N * 16) || !ids.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) ||
-            !q0.alloc(N * 4) || !q1.alloc(N * 4) || !sq_id.alloc(NS * 4) || !sq_o.alloc(NS * 16) ||
+        if (!ro0.alloc(N * 16) || !rd0.alloc(N * 16) || !ro1.alloc(N * 16) || !rd1.alloc(N * 16) ||
+            !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) || !last_n.alloc(N * 16) ||
+            !ids.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) || !sq_o.alloc(NS * 16) ||
             !sq_d.alloc(NS * 16) || !ctr.alloc(64) || !stats.alloc(64)) {
             capacity = 0;
             return PE_ERR_OOM;
@@ -561,8 +612,10 @@ struct Wavefront {
 
     PathBuffers view() {
         PathBuffers pb;
-        pb.o = o.as<float4>();
-        pb.d = d.as<float4>();
+        pb.ro[0] = ro0.as<float4>();
+        pb.rd[0] = rd0.as<float4>();
+        pb.ro[1] = ro1.as<float4>();
+        pb.rd[1] = rd1.as<float4>();
         pb.thr = thr.as<float4>();
         pb.L = L.as<float4>();
         pb.last_p = last_p.as<float4>();
@@ -570,9 +623,6 @@ struct Wavefront {
         pb.ids = ids.as<uint4>();
         pb.pend = pend.as<float4>();
         pb.hits = hits.as<float4>();
-        pb.q[0] = q0.as<uint32_t>();
-        pb.q[1] = q1.as<uint32_t>();
-        pb.sq_id = sq_id.as<uint32_t>();
         pb.sq_o = sq_o.as<float4>();
         pb.sq_d = sq_d.as<float4>();
         pb.ctr = ctr.as<uint32_t>();
@@ -638,7 +688,17 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
         Wavefront* wf = wavefront_of(dev);
         const int S = std::clamp(settings->shadow_quality, 1, kMaxShadowSamples);  // PathTracer ctor (path_tracer.h:43)
         const uint32_t pixels = static_cast<uint32_t>(film->width) * static_cast<uint32_t>(film->height);
-        const int spl_req = std::max(1, settings->samples_per_launch);
+        // Samples per launch: as asked, or (0) as many as keep about 2^26 paths in flight within a quarter of
+        // the free device memory. Deep launches amortise the short tail bounces, whose time is set by the
+        // longest traversal rather than by the number of rays.
+        int spl_req = settings->samples_per_launch;
+        if (spl_req <= 0) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = size_t{1} << 32;
+            const uint64_t per_path = Wavefront::bytes_per_path(S);
+            const uint64_t target = std::min<uint64_t>(uint64_t{1} << 26, free_b / 4 / per_path);
+            spl_req = static_cast<int>(std::clamp<uint64_t>(target / std::max<uint32_t>(1, pixels), 1, 1024));
+        }
         const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(uint64_t{1} << 31, uint64_t{pixels} * spl_req));
         const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);

@@ -69,6 +69,11 @@ struct EnvView {
     const float* func;       // w * h (sampling distribution of the mis-compensated image)
     const float* marginal;   // h + 1
     const float* cond;       // h * (w + 1)
+    // Guide tables of the two CDFs: entry k of a table with K buckets is the first CDF index whose value
+    // exceeds k / K (K a power of two), so the search for u only spans entries floor(u K) .. floor(u K) + 1.
+    const uint32_t* marginal_guide;  // marginal_k + 1
+    const uint32_t* cond_guide;      // h * (cond_k + 1)
+    int32_t marginal_k, cond_k;
     int32_t w, h;
     float func_int, rotation, intensity, mean_luminance;
     int32_t valid;           // distribution valid

@@ -240,9 +240,15 @@ __device__ __forceinline__ float env_pdf(const EnvView& e, V3 wi) {  // environm
     return pdf;
 }
 
-// sampleCdf (environment_light.cpp:22-36): first entry of cdf[0..n] greater than u, by binary search.
-__device__ __forceinline__ void sample_cdf(const float* cdf, int n, float u, int& index, float& pdf, float& coord) {
-    int lo = 0, count = n + 1;
+// sampleCdf (environment_light.cpp:22-36): the first entry of cdf[0..n] greater than u. The guide table
+// brackets that entry between guide[k] and guide[k + 1] for k = floor(u K); the binary search over the
+// bracket returns the same index as a search over the whole CDF, in a few dependent loads instead of log2(n).
+__device__ __forceinline__ void sample_cdf(const float* cdf, int n, const uint32_t* guide, int K, float u, int& index,
+                                           float& pdf, float& coord) {
+    int k = static_cast<int>(u * static_cast<float>(K));
+    k = k < 0 ? 0 : (k > K - 1 ? K - 1 : k);
+    int lo = static_cast<int>(guide[k]);
+    int count = static_cast<int>(guide[k + 1]) - lo;
     while (count > 0) {
         const int step = count >> 1;
         const int it = lo + step;
@@ -294,8 +300,9 @@ __device__ __forceinline__ LightSample env_sample(const EnvView& e, float sx, fl
     } else {
         int ri, ci;
         float rp, rc, cp, cc;
-        sample_cdf(e.marginal, e.h, sy, ri, rp, rc);
-        sample_cdf(e.cond + static_cast<size_t>(ri) * static_cast<size_t>(e.w + 1), e.w, sx, ci, cp, cc);
+        sample_cdf(e.marginal, e.h, e.marginal_guide, e.marginal_k, sy, ri, rp, rc);
+        sample_cdf(e.cond + static_cast<size_t>(ri) * static_cast<size_t>(e.w + 1), e.w,
+                   e.cond_guide + static_cast<size_t>(ri) * static_cast<size_t>(e.cond_k + 1), e.cond_k, sx, ci, cp, cc);
         const float theta = rc * kPi;
         const float sin_theta = fmaxf(sinf(theta), 1e-6f);
         const float st = sinf(rc * kPi);

@@ -1,17 +1,32 @@
-// pe_traverse.h — closest-hit and any-hit BVH traversal for one ray per lane on gfx950.
+// pe_traverse.h — closest-hit and any-hit BVH traversal of a ray queue on gfx950.
 //
 // Per-triangle arithmetic is the reference's lean Möller–Trumbore test (geometry/bvh.cpp:302-316, same
 // operations as Triangle::intersect, triangle.cpp:11-50) with contraction disabled, so a triangle accepts
 // or rejects a ray, and reports t, bit for bit as the reference's IEEE build does. Accepting t <= tMax with
 // a shrinking tMax (bvh.cpp:315, :343) means that of two triangles at exactly the same t the one tested
-// last wins; that order depends on the tree, and such rays are the documented "genuine ties".
+// last wins; that order depends on the tree and on the traversal order, and such rays are the documented
+// "genuine ties". Any-hit queries return the same boolean whatever the order.
 //
 // Box test: slab test with the reference's exit pad t1 * 1.0000004 (bvh.cpp:269-283). NaN slabs (0 * inf,
 // a ray in a slab plane) are dropped by fminf/fmaxf instead of the reference's argument-order rule.
 //
-// Traversal stack: kLdsDepth entries per lane live in LDS, column-major (entry e of lane l at e * blockDim + l,
-// conflict-free), deeper entries spill to a per-lane column of a global buffer. The BVH build caps the
-// interior depth, which bounds the stack at kStackDepth.
+// Scheduling. A traversal is a sequence of items, each one 64-byte gather: an interior node (the boxes of
+// both children) or one triangle of a leaf (48 B, read as 64 B: the triangle array carries 16 B of tail
+// padding). Every lane of a wave handles one item per loop step, so a step costs one memory latency
+// whatever mix of nodes and triangles the lanes hold — traversal here is bound by that latency, not by HBM
+// bandwidth. A lane whose ray has ended takes the next ray of its wave's share at the top of the next step
+// (a ballot ranks the idle lanes), and the new ray's record is loaded together with the first node, so lanes
+// stay busy until the wave's share is used up. The share of wave w is every W-th 64-ray chunk of the queue
+// from chunk w (W = waves in the grid): cheap and expensive regions of the queue are spread over all waves
+// without an atomic counter.
+//
+// Item encoding (stack entries are items too): an interior node is its index; a leaf run is
+// kLeafItem | (remaining triangles - 1) << 27 | first triangle, so a leaf holds at most 16 triangles and
+// a scene at most 2^27 triangles (both enforced at upload).
+//
+// Traversal stack: kLdsDepth entries per lane live in LDS, column-major (entry e of lane l at
+// e * kTraceBlock + l, conflict-free), deeper entries spill to a per-lane column of a global buffer. One entry
+// per tree level at most, so the tree depth bound of the build bounds the stack at kStackDepth.
 #pragma once
 
 #include "pe_gpu_types.h"
@@ -20,44 +35,51 @@
 namespace pe {
 
 constexpr int kLdsDepth = 16;
-constexpr int kStackDepth = 64;
+constexpr int kStackDepth = 96;
 constexpr int kTraceBlock = 256;
+constexpr uint32_t kLeafItem = 0x80000000u;
+constexpr uint32_t kLeafFirstMask = (1u << 27) - 1u;
+constexpr uint32_t kMaxLeafItemTris = 16;
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+constexpr uint32_t kNoRay = 0xFFFFFFFFu;
+constexpr uint64_t kMaxTraceTriangles = uint64_t{1} << 27;
 
 struct TraceStack {
-    uint32_t* lds;      // kLdsDepth * blockDim entries, this lane's column at lds[threadIdx.x]
-    uint32_t* spill;    // this lane's column in the spill buffer
+    uint32_t* lds;      // this lane's column: entry e at lds[e * kTraceBlock]
+    uint32_t* spill;    // this lane's column in the spill buffer: entry e at spill[(e - kLdsDepth) * spill_stride]
     uint32_t spill_stride;
     int sp;
 
-    // The BVH build keeps interior depth below kStackDepth, so a push never finds the stack full; were it
-    // to, the entry is dropped rather than written out of bounds.
-    __device__ void push(uint32_t v) {
+    // The build bounds the tree depth below kStackDepth, so a push never finds the stack full; were it to,
+    // the entry is dropped rather than written out of bounds.
+    __device__ __forceinline__ void push(uint32_t v) {
         if (sp < kLdsDepth) lds[sp * kTraceBlock] = v;
         else if (sp < kStackDepth) spill[static_cast<size_t>(sp - kLdsDepth) * spill_stride] = v;
         else return;
         ++sp;
     }
-    __device__ uint32_t pop() {
+    __device__ __forceinline__ uint32_t pop() {
         --sp;
         if (sp < kLdsDepth) return lds[sp * kTraceBlock];
         return spill[static_cast<size_t>(sp - kLdsDepth) * spill_stride];
     }
 };
 
-struct RayHit {
-    float t;
-    int32_t tri;
-    float u, v;
+// One ray of a queue, decoded by the queue's job.
+struct TraceRay {
+    V3 o, d;
+    float tmin, tmax;
+    uint32_t tag;  // job-defined (e.g. where an occluded shadow ray's contribution lives)
 };
 
-__device__ __forceinline__ void box_test2(const BvhNode& n, V3 o, V3 inv, float tmin, float tmax, bool& h0, bool& h1,
-                                          float& t0, float& t1) {
-    const float ax0 = (n.c0xy.x - o.x) * inv.x, ax1 = (n.c0xy.y - o.x) * inv.x;
-    const float ay0 = (n.c0xy.z - o.y) * inv.y, ay1 = (n.c0xy.w - o.y) * inv.y;
-    const float az0 = (n.cz.x - o.z) * inv.z, az1 = (n.cz.y - o.z) * inv.z;
-    const float bx0 = (n.c1xy.x - o.x) * inv.x, bx1 = (n.c1xy.y - o.x) * inv.x;
-    const float by0 = (n.c1xy.z - o.y) * inv.y, by1 = (n.c1xy.w - o.y) * inv.y;
-    const float bz0 = (n.cz.z - o.z) * inv.z, bz1 = (n.cz.w - o.z) * inv.z;
+__device__ __forceinline__ void box_test2(const float4& c0xy, const float4& c1xy, const float4& cz, V3 o, V3 inv,
+                                          float tmin, float tmax, bool& h0, bool& h1, float& t0, float& t1) {
+    const float ax0 = (c0xy.x - o.x) * inv.x, ax1 = (c0xy.y - o.x) * inv.x;
+    const float ay0 = (c0xy.z - o.y) * inv.y, ay1 = (c0xy.w - o.y) * inv.y;
+    const float az0 = (cz.x - o.z) * inv.z, az1 = (cz.y - o.z) * inv.z;
+    const float bx0 = (c1xy.x - o.x) * inv.x, bx1 = (c1xy.y - o.x) * inv.x;
+    const float by0 = (c1xy.z - o.y) * inv.y, by1 = (c1xy.w - o.y) * inv.y;
+    const float bz0 = (cz.z - o.z) * inv.z, bz1 = (cz.w - o.z) * inv.z;
     const float an = fmaxf(fmaxf(tmin, fminf(ax0, ax1)), fmaxf(fminf(ay0, ay1), fminf(az0, az1)));
     const float af = fminf(fminf(tmax, fmaxf(ax0, ax1)), fminf(fmaxf(ay0, ay1), fmaxf(az0, az1))) * 1.0000004f;
     const float bn = fmaxf(fmaxf(tmin, fminf(bx0, bx1)), fmaxf(fminf(by0, by1), fminf(bz0, bz1)));
@@ -68,18 +90,19 @@ __device__ __forceinline__ void box_test2(const BvhNode& n, V3 o, V3 inv, float 
     t1 = bn;
 }
 
-// Lean Möller–Trumbore (bvh.cpp:302-316). Returns true and t/u/v when tmin <= t <= tmax. Every dot and cross
-// product is evaluated uncontracted (dot_exact / cross_exact): hipcc would otherwise fuse them into FMAs.
-__device__ __forceinline__ bool tri_test(const TriAccel& T, V3 o, V3 d, float tmin, float tmax, float& t, float& u,
-                                         float& v) {
+// Lean Möller–Trumbore (bvh.cpp:302-316) on (v0 | eps, e1, e2). Returns true and t/u/v when
+// tmin <= t <= tmax. Every dot and cross product is evaluated uncontracted (dot_exact / cross_exact): hipcc
+// would otherwise fuse them into FMAs.
+__device__ __forceinline__ bool tri_test(const float4& v0, const float4& e1v, const float4& e2v, V3 o, V3 d,
+                                         float tmin, float tmax, float& t, float& u, float& v) {
 #pragma clang fp contract(off)
-    const V3 e1 = mk(T.e1.x, T.e1.y, T.e1.z);
-    const V3 e2 = mk(T.e2.x, T.e2.y, T.e2.z);
+    const V3 e1 = mk(e1v.x, e1v.y, e1v.z);
+    const V3 e2 = mk(e2v.x, e2v.y, e2v.z);
     const V3 h = cross_exact(d, e2);
     const float a = dot_exact(e1, h);
-    if (a * a <= T.v0.w) return false;
+    if (a * a <= v0.w) return false;
     const float f = 1.0f / a;
-    const V3 s = o - mk(T.v0.x, T.v0.y, T.v0.z);
+    const V3 s = o - mk(v0.x, v0.y, v0.z);
     u = f * dot_exact(s, h);
     if (!(u >= 0.0f && u <= 1.0f)) return false;
     const V3 q = cross_exact(s, e1);
@@ -89,147 +112,113 @@ __device__ __forceinline__ bool tri_test(const TriAccel& T, V3 o, V3 d, float tm
     return t >= tmin && t <= tmax;
 }
 
-__device__ __forceinline__ BvhNode load_node(const BvhNode* __restrict__ nodes, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(nodes + i);
-    BvhNode n;
-    n.c0xy = p[0];
-    n.c1xy = p[1];
-    n.cz = p[2];
-    n.ref = reinterpret_cast<const int4*>(p)[3];
-    return n;
+// The item a child reference of a node stands for.
+__device__ __forceinline__ uint32_t child_item(int32_t ref, int32_t count) {
+    if (count <= 0) return static_cast<uint32_t>(ref);
+    return kLeafItem | (static_cast<uint32_t>(count - 1) << 27) | static_cast<uint32_t>(ref);
 }
 
-__device__ __forceinline__ TriAccel load_tri(const TriAccel* __restrict__ tris, uint32_t i) {
-    const float4* p = reinterpret_cast<const float4*>(tris + i);
-    TriAccel t;
-    t.v0 = p[0];
-    t.e1 = p[1];
-    t.e2 = p[2];
-    return t;
-}
-
-// Closest hit. hit.tri = -1 on a miss. Visit counters (nodes, triangles) are accumulated when non-null.
-__device__ bool trace_closest(const BvhNode* __restrict__ nodes, const TriAccel* __restrict__ tris, V3 o, V3 d,
-                              float tmin, float tmax, TraceStack& st, RayHit& hit, uint32_t* node_visits,
-                              uint32_t* tri_visits) {
-    const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    hit.tri = -1;
-    hit.t = tmax;
-    hit.u = hit.v = 0.0f;
-    st.sp = 0;
-    uint32_t node = 0;
-    uint32_t nv = 0, tv = 0;
-    while (true) {
-        const BvhNode n = load_node(nodes, node);
-        ++nv;
-        bool h0, h1;
-        float t0, t1;
-        box_test2(n, o, inv, tmin, hit.t, h0, h1, t0, t1);
-        if (h0 && n.ref.z > 0) {
-            for (int k = 0; k < n.ref.z; ++k) {
-                const TriAccel T = load_tri(tris, static_cast<uint32_t>(n.ref.x + k));
-                ++tv;
-                float t, u, v;
-                if (tri_test(T, o, d, tmin, hit.t, t, u, v)) {
-                    hit.t = t;
-                    hit.u = u;
-                    hit.v = v;
-                    hit.tri = __float_as_int(T.e1.w);
+// Traverses every ray of a queue of n rays with the calling grid (all lanes of all waves must call it).
+//   job.fetch(i, a, b)            issue the loads of ray i's record (two float4)
+//   job.decode(a, b) -> TraceRay  decode a fetched record
+//   closest: job.closest(i, ray, t, tri, u, v)   t = tmax and tri = -1 on a miss
+//   any:     job.any(i, ray, occluded)
+// nv / tv accumulate the node and triangle items this lane handled (kCount only).
+template <bool kAny, bool kCount, class Job>
+__device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const BvhNode* __restrict__ nodes,
+                                            const TriAccel* __restrict__ tris, TraceStack& st, uint32_t& nv,
+                                            uint32_t& tv) {
+    const uint32_t wpb = blockDim.x >> 6;
+    const uint32_t W = gridDim.x * wpb;
+    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
+    const uint32_t nchunks = (n + 63u) >> 6;
+    const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
+    uint32_t taken = 0;  // rays of this wave's share handed out so far (wave-uniform)
+    uint32_t slot = kNoRay;
+    uint32_t item = 0;
+    bool fresh = false;
+    float4 ra = make_float4(0.0f, 0.0f, 0.0f, 0.0f), rb = ra;
+    TraceRay ray{};
+    V3 inv = mk(0.0f, 0.0f, 0.0f);
+    float ht = 0.0f, hu = 0.0f, hv = 0.0f;
+    int32_t htri = -1;
+    for (;;) {
+        // Idle lanes take the next rays of the wave's share, in lane order.
+        const uint64_t idle = __ballot(slot == kNoRay);
+        if (idle != 0 && w + (taken >> 6) * W < nchunks) {
+            if (slot == kNoRay) {
+                const uint32_t j = taken + static_cast<uint32_t>(__popcll(idle & below));
+                const uint32_t c = w + (j >> 6) * W;
+                const uint32_t idx = (c << 6) | (j & 63u);
+                if (c < nchunks && idx < n) {
+                    slot = idx;
+                    job.fetch(idx, ra, rb);
+                    fresh = true;
+                    item = 0;
+                    st.sp = 0;
                 }
             }
-            h0 = false;
+            taken += static_cast<uint32_t>(__popcll(idle));
         }
-        if (h1 && n.ref.w > 0) {
-            for (int k = 0; k < n.ref.w; ++k) {
-                const TriAccel T = load_tri(tris, static_cast<uint32_t>(n.ref.y + k));
-                ++tv;
-                float t, u, v;
-                if (tri_test(T, o, d, tmin, hit.t, t, u, v)) {
-                    hit.t = t;
-                    hit.u = u;
-                    hit.v = v;
-                    hit.tri = __float_as_int(T.e1.w);
+        if (__ballot(slot != kNoRay) == 0) break;
+        if (slot == kNoRay) continue;
+
+        // One item: a 64-byte gather from the node or the triangle array.
+        const bool leaf = (item & kLeafItem) != 0u;
+        const float4* p = leaf ? reinterpret_cast<const float4*>(tris + (item & kLeafFirstMask))
+                               : reinterpret_cast<const float4*>(nodes + item);
+        const float4 x0 = p[0], x1 = p[1], x2 = p[2], x3 = p[3];
+        if (fresh) {
+            ray = job.decode(ra, rb);
+            inv = mk(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
+            ht = ray.tmax;
+            hu = hv = 0.0f;
+            htri = -1;
+            fresh = false;
+        }
+        uint32_t next = kNoItem;
+        bool occluded = false;
+        if (leaf) {
+            if (kCount) ++tv;
+            float t, u, v;
+            if (tri_test(x0, x1, x2, ray.o, ray.d, ray.tmin, kAny ? ray.tmax : ht, t, u, v)) {
+                if (kAny) {
+                    occluded = true;
+                } else {
+                    ht = t;
+                    hu = u;
+                    hv = v;
+                    htri = __float_as_int(x1.w);
                 }
             }
-            h1 = false;
-        }
-        h0 = h0 && t0 <= hit.t;
-        h1 = h1 && t1 <= hit.t;
-        if (h0 && h1) {
-            const bool swap = t1 < t0;
-            st.push(static_cast<uint32_t>(swap ? n.ref.x : n.ref.y));
-            node = static_cast<uint32_t>(swap ? n.ref.y : n.ref.x);
-            continue;
-        }
-        if (h0) {
-            node = static_cast<uint32_t>(n.ref.x);
-            continue;
-        }
-        if (h1) {
-            node = static_cast<uint32_t>(n.ref.y);
-            continue;
-        }
-        if (st.sp == 0) break;
-        node = st.pop();
-    }
-    if (node_visits) *node_visits += nv;
-    if (tri_visits) *tri_visits += tv;
-    return hit.tri >= 0;
-}
-
-// Any hit within [tmin, tmax] (Scene::intersectAny, bvh.cpp:404-446): returns at the first accepted triangle.
-__device__ bool trace_any(const BvhNode* __restrict__ nodes, const TriAccel* __restrict__ tris, V3 o, V3 d, float tmin,
-                          float tmax, TraceStack& st, uint32_t* node_visits, uint32_t* tri_visits) {
-    const V3 inv = mk(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    st.sp = 0;
-    uint32_t node = 0;
-    uint32_t nv = 0, tv = 0;
-    bool found = false;
-    while (true) {
-        const BvhNode n = load_node(nodes, node);
-        ++nv;
-        bool h0, h1;
-        float t0, t1;
-        box_test2(n, o, inv, tmin, tmax, h0, h1, t0, t1);
-        if (h0 && n.ref.z > 0) {
-            for (int k = 0; k < n.ref.z && !found; ++k) {
-                const TriAccel T = load_tri(tris, static_cast<uint32_t>(n.ref.x + k));
-                ++tv;
-                float t, u, v;
-                found = tri_test(T, o, d, tmin, tmax, t, u, v);
+            const uint32_t left = (item >> 27) & 15u;
+            if (left > 0u) next = kLeafItem | ((left - 1u) << 27) | ((item & kLeafFirstMask) + 1u);
+        } else {
+            if (kCount) ++nv;
+            bool h0, h1;
+            float t0, t1;
+            box_test2(x0, x1, x2, ray.o, inv, ray.tmin, kAny ? ray.tmax : ht, h0, h1, t0, t1);
+            const uint32_t c0 = child_item(__float_as_int(x3.x), __float_as_int(x3.z));
+            const uint32_t c1 = child_item(__float_as_int(x3.y), __float_as_int(x3.w));
+            if (h0 && h1) {
+                const bool swap = t1 < t0;
+                st.push(swap ? c0 : c1);
+                next = swap ? c1 : c0;
+            } else if (h0) {
+                next = c0;
+            } else if (h1) {
+                next = c1;
             }
-            h0 = false;
         }
-        if (!found && h1 && n.ref.w > 0) {
-            for (int k = 0; k < n.ref.w && !found; ++k) {
-                const TriAccel T = load_tri(tris, static_cast<uint32_t>(n.ref.y + k));
-                ++tv;
-                float t, u, v;
-                found = tri_test(T, o, d, tmin, tmax, t, u, v);
-            }
-            h1 = false;
+        if (next == kNoItem && !occluded && st.sp > 0) next = st.pop();
+        if (occluded || next == kNoItem) {
+            if constexpr (kAny) job.any(slot, ray, occluded);
+            else job.closest(slot, ray, ht, htri, hu, hv);
+            slot = kNoRay;
+        } else {
+            item = next;
         }
-        if (found) break;
-        if (h0 && h1) {
-            const bool swap = t1 < t0;
-            st.push(static_cast<uint32_t>(swap ? n.ref.x : n.ref.y));
-            node = static_cast<uint32_t>(swap ? n.ref.y : n.ref.x);
-            continue;
-        }
-        if (h0) {
-            node = static_cast<uint32_t>(n.ref.x);
-            continue;
-        }
-        if (h1) {
-            node = static_cast<uint32_t>(n.ref.y);
-            continue;
-        }
-        if (st.sp == 0) break;
-        node = st.pop();
     }
-    if (node_visits) *node_visits += nv;
-    if (tri_visits) *tri_visits += tv;
-    return found;
 }
 
 }  // namespace pe

@@ -189,6 +189,31 @@ void build_cdf(Distribution& d, int w, int h) {
     }
 }
 
+int guide_buckets(int n, int cap) {
+    int k = 1;
+    while (k < n && k < cap) k <<= 1;
+    return k;
+}
+
+// Guide table of cdf[0..n] with K buckets (see EnvView): g[k] = first index whose value exceeds k / K, or n + 1.
+// A CDF that is not non-decreasing (NaN entries) gets the whole range for every bucket, which is the plain
+// binary search.
+void build_guide(const float* cdf, int n, int K, uint32_t* g) {
+    bool monotone = true;
+    for (int i = 0; i < n && monotone; ++i) monotone = cdf[i] <= cdf[i + 1];
+    if (!monotone) {
+        for (int k = 0; k <= K; ++k) g[k] = k == K ? static_cast<uint32_t>(n + 1) : 0u;
+        return;
+    }
+    int idx = 0;
+    for (int k = 0; k <= K; ++k) {
+        const float t = static_cast<float>(k) / static_cast<float>(K);
+        while (idx <= n && !(t < cdf[idx])) ++idx;
+        g[k] = static_cast<uint32_t>(idx);
+    }
+    g[0] = 0u;
+}
+
 }  // namespace
 
 // EnvironmentLight(image, rotation, intensity) + setSun (environment_light.cpp:64-182); the sampling
@@ -284,10 +309,21 @@ int prepare_environment(pe_device* dev, const pe_env_desc& e, pe_scene& sc, EnvV
     hipStream_t s = dev->stream;
     if (has_image) {
         if (!sc.env_rgb.upload(e.rgb, 3ull * w * h * sizeof(float), s)) return PE_ERR_OOM;
+        std::vector<uint32_t> mguide, cguide;
         if (mis.valid()) {
+            ev.marginal_k = guide_buckets(h, 4096);
+            ev.cond_k = guide_buckets(w / 2, 1024);
+            mguide.resize(static_cast<size_t>(ev.marginal_k) + 1);
+            cguide.resize(static_cast<size_t>(h) * static_cast<size_t>(ev.cond_k + 1));
+            build_guide(mis.marginal.data(), h, ev.marginal_k, mguide.data());
+            for (int y = 0; y < h; ++y)
+                build_guide(mis.cond.data() + static_cast<size_t>(y) * static_cast<size_t>(w + 1), w, ev.cond_k,
+                            cguide.data() + static_cast<size_t>(y) * static_cast<size_t>(ev.cond_k + 1));
             if (!sc.env_func.upload(mis.func.data(), mis.func.size() * sizeof(float), s) ||
                 !sc.env_marginal.upload(mis.marginal.data(), mis.marginal.size() * sizeof(float), s) ||
-                !sc.env_cond.upload(mis.cond.data(), mis.cond.size() * sizeof(float), s))
+                !sc.env_cond.upload(mis.cond.data(), mis.cond.size() * sizeof(float), s) ||
+                !sc.env_mguide.upload(mguide.data(), mguide.size() * sizeof(uint32_t), s) ||
+                !sc.env_cguide.upload(cguide.data(), cguide.size() * sizeof(uint32_t), s))
                 return PE_ERR_OOM;
         }
         const hipError_t er = hipStreamSynchronize(s);
@@ -300,6 +336,8 @@ int prepare_environment(pe_device* dev, const pe_env_desc& e, pe_scene& sc, EnvV
     ev.func = sc.env_func.as<float>();
     ev.marginal = sc.env_marginal.as<float>();
     ev.cond = sc.env_cond.as<float>();
+    ev.marginal_guide = sc.env_mguide.as<uint32_t>();
+    ev.cond_guide = sc.env_cguide.as<uint32_t>();
     return PE_OK;
 }
 

@@ -5,6 +5,7 @@
 // must be the same floats the reference computes (geometry/bvh.cpp:45-50, triangle.cpp:12-22).
 #include "pe_scene_host.h"
 #include "bvh_build.h"
+#include "pe_traverse.h"
 
 #include <chrono>
 #include <cstring>
@@ -70,9 +71,9 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         set_error("scene: no triangles");
         return PE_ERR_INVALID;
     }
-    if (nt >= (1ull << 31)) {
-        set_error("scene: more than 2^31 triangles");
-        return PE_ERR_INVALID;
+    if (nt > kMaxTraceTriangles) {
+        set_error("scene: more than 2^27 triangles (the traversal's leaf items address 2^27)");
+        return PE_ERR_UNSUPPORTED;
     }
 
     std::vector<float4> pos(nv), nrm(nv);
@@ -115,7 +116,9 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     BvhBuildOptions opt;
     const BvhBuildResult bvh = build_bvh(lo.data(), hi.data(), nt, opt);
 
-    std::vector<TriAccel> acc(nt);
+    // One zero record of tail padding: traversal reads every triangle as a 64-byte item.
+    std::vector<TriAccel> acc(nt + 1);
+    acc[nt] = TriAccel{};
     for (uint64_t i = 0; i < nt; ++i) {
         const uint32_t t = bvh.order[i];
         const TriShade& r = shade[t];

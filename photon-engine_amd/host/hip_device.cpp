@@ -1,0 +1,428 @@
+// hip_device.cpp — HipDevice: the reference's RenderDevice interface on top of libphoton_hip.so.
+//
+// setScene packs the immutable Scene into one pe_scene_desc (engine/scene.h:31-165 walked through its public
+// getters) and uploads it once; render() maps the camera (Camera::describeGpu, camera/camera.h:17-31) and the
+// settings onto pe_camera / pe_settings, renders the sample range into a device film, and merges the film's
+// per-pixel sums into the caller's Film with Film::addBatch (core/image/film.cpp:119-137), as the Vulkan
+// device merges its batches.
+//
+// What the HIP path does not implement is refused, never approximated: setScene leaves the device without a
+// scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle, lights other
+// than AreaLight / TriangleLight, textured or flaked Disney materials and materials other than Disney /
+// Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO,
+// motion blur and path guiding.
+#include "hip_device.h"
+
+#include "camera/camera.h"
+#include "core/image/film.h"
+#include "core/image/image.h"
+#include "engine/scene.h"
+#include "geometry/mesh.h"
+#include "geometry/triangle.h"
+#include "lights/area_light.h"
+#include "lights/environment_light.h"
+#include "materials/dielectric.h"
+#include "materials/disney.h"
+#include "materials/lambertian.h"
+
+#include <algorithm>
+#include <unordered_map>
+
+namespace photon {
+
+namespace {
+
+void put3(float* d, const Vec3f& v) {
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = v.z;
+}
+
+void putc(float* d, const Color3f& c) {
+    d[0] = c.r;
+    d[1] = c.g;
+    d[2] = c.b;
+}
+
+std::string lastHipError() {
+    const char* e = pe_last_error();
+    return e ? std::string(e) : std::string();
+}
+
+// The flattened scene and the host arrays its pointers refer to.
+struct PackedScene {
+    std::vector<pe_mesh_desc> meshes;
+    std::vector<pe_material_desc> materials;
+    std::vector<pe_light_desc> lights;
+    std::vector<std::vector<float>> floats;
+    std::vector<std::vector<uint32_t>> indices;
+    pe_env_desc env{};
+    bool hasEnv = false;
+
+    const float* keep(std::vector<float> v) {
+        floats.push_back(std::move(v));
+        return floats.back().data();
+    }
+    pe_scene_desc desc() const {
+        pe_scene_desc d{};
+        d.meshes = meshes.data();
+        d.num_meshes = static_cast<int32_t>(meshes.size());
+        d.materials = materials.data();
+        d.num_materials = static_cast<int32_t>(materials.size());
+        d.lights = lights.data();
+        d.num_lights = static_cast<int32_t>(lights.size());
+        d.environment = hasEnv ? &env : nullptr;
+        return d;
+    }
+};
+
+bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
+    o = pe_material_desc{};
+    SurfaceInteraction si;
+    putc(o.emission, m->emitted(si));
+    if (auto* d = dynamic_cast<const DisneyMaterial*>(m)) {
+        if (d->flakeDensity() > 0.0f || d->albedoImage() || d->normalImage() || d->roughnessImage() ||
+            d->metalnessImage() || d->opacityImage()) {
+            err = "DisneyMaterial with texture maps or flakes is not implemented on the HIP device";
+            return false;
+        }
+        o.type = PE_MAT_DISNEY;
+        putc(o.base_color, d->baseColor());
+        o.metallic = d->metallic();
+        o.roughness = d->roughness();
+        o.specular = d->specular();
+        o.clearcoat = d->clearCoat();
+        o.clearcoat_roughness = d->clearCoatRoughness();
+        o.coat_ior = d->coatIor();
+        putc(o.coat_tint, d->coatTint());
+        o.ior = d->ior();
+        o.specular_transmission = d->specularTransmission();
+        o.anisotropy = d->anisotropy();
+        o.sheen = d->sheen();
+        o.sheen_roughness = d->sheenRoughness();
+        putc(o.sheen_color, d->sheenColor());
+        o.has_sheen_color = d->hasCustomSheenColor() ? 1 : 0;
+        o.diffuse_roughness = d->diffuseRoughness();
+        o.diffuse_transmission = d->diffuseTransmission();
+        o.subsurface = d->subsurface();
+        putc(o.subsurface_radius, d->subsurfaceRadius());
+        o.subsurface_scale = d->subsurfaceScale();
+        o.subsurface_anisotropy = d->subsurfaceAnisotropy();
+        putc(o.edge_tint, d->edgeTint());
+        o.thin_film_thickness = d->thinFilmThickness();
+        o.thin_film_ior = d->thinFilmIor();
+        return true;
+    }
+    if (auto* l = dynamic_cast<const Lambertian*>(m)) {
+        o.type = PE_MAT_LAMBERT;
+        putc(o.base_color, l->albedo());
+        return true;
+    }
+    if (auto* g = dynamic_cast<const Dielectric*>(m)) {
+        o.type = PE_MAT_DIELECTRIC;
+        putc(o.base_color, g->tint());
+        o.ior = g->ior();
+        o.roughness = g->roughness();
+        o.abbe = g->abbe();
+        putc(o.absorption_color, g->absorptionColor());
+        o.absorption_distance = g->absorptionDistance();
+        o.thin_walled = g->thinWalled() ? 1 : 0;
+        o.thin_film_thickness = g->thinFilmThickness();
+        return true;
+    }
+    err = "material type not implemented on the HIP device";
+    return false;
+}
+
+bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
+    std::unordered_map<const Material*, int> matIndex;
+    auto material = [&](const Material* m) -> int {
+        if (!m) return -1;
+        auto it = matIndex.find(m);
+        if (it != matIndex.end()) return it->second;
+        pe_material_desc d;
+        if (!packMaterial(m, d, err)) return -1;
+        p.materials.push_back(d);
+        const int k = static_cast<int>(p.materials.size()) - 1;
+        matIndex.emplace(m, k);
+        return k;
+    };
+
+    // Lights in Scene::lights() order: the light index is part of the sampling sequence.
+    for (const Light* L : scene.lights()) {
+        pe_light_desc d{};
+        if (auto* a = dynamic_cast<const AreaLight*>(L)) {
+            d.type = PE_LIGHT_AREA;
+            put3(d.p0, a->position());
+            put3(d.p1, a->u());
+            put3(d.p2, a->v());
+            putc(d.radiance, a->radiance());
+            d.visible_to_camera = a->visibleToCamera() ? 1 : 0;
+        } else if (auto* t = dynamic_cast<const TriangleLight*>(L)) {
+            d.type = PE_LIGHT_TRIANGLE;
+            put3(d.p0, t->p0());
+            put3(d.p1, t->p1());
+            put3(d.p2, t->p2());
+            putc(d.radiance, t->radiance());
+            d.visible_to_camera = 1;
+        } else {
+            err = "light type not implemented on the HIP device (AreaLight and emissive triangles are)";
+            return false;
+        }
+        p.lights.push_back(d);
+    }
+
+    for (const auto& shape : scene.shapes()) {
+        SurfaceInteraction probe;
+        probe.hitObject = shape.get();
+        probe.primIndex = 0;
+        pe_mesh_desc md{};
+        if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape.get())) {
+            std::vector<float> pos, nrm, uv;
+            for (const Vec3f& v : mesh->positions()) pos.insert(pos.end(), {v.x, v.y, v.z});
+            for (const Vec3f& v : mesh->normals()) nrm.insert(nrm.end(), {v.x, v.y, v.z});
+            for (const Vec2f& v : mesh->uvs()) uv.insert(uv.end(), {v.x, v.y});
+            p.indices.push_back(mesh->indices());
+            md.num_vertices = static_cast<uint32_t>(mesh->positions().size());
+            md.num_triangles = static_cast<uint32_t>(mesh->numTriangles());
+            md.positions = p.keep(std::move(pos));
+            md.normals = nrm.empty() ? nullptr : p.keep(std::move(nrm));
+            md.uvs = uv.empty() ? nullptr : p.keep(std::move(uv));
+            md.indices = p.indices.back().data();
+            md.material = material(mesh->material());
+            md.emitter_base = scene.emitterIndex(probe);
+            md.standalone = 0;
+        } else if (auto* tri = dynamic_cast<const Triangle*>(shape.get())) {
+            const bool normals = tri->n0().lengthSquared() > 0.0f && tri->n1().lengthSquared() > 0.0f &&
+                                 tri->n2().lengthSquared() > 0.0f;
+            md.num_vertices = 3;
+            md.num_triangles = 1;
+            md.positions = p.keep({tri->v0().x, tri->v0().y, tri->v0().z, tri->v1().x, tri->v1().y, tri->v1().z,
+                                   tri->v2().x, tri->v2().y, tri->v2().z});
+            md.normals = normals ? p.keep({tri->n0().x, tri->n0().y, tri->n0().z, tri->n1().x, tri->n1().y,
+                                           tri->n1().z, tri->n2().x, tri->n2().y, tri->n2().z})
+                                 : nullptr;
+            md.uvs = p.keep({tri->uv0().x, tri->uv0().y, tri->uv1().x, tri->uv1().y, tri->uv2().x, tri->uv2().y});
+            p.indices.push_back({0u, 1u, 2u});
+            md.indices = p.indices.back().data();
+            md.material = material(tri->material());
+            probe.primIndex = -1;
+            md.emitter_base = scene.emitterIndex(probe);
+            md.standalone = 1;
+            md.flags = scene.cameraInvisible(probe) ? PE_SHAPE_CAMERA_INVISIBLE : 0u;
+        } else {
+            err = "shape type not implemented on the HIP device (TriangleMesh and Triangle are)";
+            return false;
+        }
+        if (md.material < 0) {
+            if (err.empty()) err = "shape without a material";
+            return false;
+        }
+        p.meshes.push_back(md);
+    }
+
+    if (const EnvironmentLight* env = scene.environment()) {
+        p.hasEnv = true;
+        pe_env_desc& e = p.env;
+        e = pe_env_desc{};
+        e.rotation = env->rotation();
+        e.intensity = env->intensity();
+        if (const auto& img = env->image()) {
+            std::vector<float> rgb;
+            rgb.reserve(3ull * static_cast<size_t>(img->width()) * static_cast<size_t>(img->height()));
+            for (int y = 0; y < img->height(); ++y)
+                for (int x = 0; x < img->width(); ++x) {
+                    const Color3f c = img->getPixel(x, y);
+                    rgb.insert(rgb.end(), {c.r, c.g, c.b});
+                }
+            e.width = img->width();
+            e.height = img->height();
+            e.rgb = p.keep(std::move(rgb));
+        }
+        if (env->hasSun()) {
+            e.has_sun = 1;
+            put3(e.sun_direction, env->sunDirection());
+            e.sun_cos_max = env->sunCosMax();
+            // The light stores the sun radiance with its intensity applied and the direction normalised
+            // (setSun, environment_light.cpp:168-171); the descriptor takes setSun's arguments, so both are
+            // handed back as setSun's inputs (exact for intensity 1, otherwise within a rounding).
+            const float k = env->intensity() != 0.0f ? 1.0f / env->intensity() : 0.0f;
+            putc(e.sun_radiance, env->sunRadiance() * k);
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+std::vector<DeviceInfo> enumerateHipDevices() {
+    std::vector<DeviceInfo> out;
+    int n = 0;
+    if (pe_device_count(&n) != PE_OK) return out;
+    for (int i = 0; i < n; ++i) {
+        pe_device_info pi{};
+        if (pe_device_info_get(i, &pi) != PE_OK) continue;
+        DeviceInfo d;
+        d.id = pi.id;
+        // The reference's DeviceKind has no HIP value; Cuda is its GPU-compute kind until it gains one.
+        d.kind = DeviceKind::Cuda;
+        d.name = pi.name;
+        d.memoryBytes = pi.memory_bytes;
+        d.usable = pi.usable != 0;
+        d.note = std::string(pi.arch) + ", " + std::to_string(pi.compute_units) + " CUs" +
+                 (pi.note[0] ? std::string(": ") + pi.note : std::string());
+        out.push_back(std::move(d));
+    }
+    return out;
+}
+
+std::unique_ptr<HipDevice> HipDevice::create(int ordinal, std::string* error) {
+    std::unique_ptr<HipDevice> dev(new HipDevice());
+    pe_device_info pi{};
+    if (pe_device_info_get(ordinal, &pi) != PE_OK || pe_device_create(ordinal, &dev->m_dev) != PE_OK) {
+        if (error) *error = lastHipError();
+        return nullptr;
+    }
+    dev->m_info.id = pi.id;
+    dev->m_info.kind = DeviceKind::Cuda;
+    dev->m_info.name = pi.name;
+    dev->m_info.memoryBytes = pi.memory_bytes;
+    dev->m_info.usable = true;
+    dev->m_info.note = std::string(pi.arch) + ", " + std::to_string(pi.compute_units) + " CUs";
+    return dev;
+}
+
+HipDevice::~HipDevice() {
+    if (m_film) pe_film_destroy(m_film);
+    if (m_scene) pe_scene_release(m_scene);
+    if (m_dev) pe_device_destroy(m_dev);
+}
+
+void HipDevice::setScene(std::shared_ptr<const Scene> scene) {
+    if (m_scene) {
+        pe_scene_release(m_scene);
+        m_scene = nullptr;
+    }
+    m_sceneRef.reset();
+    m_error.clear();
+    if (!scene) return;
+    PackedScene packed;
+    std::string err;
+    if (!packScene(*scene, packed, err)) {
+        m_error = "HipDevice::setScene: " + err;
+        return;
+    }
+    const pe_scene_desc desc = packed.desc();
+    if (pe_scene_upload(m_dev, &desc, &m_scene) != PE_OK) {
+        m_error = "HipDevice::setScene: " + lastHipError();
+        m_scene = nullptr;
+        return;
+    }
+    m_sceneRef = std::move(scene);
+}
+
+void HipDevice::updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind kind) {
+    if (!m_scene || !scene || kind == SceneUpdateKind::Full || kind == SceneUpdateKind::Transforms) {
+        setScene(std::move(scene));
+        return;
+    }
+    PackedScene packed;
+    std::string err;
+    if (!packScene(*scene, packed, err)) {
+        setScene(nullptr);
+        m_error = "HipDevice::updateScene: " + err;
+        return;
+    }
+    const pe_scene_desc desc = packed.desc();
+    const int k = kind == SceneUpdateKind::Materials ? PE_UPDATE_MATERIALS : PE_UPDATE_LIGHTS;
+    if (pe_scene_update(m_scene, k, &desc) != PE_OK) {
+        // An update the library cannot apply in place becomes a full upload.
+        setScene(std::move(scene));
+        return;
+    }
+    m_sceneRef = std::move(scene);
+}
+
+SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& settings, RenderTargets& targets,
+                                    int firstSample, int count, const std::atomic<bool>* cancel) {
+    SampleBatchResult res{false, 0};
+    m_lastRays = 0;
+    if (!m_scene || !targets.beauty) return res;
+    Film& film = *targets.beauty;
+    const CameraGpuDesc cd = camera.describeGpu();
+    if (cd.orthographic || cd.lensRadius > 0.0f) {
+        m_error = "HipDevice::render: only the pinhole perspective camera is implemented";
+        return res;
+    }
+    if (settings.aoStrength > 0.0f || settings.motionBlur || settings.pathGuiding) {
+        m_error = "HipDevice::render: AO, motion blur and path guiding are not implemented";
+        return res;
+    }
+    pe_camera cam{};
+    put3(cam.position, cd.position);
+    put3(cam.lower_left, cd.lowerLeft);
+    put3(cam.horizontal, cd.horizontal);
+    put3(cam.vertical, cd.vertical);
+    cam.lens_radius = 0.0f;
+    pe_settings st{};
+    st.max_bounces = settings.maxBounces;
+    st.rr_depth = 3;                                   // Renderer::renderSamples (renderer.cpp:432)
+    st.shadow_quality = settings.shadowQuality;
+    st.filter_type = static_cast<int32_t>(settings.filter.type);
+    st.filter_width = settings.filter.width;
+    st.clamp_indirect = settings.clampIndirect;
+    st.seed = 12345;                                   // createRenderSampler default (renderer.h:37)
+    st.samples_per_launch = 0;                         // the library sizes launches for the device
+
+    const int w = film.width(), h = film.height();
+    if (!m_film || m_filmW != w || m_filmH != h) {
+        if (m_film) pe_film_destroy(m_film);
+        m_film = nullptr;
+        if (pe_film_create(m_dev, w, h, &m_film) != PE_OK) {
+            m_error = "HipDevice::render: " + lastHipError();
+            return res;
+        }
+        m_filmW = w;
+        m_filmH = h;
+    }
+    if (pe_film_clear(m_film) != PE_OK) {
+        m_error = "HipDevice::render: " + lastHipError();
+        return res;
+    }
+    // Cancellation is polled between chunks; one chunk when nobody can cancel.
+    const int chunk = cancel ? std::max(1, std::min(count, 8)) : std::max(1, count);
+    int done = 0;
+    bool cancelled = false;
+    while (done < count) {
+        if (cancel && cancel->load(std::memory_order_relaxed)) {
+            cancelled = true;
+            break;
+        }
+        const int n = std::min(chunk, count - done);
+        pe_batch_result br{};
+        if (pe_render(m_dev, m_scene, &cam, &st, m_film, firstSample + done, n, nullptr, &br) != PE_OK) {
+            m_error = "HipDevice::render: " + lastHipError();
+            return res;
+        }
+        m_lastRays += br.extension_rays + br.shadow_rays;
+        done += n;
+    }
+    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    std::vector<float> sum(3 * pixels), lum2(pixels);
+    std::vector<uint32_t> cnt(pixels);
+    uint64_t rejected = 0;
+    if (pe_film_download(m_film, sum.data(), cnt.data(), lum2.data(), &rejected) != PE_OK) {
+        m_error = "HipDevice::render: " + lastHipError();
+        return res;
+    }
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t p = static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x);
+            film.addBatch(x, y, Color3f(sum[3 * p], sum[3 * p + 1], sum[3 * p + 2]), cnt[p]);
+        }
+    res.completed = !cancelled;
+    res.samplesTraced = static_cast<uint64_t>(pixels) * static_cast<uint64_t>(done);
+    return res;
+}
+
+}  // namespace photon

@@ -1,0 +1,56 @@
+// hip_device.h — RenderDevice backed by libphoton_hip.so (the gfx950 wavefront path tracer).
+//
+// Written against PhotonEngine's own engine headers: it is the file a maintainer adds as
+// src/engine/hip_device.{h,cpp}, registered by enumerateDevices()/createDevice() under the id "hip:N"
+// (engine/render_device.cpp:80-157). It packs the immutable Scene into a pe_scene_desc once per setScene and
+// implements render() as pe_render into a device film whose sums are merged into the caller's Film with
+// Film::addBatch, the way the Vulkan device merges its batch (vulkan_device.cpp:2152-2208).
+#pragma once
+
+#include "engine/render_device.h"
+#include "photon_hip.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace photon {
+
+class HipDevice final : public RenderDevice {
+public:
+    // nullptr and *error set when the ordinal has no usable gfx950 device (createDevice contract).
+    static std::unique_ptr<HipDevice> create(int ordinal, std::string* error);
+    ~HipDevice() override;
+
+    const DeviceInfo& info() const override { return m_info; }
+    // A first HIP device reports no AOVs / light passes / adaptive sampling; RenderSession turns them off
+    // with a warning (render_session.cpp:23-35).
+    DeviceFeatures features() const override { return DeviceFeatures{}; }
+    void setScene(std::shared_ptr<const Scene> scene) override;
+    void updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind kind) override;
+    bool hasScene() const override { return m_scene != nullptr; }
+    SampleBatchResult render(const Camera& camera, const RenderSettings& settings, RenderTargets& targets,
+                             int firstSample, int count, const std::atomic<bool>* cancel) override;
+
+    // Why the last setScene left the device without a scene (unsupported feature), empty otherwise.
+    const std::string& lastError() const { return m_error; }
+    // Rays the last render() traced (extension + shadow), for Mrays/s reporting.
+    uint64_t lastRays() const { return m_lastRays; }
+
+private:
+    HipDevice() = default;
+
+    DeviceInfo m_info;
+    pe_device* m_dev = nullptr;
+    pe_scene* m_scene = nullptr;
+    pe_film* m_film = nullptr;
+    int m_filmW = 0, m_filmH = 0;
+    std::shared_ptr<const Scene> m_sceneRef;
+    std::string m_error;
+    uint64_t m_lastRays = 0;
+};
+
+// The usable and unusable HIP devices, in the DeviceInfo form enumerateDevices() lists.
+std::vector<DeviceInfo> enumerateHipDevices();
+
+}  // namespace photon

@@ -150,8 +150,13 @@ class Scene:
         p1 = p0 + u
         p2 = p1 + v
         p3 = p0 + v
-        n = np.cross(u, v)
-        n = (n * np.float32(1.0 / np.sqrt(np.float32(np.dot(n, n))))).astype(np.float32)
+        # AreaLight ctor (area_light.h:21-24): u.cross(v).normalized() = n * (1 / sqrt(x*x + y*y + z*z)), each
+        # operation one float32 rounding in this order.
+        f = np.float32
+        n = np.array([f(u[1] * v[2]) - f(u[2] * v[1]), f(u[2] * v[0]) - f(u[0] * v[2]),
+                      f(u[0] * v[1]) - f(u[1] * v[0])], dtype=np.float32)
+        inv = f(f(1.0) / np.sqrt(f(f(f(n[0] * n[0]) + f(n[1] * n[1])) + f(n[2] * n[2]))))
+        n = (n * inv).astype(np.float32)
         flags = 0 if light.visible_to_camera else capi.PE_SHAPE_CAMERA_INVISIBLE
         for tri, uvs in (((p0, p1, p2), ((0, 0), (1, 0), (1, 1))), ((p0, p2, p3), ((0, 0), (1, 1), (0, 1)))):
             m = Mesh(np.stack(tri), np.array([[0, 1, 2]]), np.stack([n, n, n]), np.array(uvs, dtype=np.float32))
