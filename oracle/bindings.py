@@ -60,6 +60,7 @@ _ORACLE_SIGS = {
     "bsdf_eval": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F, _F]),
     "bsdf_sample": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F]),
     "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
+    "light_select": (C.c_int, [_VP, C.c_size_t, _F, _F, _F, _F, _I, _F, _I]),
 }
 
 _REF_SIGS = {
@@ -132,6 +133,24 @@ class OracleScene:
         occ = np.zeros(rays.shape[0], np.uint8)
         self._lib.oracle_occluded(self.h, rays.shape[0], _fp(rays), capi.ptr(occ, C.c_uint8))
         return occ
+
+    def light_select(self, p, n, u, with_pmf=True):
+        """LightSampler::Context at each (p, n): (pmf (N, candidates) or None, sampled index (N,), its pmf (N,),
+        whether the light BVH is used)."""
+        p = np.ascontiguousarray(p, np.float32)
+        n = np.ascontiguousarray(n, np.float32)
+        u = np.ascontiguousarray(u, np.float32)
+        N = p.shape[0]
+        cnt = self._lib.oracle_light_select(self.h, 0, _fp(p), _fp(n), _fp(u), None, None, None, None)
+        pmf = np.zeros((N, max(cnt, 1)), np.float32) if with_pmf else None
+        idx = np.zeros(N, np.int32)
+        spmf = np.zeros(N, np.float32)
+        bvh = C.c_int32()
+        rc = self._lib.oracle_light_select(self.h, N, _fp(p), _fp(n), _fp(u), _fp(pmf) if with_pmf else None,
+                                           capi.ptr(idx, C.c_int32), _fp(spmf), C.byref(bvh))
+        if rc < 0:
+            raise RuntimeError("oracle_light_select failed")
+        return pmf, idx, spmf, bool(bvh.value)
 
     def trace_samples(self, camera, settings, width, height, px, py, s):
         """Per-sample radiance (tracePixel + PathTracer::Li). Returns (L (n, 3), (extension, shadow) rays)."""

@@ -7,7 +7,9 @@
 #include "oracle_scene.h"
 #include "oracle_shading.h"
 
+#include <algorithm>
 #include <atomic>
+#include <limits>
 #include <chrono>
 #include <cstring>
 #include <stdexcept>
@@ -598,21 +600,331 @@ struct Env {
     }
 };
 
+// ------------------------------------------------------------------------------------ light_bounds.cpp
+struct LBox {  // core/math/aabb.h:24-31, empty = [+max, -max]; cwiseMin/Max are std::min/std::max
+    V3 lo{std::numeric_limits<float>::max()}, hi{-std::numeric_limits<float>::max()};
+    LBox() = default;
+    LBox(V3 a, V3 b) : lo(vmin(a, b)), hi(vmax(a, b)) {}   // AABB(a, b)
+    void merge(const LBox& b) {
+        lo = vmin(lo, b.lo);
+        hi = vmax(hi, b.hi);
+    }
+    void merge(V3 p) {
+        lo = vmin(lo, p);
+        hi = vmax(hi, p);
+    }
+    LBox merged(const LBox& b) const { return LBox(vmin(lo, b.lo), vmax(hi, b.hi)); }  // aabb.h merged()
+    V3 centroid() const { return (lo + hi) * 0.5f; }
+    V3 diagonal() const { return hi - lo; }
+    float surfaceArea() const {
+        const V3 d = diagonal();
+        return 2.0f * (d.x * d.y + d.x * d.z + d.y * d.z);
+    }
+    int maxExtent() const {
+        const V3 d = diagonal();
+        if (d.x > d.y && d.x > d.z) return 0;
+        if (d.y > d.z) return 1;
+        return 2;
+    }
+};
+
+float safeSqrtF(float x) { return std::sqrt(std::max(0.0f, x)); }
+float safeAcosF(float x) { return std::acos(std::clamp(x, -1.0f, 1.0f)); }
+float cosSubClamped(float sinA, float cosA, float sinB, float cosB) { return cosA > cosB ? 1.0f : cosA * cosB + sinA * sinB; }
+float sinSubClamped(float sinA, float cosA, float sinB, float cosB) { return cosA > cosB ? 0.0f : sinA * cosB - cosA * sinB; }
+
+struct LightBounds {  // light_bounds.h:15-25
+    LBox bounds;
+    V3 w{0.0f, 0.0f, 1.0f};
+    float phi = 0.0f, cosThetaO = 1.0f, cosThetaE = 0.0f;
+    bool twoSided = false;
+
+    float importance(V3 p, V3 n) const {  // light_bounds.cpp:39-82
+        if (!(phi > 0.0f)) return 0.0f;
+        const V3 pc = bounds.centroid();
+        const V3 diag = bounds.diagonal();
+        const float d2 = std::max(lengthSq(p - pc), 0.25f * lengthSq(diag));
+        if (!(d2 > 0.0f)) return phi;
+        V3 wi = p - pc;
+        const float len = length(wi);
+        wi = len > 0.0f ? wi / len : w;
+        float cosW = dot(w, wi);
+        if (twoSided) cosW = std::abs(cosW);
+        const float sinW = safeSqrtF(1.0f - cosW * cosW);
+        const float r2 = 0.25f * lengthSq(diag);
+        const float dc2 = lengthSq(p - pc);
+        const float cosB = dc2 <= r2 ? -1.0f : safeSqrtF(1.0f - r2 / dc2);
+        const float sinB = safeSqrtF(1.0f - cosB * cosB);
+        const float sinO = safeSqrtF(1.0f - cosThetaO * cosThetaO);
+        const float cosX = cosSubClamped(sinW, cosW, sinO, cosThetaO);
+        const float sinX = sinSubClamped(sinW, cosW, sinO, cosThetaO);
+        const float cosP = cosSubClamped(sinX, cosX, sinB, cosB);
+        if (cosP < cosThetaE) return 0.0f;
+        float imp = phi * cosP / d2;
+        if (lengthSq(n) > 0.0f) {
+            const float cosI = std::abs(dot(wi, n));
+            const float sinI = safeSqrtF(1.0f - cosI * cosI);
+            imp *= cosSubClamped(sinI, cosI, sinB, cosB);
+        }
+        return std::max(imp, 0.0f);
+    }
+};
+
+V3 rotateAround(V3 v, V3 axis, float theta) {  // light_bounds.cpp:30-33 (Rodrigues)
+    const float c = std::cos(theta), s = std::sin(theta);
+    return v * c + cross(axis, v) * s + axis * (dot(axis, v) * (1.0f - c));
+}
+
+LightBounds unionBounds(const LightBounds& a, const LightBounds& b) {  // light_bounds.cpp:84-109
+    if (!(a.phi > 0.0f)) return b;
+    if (!(b.phi > 0.0f)) return a;
+    LightBounds u;
+    u.bounds = a.bounds.merged(b.bounds);
+    u.phi = a.phi + b.phi;
+    u.cosThetaE = std::min(a.cosThetaE, b.cosThetaE);
+    u.twoSided = a.twoSided || b.twoSided;
+    const float thetaA = safeAcosF(a.cosThetaO), thetaB = safeAcosF(b.cosThetaO);
+    const float thetaD = safeAcosF(dot(a.w, b.w));
+    if (std::min(thetaD + thetaB, PI) <= thetaA) {
+        u.w = a.w;
+        u.cosThetaO = a.cosThetaO;
+        return u;
+    }
+    if (std::min(thetaD + thetaA, PI) <= thetaB) {
+        u.w = b.w;
+        u.cosThetaO = b.cosThetaO;
+        return u;
+    }
+    const float thetaO = 0.5f * (thetaA + thetaD + thetaB);
+    const V3 axis = cross(a.w, b.w);
+    if (thetaO >= PI || lengthSq(axis) < 1e-12f) {
+        u.w = a.w;
+        u.cosThetaO = -1.0f;
+        return u;
+    }
+    u.w = normalized(rotateAround(a.w, normalized(axis), thetaO - thetaA));
+    u.cosThetaO = std::cos(thetaO);
+    return u;
+}
+
+// AreaLight::bounds (area_light.cpp:260-273) and TriangleLight::bounds (:219-228); false = unbounded.
+bool lightBoundsOf(const LightRec& L, LightBounds& out) {
+    if (!(L.area > 0.0f) || isBlack(L.radiance)) return false;
+    if (L.type == PE_LIGHT_AREA) {
+        LBox box(L.p0, L.p0);
+        box.merge(L.p0 + L.p1);
+        box.merge(L.p0 + L.p2);
+        box.merge(L.p0 + L.p1 + L.p2);
+        out.bounds = box;
+    } else {
+        out.bounds = LBox(vmin(vmin(L.p0, L.p1), L.p2), vmax(vmax(L.p0, L.p1), L.p2));
+    }
+    out.w = L.normal;
+    out.phi = std::max(0.0f, luminance(L.radiance)) * L.area * PI;
+    out.cosThetaO = 1.0f;
+    out.cosThetaE = 0.0f;
+    out.twoSided = true;
+    return true;
+}
+
 // ------------------------------------------------------------------------------------ light_sampler.cpp
 constexpr float kUniformShare = 0.15f;
 constexpr int kMaxExhaustive = 33;
+constexpr int kMaxInfinite = 16;
+constexpr int kLightBuckets = 12;
+constexpr int kLightMaxDepth = 63;
+
+float groupCost(const LightBounds& b, const LBox& parent, int dim) {  // light_sampler.cpp:24-36
+    const float thetaO = std::acos(std::clamp(b.cosThetaO, -1.0f, 1.0f));
+    const float thetaE = std::acos(std::clamp(b.cosThetaE, -1.0f, 1.0f));
+    const float thetaW = std::min(thetaO + thetaE, PI);
+    const float sinO = std::sqrt(std::max(0.0f, 1.0f - b.cosThetaO * b.cosThetaO));
+    const float mOmega = TWO_PI * (1.0f - b.cosThetaO) +
+                         0.5f * PI * (2.0f * thetaW * sinO - std::cos(thetaO - 2.0f * thetaW) - 2.0f * thetaO * sinO +
+                                      b.cosThetaO);
+    const V3 d = parent.diagonal();
+    const float maxD = std::max({d.x, d.y, d.z});
+    const float kr = d[dim] > 0.0f ? maxD / d[dim] : 1.0f;
+    return b.phi * mOmega * kr * b.bounds.surfaceArea();
+}
+
+struct LightNode {
+    LightBounds lb;
+    int child1 = -1;  // interior: second child (the first is the next node)
+    int light = -1;   // leaf: candidate index
+};
 
 struct Integrator {
     std::vector<LightRec> lights;
     bool hasEnv = false;
     Env env;
     int count = 0;
+    // Light BVH (more than kMaxExhaustive candidates), LightSampler::build (light_sampler.cpp:42-77)
+    bool bvh = false;
+    std::vector<LightNode> nodes;
+    std::vector<int> infinite;
+    std::vector<int> topSlot;
+    std::vector<uint64_t> trail;
+    std::vector<int> depthOf;
 
     int explicitLights() const { return static_cast<int>(lights.size()); }
+
+    float candidateImportance(int i, V3 p, V3 n) const;
+
+    struct Item {
+        int index;
+        LightBounds lb;
+    };
+
+    int buildNode(std::vector<Item>& items, int begin, int end, uint64_t tr, int depth) {  // :79-148
+        const int ni = static_cast<int>(nodes.size());
+        nodes.emplace_back();
+        if (end - begin == 1) {
+            const Item& it = items[static_cast<size_t>(begin)];
+            nodes[static_cast<size_t>(ni)].lb = it.lb;
+            nodes[static_cast<size_t>(ni)].light = it.index;
+            trail[static_cast<size_t>(it.index)] = tr;
+            depthOf[static_cast<size_t>(it.index)] = depth;
+            return ni;
+        }
+        LBox bounds, centroids;
+        for (int i = begin; i < end; ++i) {
+            bounds.merge(items[static_cast<size_t>(i)].lb.bounds);
+            centroids.merge(items[static_cast<size_t>(i)].lb.bounds.centroid());
+        }
+        float bestCost = std::numeric_limits<float>::infinity();
+        int bestDim = -1, bestBucket = -1;
+        if (depth < kLightMaxDepth - 8) {
+            for (int dim = 0; dim < 3; ++dim) {
+                const float lo = centroids.lo[dim], hi = centroids.hi[dim];
+                if (!(hi > lo)) continue;
+                LightBounds bucket[kLightBuckets];
+                for (int i = begin; i < end; ++i) {
+                    const LightBounds& lb = items[static_cast<size_t>(i)].lb;
+                    int b = static_cast<int>(kLightBuckets * (lb.bounds.centroid()[dim] - lo) / (hi - lo));
+                    b = std::clamp(b, 0, kLightBuckets - 1);
+                    bucket[b] = unionBounds(bucket[b], lb);
+                }
+                for (int split = 0; split < kLightBuckets - 1; ++split) {
+                    LightBounds below, above;
+                    for (int b = 0; b <= split; ++b) below = unionBounds(below, bucket[b]);
+                    for (int b = split + 1; b < kLightBuckets; ++b) above = unionBounds(above, bucket[b]);
+                    if (!(below.phi > 0.0f) || !(above.phi > 0.0f)) continue;
+                    const float cost = groupCost(below, bounds, dim) + groupCost(above, bounds, dim);
+                    if (cost > 0.0f && cost < bestCost) {
+                        bestCost = cost;
+                        bestDim = dim;
+                        bestBucket = split;
+                    }
+                }
+            }
+        }
+        int mid = -1;
+        if (bestDim >= 0) {
+            const float lo = centroids.lo[bestDim], hi = centroids.hi[bestDim];
+            auto it = std::partition(items.begin() + begin, items.begin() + end, [&](const Item& item) {
+                int b = static_cast<int>(kLightBuckets * (item.lb.bounds.centroid()[bestDim] - lo) / (hi - lo));
+                return std::clamp(b, 0, kLightBuckets - 1) <= bestBucket;
+            });
+            mid = static_cast<int>(it - items.begin());
+        }
+        if (mid <= begin || mid >= end) {
+            mid = (begin + end) / 2;
+            const int dim = centroids.maxExtent();
+            std::nth_element(items.begin() + begin, items.begin() + mid, items.begin() + end,
+                             [dim](const Item& a, const Item& b) {
+                                 return a.lb.bounds.centroid()[dim] < b.lb.bounds.centroid()[dim];
+                             });
+        }
+        const int c0 = buildNode(items, begin, mid, tr, depth + 1);
+        const int c1 = buildNode(items, mid, end, tr | (uint64_t{1} << depth), depth + 1);
+        LightNode& nd = nodes[static_cast<size_t>(ni)];
+        nd.lb = unionBounds(nodes[static_cast<size_t>(c0)].lb, nodes[static_cast<size_t>(c1)].lb);
+        nd.child1 = c1;
+        return ni;
+    }
+
+    void buildSampler() {  // LightSampler::build (light_sampler.cpp:42-77)
+        topSlot.assign(static_cast<size_t>(count), -1);
+        trail.assign(static_cast<size_t>(count), 0);
+        depthOf.assign(static_cast<size_t>(count), 0);
+        if (count <= kMaxExhaustive) return;
+        std::vector<Item> items;
+        for (int i = 0; i < explicitLights(); ++i) {
+            LightBounds lb;
+            if (lightBoundsOf(lights[static_cast<size_t>(i)], lb) && lb.phi > 0.0f) items.push_back({i, lb});
+            else infinite.push_back(i);
+        }
+        if (hasEnv) infinite.push_back(explicitLights());
+        if (static_cast<int>(infinite.size()) + 1 > kMaxInfinite) {
+            infinite.clear();
+            return;
+        }
+        for (size_t k = 0; k < infinite.size(); ++k) topSlot[static_cast<size_t>(infinite[k])] = static_cast<int>(k);
+        if (!items.empty()) buildNode(items, 0, static_cast<int>(items.size()), 0, 0);
+        bvh = true;
+    }
+
+    int bvhSample(V3 p, V3 n, float u, float& pmf) const {  // light_sampler.cpp:242-266
+        pmf = 1.0f;
+        int node = 0;
+        for (;;) {
+            const LightNode& nd = nodes[static_cast<size_t>(node)];
+            if (nd.light >= 0) return nd.light;
+            const int c0 = node + 1, c1 = nd.child1;
+            const float i0 = nodes[static_cast<size_t>(c0)].lb.importance(p, n);
+            const float i1 = nodes[static_cast<size_t>(c1)].lb.importance(p, n);
+            if (!(i0 > 0.0f) && !(i1 > 0.0f)) {
+                pmf = 0.0f;
+                return -1;
+            }
+            const float p0 = i0 / (i0 + i1);
+            if (u < p0) {
+                node = c0;
+                u = std::min(u / p0, 0.99999994f);
+                pmf *= p0;
+            } else {
+                node = c1;
+                u = std::min((u - p0) / (1.0f - p0), 0.99999994f);
+                pmf *= 1.0f - p0;
+            }
+        }
+    }
+
+    float bvhPmf(V3 p, V3 n, int index) const {  // light_sampler.cpp:268-293
+        if (index < 0 || index >= static_cast<int>(trail.size())) return 0.0f;
+        uint64_t tr = trail[static_cast<size_t>(index)];
+        const int depth = depthOf[static_cast<size_t>(index)];
+        float pmf = 1.0f;
+        int node = 0;
+        for (int d = 0; d < depth; ++d) {
+            const LightNode& nd = nodes[static_cast<size_t>(node)];
+            if (nd.light >= 0) break;
+            const int c0 = node + 1, c1 = nd.child1;
+            const float i0 = nodes[static_cast<size_t>(c0)].lb.importance(p, n);
+            const float i1 = nodes[static_cast<size_t>(c1)].lb.importance(p, n);
+            const float sum = i0 + i1;
+            if (!(sum > 0.0f)) return 0.0f;
+            if (tr & 1u) {
+                pmf *= i1 / sum;
+                node = c1;
+            } else {
+                pmf *= i0 / sum;
+                node = c0;
+            }
+            tr >>= 1;
+        }
+        return nodes[static_cast<size_t>(node)].light == index ? pmf : 0.0f;
+    }
 };
 
-struct LightContext {  // LightSampler::Context (light_sampler.h:39-57), exhaustive and uniform modes
+float Integrator::candidateImportance(int i, V3 p, V3 n) const {
+    return i < explicitLights() ? lightImportance(lights[static_cast<size_t>(i)], p, n) : env.importance(n);
+}
+
+struct LightContext {  // LightSampler::Context (light_sampler.h:39-57)
     const Integrator* owner = nullptr;
+    V3 p, n;
     bool uniform = true;
     float pmfs[kMaxExhaustive] = {};
     int topCount = 0;
@@ -620,11 +932,11 @@ struct LightContext {  // LightSampler::Context (light_sampler.h:39-57), exhaust
     int sample(float u, float& pmf) const {  // light_sampler.cpp:196-229
         pmf = 0.0f;
         if (!owner) return -1;
-        const int n = owner->count;
-        if (n <= 0) return -1;
+        const int cnt = owner->count;
+        if (cnt <= 0) return -1;
         if (uniform) {
-            pmf = 1.0f / static_cast<float>(n);
-            return std::min(static_cast<int>(u * static_cast<float>(n)), n - 1);
+            pmf = 1.0f / static_cast<float>(cnt);
+            return std::min(static_cast<int>(u * static_cast<float>(cnt)), cnt - 1);
         }
         float acc = 0.0f;
         int k = topCount - 1;
@@ -635,26 +947,57 @@ struct LightContext {  // LightSampler::Context (light_sampler.h:39-57), exhaust
             }
             acc += pmfs[i];
         }
-        pmf = pmfs[k];
-        return k;
+        const float pk = pmfs[k];
+        if (!owner->bvh) {
+            pmf = pk;
+            return k;
+        }
+        if (k < static_cast<int>(owner->infinite.size())) {
+            pmf = pk;
+            return owner->infinite[static_cast<size_t>(k)];
+        }
+        const float ur = std::min((u - acc) / pk, 0.99999994f);
+        float treePmf = 0.0f;
+        const int light = owner->bvhSample(p, n, std::max(ur, 0.0f), treePmf);
+        pmf = pk * treePmf;
+        return light;
     }
     float pmf(int index) const {  // light_sampler.cpp:231-240
         if (!owner) return 0.0f;
         if (index < 0 || index >= owner->count) return 0.0f;
         if (uniform) return 1.0f / static_cast<float>(owner->count);
-        return pmfs[index];
+        if (!owner->bvh) return pmfs[index];
+        const int slot = owner->topSlot[static_cast<size_t>(index)];
+        if (slot >= 0) return pmfs[slot];
+        if (owner->nodes.empty()) return 0.0f;
+        return pmfs[owner->infinite.size()] * owner->bvhPmf(p, n, index);
     }
 };
 
-LightContext prepare(const Integrator& it, V3 p, V3 n) {  // light_sampler.cpp:150-194 (exhaustive)
+LightContext prepare(const Integrator& it, V3 p, V3 n) {  // light_sampler.cpp:150-194
     LightContext c;
     c.owner = &it;
+    c.p = p;
+    c.n = n;
     if (it.count <= 1) return c;
-    if (it.count > kMaxExhaustive) throw std::runtime_error("oracle: light BVH selection is not restated yet");
-    const int count = it.count;
-    for (int i = 0; i < count; ++i) {
-        const float w = i < it.explicitLights() ? lightImportance(it.lights[static_cast<size_t>(i)], p, n) : it.env.importance(n);
-        c.pmfs[i] = (w > 0.0f && std::isfinite(w)) ? w : 0.0f;
+    int count;
+    if (!it.bvh) {
+        if (it.count > kMaxExhaustive) return c;  // uniform (too many unbounded lights)
+        count = it.count;
+        for (int i = 0; i < count; ++i) {
+            const float w = it.candidateImportance(i, p, n);
+            c.pmfs[i] = (w > 0.0f && std::isfinite(w)) ? w : 0.0f;
+        }
+    } else {
+        count = static_cast<int>(it.infinite.size()) + (it.nodes.empty() ? 0 : 1);
+        for (size_t k = 0; k < it.infinite.size(); ++k) {
+            const float w = it.candidateImportance(it.infinite[k], p, n);
+            c.pmfs[k] = (w > 0.0f && std::isfinite(w)) ? w : 0.0f;
+        }
+        if (!it.nodes.empty()) {
+            const float w = it.nodes[0].lb.importance(p, n) * INV_PI;
+            c.pmfs[it.infinite.size()] = (w > 0.0f && std::isfinite(w)) ? w : 0.0f;
+        }
     }
     c.topCount = count;
     float sum = 0.0f;
@@ -938,6 +1281,7 @@ std::shared_ptr<Integrator> integratorFor(const Scene& s) {
         if (s.env.has_sun) e.setSun(c3(s.env.sun_direction), s.env.sun_cos_max, c3(s.env.sun_radiance));
     }
     it->count = static_cast<int>(it->lights.size()) + (it->hasEnv ? 1 : 0);
+    it->buildSampler();
     return it;
 }
 
@@ -954,6 +1298,30 @@ const Integrator& integratorOf(const Scene& s) {
 using namespace oracle;
 
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
+
+// LightSampler::Context at n points: pmf of every candidate (pmf: n * candidates) and one sample per point
+// (u[i] -> index[i], sample_pmf[i]). Returns the candidate count, or -1 on failure.
+ORACLE_API int oracle_light_select(void* sp, size_t n, const float* p, const float* nrm, const float* u, float* pmf,
+                                   int32_t* index, float* sample_pmf, int32_t* uses_bvh) {
+    try {
+        const Scene& scene = *static_cast<Scene*>(sp);
+        const Integrator& it = integratorOf(scene);
+        if (uses_bvh) *uses_bvh = it.bvh ? 1 : 0;
+        for (size_t i = 0; i < n; ++i) {
+            const V3 P(p[3 * i], p[3 * i + 1], p[3 * i + 2]), N(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+            const LightContext c = prepare(it, P, N);
+            if (pmf)
+                for (int k = 0; k < it.count; ++k) pmf[i * static_cast<size_t>(it.count) + static_cast<size_t>(k)] = c.pmf(k);
+            float sp_ = 0.0f;
+            const int k = c.sample(u[i], sp_);
+            if (index) index[i] = k;
+            if (sample_pmf) sample_pmf[i] = sp_;
+        }
+        return it.count;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
 
 // Per-sample radiance (tracePixel + Li) for n (px, py, sample) triples.
 ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,

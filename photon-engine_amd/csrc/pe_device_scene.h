@@ -63,6 +63,25 @@ struct alignas(16) GpuLight {
     float len_u, len_v;   // |u|, |v| (AREA)
 };
 
+// Light-BVH node (LightSampler::Node + LightBounds, light_sampler.h:64-69, light_bounds.h:15-25), 64 B. Nodes
+// are depth-first: an interior node's first child is the next node, its second child is `child1`.
+struct alignas(16) GpuLightNode {
+    float4 lo_phi;    // bounds lo xyz, phi (power measure)
+    float4 hi_cos_o;  // bounds hi xyz, cosThetaO
+    float4 w_cos_e;   // cone axis xyz, cosThetaE
+    int4 ref;         // child1 (interior) or -1, light candidate (leaf) or -1, twoSided, 0
+};
+static_assert(sizeof(GpuLightNode) == 64, "GpuLightNode must be 64 bytes");
+
+// Per light candidate in BVH mode: the root-to-leaf bit trail (child1 = 1, lowest bit first), its length,
+// and the candidate's top-level slot (unbounded lights and the environment), -1 for lights in the tree.
+struct LightPath {
+    uint32_t trail_lo, trail_hi;
+    int32_t depth, top_slot;
+};
+
+constexpr int kMaxTopCandidates = 16;  // LightSampler::kMaxInfinite
+
 // EnvironmentLight: image, MIS-compensated sampling distribution and the optional sun cone.
 struct EnvView {
     const float* rgb;        // 3 * w * h
@@ -81,7 +100,7 @@ struct EnvView {
     float sun_cos_max, sun_pdf_weight;
 };
 
-enum : int32_t { kPickUniform = 0, kPickExhaustive = 1 };
+enum : int32_t { kPickUniform = 0, kPickExhaustive = 1, kPickBvh = 2 };
 
 struct SceneView {
     const BvhNode* nodes;
@@ -98,6 +117,12 @@ struct SceneView {
     int32_t has_env;
     int32_t light_count;     // candidates incl. the environment
     int32_t pick_mode;       // kPick*
+    // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
+    const GpuLightNode* light_nodes;
+    const LightPath* light_paths;  // light_count entries
+    int32_t num_light_nodes;
+    int32_t num_top;               // unbounded candidates (top-level slots before the tree's)
+    int32_t top[kMaxTopCandidates];
 };
 
 }  // namespace pe

@@ -75,14 +75,15 @@ struct pe_scene {
     int num_shapes = 0;
     int bvh_depth = 0;
     pe::DevBuf nodes, tris, tri_shade, positions, normals, uvs;
-    pe::DevBuf materials, lights, tables, env_rgb, env_func, env_marginal, env_cond, env_mguide, env_cguide;
+    pe::DevBuf materials, lights, tables, env_rgb, env_func, env_marginal, env_cond, env_mguide, env_cguide,
+        light_nodes, light_paths;
     pe::SceneView view{};
     uint64_t num_nodes = 0;
     uint64_t device_bytes() const {
         return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes() +
                materials.bytes() + lights.bytes() + tables.bytes() + env_rgb.bytes() + env_func.bytes() +
                env_marginal.bytes() + env_cond.bytes() + env_mguide.bytes() +
-               env_cguide.bytes();
+               env_cguide.bytes() + light_nodes.bytes() + light_paths.bytes();
     }
 };
 

@@ -323,17 +323,127 @@ __device__ __forceinline__ float env_importance(const EnvView& e, V3 n) {  // en
 }
 
 // ------------------------------------------------------------------------------------ light selection
-// LightSampler::Context restated for the uniform and exhaustive (<= 33 candidates) modes
-// (light_sampler.cpp:150-240). The distribution depends only on (p, n): the picker at the previous vertex is
-// recomputed from its (p, n) where a later emission hit needs its pmf, instead of being stored.
+// LightSampler::Context (light_sampler.cpp:150-240) in its three modes: uniform, exhaustive (<= 33 candidates)
+// and light BVH (more; the unbounded candidates form a top level together with the tree, light_sampler.cpp:42-77).
+// The distribution depends only on (p, n): the picker at the previous vertex is recomputed from its (p, n) where
+// a later emission hit needs its pmf, instead of being stored, and the top-level weights are recomputed where
+// they are needed instead of being kept in registers.
 __device__ __forceinline__ float candidate_weight(const SceneView& sc, int i, V3 p, V3 n) {
     const float w = i < sc.num_lights ? light_importance(sc.lights[i], p, n) : env_importance(sc.env, n);
     return (w > 0.0f && isfinite(w)) ? w : 0.0f;
 }
 
+__device__ __forceinline__ float safe_sqrt(float x) { return sqrtf(fmaxf(0.0f, x)); }
+__device__ __forceinline__ float cos_sub_clamped(float sin_a, float cos_a, float sin_b, float cos_b) {
+    return cos_a > cos_b ? 1.0f : cos_a * cos_b + sin_a * sin_b;
+}
+__device__ __forceinline__ float sin_sub_clamped(float sin_a, float cos_a, float sin_b, float cos_b) {
+    return cos_a > cos_b ? 0.0f : sin_a * cos_b - cos_a * sin_b;
+}
+
+// LightBounds::importance (light_bounds.cpp:39-82) of a light-BVH node seen from (p, n).
+__device__ __forceinline__ float node_importance(const GpuLightNode& nd, V3 p, V3 n) {
+    const float phi = nd.lo_phi.w;
+    if (!(phi > 0.0f)) return 0.0f;
+    const V3 lo = mk(nd.lo_phi.x, nd.lo_phi.y, nd.lo_phi.z), hi = mk(nd.hi_cos_o.x, nd.hi_cos_o.y, nd.hi_cos_o.z);
+    const V3 pc = (lo + hi) * 0.5f;
+    const V3 diag = hi - lo;
+    const float d2 = fmaxf(length_sq(p - pc), 0.25f * length_sq(diag));
+    if (!(d2 > 0.0f)) return phi;
+    const V3 axis = mk(nd.w_cos_e.x, nd.w_cos_e.y, nd.w_cos_e.z);
+    V3 wi = p - pc;
+    const float len = length(wi);
+    wi = len > 0.0f ? wi / len : axis;
+    float cos_w = dot(axis, wi);
+    if (nd.ref.z) cos_w = fabsf(cos_w);
+    const float sin_w = safe_sqrt(1.0f - cos_w * cos_w);
+    const float r2 = 0.25f * length_sq(diag);
+    const float dc2 = length_sq(p - pc);
+    const float cos_b = dc2 <= r2 ? -1.0f : safe_sqrt(1.0f - r2 / dc2);
+    const float sin_b = safe_sqrt(1.0f - cos_b * cos_b);
+    const float cos_o = nd.hi_cos_o.w;
+    const float sin_o = safe_sqrt(1.0f - cos_o * cos_o);
+    const float cos_x = cos_sub_clamped(sin_w, cos_w, sin_o, cos_o);
+    const float sin_x = sin_sub_clamped(sin_w, cos_w, sin_o, cos_o);
+    const float cos_p = cos_sub_clamped(sin_x, cos_x, sin_b, cos_b);
+    if (cos_p < nd.w_cos_e.w) return 0.0f;
+    float imp = phi * cos_p / d2;
+    if (length_sq(n) > 0.0f) {
+        const float cos_i = fabsf(dot(wi, n));
+        const float sin_i = safe_sqrt(1.0f - cos_i * cos_i);
+        imp *= cos_sub_clamped(sin_i, cos_i, sin_b, cos_b);
+    }
+    return fmaxf(imp, 0.0f);
+}
+
+// Candidates the top-level weights run over: every candidate (exhaustive) or the unbounded ones plus the tree.
+__device__ __forceinline__ int top_count(const SceneView& sc) {
+    return sc.pick_mode == kPickBvh ? sc.num_top + (sc.num_light_nodes > 0 ? 1 : 0) : sc.light_count;
+}
+
+__device__ __forceinline__ float top_weight(const SceneView& sc, int k, V3 p, V3 n) {
+    if (sc.pick_mode != kPickBvh) return candidate_weight(sc, k, p, n);
+    if (k < sc.num_top) return candidate_weight(sc, sc.top[k], p, n);
+    // LightBounds measures pi x radiance, Light::importance radiance (light_sampler.cpp:175-178).
+    const float w = node_importance(sc.light_nodes[0], p, n) * kInvPi;
+    return (w > 0.0f && isfinite(w)) ? w : 0.0f;
+}
+
+// bvhSample (light_sampler.cpp:242-266): descend by the children's importance, re-stretching u at each level.
+__device__ __forceinline__ int bvh_sample(const SceneView& sc, V3 p, V3 n, float u, float& pmf) {
+    pmf = 1.0f;
+    int node = 0;
+    for (;;) {
+        const int4 r = sc.light_nodes[node].ref;
+        if (r.y >= 0) return r.y;
+        const int c0 = node + 1, c1 = r.x;
+        const float i0 = node_importance(sc.light_nodes[c0], p, n);
+        const float i1 = node_importance(sc.light_nodes[c1], p, n);
+        if (!(i0 > 0.0f) && !(i1 > 0.0f)) {
+            pmf = 0.0f;
+            return -1;
+        }
+        const float p0 = i0 / (i0 + i1);
+        if (u < p0) {
+            node = c0;
+            u = fminf(u / p0, 0.99999994f);
+            pmf *= p0;
+        } else {
+            node = c1;
+            u = fminf((u - p0) / (1.0f - p0), 0.99999994f);
+            pmf *= 1.0f - p0;
+        }
+    }
+}
+
+// bvhPmf (light_sampler.cpp:268-293): the product of the branch probabilities along the candidate's bit trail.
+__device__ __forceinline__ float bvh_pmf(const SceneView& sc, const LightPath& lp, int index, V3 p, V3 n) {
+    uint64_t trail = (static_cast<uint64_t>(lp.trail_hi) << 32) | lp.trail_lo;
+    float pmf = 1.0f;
+    int node = 0;
+    for (int d = 0; d < lp.depth; ++d) {
+        const int4 r = sc.light_nodes[node].ref;
+        if (r.y >= 0) break;
+        const int c0 = node + 1, c1 = r.x;
+        const float i0 = node_importance(sc.light_nodes[c0], p, n);
+        const float i1 = node_importance(sc.light_nodes[c1], p, n);
+        const float sum = i0 + i1;
+        if (!(sum > 0.0f)) return 0.0f;
+        if (trail & 1u) {
+            pmf *= i1 / sum;
+            node = c1;
+        } else {
+            pmf *= i0 / sum;
+            node = c0;
+        }
+        trail >>= 1;
+    }
+    return sc.light_nodes[node].ref.y == index ? pmf : 0.0f;
+}
+
 struct Picker {
-    float a, b;      // pmf(i) = w_i * a + b
-    bool uniform;
+    float a, b;      // top-level pmf(k) = w_k * a + b
+    bool uniform;    // Context::m_uniform: every candidate 1 / light_count
 };
 
 __device__ __forceinline__ Picker picker_prepare(const SceneView& sc, V3 p, V3 n) {
@@ -342,24 +452,28 @@ __device__ __forceinline__ Picker picker_prepare(const SceneView& sc, V3 p, V3 n
     pk.a = 0.0f;
     pk.b = sc.light_count > 0 ? 1.0f / static_cast<float>(sc.light_count) : 0.0f;
     if (sc.light_count <= 1 || sc.pick_mode == kPickUniform) return pk;
+    const int count = top_count(sc);
     float sum = 0.0f;
-    for (int i = 0; i < sc.light_count; ++i) sum += candidate_weight(sc, i, p, n);
+    for (int k = 0; k < count; ++k) sum += top_weight(sc, k, p, n);
     pk.uniform = false;
     if (!(sum > 0.0f) || !isfinite(sum)) {
-        pk.a = 0.0f;
-        pk.b = 1.0f / static_cast<float>(sc.light_count);
-        pk.uniform = true;  // all candidates 1/count (light_sampler.cpp:185-186)
+        pk.a = 0.0f;  // every top-level entry 1 / count (light_sampler.cpp:185-186)
+        pk.b = 1.0f / static_cast<float>(count);
         return pk;
     }
     pk.a = (1.0f - 0.15f) / sum;
-    pk.b = 0.15f / static_cast<float>(sc.light_count);
+    pk.b = 0.15f / static_cast<float>(count);
     return pk;
 }
 
 __device__ __forceinline__ float picker_pmf(const SceneView& sc, const Picker& pk, int index, V3 p, V3 n) {
     if (index < 0 || index >= sc.light_count) return 0.0f;
     if (pk.uniform) return pk.b;
-    return candidate_weight(sc, index, p, n) * pk.a + pk.b;
+    if (sc.pick_mode != kPickBvh) return top_weight(sc, index, p, n) * pk.a + pk.b;
+    const LightPath lp = sc.light_paths[index];
+    if (lp.top_slot >= 0) return top_weight(sc, lp.top_slot, p, n) * pk.a + pk.b;
+    if (sc.num_light_nodes == 0) return 0.0f;
+    return (top_weight(sc, sc.num_top, p, n) * pk.a + pk.b) * bvh_pmf(sc, lp, index, p, n);
 }
 
 __device__ __forceinline__ int picker_sample(const SceneView& sc, const Picker& pk, float u, V3 p, V3 n, float& pmf) {
@@ -371,17 +485,31 @@ __device__ __forceinline__ int picker_sample(const SceneView& sc, const Picker& 
         const int k = static_cast<int>(u * static_cast<float>(count));
         return k < count - 1 ? k : count - 1;
     }
+    const int tc = top_count(sc);
     float acc = 0.0f;
-    for (int i = 0; i < count - 1; ++i) {
-        const float pi = candidate_weight(sc, i, p, n) * pk.a + pk.b;
+    int k = tc - 1;
+    for (int i = 0; i < tc - 1; ++i) {
+        const float pi = top_weight(sc, i, p, n) * pk.a + pk.b;
         if (u < acc + pi) {
-            pmf = pi;
-            return i;
+            k = i;
+            break;
         }
         acc += pi;
     }
-    pmf = candidate_weight(sc, count - 1, p, n) * pk.a + pk.b;
-    return count - 1;
+    const float pk_k = top_weight(sc, k, p, n) * pk.a + pk.b;
+    if (sc.pick_mode != kPickBvh) {
+        pmf = pk_k;
+        return k;
+    }
+    if (k < sc.num_top) {
+        pmf = pk_k;
+        return sc.top[k];
+    }
+    const float ur = fminf((u - acc) / pk_k, 0.99999994f);
+    float tree_pmf = 0.0f;
+    const int light = bvh_sample(sc, p, n, fmaxf(ur, 0.0f), tree_pmf);
+    pmf = pk_k * tree_pmf;
+    return light;
 }
 
 }  // namespace pe

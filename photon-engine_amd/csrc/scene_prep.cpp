@@ -5,6 +5,7 @@
 // and without contraction, so the device sees the same floats the reference's IEEE build computes.
 #include "pe_scene_host.h"
 #include "pe_microfacet.h"
+#include "light_bvh.h"
 
 #include <algorithm>
 #include <cmath>
@@ -386,12 +387,26 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         const int rc = prepare_environment(dev, *desc.environment, sc, v.env);
         if (rc != PE_OK) return rc;
     }
-    // LightSampler::build (light_sampler.cpp:42-74): at most 33 candidates are chosen exhaustively.
-    if (v.light_count > 33) {
-        set_error("more than 33 light candidates: the light-BVH selection is not implemented on the gfx950 path yet");
-        return PE_ERR_UNSUPPORTED;
+    // LightSampler::build (light_sampler.cpp:42-77): up to 33 candidates are weighed exhaustively, more go into
+    // the light BVH with the unbounded ones on a top level.
+    LightBvh lb;
+    build_light_bvh(lights, desc.num_lights, v.has_env != 0, lb);
+    v.pick_mode = lb.mode;
+    if (lb.mode == kPickBvh) {
+        if (!lb.nodes.empty() && !sc.light_nodes.upload(lb.nodes.data(), lb.nodes.size() * sizeof(GpuLightNode), s))
+            return PE_ERR_OOM;
+        if (!sc.light_paths.upload(lb.paths.data(), lb.paths.size() * sizeof(LightPath), s)) return PE_ERR_OOM;
+        const hipError_t el = hipStreamSynchronize(s);
+        if (el != hipSuccess) {
+            set_error(std::string("light BVH upload: ") + hipGetErrorString(el));
+            return PE_ERR_HIP;
+        }
+        v.light_nodes = sc.light_nodes.as<GpuLightNode>();
+        v.light_paths = sc.light_paths.as<LightPath>();
+        v.num_light_nodes = static_cast<int32_t>(lb.nodes.size());
+        v.num_top = static_cast<int32_t>(lb.top.size());
+        for (size_t k = 0; k < lb.top.size(); ++k) v.top[k] = lb.top[k];
     }
-    v.pick_mode = v.light_count <= 1 ? kPickUniform : kPickExhaustive;
     v.nodes = sc.nodes.as<BvhNode>();
     v.tris = sc.tris.as<TriAccel>();
     v.shade = sc.tri_shade.as<TriShade>();

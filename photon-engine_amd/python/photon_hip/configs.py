@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .scene import AreaLight, Environment, Mesh, Scene, disney
+from .scene import AreaLight, Environment, Mesh, Scene, disney, lambert
 
 
 @dataclass
@@ -87,6 +87,33 @@ def box_icosphere(seed: int = 0, level: int = 5, width: int = 512, height: int =
     sc.add_shape(icosphere(level, 0.6, (0.0, -0.4, 0.0)), m)
     sc.add_light(AreaLight((-0.25, 0.999, -0.25), (0.5, 0.0, 0.0), (0.0, 0.0, 0.5), (10.0, 10.0, 10.0)))
     view = View((0.0, 0.0, 0.95), (0.0, -0.2, 0.0), (0.0, 1.0, 0.0), 60.0, width, height, spp, max_bounces)
+    return sc, view
+
+
+# ---- LED strip (many emitters) --------------------------------------------------------------------------
+
+def led_strip(quads: int = 1000, width: int = 48, height: int = 48, spp: int = 8, max_bounces: int = 1,
+              radiance: float = 4.0) -> tuple[Scene, View]:
+    """The reference's many-emitter scene (tests/test_light_sampler.cpp:104-127): a 12 x 12 Lambert floor under a
+    spiral of `quads` small emissive quads (2 * quads emissive triangles, Lambert(0, radiance)), so light
+    selection runs on the light BVH (more than 33 candidates)."""
+    sc = Scene()
+    floor = sc.add_material(lambert((0.7, 0.7, 0.7)))
+    up = np.array([0.0, 1.0, 0.0], dtype=np.float32)
+    sc.add_shape(Mesh(np.array([[-6, 0, -6], [6, 0, -6], [6, 0, 6], [-6, 0, 6]], np.float32),
+                      np.array([[0, 2, 1], [0, 3, 2]], np.uint32), np.stack([up] * 4)), floor)
+    f = np.float32
+    t = np.arange(quads, dtype=np.float32) / f(quads)
+    a, r = t * f(25.0), f(0.5) + f(4.5) * t
+    c = np.stack([r * np.cos(a), np.full_like(t, 0.6), r * np.sin(a)], 1).astype(np.float32)
+    off = np.array([[-0.03, 0, -0.03], [0.03, 0, -0.03], [0.03, 0, 0.03], [-0.03, 0, 0.03]], np.float32)
+    pos = (c[:, None, :] + off[None, :, :]).reshape(-1, 3)
+    b = (np.arange(quads, dtype=np.uint32) * 4)[:, None]
+    idx = np.concatenate([b + np.uint32(0), b + np.uint32(1), b + np.uint32(2),
+                          b + np.uint32(0), b + np.uint32(2), b + np.uint32(3)], 1).reshape(-1, 3)
+    led = sc.add_material(lambert((0.0, 0.0, 0.0), (radiance, radiance, radiance)))
+    sc.add_shape(Mesh(pos, idx), led)
+    view = View((0.0, 7.0, 7.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, width, height, spp, max_bounces)
     return sc, view
 
 
