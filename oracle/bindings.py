@@ -61,6 +61,7 @@ _ORACLE_SIGS = {
     "bsdf_sample": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F]),
     "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
     "light_select": (C.c_int, [_VP, C.c_size_t, _F, _F, _F, _F, _I, _F, _I]),
+    "spectral": (C.c_int, [C.c_size_t, _F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
 }
 
 _REF_SIGS = {
@@ -353,3 +354,14 @@ class HipDeviceSession:
             view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
+
+
+def spectral(u, nd=1.5, abbe=30.0, wavelengths=()):
+    """Oracle sampleWavelength(u) -> (lambda (n,), weight (n, 3)) and cauchyIor(nd, abbe, wl) -> (m,)."""
+    u = np.ascontiguousarray(u, np.float32)
+    wl = np.ascontiguousarray(wavelengths, np.float32)
+    lam = np.zeros(u.shape[0], np.float32)
+    w = np.zeros((u.shape[0], 3), np.float32)
+    ior = np.zeros(wl.shape[0], np.float32)
+    oracle().oracle_spectral(u.shape[0], _fp(u), _fp(lam), _fp(w), nd, abbe, wl.shape[0], _fp(wl), _fp(ior))
+    return lam, w, ior

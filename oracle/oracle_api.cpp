@@ -20,6 +20,21 @@ Scene* buildScene(const pe_scene_desc& d) {
             s->uv.push_back(md.uvs ? md.uvs[2 * v] : 0.0f);
             s->uv.push_back(md.uvs ? md.uvs[2 * v + 1] : 0.0f);
         }
+        // TriangleMesh without normals: area-weighted vertex normals (mesh.cpp:22-54).
+        const bool computed = !md.normals && !md.standalone && md.num_vertices > 0 && md.num_triangles > 0;
+        if (computed) {
+            for (uint32_t t = 0; t < md.num_triangles; ++t) {
+                const uint32_t i0 = vb + md.indices[3 * t], i1 = vb + md.indices[3 * t + 1], i2 = vb + md.indices[3 * t + 2];
+                const V3 n = cross(s->pos[i1] - s->pos[i0], s->pos[i2] - s->pos[i0]);
+                s->nrm[i0] += n;
+                s->nrm[i1] += n;
+                s->nrm[i2] += n;
+            }
+            for (uint32_t v = 0; v < md.num_vertices; ++v) {
+                V3& n = s->nrm[vb + v];
+                if (lengthSq(n) > 0.0f) n = normalized(n);
+            }
+        }
         for (uint32_t t = 0; t < md.num_triangles; ++t) {
             TriRec r;
             for (int k = 0; k < 3; ++k) r.v[k] = vb + md.indices[3 * t + k];
@@ -27,7 +42,7 @@ Scene* buildScene(const pe_scene_desc& d) {
             r.prim = md.standalone ? -1 : static_cast<int>(t);
             r.material = md.material;
             r.emitter = md.emitter_base < 0 ? -1 : (md.standalone ? md.emitter_base : md.emitter_base + static_cast<int>(t));
-            r.hasNormals = md.normals != nullptr;
+            r.hasNormals = md.normals != nullptr || computed;
             r.hasUv = md.uvs != nullptr;
             r.cameraInvisible = (md.flags & PE_SHAPE_CAMERA_INVISIBLE) != 0;
             s->tris.push_back(r);

@@ -1041,12 +1041,148 @@ bool makeShadowRay(const SI& isect, const LightSample& ls, Ray& out) {  // path_
     return spawnRayTo(isect, isect.point + ls.wi * ls.distance, out);
 }
 
+// ---------------------------------------------------------------------------- media, walks, transmittance
+V3 expNeg(V3 sigma, float t) { return V3(std::exp(-sigma.x * t), std::exp(-sigma.y * t), std::exp(-sigma.z * t)); }
+
+// sampleFreeFlight (path_tracer.cpp:97-128): spectral-MIS distance sampling in a homogeneous medium.
+bool sampleFreeFlight(V3 sigmaT, V3 sigmaS, float tMax, bool hitSurface, V3 channelWeights, Sobol& sampler,
+                      V3& throughput, float& tScatter) {
+    float wsum = channelWeights.x + channelWeights.y + channelWeights.z;
+    if (!(wsum > 0.0f)) {
+        channelWeights = V3(1.0f);
+        wsum = 3.0f;
+    }
+    const V3 pc = channelWeights / wsum;
+    const float uc = sampler.get1D();
+    const int c = uc < pc.x ? 0 : (uc < pc.x + pc.y ? 1 : 2);
+    const float u = sampler.get1D();
+    const float stc = sigmaT[c];
+    const float t = stc > 0.0f ? -std::log(std::max(1.0f - u, 1e-12f)) / stc : INFINITY_F;
+    if (t < tMax) {
+        const V3 Tr = expNeg(sigmaT, t);
+        const float pdf = pc.x * sigmaT.x * Tr.x + pc.y * sigmaT.y * Tr.y + pc.z * sigmaT.z * Tr.z;
+        if (!(pdf > 0.0f)) {
+            throughput = V3(0.0f);
+            return false;
+        }
+        throughput *= sigmaS * Tr / pdf;
+        tScatter = t;
+        return true;
+    }
+    V3 Tr;
+    for (int k = 0; k < 3; ++k) Tr[k] = hitSurface ? std::exp(-sigmaT[k] * tMax) : (sigmaT[k] > 0.0f ? 0.0f : 1.0f);
+    const float P = pc.x * Tr.x + pc.y * Tr.y + pc.z * Tr.z;
+    throughput = P > 0.0f ? throughput * Tr / P : V3(0.0f);
+    return false;
+}
+
+V3 sampleHenyeyGreenstein(V3 d, float g, float u0, float u1) {  // core/sampling/phase.h:27-39
+    float cosTheta;
+    if (std::abs(g) < 1e-3f) {
+        cosTheta = 1.0f - 2.0f * u0;
+    } else {
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
+        cosTheta = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    cosTheta = clampf(cosTheta, -1.0f, 1.0f);
+    const float sinTheta = std::sqrt(std::max(0.0f, 1.0f - cosTheta * cosTheta));
+    const float phi = TWO_PI * u1;
+    const Frame f(d);
+    return normalized(f.toWorld(V3(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta)));
+}
+
+// PathTracer::randomWalkSubsurface (path_tracer.cpp:263-292): a walk inside the closed mesh from the entry
+// point; true with the exit hit when the walk leaves through a surface.
+bool randomWalkSubsurface(const Scene& scene, const SI& entry, V3 dir, V3 sigmaT, V3 albedo, float g,
+                          Sobol& sampler, V3& throughput, SI& exitSi, RayCounters& rc) {
+    const V3 sigmaS = albedo * sigmaT;
+    V3 d = normalized(dir);
+    V3 origin = spawnOrigin(entry, d);
+    for (int step = 0; step < 256; ++step) {
+        Ray r;
+        r.o = origin;
+        r.d = d;
+        Hit h;
+        ++rc.extension;
+        const bool hit = intersect(scene, r, h);
+        SI hitSi;
+        if (hit) fillHit(scene, r, h, hitSi);
+        const float tMax = hit ? hitSi.t : INFINITY_F;
+        float t = 0.0f;
+        const bool scattered = sampleFreeFlight(sigmaT, sigmaS, tMax, hit, throughput * albedo, sampler, throughput, t);
+        if (!isFinite(throughput) || isBlack(throughput)) return false;
+        if (!scattered) {
+            if (!hit) return false;
+            exitSi = hitSi;
+            return true;
+        }
+        origin = r.o + r.d * t;
+        float u0, u1;
+        sampler.get2D(u0, u1);
+        d = sampleHenyeyGreenstein(d, g, u0, u1);
+        if (step >= 8 && !russianRoulette(throughput, sampler)) return false;
+    }
+    return false;
+}
+
+// shadowTransmittance (path_tracer.cpp:172-202), without shadow linking: thin-walled surfaces pass the shadow
+// ray on with their transmittance, anything else blocks it.
+V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow, RayCounters& rc) {
+    if (!transmitters) return intersectAny(scene, shadow) ? V3(0.0f) : V3(1.0f);
+    V3 T(1.0f);
+    V3 origin = shadow.o;
+    const V3 dir = shadow.d;
+    float remaining = shadow.tMax;
+    for (int i = 0; i < 64; ++i) {
+        Ray r;
+        r.o = origin;
+        r.d = dir;
+        r.tMin = 0.0f;
+        r.tMax = remaining;
+        Hit h;
+        if (i > 0) ++rc.shadow;
+        if (!intersect(scene, r, h)) return T;
+        SI hit;
+        fillHit(scene, r, h, hit);
+        const pe_material_desc& m = scene.materials[static_cast<size_t>(hit.material)];
+        if (!materialTransmitsShadows(m)) return V3(0.0f);
+        ShadingPoint sp{hit.normal, hit.tangent, hit.frontFace, hit.ng, 0.0f};
+        T *= materialShadowTransmittance(m, sp, dir);
+        if (isBlack(T)) return T;
+        origin = offsetRayOrigin(hit.point, hit.pError, lengthSq(hit.ng) > 0.0f ? hit.ng : hit.normal, dir);
+        remaining -= hit.t;
+        if (remaining <= 0.0f) return T;
+    }
+    return V3(0.0f);
+}
+
+struct MediumStack {  // path_tracer.cpp:63-83 (entries are material indices; -1 = the walk's exit material)
+    int owner[4] = {0, 0, 0, 0};
+    int n = 0;
+    int current() const { return n > 0 ? owner[n - 1] : -1; }
+    void enter(int m) {
+        if (n < 4) owner[n++] = m;
+    }
+    void leave(int m) {
+        for (int i = n - 1; i >= 0; --i) {
+            if (owner[i] != m) continue;
+            for (int j = i; j < n - 1; ++j) owner[j] = owner[j + 1];
+            --n;
+            return;
+        }
+    }
+};
+
 V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& ray, Sobol& sampler, RayCounters& rc) {
     V3 L(0.0f), throughput(1.0f);  // path_tracer.cpp:321-890
     Ray cur = ray;
     bool specularBounce = true;
     float lastBsdfPdf = 0.0f;
     V3 lastPoint = ray.o;
+    MediumStack media;
+    float lambda = 0.0f;
+    bool pendingExit = false;
+    SI isect;
     int depth = 0;
     const int explicitLights = it.explicitLights();
     const int lightCount = explicitLights + (it.hasEnv ? 1 : 0);
@@ -1054,6 +1190,15 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
     const int shadowSamples = st.shadowSamples;
     bool cameraRay = true;
     int diffuseBounces = 0;
+    int passThroughs = 0;
+    bool transmitters = false;  // Scene::hasShadowTransmitters: every descriptor material is retained
+    for (const pe_material_desc& m : scene.materials) transmitters = transmitters || materialTransmitsShadows(m);
+    pe_material_desc exitMaterial{};  // subsurfaceExitMaterial (path_tracer.cpp:56-59): white Lambert
+    exitMaterial.type = PE_MAT_LAMBERT;
+    exitMaterial.base_color[0] = exitMaterial.base_color[1] = exitMaterial.base_color[2] = 1.0f;
+    auto materialOf = [&](int index) -> const pe_material_desc& {
+        return index < 0 ? exitMaterial : scene.materials[static_cast<size_t>(index)];
+    };
     auto accumulate = [&L](V3 c) {
         if (!isFinite(c)) return;
         L += c;
@@ -1069,55 +1214,77 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
         accumulate(c);
     };
     for (;;) {
-        SI isect;
-        Hit h;
-        ++rc.extension;
-        bool hit = intersect(scene, cur, h);
-        if (hit) fillHit(scene, cur, h, isect);
-        for (int skip = 0; hit && skip < 64; ++skip) {  // :423-441 (no cutouts: opacity is 1)
-            if (!(cameraRay && isect.cameraInvisible)) break;
-            Ray next;
-            next.o = offsetRayOrigin(isect.point, isect.pError, isect.ng, cur.d);
-            next.d = cur.d;
-            next.tMin = 0.0f;
-            next.tMax = 1e30f;
-            cur = next;
+        V3 wo;
+        if (pendingExit) {  // :415-418
+            pendingExit = false;
+            wo = isect.normal;
+        } else {
             isect = SI{};
+            const V3 segmentStart = cur.o;
+            Hit h;
             ++rc.extension;
-            hit = intersect(scene, cur, h);
+            bool hit = intersect(scene, cur, h);
             if (hit) fillHit(scene, cur, h, isect);
-        }
-        const V3 wo = -cur.d;
-        if (!hit) {  // :482-514
-            if (it.hasEnv) {
-                const V3 Le = it.env.eval(cur.d);
+            for (int skip = 0; hit && skip < 64; ++skip) {  // :423-441 (no cutouts: opacity is 1)
+                if (!(cameraRay && isect.cameraInvisible)) break;
+                Ray next;
+                next.o = offsetRayOrigin(isect.point, isect.pError, isect.ng, cur.d);
+                next.d = cur.d;
+                next.tMin = 0.0f;
+                next.tMax = 1e30f;
+                cur = next;
+                isect = SI{};
+                ++rc.extension;
+                hit = intersect(scene, cur, h);
+                if (hit) fillHit(scene, cur, h, isect);
+            }
+            // Inside a volume: Beer-Lambert absorption along the segment (:457-479; no scattering media here).
+            const int medOwner = media.current();
+            V3 sigmaA;
+            if (medOwner >= 0 && materialInteriorAbsorption(scene.materials[static_cast<size_t>(medOwner)], sigmaA)) {
+                const float segment = hit ? length(isect.point - segmentStart) : INFINITY_F;
+                if (hit) throughput *= expNeg(sigmaA, segment);
+                else if (!isBlack(sigmaA)) break;
+                if (isBlack(throughput) || !isFinite(throughput)) break;
+            }
+            wo = -cur.d;
+            if (!hit) {  // :482-514
+                if (it.hasEnv) {
+                    const V3 Le = it.env.eval(cur.d);
+                    float weight = 1.0f;
+                    if (!specularBounce) {
+                        const float lightPdf = picker.pmf(explicitLights) * it.env.pdfLi(cur.d);
+                        weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
+                    }
+                    const V3 envLocal = Le * weight;
+                    if (diffuseBounces >= 2) accumulateIndirect(throughput * envLocal);
+                    else accumulate(throughput * envLocal);
+                }
+                break;
+            }
+            const V3 emitted = materialEmitted(materialOf(isect.material));
+            if (!isBlack(emitted)) {  // :565-593
+                const int li = isect.emitter;
                 float weight = 1.0f;
                 if (!specularBounce) {
-                    const float lightPdf = picker.pmf(explicitLights) * it.env.pdfLi(cur.d);
-                    weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
+                    const float lightPdf =
+                        li >= 0 ? picker.pmf(li) * lightPdfLiHit(it.lights[static_cast<size_t>(li)], lastPoint, isect.point) : 0.0f;
+                    if (lightPdf > 0.0f && lastBsdfPdf > 0.0f) weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
                 }
-                const V3 envLocal = Le * weight;
-                if (diffuseBounces >= 2) accumulateIndirect(throughput * envLocal);
-                else accumulate(throughput * envLocal);
+                const V3 emitLocal = emitted * weight;
+                if (diffuseBounces >= 2) accumulateIndirect(throughput * emitLocal);
+                else accumulate(throughput * emitLocal);
             }
-            break;
-        }
-        const pe_material_desc& mat = scene.materials[static_cast<size_t>(isect.material)];
-        const V3 emitted = materialEmitted(mat);
-        if (!isBlack(emitted)) {  // :565-593
-            const int li = isect.emitter;
-            float weight = 1.0f;
-            if (!specularBounce) {
-                const float lightPdf = li >= 0 ? picker.pmf(li) * lightPdfLiHit(it.lights[static_cast<size_t>(li)], lastPoint, isect.point) : 0.0f;
-                if (lightPdf > 0.0f && lastBsdfPdf > 0.0f) weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
-            }
-            const V3 emitLocal = emitted * weight;
-            if (diffuseBounces >= 2) accumulateIndirect(throughput * emitLocal);
-            else accumulate(throughput * emitLocal);
         }
         if (depth >= st.maxDepth) break;  // :596-599
+        const pe_material_desc& mat = materialOf(isect.material);
+        if (lambda == 0.0f && materialIsDispersive(mat)) {  // :602-607
+            V3 w;
+            lambda = sampleWavelength(sampler.get1D(), w);
+            throughput *= w;
+        }
         const bool isSpecular = materialIsDelta(mat);
-        ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace};
+        const ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace, isect.ng, lambda};
         if (lightCount > 0 && !isSpecular) {  // NEE, :631-698
             picker = prepare(it, isect.point, isect.normal);
             for (int s = 0; s < shadowSamples; ++s) {
@@ -1136,15 +1303,15 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                 Ray shadow;
                 if (!makeShadowRay(isect, ls, shadow)) continue;
                 ++rc.shadow;
-                const bool occluded = intersectAny(scene, shadow);
-                if (!occluded) {
+                const V3 vis = shadowTransmittance(scene, transmitters, shadow, rc);
+                if (!isBlack(vis)) {
                     float bsdfPdf = 0.0f;
                     const V3 f = evalPdf(mat, wo, ls.wi, sp, bsdfPdf);
                     const float cosTheta = std::abs(dot(ls.wi, isect.normal));
                     if (!isBlack(f) && cosTheta > 0.0f) {
                         const float lightPdf = lightPmf * ls.pdf;
                         const float weight = powerHeuristic(shadowSamples, lightPdf, 1, bsdfPdf);
-                        const V3 neeLocal = f * ls.Li * V3(1.0f) *
+                        const V3 neeLocal = f * ls.Li * vis *
                                             (cosTheta * weight / (lightPdf * static_cast<float>(shadowSamples)));
                         const V3 c = throughput * neeLocal;
                         if (diffuseBounces >= 1) accumulateIndirect(c);
@@ -1158,9 +1325,28 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
         sampler.get2D(u0, u1);
         BsdfSample bs;
         if (!sampleFull(mat, wo, sp, uc, u0, u1, bs)) break;
-        if (bs.subsurface) throw std::runtime_error("oracle: subsurface random walk is not restated yet");
+        if (bs.subsurface) {  // :746-764
+            V3 sigmaT, albedo;
+            float g = 0.0f;
+            if (!subsurfaceCoeffs(mat, sp, sigmaT, albedo, g)) break;
+            throughput *= bs.weight;
+            SI exitSi;
+            if (!randomWalkSubsurface(scene, isect, bs.wi, sigmaT, albedo, g, sampler, throughput, exitSi, rc)) break;
+            exitSi.normal = -exitSi.normal;
+            exitSi.material = -1;
+            isect = exitSi;
+            pendingExit = true;
+            cameraRay = false;
+            continue;
+        }
         if (bs.pdf <= 0.0f || isBlack(bs.f)) break;
-        if (bs.passThrough) throw std::runtime_error("oracle: thin-wall pass-through is not restated yet");
+        if (bs.passThrough) {  // :771-786
+            throughput *= bs.f * (std::abs(dot(bs.wi, isect.normal)) / bs.pdf);
+            if (!isFinite(throughput)) break;
+            cur = spawnRay(isect, bs.wi);
+            if (++passThroughs > 64) break;
+            continue;
+        }
         const V3 wi = bs.wi;
         const float cosTheta = std::abs(dot(wi, isect.normal));  // :814-816
         throughput *= bs.f * (cosTheta / bs.pdf);
@@ -1170,6 +1356,13 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
         specularBounce = bs.delta;
         if (!bs.delta) ++diffuseBounces;
         cameraRay = cameraRay && bs.delta && dot(wi, isect.normal) < 0.0f;
+        if (materialIsVolumeBoundary(mat)) {  // :840-848
+            const float a = dot(wo, isect.ng), b = dot(wi, isect.ng);
+            if (a * b < 0.0f) {
+                if (a > 0.0f) media.enter(isect.material);
+                else media.leave(isect.material);
+            }
+        }
         cur = spawnRay(isect, wi);
         ++depth;
         if (depth > st.rrDepth) {  // :862-870
@@ -1299,6 +1492,20 @@ using namespace oracle;
 
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
 
+// sampleWavelength for n values of u (lambda[i], weight[3i..3i+2]) and cauchyIor(nd, abbe, lambda) at m wavelengths.
+ORACLE_API int oracle_spectral(size_t n, const float* u, float* lambda, float* weight, float nd, float abbe, size_t m,
+                               const float* wl, float* ior) {
+    for (size_t i = 0; i < n; ++i) {
+        V3 w;
+        lambda[i] = sampleWavelength(u[i], w);
+        weight[3 * i] = w.x;
+        weight[3 * i + 1] = w.y;
+        weight[3 * i + 2] = w.z;
+    }
+    for (size_t i = 0; i < m; ++i) ior[i] = cauchyIor(nd, abbe, wl[i]);
+    return 0;
+}
+
 // LightSampler::Context at n points: pmf of every candidate (pmf: n * candidates) and one sample per point
 // (u[i] -> index[i], sample_pmf[i]). Returns the candidate count, or -1 on failure.
 ORACLE_API int oracle_light_select(void* sp, size_t n, const float* p, const float* nrm, const float* u, float* pmf,
@@ -1417,8 +1624,9 @@ ORACLE_API int oracle_bsdf_eval(const pe_material_desc* m, size_t n, const float
                                 const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
     try {
         for (size_t i = 0; i < n; ++i) {
-            ShadingPoint sp{V3(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]),
-                            V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0};
+            // As ref_driver's shadingPoint(): the geometric normal is the given normal.
+            const V3 nn(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f};
             float p = 0.0f;
             const V3 v = evalPdf(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]),
                                  sp, p);
@@ -1437,8 +1645,9 @@ ORACLE_API int oracle_bsdf_sample(const pe_material_desc* m, size_t n, const flo
                                   const int32_t* frontFace, const float* wo, const float* u3, float* out) {
     try {
         for (size_t i = 0; i < n; ++i) {
-            ShadingPoint sp{V3(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]),
-                            V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0};
+            // As ref_driver's shadingPoint(): the geometric normal is the given normal.
+            const V3 nn(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f};
             BsdfSample bs;
             const bool ok = sampleFull(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), sp, u3[3 * i], u3[3 * i + 1],
                                        u3[3 * i + 2], bs);

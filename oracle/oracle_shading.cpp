@@ -638,7 +638,286 @@ bool disneySampleFull(const Params& p, V3 wo, float uc, float u0, float u1, Bsdf
     return out.pdf > 0.0f;
 }
 
+// ---------------------------------------------------------------------------------- spectral.cpp
+float cieLobe(float x, float mu, float s1, float s2) {  // :14-17
+    const float t = (x - mu) / (x < mu ? s1 : s2);
+    return std::exp(-0.5f * t * t);
+}
+
+V3 rawRgb(float l) {  // :19-30 (Wyman, Sloan, Shirley 2013 fit, XYZ -> linear sRGB)
+    const float X = 1.056f * cieLobe(l, 599.8f, 37.9f, 31.0f) + 0.362f * cieLobe(l, 442.0f, 16.0f, 26.7f) -
+                    0.065f * cieLobe(l, 501.1f, 20.4f, 26.2f);
+    const float Y = 0.821f * cieLobe(l, 568.8f, 46.9f, 40.5f) + 0.286f * cieLobe(l, 530.9f, 16.3f, 31.1f);
+    const float Z = 1.217f * cieLobe(l, 437.0f, 11.8f, 36.0f) + 0.681f * cieLobe(l, 459.0f, 26.0f, 13.8f);
+    const float r = 3.2404542f * X - 1.5371385f * Y - 0.4985314f * Z;
+    const float g = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+    const float b = 0.0556434f * X - 0.2040259f * Y + 1.0572252f * Z;
+    return V3(std::max(0.0f, r), std::max(0.0f, g), std::max(0.0f, b));
+}
+
+constexpr float kLambdaMin = 380.0f, kLambdaMax = 730.0f;
+constexpr int kSpectralBins = 350;
+
+struct SpectralTables {  // :34-62
+    V3 norm{1.0f};
+    float cdf[kSpectralBins + 1] = {};
+    float pdf[kSpectralBins] = {};
+    SpectralTables() {
+        V3 sum(0.0f);
+        constexpr int kSteps = 3500;
+        const float dl = (kLambdaMax - kLambdaMin) / static_cast<float>(kSteps);
+        for (int i = 0; i < kSteps; ++i) sum += rawRgb(kLambdaMin + (static_cast<float>(i) + 0.5f) * dl);
+        const V3 mean = sum / static_cast<float>(kSteps);
+        norm = V3(1.0f / mean.x, 1.0f / mean.y, 1.0f / mean.z);
+        const float bw = (kLambdaMax - kLambdaMin) / static_cast<float>(kSpectralBins);
+        float acc = 0.0f;
+        cdf[0] = 0.0f;
+        float s[kSpectralBins];
+        for (int i = 0; i < kSpectralBins; ++i) {
+            const V3 c = rawRgb(kLambdaMin + (static_cast<float>(i) + 0.5f) * bw) * norm;
+            s[i] = c.x + c.y + c.z + 0.02f;
+            acc += s[i];
+            cdf[i + 1] = acc;
+        }
+        for (int i = 0; i <= kSpectralBins; ++i) cdf[i] /= acc;
+        for (int i = 0; i < kSpectralBins; ++i) pdf[i] = s[i] / (acc * bw);
+    }
+};
+
+const SpectralTables& spectralTables() {
+    static const SpectralTables t;
+    return t;
+}
+
+// ---------------------------------------------------------------------------------- dielectric.cpp
+bool glassSharp(float roughness) { return roughness <= 1e-3f; }                    // :19
+float glassAlpha(float roughness) { return ggxClampAlpha(roughness * roughness); }  // :21
+
+struct GlassLocal {
+    Frame frame;
+    V3 wo;
+};
+
+GlassLocal glassFrame(V3 woWorld, const ShadingPoint& sp) {  // :36-45
+    const V3 outward = sp.frontFace ? sp.normal : -sp.normal;
+    const V3 ng = lengthSq(sp.ng) > 1e-12f ? normalized(sp.ng) : normalized(outward);
+    V3 ns = lengthSq(sp.normal) > 1e-12f ? normalized(sp.normal) : ng;
+    if (dot(ns, ng) < 0.0f) ns = -ns;
+    if (dot(woWorld, ns) * dot(woWorld, ng) <= 0.0f) ns = ng;
+    GlassLocal g{Frame(ns), V3(0.0f)};
+    g.wo = g.frame.toLocal(woWorld);
+    return g;
+}
+
+bool generalizedHalf(V3 wo, V3 wi, float ior, V3& wm, float& etap) {  // :55-67
+    const float cosO = wo.z, cosI = wi.z;
+    if (cosO == 0.0f || cosI == 0.0f) return false;
+    const bool reflect = cosO * cosI > 0.0f;
+    etap = reflect ? 1.0f : (cosO > 0.0f ? ior : 1.0f / ior);
+    wm = wi * etap + wo;
+    if (lengthSq(wm) <= 1e-12f) return false;
+    wm = normalized(wm);
+    if (wm.z < 0.0f) wm = -wm;
+    if (dot(wm, wi) * cosI <= 0.0f || dot(wm, wo) * cosO <= 0.0f) return false;
+    return true;
+}
+
+V3 evalRoughLocal(V3 wo, V3 wi, float ior, float alpha, V3 tint) {  // :79-97
+    V3 wm;
+    float etap = 1.0f;
+    if (!generalizedHalf(wo, wi, ior, wm, etap)) return V3(0.0f);
+    const float cosO = wo.z, cosI = wi.z;
+    const float woDotH = dot(wo, wm), wiDotH = dot(wi, wm);
+    const float F = fresnelDielectric(woDotH, 1.0f, ior);
+    const float D = ggxD(wm, alpha, alpha);
+    const float G = ggxG(wo, wi, alpha, alpha);
+    if (cosO * cosI > 0.0f) return tint * (D * G * F / std::abs(4.0f * cosI * cosO));
+    const float s = wiDotH + woDotH / etap;
+    const float denom = s * s * cosI * cosO;
+    if (std::abs(denom) <= 1e-12f) return V3(0.0f);
+    const float ft = D * (1.0f - F) * G * std::abs(wiDotH * woDotH / denom);
+    return tint * (ft / (etap * etap));
+}
+
+float pdfRoughLocal(V3 wo, V3 wi, float ior, float alpha) {  // :105-119
+    V3 wm;
+    float etap = 1.0f;
+    if (!generalizedHalf(wo, wi, ior, wm, etap)) return 0.0f;
+    const float woDotH = dot(wo, wm), wiDotH = dot(wi, wm);
+    const float R = fresnelDielectric(woDotH, 1.0f, ior);
+    const float visible = ggxVisiblePdf(wo, wm, alpha, alpha);
+    if (wo.z * wi.z > 0.0f) return visible / (4.0f * std::abs(woDotH)) * R;
+    const float s = wiDotH + woDotH / etap;
+    if (s * s <= 1e-12f) return 0.0f;
+    return visible * std::abs(wiDotH) / (s * s) * (1.0f - R);
+}
+
+struct Glass {  // Dielectric(ior, tint, roughness) + setAbbe / setThinWalled as ref_driver builds it
+    float ior, roughness, abbe;
+    V3 tint;
+    bool thin;
+    explicit Glass(const pe_material_desc& d)
+        : ior(d.ior), roughness(d.roughness), abbe(d.abbe > 0.0f ? std::max(d.abbe, 10.0f) : 0.0f),
+          tint(c3(d.base_color)), thin(d.thin_walled != 0) {}
+    float iorAt(float wavelength) const {  // :174-177
+        if (abbe <= 0.0f || wavelength <= 0.0f) return ior;
+        return std::max(1.0001f, cauchyIor(ior, abbe, wavelength));
+    }
+    float thinWallReflectance(float cosTheta) const {  // :153-161 (no interference film)
+        cosTheta = clampf(std::abs(cosTheta), 0.0f, 1.0f);
+        const float R = fresnelDielectric(cosTheta, 1.0f, ior);
+        return 2.0f * R / (1.0f + R);
+    }
+};
+
+bool glassSampleWithLobe(const Glass& m, V3 woWorld, const ShadingPoint& sp, float uc, float u0, float u1, V3& wi,
+                         V3& f, float& pdf) {  // :186-253
+    const GlassLocal g = glassFrame(woWorld, sp);
+    const V3 wo = g.wo;
+    if (wo.z == 0.0f) return false;
+    const bool entering = wo.z > 0.0f;
+    const float ior = m.iorAt(sp.wavelength);
+    const float eta = entering ? (1.0f / ior) : ior;
+    if (glassSharp(m.roughness)) {
+        const float F = fresnelDielectric(wo.z, 1.0f, ior);
+        V3 wiLocal(-wo.x, -wo.y, wo.z);
+        V3 weight;
+        if (uc < F) {
+            pdf = F;
+            weight = m.tint * F;
+        } else {
+            const V3 n(0.0f, 0.0f, entering ? 1.0f : -1.0f);
+            if (!refractVec(wo, n, eta, wiLocal)) {
+                wiLocal = V3(-wo.x, -wo.y, wo.z);
+                pdf = 1.0f;
+                weight = m.tint;
+            } else {
+                pdf = 1.0f - F;
+                weight = m.tint * ((1.0f - F) * eta * eta);
+            }
+        }
+        wi = normalized(g.frame.toWorld(wiLocal));
+        f = weight / std::max(std::abs(dot(wi, sp.normal)), 1e-6f);
+        return true;
+    }
+    const float alpha = glassAlpha(m.roughness);
+    const V3 wm = ggxSampleVisibleNormal(wo, alpha, alpha, u0, u1);
+    const float R = fresnelDielectric(dot(wo, wm), 1.0f, ior);
+    V3 wiLocal;
+    if (uc < R) {
+        wiLocal = reflectVec(wo, wm);
+        if (wiLocal.z * wo.z <= 0.0f) return false;
+    } else {
+        const V3 n = entering ? wm : -wm;
+        if (!refractVec(wo, n, eta, wiLocal)) return false;
+        if (wiLocal.z * wo.z >= 0.0f) return false;
+    }
+    f = evalRoughLocal(wo, wiLocal, ior, alpha, m.tint);
+    pdf = pdfRoughLocal(wo, wiLocal, ior, alpha);
+    wi = normalized(g.frame.toWorld(wiLocal));
+    return pdf > 0.0f && !isBlack(f);
+}
+
+bool glassSampleFull(const Glass& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1,
+                     BsdfSample& out) {  // :169-198
+    out.subsurface = false;
+    out.passThrough = false;
+    if (!m.thin) {
+        out.delta = m.thin || glassSharp(m.roughness);
+        return glassSampleWithLobe(m, wo, sp, uc, u0, u1, out.wi, out.f, out.pdf);
+    }
+    out.delta = true;
+    const V3 n = sp.normal;
+    const float cosO = dot(wo, n);
+    if (cosO == 0.0f) return false;
+    const float R = m.thinWallReflectance(cosO);
+    const float pr = clampf((R + R + R) / 3.0f, 1e-4f, 1.0f - 1e-4f);
+    const float c = std::abs(cosO);
+    if (uc < pr) {
+        out.wi = normalized(n * (2.0f * cosO) - wo);
+        out.pdf = pr;
+        out.f = m.tint * R / c;
+    } else {
+        out.wi = -wo;
+        out.pdf = 1.0f - pr;
+        out.f = m.tint * (1.0f - R) / c;
+        out.passThrough = true;
+    }
+    return true;
+}
+
 }  // namespace
+
+float sampleWavelength(float u, V3& weight) {  // spectral.cpp:75-87
+    const SpectralTables& t = spectralTables();
+    const float* it = std::upper_bound(t.cdf, t.cdf + kSpectralBins + 1, u);
+    int i = static_cast<int>(it - t.cdf) - 1;
+    i = std::clamp(i, 0, kSpectralBins - 1);
+    const float lo = t.cdf[i], hi = t.cdf[i + 1];
+    const float f = hi > lo ? clampf((u - lo) / (hi - lo), 0.0f, 1.0f) : 0.5f;
+    const float bw = (kLambdaMax - kLambdaMin) / static_cast<float>(kSpectralBins);
+    const float lambda = kLambdaMin + (static_cast<float>(i) + f) * bw;
+    weight = (rawRgb(lambda) * t.norm) / ((kLambdaMax - kLambdaMin) * t.pdf[i]);
+    return lambda;
+}
+
+float cauchyIor(float nd, float abbe, float lambdaNm) {  // spectral.cpp:89-97
+    if (abbe <= 0.0f) return nd;
+    constexpr float lD = 0.58756f, lF = 0.48613f, lC = 0.65627f;
+    const float B = (nd - 1.0f) / (abbe * (1.0f / (lF * lF) - 1.0f / (lC * lC)));
+    const float A = nd - B / (lD * lD);
+    const float l = lambdaNm * 1e-3f;
+    return A + B / (l * l);
+}
+
+bool materialIsVolumeBoundary(const pe_material_desc& m) {
+    if (m.type == PE_MAT_DIELECTRIC) return m.thin_walled == 0;
+    if (m.type == PE_MAT_DISNEY) return clampf(m.specular_transmission, 0.0f, 1.0f) > 0.0f;
+    return false;
+}
+
+bool materialTransmitsShadows(const pe_material_desc& m) { return m.type == PE_MAT_DIELECTRIC && m.thin_walled != 0; }
+
+bool materialIsDispersive(const pe_material_desc& m) { return m.type == PE_MAT_DIELECTRIC && m.abbe > 0.0f; }
+
+bool materialInteriorAbsorption(const pe_material_desc& m, V3& sigmaA) {
+    if (m.type != PE_MAT_DIELECTRIC || m.thin_walled != 0 || !(m.absorption_distance > 0.0f)) return false;
+    const V3 color = clamp01(c3(m.absorption_color));  // setAbsorption (dielectric.cpp:123-127)
+    const float dist = std::max(m.absorption_distance, 1e-5f);
+    for (int c = 0; c < 3; ++c) sigmaA[c] = -std::log(std::max(color[c], 1e-4f)) / dist;
+    return !isBlack(sigmaA);  // Medium::empty (material.h:25)
+}
+
+V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp, V3 dir) {
+    if (!materialTransmitsShadows(m)) return V3(0.0f);
+    const Glass g(m);
+    const V3 n = lengthSq(sp.ng) > 0.0f ? normalized(sp.ng) : sp.normal;
+    return (V3(1.0f) - V3(g.thinWallReflectance(dot(dir, n)))) * g.tint;
+}
+
+bool subsurfaceCoeffs(const pe_material_desc& m, const ShadingPoint& sp, V3& sigmaT, V3& albedo, float& g) {
+    if (m.type != PE_MAT_DISNEY) return false;
+    const Params p = resolve(m, sp);
+    if (p.subsurface <= 0.0f) return false;
+    g = clampf(m.subsurface_anisotropy, -0.9f, 0.9f);
+    const V3 radius = V3(clampf(m.subsurface_radius[0], 1e-4f, 1e4f), clampf(m.subsurface_radius[1], 1e-4f, 1e4f),
+                         clampf(m.subsurface_radius[2], 1e-4f, 1e4f));
+    const float scale = std::max(m.subsurface_scale, 1e-6f);
+    for (int c = 0; c < 3; ++c) {
+        const float A = clampf(p.baseColor[c], 0.0f, 0.999f);
+        const float d = std::max(radius[c] * scale, 1e-7f);
+        const float alpha = 1.0f - std::exp(A * (-5.09406f + A * (2.61188f - 4.31805f * A)));
+        const float sVal = 1.9f - A + 3.5f * (A - 0.8f) * (A - 0.8f);
+        float st = 1.0f / std::max(d * sVal, 1e-12f);
+        float ss = alpha * st;
+        const float sa = st - ss;
+        ss /= (1.0f - g);
+        st = ss + sa;
+        sigmaT[c] = st;
+        albedo[c] = st > 0.0f ? ss / st : 0.0f;
+    }
+    return true;
+}
 
 bool materialIsDelta(const pe_material_desc& m) {
     if (m.type == PE_MAT_DIELECTRIC) return m.thin_walled != 0 || m.roughness <= 1e-3f;  // dielectric.h:37
@@ -657,6 +936,18 @@ V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, floa
         }
         pdf = cosineHemispherePdf(ci);
         return c3(m.base_color) * INV_PI;
+    }
+    if (m.type == PE_MAT_DIELECTRIC) {  // dielectric.cpp:262-292
+        const Glass g(m);
+        if (glassSharp(g.roughness)) {
+            pdf = 0.0f;
+            return V3(0.0f);
+        }
+        const GlassLocal gl = glassFrame(wo, sp);
+        const float ior = g.iorAt(sp.wavelength);
+        const V3 wiL = gl.frame.toLocal(wi);
+        pdf = pdfRoughLocal(gl.wo, wiL, ior, glassAlpha(g.roughness));
+        return evalRoughLocal(gl.wo, wiL, ior, glassAlpha(g.roughness), g.tint);
     }
     if (m.type != PE_MAT_DISNEY) throw std::runtime_error("oracle: material type not restated");
     const Params p = resolve(m, sp);  // disney.cpp:664-674
@@ -679,6 +970,7 @@ bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float 
         out.pdf = cosineHemispherePdf(wiL.z);
         return true;
     }
+    if (m.type == PE_MAT_DIELECTRIC) return glassSampleFull(Glass(m), wo, sp, uc, u0, u1, out);
     if (m.type != PE_MAT_DISNEY) throw std::runtime_error("oracle: material type not restated");
     return disneySampleFull(resolve(m, sp), wo, uc, u0, u1, out);
 }

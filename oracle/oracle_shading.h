@@ -67,6 +67,8 @@ struct ShadingPoint {
     V3 normal;   // shading normal facing the incoming ray
     V3 tangent;
     bool frontFace = true;
+    V3 ng;                   // geometric normal, not flipped (SurfaceInteraction::ng)
+    float wavelength = 0.0f; // > 0: the path's single wavelength (nm) after a dispersive hit
 };
 
 struct BsdfSample {  // material.h:37-49
@@ -77,6 +79,21 @@ struct BsdfSample {  // material.h:37-49
 };
 
 bool materialIsDelta(const pe_material_desc& m);
+// Material::isVolumeBoundary / transmitsShadows / isDispersive (material.h:109-122; dielectric.h:48-78;
+// disney.h:76).
+bool materialIsVolumeBoundary(const pe_material_desc& m);
+bool materialTransmitsShadows(const pe_material_desc& m);
+bool materialIsDispersive(const pe_material_desc& m);
+// Dielectric interior medium (Dielectric::rebuildMedium, dielectric.cpp:136-144): absorption only (the scene
+// description carries no scattering). false = none (interiorMedium() == nullptr).
+bool materialInteriorAbsorption(const pe_material_desc& m, V3& sigmaA);
+// Dielectric::shadowTransmittance (dielectric.cpp:163-167).
+V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp, V3 dir);
+// DisneyMaterial::subsurfaceCoeffs (disney.cpp:738-757).
+bool subsurfaceCoeffs(const pe_material_desc& m, const ShadingPoint& sp, V3& sigmaT, V3& albedo, float& g);
+// spectral.cpp: sampleWavelength (:75-87) and cauchyIor (:89-97).
+float sampleWavelength(float u, V3& weight);
+float cauchyIor(float nd, float abbe, float lambdaNm);
 V3 materialEmitted(const pe_material_desc& m);
 V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, float& pdf);
 bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1, BsdfSample& out);

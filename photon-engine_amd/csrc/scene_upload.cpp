@@ -89,6 +89,34 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
                                     : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             uv[vb + v] = m.uvs ? make_float2(m.uvs[2 * v], m.uvs[2 * v + 1]) : make_float2(0.0f, 0.0f);
         }
+        // A TriangleMesh given without normals gets area-weighted vertex normals (TriangleMesh ctor and
+        // calculateNormals, mesh.cpp:22-54): face normals (v1 - v0) x (v2 - v0) summed per vertex in triangle
+        // order, then normalised. Standalone triangles keep what they were given.
+        const bool computed = !m.normals && !m.standalone && m.num_vertices > 0 && m.num_triangles > 0;
+        if (computed) {
+            std::vector<V3> acc(m.num_vertices, mk(0.0f, 0.0f, 0.0f));
+            auto P = [&](uint32_t i) { return mk(m.positions[3 * i], m.positions[3 * i + 1], m.positions[3 * i + 2]); };
+            for (uint32_t t = 0; t < m.num_triangles; ++t) {
+                const uint32_t i0 = m.indices[3 * t], i1 = m.indices[3 * t + 1], i2 = m.indices[3 * t + 2];
+                const V3 n = cross(P(i1) - P(i0), P(i2) - P(i0));
+                acc[i0] = acc[i0] + n;
+                acc[i1] = acc[i1] + n;
+                acc[i2] = acc[i2] + n;
+            }
+            for (uint32_t v = 0; v < m.num_vertices; ++v) {
+                V3 n = acc[v];
+                if (dot(n, n) > 0.0f) {
+                    const float len = std::sqrt(dot(n, n));
+                    if (len <= 1e-20f) {
+                        n = mk(0.0f, 0.0f, 0.0f);
+                    } else {
+                        const float inv = 1.0f / len;
+                        n = mk(n.x * inv, n.y * inv, n.z * inv);
+                    }
+                }
+                nrm[vb + v] = make_float4(n.x, n.y, n.z, 0.0f);
+            }
+        }
         for (uint32_t t = 0; t < m.num_triangles; ++t) {
             TriShade& r = shade[tb + t];
             r.v0 = static_cast<uint32_t>(vb + m.indices[3 * t]);
@@ -98,7 +126,7 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             r.shape = s;
             r.prim = m.standalone ? -1 : static_cast<int32_t>(t);
             r.emitter = m.emitter_base < 0 ? -1 : (m.standalone ? m.emitter_base : m.emitter_base + static_cast<int32_t>(t));
-            r.flags = (m.normals ? kTriHasNormals : 0u) | (m.uvs ? kTriHasUv : 0u) |
+            r.flags = ((m.normals || computed) ? kTriHasNormals : 0u) | (m.uvs ? kTriHasUv : 0u) |
                       ((m.flags & PE_SHAPE_CAMERA_INVISIBLE) ? kTriCameraInvisible : 0u);
             const float4 p[3] = {pos[r.v0], pos[r.v1], pos[r.v2]};
             for (int k = 0; k < 3; ++k) {

@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .scene import AreaLight, Environment, Mesh, Scene, disney, lambert
+from .scene import AreaLight, Environment, Mesh, Scene, dielectric, disney, lambert
 
 
 @dataclass
@@ -114,6 +114,92 @@ def led_strip(quads: int = 1000, width: int = 48, height: int = 48, spp: int = 8
     led = sc.add_material(lambert((0.0, 0.0, 0.0), (radiance, radiance, radiance)))
     sc.add_shape(Mesh(pos, idx), led)
     view = View((0.0, 7.0, 7.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 50.0, width, height, spp, max_bounces)
+    return sc, view
+
+
+# ---- transmissive studio (glass, dispersion, absorbing liquid, subsurface) -------------------------------
+
+def box_mesh(lo, hi) -> Mesh:
+    """Closed box with outward-facing triangles (a volume boundary for glass, liquid and subsurface walks)."""
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    p = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    f = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                  [3, 7, 6], [3, 6, 2], [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], np.uint32)
+    return Mesh(p, f)
+
+
+def prism_mesh(center, size, height, subdiv: int = 1) -> Mesh:
+    """Closed triangular prism (outward normals) standing on y = center.y, its sides split into subdiv^2 quads."""
+    cx, cy, cz = center
+    a = [(cx + size * np.cos(t), cz + size * np.sin(t)) for t in (np.pi / 2, np.pi / 2 + 2 * np.pi / 3,
+                                                                    np.pi / 2 + 4 * np.pi / 3)]
+    pos, tris = [], []
+
+    def quad_grid(p00, p10, p01, p11):
+        base = len(pos)
+        for j in range(subdiv + 1):
+            for i in range(subdiv + 1):
+                u, v = i / subdiv, j / subdiv
+                q = (1 - u) * (1 - v) * np.array(p00) + u * (1 - v) * np.array(p10) + (1 - u) * v * np.array(p01) + \
+                    u * v * np.array(p11)
+                pos.append(q)
+        for j in range(subdiv):
+            for i in range(subdiv):
+                k = base + j * (subdiv + 1) + i
+                tris.append((k, k + 1, k + subdiv + 2))
+                tris.append((k, k + subdiv + 2, k + subdiv + 1))
+
+    for k in range(3):
+        (x0, z0), (x1, z1) = a[k], a[(k + 1) % 3]
+        quad_grid((x0, cy, z0), (x0, cy + height, z0), (x1, cy, z1), (x1, cy + height, z1))
+    b = len(pos)
+    pos += [(x, cy, z) for x, z in a] + [(x, cy + height, z) for x, z in a]
+    tris += [(b, b + 1, b + 2), (b + 3, b + 5, b + 4)]
+    return Mesh(np.array(pos, np.float32), np.array(tris, np.uint32))
+
+
+def glass_studio(width: int = 64, height: int = 48, spp: int = 16, max_bounces: int = 16, led_quads: int = 0,
+                 prism_subdiv: int = 2) -> tuple[Scene, View]:
+    """Transmissive studio after the spectral/volume workload: a dispersive prism (Dielectric 1.5, Abbe 30), an
+    absorbing liquid block (Dielectric 1.33 with setAbsorption), a subsurface block (Disney subsurface 1,
+    mean free path 1-5 mm), a thin-walled pane, a rough glass sphere and a Disney material with specular and
+    diffuse transmission, on a Lambert floor under an area light (plus an LED spiral of `led_quads` emissive
+    quads when asked)."""
+    rng = np.random.default_rng(2)
+    sc = Scene()
+    floor = sc.add_material(lambert((0.6, 0.6, 0.6)))
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    sc.add_shape(Mesh(np.array([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], np.float32),
+                      np.array([[0, 2, 1], [0, 3, 2]], np.uint32), np.stack([up] * 4)), floor)
+    back = sc.add_material(disney((0.3, 0.35, 0.5), 0.0, 0.6, 0.5))
+    sc.add_shape(_quad((-4, 0, -1.5), (4, 0, -1.5), (4, 3, -1.5), (-4, 3, -1.5)), back)
+    prism = sc.add_material(dielectric(1.5, abbe=30.0))
+    sc.add_shape(prism_mesh((-1.1, 0.002, 0.0), 0.45, 0.9, prism_subdiv), prism)
+    liquid = sc.add_material(dielectric(1.33, absorption_color=rng.uniform(0.2, 0.9, 3), absorption_distance=0.3))
+    sc.add_shape(box_mesh((-0.45, 0.002, -0.2), (0.05, 0.6, 0.3)), liquid)
+    sss = sc.add_material(disney((0.85, 0.55, 0.4), 0.0, 0.4, 0.5, subsurface=1.0,
+                                 subsurface_radius=rng.uniform(1.0, 5.0, 3), subsurface_scale=1e-3))
+    sc.add_shape(box_mesh((0.3, 0.002, -0.25), (0.8, 0.5, 0.25)), sss)
+    pane = sc.add_material(dielectric(1.5, tint=(0.9, 0.95, 1.0), thin_walled=True))
+    sc.add_shape(_quad((0.9, 0.0, 0.5), (1.7, 0.0, 0.2), (1.7, 0.9, 0.2), (0.9, 0.9, 0.5)), pane)
+    rough = sc.add_material(dielectric(1.45, tint=(1.0, 0.97, 0.9), roughness=0.25))
+    sc.add_shape(icosphere(2, 0.25, (1.15, 0.25, -0.55)), rough)
+    trans = sc.add_material(disney((0.5, 0.8, 0.4), 0.0, 0.35, 0.5, specular_transmission=0.6,
+                                   diffuse_transmission=0.3, ior=1.4))
+    sc.add_shape(icosphere(2, 0.22, (-0.2, 0.85, 0.6)), trans)
+    sc.add_light(AreaLight((-0.6, 2.5, -0.2), (1.2, 0.0, 0.0), (0.0, 0.0, 0.8), (12.0, 12.0, 12.0)))
+    if led_quads > 0:
+        led = sc.add_material(lambert((0.0, 0.0, 0.0), (3.0, 2.5, 2.0)))
+        t = np.arange(led_quads, dtype=np.float32) / np.float32(led_quads)
+        a, r = t * np.float32(12.0), np.float32(0.6) + np.float32(2.2) * t
+        c = np.stack([r * np.cos(a), np.full_like(t, 1.6), r * np.sin(a) - np.float32(0.3)], 1).astype(np.float32)
+        off = np.array([[-0.03, 0, -0.03], [-0.03, 0, 0.03], [0.03, 0, 0.03], [0.03, 0, -0.03]], np.float32)
+        pos = (c[:, None, :] + off[None, :, :]).reshape(-1, 3)
+        b = (np.arange(led_quads, dtype=np.uint32) * 4)[:, None]
+        idx = np.concatenate([b, b + 1, b + 2, b, b + 2, b + 3], 1).reshape(-1, 3).astype(np.uint32)
+        sc.add_shape(Mesh(pos, idx), led)
+    view = View((0.2, 1.3, 3.4), (0.0, 0.45, 0.0), (0.0, 1.0, 0.0), 45.0, width, height, spp, max_bounces)
     return sc, view
 
 

@@ -47,6 +47,20 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
     return m
 
 
+def dielectric(ior=1.5, tint=(1.0, 1.0, 1.0), roughness=0.0, abbe=0.0, absorption_color=None,
+               absorption_distance=0.0, thin_walled=False) -> capi.MaterialDesc:
+    """Dielectric(ior, tint, roughness) (dielectric.h:18-20) with setAbbe (:74), setAbsorption(color, distance)
+    (:57) when absorption_color is given, and setThinWalled (:43)."""
+    m = disney(tint)
+    m.type = capi.PE_MAT_DIELECTRIC
+    m.ior, m.roughness, m.abbe = float(ior), float(roughness), float(abbe)
+    if absorption_color is not None:
+        m.absorption_color[:] = list(map(float, absorption_color))
+        m.absorption_distance = float(absorption_distance)
+    m.thin_walled = 1 if thin_walled else 0
+    return m
+
+
 def lambert(albedo, emission=(0.0, 0.0, 0.0)) -> capi.MaterialDesc:
     """Lambertian(albedo, emission) (lambertian.h:16)."""
     m = disney(albedo)
