@@ -83,7 +83,8 @@ struct BatchClosest : BatchRays {
     const TriShade* shade;
     int4* hits;
     float2* bary;
-    __device__ __forceinline__ void closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v) const {
+    __device__ __forceinline__ bool closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v, float4&,
+                                            float4&) const {
         int4 o;
         if (tri >= 0) {
             const TriShade s = shade[tri];
@@ -94,6 +95,7 @@ struct BatchClosest : BatchRays {
         }
         hits[i] = o;
         if (bary) bary[i] = make_float2(u, v);
+        return false;
     }
 };
 

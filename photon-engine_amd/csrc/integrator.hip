@@ -46,6 +46,7 @@ struct PathBuffers {
     float4* last_p;   // previous vertex point (MIS)
     float4* last_n;   // previous vertex normal (MIS)
     uint4* ids;       // pixel, sample, Sobol pixel hash, packed bits (see pack_bits)
+    uint4* ext;       // transport state beyond the surface bounce (see the kExt* fields)
     float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
@@ -63,6 +64,50 @@ __device__ __forceinline__ bool get_camera(uint32_t w) { return (w >> 31) & 1u; 
 __device__ __forceinline__ uint32_t pack_bits(uint32_t dim, uint32_t depth, uint32_t diffuse, bool spec, bool cam) {
     return (dim & 0xFFFFu) | ((depth & 0x7Fu) << 16) | ((diffuse & 0x7Fu) << 23) | (spec ? (1u << 30) : 0u) |
            (cam ? (1u << 31) : 0u);
+}
+
+// ext column: x = the path's wavelength in nm (float bits, 0 = RGB transport); y, z = the medium stack, four
+// 16-bit material indices, bottom first (MediumStack, path_tracer.cpp:63-83); w = stack depth (bits 0-2),
+// thin-wall pass-throughs so far (3-9), subsurface walk step (10-18), walk in progress (19). While a walk is in
+// progress the walk's material index is in last_n.w.
+constexpr uint32_t kExtDepthMask = 7u;
+constexpr uint32_t kExtPassShift = 3u, kExtPassMask = 0x7Fu << kExtPassShift;
+constexpr uint32_t kExtWalkStepShift = 10u, kExtWalkStepMask = 0x1FFu << kExtWalkStepShift;
+constexpr uint32_t kExtWalk = 1u << 19;
+constexpr uint32_t kMaxWalkSteps = 256;   // randomWalkSubsurface's step limit (path_tracer.cpp:270)
+constexpr uint32_t kMaxPassThroughs = 64;  // path_tracer.cpp:783
+constexpr uint32_t kMaxShadowQueries = 64;  // shadowTransmittance (path_tracer.cpp:180)
+
+__device__ __forceinline__ uint32_t ext_pass_throughs(uint32_t w) { return (w & kExtPassMask) >> kExtPassShift; }
+__device__ __forceinline__ uint32_t ext_walk_step(uint32_t w) { return (w & kExtWalkStepMask) >> kExtWalkStepShift; }
+__device__ __forceinline__ uint64_t ext_stack(const uint4& e) { return e.y | (static_cast<uint64_t>(e.z) << 32); }
+__device__ __forceinline__ void ext_set_stack(uint4& e, uint64_t st, uint32_t depth) {
+    e.y = static_cast<uint32_t>(st);
+    e.z = static_cast<uint32_t>(st >> 32);
+    e.w = (e.w & ~kExtDepthMask) | depth;
+}
+// Material of the innermost medium, -1 outside every medium.
+__device__ __forceinline__ int ext_media_top(const uint4& e) {
+    const uint32_t n = e.w & kExtDepthMask;
+    if (n == 0) return -1;
+    return static_cast<int>((ext_stack(e) >> (16u * (n - 1u))) & 0xFFFFu);
+}
+__device__ __forceinline__ void ext_media_enter(uint4& e, int mat) {  // MediumStack::enter: full stacks ignore
+    const uint32_t n = e.w & kExtDepthMask;
+    if (n >= 4) return;
+    const uint64_t st = ext_stack(e) | (static_cast<uint64_t>(mat & 0xFFFF) << (16u * n));
+    ext_set_stack(e, st, n + 1u);
+}
+__device__ __forceinline__ void ext_media_leave(uint4& e, int mat) {  // MediumStack::leave: innermost match
+    const uint32_t n = e.w & kExtDepthMask;
+    const uint64_t st = ext_stack(e);
+    for (int i = static_cast<int>(n) - 1; i >= 0; --i) {
+        if (static_cast<int>((st >> (16 * i)) & 0xFFFFu) != (mat & 0xFFFF)) continue;
+        const uint64_t below = i == 0 ? 0ull : (st & ((1ull << (16 * i)) - 1ull));
+        const uint64_t above = i >= 3 ? 0ull : (st >> (16 * (i + 1)));
+        ext_set_stack(e, below | (above << (16 * i)), n - 1u);
+        return;
+    }
 }
 
 struct LaunchParams {
@@ -145,6 +190,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
         pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
+        pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -173,8 +219,10 @@ struct ExtensionJob {
     __device__ __forceinline__ TraceRay decode(const float4& a, const float4& b) const {
         return TraceRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 0.0f, 1e30f, 0u};
     }
-    __device__ __forceinline__ void closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v) const {
+    __device__ __forceinline__ bool closest(uint32_t i, const TraceRay&, float t, int32_t tri, float u, float v, float4&,
+                                            float4&) const {
         hits[i] = make_float4(t, __int_as_float(tri), u, v);
+        return false;
     }
 };
 
@@ -192,6 +240,73 @@ struct ShadowJob {
     }
     __device__ __forceinline__ void any(uint32_t, const TraceRay& r, bool occluded) const {
         if (occluded) pend[r.tag] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+};
+
+__device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
+    if (limit > 0.0f) {
+        const float m = fmaxf(c.x, fmaxf(c.y, c.z));
+        if (m > limit) return c * (limit / m);
+    }
+    return c;
+}
+
+// Held-back NEE records in scenes with thin walls: pend.w carries, as uint bits, the deferred indirect clamp
+// (bit 0) and the number of closest-hit queries the shadow ray has made (bits 1-7).
+constexpr uint32_t kPendClamp = 1u;
+
+// Shadow rays in scenes with thin walls (shadowTransmittance, path_tracer.cpp:172-202): a closest-hit query;
+// a thin wall multiplies the held-back contribution by its transmittance and the query continues from beyond
+// it with the remaining distance, anything else zeroes it; at most 64 queries. The indirect clamp applies to
+// the attenuated contribution, once the ray has reached the light.
+struct TransmitShadowJob {
+    const float4* so;
+    const float4* sd;
+    float4* pend;
+    SceneView sc;
+    float clamp_limit;
+    __device__ __forceinline__ void fetch(uint32_t i, float4& a, float4& b) const {
+        a = so[i];
+        b = sd[i];
+    }
+    __device__ __forceinline__ TraceRay decode(const float4& a, const float4& b) const {
+        return TraceRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 0.0f, a.w, __float_as_uint(b.w)};
+    }
+    __device__ __forceinline__ bool closest(uint32_t, const TraceRay& r, float t, int32_t tri, float u, float v,
+                                            float4& a, float4& b) const {
+        float4 p = pend[r.tag];
+        const uint32_t bits = __float_as_uint(p.w);
+        V3 c = mk(p.x, p.y, p.z);
+        bool more = false;
+        if (tri >= 0) {
+            Surface si;
+            fill_hit(sc, tri, r.d, u, v, si);
+            const GpuMaterial& m = sc.materials[si.material];
+            const uint32_t queries = (bits >> 1) + 1u;
+            if (!(m.flags & kMatFlagThinWall)) {
+                c = splat(0.0f);
+            } else {
+                c = c * thin_wall_transmittance(m, si, r.d);
+                const float remaining = r.tmax - t;
+                if (!is_black(c) && remaining > 0.0f) {
+                    if (queries >= kMaxShadowQueries) {
+                        c = splat(0.0f);
+                    } else {
+                        const V3 o = offset_ray_origin(si.point, si.p_error, length_sq(si.ng) > 0.0f ? si.ng : si.normal, r.d);
+                        a = make_float4(o.x, o.y, o.z, remaining);
+                        b = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(r.tag));
+                        more = true;
+                    }
+                }
+            }
+            if (more) {
+                pend[r.tag] = make_float4(c.x, c.y, c.z, __uint_as_float((bits & kPendClamp) | (queries << 1)));
+                return true;
+            }
+        }
+        if (bits & kPendClamp) c = clamp_indirect(c, clamp_limit);
+        pend[r.tag] = make_float4(c.x, c.y, c.z, 0.0f);
+        return false;
     }
 };
 
@@ -221,12 +336,19 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffer
     }
 }
 
-__device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
-    if (limit > 0.0f) {
-        const float m = fmaxf(c.x, fmaxf(c.y, c.z));
-        if (m > limit) return c * (limit / m);
+template <bool kCount>
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, PathBuffers pb, float clamp_limit,
+                                                                 uint32_t* spill) {
+    __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
+    uint32_t nv = 0, tv = 0;
+    trace_queue<false, kCount>(TransmitShadowJob{pb.sq_o, pb.sq_d, pb.pend, sc, clamp_limit}, pb.ctr[2], sc.nodes,
+                               sc.tris, st, nv, tv);
+    if (kCount) {
+        wave_count(&pb.stats[4], nv);
+        wave_count(&pb.stats[5], tv);
     }
-    return c;
 }
 
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
@@ -274,12 +396,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
         const uint32_t i = active ? s_slot[threadIdx.x] : 0u;
         __syncthreads();  // s_slot and s_count are rewritten by the next round
         bool extend = false;
-        bool alive = false;
         uint32_t pid = 0;
         V3 new_o, new_d;
         if (active) {
             pid = __float_as_uint(ro[i].w);
-            alive = true;
+            const float4 o4 = ro[i];
             const float4 d4 = rd[i];
             const V3 dir = mk(d4.x, d4.y, d4.z);
             const float4 t4 = pb.thr[pid];
@@ -288,9 +409,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             const float4 L4 = pb.L[pid];
             V3 L = mk(L4.x, L4.y, L4.z);
             const uint4 ids = pb.ids[pid];
+            uint4 ext = pb.ext[pid];
+            float lambda = __uint_as_float(ext.x);
             SobolState ss{ids.z, rev_bits(ids.y), get_dim(ids.w)};
             uint32_t depth = get_depth(ids.w), diffuse = get_diffuse(ids.w);
             bool specular = get_specular(ids.w), camera = get_camera(ids.w);
+            bool alive = true;
             // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
             for (int s = 0; s < S; ++s) {
                 const size_t k = static_cast<size_t>(pid) * S + s;
@@ -302,9 +426,56 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             const int tri = __float_as_int(h.y);
             const float4 lp4 = pb.last_p[pid], ln4 = pb.last_n[pid];
             const V3 last_p = mk(lp4.x, lp4.y, lp4.z), last_n = mk(ln4.x, ln4.y, ln4.z);
-            if (tri < 0) {
-                // Miss: environment with MIS against NEE (path_tracer.cpp:482-514).
-                if (sc.has_env) {
+            Surface si;
+            V3 wo = -dir;
+            bool vertex = false;        // a surface vertex is shaded below
+            bool exit_vertex = false;   // ... the walk's exit, a white Lambert facing out (path_tracer.cpp:758-763)
+            if (ext.w & kExtWalk) {
+                // One step of the subsurface random walk (randomWalkSubsurface, path_tracer.cpp:263-292).
+                const GpuMaterial& wm = sc.materials[__float_as_uint(ln4.w)];
+                const float tmax = tri >= 0 ? h.x : kFloatMax;
+                float t = 0.0f;
+                const bool scattered = free_flight(wm.sss_sigma_t, wm.sss_albedo * wm.sss_sigma_t, tmax, tri >= 0,
+                                                   thr * wm.sss_albedo, ss, thr, t);
+                const uint32_t step = ext_walk_step(ext.w);
+                if (!finite3(thr) || is_black(thr)) {
+                    alive = false;
+                } else if (!scattered) {
+                    if (tri < 0) {
+                        alive = false;  // escaped through open geometry
+                    } else {
+                        fill_hit(sc, tri, dir, h.z, h.w, si);
+                        si.normal = -si.normal;
+                        wo = si.normal;
+                        ext.w &= ~(kExtWalk | kExtWalkStepMask);
+                        camera = false;
+                        vertex = exit_vertex = true;
+                    }
+                } else {
+                    float u0, u1;
+                    sobol_get2d(ss, u0, u1);
+                    new_d = sample_hg(dir, wm.sss_g, u0, u1);
+                    new_o = mk(o4.x, o4.y, o4.z) + dir * t;
+                    if (step >= 8) {
+                        const float mx = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                        if (mx < 1.0f) {
+                            if (!(mx > 0.0f) || sobol_get1d(ss) >= mx) alive = false;
+                            else thr = thr / mx;
+                        }
+                    }
+                    if (alive && step + 1 < kMaxWalkSteps) {
+                        ext.w = (ext.w & ~kExtWalkStepMask) | ((step + 1) << kExtWalkStepShift);
+                        extend = true;
+                    } else {
+                        alive = false;
+                    }
+                }
+            } else if (tri < 0) {
+                // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
+                // environment with MIS against NEE (:482-514).
+                const int top = ext_media_top(ext);
+                const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
+                if (sc.has_env && !absorbed) {
                     const V3 le = env_eval(sc.env, dir);
                     float weight = 1.0f;
                     if (!specular) {
@@ -318,7 +489,6 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                 }
                 alive = false;
             } else {
-                Surface si;
                 fill_hit(sc, tri, dir, h.z, h.w, si);
                 if (camera && (si.flags & kTriCameraInvisible)) {
                     // Camera rays pass through lights hidden from the camera (path_tracer.cpp:423-441): the ray
@@ -327,108 +497,164 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     new_d = dir;
                     extend = true;
                 } else {
-                    const GpuMaterial& m = sc.materials[si.material];
-                    if (m.flags & kMatFlagEmissive) {  // path_tracer.cpp:565-593
-                        const int li = si.emitter;
-                        float weight = 1.0f;
-                        if (!specular) {
-                            float light_pdf = 0.0f;
-                            if (li >= 0) {
-                                const Picker pk = picker_prepare(sc, last_p, last_n);
-                                light_pdf = picker_pmf(sc, pk, li, last_p, last_n) * light_pdf_hit(sc.lights[li], last_p, si.point);
-                            }
-                            if (light_pdf > 0.0f && last_pdf > 0.0f) weight = power_heuristic(1, last_pdf, S, light_pdf);
-                        }
-                        V3 c = thr * (m.emission * weight);
-                        if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
-                        if (finite3(c)) L += c;
+                    // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
+                    const int top = ext_media_top(ext);
+                    if (top >= 0 && (sc.materials[top].flags & kMatFlagMedium)) {
+                        const V3 sa = sc.materials[top].sigma_a;
+                        const float seg = length(si.point - mk(o4.x, o4.y, o4.z));
+                        thr *= mk(expf(-sa.x * seg), expf(-sa.y * seg), expf(-sa.z * seg));
+                        if (is_black(thr) || !finite3(thr)) alive = false;
                     }
-                    if (static_cast<int>(depth) >= lp.max_depth) {
-                        alive = false;
-                    } else {
-                        const V3 wo = -dir;
-                        const bool is_delta = (m.flags & kMatFlagDelta) != 0;
-                        if (sc.light_count > 0 && !is_delta) {  // NEE, path_tracer.cpp:631-698
-                            const Picker pk = picker_prepare(sc, si.point, si.normal);
-                            // Each NEE sample's shadow ray is appended as soon as it is built (the append is
-                            // wave-aggregated over the lanes that reach it), so no shadow record stays live
-                            // across the loop.
-                            for (int s = 0; s < S; ++s) {
-                                float pmf = 0.0f;
-                                const float u = sobol_get1d(ss);
-                                const int li = picker_sample(sc, pk, u, si.point, si.normal, pmf);
-                                float ux, uy;
-                                sobol_get2d(ss, ux, uy);
-                                bool ok = false;
-                                V3 so, sd;
-                                float st = 0.0f;
-                                do {
-                                    if (li < 0 || !(pmf > 0.0f)) break;
-                                    const bool from_env = li >= sc.num_lights;
-                                    const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
-                                                                    : light_sample(sc.lights[li], si.point, ux, uy);
-                                    if (!(ls.pdf > 0.0f) || is_black(ls.li)) break;
-                                    if (!(ls.distance < kFloatMax) || ls.distance >= 1e29f) {
-                                        so = shadow_origin(si, ls.wi);
-                                        sd = ls.wi;
-                                        st = 1e30f;
-                                    } else if (!spawn_ray_to(si, si.point + ls.wi * ls.distance, so, sd, st)) {
-                                        break;
-                                    }
-                                    float bsdf_pdf = 0.0f;
-                                    const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, bsdf_pdf);
-                                    const float cos_t = fabsf(dot(ls.wi, si.normal));
-                                    if (is_black(f) || !(cos_t > 0.0f)) break;
-                                    const float light_pdf = pmf * ls.pdf;
-                                    const float weight = power_heuristic(S, light_pdf, 1, bsdf_pdf);
-                                    const V3 nee = f * ls.li * (cos_t * weight / (light_pdf * static_cast<float>(S)));
-                                    V3 c = thr * nee;
-                                    if (diffuse >= 1) c = clamp_indirect(c, lp.clamp_indirect);
-                                    if (!finite3(c)) break;
-                                    pb.pend[static_cast<size_t>(pid) * S + s] = make_float4(c.x, c.y, c.z, 0.0f);
-                                    ok = true;
-                                } while (false);
-                                const uint32_t ss_slot = wave_append(&pb.ctr[2], ok);
-                                if (ok) {
-                                    pb.sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
-                                    pb.sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
-                                                                   __uint_as_float(pid * static_cast<uint32_t>(S) +
-                                                                                   static_cast<uint32_t>(s)));
+                    if (alive) {
+                        const GpuMaterial& m = sc.materials[si.material];
+                        if (m.flags & kMatFlagEmissive) {  // path_tracer.cpp:565-593
+                            const int li = si.emitter;
+                            float weight = 1.0f;
+                            if (!specular) {
+                                float light_pdf = 0.0f;
+                                if (li >= 0) {
+                                    const Picker pk = picker_prepare(sc, last_p, last_n);
+                                    light_pdf = picker_pmf(sc, pk, li, last_p, last_n) *
+                                                light_pdf_hit(sc.lights[li], last_p, si.point);
                                 }
+                                if (light_pdf > 0.0f && last_pdf > 0.0f) weight = power_heuristic(1, last_pdf, S, light_pdf);
+                            }
+                            V3 c = thr * (m.emission * weight);
+                            if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
+                            if (finite3(c)) L += c;
+                        }
+                        vertex = true;
+                    }
+                }
+            }
+            if (vertex) {
+                const int mat = exit_vertex ? sc.exit_material : si.material;
+                const GpuMaterial& m = sc.materials[mat];
+                if (static_cast<int>(depth) >= lp.max_depth) {
+                    alive = false;
+                } else {
+                    // Dispersion: the first wavelength-dependent surface picks the path's wavelength (:602-607).
+                    if (lambda == 0.0f && (m.flags & kMatFlagDispersive)) {
+                        V3 w;
+                        lambda = sample_wavelength(T, sobol_get1d(ss), w);
+                        thr *= w;
+                        ext.x = __float_as_uint(lambda);
+                    }
+                    const bool is_delta = (m.flags & kMatFlagDelta) != 0;
+                    if (sc.light_count > 0 && !is_delta) {  // NEE, path_tracer.cpp:631-698
+                        const Picker pk = picker_prepare(sc, si.point, si.normal);
+                        // Each NEE sample's shadow ray is appended as soon as it is built (the append is
+                        // wave-aggregated over the lanes that reach it), so no shadow record stays live
+                        // across the loop.
+                        for (int s = 0; s < S; ++s) {
+                            float pmf = 0.0f;
+                            const float u = sobol_get1d(ss);
+                            const int li = picker_sample(sc, pk, u, si.point, si.normal, pmf);
+                            float ux, uy;
+                            sobol_get2d(ss, ux, uy);
+                            bool ok = false;
+                            V3 so, sd;
+                            float st = 0.0f;
+                            do {
+                                if (li < 0 || !(pmf > 0.0f)) break;
+                                const bool from_env = li >= sc.num_lights;
+                                const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
+                                                                : light_sample(sc.lights[li], si.point, ux, uy);
+                                if (!(ls.pdf > 0.0f) || is_black(ls.li)) break;
+                                if (!(ls.distance < kFloatMax) || ls.distance >= 1e29f) {
+                                    so = shadow_origin(si, ls.wi);
+                                    sd = ls.wi;
+                                    st = 1e30f;
+                                } else if (!spawn_ray_to(si, si.point + ls.wi * ls.distance, so, sd, st)) {
+                                    break;
+                                }
+                                float bsdf_pdf = 0.0f;
+                                const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, lambda, bsdf_pdf);
+                                const float cos_t = fabsf(dot(ls.wi, si.normal));
+                                if (is_black(f) || !(cos_t > 0.0f)) break;
+                                const float light_pdf = pmf * ls.pdf;
+                                const float weight = power_heuristic(S, light_pdf, 1, bsdf_pdf);
+                                const V3 nee = f * ls.li * (cos_t * weight / (light_pdf * static_cast<float>(S)));
+                                V3 c = thr * nee;
+                                uint32_t pend_bits = 0;
+                                if (diffuse >= 1) {
+                                    // With thin walls the clamp waits for the shadow ray's transmittance.
+                                    if (sc.has_transmitters) pend_bits = kPendClamp;
+                                    else c = clamp_indirect(c, lp.clamp_indirect);
+                                }
+                                if (!finite3(c)) break;
+                                pb.pend[static_cast<size_t>(pid) * S + s] =
+                                    make_float4(c.x, c.y, c.z, __uint_as_float(pend_bits));
+                                ok = true;
+                            } while (false);
+                            const uint32_t ss_slot = wave_append(&pb.ctr[2], ok);
+                            if (ok) {
+                                pb.sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
+                                pb.sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
+                                                               __uint_as_float(pid * static_cast<uint32_t>(S) +
+                                                                               static_cast<uint32_t>(s)));
                             }
                         }
-                        // BSDF sample (path_tracer.cpp:736-743), throughput and roulette (:814-870).
-                        const float uc = sobol_get1d(ss);
-                        float u0, u1;
-                        sobol_get2d(ss, u0, u1);
-                        BsdfSample bs;
-                        if (!bsdf_sample(m, T, si, wo, uc, u0, u1, bs) || bs.pdf <= 0.0f || is_black(bs.f)) {
+                    }
+                    // BSDF sample (path_tracer.cpp:736-743), then the walk entry (:746-764), the thin-wall
+                    // pass-through (:771-786) or the bounce with throughput and roulette (:814-870).
+                    const float uc = sobol_get1d(ss);
+                    float u0, u1;
+                    sobol_get2d(ss, u0, u1);
+                    BsdfSample bs;
+                    if (!bsdf_sample(m, T, si, wo, uc, u0, u1, lambda, bs)) {
+                        alive = false;
+                    } else if (bs.subsurface) {
+                        thr *= bs.weight;
+                        new_d = normalized(bs.wi);
+                        new_o = spawn_origin(si, new_d);
+                        ext.w = (ext.w & ~kExtWalkStepMask) | kExtWalk;
+                        pb.last_n[pid] = make_float4(ln4.x, ln4.y, ln4.z, __uint_as_float(static_cast<uint32_t>(mat)));
+                        extend = true;
+                    } else if (bs.pdf <= 0.0f || is_black(bs.f)) {
+                        alive = false;
+                    } else if (bs.pass_through) {
+                        thr *= bs.f * (fabsf(dot(bs.wi, si.normal)) / bs.pdf);
+                        const uint32_t passes = ext_pass_throughs(ext.w) + 1;
+                        if (!finite3(thr) || passes > 64) {
                             alive = false;
                         } else {
-                            const float cos_t = fabsf(dot(bs.wi, si.normal));
-                            thr *= bs.f * (cos_t / bs.pdf);
-                            if (!finite3(thr)) {
-                                alive = false;
-                            } else {
-                                last_pdf = bs.pdf;
-                                pb.last_p[pid] = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
-                                pb.last_n[pid] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
-                                specular = bs.delta;
-                                if (!bs.delta) ++diffuse;
-                                camera = camera && bs.delta && dot(bs.wi, si.normal) < 0.0f;
-                                new_o = spawn_origin(si, bs.wi);
-                                new_d = bs.wi;
-                                ++depth;
-                                extend = true;
-                                if (static_cast<int>(depth) > lp.rr_depth) {
-                                    const float mx = fmaxf(thr.x, fmaxf(thr.y, thr.z));
-                                    if (mx < 1.0f) {
-                                        if (!(mx > 0.0f) || sobol_get1d(ss) >= mx) {
-                                            extend = false;
-                                            alive = false;
-                                        } else {
-                                            thr = thr / mx;
-                                        }
+                            ext.w = (ext.w & ~kExtPassMask) | (passes << kExtPassShift);
+                            new_o = spawn_origin(si, bs.wi);
+                            new_d = bs.wi;
+                            extend = true;
+                        }
+                    } else {
+                        const float cos_t = fabsf(dot(bs.wi, si.normal));
+                        thr *= bs.f * (cos_t / bs.pdf);
+                        if (!finite3(thr)) {
+                            alive = false;
+                        } else {
+                            last_pdf = bs.pdf;
+                            pb.last_p[pid] = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
+                            pb.last_n[pid] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                            specular = bs.delta;
+                            if (!bs.delta) ++diffuse;
+                            camera = camera && bs.delta && dot(bs.wi, si.normal) < 0.0f;
+                            if (m.flags & kMatFlagVolume) {  // medium stack (path_tracer.cpp:840-848)
+                                const float a = dot(wo, si.ng), b = dot(bs.wi, si.ng);
+                                if (a * b < 0.0f) {
+                                    if (a > 0.0f) ext_media_enter(ext, mat);
+                                    else ext_media_leave(ext, mat);
+                                }
+                            }
+                            new_o = spawn_origin(si, bs.wi);
+                            new_d = bs.wi;
+                            ++depth;
+                            extend = true;
+                            if (static_cast<int>(depth) > lp.rr_depth) {
+                                const float mx = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+                                if (mx < 1.0f) {
+                                    if (!(mx > 0.0f) || sobol_get1d(ss) >= mx) {
+                                        extend = false;
+                                        alive = false;
+                                    } else {
+                                        thr = thr / mx;
                                     }
                                 }
                             }
@@ -436,12 +662,13 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     }
                 }
             }
+            if (!alive) extend = false;
             pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
             if (extend) {
                 pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
+                pb.ext[pid] = ext;
             }
-            (void)alive;
         }
         const uint32_t slot = wave_append(&pb.ctr[1], extend);
         if (extend) {
@@ -586,21 +813,21 @@ struct Wavefront {
     Profiler prof;
     uint32_t capacity = 0;
     int shadow_samples = 0;
-    DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, pend, hits, sq_o, sq_d, ctr, stats, filter;
+    DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, ctr, stats, filter;
     int filter_type = -1;
     float filter_width = 0.0f;
 
     ~Wavefront() { prof.drain(); }
 
     // Device bytes of the wavefront buffers per path in flight.
-    static uint64_t bytes_per_path(int S) { return 4 * 16 + 6 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
+    static uint64_t bytes_per_path(int S) { return 4 * 16 + 7 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
 
     int ensure(uint32_t n, int S) {
         if (n <= capacity && S == shadow_samples) return PE_OK;
         const size_t N = n, NS = static_cast<size_t>(n) * S;
         if (!ro0.alloc(N * 16) || !rd0.alloc(N * 16) || !ro1.alloc(N * 16) || !rd1.alloc(N * 16) ||
             !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) || !last_n.alloc(N * 16) ||
-            !ids.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) || !sq_o.alloc(NS * 16) ||
+            !ids.alloc(N * 16) || !ext.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) || !sq_o.alloc(NS * 16) ||
             !sq_d.alloc(NS * 16) || !ctr.alloc(64) || !stats.alloc(64)) {
             capacity = 0;
             return PE_ERR_OOM;
@@ -621,6 +848,7 @@ struct Wavefront {
         pb.last_p = last_p.as<float4>();
         pb.last_n = last_n.as<float4>();
         pb.ids = ids.as<uint4>();
+        pb.ext = ext.as<uint4>();
         pb.pend = pend.as<float4>();
         pb.hits = hits.as<float4>();
         pb.sq_o = sq_o.as<float4>();
@@ -699,7 +927,13 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
             const uint64_t target = std::min<uint64_t>(uint64_t{1} << 26, free_b / 4 / per_path);
             spl_req = static_cast<int>(std::clamp<uint64_t>(target / std::max<uint32_t>(1, pixels), 1, 1024));
         }
-        const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(uint64_t{1} << 31, uint64_t{pixels} * spl_req));
+        // Path ids and held-back slots (pid * S + s) are 32-bit.
+        const uint64_t path_limit = std::min<uint64_t>(uint64_t{1} << 31, 0xFFFFFFFFull / static_cast<uint64_t>(S));
+        if (pixels > path_limit) {
+            set_error("pe_render: film too large for 32-bit path ids");
+            return PE_ERR_INVALID;
+        }
+        const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(path_limit, uint64_t{pixels} * spl_req));
         const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
         if (rc != PE_OK) return rc;
@@ -755,8 +989,18 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
             hipLaunchKernelGGL(k_shade, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
             wf->prof.end(s);
             wf->prof.begin(kKShadow, s);
-            if (count) hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
-            else hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+            if (view.has_transmitters) {
+                if (count)
+                    hipLaunchKernelGGL(k_shadow_transmit<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb,
+                                       lp.clamp_indirect, spill);
+                else
+                    hipLaunchKernelGGL(k_shadow_transmit<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb,
+                                       lp.clamp_indirect, spill);
+            } else if (count) {
+                hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+            } else {
+                hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+            }
             wf->prof.end(s);
             hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, s, pb);
         };
@@ -775,22 +1019,24 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
             wf->prof.begin(kKGenerate, s);
             hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, lp, pb);
             wf->prof.end(s);
-            // max_depth + 1 vertices per path; a camera-invisible light adds a continuation without a vertex,
-            // so the loop runs on (checking the queue on the host) until every path has ended.
+            // max_depth + 1 vertices per path. Continuations that are not vertices (past a camera-invisible
+            // light, through a thin wall, the steps of a subsurface walk) lengthen a path, so the loop runs on,
+            // checking the queue on the host every few bounces, until every path has ended. The bound is what
+            // the reference's own limits allow: 256 walk steps per vertex, 64 pass-throughs, 64 skips.
             int it = 0;
             const int planned = lp.max_depth + 1;
+            const int limit = planned * static_cast<int>(kMaxWalkSteps + 1) + 2 * 64 + 8;
             for (;;) {
                 for (; it < planned; ++it) bounce(it & 1);
                 uint32_t left = 0;
                 PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr, 4, hipMemcpyDeviceToHost, s));
                 PE_HIP_CHECK(hipStreamSynchronize(s));
                 if (left == 0) break;
-                if (it > planned + 256) {
-                    set_error("pe_render: paths still alive after max_bounces + 256 continuations");
+                if (it > limit) {
+                    set_error("pe_render: paths still alive past the bounce, walk and pass-through limits");
                     return PE_ERR_INVALID;
                 }
-                bounce(it & 1);
-                ++it;
+                for (int k = 0; k < 4; ++k, ++it) bounce(it & 1);
             }
             const int film_blocks = static_cast<int>(std::min<uint32_t>((pixels + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKFilm, s);

@@ -16,6 +16,11 @@ struct EnergyTables {
     float eb[kTableN * kTableN];
     float avg0[kTableN], avg1[kTableN], avg2[kTableN];
     float sheen[kTableN * kTableN];
+    // Wavelength sampling (core/color/spectral.cpp:33-62): 1 nm bins over [380, 730] nm, CDF and pdf proportional
+    // to the normalised r + g + b response, and the per-channel normalisation.
+    float spec_cdf[351];
+    float spec_pdf[350];
+    float spec_norm[3];
 };
 
 // A material with its parameters clamped as the reference's setters clamp them and every quantity that depends
@@ -46,9 +51,28 @@ struct alignas(16) GpuMaterial {
     float st_w, opaque_dielectric, trans_w, diff_w;                      // evalLobes weights (:230-237, :263)
     V3 coat_tint_factor;             // coatTintFactor (:155-157)
     float pad1;
+    // Dielectric (dielectric.h): ior above, tint = base_color, roughness above
+    V3 sigma_a;                      // interior Beer-Lambert absorption (rebuildMedium, dielectric.cpp:136-144)
+    float abbe;                      // > 0: Cauchy dispersion (setAbbe, dielectric.h:74)
+    float glass_alpha;               // ggxClampAlpha(roughness^2) (dielectric.cpp:21)
+    float pad2, pad3, pad4;
+    // Disney subsurface random walk (subsurfaceCoeffs, disney.cpp:738-757)
+    V3 sss_sigma_t;
+    float sss_g;
+    V3 sss_albedo;
+    float pad5;
 };
 
-enum : uint32_t { kMatFlagEmissive = 1u, kMatFlagDelta = 2u, kMatFlagSheen = 4u, kMatFlagCoat = 8u };
+enum : uint32_t {
+    kMatFlagEmissive = 1u,
+    kMatFlagDelta = 2u,
+    kMatFlagSheen = 4u,
+    kMatFlagCoat = 8u,
+    kMatFlagVolume = 16u,            // isVolumeBoundary: refraction enters / leaves the medium stack
+    kMatFlagThinWall = 32u,          // thin-walled dielectric: pass-through and shadow transmittance
+    kMatFlagDispersive = 64u,        // isDispersive
+    kMatFlagMedium = 128u,           // interior medium absorbs (sigma_a != 0)
+};
 
 // AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.
 struct alignas(16) GpuLight {
@@ -117,6 +141,8 @@ struct SceneView {
     int32_t has_env;
     int32_t light_count;     // candidates incl. the environment
     int32_t pick_mode;       // kPick*
+    int32_t exit_material;   // white Lambert shading a subsurface walk's exit (one past the scene's materials)
+    int32_t has_transmitters;  // a thin-walled surface passes shadow rays on with its transmittance
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;
     const LightPath* light_paths;  // light_count entries

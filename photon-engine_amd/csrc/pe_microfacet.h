@@ -10,6 +10,26 @@
 
 namespace pe {
 
+// The piecewise-Gaussian CIE fit (Wyman, Sloan, Shirley 2013) mapped to linear sRGB and clipped at 0
+// (core/color/spectral.cpp:14-30). The host builds the wavelength tables from it; the device evaluates it for
+// a sampled wavelength.
+PE_HD float cie_lobe(float x, float mu, float s1, float s2) {
+    const float t = (x - mu) / (x < mu ? s1 : s2);
+    return expf(-0.5f * t * t);
+}
+PE_HD V3 spectral_rgb_raw(float l) {
+    const float X = 1.056f * cie_lobe(l, 599.8f, 37.9f, 31.0f) + 0.362f * cie_lobe(l, 442.0f, 16.0f, 26.7f) -
+                    0.065f * cie_lobe(l, 501.1f, 20.4f, 26.2f);
+    const float Y = 0.821f * cie_lobe(l, 568.8f, 46.9f, 40.5f) + 0.286f * cie_lobe(l, 530.9f, 16.3f, 31.1f);
+    const float Z = 1.217f * cie_lobe(l, 437.0f, 11.8f, 36.0f) + 0.681f * cie_lobe(l, 459.0f, 26.0f, 13.8f);
+    const float r = 3.2404542f * X - 1.5371385f * Y - 0.4985314f * Z;
+    const float g = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+    const float b = 0.0556434f * X - 0.2040259f * Y + 1.0572252f * Z;
+    return mk(fmaxf(0.0f, r), fmaxf(0.0f, g), fmaxf(0.0f, b));
+}
+constexpr float kLambdaMin = 380.0f, kLambdaMax = 730.0f;
+constexpr int kSpectralBins = 350;
+
 constexpr float kGgxMinAlpha = 1e-4f;
 PE_HD float ggx_clamp_alpha(float a) { return fmaxf(a, kGgxMinAlpha); }
 

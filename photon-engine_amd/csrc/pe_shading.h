@@ -3,13 +3,14 @@
 // Per-hit surface data follows Triangle::fillHit (geometry/triangle.cpp:54-133); spawning follows
 // geometry/ray_spawn.h:25-87; the BSDF is the reference's Principled/OpenPBR layering (materials/disney.cpp)
 // evaluated from a GpuMaterial whose material-only terms the host already computed (scene_prep.cpp).
-// Not on this path yet (rejected at scene upload): texture maps, metallic flakes, thin-film layers,
-// specular/diffuse-transmission and subsurface lobes, Dielectric.
+// The Dielectric (smooth / rough / thin-walled, with Cauchy dispersion) follows materials/dielectric.cpp.
+// Not on this path (rejected at scene upload): texture maps, metallic flakes, thin-film interference layers.
 #pragma once
 
 #include "pe_device_scene.h"
 #include "pe_math.h"
 #include "pe_microfacet.h"
+#include "pe_sobol.h"
 
 namespace pe {
 
@@ -235,11 +236,97 @@ __device__ __forceinline__ V3 eon_eval(V3 rho, float r, V3 wi, V3 wo) {
     return fss + rho_ms * (kInvPi * ms);
 }
 
-// evalLobes (disney.cpp:223-311) in the local frame, reflection half only (transmission lobes are rejected at
-// upload, so wi below the surface gives 0, as the reference does with both transmission weights at 0).
-__device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, V3 wo, V3 wi) {
-    if (wo.z <= 0.0f || wi.z <= 0.0f) return splat(0.0f);
-    const float cos_o = wo.z, cos_i = wi.z;
+// ------------------------------------------------------------------------------------- Fresnel, Snell
+// fresnelDielectric (core/math/utils.h:206-233): unpolarised reflectance, indices swapped from the inside.
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {
+    cos_i = clampf(cos_i, -1.0f, 1.0f);
+    if (!(cos_i > 0.0f)) {
+        const float tmp = eta_i;
+        eta_i = eta_t;
+        eta_t = tmp;
+        cos_i = fabsf(cos_i);
+    }
+    const float sin_i = sqrtf(fmaxf(0.0f, 1.0f - cos_i * cos_i));
+    const float sin_t = (eta_i / eta_t) * sin_i;
+    if (sin_t >= 1.0f) return 1.0f;
+    const float cos_t = sqrtf(fmaxf(0.0f, 1.0f - sin_t * sin_t));
+    const float rs = (eta_t * cos_i - eta_i * cos_t) / (eta_t * cos_i + eta_i * cos_t);
+    const float rp = (eta_i * cos_i - eta_t * cos_t) / (eta_i * cos_i + eta_t * cos_t);
+    return (rs * rs + rp * rp) * 0.5f;
+}
+
+// refractVec (utils.h:163-174): wi points away from the surface on n's side, eta = eta_i / eta_t.
+__device__ __forceinline__ bool refract_vec(V3 wi, V3 n, float eta, V3& wt) {
+    const float cos_i = dot(n, wi);
+    const float sin2_i = fmaxf(0.0f, 1.0f - cos_i * cos_i);
+    const float sin2_t = eta * eta * sin2_i;
+    if (sin2_t >= 1.0f) return false;
+    const float cos_t = sqrtf(1.0f - sin2_t);
+    wt = eta * (-wi) + (eta * cos_i - cos_t) * n;
+    return true;
+}
+
+// -------------------------------------------------------------------------------- Disney transmission
+// evalSpecTransmission / pdfSpecTransmission (disney.cpp:185-220): rough refraction through the base layer.
+__device__ __forceinline__ bool spec_trans_half(const GpuMaterial& m, bool front, V3 wo, V3 wi, V3& wm, float& etap,
+                                                float& wo_h, float& wi_h) {
+    if (wo.z <= 0.0f || wi.z >= 0.0f) return false;
+    etap = front ? m.ior : (1.0f / m.ior);
+    wm = wi * etap + wo;
+    if (length_sq(wm) <= 1e-12f) return false;
+    wm = normalized(wm);
+    if (wm.z < 0.0f) wm = -wm;
+    wo_h = dot(wo, wm);
+    wi_h = dot(wi, wm);
+    return wo_h > 0.0f && wi_h < 0.0f;
+}
+
+__device__ __forceinline__ V3 eval_spec_trans(const GpuMaterial& m, bool front, V3 wo, V3 wi) {
+    if (m.st_w <= 0.0f) return splat(0.0f);
+    V3 wm;
+    float etap, wo_h, wi_h;
+    if (!spec_trans_half(m, front, wo, wi, wm, etap, wo_h, wi_h)) return splat(0.0f);
+    const float alpha = ggx_clamp_alpha(m.roughness * m.roughness);
+    const float F = fresnel_dielectric(front ? wo_h : -wo_h, 1.0f, m.ior);
+    const float D = ggx_d(wm, alpha, alpha);
+    const float G = ggx_g(wo, mk(wi.x, wi.y, -wi.z), alpha, alpha);
+    const float s = wi_h + wo_h / etap;
+    const float denom = s * s * fabsf(wi.z * wo.z);
+    if (denom <= 1e-12f) return splat(0.0f);
+    const float ft = D * (1.0f - F) * G * fabsf(wi_h * wo_h) / (denom * etap * etap);
+    return m.base_color * (m.st_w * ft);
+}
+
+__device__ __forceinline__ float pdf_spec_trans(const GpuMaterial& m, bool front, V3 wo, V3 wi) {
+    V3 wm;
+    float etap, wo_h, wi_h;
+    if (!spec_trans_half(m, front, wo, wi, wm, etap, wo_h, wi_h)) return 0.0f;
+    const float alpha = ggx_clamp_alpha(m.roughness * m.roughness);
+    const float visible = ggx_visible_pdf(wo, wm, alpha, alpha);
+    const float s = wi_h + wo_h / etap;
+    if (s * s <= 1e-12f) return 0.0f;
+    return visible * fabsf(wi_h) / (s * s);
+}
+
+// evalLobes (disney.cpp:223-311) in the local frame: reflection above the surface; diffuse and specular
+// transmission below it.
+__device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, bool front, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f) return splat(0.0f);
+    const float cos_o = wo.z;
+    if (wi.z <= 0.0f) {
+        V3 ft = splat(0.0f);
+        if (m.trans_w <= 0.0f && m.st_w <= 0.0f) return ft;
+        const float coat_o = coat_transmit(m, T, cos_o);
+        if (m.trans_w > 0.0f) {
+            const float mu_i = -wi.z;
+            const float factor = layer_transmit(m, T, cos_o) * layer_transmit(m, T, mu_i) * m.inv_one_minus_avg;
+            const float sh = sheen_scale(m, T, cos_o, mu_i);
+            ft += m.base_color * (m.trans_w * kInvPi * coat_o * factor * sh) * m.coat_tint_factor;
+        }
+        if (m.st_w > 0.0f) ft += eval_spec_trans(m, front, wo, wi) * coat_o * m.coat_tint_factor;
+        return ft;
+    }
+    const float cos_i = wi.z;
     const V3 h = normalized(wo + wi);
     const float cos_d = dot(wo, h);
     V3 f_diffuse = splat(0.0f);
@@ -252,6 +339,7 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
     // specularFresnel (disney.cpp:79-96) without thin film
     V3 F = splat(m.wd * schlick_f(m.f0d, cos_d));
     if (m.metallic > 0.0f) F = lerp3(F, f82_tint(m.base_color, m.edge_tint, cos_d), m.metallic);
+    if (m.st_w > 0.0f && !front) F = lerp3(F, splat(fresnel_dielectric(-cos_d, 1.0f, m.ior)), m.st_w);
     // multiScatterFactor (disney.cpp:99-103)
     const float E = ggx_directional_albedo(T, cos_o, sqrtf(m.ax * m.ay));
     const V3 f0 = lerp3(splat(m.f0d), m.base_color, m.metallic);
@@ -275,9 +363,15 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
     return base * coat_att + f_coat;
 }
 
-// pdfLobes (disney.cpp:340-378), reflection half.
-__device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, V3 wo, V3 wi) {
-    if (wo.z <= 0.0f || wi.z <= 0.0f) return 0.0f;
+// pdfLobes (disney.cpp:340-378).
+__device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, bool front, V3 wo, V3 wi) {
+    if (wo.z <= 0.0f) return 0.0f;
+    if (wi.z <= 0.0f) {
+        float pdf = 0.0f;
+        if (m.p_trans > 0.0f) pdf += m.p_trans * (-wi.z * kInvPi);
+        if (m.p_spec_trans > 0.0f) pdf += m.p_spec_trans * pdf_spec_trans(m, front, wo, wi);
+        return pdf;
+    }
     const V3 h = normalized(wo + wi);
     const float cos_d = dot(wo, h);
     if (cos_d <= 0.0f) return 0.0f;
@@ -288,9 +382,92 @@ __device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, V3 wo, V
     return m.p_diff_refl * pdf_diffuse + m.p_spec * pdf_spec + m.p_coat * pdf_coat;
 }
 
-// Material::evalPdf for the surface's material; world-space directions.
+// ----------------------------------------------------------------------------------------- Dielectric
+__device__ __forceinline__ float cauchy_ior(float nd, float abbe, float lambda_nm) {  // spectral.cpp:89-97
+    if (abbe <= 0.0f) return nd;
+    const float lD = 0.58756f, lF = 0.48613f, lC = 0.65627f;
+    const float B = (nd - 1.0f) / (abbe * (1.0f / (lF * lF) - 1.0f / (lC * lC)));
+    const float A = nd - B / (lD * lD);
+    const float l = lambda_nm * 1e-3f;
+    return A + B / (l * l);
+}
+
+__device__ __forceinline__ float glass_ior(const GpuMaterial& m, float lambda) {  // iorAt (dielectric.cpp:174-177)
+    if (m.abbe <= 0.0f || lambda <= 0.0f) return m.ior;
+    return fmaxf(1.0001f, cauchy_ior(m.ior, m.abbe, lambda));
+}
+
+// glassFrame (dielectric.cpp:36-45): shading frame on the geometric normal's side.
+__device__ __forceinline__ Frame glass_frame(const Surface& si, V3 wo_world) {
+    const V3 outward = si.front_face ? si.normal : -si.normal;
+    const V3 ng = length_sq(si.ng) > 1e-12f ? normalized(si.ng) : normalized(outward);
+    V3 ns = length_sq(si.normal) > 1e-12f ? normalized(si.normal) : ng;
+    if (dot(ns, ng) < 0.0f) ns = -ns;
+    if (dot(wo_world, ns) * dot(wo_world, ng) <= 0.0f) ns = ng;
+    return frame_from_normal(ns);
+}
+
+// generalizedHalf (dielectric.cpp:55-67).
+__device__ __forceinline__ bool generalized_half(V3 wo, V3 wi, float ior, V3& wm, float& etap) {
+    const float cos_o = wo.z, cos_i = wi.z;
+    if (cos_o == 0.0f || cos_i == 0.0f) return false;
+    const bool refl = cos_o * cos_i > 0.0f;
+    etap = refl ? 1.0f : (cos_o > 0.0f ? ior : 1.0f / ior);
+    wm = wi * etap + wo;
+    if (length_sq(wm) <= 1e-12f) return false;
+    wm = normalized(wm);
+    if (wm.z < 0.0f) wm = -wm;
+    return !(dot(wm, wi) * cos_i <= 0.0f || dot(wm, wo) * cos_o <= 0.0f);
+}
+
+__device__ __forceinline__ V3 glass_eval_rough(V3 wo, V3 wi, float ior, float alpha, V3 tint) {  // :79-97
+    V3 wm;
+    float etap = 1.0f;
+    if (!generalized_half(wo, wi, ior, wm, etap)) return splat(0.0f);
+    const float cos_o = wo.z, cos_i = wi.z;
+    const float wo_h = dot(wo, wm), wi_h = dot(wi, wm);
+    const float F = fresnel_dielectric(wo_h, 1.0f, ior);
+    const float D = ggx_d(wm, alpha, alpha);
+    const float G = ggx_g(wo, wi, alpha, alpha);
+    if (cos_o * cos_i > 0.0f) return tint * (D * G * F / fabsf(4.0f * cos_i * cos_o));
+    const float s = wi_h + wo_h / etap;
+    const float denom = s * s * cos_i * cos_o;
+    if (fabsf(denom) <= 1e-12f) return splat(0.0f);
+    const float ft = D * (1.0f - F) * G * fabsf(wi_h * wo_h / denom);
+    return tint * (ft / (etap * etap));
+}
+
+__device__ __forceinline__ float glass_pdf_rough(V3 wo, V3 wi, float ior, float alpha) {  // :105-119
+    V3 wm;
+    float etap = 1.0f;
+    if (!generalized_half(wo, wi, ior, wm, etap)) return 0.0f;
+    const float wo_h = dot(wo, wm), wi_h = dot(wi, wm);
+    const float R = fresnel_dielectric(wo_h, 1.0f, ior);
+    const float visible = ggx_visible_pdf(wo, wm, alpha, alpha);
+    if (wo.z * wi.z > 0.0f) return visible / (4.0f * fabsf(wo_h)) * R;
+    const float s = wi_h + wo_h / etap;
+    if (s * s <= 1e-12f) return 0.0f;
+    return visible * fabsf(wi_h) / (s * s) * (1.0f - R);
+}
+
+// The thin wall's reflectance: a two-sided slab with incoherent inner reflections, 2R / (1 + R)
+// (thinWallReflectance, dielectric.cpp:153-161, no interference film).
+__device__ __forceinline__ float thin_wall_reflectance(const GpuMaterial& m, float cos_theta) {
+    cos_theta = clampf(fabsf(cos_theta), 0.0f, 1.0f);
+    const float R = fresnel_dielectric(cos_theta, 1.0f, m.ior);
+    return 2.0f * R / (1.0f + R);
+}
+
+// Dielectric::shadowTransmittance (dielectric.cpp:163-167) of a thin wall crossed in direction dir.
+__device__ __forceinline__ V3 thin_wall_transmittance(const GpuMaterial& m, const Surface& si, V3 dir) {
+    const V3 n = length_sq(si.ng) > 0.0f ? normalized(si.ng) : si.normal;
+    return (splat(1.0f) - splat(thin_wall_reflectance(m, dot(dir, n)))) * m.base_color;
+}
+
+// ------------------------------------------------------------------------------------ material dispatch
+// Material::evalPdf for the surface's material; world-space directions; lambda = the path's wavelength (0: RGB).
 __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
-                                            V3 wi, float& pdf) {
+                                            V3 wi, float lambda, float& pdf) {
     if (m.type == kMatLambert) {  // lambertian.cpp:38-60
         const Frame f = frame_from_normal(si.normal);
         const float ci = to_local(f, wi).z, co = to_local(f, wo).z;
@@ -301,22 +478,84 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
         pdf = ci * kInvPi;
         return m.base_color * kInvPi;
     }
+    if (m.type == kMatDielectric) {  // dielectric.cpp:262-292
+        if (m.flags & kMatFlagDelta) {  // smooth or thin: no finite BSDF value
+            pdf = 0.0f;
+            return splat(0.0f);
+        }
+        const Frame f = glass_frame(si, wo);
+        const V3 wol = to_local(f, wo), wil = to_local(f, wi);
+        const float ior = glass_ior(m, lambda);
+        pdf = glass_pdf_rough(wol, wil, ior, m.glass_alpha);
+        return glass_eval_rough(wol, wil, ior, m.glass_alpha, m.base_color);
+    }
     const Frame f = shading_frame(si.normal, si.tangent);
     const V3 wol = to_local(f, wo), wil = to_local(f, wi);
-    pdf = disney_pdf_local(m, wol, wil);
-    return disney_eval_local(m, T, wol, wil);
+    pdf = disney_pdf_local(m, si.front_face, wol, wil);
+    return disney_eval_local(m, T, si.front_face, wol, wil);
 }
 
 struct BsdfSample {
     V3 wi, f;
     float pdf;
-    bool delta;
+    bool delta, pass_through, subsurface;
+    V3 weight;  // subsurface: the throughput weight of entering the walk
 };
 
-// Material::sampleFull (disney.cpp:680-729; lambertian.cpp:8-35).
+// Dielectric::sampleWithLobe (dielectric.cpp:186-253).
+__device__ __forceinline__ bool glass_sample(const GpuMaterial& m, const Surface& si, V3 wo_world, float uc, float u0,
+                                             float u1, float lambda, BsdfSample& out) {
+    const Frame fr = glass_frame(si, wo_world);
+    const V3 wo = to_local(fr, wo_world);
+    if (wo.z == 0.0f) return false;
+    const bool entering = wo.z > 0.0f;
+    const float ior = glass_ior(m, lambda);
+    const float eta = entering ? (1.0f / ior) : ior;
+    if (m.roughness <= 1e-3f) {
+        const float F = fresnel_dielectric(wo.z, 1.0f, ior);
+        V3 wil = mk(-wo.x, -wo.y, wo.z);
+        V3 weight;
+        if (uc < F) {
+            out.pdf = F;
+            weight = m.base_color * F;
+        } else {
+            const V3 n = mk(0.0f, 0.0f, entering ? 1.0f : -1.0f);
+            if (!refract_vec(wo, n, eta, wil)) {
+                wil = mk(-wo.x, -wo.y, wo.z);
+                out.pdf = 1.0f;
+                weight = m.base_color;
+            } else {
+                out.pdf = 1.0f - F;
+                weight = m.base_color * ((1.0f - F) * eta * eta);
+            }
+        }
+        out.wi = normalized(to_world(fr, wil));
+        out.f = weight / fmaxf(fabsf(dot(out.wi, si.normal)), 1e-6f);
+        return true;
+    }
+    const V3 wm = ggx_sample_visible_normal(wo, m.glass_alpha, m.glass_alpha, u0, u1);
+    const float R = fresnel_dielectric(dot(wo, wm), 1.0f, ior);
+    V3 wil;
+    if (uc < R) {
+        wil = reflect_vec(wo, wm);
+        if (wil.z * wo.z <= 0.0f) return false;
+    } else {
+        const V3 n = entering ? wm : -wm;
+        if (!refract_vec(wo, n, eta, wil)) return false;
+        if (wil.z * wo.z >= 0.0f) return false;
+    }
+    out.f = glass_eval_rough(wo, wil, ior, m.glass_alpha, m.base_color);
+    out.pdf = glass_pdf_rough(wo, wil, ior, m.glass_alpha);
+    out.wi = normalized(to_world(fr, wil));
+    return out.pdf > 0.0f && !is_black(out.f);
+}
+
+// Material::sampleFull (disney.cpp:680-729; lambertian.cpp:8-35; dielectric.cpp:169-198).
 __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
-                                            float uc, float u0, float u1, BsdfSample& out) {
+                                            float uc, float u0, float u1, float lambda, BsdfSample& out) {
     out.delta = false;
+    out.pass_through = false;
+    out.subsurface = false;
     if (m.type == kMatLambert) {
         const Frame f = frame_from_normal(si.normal);
         if (to_local(f, wo).z <= 0.0f) return false;
@@ -325,6 +564,31 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         if (wil.z <= 0.0f) return false;
         out.f = m.base_color * kInvPi;
         out.pdf = wil.z * kInvPi;
+        return true;
+    }
+    if (m.type == kMatDielectric) {
+        if (!(m.flags & kMatFlagThinWall)) {
+            out.delta = (m.flags & kMatFlagDelta) != 0;
+            return glass_sample(m, si, wo, uc, u0, u1, lambda, out);
+        }
+        // Thin wall: reflect with the mean slab reflectance, else pass straight through (dielectric.cpp:176-197).
+        out.delta = true;
+        const V3 n = si.normal;
+        const float cos_o = dot(wo, n);
+        if (cos_o == 0.0f) return false;
+        const float R = thin_wall_reflectance(m, cos_o);
+        const float pr = clampf((R + R + R) / 3.0f, 1e-4f, 1.0f - 1e-4f);
+        const float c = fabsf(cos_o);
+        if (uc < pr) {
+            out.wi = normalized(n * (2.0f * cos_o) - wo);
+            out.pdf = pr;
+            out.f = m.base_color * R / c;
+        } else {
+            out.wi = -wo;
+            out.pdf = 1.0f - pr;
+            out.f = m.base_color * (1.0f - R) / c;
+            out.pass_through = true;
+        }
         return true;
     }
     const Frame f = shading_frame(si.normal, si.tangent);
@@ -337,20 +601,120 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
     } else if (uc < m.p_coat + m.p_spec) {
         wil = reflect_vec(wol, ggx_sample_visible_normal(wol, m.ax, m.ay, u0, u1));
         if (!(wil.z > 0.0f)) return false;
+    } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans) {  // sampleSpecTransLobe (disney.cpp:398-405)
+        const float alpha = ggx_clamp_alpha(m.roughness * m.roughness);
+        const V3 wm = ggx_sample_visible_normal(wol, alpha, alpha, u0, u1);
+        const float eta = si.front_face ? (1.0f / m.ior) : m.ior;
+        if (!refract_vec(wol, wm, eta, wil)) return false;
+        if (!(wil.z < 0.0f)) return false;
     } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans + m.p_diff_refl) {
-        // p_spec_trans is 0 on this path (rejected at upload), so this is the diffuse-reflection branch.
         wil = cosine_sample_hemisphere(u0, u1);
+    } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans + m.p_diff_refl + m.p_sss) {
+        // Subsurface lobe (disney.cpp:704-720): enter the random walk below the surface.
+        wil = cosine_sample_hemisphere(u0, u1);
+        wil.z = -wil.z;
+        const float coat_o = coat_transmit(m, T, wol.z);
+        const float sheen_o = sheen_scale(m, T, wol.z, 1.0f);
+        const float layer_o = layer_transmit(m, T, wol.z);
+        const float w = m.subsurface * m.opaque_dielectric * (1.0f - m.diffuse_transmission) * coat_o * sheen_o *
+                        layer_o / fmaxf(m.p_sss, 1e-6f);
+        out.wi = normalized(to_world(f, wil));
+        out.subsurface = true;
+        out.weight = m.coat_tint_factor * w;
+        out.pdf = m.p_sss * (-wil.z * kInvPi);
+        out.f = out.weight * out.pdf / fmaxf(-wil.z, 1e-6f);
+        return out.pdf > 0.0f;
     } else {
-        // Remaining probability: the diffuse-transmission lobe (weight 0 here), sampled below the surface,
-        // where the reflection-only BSDF is 0, so the sample ends the path as in the reference.
         wil = cosine_sample_hemisphere(u0, u1);
         wil.z = -wil.z;
     }
     out.wi = normalized(to_world(f, wil));
     const V3 wil2 = to_local(f, out.wi);
-    out.f = disney_eval_local(m, T, wol, wil2);
-    out.pdf = disney_pdf_local(m, wol, wil2);
+    out.f = disney_eval_local(m, T, si.front_face, wol, wil2);
+    out.pdf = disney_pdf_local(m, si.front_face, wol, wil2);
     return out.pdf > 0.0f;
+}
+
+// ------------------------------------------------------------------ wavelengths, media, subsurface walks
+
+// sampleWavelength (core/color/spectral.cpp:75-87): inverse CDF of the 350-bin wavelength table, returning the
+// wavelength and the RGB weight rgb(lambda) * norm / pdf(lambda) that keeps the estimator unbiased.
+__device__ __forceinline__ float sample_wavelength(const EnergyTables& T, float u, V3& weight) {
+    int lo = 0, count = kSpectralBins + 1;  // std::upper_bound over spec_cdf
+    while (count > 0) {
+        const int step = count >> 1;
+        const int it = lo + step;
+        if (!(u < T.spec_cdf[it])) {
+            lo = it + 1;
+            count -= step + 1;
+        } else {
+            count = step;
+        }
+    }
+    int i = lo - 1;
+    i = i < 0 ? 0 : (i > kSpectralBins - 1 ? kSpectralBins - 1 : i);
+    const float c0 = T.spec_cdf[i], c1 = T.spec_cdf[i + 1];
+    const float f = c1 > c0 ? clampf((u - c0) / (c1 - c0), 0.0f, 1.0f) : 0.5f;
+    const float bw = (kLambdaMax - kLambdaMin) / static_cast<float>(kSpectralBins);
+    const float lambda = kLambdaMin + (static_cast<float>(i) + f) * bw;
+    const V3 norm = mk(T.spec_norm[0], T.spec_norm[1], T.spec_norm[2]);
+    weight = (spectral_rgb_raw(lambda) * norm) / ((kLambdaMax - kLambdaMin) * T.spec_pdf[i]);
+    return lambda;
+}
+
+// sampleFreeFlight (integrators/path_tracer.cpp:97-128): distance sampling in a homogeneous medium with one
+// colour channel picked in proportion to `channel_w` and the spectral-MIS pdf over all three. True with the
+// scatter distance when the flight ends inside the medium (throughput *= sigma_s Tr / pdf); false when it
+// reaches the surface at t_max (throughput *= Tr / P) or leaves an open medium.
+__device__ __forceinline__ bool free_flight(V3 sigma_t, V3 sigma_s, float t_max, bool hit_surface, V3 channel_w,
+                                            SobolState& ss, V3& thr, float& t_scatter) {
+    float wsum = channel_w.x + channel_w.y + channel_w.z;
+    if (!(wsum > 0.0f)) {
+        channel_w = splat(1.0f);
+        wsum = 3.0f;
+    }
+    const V3 pc = channel_w / wsum;
+    const float uc = sobol_get1d(ss);
+    const int c = uc < pc.x ? 0 : (uc < pc.x + pc.y ? 1 : 2);
+    const float u = sobol_get1d(ss);
+    const float stc = comp(sigma_t, c);
+    const float t = stc > 0.0f ? -logf(fmaxf(1.0f - u, 1e-12f)) / stc : kFloatMax;
+    if (t < t_max) {
+        const V3 tr = mk(expf(-sigma_t.x * t), expf(-sigma_t.y * t), expf(-sigma_t.z * t));
+        const float pdf = pc.x * sigma_t.x * tr.x + pc.y * sigma_t.y * tr.y + pc.z * sigma_t.z * tr.z;
+        if (!(pdf > 0.0f)) {
+            thr = splat(0.0f);
+            return false;
+        }
+        thr *= sigma_s * tr / pdf;
+        t_scatter = t;
+        return true;
+    }
+    V3 tr;
+    if (hit_surface) {
+        tr = mk(expf(-sigma_t.x * t_max), expf(-sigma_t.y * t_max), expf(-sigma_t.z * t_max));
+    } else {
+        tr = mk(sigma_t.x > 0.0f ? 0.0f : 1.0f, sigma_t.y > 0.0f ? 0.0f : 1.0f, sigma_t.z > 0.0f ? 0.0f : 1.0f);
+    }
+    const float P = pc.x * tr.x + pc.y * tr.y + pc.z * tr.z;
+    thr = P > 0.0f ? thr * tr / P : splat(0.0f);
+    return false;
+}
+
+// Henyey-Greenstein direction about `d` (core/sampling/phase.h:27-39).
+__device__ __forceinline__ V3 sample_hg(V3 d, float g, float u0, float u1) {
+    float cos_t;
+    if (fabsf(g) < 1e-3f) {
+        cos_t = 1.0f - 2.0f * u0;
+    } else {
+        const float sq = (1.0f - g * g) / (1.0f - g + 2.0f * g * u0);
+        cos_t = (1.0f + g * g - sq * sq) / (2.0f * g);
+    }
+    cos_t = clampf(cos_t, -1.0f, 1.0f);
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = kTwoPi * u1;
+    const Frame f = frame_from_normal(d);
+    return normalized(to_world(f, mk(sin_t * cosf(phi), sin_t * sinf(phi), cos_t)));
 }
 
 }  // namespace pe

@@ -121,7 +121,8 @@ __device__ __forceinline__ uint32_t child_item(int32_t ref, int32_t count) {
 // Traverses every ray of a queue of n rays with the calling grid (all lanes of all waves must call it).
 //   job.fetch(i, a, b)            issue the loads of ray i's record (two float4)
 //   job.decode(a, b) -> TraceRay  decode a fetched record
-//   closest: job.closest(i, ray, t, tri, u, v)   t = tmax and tri = -1 on a miss
+//   closest: job.closest(i, ray, t, tri, u, v, a, b) -> bool   t = tmax and tri = -1 on a miss; true when the
+//            job continues the query with the new ray record it wrote into (a, b)
 //   any:     job.any(i, ray, occluded)
 // nv / tv accumulate the node and triangle items this lane handled (kCount only).
 template <bool kAny, bool kCount, class Job>
@@ -212,9 +213,17 @@ __device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const Bv
         }
         if (next == kNoItem && !occluded && st.sp > 0) next = st.pop();
         if (occluded || next == kNoItem) {
-            if constexpr (kAny) job.any(slot, ray, occluded);
-            else job.closest(slot, ray, ht, htri, hu, hv);
-            slot = kNoRay;
+            if constexpr (kAny) {
+                job.any(slot, ray, occluded);
+                slot = kNoRay;
+            } else if (job.closest(slot, ray, ht, htri, hu, hv, ra, rb)) {
+                // The job continues the query with a new ray in (ra, rb): restart at the root.
+                fresh = true;
+                item = 0;
+                st.sp = 0;
+            } else {
+                slot = kNoRay;
+            }
         } else {
             item = next;
         }

@@ -37,8 +37,31 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
         return PE_OK;
     }
     if (d.type == PE_MAT_DIELECTRIC) {
-        err = "Dielectric materials are not implemented on the gfx950 path yet";
-        return PE_ERR_UNSUPPORTED;
+        // Dielectric(ior, tint, roughness) + setAbbe / setAbsorption / setThinWalled (dielectric.h:18-78,
+        // dielectric.cpp:123-144). The interior medium only absorbs: the description carries no scattering.
+        g.type = kMatDielectric;
+        g.base_color = c3(d.base_color);
+        g.ior = d.ior;
+        g.roughness = d.roughness;
+        g.glass_alpha = ggx_clamp_alpha(d.roughness * d.roughness);
+        g.abbe = d.abbe > 0.0f ? std::fmax(d.abbe, 10.0f) : 0.0f;
+        const bool thin = d.thin_walled != 0;
+        if (thin || d.roughness <= 1e-3f) g.flags |= kMatFlagDelta;
+        if (thin) g.flags |= kMatFlagThinWall;
+        else g.flags |= kMatFlagVolume;
+        if (g.abbe > 0.0f) g.flags |= kMatFlagDispersive;
+        if (d.thin_film_thickness > 0.0f) {
+            err = "Dielectric thin-film interference is not implemented on the gfx950 path";
+            return PE_ERR_UNSUPPORTED;
+        }
+        if (!thin && d.absorption_distance > 0.0f) {
+            const V3 color = clamp01v(c3(d.absorption_color));
+            const float dist = std::fmax(d.absorption_distance, 1e-5f);
+            g.sigma_a = mk(-std::log(std::fmax(color.x, 1e-4f)) / dist, -std::log(std::fmax(color.y, 1e-4f)) / dist,
+                           -std::log(std::fmax(color.z, 1e-4f)) / dist);
+            if (!is_black(g.sigma_a)) g.flags |= kMatFlagMedium;
+        }
+        return PE_OK;
     }
     if (d.type != PE_MAT_DISNEY) {
         err = "unknown material type " + std::to_string(d.type);
@@ -66,10 +89,7 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
     g.diffuse_transmission = clampf(d.diffuse_transmission, 0.0f, 1.0f);
     g.subsurface = clampf(d.subsurface, 0.0f, 1.0f);
     g.edge_tint = clamp01v(c3(d.edge_tint));
-    if (g.specular_transmission > 0.0f || g.subsurface > 0.0f) {
-        err = "Disney specular transmission / subsurface lobes are not implemented on the gfx950 path yet";
-        return PE_ERR_UNSUPPORTED;
-    }
+    if (g.specular_transmission > 0.0f) g.flags |= kMatFlagVolume;  // isVolumeBoundary (disney.h:76)
 
     // resolvedSheenColor (disney.cpp:160-166)
     V3 sc = splat(0.0f);
@@ -125,6 +145,30 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
     g.opaque_dielectric = (1.0f - g.metallic) * (1.0f - clampf(g.specular_transmission, 0.0f, 1.0f));
     g.trans_w = clampf(g.diffuse_transmission, 0.0f, 1.0f) * g.opaque_dielectric;
     g.diff_w = g.opaque_dielectric * (1.0f - clampf(g.diffuse_transmission, 0.0f, 1.0f)) * (1.0f - g.subsurface);
+
+    // subsurfaceCoeffs (disney.cpp:738-757, Chiang et al. 2016 inversion): material-only without texture maps.
+    if (g.subsurface > 0.0f) {
+        const float gg = clampf(d.subsurface_anisotropy, -0.9f, 0.9f);
+        const V3 radius = clamp3(c3(d.subsurface_radius), 1e-4f, 1e4f);
+        const float scale = std::fmax(d.subsurface_scale, 1e-6f);
+        float st[3], alb[3];
+        for (int c = 0; c < 3; ++c) {
+            const float A = clampf(comp(g.base_color, c), 0.0f, 0.999f);
+            const float dd = std::fmax(comp(radius, c) * scale, 1e-7f);
+            const float alpha = 1.0f - std::exp(A * (-5.09406f + A * (2.61188f - 4.31805f * A)));
+            const float s_val = 1.9f - A + 3.5f * (A - 0.8f) * (A - 0.8f);
+            float sigma_t = 1.0f / std::fmax(dd * s_val, 1e-12f);
+            float sigma_s = alpha * sigma_t;
+            const float sigma_a = sigma_t - sigma_s;
+            sigma_s /= (1.0f - gg);
+            sigma_t = sigma_s + sigma_a;
+            st[c] = sigma_t;
+            alb[c] = sigma_t > 0.0f ? sigma_s / sigma_t : 0.0f;
+        }
+        g.sss_sigma_t = mk(st[0], st[1], st[2]);
+        g.sss_albedo = mk(alb[0], alb[1], alb[2]);
+        g.sss_g = gg;
+    }
     return PE_OK;
 }
 
@@ -345,15 +389,31 @@ int prepare_environment(pe_device* dev, const pe_env_desc& e, pe_scene& sc, EnvV
 int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     EnergyTables* T = new EnergyTables();
     build_energy_tables(*T);
-    std::vector<GpuMaterial> mats(static_cast<size_t>(std::max(1, desc.num_materials)));
-    for (int i = 0; i < desc.num_materials; ++i) {
+    // One slot past the scene's materials holds the white Lambert that a subsurface walk's exit vertex is
+    // shaded with (subsurfaceExitMaterial, path_tracer.cpp:56-59).
+    const int num_mats = std::max(0, desc.num_materials);
+    std::vector<GpuMaterial> mats(static_cast<size_t>(num_mats) + 1);
+    bool transmitters = false;
+    for (int i = 0; i <= num_mats; ++i) {
+        pe_material_desc exit_desc{};
+        if (i == num_mats) {
+            exit_desc.type = PE_MAT_LAMBERT;
+            exit_desc.base_color[0] = exit_desc.base_color[1] = exit_desc.base_color[2] = 1.0f;
+        }
+        const pe_material_desc& d = i < num_mats ? desc.materials[i] : exit_desc;
         std::string err;
-        const int rc = prepare_material(desc.materials[i], *T, mats[static_cast<size_t>(i)], err);
+        int rc = prepare_material(d, *T, mats[static_cast<size_t>(i)], err);
+        if (rc == PE_OK && (mats[static_cast<size_t>(i)].flags & kMatFlagVolume) && i > 0xFFFF) {
+            // A path's medium stack holds 16-bit material indices.
+            err = "volume-boundary material index above 65535";
+            rc = PE_ERR_UNSUPPORTED;
+        }
         if (rc != PE_OK) {
             delete T;
             set_error("material " + std::to_string(i) + ": " + err);
             return rc;
         }
+        transmitters = transmitters || (mats[static_cast<size_t>(i)].flags & kMatFlagThinWall) != 0;
     }
     std::vector<GpuLight> lights(static_cast<size_t>(std::max(1, desc.num_lights)));
     for (int i = 0; i < desc.num_lights; ++i) {
@@ -381,6 +441,8 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     SceneView& v = sc.view;
     v = SceneView{};
     v.num_lights = desc.num_lights;
+    v.exit_material = num_mats;
+    v.has_transmitters = transmitters ? 1 : 0;
     v.has_env = desc.environment ? 1 : 0;
     v.light_count = desc.num_lights + v.has_env;
     if (desc.environment) {

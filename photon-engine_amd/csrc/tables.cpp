@@ -11,9 +11,37 @@ namespace pe {
 
 namespace {
 float node_mu(int im) { return std::fmax(static_cast<float>(im) / (kTableN - 1), 1e-3f); }
+
+// spectral.cpp Tables (:34-62): channel means by a 3500-step midpoint rule, then a 350-bin CDF of r + g + b + 0.02.
+void build_spectral_tables(EnergyTables& T) {
+    constexpr float lmin = 380.0f, lmax = 730.0f;
+    V3 sum = splat(0.0f);
+    constexpr int kSteps = 3500;
+    const float dl = (lmax - lmin) / static_cast<float>(kSteps);
+    for (int i = 0; i < kSteps; ++i) sum = sum + spectral_rgb_raw(lmin + (static_cast<float>(i) + 0.5f) * dl);
+    const float inv_steps = 1.0f / static_cast<float>(kSteps);
+    const V3 mean = mk(sum.x * inv_steps, sum.y * inv_steps, sum.z * inv_steps);
+    const V3 norm = mk(1.0f / mean.x, 1.0f / mean.y, 1.0f / mean.z);
+    T.spec_norm[0] = norm.x;
+    T.spec_norm[1] = norm.y;
+    T.spec_norm[2] = norm.z;
+    const float bw = (lmax - lmin) / 350.0f;
+    float acc = 0.0f;
+    float s[350];
+    T.spec_cdf[0] = 0.0f;
+    for (int i = 0; i < 350; ++i) {
+        const V3 c = spectral_rgb_raw(lmin + (static_cast<float>(i) + 0.5f) * bw) * norm;
+        s[i] = c.x + c.y + c.z + 0.02f;
+        acc += s[i];
+        T.spec_cdf[i + 1] = acc;
+    }
+    for (int i = 0; i <= 350; ++i) T.spec_cdf[i] /= acc;
+    for (int i = 0; i < 350; ++i) T.spec_pdf[i] = s[i] / (acc * bw);
+}
 }  // namespace
 
 void build_energy_tables(EnergyTables& T) {
+    build_spectral_tables(T);
     constexpr int kS = 24;
     for (int ia = 0; ia < kTableN; ++ia) {
         const float alpha = static_cast<float>(ia) / (kTableN - 1);

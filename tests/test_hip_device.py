@@ -15,7 +15,7 @@ import pytest
 from oracle import bindings
 from photon_hip import camera_perspective, configs, render, settings
 from photon_hip import capi
-from photon_hip.scene import Mesh, Scene, lambert
+from photon_hip.scene import Mesh, Scene, disney, lambert
 
 
 def _harness_or_skip():
@@ -74,11 +74,10 @@ def test_session_through_hip_device_matches_direct_render(dev, name):
 def test_hip_device_refuses_what_it_does_not_implement(dev):
     _harness_or_skip()
     sc = Scene()
-    glass = capi.MaterialDesc()
-    glass.type = capi.PE_MAT_DIELECTRIC
-    glass.ior = 1.5
-    glass.base_color[:] = [1.0, 1.0, 1.0]
-    m = sc.add_material(glass)
+    film = disney((0.8, 0.8, 0.8), 0.0, 0.3, 0.5)
+    film.thin_film_thickness = 400.0
+    film.thin_film_ior = 1.3
+    m = sc.add_material(film)
     floor = sc.add_material(lambert((0.5, 0.5, 0.5)))
     quad = Mesh(np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32),
                 np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
@@ -87,4 +86,4 @@ def test_hip_device_refuses_what_it_does_not_implement(dev):
     _, view = configs.box_icosphere(level=1, width=8, height=8)
     rc, _, _, _, err = bindings.HipDeviceSession(sc).render(view, 1)
     assert rc == 1
-    assert "PE_ERR_UNSUPPORTED" in err or "not implemented" in err or "Dielectric" in err, err
+    assert "PE_ERR_UNSUPPORTED" in err or "not implemented" in err, err

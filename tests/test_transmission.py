@@ -158,3 +158,64 @@ def test_wavelength_weights_have_unit_mean():
     assert np.all(np.diff(lam) >= 0)
     np.testing.assert_allclose(w.mean(0), 1.0, rtol=2e-3)
     assert (w >= 0).all()
+
+
+# ------------------------------------------------------------------------------------------------- GPU
+
+def _gpu_image(dev, sc, view, spp, first=0, spl=1):
+    gs = dev.upload(sc)
+    film = dev.film(view.width, view.height)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=spl)
+    res = render(dev, gs, _cam(view), st, film, first, spp)
+    s, c, _, rej = film.download()
+    return s, c, rej, res
+
+
+def _gpu_per_sample(dev, sc, view, sample=3):
+    s, c, rej, res = _gpu_image(dev, sc, view, 1, first=sample)
+    assert res.completed == 1 and (c == 1).all() and rej == 0
+    yy, xx = np.mgrid[0:view.height, 0:view.width]
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    Lo, _ = bindings.OracleScene(sc).trace_samples(_cam(view), st, view.width, view.height, xx.ravel(), yy.ravel(),
+                                                   np.full(xx.size, sample))
+    Lg = s.reshape(-1, 3)
+    # Same Sobol samples, same paths: a sample differs only where a device transcendental (exp, log, sin) one
+    # ulp away from the host's flips a later decision.
+    close = np.all(np.abs(Lg - Lo) <= 1e-4 * np.maximum(1.0, np.abs(Lo)), axis=1)
+    return close.mean(), Lg, Lo
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", list(FEATURES))
+def test_gpu_radiance_matches_oracle_per_sample(dev, name):
+    frac, Lg, Lo = _gpu_per_sample(dev, feature_scene(name), VIEW)
+    assert frac >= 0.95, frac
+    assert Lo.max() > 0
+
+
+@pytest.mark.gpu
+def test_gpu_glass_studio_matches_oracle_per_sample(dev):
+    sc, view = configs.glass_studio(led_quads=100)
+    frac, _, _ = _gpu_per_sample(dev, sc, view)
+    assert frac >= 0.99, frac
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["glass_studio", "subsurface_block", "thin_pane", "absorbing_liquid"])
+def test_gpu_image_matches_oracle_relmse(dev, name):
+    if name == "glass_studio":
+        sc, view = configs.glass_studio(led_quads=100)
+    else:
+        sc, view = feature_scene(name), VIEW
+    # About 0.1% of the samples of these scenes take another path on the device (a caustic through glass after
+    # a one-ulp difference in exp/log/sin); the reference's own -ffast-math build departs from its IEEE build
+    # on the same samples at the same rate and by the same fireflies (relMSE 7.7e-3 at 32 spp, 1.6e-3 at 128 on
+    # the glass studio). The image check therefore runs where those flips have averaged out.
+    spp = 512
+    s, c, rej, res = _gpu_image(dev, sc, view, spp, spl=64)
+    assert (c == spp).all() and rej == 0
+    gpu = s / c[..., None]
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    (so, co, _), _, _ = bindings.OracleScene(sc).render(_cam(view), st, view.width, view.height, 0, spp, threads=8)
+    ora = so / co[..., None]
+    assert relmse(gpu, ora) <= 1e-3, relmse(gpu, ora)
