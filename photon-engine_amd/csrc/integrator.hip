@@ -119,6 +119,7 @@ struct LaunchParams {
     float clamp_indirect;
     uint64_t seed;
     int filter_box;
+    int transport;        // the scene needs the ext column (k_shade<true>)
     float filter_width, filter_radius;
     const float* filter_cdf;  // 513 entries when not a box
     V3 cam_pos, cam_ll, cam_h, cam_v;
@@ -190,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
         pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
-        pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (lp.transport) pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -351,6 +352,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, P
     }
 }
 
+// kTransport = false: the scene has no Dielectric and no transmitting or subsurface material, so no path
+// carries a wavelength, a medium, a pass-through count or a walk, and the ext column is neither read nor written.
+template <bool kTransport>
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
     const uint32_t n = pb.ctr[0];
     const float4* ro = pb.ro[cur];
@@ -409,7 +413,8 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             const float4 L4 = pb.L[pid];
             V3 L = mk(L4.x, L4.y, L4.z);
             const uint4 ids = pb.ids[pid];
-            uint4 ext = pb.ext[pid];
+            uint4 ext = make_uint4(0u, 0u, 0u, 0u);
+            if constexpr (kTransport) ext = pb.ext[pid];
             float lambda = __uint_as_float(ext.x);
             SobolState ss{ids.z, rev_bits(ids.y), get_dim(ids.w)};
             uint32_t depth = get_depth(ids.w), diffuse = get_diffuse(ids.w);
@@ -430,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             V3 wo = -dir;
             bool vertex = false;        // a surface vertex is shaded below
             bool exit_vertex = false;   // ... the walk's exit, a white Lambert facing out (path_tracer.cpp:758-763)
-            if (ext.w & kExtWalk) {
+            if (kTransport && (ext.w & kExtWalk)) {
                 // One step of the subsurface random walk (randomWalkSubsurface, path_tracer.cpp:263-292).
                 const GpuMaterial& wm = sc.materials[__float_as_uint(ln4.w)];
                 const float tmax = tri >= 0 ? h.x : kFloatMax;
@@ -473,7 +478,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             } else if (tri < 0) {
                 // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
                 // environment with MIS against NEE (:482-514).
-                const int top = ext_media_top(ext);
+                const int top = kTransport ? ext_media_top(ext) : -1;
                 const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
                 if (sc.has_env && !absorbed) {
                     const V3 le = env_eval(sc.env, dir);
@@ -498,7 +503,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     extend = true;
                 } else {
                     // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
-                    const int top = ext_media_top(ext);
+                    const int top = kTransport ? ext_media_top(ext) : -1;
                     if (top >= 0 && (sc.materials[top].flags & kMatFlagMedium)) {
                         const V3 sa = sc.materials[top].sigma_a;
                         const float seg = length(si.point - mk(o4.x, o4.y, o4.z));
@@ -534,7 +539,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     alive = false;
                 } else {
                     // Dispersion: the first wavelength-dependent surface picks the path's wavelength (:602-607).
-                    if (lambda == 0.0f && (m.flags & kMatFlagDispersive)) {
+                    if (kTransport && lambda == 0.0f && (m.flags & kMatFlagDispersive)) {
                         V3 w;
                         lambda = sample_wavelength(T, sobol_get1d(ss), w);
                         thr *= w;
@@ -569,7 +574,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                                     break;
                                 }
                                 float bsdf_pdf = 0.0f;
-                                const V3 f = bsdf_eval_pdf(m, T, si, wo, ls.wi, lambda, bsdf_pdf);
+                                const V3 f = bsdf_eval_pdf<kTransport>(m, T, si, wo, ls.wi, lambda, bsdf_pdf);
                                 const float cos_t = fabsf(dot(ls.wi, si.normal));
                                 if (is_black(f) || !(cos_t > 0.0f)) break;
                                 const float light_pdf = pmf * ls.pdf;
@@ -579,7 +584,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                                 uint32_t pend_bits = 0;
                                 if (diffuse >= 1) {
                                     // With thin walls the clamp waits for the shadow ray's transmittance.
-                                    if (sc.has_transmitters) pend_bits = kPendClamp;
+                                    if (kTransport && sc.has_transmitters) pend_bits = kPendClamp;
                                     else c = clamp_indirect(c, lp.clamp_indirect);
                                 }
                                 if (!finite3(c)) break;
@@ -602,9 +607,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     float u0, u1;
                     sobol_get2d(ss, u0, u1);
                     BsdfSample bs;
-                    if (!bsdf_sample(m, T, si, wo, uc, u0, u1, lambda, bs)) {
+                    if (!bsdf_sample<kTransport>(m, T, si, wo, uc, u0, u1, lambda, bs)) {
                         alive = false;
-                    } else if (bs.subsurface) {
+                    } else if (kTransport && bs.subsurface) {
                         thr *= bs.weight;
                         new_d = normalized(bs.wi);
                         new_o = spawn_origin(si, new_d);
@@ -613,7 +618,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                         extend = true;
                     } else if (bs.pdf <= 0.0f || is_black(bs.f)) {
                         alive = false;
-                    } else if (bs.pass_through) {
+                    } else if (kTransport && bs.pass_through) {
                         thr *= bs.f * (fabsf(dot(bs.wi, si.normal)) / bs.pdf);
                         const uint32_t passes = ext_pass_throughs(ext.w) + 1;
                         if (!finite3(thr) || passes > 64) {
@@ -636,7 +641,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                             specular = bs.delta;
                             if (!bs.delta) ++diffuse;
                             camera = camera && bs.delta && dot(bs.wi, si.normal) < 0.0f;
-                            if (m.flags & kMatFlagVolume) {  // medium stack (path_tracer.cpp:840-848)
+                            if (kTransport && (m.flags & kMatFlagVolume)) {  // medium stack (path_tracer.cpp:840-848)
                                 const float a = dot(wo, si.ng), b = dot(bs.wi, si.ng);
                                 if (a * b < 0.0f) {
                                     if (a > 0.0f) ext_media_enter(ext, mat);
@@ -667,7 +672,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             if (extend) {
                 pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
-                pb.ext[pid] = ext;
+                if constexpr (kTransport) pb.ext[pid] = ext;
             }
         }
         const uint32_t slot = wave_append(&pb.ctr[1], extend);
@@ -952,6 +957,7 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
         lp.filter_width = std::clamp(settings->filter_width, 0.25f, 8.0f);
         lp.filter_radius = 0.5f * lp.filter_width;
         lp.filter_box = settings->filter_type == PE_FILTER_BOX ? 1 : 0;
+        lp.transport = scene->view.transport;
         if (!lp.filter_box) {
             if (wf->filter_type != settings->filter_type || wf->filter_width != settings->filter_width) {
                 const std::vector<float> cdf = filter_cdf(settings->filter_type, settings->filter_width);
@@ -986,7 +992,8 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
             else hipLaunchKernelGGL(k_intersect<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
             wf->prof.end(s);
             wf->prof.begin(kKShade, s);
-            hipLaunchKernelGGL(k_shade, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+            if (lp.transport) hipLaunchKernelGGL(k_shade<true>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+            else hipLaunchKernelGGL(k_shade<false>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
             wf->prof.end(s);
             wf->prof.begin(kKShadow, s);
             if (view.has_transmitters) {

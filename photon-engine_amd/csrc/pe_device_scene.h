@@ -143,6 +143,7 @@ struct SceneView {
     int32_t pick_mode;       // kPick*
     int32_t exit_material;   // white Lambert shading a subsurface walk's exit (one past the scene's materials)
     int32_t has_transmitters;  // a thin-walled surface passes shadow rays on with its transmittance
+    int32_t transport;         // a Dielectric or a transmitting / subsurface material is present
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;
     const LightPath* light_paths;  // light_count entries

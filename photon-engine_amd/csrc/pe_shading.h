@@ -309,12 +309,15 @@ __device__ __forceinline__ float pdf_spec_trans(const GpuMaterial& m, bool front
 }
 
 // evalLobes (disney.cpp:223-311) in the local frame: reflection above the surface; diffuse and specular
-// transmission below it.
+// transmission below it. kTransport = false is the variant for scenes whose materials all have zero
+// transmission weights, where the lobes below the surface are exactly zero.
+template <bool kTransport>
 __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, bool front, V3 wo, V3 wi) {
     if (wo.z <= 0.0f) return splat(0.0f);
     const float cos_o = wo.z;
     if (wi.z <= 0.0f) {
         V3 ft = splat(0.0f);
+        if (!kTransport) return ft;
         if (m.trans_w <= 0.0f && m.st_w <= 0.0f) return ft;
         const float coat_o = coat_transmit(m, T, cos_o);
         if (m.trans_w > 0.0f) {
@@ -339,7 +342,7 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
     // specularFresnel (disney.cpp:79-96) without thin film
     V3 F = splat(m.wd * schlick_f(m.f0d, cos_d));
     if (m.metallic > 0.0f) F = lerp3(F, f82_tint(m.base_color, m.edge_tint, cos_d), m.metallic);
-    if (m.st_w > 0.0f && !front) F = lerp3(F, splat(fresnel_dielectric(-cos_d, 1.0f, m.ior)), m.st_w);
+    if (kTransport && m.st_w > 0.0f && !front) F = lerp3(F, splat(fresnel_dielectric(-cos_d, 1.0f, m.ior)), m.st_w);
     // multiScatterFactor (disney.cpp:99-103)
     const float E = ggx_directional_albedo(T, cos_o, sqrtf(m.ax * m.ay));
     const V3 f0 = lerp3(splat(m.f0d), m.base_color, m.metallic);
@@ -364,10 +367,12 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
 }
 
 // pdfLobes (disney.cpp:340-378).
+template <bool kTransport>
 __device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, bool front, V3 wo, V3 wi) {
     if (wo.z <= 0.0f) return 0.0f;
     if (wi.z <= 0.0f) {
         float pdf = 0.0f;
+        if (!kTransport) return pdf;
         if (m.p_trans > 0.0f) pdf += m.p_trans * (-wi.z * kInvPi);
         if (m.p_spec_trans > 0.0f) pdf += m.p_spec_trans * pdf_spec_trans(m, front, wo, wi);
         return pdf;
@@ -466,6 +471,8 @@ __device__ __forceinline__ V3 thin_wall_transmittance(const GpuMaterial& m, cons
 
 // ------------------------------------------------------------------------------------ material dispatch
 // Material::evalPdf for the surface's material; world-space directions; lambda = the path's wavelength (0: RGB).
+// kTransport = false: the scene has no Dielectric and no transmitting or subsurface Disney material.
+template <bool kTransport>
 __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
                                             V3 wi, float lambda, float& pdf) {
     if (m.type == kMatLambert) {  // lambertian.cpp:38-60
@@ -478,7 +485,7 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
         pdf = ci * kInvPi;
         return m.base_color * kInvPi;
     }
-    if (m.type == kMatDielectric) {  // dielectric.cpp:262-292
+    if (kTransport && m.type == kMatDielectric) {  // dielectric.cpp:262-292
         if (m.flags & kMatFlagDelta) {  // smooth or thin: no finite BSDF value
             pdf = 0.0f;
             return splat(0.0f);
@@ -491,8 +498,8 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
     }
     const Frame f = shading_frame(si.normal, si.tangent);
     const V3 wol = to_local(f, wo), wil = to_local(f, wi);
-    pdf = disney_pdf_local(m, si.front_face, wol, wil);
-    return disney_eval_local(m, T, si.front_face, wol, wil);
+    pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil);
+    return disney_eval_local<kTransport>(m, T, si.front_face, wol, wil);
 }
 
 struct BsdfSample {
@@ -550,7 +557,9 @@ __device__ __forceinline__ bool glass_sample(const GpuMaterial& m, const Surface
     return out.pdf > 0.0f && !is_black(out.f);
 }
 
-// Material::sampleFull (disney.cpp:680-729; lambertian.cpp:8-35; dielectric.cpp:169-198).
+// Material::sampleFull (disney.cpp:680-729; lambertian.cpp:8-35; dielectric.cpp:169-198). With kTransport = false
+// the specular-transmission and subsurface lobes have zero probability and are left out.
+template <bool kTransport>
 __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
                                             float uc, float u0, float u1, float lambda, BsdfSample& out) {
     out.delta = false;
@@ -566,7 +575,7 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         out.pdf = wil.z * kInvPi;
         return true;
     }
-    if (m.type == kMatDielectric) {
+    if (kTransport && m.type == kMatDielectric) {
         if (!(m.flags & kMatFlagThinWall)) {
             out.delta = (m.flags & kMatFlagDelta) != 0;
             return glass_sample(m, si, wo, uc, u0, u1, lambda, out);
@@ -601,7 +610,7 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
     } else if (uc < m.p_coat + m.p_spec) {
         wil = reflect_vec(wol, ggx_sample_visible_normal(wol, m.ax, m.ay, u0, u1));
         if (!(wil.z > 0.0f)) return false;
-    } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans) {  // sampleSpecTransLobe (disney.cpp:398-405)
+    } else if (kTransport && uc < m.p_coat + m.p_spec + m.p_spec_trans) {  // sampleSpecTransLobe (disney.cpp:398-405)
         const float alpha = ggx_clamp_alpha(m.roughness * m.roughness);
         const V3 wm = ggx_sample_visible_normal(wol, alpha, alpha, u0, u1);
         const float eta = si.front_face ? (1.0f / m.ior) : m.ior;
@@ -609,7 +618,7 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         if (!(wil.z < 0.0f)) return false;
     } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans + m.p_diff_refl) {
         wil = cosine_sample_hemisphere(u0, u1);
-    } else if (uc < m.p_coat + m.p_spec + m.p_spec_trans + m.p_diff_refl + m.p_sss) {
+    } else if (kTransport && uc < m.p_coat + m.p_spec + m.p_spec_trans + m.p_diff_refl + m.p_sss) {
         // Subsurface lobe (disney.cpp:704-720): enter the random walk below the surface.
         wil = cosine_sample_hemisphere(u0, u1);
         wil.z = -wil.z;
@@ -630,8 +639,8 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
     }
     out.wi = normalized(to_world(f, wil));
     const V3 wil2 = to_local(f, out.wi);
-    out.f = disney_eval_local(m, T, si.front_face, wol, wil2);
-    out.pdf = disney_pdf_local(m, si.front_face, wol, wil2);
+    out.f = disney_eval_local<kTransport>(m, T, si.front_face, wol, wil2);
+    out.pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil2);
     return out.pdf > 0.0f;
 }
 

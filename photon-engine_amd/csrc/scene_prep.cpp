@@ -393,7 +393,7 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     // shaded with (subsurfaceExitMaterial, path_tracer.cpp:56-59).
     const int num_mats = std::max(0, desc.num_materials);
     std::vector<GpuMaterial> mats(static_cast<size_t>(num_mats) + 1);
-    bool transmitters = false;
+    bool transmitters = false, transport = false;
     for (int i = 0; i <= num_mats; ++i) {
         pe_material_desc exit_desc{};
         if (i == num_mats) {
@@ -414,6 +414,10 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             return rc;
         }
         transmitters = transmitters || (mats[static_cast<size_t>(i)].flags & kMatFlagThinWall) != 0;
+        const GpuMaterial& g = mats[static_cast<size_t>(i)];
+        transport = transport || g.type == kMatDielectric || g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f ||
+                    g.p_spec_trans > 0.0f || g.p_sss > 0.0f ||
+                    (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium)) != 0;
     }
     std::vector<GpuLight> lights(static_cast<size_t>(std::max(1, desc.num_lights)));
     for (int i = 0; i < desc.num_lights; ++i) {
@@ -443,6 +447,7 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     v.num_lights = desc.num_lights;
     v.exit_material = num_mats;
     v.has_transmitters = transmitters ? 1 : 0;
+    v.transport = transport ? 1 : 0;
     v.has_env = desc.environment ? 1 : 0;
     v.light_count = desc.num_lights + v.has_env;
     if (desc.environment) {
