@@ -155,6 +155,13 @@ typedef struct pe_scene_desc {
 #define PE_UPDATE_FULL 0         /* SceneUpdateKind::Full (render_device.h:33) */
 #define PE_UPDATE_MATERIALS 1    /* SceneUpdateKind::Materials */
 #define PE_UPDATE_LIGHTS 2       /* SceneUpdateKind::Lights */
+#define PE_UPDATE_TRANSFORMS 3   /* SceneUpdateKind::Transforms: BVH rebuilt, like Full */
+/* RenderDevice::updateScene (render_device.h:77-79; VulkanDevice::updateScene, vulkan_device.cpp:1223-1256).
+ * `desc` is the full updated scene. Materials rebuilds the material table and Lights the lights, environment
+ * and light-selection structure, both keeping the BVH and triangle data resident; the meshes' material and
+ * emitter references must stay in range (PE_ERR_INVALID otherwise, scene unchanged). Full and Transforms
+ * rebuild everything and replace the scene only on success. A partial update that fails for another reason
+ * (PE_ERR_OOM, PE_ERR_HIP) leaves the scene unusable until the next Full update. */
 
 int pe_scene_upload(pe_device* dev, const pe_scene_desc* desc, pe_scene** out);
 int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc);

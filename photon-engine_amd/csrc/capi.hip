@@ -285,6 +285,58 @@ int pe_scene_upload(pe_device* dev, const pe_scene_desc* desc, pe_scene** out) {
     PE_GUARD_END
 }
 
+// RenderDevice::updateScene (render_device.h:77-79) and the Vulkan device's partial rebuilds
+// (rebuildMaterials / rebuildLights): the BVH and the triangle data stay resident; only the material table or
+// the lights, environment and light-selection structure are rebuilt from `desc`, the full updated description.
+// Full and Transforms rebuild everything into a new set of buffers that replaces the old one on success.
+int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
+    PE_GUARD_BEGIN
+    if (!scene || !desc) {
+        set_error("pe_scene_update: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    pe_device* dev = scene->dev;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    // Renders already queued on the device's stream finish with the old data.
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    if (kind == PE_UPDATE_FULL || kind == PE_UPDATE_TRANSFORMS) {
+        auto fresh = std::make_unique<pe_scene>();
+        fresh->dev = dev;
+        const int rc = upload_scene(dev, *desc, *fresh);
+        if (rc != PE_OK) return rc;  // the old scene is untouched
+        PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+        std::swap(*scene, *fresh);
+        return PE_OK;
+    }
+    if (kind != PE_UPDATE_MATERIALS && kind != PE_UPDATE_LIGHTS) {
+        set_error("pe_scene_update: unknown update kind " + std::to_string(kind));
+        return PE_ERR_INVALID;
+    }
+    if (scene->broken) {
+        set_error("pe_scene_update: an earlier update failed; upload the scene again");
+        return PE_ERR_INVALID;
+    }
+    int rc = PE_OK;
+    if (kind == PE_UPDATE_MATERIALS) {
+        if (desc->num_materials <= scene->max_material_ref) {
+            set_error("pe_scene_update: a mesh uses material " + std::to_string(scene->max_material_ref) +
+                      " but the update has " + std::to_string(desc->num_materials));
+            return PE_ERR_INVALID;
+        }
+        rc = prepare_materials(dev, *desc, *scene);
+    } else {
+        if (desc->num_lights <= scene->max_emitter_ref) {
+            set_error("pe_scene_update: a mesh emits as light " + std::to_string(scene->max_emitter_ref) +
+                      " but the update has " + std::to_string(desc->num_lights));
+            return PE_ERR_INVALID;
+        }
+        rc = prepare_lights(dev, *desc, *scene);
+    }
+    if (rc != PE_OK) scene->broken = true;
+    return rc;
+    PE_GUARD_END
+}
+
 int pe_scene_release(pe_scene* scene) {
     if (!scene) return PE_OK;
     (void)hipSetDevice(scene->dev->ordinal);

@@ -79,6 +79,9 @@ struct pe_scene {
         light_nodes, light_paths;
     pe::SceneView view{};
     uint64_t num_nodes = 0;
+    int max_material_ref = -1;  // largest material index a mesh uses
+    int max_emitter_ref = -1;   // largest light index a mesh's emitter range reaches
+    bool broken = false;        // an update failed part-way: the scene must be uploaded again
     uint64_t device_bytes() const {
         return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes() +
                materials.bytes() + lights.bytes() + tables.bytes() + env_rgb.bytes() + env_func.bytes() +

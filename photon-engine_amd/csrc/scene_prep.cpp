@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -386,12 +387,18 @@ int prepare_environment(pe_device* dev, const pe_env_desc& e, pe_scene& sc, EnvV
     return PE_OK;
 }
 
-int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
-    EnergyTables* T = new EnergyTables();
+// Material table (the scene's materials, then the walk-exit Lambert) and the energy tables; the view's material
+// fields. Everything is validated before anything is uploaded, so a refused update leaves the scene as it was.
+int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
+    if (desc.num_materials < 0 || (desc.num_materials > 0 && !desc.materials)) {
+        set_error("scene: bad material list");
+        return PE_ERR_INVALID;
+    }
+    std::unique_ptr<EnergyTables> T(new EnergyTables());
     build_energy_tables(*T);
     // One slot past the scene's materials holds the white Lambert that a subsurface walk's exit vertex is
     // shaded with (subsurfaceExitMaterial, path_tracer.cpp:56-59).
-    const int num_mats = std::max(0, desc.num_materials);
+    const int num_mats = desc.num_materials;
     std::vector<GpuMaterial> mats(static_cast<size_t>(num_mats) + 1);
     bool transmitters = false, transport = false;
     for (int i = 0; i <= num_mats; ++i) {
@@ -409,80 +416,106 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             rc = PE_ERR_UNSUPPORTED;
         }
         if (rc != PE_OK) {
-            delete T;
             set_error("material " + std::to_string(i) + ": " + err);
             return rc;
         }
-        transmitters = transmitters || (mats[static_cast<size_t>(i)].flags & kMatFlagThinWall) != 0;
         const GpuMaterial& g = mats[static_cast<size_t>(i)];
+        transmitters = transmitters || (g.flags & kMatFlagThinWall) != 0;
         transport = transport || g.type == kMatDielectric || g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f ||
                     g.p_spec_trans > 0.0f || g.p_sss > 0.0f ||
                     (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium)) != 0;
+    }
+    hipStream_t s = dev->stream;
+    if (!sc.materials.upload(mats.data(), mats.size() * sizeof(GpuMaterial), s) ||
+        !sc.tables.upload(T.get(), sizeof(EnergyTables), s))
+        return PE_ERR_OOM;
+    const hipError_t er = hipStreamSynchronize(s);  // the host copies die at return
+    if (er != hipSuccess) {
+        set_error(std::string("material upload: ") + hipGetErrorString(er));
+        return PE_ERR_HIP;
+    }
+    SceneView& v = sc.view;
+    v.materials = sc.materials.as<GpuMaterial>();
+    v.tables = sc.tables.as<EnergyTables>();
+    v.exit_material = num_mats;
+    v.has_transmitters = transmitters ? 1 : 0;
+    v.transport = transport ? 1 : 0;
+    return PE_OK;
+}
+
+// Explicit lights, the environment and the light-selection structure; the view's light fields.
+int prepare_lights(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
+    if (desc.num_lights < 0 || (desc.num_lights > 0 && !desc.lights)) {
+        set_error("scene: bad light list");
+        return PE_ERR_INVALID;
     }
     std::vector<GpuLight> lights(static_cast<size_t>(std::max(1, desc.num_lights)));
     for (int i = 0; i < desc.num_lights; ++i) {
         const pe_light_desc& L = desc.lights[i];
         if (L.type != PE_LIGHT_AREA && L.type != PE_LIGHT_TRIANGLE) {
-            delete T;
             set_error("light " + std::to_string(i) + ": unknown type " + std::to_string(L.type));
             return PE_ERR_INVALID;
         }
         lights[static_cast<size_t>(i)] = prepare_light(L);
     }
-    hipStream_t s = dev->stream;
-    if (!sc.materials.upload(mats.data(), mats.size() * sizeof(GpuMaterial), s) ||
-        !sc.lights.upload(lights.data(), lights.size() * sizeof(GpuLight), s) ||
-        !sc.tables.upload(T, sizeof(EnergyTables), s)) {
-        delete T;
-        return PE_ERR_OOM;
-    }
-    const hipError_t er = hipStreamSynchronize(s);
-    delete T;
-    if (er != hipSuccess) {
-        set_error(std::string("shading upload: ") + hipGetErrorString(er));
-        return PE_ERR_HIP;
-    }
-    SceneView& v = sc.view;
-    v = SceneView{};
-    v.num_lights = desc.num_lights;
-    v.exit_material = num_mats;
-    v.has_transmitters = transmitters ? 1 : 0;
-    v.transport = transport ? 1 : 0;
-    v.has_env = desc.environment ? 1 : 0;
-    v.light_count = desc.num_lights + v.has_env;
-    if (desc.environment) {
-        const int rc = prepare_environment(dev, *desc.environment, sc, v.env);
-        if (rc != PE_OK) return rc;
-    }
     // LightSampler::build (light_sampler.cpp:42-77): up to 33 candidates are weighed exhaustively, more go into
     // the light BVH with the unbounded ones on a top level.
+    const bool has_env = desc.environment != nullptr;
     LightBvh lb;
-    build_light_bvh(lights, desc.num_lights, v.has_env != 0, lb);
-    v.pick_mode = lb.mode;
+    build_light_bvh(lights, desc.num_lights, has_env, lb);
+    hipStream_t s = dev->stream;
+    if (!sc.lights.upload(lights.data(), lights.size() * sizeof(GpuLight), s)) return PE_ERR_OOM;
+    EnvView env{};
+    if (has_env) {
+        const int rc = prepare_environment(dev, *desc.environment, sc, env);
+        if (rc != PE_OK) return rc;
+    } else {
+        for (DevBuf* b : {&sc.env_rgb, &sc.env_func, &sc.env_marginal, &sc.env_cond, &sc.env_mguide, &sc.env_cguide})
+            b->release();
+    }
     if (lb.mode == kPickBvh) {
         if (!lb.nodes.empty() && !sc.light_nodes.upload(lb.nodes.data(), lb.nodes.size() * sizeof(GpuLightNode), s))
             return PE_ERR_OOM;
+        if (lb.nodes.empty()) sc.light_nodes.release();
         if (!sc.light_paths.upload(lb.paths.data(), lb.paths.size() * sizeof(LightPath), s)) return PE_ERR_OOM;
-        const hipError_t el = hipStreamSynchronize(s);
-        if (el != hipSuccess) {
-            set_error(std::string("light BVH upload: ") + hipGetErrorString(el));
-            return PE_ERR_HIP;
-        }
-        v.light_nodes = sc.light_nodes.as<GpuLightNode>();
-        v.light_paths = sc.light_paths.as<LightPath>();
-        v.num_light_nodes = static_cast<int32_t>(lb.nodes.size());
-        v.num_top = static_cast<int32_t>(lb.top.size());
-        for (size_t k = 0; k < lb.top.size(); ++k) v.top[k] = lb.top[k];
+    } else {
+        sc.light_nodes.release();
+        sc.light_paths.release();
     }
+    const hipError_t el = hipStreamSynchronize(s);
+    if (el != hipSuccess) {
+        set_error(std::string("light upload: ") + hipGetErrorString(el));
+        return PE_ERR_HIP;
+    }
+    SceneView& v = sc.view;
+    v.lights = sc.lights.as<GpuLight>();
+    v.num_lights = desc.num_lights;
+    v.has_env = has_env ? 1 : 0;
+    v.light_count = desc.num_lights + v.has_env;
+    v.env = env;
+    v.pick_mode = lb.mode;
+    v.light_nodes = sc.light_nodes.as<GpuLightNode>();
+    v.light_paths = sc.light_paths.as<LightPath>();
+    v.num_light_nodes = lb.mode == kPickBvh ? static_cast<int32_t>(lb.nodes.size()) : 0;
+    v.num_top = lb.mode == kPickBvh ? static_cast<int32_t>(lb.top.size()) : 0;
+    for (int k = 0; k < kMaxTopCandidates; ++k)
+        v.top[k] = (lb.mode == kPickBvh && k < static_cast<int>(lb.top.size())) ? lb.top[static_cast<size_t>(k)] : 0;
+    return PE_OK;
+}
+
+int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
+    sc.view = SceneView{};
+    int rc = prepare_materials(dev, desc, sc);
+    if (rc != PE_OK) return rc;
+    rc = prepare_lights(dev, desc, sc);
+    if (rc != PE_OK) return rc;
+    SceneView& v = sc.view;
     v.nodes = sc.nodes.as<BvhNode>();
     v.tris = sc.tris.as<TriAccel>();
     v.shade = sc.tri_shade.as<TriShade>();
     v.pos = sc.positions.as<float4>();
     v.nrm = sc.normals.as<float4>();
     v.uv = sc.uvs.as<float2>();
-    v.materials = sc.materials.as<GpuMaterial>();
-    v.lights = sc.lights.as<GpuLight>();
-    v.tables = sc.tables.as<EnergyTables>();
     return PE_OK;
 }
 

@@ -66,6 +66,12 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         }
         nv += m.num_vertices;
         nt += m.num_triangles;
+        sc.max_material_ref = std::max(sc.max_material_ref, m.material);
+        if (m.emitter_base >= 0) {
+            const uint64_t last = m.standalone || m.num_triangles == 0 ? static_cast<uint64_t>(m.emitter_base)
+                                                                        : m.emitter_base + m.num_triangles - 1;
+            sc.max_emitter_ref = std::max(sc.max_emitter_ref, static_cast<int>(last));
+        }
     }
     if (nt == 0) {
         set_error("scene: no triangles");

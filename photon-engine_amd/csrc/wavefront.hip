@@ -127,15 +127,4 @@ int pe_film_download(pe_film* film, float* sum_rgb, uint32_t* count, float* sum_
     return PE_OK;
 }
 
-int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
-    (void)kind;
-    (void)desc;
-    if (!scene) {
-        set_error("pe_scene_update: NULL scene");
-        return PE_ERR_INVALID;
-    }
-    set_error("pe_scene_update: not implemented yet");
-    return PE_ERR_UNSUPPORTED;
-}
-
 }  // extern "C"

@@ -26,7 +26,7 @@ PE_MAT_DISNEY, PE_MAT_LAMBERT, PE_MAT_DIELECTRIC = 0, 1, 2
 PE_LIGHT_AREA, PE_LIGHT_TRIANGLE = 1, 2
 PE_SHAPE_CAMERA_INVISIBLE = 1
 PE_FILTER_BOX, PE_FILTER_TRIANGLE, PE_FILTER_GAUSSIAN, PE_FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3
-PE_UPDATE_FULL, PE_UPDATE_MATERIALS, PE_UPDATE_LIGHTS = 0, 1, 2
+PE_UPDATE_FULL, PE_UPDATE_MATERIALS, PE_UPDATE_LIGHTS, PE_UPDATE_TRANSFORMS = 0, 1, 2, 3
 
 F3 = C.c_float * 3
 

@@ -84,6 +84,13 @@ class GpuScene:
         del keep
         self.h = h
 
+    def update(self, scene, kind: int) -> int:
+        """pe_scene_update with the full updated scene; returns the status code (no exception)."""
+        desc, keep = scene.pack()
+        rc = self._lib.pe_scene_update(self.h, int(kind), C.byref(desc))
+        del keep
+        return rc
+
     def close(self):
         if getattr(self, "h", None):
             self._lib.pe_scene_release(self.h)
