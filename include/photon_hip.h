@@ -34,6 +34,7 @@ extern "C" {
 #define PE_ERR_OOM (-4)          /* device memory allocation failed */
 #define PE_ERR_UNSUPPORTED (-5)  /* feature not implemented on this device */
 #define PE_ERR_CANCELLED (-6)    /* render stopped by the cancel flag (partial result is valid) */
+#define PE_ERR_TIMEOUT (-7)      /* pe_job_wait: the job has not ended within the timeout */
 
 typedef struct pe_device pe_device;
 typedef struct pe_scene pe_scene;
@@ -256,6 +257,70 @@ int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, c
 int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary);
 /* Any hit (shadow rays): occluded[i] = 1 when something lies within [tmin, tmax]. */
 int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded);
+
+/* ---- serving: continuous batching of render jobs ----------------------------------------------------
+ * No single counterpart in the reference: RenderSession (engine/render_session.cpp:69-101) drives one camera
+ * and one film through RenderDevice::render in batches; a server drives many render jobs of different scenes,
+ * film sizes and sample counts through the same path at once. It owns `slots` render lanes, each a HIP stream
+ * with its own path state. A submitted job waits until its film fits the film budget and the admission cap,
+ * then holds a device film; every free lane takes the next chunk (a sample range of every pixel) from the
+ * admitted jobs, jobs short of their preview first, then in submission order, so chunks of different jobs
+ * run on the device together and a job that arrives while others render starts as soon as a lane frees up.
+ * A job's chunks run one after another in sample order and their boundaries depend on the job and the
+ * server configuration only, so a job's film is the same bit for bit whether it ran alone or among others.
+ * Scenes are uploaded by the caller (pe_scene_upload) and stay resident across jobs. */
+typedef struct pe_server pe_server;
+
+typedef struct pe_server_config {
+    int32_t slots;              /* render lanes (streams) working at once; 0 = 4 */
+    int32_t chunk_spp;          /* samples of every pixel per chunk after the preview; 0 = 16 */
+    uint64_t max_chunk_paths;   /* paths one chunk keeps in flight (splits chunks of large films); 0 = 2^25 */
+    uint64_t film_budget;       /* device bytes for the films of admitted jobs; 0 = an eighth of free memory */
+    int32_t max_admitted;       /* jobs holding a film at once; 0 = 64 */
+    int32_t reserved[5];
+} pe_server_config;
+
+typedef struct pe_job_desc {
+    pe_scene* scene;            /* an uploaded scene; it must outlive the job's last chunk */
+    pe_camera camera;
+    pe_settings settings;       /* samples_per_launch is ignored: a chunk is one launch */
+    int32_t width, height;      /* film size */
+    int32_t spp;                /* samples of every pixel to render */
+    int32_t preview_spp;        /* the first milestone, 0 = 16 (at most spp) */
+    int32_t reserved[4];
+} pe_job_desc;
+
+#define PE_JOB_QUEUED 0         /* submitted; waiting for film memory or an admission slot */
+#define PE_JOB_ADMITTED 1       /* holds a film; its chunks interleave with other admitted jobs */
+#define PE_JOB_DONE 2
+#define PE_JOB_CANCELLED 3
+#define PE_JOB_FAILED 4
+
+typedef struct pe_job_status {
+    int32_t state;              /* PE_JOB_* */
+    int32_t samples_done;       /* samples of every pixel accumulated in the job's film */
+    int32_t error;              /* PE_* status of a failed job */
+    int32_t chunks;             /* chunks rendered */
+    double t_submit, t_admit, t_preview, t_done;  /* seconds on the server clock; -1 = not yet */
+    uint64_t extension_rays, shadow_rays;
+    char message[128];          /* error message of a failed job */
+} pe_job_status;
+
+int pe_server_create(pe_device* dev, const pe_server_config* config, pe_server** out);
+/* Cancels the jobs left, waits for the lanes and frees every film. */
+int pe_server_destroy(pe_server* server);
+/* Seconds since the server was created (the clock of pe_job_status). */
+double pe_server_clock(const pe_server* server);
+int pe_job_submit(pe_server* server, const pe_job_desc* job, uint64_t* id);
+int pe_job_poll(pe_server* server, uint64_t id, pe_job_status* status);
+/* Waits until the job is done, cancelled or failed (timeout_ms < 0: no limit); PE_ERR_TIMEOUT otherwise. */
+int pe_job_wait(pe_server* server, uint64_t id, int32_t timeout_ms, pe_job_status* status);
+/* Stops the job after its current chunk; the film keeps the chunks already added. */
+int pe_job_cancel(pe_server* server, uint64_t id);
+/* Copies the job's film (whole chunks only) into host arrays, as pe_film_download; any pointer may be NULL. */
+int pe_job_download(pe_server* server, uint64_t id, float* sum_rgb, uint32_t* count, float* sum_lum2);
+/* Forgets an ended job and frees its film (a running job is cancelled first). */
+int pe_job_release(pe_server* server, uint64_t id);
 
 #ifdef __cplusplus
 }

@@ -135,6 +135,10 @@ int ensure_trace_resources(pe_device* dev, int& blocks) {
     return PE_OK;
 }
 
+size_t trace_spill_bytes(int grid_blocks) {
+    return static_cast<size_t>(grid_blocks) * kTraceBlock * (kStackDepth - kLdsDepth) * sizeof(uint32_t);
+}
+
 int grid_for(pe_device* dev, size_t n) {
     const size_t want = (n + 255) / 256;
     return static_cast<int>(std::max<size_t>(1, std::min<size_t>(want, static_cast<size_t>(dev->max_grid_blocks))));

@@ -885,9 +885,23 @@ void wavefront_release(pe_device* dev) {
 
 }  // namespace pe
 
-extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings,
-                         pe_film* film, int first_sample, int sample_count, const volatile int32_t* cancel,
-                         pe_batch_result* result) {
+namespace pe {
+
+Wavefront* wavefront_new() { return new Wavefront(); }
+void wavefront_delete(Wavefront* wf) { delete wf; }
+int wavefront_reserve(Wavefront* wf, uint32_t paths, int shadow_quality) {
+    return wf->ensure(paths, std::clamp(shadow_quality, 1, kMaxShadowSamples));
+}
+uint64_t wavefront_bytes_per_path(int shadow_quality) {
+    return Wavefront::bytes_per_path(std::clamp(shadow_quality, 1, kMaxShadowSamples));
+}
+
+// Renders samples [first_sample, first_sample + sample_count) of every pixel into `film` on `lane`: the
+// work of RenderDevice::render (render_device.h:84-85) for one stream, with that stream's own path state and
+// traversal-stack spill, so lanes on different streams never share device scratch.
+int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
+                   const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
+                   const volatile int32_t* cancel, pe_batch_result* result) {
     if (result) {
         result->completed = 0;
         result->samples_traced = 0;
@@ -900,6 +914,10 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
     }
     if (scene->dev != dev || film->dev != dev) {
         set_error("pe_render: scene or film belongs to another device");
+        return PE_ERR_INVALID;
+    }
+    if (scene->broken) {
+        set_error("pe_render: the scene's last update failed; upload it again");
         return PE_ERR_INVALID;
     }
     if (first_sample < 0 || sample_count < 0) {
@@ -918,7 +936,7 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
     if (sample_count == 0) return PE_OK;
     try {
         PE_HIP_CHECK(hipSetDevice(dev->ordinal));
-        Wavefront* wf = wavefront_of(dev);
+        Wavefront* wf = lane.wf;
         const int S = std::clamp(settings->shadow_quality, 1, kMaxShadowSamples);  // PathTracer ctor (path_tracer.h:43)
         const uint32_t pixels = static_cast<uint32_t>(film->width) * static_cast<uint32_t>(film->height);
         // Samples per launch: as asked, or (0) as many as keep about 2^26 paths in flight within a quarter of
@@ -942,9 +960,6 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
         const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
         if (rc != PE_OK) return rc;
-        int blocks = 0;
-        rc = ensure_trace_resources(dev, blocks);
-        if (rc != PE_OK) return rc;
 
         LaunchParams lp{};
         lp.width = film->width;
@@ -964,8 +979,8 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
                 if (cdf.empty()) {
                     lp.filter_box = 1;
                 } else {
-                    if (!wf->filter.upload(cdf.data(), cdf.size() * 4, dev->stream)) return PE_ERR_OOM;
-                    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+                    if (!wf->filter.upload(cdf.data(), cdf.size() * 4, lane.stream)) return PE_ERR_OOM;
+                    PE_HIP_CHECK(hipStreamSynchronize(lane.stream));
                     wf->filter_type = settings->filter_type;
                     wf->filter_width = settings->filter_width;
                 }
@@ -978,12 +993,11 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
         lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
 
         PathBuffers pb = wf->view();
-        hipStream_t s = dev->stream;
-        const int gblocks = dev->max_grid_blocks;
-        (void)blocks;
+        hipStream_t s = lane.stream;
+        const int gblocks = lane.grid_blocks;
         hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, s, pb);
         const bool count = settings->count_visits != 0;
-        uint32_t* spill = dev->spill.as<uint32_t>();
+        uint32_t* spill = lane.spill;
         const SceneView view = scene->view;
         // One path vertex for every path in flight: closest hit, shade, shadow rays, queue rotation.
         auto bounce = [&](int cur) {
@@ -1073,6 +1087,29 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
             return PE_ERR_CANCELLED;
         }
         return PE_OK;
+    } catch (const std::exception& e) {
+        set_error(std::string("pe_render: ") + e.what());
+        return PE_ERR_INVALID;
+    }
+}
+
+}  // namespace pe
+
+extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings,
+                         pe_film* film, int first_sample, int sample_count, const volatile int32_t* cancel,
+                         pe_batch_result* result) {
+    if (!dev) {
+        if (result) *result = pe_batch_result{};
+        set_error("pe_render: NULL device");
+        return PE_ERR_INVALID;
+    }
+    try {
+        PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+        int blocks = 0;
+        const int rc = ensure_trace_resources(dev, blocks);
+        if (rc != PE_OK) return rc;
+        const RenderLane lane{dev->stream, wavefront_of(dev), dev->spill.as<uint32_t>(), blocks};
+        return render_samples(dev, lane, scene, camera, settings, film, first_sample, sample_count, cancel, result);
     } catch (const std::exception& e) {
         set_error(std::string("pe_render: ") + e.what());
         return PE_ERR_INVALID;

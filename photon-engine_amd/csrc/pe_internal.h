@@ -101,6 +101,26 @@ namespace pe {
 int ensure_trace_resources(pe_device* dev, int& blocks);
 // Releases the integrator's buffers of a device (integrator.hip).
 void wavefront_release(pe_device* dev);
+
+// Path-state buffers of one render stream (integrator.hip).
+struct Wavefront;
+Wavefront* wavefront_new();
+void wavefront_delete(Wavefront* wf);
+int wavefront_reserve(Wavefront* wf, uint32_t paths, int shadow_quality);
+uint64_t wavefront_bytes_per_path(int shadow_quality);
+
+// One render stream: a HIP stream with its own path state and traversal-stack spill (grid_blocks x 256 lanes).
+struct RenderLane {
+    hipStream_t stream;
+    Wavefront* wf;
+    uint32_t* spill;
+    int grid_blocks;
+};
+int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
+                   const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
+                   const volatile int32_t* cancel, pe_batch_result* result);
+// Bytes of traversal-stack spill a lane of `grid_blocks` blocks needs (capi.hip).
+size_t trace_spill_bytes(int grid_blocks);
 }  // namespace pe
 
 #define PE_HIP_CHECK(call)                                                                              \

@@ -21,6 +21,8 @@ PE_ERR_HIP = -3
 PE_ERR_OOM = -4
 PE_ERR_UNSUPPORTED = -5
 PE_ERR_CANCELLED = -6
+PE_ERR_TIMEOUT = -7
+PE_JOB_QUEUED, PE_JOB_ADMITTED, PE_JOB_DONE, PE_JOB_CANCELLED, PE_JOB_FAILED = 0, 1, 2, 3, 4
 
 PE_MAT_DISNEY, PE_MAT_LAMBERT, PE_MAT_DIELECTRIC = 0, 1, 2
 PE_LIGHT_AREA, PE_LIGHT_TRIANGLE = 1, 2
@@ -105,6 +107,23 @@ class KernelStats(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_int32), ("chunk_spp", C.c_int32), ("max_chunk_paths", C.c_uint64),
+                ("film_budget", C.c_uint64), ("max_admitted", C.c_int32), ("reserved", C.c_int32 * 5)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", Camera), ("settings", Settings), ("width", C.c_int32),
+                ("height", C.c_int32), ("spp", C.c_int32), ("preview_spp", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("samples_done", C.c_int32), ("error", C.c_int32), ("chunks", C.c_int32),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_preview", C.c_double),
+                ("t_done", C.c_double), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("message", C.c_char * 128)]
+
+
 class PhotonHipError(RuntimeError):
     def __init__(self, func: str, code: int, msg: str):
         super().__init__(f"{func} -> {code}: {msg}")
@@ -141,6 +160,15 @@ SIGNATURES = {
                                  _P(C.c_uint32), _P(C.c_uint32)]),
     "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
     "pe_occluded_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_uint8)]),
+    "pe_server_create": (C.c_int, [_VP, _P(ServerConfig), _P(_VP)]),
+    "pe_server_destroy": (C.c_int, [_VP]),
+    "pe_server_clock": (C.c_double, [_VP]),
+    "pe_job_submit": (C.c_int, [_VP, _P(JobDesc), _P(C.c_uint64)]),
+    "pe_job_poll": (C.c_int, [_VP, C.c_uint64, _P(JobStatus)]),
+    "pe_job_wait": (C.c_int, [_VP, C.c_uint64, C.c_int32, _P(JobStatus)]),
+    "pe_job_cancel": (C.c_int, [_VP, C.c_uint64]),
+    "pe_job_download": (C.c_int, [_VP, C.c_uint64, _P(C.c_float), _P(C.c_uint32), _P(C.c_float)]),
+    "pe_job_release": (C.c_int, [_VP, C.c_uint64]),
 }
 
 _lib = None

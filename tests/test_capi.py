@@ -38,7 +38,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
                "pe_material_desc": capi.MaterialDesc, "pe_mesh_desc": capi.MeshDesc, "pe_light_desc": capi.LightDesc,
                "pe_env_desc": capi.EnvDesc, "pe_scene_desc": capi.SceneDesc, "pe_camera": capi.Camera,
                "pe_settings": capi.Settings, "pe_batch_result": capi.BatchResult,
-               "pe_kernel_stats": capi.KernelStats}
+               "pe_kernel_stats": capi.KernelStats, "pe_server_config": capi.ServerConfig,
+               "pe_job_desc": capi.JobDesc, "pe_job_status": capi.JobStatus}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "photon_hip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')

@@ -1,0 +1,139 @@
+"""The render-job server (pe_server_* / pe_job_*): continuous batching of render jobs of different scenes,
+film sizes and sample counts on one device. Its model is RenderSession's batch loop (engine/
+render_session.cpp:69-101) run for many sessions at once.
+
+What is checked: a job's film is bit for bit the same alone and in a mix of concurrent jobs, and the same as
+pe_render over the job's chunk boundaries; admission waits for film memory; a short job's preview overtakes
+a long job's later chunks; cancel, release and the error paths.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from photon_hip import capi, configs
+from photon_hip.device import camera_perspective, render, settings
+from photon_hip.serve import Server
+
+CHUNK_PATHS = 1 << 16
+
+
+def _cam(view):
+    return camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+
+
+def _scenes():
+    a = configs.box_icosphere(level=3, width=48, height=40, max_bounces=6)
+    b = configs.product_sphere(width=64, height=36, lon=64, lat=48, env_width=128, env_height=64)
+    c = configs.glass_studio(width=40, height=30)
+    d = configs.led_strip(quads=60, width=32, height=32, max_bounces=2)
+    return [a, b, c, d]
+
+
+def _submit(srv, gs, view, spp, preview=8):
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    return srv.submit(gs, _cam(view), st, view.width, view.height, spp, preview)
+
+
+@pytest.mark.gpu
+def test_job_alone_equals_job_in_mix_and_chunked_render(dev):
+    scenes = _scenes()
+    gscenes = [dev.upload(sc) for sc, _ in scenes]
+    (sc_a, view_a), gs_a = scenes[0], gscenes[0]
+    srv = Server(dev, slots=4, chunk_spp=4, max_chunk_paths=CHUNK_PATHS)
+    try:
+        jid = _submit(srv, gs_a, view_a, spp=24)
+        st = srv.wait(jid, 60000)
+        assert st.state == capi.PE_JOB_DONE and st.samples_done == 24, st
+        alone = srv.download(jid)
+        srv.release(jid)
+        ids = []
+        for k in range(3):                       # the mix: every scene, some twice, all at once
+            for (sc, view), gs in zip(scenes, gscenes):
+                ids.append((gs is gs_a and k == 1, _submit(srv, gs, view, spp=12 + 4 * k)))
+        target = [j for is_a, j in ids if is_a][0]
+        for _, j in ids:
+            s = srv.wait(j, 120000)
+            assert s.state == capi.PE_JOB_DONE, s
+        mixed = srv.download(target)            # 16 spp of scene a, rendered among eleven other jobs
+        np.testing.assert_array_equal(mixed[1], np.full((view_a.height, view_a.width), 16, np.uint32))
+        for _, j in ids:
+            srv.release(j)
+        jid2 = _submit(srv, gs_a, view_a, spp=16)  # the same job alone
+        srv.wait(jid2, 60000)
+        alone16 = srv.download(jid2)
+        np.testing.assert_array_equal(mixed[0], alone16[0])
+        np.testing.assert_array_equal(mixed[2], alone16[2])
+    finally:
+        srv.close()
+    # pe_render over the same chunk boundaries (preview 8, then chunks of 4) gives the same film.
+    film = dev.film(view_a.width, view_a.height)
+    first = 0
+    for n in (8, 4, 4, 4, 4):
+        st = settings(max_bounces=view_a.max_bounces, seed=12345, samples_per_launch=n)
+        render(dev, gs_a, _cam(view_a), st, film, first, n)
+        first += n
+    s, c, l2, _ = film.download()
+    np.testing.assert_array_equal(s, alone[0])
+    np.testing.assert_array_equal(l2, alone[2])
+
+
+@pytest.mark.gpu
+def test_admission_waits_for_film_memory(dev):
+    (sc, view) = _scenes()[0]
+    gs = dev.upload(sc)
+    one_film = view.width * view.height * 20 + 8
+    srv = Server(dev, slots=2, chunk_spp=4, max_chunk_paths=CHUNK_PATHS, film_budget=one_film)
+    try:
+        j1 = _submit(srv, gs, view, spp=8)
+        j2 = _submit(srv, gs, view, spp=8)
+        assert srv.wait(j1, 60000).state == capi.PE_JOB_DONE
+        assert srv.poll(j2).state == capi.PE_JOB_QUEUED      # j1 still holds the only film
+        srv.release(j1)
+        s2 = srv.wait(j2, 60000)
+        assert s2.state == capi.PE_JOB_DONE and s2.t_admit >= 0
+        big = srv.try_submit(gs, _cam(view), settings(), view.width * 2, view.height, 4)
+        assert big[0] == capi.PE_ERR_OOM                      # can never fit the budget
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_short_job_preview_overtakes_long_job(dev):
+    scenes = _scenes()
+    (sc_l, view_l), (sc_s, view_s) = scenes[1], scenes[0]
+    gl, gsh = dev.upload(sc_l), dev.upload(sc_s)
+    srv = Server(dev, slots=1, chunk_spp=2, max_chunk_paths=CHUNK_PATHS)
+    try:
+        long_job = _submit(srv, gl, view_l, spp=200, preview=2)
+        short_job = _submit(srv, gsh, view_s, spp=64, preview=4)
+        s_short = srv.wait(short_job, 120000)
+        s_long = srv.wait(long_job, 120000)
+        assert s_short.state == s_long.state == capi.PE_JOB_DONE
+        assert s_short.t_preview < s_long.t_done
+        assert s_long.t_preview <= s_short.t_preview          # the long job's own preview went first
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_cancel_release_and_errors(dev):
+    (sc, view) = _scenes()[1]
+    gs = dev.upload(sc)
+    srv = Server(dev, slots=1, chunk_spp=1, max_chunk_paths=CHUNK_PATHS)
+    try:
+        j = _submit(srv, gs, view, spp=100000, preview=1)
+        srv.cancel(j)
+        s = srv.wait(j, 60000)
+        assert s.state == capi.PE_JOB_CANCELLED and s.samples_done < 100000
+        _, cnt, _ = srv.download(j)
+        assert (cnt == s.samples_done).all()                  # whole chunks only
+        srv.release(j)
+        with pytest.raises(capi.PhotonHipError):
+            srv.poll(j)
+        assert srv.try_submit(gs, _cam(view), settings(), 0, 8, 4)[0] == capi.PE_ERR_INVALID
+        assert srv.try_submit(gs, _cam(view), settings(), 8, 8, 0)[0] == capi.PE_ERR_INVALID
+        j2 = _submit(srv, gs, view, spp=3, preview=1)
+        assert srv.wait(j2, 60000).samples_done == 3
+    finally:
+        srv.close()
