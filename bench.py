@@ -79,6 +79,83 @@ def cpu_baseline(scene, view, rays_per_sample: float, target_s: float) -> dict |
             "sample": f"oracle restatement, 1 spp on rows 0..{rows} of the 1920x1080 frame, {cores} threads"}
 
 
+def serving_run(dev, main_scene, seconds: float, rate: float, seed: int = 3, drain_s: float = 150.0) -> dict:
+    """The serving stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job a scene drawn from a resident
+    pool (C0 box + icosphere, C1 displaced spheres of 0.1 M / 2 M / 5 M triangles, the C2 studio with its LED
+    strip, dispersive prism, absorbing liquid and subsurface block), a film of 256^2, 512^2, 1920x1080 or 2048^2, and log-uniform 16..1024 spp with a 16-spp preview.
+    Returns p50/p99 time to preview and to completion, offered and served load."""
+    import numpy as np
+
+    from photon_hip import camera_perspective, configs, settings
+    from photon_hip import capi
+    from photon_hip.serve import Server
+
+    t = time.time()
+    pool = [("c0_box_icosphere_20k", *configs.box_icosphere(level=5)),
+            ("c1_sphere_100k", *configs.product_sphere(lon=256, lat=200, env_width=1024, env_height=512)),
+            ("c1_sphere_5m", *configs.product_sphere(lon=1600, lat=1600)),
+            ("c2_lights_media_204k", *configs.lights_media_studio())]
+    resident = [(name, dev.upload(sc), view) for name, sc, view in pool]
+    resident.append(("c1_sphere_2m", main_scene[0], main_scene[1]))
+    log(f"serving: {len(resident)} resident scenes ready in {time.time() - t:.1f} s")
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 512), (1920, 1080), (2048, 2048)]
+    n_jobs = int(rng.poisson(rate * seconds))
+    arrivals = np.sort(rng.uniform(0.0, seconds, n_jobs))
+    plan = []
+    for a in arrivals:
+        name, gs, view = resident[int(rng.integers(len(resident)))]
+        w, h = sizes[int(rng.integers(len(sizes)))]
+        spp = int(round(math.exp(rng.uniform(math.log(16), math.log(1024)))))
+        plan.append((float(a), name, gs, view, w, h, spp))
+    srv = Server(dev, slots=4, chunk_spp=16, max_admitted=4096)
+    offered = sum(w * h * spp for _, _, _, _, w, h, spp in plan)
+    ids = []
+    t0 = srv.clock()
+    k = 0
+    next_log = 5.0
+    while k < len(plan):
+        now = srv.clock() - t0
+        while k < len(plan) and plan[k][0] <= now:
+            _, name, gs, view, w, h, spp = plan[k]
+            cam = camera_perspective(view.position, view.target, view.up, view.vfov, w / h)
+            st = settings(max_bounces=view.max_bounces, seed=12345)
+            ids.append((srv.submit(gs, cam, st, w, h, spp, 16), w * h * spp))
+            k += 1
+        if now >= next_log:
+            log(f"serving: {k}/{len(plan)} jobs submitted at t={now:.0f} s")
+            next_log += 5.0
+        time.sleep(0.002)
+    deadline = time.time() + drain_s
+    stats = []
+    for i, (jid, work) in enumerate(ids):
+        st = srv.wait(jid, max(0, int((deadline - time.time()) * 1000)))
+        if not st.ended:
+            srv.cancel(jid)
+            st = srv.wait(jid, -1)
+        stats.append((st, work))
+        if (i + 1) % 50 == 0:
+            log(f"serving: {i + 1}/{len(ids)} jobs ended")
+    wall = srv.clock() - t0
+    for jid, _ in ids:
+        srv.release(jid)
+    srv.close()
+    done = [s for s, _ in stats if s.state == capi.PE_JOB_DONE]
+    prev = np.array([s.t_preview - s.t_submit for s, _ in stats if s.t_preview >= 0])
+    comp = np.array([s.t_done - s.t_submit for s in done])
+    served = sum(s.samples_done * (w * h * spp) / spp for (s, _), (_, _, _, _, w, h, spp) in zip(stats, plan))
+    rays = sum(s.extension_rays + s.shadow_rays for s, _ in stats)
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3) if len(a) else None  # noqa: E731
+    return {"workload": f"Poisson {rate:g} jobs/s for {seconds:g} s (seed {seed}); scenes {[r[0] for r in resident]}; "
+                        f"films 256^2/512^2/1080p/2048^2; spp log-uniform 16-1024; preview 16 spp",
+            "slots": 4, "chunk_spp": 16, "jobs": len(ids), "completed": len(done),
+            "preview_p50_s": pct(prev, 50), "preview_p99_s": pct(prev, 99),
+            "done_p50_s": pct(comp, 50), "done_p99_s": pct(comp, 99),
+            "offered_msamples_per_s": round(offered / seconds / 1e6, 1),
+            "served_msamples_per_s": round(served / wall / 1e6, 1),
+            "mrays_per_s": round(rays / wall / 1e6, 1), "wall_s": round(wall, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +165,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene (smoke runs only; not the workload)")
+    ap.add_argument("--serving-seconds", type=float, default=5.0,
+                    help="length of the serving stream's arrivals (0: no serving run)")
+    ap.add_argument("--serving-rate", type=float, default=20.0, help="serving arrivals per second")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -205,6 +285,9 @@ def main() -> int:
                      "launches": isect["launches"],
                      "avg_launch_ms": round(isect["ms"] / max(1, isect["launches"]), 4)},
     }
+    if args.serving_seconds > 0:
+        log("serving stream ...")
+        result["serving"] = serving_run(dev, (gs, view), args.serving_seconds, args.serving_rate)
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline(scene, view, (ext + shadow) / samples, args.cpu_seconds)

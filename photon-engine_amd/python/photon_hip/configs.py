@@ -203,6 +203,13 @@ def glass_studio(width: int = 64, height: int = 48, spp: int = 16, max_bounces: 
     return sc, view
 
 
+def lights_media_studio(width: int = 1280, height: int = 720, spp: int = 512, max_bounces: int = 16) -> tuple[Scene, View]:
+    """The many-lights + media workload at full size: the glass studio with 1000 LED quads (2000 emissive
+    triangles, light BVH) and a 200,934-triangle dispersive prism (sides split 183 x 183), 1280 x 720,
+    512 spp, depth 16."""
+    return glass_studio(width, height, spp, max_bounces, led_quads=1000, prism_subdiv=183)
+
+
 # ---- product_sphere --------------------------------------------------------------------------------------
 
 def _perlin3(p: np.ndarray, perm: np.ndarray) -> np.ndarray:
