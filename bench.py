@@ -28,6 +28,9 @@ sys.path.insert(0, str(ROOT))
 sys.path.insert(0, str(ROOT / "photon-engine_amd" / "python"))
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
+# Guiding MLP per query: 2 x (160*64 + 4*64*64) multiply-adds (the padded layer shapes the MFMAs compute).
+GUIDE_FLOPS_PER_QUERY = 2 * (160 * 64 + 4 * 64 * 64)
 NODE_BYTES, TRI_BYTES, RAY_IN_BYTES, HIT_OUT_BYTES = 64, 48, 32, 16
 
 
@@ -156,6 +159,85 @@ def serving_run(dev, main_scene, seconds: float, rate: float, seed: int = 3, dra
             "mrays_per_s": round(rays / wall / 1e6, 1), "wall_s": round(wall, 1)}
 
 
+def guiding_run(dev, budget_spp: int, ref_spp: int, spl: int = 16, seed: int = 13) -> dict:
+    """Equal-time comparison on the learned-guiding workload (the 493k-triangle room lit through a 5 cm door gap,
+    1920x1080, depth 10): the wall time of a plain render of `budget_spp` spp is the budget; the guided render
+    (guide created, trained online during its first launches, then guiding) adds launches of `spl` spp into
+    its film until it has used the same time. Both are compared with a plain `ref_spp`-spp image of independent
+    samples by relMSE."""
+    import numpy as np
+
+    from photon_hip import camera_perspective, configs, settings, render
+    from photon_hip.guide import Guide, render_guided
+
+    sc, view = configs.slit_room()
+    t = time.time()
+    gs = dev.upload(sc)
+    log(f"guiding: {sc.num_triangles} triangles uploaded in {time.time() - t:.1f} s")
+    cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=spl)
+
+    def mean(film):
+        s, c, _, _ = film.download()
+        return s / np.maximum(c, 1)[..., None]
+
+    def relmse(x, r):
+        return float(np.mean((x - r) ** 2 / (r ** 2 + 0.01)))
+
+    ref_film = dev.film(view.width, view.height)
+    t = time.time()
+    render(dev, gs, cam, settings(max_bounces=view.max_bounces, seed=12345), ref_film, 1 << 20, ref_spp)
+    dev.synchronize()
+    log(f"guiding: reference {ref_spp} spp in {time.time() - t:.1f} s")
+    ref = mean(ref_film)
+    plain_film = dev.film(view.width, view.height)
+    render(dev, gs, cam, st, plain_film, 1 << 22, spl)  # warm-up (kernels, buffers), other samples
+    plain_film = dev.film(view.width, view.height)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    render(dev, gs, cam, st, plain_film, 0, budget_spp)
+    dev.synchronize()
+    budget = time.perf_counter() - t0
+    plain = mean(plain_film)
+    guided_film = dev.film(view.width, view.height)
+    dev.synchronize()
+    t0 = time.perf_counter()
+    guide = Guide(dev, gs, seed=seed, learning_rate=2e-3, steps_per_launch=16, train_launches=6)
+    spp = 0
+    while time.perf_counter() - t0 < budget:
+        render_guided(dev, gs, guide, cam, st, guided_film, spp, spl)
+        dev.synchronize()
+        spp += spl
+    used = time.perf_counter() - t0
+    stats = guide.stats()
+    guided = mean(guided_film)
+    # MFMA roofline of the fused inference kernel: a separately profiled guided render with the trained guide.
+    dev.synchronize()
+    dev._lib.pe_profile_enable(dev.h, 1)
+    before = guide.stats()["inferred"]
+    render_guided(dev, gs, guide, cam, st, dev.film(view.width, view.height), 1 << 21, 2 * spl)
+    dev.synchronize()
+    kt = kernel_times(dev)
+    dev._lib.pe_profile_enable(dev.h, 0)
+    queries = guide.stats()["inferred"] - before
+    guide.close()
+    flops = queries * GUIDE_FLOPS_PER_QUERY
+    infer = kt.get("guide_infer", {"launches": 0, "ms": 0.0})
+    tflops = flops / (infer["ms"] * 1e-3) / 1e12 if infer["ms"] > 0 else 0.0
+    e_p, e_g = relmse(plain, ref), relmse(guided, ref)
+    return {"workload": f"slit room {sc.num_triangles} tris, {view.width}x{view.height}, depth {view.max_bounces}, "
+                        f"5 cm door gap; reference {ref_spp} spp plain",
+            "budget_s": round(budget, 3), "guided_s": round(used, 3), "plain_spp": budget_spp, "guided_spp": spp,
+            "plain_relmse": round(e_p, 5), "guided_relmse": round(e_g, 5), "ratio": round(e_g / e_p, 3),
+            "train_steps": stats["steps"], "records": stats["records"], "final_loss": round(stats["last_loss"], 3),
+            "kernels_ms": {k: round(v["ms"], 2) for k, v in kt.items()},
+            "roofline": {"kernel": "guide_infer", "bound": "mfma", "achieved": round(tflops, 1),
+                         "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / BF16_PEAK_TFLOPS, 4),
+                         "traffic": None, "flops_per_query": GUIDE_FLOPS_PER_QUERY, "queries": queries,
+                         "launches": infer["launches"],
+                         "avg_launch_ms": round(infer["ms"] / max(1, infer["launches"]), 4)}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,6 +250,9 @@ def main() -> int:
     ap.add_argument("--serving-seconds", type=float, default=5.0,
                     help="length of the serving stream's arrivals (0: no serving run)")
     ap.add_argument("--serving-rate", type=float, default=20.0, help="serving arrivals per second")
+    ap.add_argument("--guiding-spp", type=int, default=256,
+                    help="plain spp whose wall time is the equal-time budget of the guiding comparison (0: skip)")
+    ap.add_argument("--guiding-ref-spp", type=int, default=2048)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only run a plain device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -288,6 +373,9 @@ def main() -> int:
     if args.serving_seconds > 0:
         log("serving stream ...")
         result["serving"] = serving_run(dev, (gs, view), args.serving_seconds, args.serving_rate)
+    if args.guiding_spp > 0:
+        log("guiding (equal time) ...")
+        result["guiding"] = guiding_run(dev, args.guiding_spp, args.guiding_ref_spp)
     if not args.no_cpu:
         log("cpu baseline ...")
         cb = cpu_baseline(scene, view, (ext + shadow) / samples, args.cpu_seconds)

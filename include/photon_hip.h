@@ -39,6 +39,7 @@ extern "C" {
 typedef struct pe_device pe_device;
 typedef struct pe_scene pe_scene;
 typedef struct pe_film pe_film;
+typedef struct pe_guide pe_guide;
 
 /* ---- device registry --------------------------------------------------------------------------------
  * Mirrors DeviceInfo (render_device.h:37-48). kind is always PE_DEVICE_HIP here. */
@@ -257,6 +258,58 @@ int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, c
 int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary);
 /* Any hit (shadow rays): occluded[i] = 1 when something lies within [tmin, tmax]. */
 int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded);
+
+/* ---- learned path guiding ------------------------------------------------------------------------------
+ * The reference guides with an SD-tree (GuidingField, integrators/path_guiding.cpp:11-388) trained in the first
+ * samples and one-sample-MIS-combined with BSDF sampling (path_tracer.cpp:700-735, NEE pdf :668-671). This
+ * library's guide is a small MLP instead: input the shading point and the outgoing direction, each of the six
+ * coordinates encoded with 12 sine/cosine frequencies (144 features, plus the 6 raw values, padded to 160);
+ * four hidden layers of 64 (ReLU); output an 8-lobe von Mises-Fisher mixture (per lobe a mean direction, a
+ * concentration exp(o) and a softmax weight). Weights and activations are bf16, accumulation fp32, on MFMA.
+ * At a non-delta vertex a render with a guide samples the mixture with probability `guide_fraction` and the
+ * BSDF otherwise, and weighs by the mixture density (1 - a) p_bsdf + a q — also in the NEE and emission MIS —
+ * so the estimate stays unbiased for any weights. The guide trains online (Adam) on the radiance the paths of
+ * each launch bring back along their sampled directions. A render without a guide is the plain path tracer. */
+typedef struct pe_guide_config {
+    float learning_rate;        /* Adam step size; 0 = 1e-3 */
+    float guide_fraction;       /* probability of sampling the mixture at a guided vertex; 0 = 0.5 */
+    int32_t batch;              /* training records per Adam step; 0 = 2^18 (rounded up to a multiple of 256) */
+    int32_t steps_per_launch;   /* Adam steps after each render launch while training; 0 = 4 */
+    int32_t train_launches;     /* launches that train (then the weights stay fixed); 0 = unlimited */
+    uint32_t seed;              /* weight initialisation */
+    float bounds_min[3];        /* position normalisation box; min == max: the scene's bounds */
+    float bounds_max[3];
+    float weight_clamp;         /* record weights above this multiple of the batch mean are clamped; 0 = off */
+    int32_t guided_bounces;     /* bounces (from the camera) whose vertices are guided; 0 = 3 */
+    int32_t reserved[4];
+} pe_guide_config;
+
+/* Network geometry: layer l maps in[l] -> out[l]; parameters are stored per layer as W[out][in] then b[out]. */
+#define PE_GUIDE_INPUTS 160
+#define PE_GUIDE_HIDDEN 64
+#define PE_GUIDE_OUTPUTS 64     /* 40 used: lobe j = outputs 5j..5j+4 (mean xyz, log concentration, weight logit) */
+#define PE_GUIDE_LAYERS 5
+#define PE_GUIDE_LOBES 8
+#define PE_GUIDE_PARAMS 26944   /* 64*160+64 + 4*(64*64+64) */
+
+int pe_guide_create(pe_device* dev, pe_scene* scene, const pe_guide_config* config, pe_guide** out);
+int pe_guide_destroy(pe_guide* guide);
+/* fp32 master parameters (PE_GUIDE_PARAMS floats, layer order W then b); set also resets the Adam state. */
+int pe_guide_get_params(pe_guide* guide, float* params);
+int pe_guide_set_params(pe_guide* guide, const float* params);
+/* Mixture of n queries (positions and outgoing directions, 3 floats each): per query 8 lobes x (mean xyz,
+ * concentration, weight) = 40 floats. Runs the same fused MFMA kernel the renderer uses. */
+int pe_guide_infer(pe_guide* guide, size_t n, const float* pos, const float* dir, float* lobes);
+/* One Adam step on n records (pos xyz, dir xyz, sampled direction xyz, weight: 10 floats each; n <= batch):
+ * the loss is -sum w log q(direction) / sum w. loss (may be NULL) receives the loss before the step; grad (may
+ * be NULL) the fp32 gradient (PE_GUIDE_PARAMS floats) the step used. */
+int pe_guide_train_step(pe_guide* guide, size_t n, const float* records, float* loss, float* grad);
+/* Statistics: Adam steps taken, loss of the last step, records collected and mixtures inferred by renders. */
+int pe_guide_stats(pe_guide* guide, uint64_t* steps, float* last_loss, uint64_t* records, uint64_t* inferred);
+/* pe_render with a guide (NULL = pe_render). Trains after each launch while the guide's training lasts. */
+int pe_render_guided(pe_device* dev, pe_scene* scene, pe_guide* guide, const pe_camera* camera,
+                     const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
+                     const volatile int32_t* cancel, pe_batch_result* result);
 
 /* ---- serving: continuous batching of render jobs ----------------------------------------------------
  * No single counterpart in the reference: RenderSession (engine/render_session.cpp:69-101) drives one camera

@@ -36,7 +36,7 @@ HIP_FLAGS = [
 # Files whose host arithmetic must match the reference float for float (no a*b+c contraction).
 STRICT_HOST = {"scene_upload.cpp", "tables.cpp", "scene_prep.cpp", "light_bvh.cpp"}
 
-HIP_SOURCES = ["capi.hip", "wavefront.hip", "integrator.hip", "serve.hip"]
+HIP_SOURCES = ["capi.hip", "wavefront.hip", "integrator.hip", "serve.hip", "guide.hip"]
 HOST_SOURCES = ["bvh_build.cpp", "scene_upload.cpp", "tables.cpp", "scene_prep.cpp", "light_bvh.cpp"]
 
 

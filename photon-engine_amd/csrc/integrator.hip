@@ -20,6 +20,7 @@
 //
 // Path state is SoA in HBM (float4 / uint4 columns, 16-byte coalesced loads per lane).
 #include "photon_hip.h"
+#include "pe_guide_host.h"
 #include "pe_internal.h"
 #include "pe_lights.h"
 #include "pe_shading.h"
@@ -47,6 +48,8 @@ struct PathBuffers {
     float4* last_n;   // previous vertex normal (MIS)
     uint4* ids;       // pixel, sample, Sobol pixel hash, packed bits (see pack_bits)
     uint4* ext;       // transport state beyond the surface bounce (see the kExt* fields)
+    float4* mix;      // guiding: the mixture of every queue slot (kGuideMixFloat4 float4 each)
+    float4* grec;     // guiding: two training records per path, 4 float4 each (see k_guide_collect)
     float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
@@ -74,6 +77,9 @@ constexpr uint32_t kExtDepthMask = 7u;
 constexpr uint32_t kExtPassShift = 3u, kExtPassMask = 0x7Fu << kExtPassShift;
 constexpr uint32_t kExtWalkStepShift = 10u, kExtWalkStepMask = 0x1FFu << kExtWalkStepShift;
 constexpr uint32_t kExtWalk = 1u << 19;
+constexpr uint32_t kExtRecShift = 20u, kExtRecMask = 3u << kExtRecShift;  // guiding records written (0..2)
+constexpr uint32_t kExtRecOpen = 1u << 22;  // the last record waits for the radiance baseline
+constexpr int kGuideRecords = 2;
 constexpr uint32_t kMaxWalkSteps = 256;   // randomWalkSubsurface's step limit (path_tracer.cpp:270)
 constexpr uint32_t kMaxPassThroughs = 64;  // path_tracer.cpp:783
 constexpr uint32_t kMaxShadowQueries = 64;  // shadowTransmittance (path_tracer.cpp:180)
@@ -120,6 +126,8 @@ struct LaunchParams {
     uint64_t seed;
     int filter_box;
     int transport;        // the scene needs the ext column (k_shade<true>)
+    float guide_fraction; // guiding: probability of sampling the mixture at a guided vertex (0: BSDF only)
+    int guide_records;    // guiding: capture training records
     float filter_width, filter_radius;
     const float* filter_cdf;  // 513 entries when not a box
     V3 cam_pos, cam_ll, cam_h, cam_v;
@@ -191,7 +199,9 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
         pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
-        if (lp.transport) pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (lp.transport || lp.guide_records) pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (lp.guide_records)
+            for (int r = 0; r < kGuideRecords; ++r) pb.grec[static_cast<size_t>(i) * 8 + 4 * r + 3] = make_float4(0, 0, 0, 0);
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -352,10 +362,81 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, P
     }
 }
 
+// Guiding queries of the current ray queue: the hit point and the outgoing direction of every slot that hit.
+struct RenderQueries {
+    const float4* ro;
+    const float4* rd;
+    const float4* hits;
+    const uint32_t* count_ptr;
+    float4* mix;
+    __device__ __forceinline__ bool fetch(uint32_t i, V3& p, V3& d) const {
+        const float4 h = hits[i];
+        if (__float_as_int(h.y) < 0) return false;
+        const float4 o = ro[i], r = rd[i];
+        const V3 dir = mk(r.x, r.y, r.z);
+        p = mk(o.x, o.y, o.z) + dir * h.x;
+        d = -dir;
+        return true;
+    }
+    __device__ __forceinline__ uint32_t count(uint32_t n) const { return min(n, *count_ptr); }
+    __device__ __forceinline__ void store_part(uint32_t i, int part, float4 a, float4 b, float2 cdf, float2 coef) const {
+        guide_store_part(mix + static_cast<size_t>(i) * kGuideMixFloat4, part, a, b, cdf, coef);
+    }
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// Training records of every ended path. A record of a guided vertex holds the point, the outgoing and the
+// sampled direction, the throughput before the vertex and the path radiance once the vertex's own NEE had
+// entered it (written at the start of the next vertex); the radiance the path then brought back along the
+// sampled direction, divided by that throughput, is f cos L_i / pdf for the sampled direction: its luminance
+// weights the record. Records go to hash-chosen slots of the guide's buffer (later ones overwrite), so any
+// contiguous batch of the buffer is a spatially unbiased sample.
+__global__ void __launch_bounds__(kBlock) k_guide_collect(LaunchParams lp, PathBuffers pb, float4* __restrict__ out,
+                                                          uint32_t slots, uint64_t salt) {
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const int S = lp.shadow_samples;
+    for (uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < lp.num_paths; pid += stride) {
+        const float4 L4 = pb.L[pid];
+        V3 L = mk(L4.x, L4.y, L4.z);
+        for (int s = 0; s < S; ++s) {
+            const float4 e = pb.pend[static_cast<size_t>(pid) * S + s];
+            L += mk(e.x, e.y, e.z);
+        }
+        uint32_t written = 0;
+        for (int r = 0; r < kGuideRecords; ++r) {
+            const float4* rec = pb.grec + static_cast<size_t>(pid) * 8 + 4 * r;
+            const float4 base = rec[3];
+            if (base.w != 1.0f) continue;
+            const float4 a = rec[0], b = rec[1], c = rec[2];
+            const V3 inc = L - mk(base.x, base.y, base.z);
+            const V3 li = mk(c.y > 0.0f ? inc.x / c.y : 0.0f, c.z > 0.0f ? inc.y / c.z : 0.0f, c.w > 0.0f ? inc.z / c.w : 0.0f);
+            const float w = luminance(li);
+            if (!(w > 0.0f) || !finite_f(w)) continue;
+            const uint32_t slot = static_cast<uint32_t>(mix64((static_cast<uint64_t>(pid) << 1 | r) ^ salt) % slots);
+            out[3 * static_cast<size_t>(slot)] = make_float4(a.x, a.y, a.z, w);
+            out[3 * static_cast<size_t>(slot) + 1] = make_float4(a.w, b.x, b.y, 0.0f);
+            out[3 * static_cast<size_t>(slot) + 2] = make_float4(b.z, b.w, c.x, 0.0f);
+            ++written;
+        }
+        wave_count(&pb.stats[6], written);
+    }
+}
+
 // kTransport = false: the scene has no Dielectric and no transmitting or subsurface material, so no path
 // carries a wavelength, a medium, a pass-through count or a walk, and the ext column is neither read nor written.
-template <bool kTransport>
-__global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
+// kGuide: the render has a guide; a guided vertex samples the mixture of its queue slot or the BSDF and weighs
+// by the mixture density (path_tracer.cpp:700-735 with the MLP in place of the SD-tree), and records its
+// sample for training.
+template <bool kTransport, bool kGuide>
+__global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur,
+                                                  int guide_bounce) {
     const uint32_t n = pb.ctr[0];
     const float4* ro = pb.ro[cur];
     const float4* rd = pb.rd[cur];
@@ -414,7 +495,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             V3 L = mk(L4.x, L4.y, L4.z);
             const uint4 ids = pb.ids[pid];
             uint4 ext = make_uint4(0u, 0u, 0u, 0u);
-            if constexpr (kTransport) ext = pb.ext[pid];
+            if constexpr (kTransport || kGuide) ext = pb.ext[pid];
             float lambda = __uint_as_float(ext.x);
             SobolState ss{ids.z, rev_bits(ids.y), get_dim(ids.w)};
             uint32_t depth = get_depth(ids.w), diffuse = get_diffuse(ids.w);
@@ -426,6 +507,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                 const float4 p = pb.pend[k];
                 L += mk(p.x, p.y, p.z);
                 pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            if (kGuide && (ext.w & kExtRecOpen)) {  // the radiance baseline of the previous vertex's record
+                const uint32_t r = ((ext.w & kExtRecMask) >> kExtRecShift) - 1u;
+                pb.grec[static_cast<size_t>(pid) * 8 + 4 * r + 3] = make_float4(L.x, L.y, L.z, 1.0f);
+                ext.w &= ~kExtRecOpen;
             }
             const float4 h = pb.hits[i];
             const int tri = __float_as_int(h.y);
@@ -546,6 +632,11 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                         ext.x = __float_as_uint(lambda);
                     }
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
+                    // Guided: the bounce had its mixtures inferred (the first guided_bounces bounces of a launch).
+                    const bool guided = kGuide && guide_bounce && lp.guide_fraction > 0.0f && !is_delta && !exit_vertex;
+                    const float alpha = lp.guide_fraction;
+                    // The slot's mixture is read where it is used (L1/L2), not held in registers across the NEE loop.
+                    const float4* gmix = pb.mix + static_cast<size_t>(i) * kGuideMixFloat4;
                     if (sc.light_count > 0 && !is_delta) {  // NEE, path_tracer.cpp:631-698
                         const Picker pk = picker_prepare(sc, si.point, si.normal);
                         // Each NEE sample's shadow ray is appended as soon as it is built (the append is
@@ -575,6 +666,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                                 }
                                 float bsdf_pdf = 0.0f;
                                 const V3 f = bsdf_eval_pdf<kTransport>(m, T, si, wo, ls.wi, lambda, bsdf_pdf);
+                                if (guided) bsdf_pdf = (1.0f - alpha) * bsdf_pdf + alpha * guide_pdf(gmix, ls.wi);
                                 const float cos_t = fabsf(dot(ls.wi, si.normal));
                                 if (is_black(f) || !(cos_t > 0.0f)) break;
                                 const float light_pdf = pmf * ls.pdf;
@@ -607,7 +699,23 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     float u0, u1;
                     sobol_get2d(ss, u0, u1);
                     BsdfSample bs;
-                    if (!bsdf_sample<kTransport>(m, T, si, wo, uc, u0, u1, lambda, bs)) {
+                    bool sampled;
+                    if (guided && uc < alpha) {
+                        // From the mixture: the BSDF value and the mixture density of the direction.
+                        bs.wi = guide_sample(gmix, uc / alpha, u0, u1);
+                        float p_bsdf = 0.0f;
+                        bs.f = bsdf_eval_pdf<kTransport>(m, T, si, wo, bs.wi, lambda, p_bsdf);
+                        bs.pdf = (1.0f - alpha) * p_bsdf + alpha * guide_pdf(gmix, bs.wi);
+                        bs.delta = bs.pass_through = bs.subsurface = false;
+                        sampled = bs.pdf > 0.0f;
+                    } else {
+                        const float ub = guided ? (uc - alpha) / (1.0f - alpha) : uc;
+                        sampled = bsdf_sample<kTransport>(m, T, si, wo, ub, u0, u1, lambda, bs);
+                        if (guided && sampled && !bs.subsurface && !bs.pass_through)
+                            bs.pdf = bs.delta ? (1.0f - alpha) * bs.pdf
+                                              : (1.0f - alpha) * bs.pdf + alpha * guide_pdf(gmix, bs.wi);
+                    }
+                    if (!sampled) {
                         alive = false;
                     } else if (kTransport && bs.subsurface) {
                         thr *= bs.weight;
@@ -631,7 +739,16 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                         }
                     } else {
                         const float cos_t = fabsf(dot(bs.wi, si.normal));
+                        const V3 thr_before = thr;
                         thr *= bs.f * (cos_t / bs.pdf);
+                        const uint32_t nrec = (ext.w & kExtRecMask) >> kExtRecShift;
+                        if (kGuide && lp.guide_records && !bs.delta && nrec < kGuideRecords && finite3(thr)) {
+                            float4* rec = pb.grec + static_cast<size_t>(pid) * 8 + 4 * nrec;
+                            rec[0] = make_float4(si.point.x, si.point.y, si.point.z, wo.x);
+                            rec[1] = make_float4(wo.y, wo.z, bs.wi.x, bs.wi.y);
+                            rec[2] = make_float4(bs.wi.z, thr_before.x, thr_before.y, thr_before.z);
+                            ext.w = (ext.w & ~kExtRecMask) | ((nrec + 1u) << kExtRecShift) | kExtRecOpen;
+                        }
                         if (!finite3(thr)) {
                             alive = false;
                         } else {
@@ -672,7 +789,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
             if (extend) {
                 pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
-                if constexpr (kTransport) pb.ext[pid] = ext;
+                if constexpr (kTransport || kGuide) pb.ext[pid] = ext;
             }
         }
         const uint32_t slot = wave_append(&pb.ctr[1], extend);
@@ -774,8 +891,9 @@ std::vector<float> filter_cdf(int type, float width) {
 
 }  // namespace
 
-enum KernelKind { kKGenerate = 0, kKIntersect, kKShade, kKShadow, kKFilm, kKindCount };
-const char* const kKernelNames[kKindCount] = {"generate", "intersect", "shade", "shadow", "film"};
+enum KernelKind { kKGenerate = 0, kKIntersect, kKShade, kKShadow, kKFilm, kKGuide, kKTrain, kKindCount };
+const char* const kKernelNames[kKindCount] = {"generate", "intersect", "shade", "shadow", "film", "guide_infer",
+                                              "guide_train"};
 
 // HIP events around each wavefront launch on the device's stream while profiling is on.
 struct Profiler {
@@ -901,7 +1019,7 @@ uint64_t wavefront_bytes_per_path(int shadow_quality) {
 // traversal-stack spill, so lanes on different streams never share device scratch.
 int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
                    const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
-                   const volatile int32_t* cancel, pe_batch_result* result) {
+                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide) {
     if (result) {
         result->completed = 0;
         result->samples_traced = 0;
@@ -918,6 +1036,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
     }
     if (scene->broken) {
         set_error("pe_render: the scene's last update failed; upload it again");
+        return PE_ERR_INVALID;
+    }
+    if (guide && guide->dev != dev) {
+        set_error("pe_render: the guide belongs to another device");
         return PE_ERR_INVALID;
     }
     if (first_sample < 0 || sample_count < 0) {
@@ -946,7 +1068,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         if (spl_req <= 0) {
             size_t free_b = 0, total_b = 0;
             if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = size_t{1} << 32;
-            const uint64_t per_path = Wavefront::bytes_per_path(S);
+            const uint64_t per_path = Wavefront::bytes_per_path(S) + (guide ? (kGuideMixFloat4 + 8) * 16 : 0);
             const uint64_t target = std::min<uint64_t>(uint64_t{1} << 26, free_b / 4 / per_path);
             spl_req = static_cast<int>(std::clamp<uint64_t>(target / std::max<uint32_t>(1, pixels), 1, 1024));
         }
@@ -960,6 +1082,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
         if (rc != PE_OK) return rc;
+        if (guide) {
+            rc = guide_reserve_paths(guide, wf->capacity);
+            if (rc != PE_OK) return rc;
+        }
 
         LaunchParams lp{};
         lp.width = film->width;
@@ -993,6 +1119,8 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
 
         PathBuffers pb = wf->view();
+        pb.mix = guide ? guide->mix.as<float4>() : nullptr;
+        pb.grec = guide ? guide->path_records.as<float4>() : nullptr;
         hipStream_t s = lane.stream;
         const int gblocks = lane.grid_blocks;
         hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, s, pb);
@@ -1000,14 +1128,29 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         uint32_t* spill = lane.spill;
         const SceneView view = scene->view;
         // One path vertex for every path in flight: closest hit, shade, shadow rays, queue rotation.
-        auto bounce = [&](int cur) {
+        auto bounce = [&](int cur, int it) {
+            const int guide_bounce = guide && lp.guide_fraction > 0.0f && it < guide->cfg.guided_bounces ? 1 : 0;
             wf->prof.begin(kKIntersect, s);
             if (count) hipLaunchKernelGGL(k_intersect<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
             else hipLaunchKernelGGL(k_intersect<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
             wf->prof.end(s);
+            if (guide_bounce) {
+                wf->prof.begin(kKGuide, s);
+                const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, pb.ctr, pb.mix};
+                GuideView gv = guide->view(lp.guide_fraction);
+                gv.inferred = pb.stats + 7;
+                hipLaunchKernelGGL(k_guide_infer<RenderQueries>, dim3(std::max(1, dev->cu_count * 2)),
+                                   dim3(kInferBlock), 0, s, gv, q, lp.num_paths);
+                wf->prof.end(s);
+            }
             wf->prof.begin(kKShade, s);
-            if (lp.transport) hipLaunchKernelGGL(k_shade<true>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
-            else hipLaunchKernelGGL(k_shade<false>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+            if (lp.transport) {
+                if (guide) hipLaunchKernelGGL((k_shade<true, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, guide_bounce);
+                else hipLaunchKernelGGL((k_shade<true, false>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, 0);
+            } else {
+                if (guide) hipLaunchKernelGGL((k_shade<false, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, guide_bounce);
+                else hipLaunchKernelGGL((k_shade<false, false>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, 0);
+            }
             wf->prof.end(s);
             wf->prof.begin(kKShadow, s);
             if (view.has_transmitters) {
@@ -1036,6 +1179,13 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             lp.first_sample = first_sample + done;
             lp.spl = spl;
             lp.num_paths = pixels * static_cast<uint32_t>(spl);
+            // Guiding: the first launch samples the BSDF only and collects; later ones mix in the trained guide.
+            lp.guide_fraction = guide && guide->steps > 0 ? guide->cfg.guide_fraction : 0.0f;
+            lp.guide_records = guide && guide->training() ? 1 : 0;
+            if (lp.guide_records) {
+                rc = guide_clear_records(guide, s);
+                if (rc != PE_OK) return rc;
+            }
             const int gen_blocks = static_cast<int>(std::min<uint32_t>((lp.num_paths + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKGenerate, s);
             hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, lp, pb);
@@ -1048,7 +1198,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             const int planned = lp.max_depth + 1;
             const int limit = planned * static_cast<int>(kMaxWalkSteps + 1) + 2 * 64 + 8;
             for (;;) {
-                for (; it < planned; ++it) bounce(it & 1);
+                for (; it < planned; ++it) bounce(it & 1, it);
                 uint32_t left = 0;
                 PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr, 4, hipMemcpyDeviceToHost, s));
                 PE_HIP_CHECK(hipStreamSynchronize(s));
@@ -1057,7 +1207,12 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                     set_error("pe_render: paths still alive past the bounce, walk and pass-through limits");
                     return PE_ERR_INVALID;
                 }
-                for (int k = 0; k < 4; ++k, ++it) bounce(it & 1);
+                for (int k = 0; k < 4; ++k, ++it) bounce(it & 1, it);
+            }
+            if (lp.guide_records) {
+                const int cb = static_cast<int>(std::min<uint32_t>((lp.num_paths + kBlock - 1) / kBlock, 65535u));
+                hipLaunchKernelGGL(k_guide_collect, dim3(cb), dim3(kBlock), 0, s, lp, pb, guide->records.as<float4>(),
+                                   guide->record_slots, static_cast<uint64_t>(guide->launches) * 0x9e3779b97f4a7c15ull);
             }
             const int film_blocks = static_cast<int>(std::min<uint32_t>((pixels + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKFilm, s);
@@ -1066,10 +1221,19 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                                film->rejected.as<unsigned long long>());
             wf->prof.end(s);
             PE_HIP_CHECK(hipGetLastError());
+            if (guide) {
+                if (lp.guide_records) {
+                    wf->prof.begin(kKTrain, s);
+                    rc = guide_train_from_records(guide, s);
+                    wf->prof.end(s);
+                    if (rc != PE_OK) return rc;
+                }
+                ++guide->launches;
+            }
             done += spl;
             traced += static_cast<uint64_t>(pixels) * spl;
         }
-        unsigned long long st[6] = {0, 0, 0, 0, 0, 0};
+        unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         PE_HIP_CHECK(hipMemcpyAsync(st, pb.stats, sizeof(st), hipMemcpyDeviceToHost, s));
         PE_HIP_CHECK(hipStreamSynchronize(s));
         if (result) {
@@ -1081,6 +1245,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             result->tri_visits = st[3];
             result->shadow_node_visits = st[4];
             result->shadow_tri_visits = st[5];
+        }
+        if (guide) {
+            guide->collected += st[6];
+            guide->inferred += st[7];
         }
         if (cancelled) {
             set_error("pe_render: cancelled");
@@ -1109,9 +1277,32 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
         const int rc = ensure_trace_resources(dev, blocks);
         if (rc != PE_OK) return rc;
         const RenderLane lane{dev->stream, wavefront_of(dev), dev->spill.as<uint32_t>(), blocks};
-        return render_samples(dev, lane, scene, camera, settings, film, first_sample, sample_count, cancel, result);
+        return render_samples(dev, lane, scene, camera, settings, film, first_sample, sample_count, cancel, result,
+                              nullptr);
     } catch (const std::exception& e) {
         set_error(std::string("pe_render: ") + e.what());
+        return PE_ERR_INVALID;
+    }
+}
+
+extern "C" int pe_render_guided(pe_device* dev, pe_scene* scene, pe_guide* guide, const pe_camera* camera,
+                                const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
+                                const volatile int32_t* cancel, pe_batch_result* result) {
+    if (!dev) {
+        if (result) *result = pe_batch_result{};
+        set_error("pe_render_guided: NULL device");
+        return PE_ERR_INVALID;
+    }
+    try {
+        PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+        int blocks = 0;
+        const int rc = ensure_trace_resources(dev, blocks);
+        if (rc != PE_OK) return rc;
+        const RenderLane lane{dev->stream, wavefront_of(dev), dev->spill.as<uint32_t>(), blocks};
+        return render_samples(dev, lane, scene, camera, settings, film, first_sample, sample_count, cancel, result,
+                              guide);
+    } catch (const std::exception& e) {
+        set_error(std::string("pe_render_guided: ") + e.what());
         return PE_ERR_INVALID;
     }
 }

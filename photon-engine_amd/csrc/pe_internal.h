@@ -118,7 +118,7 @@ struct RenderLane {
 };
 int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
                    const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
-                   const volatile int32_t* cancel, pe_batch_result* result);
+                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide = nullptr);
 // Bytes of traversal-stack spill a lane of `grid_blocks` blocks needs (capi.hip).
 size_t trace_spill_bytes(int grid_blocks);
 }  // namespace pe

@@ -107,6 +107,17 @@ class KernelStats(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
 
+class GuideConfig(C.Structure):
+    _fields_ = [("learning_rate", C.c_float), ("guide_fraction", C.c_float), ("batch", C.c_int32),
+                ("steps_per_launch", C.c_int32), ("train_launches", C.c_int32), ("seed", C.c_uint32),
+                ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3), ("weight_clamp", C.c_float),
+                ("guided_bounces", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+PE_GUIDE_INPUTS, PE_GUIDE_HIDDEN, PE_GUIDE_OUTPUTS, PE_GUIDE_LAYERS, PE_GUIDE_LOBES = 160, 64, 64, 5, 8
+PE_GUIDE_PARAMS = 26944
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("chunk_spp", C.c_int32), ("max_chunk_paths", C.c_uint64),
                 ("film_budget", C.c_uint64), ("max_admitted", C.c_int32), ("reserved", C.c_int32 * 5)]
@@ -160,6 +171,15 @@ SIGNATURES = {
                                  _P(C.c_uint32), _P(C.c_uint32)]),
     "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
     "pe_occluded_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_uint8)]),
+    "pe_guide_create": (C.c_int, [_VP, _VP, _P(GuideConfig), _P(_VP)]),
+    "pe_guide_destroy": (C.c_int, [_VP]),
+    "pe_guide_get_params": (C.c_int, [_VP, _P(C.c_float)]),
+    "pe_guide_set_params": (C.c_int, [_VP, _P(C.c_float)]),
+    "pe_guide_infer": (C.c_int, [_VP, C.c_size_t, _P(C.c_float), _P(C.c_float), _P(C.c_float)]),
+    "pe_guide_train_step": (C.c_int, [_VP, C.c_size_t, _P(C.c_float), _P(C.c_float), _P(C.c_float)]),
+    "pe_guide_stats": (C.c_int, [_VP, _P(C.c_uint64), _P(C.c_float), _P(C.c_uint64), _P(C.c_uint64)]),
+    "pe_render_guided": (C.c_int, [_VP, _VP, _VP, _P(Camera), _P(Settings), _VP, C.c_int, C.c_int, _P(C.c_int32),
+                                   _P(BatchResult)]),
     "pe_server_create": (C.c_int, [_VP, _P(ServerConfig), _P(_VP)]),
     "pe_server_destroy": (C.c_int, [_VP]),
     "pe_server_clock": (C.c_double, [_VP]),

@@ -210,6 +210,63 @@ def lights_media_studio(width: int = 1280, height: int = 720, spp: int = 512, ma
     return glass_studio(width, height, spp, max_bounces, led_quads=1000, prism_subdiv=183)
 
 
+# ---- slit room (learned guiding) -------------------------------------------------------------------------
+
+def grid_quad(p0, u, v, nu: int, nv: int) -> Mesh:
+    """The parallelogram p0 + s u + t v (s, t in [0, 1]) split into nu x nv quads (2 nu nv triangles); the
+    triangles face u x v."""
+    s = np.linspace(0.0, 1.0, nu + 1, dtype=np.float64)
+    t = np.linspace(0.0, 1.0, nv + 1, dtype=np.float64)
+    ss, tt = np.meshgrid(s, t, indexing="xy")
+    p = (np.asarray(p0, np.float64)[None, None, :] + ss[..., None] * np.asarray(u, np.float64) +
+         tt[..., None] * np.asarray(v, np.float64)).reshape(-1, 3).astype(np.float32)
+    j, i = np.meshgrid(np.arange(nv), np.arange(nu), indexing="ij")
+    k = (j * (nu + 1) + i).ravel()
+    idx = np.concatenate([np.stack([k, k + 1, k + nu + 2], 1), np.stack([k, k + nu + 2, k + nu + 1], 1)])
+    return Mesh(p, idx.astype(np.uint32))
+
+
+def slit_room(width: int = 1920, height: int = 1080, spp: int = 64, max_bounces: int = 10, seed: int = 4,
+              cell: float = 0.018, albedo: float = 1.0) -> tuple[Scene, View]:
+    """The learned-guiding workload: a closed 4 x 4 x 3 m room (walls, floor and ceiling tessellated into
+    `cell`-sized quads: about 500k triangles at the default), diffuse and glossy Disney surfaces and a few
+    boxes, lit only through a 5 cm gap — a door left ajar in the +x wall, 2.1 m high — by an area light outside
+    it. Everything inside is lit by the thin wedge of light the gap lets in, mostly after one or more diffuse
+    bounces."""
+    rng = np.random.default_rng(seed)
+    sc = Scene()
+    a = float(albedo)
+    wall = sc.add_material(disney((0.72 * a, 0.7 * a, 0.66 * a), 0.0, 0.85, 0.5))
+    floor = sc.add_material(disney((0.45 * a, 0.38 * a, 0.3 * a), 0.0, 0.3, 0.5))
+    ceil = sc.add_material(disney((0.8 * a, 0.8 * a, 0.8 * a), 0.0, 0.9, 0.5))
+    X, Y, Z = 2.0, 3.0, 2.0
+
+    def face(p0, u, v, mat):
+        nu = max(1, int(round(np.linalg.norm(u) / cell)))
+        nv = max(1, int(round(np.linalg.norm(v) / cell)))
+        sc.add_shape(grid_quad(p0, u, v, nu, nv), mat)
+
+    face((-X, 0, -Z), (0, 0, 2 * Z), (2 * X, 0, 0), floor)          # floor, +y
+    face((-X, Y, -Z), (2 * X, 0, 0), (0, 0, 2 * Z), ceil)           # ceiling, -y
+    face((-X, 0, -Z), (2 * X, 0, 0), (0, Y, 0), wall)               # z = -2, +z
+    face((-X, 0, Z), (0, Y, 0), (2 * X, 0, 0), wall)                # z = +2, -z
+    face((-X, 0, -Z), (0, Y, 0), (0, 0, 2 * Z), wall)               # x = -2, +x
+    g0, g1, gh = 0.40, 0.45, 2.1                                     # the gap: z in [g0, g1], y in [0, gh]
+    face((X, 0, -Z), (0, 0, g0 + Z), (0, Y, 0), wall)               # x = +2, left of the gap
+    face((X, 0, g1), (0, 0, Z - g1), (0, Y, 0), wall)               # right of it
+    face((X, gh, g0), (0, 0, g1 - g0), (0, Y - gh, 0), wall)        # above it
+    for k in range(5):                                               # furniture
+        c = rng.uniform([-1.5, -1.5], [1.0, 1.5])
+        sz = rng.uniform([0.3, 0.3, 0.3], [0.8, 1.1, 0.8])
+        glossy = bool(rng.integers(2))
+        m = sc.add_material(disney(rng.uniform(0.2, 0.8, 3), 0.0, 0.15 if glossy else 0.7, 0.5))
+        sc.add_shape(box_mesh((c[0], 0.0, c[1]), (c[0] + sz[0], sz[1], c[1] + sz[2])), m)
+    # Outside: a light 1 m beyond the gap, facing it (-x), wider and taller than the gap.
+    sc.add_light(AreaLight((3.0, 0.2, -0.8), (0.0, 0.0, 2.4), (0.0, 2.4, 0.0), (120.0, 114.0, 100.0)))
+    view = View((-1.7, 1.5, -1.6), (0.9, 0.6, 0.9), (0.0, 1.0, 0.0), 70.0, width, height, spp, max_bounces)
+    return sc, view
+
+
 # ---- product_sphere --------------------------------------------------------------------------------------
 
 def _perlin3(p: np.ndarray, perm: np.ndarray) -> np.ndarray:
