@@ -62,6 +62,7 @@ _ORACLE_SIGS = {
     "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
     "light_select": (C.c_int, [_VP, C.c_size_t, _F, _F, _F, _F, _I, _F, _I]),
     "spectral": (C.c_int, [C.c_size_t, _F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
+    "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
 }
 
 _REF_SIGS = {
@@ -79,6 +80,7 @@ _REF_SIGS = {
                                 C.c_float, C.c_uint64, C.c_size_t, _I, _I, _I, _F]),
     "render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                          C.c_float, C.c_int, C.c_int, C.c_int, _F, _U, _P(C.c_double)]),
+    "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
 }
 
 _cache: dict = {}
@@ -313,7 +315,8 @@ _HIPDEV_SIGS = {
     "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
     "ref_scene_free": (None, [_VP]),
     "hipdev_session_render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
-                                        C.c_int, C.c_float, C.c_int, C.c_int, _F, _U, _I, C.c_char_p, C.c_int]),
+                                        C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, _F, _U, _I, C.c_char_p,
+                                        C.c_int]),
     "hipdev_enumerate": (C.c_int, [C.c_char_p, C.c_int]),
 }
 
@@ -342,7 +345,7 @@ class HipDeviceSession:
             self._lib.ref_scene_free(self.h)
             self.h = None
 
-    def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0):
+    def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0, ao_strength=0.0):
         """Returns (rc, mean (H,W,3), counts (H,W), batches, error text)."""
         f3 = C.c_float * 3
         mean = np.zeros((view.height, view.width, 3), np.float32)
@@ -351,7 +354,8 @@ class HipDeviceSession:
         err = C.create_string_buffer(512)
         rc = self._lib.hipdev_session_render(
             self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
-            view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, _fp(mean),
+            view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, ao_strength,
+            _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
 
@@ -365,3 +369,15 @@ def spectral(u, nd=1.5, abbe=30.0, wavelengths=()):
     ior = np.zeros(wl.shape[0], np.float32)
     oracle().oracle_spectral(u.shape[0], _fp(u), _fp(lam), _fp(w), nd, abbe, wl.shape[0], _fp(wl), _fp(ior))
     return lam, w, ior
+
+
+def thin_film(which: str, cos1, n1, n2, n3, thickness, lam):
+    """thinFilmReflectance for arrays of cos, thickness, wavelength: which = "oracle" or "ref_strict"."""
+    lib = oracle() if which == "oracle" else reference(strict=True)
+    cos1 = np.ascontiguousarray(cos1, np.float32)
+    t = np.ascontiguousarray(np.broadcast_to(thickness, cos1.shape), np.float32)
+    la = np.ascontiguousarray(np.broadcast_to(lam, cos1.shape), np.float32)
+    out = np.zeros(cos1.shape[0], np.float32)
+    fn = lib.oracle_thin_film if which == "oracle" else lib.ref_thin_film
+    fn(cos1.shape[0], _fp(cos1), n1, n2, float(np.real(n3)), float(np.imag(n3)), _fp(t), _fp(la), _fp(out))
+    return out

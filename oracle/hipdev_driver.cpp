@@ -29,12 +29,14 @@ void copy_error(const std::string& msg, char* err, int errlen) {
 extern "C" {
 
 // RenderSession(Final, targetSpp = spp) on a HipDevice for the reference scene `h`; writes the resolved mean
-// (3 floats/px), the sample counts and the number of session batches. Returns 0 on success, 1 when the device
+// (3 floats/px), the sample counts and the number of session batches. aoStrength > 0 asks for a setting the
+// adapter refuses. Returns 0 on success, 1 when the device
 // refused the scene or a batch (err says why), 2 when no HIP device could be created.
 HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* camTarget, const float* camUp,
                                      float fov, float aspect, int width, int height, int maxBounces,
                                      int shadowQuality, int filterType, float filterWidth, int spp, int maxBatch,
-                                     float* mean, uint32_t* counts, int* batches, char* err, int errlen) {
+                                     float aoStrength, float* mean, uint32_t* counts, int* batches, char* err,
+                                     int errlen) {
     std::string e;
     std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
     if (!dev) {
@@ -57,6 +59,7 @@ HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* 
     s.shadowQuality = shadowQuality;
     s.filter.type = static_cast<PixelFilterType>(filterType);
     s.filter.width = filterWidth;
+    s.aoStrength = aoStrength;
     SessionConfig cfg;
     cfg.mode = RenderMode::Final;
     cfg.targetSpp = spp;

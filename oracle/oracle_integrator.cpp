@@ -1492,6 +1492,13 @@ using namespace oracle;
 
 #define ORACLE_API extern "C" __attribute__((visibility("default")))
 
+// thinFilmReflectance(cos[i], n1, n2, n3, thickness[i], lambda[i]) for n queries.
+ORACLE_API int oracle_thin_film(size_t n, const float* cos1, float n1, float n2, float n3re, float n3im,
+                                const float* thickness, const float* lambda, float* out) {
+    for (size_t i = 0; i < n; ++i) out[i] = thinFilmReflectanceC(cos1[i], n1, n2, n3re, n3im, thickness[i], lambda[i]);
+    return 0;
+}
+
 // sampleWavelength for n values of u (lambda[i], weight[3i..3i+2]) and cauchyIor(nd, abbe, lambda) at m wavelengths.
 ORACLE_API int oracle_spectral(size_t n, const float* u, float* lambda, float* weight, float nd, float abbe, size_t m,
                                const float* wl, float* ior) {

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <complex>
 #include <cstring>
 #include <stdexcept>
 
@@ -288,6 +289,7 @@ struct Params {
     bool hasCustomSheenColor;
     float anisotropy, sheen, sheenRoughness, diffuseRoughness, diffuseTransmission, subsurface;
     V3 edgeTint;
+    float thinFilmThickness, thinFilmIor;
     V3 normal, tangent;
     bool frontFace;
 };
@@ -296,7 +298,6 @@ V3 c3(const float* v) { return V3(v[0], v[1], v[2]); }
 V3 clamp01(V3 c) { return V3(clampf(c.x, 0.0f, 1.0f), clampf(c.y, 0.0f, 1.0f), clampf(c.z, 0.0f, 1.0f)); }
 
 Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.cpp:543-603 (no textures, no flakes)
-    if (d.thin_film_thickness > 0.0f) throw std::runtime_error("oracle: thin-film Disney layers are not restated");
     Params p;
     p.baseColor = c3(d.base_color);
     p.metallic = d.metallic;
@@ -317,6 +318,8 @@ Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.c
     p.diffuseTransmission = clampf(d.diffuse_transmission, 0.0f, 1.0f);
     p.subsurface = clampf(d.subsurface, 0.0f, 1.0f);
     p.edgeTint = clamp01(c3(d.edge_tint));
+    p.thinFilmThickness = clampf(d.thin_film_thickness, 0.0f, 2000.0f);  // setThinFilm (disney.h:111-114)
+    p.thinFilmIor = clampf(d.thin_film_ior, 1.0f, 3.0f);
     p.normal = sp.normal;
     p.tangent = sp.tangent;
     p.frontFace = sp.frontFace;
@@ -370,12 +373,67 @@ Frame shadingFrame(V3 nIn, V3 tangentIn) {  // :62-73
     t = normalized(cross(b, n));
     return Frame(t, b, n);
 }
-V3 specularFresnel(const Params& p, float cosD) {  // :79-96 (thin film excluded above)
+// ------------------------------------------------------------------ thin-film interference (fresnel.h:49-107)
+using cfloat = std::complex<float>;
+
+cfloat gulbrandsenIor(float r, float g) {  // fresnel.h:49-58: metal n + ik from reflectivity and edge tint
+    r = clampf(r, 0.0f, 0.99f);
+    g = clampf(g, 0.0f, 1.0f);
+    const float sr = std::sqrt(r);
+    const float nMin = (1.0f - r) / (1.0f + r);
+    const float nMax = (1.0f + sr) / (1.0f - sr);
+    const float n = g * nMin + (1.0f - g) * nMax;
+    const float k2 = (r * (n + 1.0f) * (n + 1.0f) - (n - 1.0f) * (n - 1.0f)) / (1.0f - r);
+    return {n, std::sqrt(std::max(0.0f, k2))};
+}
+
+// Airy sum of the waves reflected inside a film n2 of thickness d between n1 and a (complex) base n3,
+// unpolarised (fresnel.h:74-92).
+float thinFilmReflectance(float cos1, float n1, float n2, cfloat n3, float thicknessNm, float lambdaNm) {
+    cos1 = clampf(cos1, 0.0f, 1.0f);
+    const float sin1sq = 1.0f - cos1 * cos1;
+    const float sin2sq = (n1 / n2) * (n1 / n2) * sin1sq;
+    if (sin2sq >= 1.0f) return 1.0f;
+    const float cos2 = std::sqrt(1.0f - sin2sq);
+    const cfloat cos3 = std::sqrt(cfloat(1.0f) - (n1 * n1 * sin1sq) / (n3 * n3));
+    const float r12s = (n1 * cos1 - n2 * cos2) / (n1 * cos1 + n2 * cos2);
+    const float r12p = (n2 * cos1 - n1 * cos2) / (n2 * cos1 + n1 * cos2);
+    const cfloat r23s = (n2 * cos2 - n3 * cos3) / (n2 * cos2 + n3 * cos3);
+    const cfloat r23p = (n3 * cos2 - n2 * cos3) / (n3 * cos2 + n2 * cos3);
+    const float delta = 4.0f * PI * n2 * thicknessNm * cos2 / lambdaNm;
+    const cfloat e = std::polar(1.0f, delta);
+    const cfloat rs = (r12s + r23s * e) / (1.0f + r12s * r23s * e);
+    const cfloat rp = (r12p + r23p * e) / (1.0f + r12p * r23p * e);
+    return clampf(0.5f * (std::norm(rs) + std::norm(rp)), 0.0f, 1.0f);
+}
+
+V3 thinFilmRGB(float cos1, float filmIor, const cfloat baseIor[3], float thicknessNm) {  // :94-107
+    static constexpr float kLambda[3][3] = {{610.0f, 640.0f, 670.0f}, {515.0f, 540.0f, 565.0f}, {440.0f, 465.0f, 490.0f}};
+    V3 out;
+    for (int c = 0; c < 3; ++c) {
+        float s = 0.0f;
+        for (float l : kLambda[c]) s += thinFilmReflectance(cos1, 1.0f, filmIor, baseIor[c], thicknessNm, l);
+        out[c] = s / 3.0f;
+    }
+    return out;
+}
+
+V3 specularFresnel(const Params& p, float cosD) {  // :79-96
     const float f0d = dielectricF0(p);
     const float w = dielectricWeight(p);
-    const V3 fd(w * schlickF(f0d, cosD));
+    if (p.thinFilmThickness <= 0.0f) {
+        const V3 fd(w * schlickF(f0d, cosD));
+        if (p.metallic <= 0.0f) return fd;
+        return lerpc(fd, fresnelF82Tint(p.baseColor, p.edgeTint, cosD), p.metallic);
+    }
+    const float sd = std::sqrt(clampf(f0d, 0.0f, 0.99f));
+    const cfloat nd((1.0f + sd) / (1.0f - sd), 0.0f);
+    const cfloat dielectric[3] = {nd, nd, nd};
+    const V3 fd = thinFilmRGB(cosD, p.thinFilmIor, dielectric, p.thinFilmThickness) * w;
     if (p.metallic <= 0.0f) return fd;
-    return lerpc(fd, fresnelF82Tint(p.baseColor, p.edgeTint, cosD), p.metallic);
+    cfloat metal[3];
+    for (int c = 0; c < 3; ++c) metal[c] = gulbrandsenIor(p.baseColor[c], p.edgeTint[c]);
+    return lerpc(fd, thinFilmRGB(cosD, p.thinFilmIor, metal, p.thinFilmThickness), p.metallic);
 }
 V3 multiScatterFactor(const Params& p, float cosO, float ax, float ay) {  // :99-103
     const float E = ggxDirectionalAlbedo(cosO, std::sqrt(ax * ay));
@@ -752,21 +810,28 @@ float pdfRoughLocal(V3 wo, V3 wi, float ior, float alpha) {  // :105-119
     return visible * std::abs(wiDotH) / (s * s) * (1.0f - R);
 }
 
-struct Glass {  // Dielectric(ior, tint, roughness) + setAbbe / setThinWalled as ref_driver builds it
-    float ior, roughness, abbe;
+struct Glass {  // Dielectric(ior, tint, roughness) + setAbbe / setThinWalled / setThinFilm as ref_driver builds it
+    float ior, roughness, abbe, film;
     V3 tint;
     bool thin;
     explicit Glass(const pe_material_desc& d)
         : ior(d.ior), roughness(d.roughness), abbe(d.abbe > 0.0f ? std::max(d.abbe, 10.0f) : 0.0f),
-          tint(c3(d.base_color)), thin(d.thin_walled != 0) {}
+          film(d.thin_film_thickness > 0.0f ? d.thin_film_thickness : 0.0f), tint(c3(d.base_color)),
+          thin(d.thin_walled != 0) {}
     float iorAt(float wavelength) const {  // :174-177
         if (abbe <= 0.0f || wavelength <= 0.0f) return ior;
         return std::max(1.0001f, cauchyIor(ior, abbe, wavelength));
     }
-    float thinWallReflectance(float cosTheta) const {  // :153-161 (no interference film)
+    // A soap-bubble film (air | film | air) when the wall has a film, else the slab's incoherent sum
+    // 2R / (1 + R) (:150-158).
+    V3 thinWallReflectance(float cosTheta) const {
         cosTheta = clampf(std::abs(cosTheta), 0.0f, 1.0f);
+        if (film > 0.0f) {
+            const cfloat air[3] = {cfloat(1.0f), cfloat(1.0f), cfloat(1.0f)};
+            return thinFilmRGB(cosTheta, ior, air, film);
+        }
         const float R = fresnelDielectric(cosTheta, 1.0f, ior);
-        return 2.0f * R / (1.0f + R);
+        return V3(2.0f * R / (1.0f + R));
     }
 };
 
@@ -830,8 +895,8 @@ bool glassSampleFull(const Glass& m, V3 wo, const ShadingPoint& sp, float uc, fl
     const V3 n = sp.normal;
     const float cosO = dot(wo, n);
     if (cosO == 0.0f) return false;
-    const float R = m.thinWallReflectance(cosO);
-    const float pr = clampf((R + R + R) / 3.0f, 1e-4f, 1.0f - 1e-4f);
+    const V3 R = m.thinWallReflectance(cosO);
+    const float pr = clampf((R.x + R.y + R.z) / 3.0f, 1e-4f, 1.0f - 1e-4f);
     const float c = std::abs(cosO);
     if (uc < pr) {
         out.wi = normalized(n * (2.0f * cosO) - wo);
@@ -840,7 +905,7 @@ bool glassSampleFull(const Glass& m, V3 wo, const ShadingPoint& sp, float uc, fl
     } else {
         out.wi = -wo;
         out.pdf = 1.0f - pr;
-        out.f = m.tint * (1.0f - R) / c;
+        out.f = m.tint * (V3(1.0f) - R) / c;
         out.passThrough = true;
     }
     return true;
@@ -892,7 +957,7 @@ V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp
     if (!materialTransmitsShadows(m)) return V3(0.0f);
     const Glass g(m);
     const V3 n = lengthSq(sp.ng) > 0.0f ? normalized(sp.ng) : sp.normal;
-    return (V3(1.0f) - V3(g.thinWallReflectance(dot(dir, n)))) * g.tint;
+    return (V3(1.0f) - g.thinWallReflectance(dot(dir, n))) * g.tint;
 }
 
 bool subsurfaceCoeffs(const pe_material_desc& m, const ShadingPoint& sp, V3& sigmaT, V3& albedo, float& g) {
@@ -973,6 +1038,10 @@ bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float 
     if (m.type == PE_MAT_DIELECTRIC) return glassSampleFull(Glass(m), wo, sp, uc, u0, u1, out);
     if (m.type != PE_MAT_DISNEY) throw std::runtime_error("oracle: material type not restated");
     return disneySampleFull(resolve(m, sp), wo, uc, u0, u1, out);
+}
+
+float thinFilmReflectanceC(float cos1, float n1, float n2, float n3re, float n3im, float thicknessNm, float lambdaNm) {
+    return thinFilmReflectance(cos1, n1, n2, cfloat(n3re, n3im), thicknessNm, lambdaNm);
 }
 
 }  // namespace oracle

@@ -98,4 +98,7 @@ V3 materialEmitted(const pe_material_desc& m);
 V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, float& pdf);
 bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1, BsdfSample& out);
 
+
+// Airy thin-film reflectance n1 | film (n2, thickness) | base n3 = (n3re, n3im) at one wavelength (fresnel.h:74-92).
+float thinFilmReflectanceC(float cos1, float n1, float n2, float n3re, float n3im, float thicknessNm, float lambdaNm);
 }  // namespace oracle

@@ -15,6 +15,7 @@
 #include "lights/area_light.h"
 #include "lights/environment_light.h"
 #include "materials/dielectric.h"
+#include "materials/fresnel.h"
 #include "materials/disney.h"
 #include "materials/lambertian.h"
 #include "materials/microfacet.h"
@@ -42,6 +43,7 @@ std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
         if (d.abbe > 0.0f) m->setAbbe(d.abbe);
         if (d.absorption_distance > 0.0f) m->setAbsorption(c3(d.absorption_color), d.absorption_distance);
         m->setThinWalled(d.thin_walled != 0);
+        if (d.thin_film_thickness > 0.0f) m->setThinFilm(d.thin_film_thickness);
         return m;
     }
     auto m = std::make_shared<DisneyMaterial>(c3(d.base_color), d.metallic, d.roughness, d.specular);
@@ -165,6 +167,14 @@ REF_API void* ref_scene_build(const pe_scene_desc* desc) {
 }
 
 REF_API void ref_scene_free(void* h) { delete static_cast<RefScene*>(h); }
+
+// The reference's own thinFilmReflectance (materials/fresnel.h) for n queries.
+REF_API int ref_thin_film(size_t n, const float* cos1, float n1, float n2, float n3re, float n3im, const float* thickness,
+                          const float* lambda, float* out) {
+    for (size_t i = 0; i < n; ++i)
+        out[i] = thinFilmReflectance(cos1[i], n1, n2, cfloat(n3re, n3im), thickness[i], lambda[i]);
+    return 0;
+}
 
 REF_API int ref_scene_counts(void* h, int* shapes, int* lights) {
     auto* rs = static_cast<RefScene*>(h);

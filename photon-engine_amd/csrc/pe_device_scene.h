@@ -61,6 +61,14 @@ struct alignas(16) GpuMaterial {
     float sss_g;
     V3 sss_albedo;
     float pad5;
+    // Thin-film interference (fresnel.h:74-107): film thickness (nm) and index; the base under the film:
+    // Disney dielectric base index (1 + sqrt f0) / (1 - sqrt f0) and metal n + ik per channel (gulbrandsenIor);
+    // a thin-walled Dielectric's film sits between air on both sides.
+    float film_thickness, film_ior, film_nd, pad6;
+    V3 film_metal_n;
+    float pad7;
+    V3 film_metal_k;
+    float pad8;
 };
 
 enum : uint32_t {
@@ -72,6 +80,7 @@ enum : uint32_t {
     kMatFlagThinWall = 32u,          // thin-walled dielectric: pass-through and shadow transmittance
     kMatFlagDispersive = 64u,        // isDispersive
     kMatFlagMedium = 128u,           // interior medium absorbs (sigma_a != 0)
+    kMatFlagFilm = 256u,             // thin-film interference on the specular Fresnel / the thin wall
 };
 
 // AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.

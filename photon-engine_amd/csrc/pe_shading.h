@@ -308,6 +308,72 @@ __device__ __forceinline__ float pdf_spec_trans(const GpuMaterial& m, bool front
     return visible * fabsf(wi_h) / (s * s);
 }
 
+// ----------------------------------------------------------------- thin-film interference (fresnel.h:74-107)
+struct Cf {
+    float re, im;
+};
+__device__ __forceinline__ Cf cf(float re, float im = 0.0f) { return Cf{re, im}; }
+__device__ __forceinline__ Cf operator+(Cf a, Cf b) { return Cf{a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ Cf operator-(Cf a, Cf b) { return Cf{a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ Cf operator*(Cf a, Cf b) { return Cf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ Cf operator*(float s, Cf a) { return Cf{s * a.re, s * a.im}; }
+__device__ __forceinline__ Cf operator/(Cf a, Cf b) {
+    const float d = b.re * b.re + b.im * b.im;
+    return Cf{(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+__device__ __forceinline__ float cnorm(Cf a) { return a.re * a.re + a.im * a.im; }
+__device__ __forceinline__ Cf csqrt(Cf z) {  // principal square root
+    const float r = sqrtf(cnorm(z));
+    const float re = sqrtf(fmaxf(0.0f, 0.5f * (r + z.re)));
+    const float im = sqrtf(fmaxf(0.0f, 0.5f * (r - z.re)));
+    return Cf{re, z.im < 0.0f ? -im : im};
+}
+
+// Unpolarised reflectance of air | film (n2, thickness nm) | base n3 at one wavelength: the Airy sum
+// r = (r12 + r23 e^{i delta}) / (1 + r12 r23 e^{i delta}), delta = 4 pi n2 d cos(theta2) / lambda.
+__device__ __forceinline__ float thin_film_reflectance(float cos1, float n2, Cf n3, float thickness, float lambda) {
+    const float n1 = 1.0f;
+    cos1 = clampf(cos1, 0.0f, 1.0f);
+    const float sin1sq = 1.0f - cos1 * cos1;
+    const float sin2sq = (n1 / n2) * (n1 / n2) * sin1sq;
+    if (sin2sq >= 1.0f) return 1.0f;
+    const float cos2 = sqrtf(1.0f - sin2sq);
+    const Cf cos3 = csqrt(cf(1.0f) - cf(n1 * n1 * sin1sq) / (n3 * n3));
+    const float r12s = (n1 * cos1 - n2 * cos2) / (n1 * cos1 + n2 * cos2);
+    const float r12p = (n2 * cos1 - n1 * cos2) / (n2 * cos1 + n1 * cos2);
+    const Cf r23s = (cf(n2 * cos2) - n3 * cos3) / (cf(n2 * cos2) + n3 * cos3);
+    const Cf r23p = (cos2 * n3 - n2 * cos3) / (cos2 * n3 + n2 * cos3);
+    const float delta = 4.0f * kPi * n2 * thickness * cos2 / lambda;
+    float sd, cd;
+    sincosf(delta, &sd, &cd);
+    const Cf e = cf(cd, sd);
+    const Cf rs = (cf(r12s) + r23s * e) / (cf(1.0f) + r12s * (r23s * e));
+    const Cf rp = (cf(r12p) + r23p * e) / (cf(1.0f) + r12p * (r23p * e));
+    return clampf(0.5f * (cnorm(rs) + cnorm(rp)), 0.0f, 1.0f);
+}
+
+// Each channel averages three wavelengths (fresnel.h:94-107).
+__device__ __forceinline__ V3 thin_film_rgb(float cos1, float film_ior, const Cf base[3], float thickness) {
+    const float lambda[3][3] = {{610.0f, 640.0f, 670.0f}, {515.0f, 540.0f, 565.0f}, {440.0f, 465.0f, 490.0f}};
+    float out[3];
+    for (int c = 0; c < 3; ++c) {
+        float s = 0.0f;
+        for (int k = 0; k < 3; ++k) s += thin_film_reflectance(cos1, film_ior, base[c], thickness, lambda[c][k]);
+        out[c] = s / 3.0f;
+    }
+    return mk(out[0], out[1], out[2]);
+}
+
+// specularFresnel with a film (disney.cpp:85-93).
+__device__ __forceinline__ V3 film_specular_fresnel(const GpuMaterial& m, float cos_d) {
+    const Cf diel[3] = {cf(m.film_nd), cf(m.film_nd), cf(m.film_nd)};
+    const V3 fd = thin_film_rgb(cos_d, m.film_ior, diel, m.film_thickness) * m.wd;
+    if (m.metallic <= 0.0f) return fd;
+    const Cf metal[3] = {cf(m.film_metal_n.x, m.film_metal_k.x), cf(m.film_metal_n.y, m.film_metal_k.y),
+                         cf(m.film_metal_n.z, m.film_metal_k.z)};
+    return lerp3(fd, thin_film_rgb(cos_d, m.film_ior, metal, m.film_thickness), m.metallic);
+}
+
 // evalLobes (disney.cpp:223-311) in the local frame: reflection above the surface; diffuse and specular
 // transmission below it. kTransport = false is the variant for scenes whose materials all have zero
 // transmission weights, where the lobes below the surface are exactly zero.
@@ -339,9 +405,14 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
     }
     const float D = ggx_d(h, m.ax, m.ay);
     const float G = ggx_g(wo, wi, m.ax, m.ay);
-    // specularFresnel (disney.cpp:79-96) without thin film
-    V3 F = splat(m.wd * schlick_f(m.f0d, cos_d));
-    if (m.metallic > 0.0f) F = lerp3(F, f82_tint(m.base_color, m.edge_tint, cos_d), m.metallic);
+    // specularFresnel (disney.cpp:79-96); films only in the transport variant (a film marks the scene so)
+    V3 F;
+    if (kTransport && (m.flags & kMatFlagFilm)) {
+        F = film_specular_fresnel(m, cos_d);
+    } else {
+        F = splat(m.wd * schlick_f(m.f0d, cos_d));
+        if (m.metallic > 0.0f) F = lerp3(F, f82_tint(m.base_color, m.edge_tint, cos_d), m.metallic);
+    }
     if (kTransport && m.st_w > 0.0f && !front) F = lerp3(F, splat(fresnel_dielectric(-cos_d, 1.0f, m.ior)), m.st_w);
     // multiScatterFactor (disney.cpp:99-103)
     const float E = ggx_directional_albedo(T, cos_o, sqrtf(m.ax * m.ay));
@@ -457,16 +528,19 @@ __device__ __forceinline__ float glass_pdf_rough(V3 wo, V3 wi, float ior, float 
 
 // The thin wall's reflectance: a two-sided slab with incoherent inner reflections, 2R / (1 + R)
 // (thinWallReflectance, dielectric.cpp:153-161, no interference film).
-__device__ __forceinline__ float thin_wall_reflectance(const GpuMaterial& m, float cos_theta) {
+__device__ __forceinline__ V3 thin_wall_reflectance(const GpuMaterial& m, float cos_theta) {
     cos_theta = clampf(fabsf(cos_theta), 0.0f, 1.0f);
+    if (m.flags & kMatFlagFilm) {  // a soap-bubble film: air | film | air
+        const Cf air[3] = {cf(1.0f), cf(1.0f), cf(1.0f)};
+        return thin_film_rgb(cos_theta, m.film_ior, air, m.film_thickness);
+    }
     const float R = fresnel_dielectric(cos_theta, 1.0f, m.ior);
-    return 2.0f * R / (1.0f + R);
+    return splat(2.0f * R / (1.0f + R));
 }
-
 // Dielectric::shadowTransmittance (dielectric.cpp:163-167) of a thin wall crossed in direction dir.
 __device__ __forceinline__ V3 thin_wall_transmittance(const GpuMaterial& m, const Surface& si, V3 dir) {
     const V3 n = length_sq(si.ng) > 0.0f ? normalized(si.ng) : si.normal;
-    return (splat(1.0f) - splat(thin_wall_reflectance(m, dot(dir, n)))) * m.base_color;
+    return (splat(1.0f) - thin_wall_reflectance(m, dot(dir, n))) * m.base_color;
 }
 
 // ------------------------------------------------------------------------------------ material dispatch
@@ -585,8 +659,8 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         const V3 n = si.normal;
         const float cos_o = dot(wo, n);
         if (cos_o == 0.0f) return false;
-        const float R = thin_wall_reflectance(m, cos_o);
-        const float pr = clampf((R + R + R) / 3.0f, 1e-4f, 1.0f - 1e-4f);
+        const V3 R = thin_wall_reflectance(m, cos_o);
+        const float pr = clampf((R.x + R.y + R.z) / 3.0f, 1e-4f, 1.0f - 1e-4f);
         const float c = fabsf(cos_o);
         if (uc < pr) {
             out.wi = normalized(n * (2.0f * cos_o) - wo);
@@ -595,7 +669,7 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         } else {
             out.wi = -wo;
             out.pdf = 1.0f - pr;
-            out.f = m.base_color * (1.0f - R) / c;
+            out.f = m.base_color * (splat(1.0f) - R) / c;
             out.pass_through = true;
         }
         return true;

@@ -51,9 +51,10 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
         if (thin) g.flags |= kMatFlagThinWall;
         else g.flags |= kMatFlagVolume;
         if (g.abbe > 0.0f) g.flags |= kMatFlagDispersive;
-        if (d.thin_film_thickness > 0.0f) {
-            err = "Dielectric thin-film interference is not implemented on the gfx950 path";
-            return PE_ERR_UNSUPPORTED;
+        if (d.thin_film_thickness > 0.0f && thin) {  // setThinFilm (dielectric.h:46); only a thin wall uses it
+            g.flags |= kMatFlagFilm;
+            g.film_thickness = d.thin_film_thickness;
+            g.film_ior = d.ior;
         }
         if (!thin && d.absorption_distance > 0.0f) {
             const V3 color = clamp01v(c3(d.absorption_color));
@@ -67,10 +68,6 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
     if (d.type != PE_MAT_DISNEY) {
         err = "unknown material type " + std::to_string(d.type);
         return PE_ERR_INVALID;
-    }
-    if (d.thin_film_thickness > 0.0f) {
-        err = "Disney thin-film interference is not implemented on the gfx950 path";
-        return PE_ERR_UNSUPPORTED;
     }
     g.type = kMatDisney;
     g.base_color = c3(d.base_color);
@@ -169,6 +166,30 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
         g.sss_sigma_t = mk(st[0], st[1], st[2]);
         g.sss_albedo = mk(alb[0], alb[1], alb[2]);
         g.sss_g = gg;
+    }
+    // Thin film on the base specular lobe (setThinFilm, disney.h:111-114; specularFresnel, disney.cpp:79-96):
+    // the dielectric base index from f0 and the metal base n + ik per channel (gulbrandsenIor, fresnel.h:49-58).
+    const float film = clampf(d.thin_film_thickness, 0.0f, 2000.0f);
+    if (film > 0.0f) {
+        g.flags |= kMatFlagFilm;
+        g.film_thickness = film;
+        g.film_ior = clampf(d.thin_film_ior, 1.0f, 3.0f);
+        const float sd = std::sqrt(clampf(g.f0d, 0.0f, 0.99f));
+        g.film_nd = (1.0f + sd) / (1.0f - sd);
+        float mn[3], mkk[3];
+        for (int c = 0; c < 3; ++c) {
+            const float r = clampf(comp(g.base_color, c), 0.0f, 0.99f);
+            const float gt = clampf(comp(g.edge_tint, c), 0.0f, 1.0f);
+            const float sr = std::sqrt(r);
+            const float n_min = (1.0f - r) / (1.0f + r);
+            const float n_max = (1.0f + sr) / (1.0f - sr);
+            const float n = gt * n_min + (1.0f - gt) * n_max;
+            const float k2 = (r * (n + 1.0f) * (n + 1.0f) - (n - 1.0f) * (n - 1.0f)) / (1.0f - r);
+            mn[c] = n;
+            mkk[c] = std::sqrt(std::fmax(0.0f, k2));
+        }
+        g.film_metal_n = mk(mn[0], mn[1], mn[2]);
+        g.film_metal_k = mk(mkk[0], mkk[1], mkk[2]);
     }
     return PE_OK;
 }
@@ -423,7 +444,8 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         transmitters = transmitters || (g.flags & kMatFlagThinWall) != 0;
         transport = transport || g.type == kMatDielectric || g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f ||
                     g.p_spec_trans > 0.0f || g.p_sss > 0.0f ||
-                    (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium)) != 0;
+                    (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium |
+                                kMatFlagFilm)) != 0;
     }
     hipStream_t s = dev->stream;
     if (!sc.materials.upload(mats.data(), mats.size() * sizeof(GpuMaterial), s) ||

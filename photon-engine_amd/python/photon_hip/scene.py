@@ -48,9 +48,9 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
 
 
 def dielectric(ior=1.5, tint=(1.0, 1.0, 1.0), roughness=0.0, abbe=0.0, absorption_color=None,
-               absorption_distance=0.0, thin_walled=False) -> capi.MaterialDesc:
+               absorption_distance=0.0, thin_walled=False, thin_film_thickness=0.0) -> capi.MaterialDesc:
     """Dielectric(ior, tint, roughness) (dielectric.h:18-20) with setAbbe (:74), setAbsorption(color, distance)
-    (:57) when absorption_color is given, and setThinWalled (:43)."""
+    (:57) when absorption_color is given, setThinWalled (:43) and setThinFilm(thickness nm) (:46)."""
     m = disney(tint)
     m.type = capi.PE_MAT_DIELECTRIC
     m.ior, m.roughness, m.abbe = float(ior), float(roughness), float(abbe)
@@ -58,6 +58,7 @@ def dielectric(ior=1.5, tint=(1.0, 1.0, 1.0), roughness=0.0, abbe=0.0, absorptio
         m.absorption_color[:] = list(map(float, absorption_color))
         m.absorption_distance = float(absorption_distance)
     m.thin_walled = 1 if thin_walled else 0
+    m.thin_film_thickness = float(thin_film_thickness)
     return m
 
 

@@ -72,18 +72,10 @@ def test_session_through_hip_device_matches_direct_render(dev, name):
 
 @pytest.mark.gpu
 def test_hip_device_refuses_what_it_does_not_implement(dev):
+    """Settings the HIP path does not implement (here ambient occlusion) make the batch fail with a message
+    instead of an approximation; the session reports it."""
     _harness_or_skip()
-    sc = Scene()
-    film = disney((0.8, 0.8, 0.8), 0.0, 0.3, 0.5)
-    film.thin_film_thickness = 400.0
-    film.thin_film_ior = 1.3
-    m = sc.add_material(film)
-    floor = sc.add_material(lambert((0.5, 0.5, 0.5)))
-    quad = Mesh(np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1]], np.float32),
-                np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
-    sc.add_shape(quad, floor)
-    sc.add_shape(Mesh(quad.positions + np.float32([0, 0.5, 0]), quad.indices), m)
-    _, view = configs.box_icosphere(level=1, width=8, height=8)
-    rc, _, _, _, err = bindings.HipDeviceSession(sc).render(view, 1)
+    sc, view = configs.box_icosphere(level=1, width=8, height=8)
+    rc, _, _, _, err = bindings.HipDeviceSession(sc).render(view, 1, ao_strength=0.5)
     assert rc == 1
-    assert "PE_ERR_UNSUPPORTED" in err or "not implemented" in err, err
+    assert "not implemented" in err, err
