@@ -173,3 +173,25 @@ def test_gpu_cancel_and_errors(dev):
     assert lib.pe_render(dev.h, gs.h, C.byref(cam), C.byref(st), film.h, -1, 1, None, C.byref(bad)) == capi.PE_ERR_INVALID
     cam.lens_radius = 0.1
     assert lib.pe_render(dev.h, gs.h, C.byref(cam), C.byref(st), film.h, 0, 1, None, C.byref(bad)) == capi.PE_ERR_UNSUPPORTED
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["box_icosphere", "product_sphere"])
+def test_gpu_converged_image_matches_reference_renderer(dev, name):
+    """The workload's criterion: a converged 4096-spp GPU image against the reference's own CPU renderer
+    (Renderer::renderSamples with its -O3 -ffast-math build, oracle/_ref) on the same frame, relMSE <= 1e-3."""
+    ref_lib = bindings.reference(strict=False)
+    if ref_lib is None:
+        pytest.skip("oracle/_ref not built (reference tree absent when this tree was built)")
+    if name == "box_icosphere":
+        sc, view = configs.box_icosphere(level=3, width=20, height=20)
+    else:
+        sc, view = configs.product_sphere(width=48, height=27, lon=64, lat=48, env_width=256, env_height=128)
+    spp = 4096
+    s, c, _, rej, _ = _gpu_image(dev, sc, view, spp, spl=512)
+    assert (c == spp).all() and rej == 0
+    gpu = s / c[..., None]
+    mean_r, cnt_r, _ = bindings.RefScene(sc, strict=False).render(view, 0, spp, threads=16)
+    assert (cnt_r == spp).all()
+    e = relmse(gpu, mean_r)
+    assert e <= 1e-3, e
