@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 # Guiding MLP per query: 2 x (160*64 + 4*64*64) multiply-adds (the padded layer shapes the MFMAs compute).
 GUIDE_FLOPS_PER_QUERY = 2 * (160 * 64 + 4 * 64 * 64)
-NODE_BYTES, TRI_BYTES, RAY_IN_BYTES, HIT_OUT_BYTES = 64, 48, 32, 16
+NODE_BYTES, TRI_BYTES, RAY_IN_BYTES, HIT_OUT_BYTES = 128, 48, 32, 16
 
 
 def log(msg: str) -> None:

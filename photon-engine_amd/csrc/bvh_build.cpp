@@ -260,4 +260,85 @@ BvhBuildResult build_bvh(const float* tri_lo, const float* tri_hi, size_t n, con
     return r;
 }
 
+namespace {
+
+struct Child4 {
+    float lo[3], hi[3];
+    uint32_t item;   // leaf item, or the binary node index (interior) until converted
+    bool interior;
+};
+
+Child4 binary_child(const BvhNode& n, int slot) {
+    const float4& xy = slot == 0 ? n.c0xy : n.c1xy;
+    Child4 c{};
+    c.lo[0] = xy.x;
+    c.hi[0] = xy.y;
+    c.lo[1] = xy.z;
+    c.hi[1] = xy.w;
+    c.lo[2] = slot == 0 ? n.cz.x : n.cz.z;
+    c.hi[2] = slot == 0 ? n.cz.y : n.cz.w;
+    const int32_t ref = slot == 0 ? n.ref.x : n.ref.y;
+    const int32_t count = slot == 0 ? n.ref.z : n.ref.w;
+    c.interior = count <= 0;
+    c.item = c.interior ? static_cast<uint32_t>(ref)
+                        : (0x80000000u | (static_cast<uint32_t>(count - 1) << 27) | static_cast<uint32_t>(ref));
+    return c;
+}
+
+int convert4(const std::vector<BvhNode>& n2, int ni, std::vector<BvhNode4>& out, int level, int& depth) {
+    depth = std::max(depth, level);
+    const int at = static_cast<int>(out.size());
+    out.push_back(BvhNode4{});
+    Child4 kids[4];
+    int k = 0;
+    for (int s = 0; s < 2; ++s) {
+        const Child4 c = binary_child(n2[static_cast<size_t>(ni)], s);
+        if (!c.interior) {
+            kids[k++] = c;
+        } else {
+            kids[k++] = binary_child(n2[c.item], 0);
+            kids[k++] = binary_child(n2[c.item], 1);
+        }
+    }
+    float lo[3][4], hi[3][4];
+    uint32_t items[4];
+    for (int j = 0; j < 4; ++j) {
+        if (j >= k) {  // empty slot: never entered (the traversal also checks the item)
+            for (int a = 0; a < 3; ++a) {
+                lo[a][j] = 1.0f;
+                hi[a][j] = -1.0f;
+            }
+            items[j] = 0xFFFFFFFFu;
+            continue;
+        }
+        for (int a = 0; a < 3; ++a) {
+            lo[a][j] = kids[j].lo[a];
+            hi[a][j] = kids[j].hi[a];
+        }
+        items[j] = kids[j].interior ? static_cast<uint32_t>(convert4(n2, static_cast<int>(kids[j].item), out, level + 1, depth))
+                                    : kids[j].item;
+    }
+    BvhNode4& n = out[static_cast<size_t>(at)];
+    n.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+    n.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+    n.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+    n.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+    n.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+    n.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+    n.item = make_uint4(items[0], items[1], items[2], items[3]);
+    n.pad = make_uint4(0u, 0u, 0u, 0u);
+    return at;
+}
+
+}  // namespace
+
+std::vector<BvhNode4> collapse_bvh4(const std::vector<BvhNode>& nodes, int& depth) {
+    std::vector<BvhNode4> out;
+    depth = 0;
+    if (nodes.empty()) return out;
+    out.reserve(nodes.size() / 2 + 1);
+    convert4(nodes, 0, out, 1, depth);
+    return out;
+}
+
 }  // namespace pe

@@ -1,4 +1,4 @@
-// bvh_build.h — host-side SAH build of the two-wide, child-boxes-in-parent BVH that the gfx950
+// bvh_build.h — host-side SAH build of the binary BVH and its collapse into the four-wide tree that the gfx950
 // traversal kernels read (layout in pe_gpu_types.h).
 #pragma once
 
@@ -29,5 +29,10 @@ struct BvhBuildResult {
 
 // tri_lo/tri_hi: per-triangle bounds (3 floats each). Never fails for n >= 1.
 BvhBuildResult build_bvh(const float* tri_lo, const float* tri_hi, size_t n, const BvhBuildOptions& opt);
+
+// The four-wide tree the traversal reads, collapsed from the binary one: a node takes its two children and
+// puts each interior child's two children in its place, so every four-wide level spans two binary levels
+// (a leaf child stays as it is). Depth-first, root at 0; `depth` receives the deepest level (root = 1).
+std::vector<BvhNode4> collapse_bvh4(const std::vector<BvhNode>& nodes, int& depth);
 
 }  // namespace pe

@@ -104,7 +104,7 @@ struct BatchAny : BatchRays {
     __device__ __forceinline__ void any(uint32_t i, const TraceRay&, bool occluded) const { occ[i] = occluded ? 1 : 0; }
 };
 
-__global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode* __restrict__ nodes,
+__global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode4* __restrict__ nodes,
                                                                  const TriAccel* __restrict__ tris, BatchClosest job,
                                                                  uint32_t n, uint32_t* __restrict__ spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode* 
     trace_queue<false, false>(job, n, nodes, tris, st, nv, tv);
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode* __restrict__ nodes,
+__global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* __restrict__ nodes,
                                                                 const TriAccel* __restrict__ tris, BatchAny job,
                                                                 uint32_t n, uint32_t* __restrict__ spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
@@ -406,7 +406,7 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
     job.shade = scene->tri_shade.as<TriShade>();
     job.hits = dhits.as<int4>();
     job.bary = bary ? dbary.as<float2>() : nullptr;
-    hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
+    hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
                        scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
@@ -437,7 +437,7 @@ int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* ra
     BatchAny job{};
     job.r = drays.as<float4>();
     job.occ = docc.as<uint8_t>();
-    hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode>(),
+    hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
                        scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(occluded, docc.get(), n, hipMemcpyDeviceToHost, dev->stream));

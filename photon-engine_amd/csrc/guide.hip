@@ -469,10 +469,22 @@ int pe_guide_create(pe_device* dev, pe_scene* scene, const pe_guide_config* conf
         V3 lo = mk(c.bounds_min[0], c.bounds_min[1], c.bounds_min[2]);
         V3 hi = mk(c.bounds_max[0], c.bounds_max[1], c.bounds_max[2]);
         if (!(hi.x > lo.x && hi.y > lo.y && hi.z > lo.z)) {
-            BvhNode root{};
+            BvhNode4 root{};
             PE_HIP_CHECK(hipMemcpy(&root, scene->nodes.get(), sizeof(root), hipMemcpyDeviceToHost));
-            lo = mk(std::fmin(root.c0xy.x, root.c1xy.x), std::fmin(root.c0xy.z, root.c1xy.z), std::fmin(root.cz.x, root.cz.z));
-            hi = mk(std::fmax(root.c0xy.y, root.c1xy.y), std::fmax(root.c0xy.w, root.c1xy.w), std::fmax(root.cz.y, root.cz.w));
+            const float lx[4] = {root.lox.x, root.lox.y, root.lox.z, root.lox.w};
+            const float hx[4] = {root.hix.x, root.hix.y, root.hix.z, root.hix.w};
+            const float ly[4] = {root.loy.x, root.loy.y, root.loy.z, root.loy.w};
+            const float hy[4] = {root.hiy.x, root.hiy.y, root.hiy.z, root.hiy.w};
+            const float lz[4] = {root.loz.x, root.loz.y, root.loz.z, root.loz.w};
+            const float hz[4] = {root.hiz.x, root.hiz.y, root.hiz.z, root.hiz.w};
+            const uint32_t items[4] = {root.item.x, root.item.y, root.item.z, root.item.w};
+            lo = splat(3.0e38f);
+            hi = splat(-3.0e38f);
+            for (int k = 0; k < 4; ++k) {
+                if (items[k] == 0xFFFFFFFFu) continue;
+                lo = vmin(lo, mk(lx[k], ly[k], lz[k]));
+                hi = vmax(hi, mk(hx[k], hy[k], hz[k]));
+            }
         }
         const V3 ext = hi - lo;
         g->lo = lo;

@@ -136,7 +136,7 @@ struct EnvView {
 enum : int32_t { kPickUniform = 0, kPickExhaustive = 1, kPickBvh = 2 };
 
 struct SceneView {
-    const BvhNode* nodes;
+    const BvhNode4* nodes;  // four-wide, root at 0
     const TriAccel* tris;
     const TriShade* shade;
     const float4* pos;

@@ -22,6 +22,16 @@ struct alignas(16) BvhNode {
 };
 static_assert(sizeof(BvhNode) == 64, "BvhNode must be 64 bytes");
 
+// Four-wide node the traversal reads: the boxes of up to four children, SoA by plane, and the children's
+// traversal items (an interior node's index, a leaf item, or 0xFFFFFFFF for an empty slot).
+struct alignas(16) BvhNode4 {
+    float4 lox, hix, loy, hiy;  // first 64 bytes
+    float4 loz, hiz;            // second 64 bytes
+    uint4 item;
+    uint4 pad;
+};
+static_assert(sizeof(BvhNode4) == 128, "BvhNode4 must be 128 bytes");
+
 struct alignas(16) TriAccel {
     float4 v0;  // xyz = vertex 0, w = parallel threshold 1e-14 |e1|^2 |e2|^2 (bvh.cpp:49-50)
     float4 e1;  // xyz = v1 - v0, w = global triangle id (int bits)

@@ -10,23 +10,27 @@
 // Box test: slab test with the reference's exit pad t1 * 1.0000004 (bvh.cpp:269-283). NaN slabs (0 * inf,
 // a ray in a slab plane) are dropped by fminf/fmaxf instead of the reference's argument-order rule.
 //
-// Scheduling. A traversal is a sequence of items, each one 64-byte gather: an interior node (the boxes of
-// both children) or one triangle of a leaf (48 B, read as 64 B: the triangle array carries 16 B of tail
-// padding). Every lane of a wave handles one item per loop step, so a step costs one memory latency
-// whatever mix of nodes and triangles the lanes hold — traversal here is bound by that latency, not by HBM
-// bandwidth. A lane whose ray has ended takes the next ray of its wave's share at the top of the next step
-// (a ballot ranks the idle lanes), and the new ray's record is loaded together with the first node, so lanes
-// stay busy until the wave's share is used up. The share of wave w is every W-th 64-ray chunk of the queue
-// from chunk w (W = waves in the grid): cheap and expensive regions of the queue are spread over all waves
-// without an atomic counter.
+// Scheduling. A traversal is a sequence of items: a four-wide interior node (the boxes of its up to four
+// children in SoA planes and their items, 128 bytes; the binary SAH tree collapsed so that each node spans two
+// binary levels) or one triangle of a leaf (48 B, read as 64 B: the triangle array carries 16 B of tail
+// padding). Every lane of a wave handles one item per loop step, so a step costs one memory latency whatever
+// mix of nodes and triangles the lanes hold — traversal here is bound by that latency, not by HBM bandwidth,
+// and the four-wide nodes halve the steps a ray needs to reach its leaves. The children a ray enters are
+// ordered by entry distance (a four-element sorting network) and all but the nearest are pushed, farthest
+// first. A lane whose ray has ended takes the next ray of its wave's share at the top of the next step (a
+// ballot ranks the idle lanes), and the new ray's record is loaded together with the root node, so lanes stay
+// busy until the wave's share is used up. The share of wave w is every W-th 64-ray chunk of the queue from
+// chunk w (W = waves in the grid): cheap and expensive regions of the queue are spread over all waves without
+// an atomic counter.
 //
 // Item encoding (stack entries are items too): an interior node is its index; a leaf run is
 // kLeafItem | (remaining triangles - 1) << 27 | first triangle, so a leaf holds at most 16 triangles and
-// a scene at most 2^27 triangles (both enforced at upload).
+// a scene at most 2^27 triangles (both enforced at upload); an empty child slot is kNoItem.
 //
 // Traversal stack: kLdsDepth entries per lane live in LDS, column-major (entry e of lane l at
-// e * kTraceBlock + l, conflict-free), deeper entries spill to a per-lane column of a global buffer. One entry
-// per tree level at most, so the tree depth bound of the build bounds the stack at kStackDepth.
+// e * kTraceBlock + l, conflict-free), deeper entries spill to a per-lane column of a global buffer. At most
+// three entries per four-wide level; the binary build's depth bound (60 SAH levels, then halving to leaves of
+// at most 16 of at most 2^27 triangles: 83 levels, 42 four-wide levels) bounds the stack at kStackDepth.
 #pragma once
 
 #include "pe_gpu_types.h"
@@ -35,7 +39,7 @@
 namespace pe {
 
 constexpr int kLdsDepth = 16;
-constexpr int kStackDepth = 96;
+constexpr int kStackDepth = 128;
 constexpr int kTraceBlock = 256;
 constexpr uint32_t kLeafItem = 0x80000000u;
 constexpr uint32_t kLeafFirstMask = (1u << 27) - 1u;
@@ -90,6 +94,29 @@ __device__ __forceinline__ void box_test2(const float4& c0xy, const float4& c1xy
     t1 = bn;
 }
 
+// Slab test of one child k of a four-wide node (components k of the SoA planes), with the exit pad.
+__device__ __forceinline__ float box_near_far(float lx, float hx, float ly, float hy, float lz, float hz, V3 o, V3 inv,
+                                              float tmin, float tmax, bool& hit) {
+    const float x0 = (lx - o.x) * inv.x, x1 = (hx - o.x) * inv.x;
+    const float y0 = (ly - o.y) * inv.y, y1 = (hy - o.y) * inv.y;
+    const float z0 = (lz - o.z) * inv.z, z1 = (hz - o.z) * inv.z;
+    const float tn = fmaxf(fmaxf(tmin, fminf(x0, x1)), fmaxf(fminf(y0, y1), fminf(z0, z1)));
+    const float tf = fminf(fminf(tmax, fmaxf(x0, x1)), fminf(fmaxf(y0, y1), fmaxf(z0, z1))) * 1.0000004f;
+    hit = tn <= tf;
+    return tn;
+}
+
+__device__ __forceinline__ void cas_item(float& ka, uint32_t& ia, float& kb, uint32_t& ib) {
+    if (kb < ka) {
+        const float tk = ka;
+        ka = kb;
+        kb = tk;
+        const uint32_t ti = ia;
+        ia = ib;
+        ib = ti;
+    }
+}
+
 // Lean Möller–Trumbore (bvh.cpp:302-316) on (v0 | eps, e1, e2). Returns true and t/u/v when
 // tmin <= t <= tmax. Every dot and cross product is evaluated uncontracted (dot_exact / cross_exact): hipcc
 // would otherwise fuse them into FMAs.
@@ -126,7 +153,7 @@ __device__ __forceinline__ uint32_t child_item(int32_t ref, int32_t count) {
 //   any:     job.any(i, ray, occluded)
 // nv / tv accumulate the node and triangle items this lane handled (kCount only).
 template <bool kAny, bool kCount, class Job>
-__device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const BvhNode* __restrict__ nodes,
+__device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const BvhNode4* __restrict__ nodes,
                                             const TriAccel* __restrict__ tris, TraceStack& st, uint32_t& nv,
                                             uint32_t& tv) {
     const uint32_t wpb = blockDim.x >> 6;
@@ -164,11 +191,17 @@ __device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const Bv
         if (__ballot(slot != kNoRay) == 0) break;
         if (slot == kNoRay) continue;
 
-        // One item: a 64-byte gather from the node or the triangle array.
+        // One item: a triangle of a leaf (the first 48 of 64 bytes read) or a four-wide node (128 bytes).
         const bool leaf = (item & kLeafItem) != 0u;
         const float4* p = leaf ? reinterpret_cast<const float4*>(tris + (item & kLeafFirstMask))
                                : reinterpret_cast<const float4*>(nodes + item);
         const float4 x0 = p[0], x1 = p[1], x2 = p[2], x3 = p[3];
+        float4 x4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x5 = x4, x6 = x4;
+        if (!leaf) {
+            x4 = p[4];
+            x5 = p[5];
+            x6 = p[6];
+        }
         if (fresh) {
             ray = job.decode(ra, rb);
             inv = mk(1.0f / ray.d.x, 1.0f / ray.d.y, 1.0f / ray.d.z);
@@ -196,20 +229,35 @@ __device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const Bv
             if (left > 0u) next = kLeafItem | ((left - 1u) << 27) | ((item & kLeafFirstMask) + 1u);
         } else {
             if (kCount) ++nv;
-            bool h0, h1;
-            float t0, t1;
-            box_test2(x0, x1, x2, ray.o, inv, ray.tmin, kAny ? ray.tmax : ht, h0, h1, t0, t1);
-            const uint32_t c0 = child_item(__float_as_int(x3.x), __float_as_int(x3.z));
-            const uint32_t c1 = child_item(__float_as_int(x3.y), __float_as_int(x3.w));
-            if (h0 && h1) {
-                const bool swap = t1 < t0;
-                st.push(swap ? c0 : c1);
-                next = swap ? c1 : c0;
-            } else if (h0) {
-                next = c0;
-            } else if (h1) {
-                next = c1;
+            const float tmax = kAny ? ray.tmax : ht;
+            const uint32_t it[4] = {__float_as_uint(x6.x), __float_as_uint(x6.y), __float_as_uint(x6.z),
+                                    __float_as_uint(x6.w)};
+            const float lxs[4] = {x0.x, x0.y, x0.z, x0.w}, hxs[4] = {x1.x, x1.y, x1.z, x1.w};
+            const float lys[4] = {x2.x, x2.y, x2.z, x2.w}, hys[4] = {x3.x, x3.y, x3.z, x3.w};
+            const float lzs[4] = {x4.x, x4.y, x4.z, x4.w}, hzs[4] = {x5.x, x5.y, x5.z, x5.w};
+            float key[4];
+            uint32_t ci[4];
+            int cnt = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                bool h;
+                const float tn = box_near_far(lxs[k], hxs[k], lys[k], hys[k], lzs[k], hzs[k], ray.o, inv, ray.tmin,
+                                              tmax, h);
+                h = h && it[k] != kNoItem;
+                key[k] = h ? tn : 3.0e38f;
+                ci[k] = h ? it[k] : kNoItem;
+                cnt += h ? 1 : 0;
             }
+            // Nearest first: a sorting network on the entry distances, then the far ones pushed farthest first.
+            cas_item(key[0], ci[0], key[1], ci[1]);
+            cas_item(key[2], ci[2], key[3], ci[3]);
+            cas_item(key[0], ci[0], key[2], ci[2]);
+            cas_item(key[1], ci[1], key[3], ci[3]);
+            cas_item(key[1], ci[1], key[2], ci[2]);
+            if (cnt > 3) st.push(ci[3]);
+            if (cnt > 2) st.push(ci[2]);
+            if (cnt > 1) st.push(ci[1]);
+            if (cnt > 0) next = ci[0];
         }
         if (next == kNoItem && !occluded && st.sp > 0) next = st.pop();
         if (occluded || next == kNoItem) {

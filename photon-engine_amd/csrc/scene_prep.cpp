@@ -532,7 +532,7 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     rc = prepare_lights(dev, desc, sc);
     if (rc != PE_OK) return rc;
     SceneView& v = sc.view;
-    v.nodes = sc.nodes.as<BvhNode>();
+    v.nodes = sc.nodes.as<BvhNode4>();
     v.tris = sc.tris.as<TriAccel>();
     v.shade = sc.tri_shade.as<TriShade>();
     v.pos = sc.positions.as<float4>();

@@ -149,6 +149,8 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
 
     BvhBuildOptions opt;
     const BvhBuildResult bvh = build_bvh(lo.data(), hi.data(), nt, opt);
+    int depth4 = 0;
+    const std::vector<BvhNode4> nodes4 = collapse_bvh4(bvh.nodes, depth4);
 
     // One zero record of tail padding: traversal reads every triangle as a 64-byte item.
     std::vector<TriAccel> acc(nt + 1);
@@ -171,7 +173,7 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     }
 
     hipStream_t s = dev->stream;
-    if (!sc.nodes.upload(bvh.nodes.data(), bvh.nodes.size() * sizeof(BvhNode), s) ||
+    if (!sc.nodes.upload(nodes4.data(), nodes4.size() * sizeof(BvhNode4), s) ||
         !sc.tris.upload(acc.data(), acc.size() * sizeof(TriAccel), s) ||
         !sc.tri_shade.upload(shade.data(), shade.size() * sizeof(TriShade), s) ||
         !sc.positions.upload(pos.data(), pos.size() * sizeof(float4), s) ||
@@ -189,8 +191,8 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     sc.num_tris = nt;
     sc.num_verts = nv;
     sc.num_shapes = desc.num_meshes;
-    sc.num_nodes = bvh.nodes.size();
-    sc.bvh_depth = bvh.depth;
+    sc.num_nodes = nodes4.size();
+    sc.bvh_depth = depth4;
     return PE_OK;
 }
 
