@@ -111,7 +111,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode4*
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, false>(job, n, nodes, tris, st, nv, tv);
+    trace_queue<false, false>(job, QueueSegs{nullptr, n, n, 1u}, nodes, tris, st, nv, tv);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* __restrict__ nodes,
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* 
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<true, false>(job, n, nodes, tris, st, nv, tv);
+    trace_queue<true, false>(job, QueueSegs{nullptr, n, n, 1u}, nodes, tris, st, nv, tv);
 }
 
 // Persistent grid for 256-lane blocks and the spill buffer sized for it.

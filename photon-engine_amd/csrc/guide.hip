@@ -306,7 +306,7 @@ struct ArrayQueries {
         d = mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
         return true;
     }
-    __device__ __forceinline__ uint32_t count(uint32_t n) const { return n; }
+    __device__ __forceinline__ QueueSegs segs(uint32_t n) const { return QueueSegs{nullptr, n, n, 1u}; }
     __device__ __forceinline__ void store_part(uint32_t i, int part, float4 a, float4 b, float2 cdf, float2 coef) const {
         guide_store_part(mix + static_cast<size_t>(i) * kGuideMixFloat4, part, a, b, cdf, coef);
     }
@@ -408,12 +408,13 @@ int read_loss(pe_guide* g, hipStream_t s, float* loss) {
 
 }  // namespace
 
-int guide_reserve_paths(pe_guide* g, uint32_t paths) {
-    if (paths <= g->path_capacity) return PE_OK;
-    if (!g->mix.alloc(static_cast<size_t>(paths) * kGuideMixFloat4 * 16) ||
+int guide_reserve_paths(pe_guide* g, uint32_t paths, uint32_t slots) {
+    if (paths <= g->path_capacity && slots <= g->slot_capacity) return PE_OK;
+    if (!g->mix.alloc(static_cast<size_t>(slots) * kGuideMixFloat4 * 16) ||
         !g->path_records.alloc(static_cast<size_t>(paths) * 2 * 4 * 16))
         return PE_ERR_OOM;
     g->path_capacity = paths;
+    g->slot_capacity = slots;
     return PE_OK;
 }
 

@@ -54,9 +54,38 @@ struct PathBuffers {
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
     float4* sq_d;     //               direction xyz, w = pend index pid * S + s (uint bits)
-    uint32_t* ctr;    // [0] current queue size, [1] next queue size, [2] shadow queue size
+    uint32_t* ctr;    // segment counters (kCtr* below)
+    uint32_t seg_cap; // slots per segment of the ray queues (S * seg_cap per shadow queue segment)
     unsigned long long* stats;  // [0] extension rays, [1] shadow rays
 };
+
+// Queue counters: the current, next and shadow queues have kQueueSegs segments each, counter of segment k of
+// queue q at ctr[(q * kQueueSegs + k) * kSegStride]; ctr[kCtrTotal] = paths in the current queue (read by the host).
+constexpr uint32_t kCtrCur = 0, kCtrNext = 1, kCtrShadow = 2;
+constexpr uint32_t kCtrTotal = 3 * kQueueSegs * kSegStride;
+constexpr size_t kCtrBytes = (kCtrTotal + kSegStride) * sizeof(uint32_t);
+__host__ __device__ __forceinline__ uint32_t* seg_counter(uint32_t* ctr, uint32_t q, uint32_t k) {
+    return ctr + (q * kQueueSegs + k) * kSegStride;
+}
+__device__ __forceinline__ QueueSegs ray_queue(const PathBuffers& pb) {
+    return QueueSegs{seg_counter(pb.ctr, kCtrCur, 0), 0u, pb.seg_cap, kQueueSegs};
+}
+__device__ __forceinline__ QueueSegs shadow_queue(const PathBuffers& pb, int S) {
+    return QueueSegs{seg_counter(pb.ctr, kCtrShadow, 0), 0u, pb.seg_cap * static_cast<uint32_t>(S), kQueueSegs};
+}
+// Slot of camera path i in the first queue: 64-path chunks dealt to the segments in turn, so that every segment
+// holds paths from all over the image (segments keep their paths for the whole launch) while a chunk stays 64
+// neighbouring pixels.
+__host__ __device__ __forceinline__ uint32_t first_slot(uint32_t i, uint32_t seg_cap) {
+    const uint32_t q = i >> 6;
+    return (q % kQueueSegs) * seg_cap + (q / kQueueSegs) * 64u + (i & 63u);
+}
+// Slots per segment for n paths: the chunks dealt to one segment, rounded up to whole 256-slot shade rounds.
+__host__ __device__ __forceinline__ uint32_t seg_capacity(uint32_t n) {
+    const uint32_t chunks = (n + 63u) / 64u;
+    const uint32_t per = (chunks + kQueueSegs - 1u) / kQueueSegs;
+    return (per * 64u + 255u) / 256u * 256u;
+}
 
 // packed: dim (bits 0-15), depth (16-22), diffuse bounces (23-29), specular bounce (30), camera ray (31)
 __device__ __forceinline__ uint32_t get_dim(uint32_t w) { return w & 0xFFFFu; }
@@ -192,8 +221,9 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
         const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
         const V3 dir = normalized(target - lp.cam_pos);
-        pb.ro[0][i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(i));
-        pb.rd[0][i] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+        const uint32_t slot = first_slot(i, pb.seg_cap);
+        pb.ro[0][slot] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(i));
+        pb.rd[0][slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
         pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
         pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
@@ -204,10 +234,14 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
             for (int r = 0; r < kGuideRecords; ++r) pb.grec[static_cast<size_t>(i) * 8 + 4 * r + 3] = make_float4(0, 0, 0, 0);
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.ctr[0] = lp.num_paths;
-        pb.ctr[1] = 0;
-        pb.ctr[2] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) {  // the paths of segment k: its chunks, the last one partial
+        const uint32_t k = threadIdx.x, n = lp.num_paths, chunks = (n + 63u) / 64u;
+        uint32_t cnt = chunks > k ? ((chunks - 1u - k) / kQueueSegs + 1u) * 64u : 0u;
+        if (chunks > 0u && (chunks - 1u) % kQueueSegs == k && (n & 63u) != 0u) cnt -= 64u - (n & 63u);
+        *seg_counter(pb.ctr, kCtrCur, k) = cnt;
+        *seg_counter(pb.ctr, kCtrNext, k) = 0u;
+        *seg_counter(pb.ctr, kCtrShadow, k) = 0u;
+        if (k == 0) pb.ctr[kCtrTotal] = n;
     }
 }
 
@@ -327,7 +361,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuf
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, pb.ctr[0], sc.nodes, sc.tris, st, nv, tv);
+    trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, ray_queue(pb), sc.nodes, sc.tris, st, nv, tv);
     if (kCount) {
         wave_count(&pb.stats[2], nv);
         wave_count(&pb.stats[3], tv);
@@ -335,12 +369,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuf
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, uint32_t* spill) {
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, int S, uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, pb.ctr[2], sc.nodes, sc.tris, st, nv, tv);
+    trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, shadow_queue(pb, S), sc.nodes, sc.tris, st, nv, tv);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
         wave_count(&pb.stats[5], tv);
@@ -348,14 +382,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffer
 }
 
 template <bool kCount>
-__global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, PathBuffers pb, float clamp_limit,
+__global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, PathBuffers pb, int S, float clamp_limit,
                                                                  uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, kCount>(TransmitShadowJob{pb.sq_o, pb.sq_d, pb.pend, sc, clamp_limit}, pb.ctr[2], sc.nodes,
-                               sc.tris, st, nv, tv);
+    trace_queue<false, kCount>(TransmitShadowJob{pb.sq_o, pb.sq_d, pb.pend, sc, clamp_limit}, shadow_queue(pb, S),
+                               sc.nodes, sc.tris, st, nv, tv);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
         wave_count(&pb.stats[5], tv);
@@ -367,7 +401,8 @@ struct RenderQueries {
     const float4* ro;
     const float4* rd;
     const float4* hits;
-    const uint32_t* count_ptr;
+    const uint32_t* counts;   // segment counters of the current queue
+    uint32_t seg_cap;
     float4* mix;
     __device__ __forceinline__ bool fetch(uint32_t i, V3& p, V3& d) const {
         const float4 h = hits[i];
@@ -378,7 +413,7 @@ struct RenderQueries {
         d = -dir;
         return true;
     }
-    __device__ __forceinline__ uint32_t count(uint32_t n) const { return min(n, *count_ptr); }
+    __device__ __forceinline__ QueueSegs segs(uint32_t) const { return QueueSegs{counts, 0u, seg_cap, kQueueSegs}; }
     __device__ __forceinline__ void store_part(uint32_t i, int part, float4 a, float4 b, float2 cdf, float2 coef) const {
         guide_store_part(mix + static_cast<size_t>(i) * kGuideMixFloat4, part, a, b, cdf, coef);
     }
@@ -437,14 +472,22 @@ __global__ void __launch_bounds__(kBlock) k_guide_collect(LaunchParams lp, PathB
 template <bool kTransport, bool kGuide>
 __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur,
                                                   int guide_bounce) {
-    const uint32_t n = pb.ctr[0];
+    // Block b works on segment b % kQueueSegs of the current queue and appends to the same segment of the next
+    // and shadow queues (whose segments therefore never receive more than this one holds).
+    const SegShare sh = seg_share(blockIdx.x, gridDim.x, kQueueSegs);
+    const uint32_t n = *seg_counter(pb.ctr, kCtrCur, sh.seg);
+    const uint32_t seg_base = sh.seg * pb.seg_cap;
+    uint32_t* next_count = seg_counter(pb.ctr, kCtrNext, sh.seg);
+    uint32_t* shadow_count = seg_counter(pb.ctr, kCtrShadow, sh.seg);
     const float4* ro = pb.ro[cur];
     const float4* rd = pb.rd[cur];
-    float4* ro_next = pb.ro[cur ^ 1];
-    float4* rd_next = pb.rd[cur ^ 1];
+    float4* ro_next = pb.ro[cur ^ 1] + seg_base;
+    float4* rd_next = pb.rd[cur ^ 1] + seg_base;
     const EnergyTables& T = *sc.tables;
     const int S = lp.shadow_samples;
-    const uint32_t stride = gridDim.x * blockDim.x;
+    float4* sq_o = pb.sq_o + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
+    float4* sq_d = pb.sq_d + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
+    const uint32_t stride = sh.workers * blockDim.x;
     // Each block takes 256 queue slots per round and reorders them in LDS, hits first and misses after, so a
     // wave runs either the long surface path or the short environment path, not both with half its lanes.
     __shared__ uint32_t s_slot[kBlock];
@@ -452,9 +495,9 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
     // The loop bound is uniform over the block, so every lane takes part in the barriers and queue appends.
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
-        const uint32_t mine = base + threadIdx.x;
-        const bool valid = mine < n;
+    for (uint32_t base = sh.rank * blockDim.x; base < n; base += stride) {
+        const uint32_t mine = seg_base + base + threadIdx.x;
+        const bool valid = base + threadIdx.x < n;
         const bool hit_here = valid && __float_as_int(pb.hits[mine].y) >= 0;
         const uint64_t hm = __ballot(hit_here), mm = __ballot(valid && !hit_here);
         if (__lane_id() == 0) {
@@ -684,10 +727,10 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                                     make_float4(c.x, c.y, c.z, __uint_as_float(pend_bits));
                                 ok = true;
                             } while (false);
-                            const uint32_t ss_slot = wave_append(&pb.ctr[2], ok);
+                            const uint32_t ss_slot = wave_append(shadow_count, ok);
                             if (ok) {
-                                pb.sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
-                                pb.sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
+                                sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
+                                sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
                                                                __uint_as_float(pid * static_cast<uint32_t>(S) +
                                                                                static_cast<uint32_t>(s)));
                             }
@@ -792,7 +835,7 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                 if constexpr (kTransport || kGuide) pb.ext[pid] = ext;
             }
         }
-        const uint32_t slot = wave_append(&pb.ctr[1], extend);
+        const uint32_t slot = wave_append(next_count, extend);
         if (extend) {
             ro_next[slot] = make_float4(new_o.x, new_o.y, new_o.z, __uint_as_float(pid));
             rd_next[slot] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
@@ -801,13 +844,26 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
 }
 
 // After a bounce: the next queue becomes current, the shadow queue is emptied, ray counts accumulate.
+// One wave, one lane per segment.
 __global__ void k_rotate(PathBuffers pb) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        pb.stats[0] += pb.ctr[0];
-        pb.stats[1] += pb.ctr[2];
-        pb.ctr[0] = pb.ctr[1];
-        pb.ctr[1] = 0;
-        pb.ctr[2] = 0;
+    const uint32_t k = threadIdx.x;
+    if (blockIdx.x != 0 || k >= kQueueSegs) return;
+    uint32_t* cur = seg_counter(pb.ctr, kCtrCur, k);
+    uint32_t* next = seg_counter(pb.ctr, kCtrNext, k);
+    uint32_t* shadow = seg_counter(pb.ctr, kCtrShadow, k);
+    unsigned long long traced = *cur, shadows = *shadow, live = *next;
+    *cur = *next;
+    *next = 0u;
+    *shadow = 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        traced += __shfl_xor(traced, off);
+        shadows += __shfl_xor(shadows, off);
+        live += __shfl_xor(live, off);
+    }
+    if (k == 0) {
+        pb.stats[0] += traced;
+        pb.stats[1] += shadows;
+        pb.ctr[kCtrTotal] = static_cast<uint32_t>(live);
     }
 }
 
@@ -934,7 +990,8 @@ struct Profiler {
 // Per-device wavefront buffers, grown on demand and kept between renders.
 struct Wavefront {
     Profiler prof;
-    uint32_t capacity = 0;
+    uint32_t capacity = 0;   // paths
+    uint32_t seg_cap = 0;    // queue slots per segment (kQueueSegs segments)
     int shadow_samples = 0;
     DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, ctr, stats, filter;
     int filter_type = -1;
@@ -945,17 +1002,24 @@ struct Wavefront {
     // Device bytes of the wavefront buffers per path in flight.
     static uint64_t bytes_per_path(int S) { return 4 * 16 + 7 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
 
+    // Queue slots: kQueueSegs segments of seg_cap (a little above the path count).
+    uint32_t slots() const { return kQueueSegs * seg_cap; }
+
     int ensure(uint32_t n, int S) {
         if (n <= capacity && S == shadow_samples) return PE_OK;
+        const uint32_t cap = seg_capacity(n);
         const size_t N = n, NS = static_cast<size_t>(n) * S;
-        if (!ro0.alloc(N * 16) || !rd0.alloc(N * 16) || !ro1.alloc(N * 16) || !rd1.alloc(N * 16) ||
+        const size_t Q = static_cast<size_t>(kQueueSegs) * cap, QS = Q * S;
+        if (!ro0.alloc(Q * 16) || !rd0.alloc(Q * 16) || !ro1.alloc(Q * 16) || !rd1.alloc(Q * 16) ||
             !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) || !last_n.alloc(N * 16) ||
-            !ids.alloc(N * 16) || !ext.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(N * 16) || !sq_o.alloc(NS * 16) ||
-            !sq_d.alloc(NS * 16) || !ctr.alloc(64) || !stats.alloc(64)) {
+            !ids.alloc(N * 16) || !ext.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(Q * 16) || !sq_o.alloc(QS * 16) ||
+            !sq_d.alloc(QS * 16) || !ctr.alloc(kCtrBytes) || !stats.alloc(64)) {
             capacity = 0;
+            seg_cap = 0;
             return PE_ERR_OOM;
         }
         capacity = n;
+        seg_cap = cap;
         shadow_samples = S;
         return PE_OK;
     }
@@ -977,6 +1041,7 @@ struct Wavefront {
         pb.sq_o = sq_o.as<float4>();
         pb.sq_d = sq_d.as<float4>();
         pb.ctr = ctr.as<uint32_t>();
+        pb.seg_cap = seg_cap;
         pb.stats = stats.as<unsigned long long>();
         return pb;
     }
@@ -1083,7 +1148,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
         if (rc != PE_OK) return rc;
         if (guide) {
-            rc = guide_reserve_paths(guide, wf->capacity);
+            rc = guide_reserve_paths(guide, wf->capacity, wf->slots());
             if (rc != PE_OK) return rc;
         }
 
@@ -1123,6 +1188,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         pb.grec = guide ? guide->path_records.as<float4>() : nullptr;
         hipStream_t s = lane.stream;
         const int gblocks = lane.grid_blocks;
+        if (gblocks < static_cast<int>(kQueueSegs)) {  // every queue segment needs a block of its own
+            set_error("pe_render: the persistent grid is smaller than the queue's segment count");
+            return PE_ERR_UNSUPPORTED;
+        }
         hipLaunchKernelGGL(k_zero_stats, dim3(1), dim3(64), 0, s, pb);
         const bool count = settings->count_visits != 0;
         uint32_t* spill = lane.spill;
@@ -1136,10 +1205,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             wf->prof.end(s);
             if (guide_bounce) {
                 wf->prof.begin(kKGuide, s);
-                const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, pb.ctr, pb.mix};
+                const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, seg_counter(pb.ctr, kCtrCur, 0), pb.seg_cap, pb.mix};
                 GuideView gv = guide->view(lp.guide_fraction);
                 gv.inferred = pb.stats + 7;
-                hipLaunchKernelGGL(k_guide_infer<RenderQueries>, dim3(std::max(1, dev->cu_count * 2)),
+                hipLaunchKernelGGL(k_guide_infer<RenderQueries>, dim3(std::max(16, dev->cu_count * 2)),
                                    dim3(kInferBlock), 0, s, gv, q, lp.num_paths);
                 wf->prof.end(s);
             }
@@ -1155,18 +1224,18 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             wf->prof.begin(kKShadow, s);
             if (view.has_transmitters) {
                 if (count)
-                    hipLaunchKernelGGL(k_shadow_transmit<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb,
+                    hipLaunchKernelGGL(k_shadow_transmit<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S,
                                        lp.clamp_indirect, spill);
                 else
-                    hipLaunchKernelGGL(k_shadow_transmit<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb,
+                    hipLaunchKernelGGL(k_shadow_transmit<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S,
                                        lp.clamp_indirect, spill);
             } else if (count) {
-                hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+                hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S, spill);
             } else {
-                hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, spill);
+                hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S, spill);
             }
             wf->prof.end(s);
-            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(1), 0, s, pb);
+            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kQueueSegs), 0, s, pb);
         };
         uint64_t traced = 0;
         bool cancelled = false;
@@ -1200,7 +1269,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             for (;;) {
                 for (; it < planned; ++it) bounce(it & 1, it);
                 uint32_t left = 0;
-                PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr, 4, hipMemcpyDeviceToHost, s));
+                PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr + kCtrTotal, 4, hipMemcpyDeviceToHost, s));
                 PE_HIP_CHECK(hipStreamSynchronize(s));
                 if (left == 0) break;
                 if (it > limit) {

@@ -14,6 +14,7 @@
 
 #include "photon_hip.h"
 #include "pe_math.h"
+#include "pe_traverse.h"
 
 #include <hip/hip_runtime.h>
 
@@ -148,12 +149,27 @@ __device__ __forceinline__ V3 guide_sample(const float4* mix, float us, float u0
 }
 
 // ---------------------------------------------------------------------------------------- fused inference
-constexpr int kInferBlock = 256;       // 4 waves, each on tiles of 32 queries
-constexpr int kActStride = 72;         // bf16 per activation row in LDS (64 + 8: conflict-free 16-byte reads)
-constexpr int kOutStride = 65;         // fp32 per output row in LDS (16 rows at a time)
-constexpr int kWaveScratchBytes = 32 * kActStride * 2;  // the activation tile; the output rows share it
-static_assert(16 * kOutStride * 4 <= kWaveScratchBytes, "16 output rows fit the wave's scratch");
-// Weights in LDS: W[out][in] rows padded by 8 bf16, so that the 16 rows a B-operand read touches start on 16
+// The network runs transposed: D = W X^T with the weights as the A operand (neurons are MFMA rows) and the
+// queries as the B operand (MFMA columns), so a lane's accumulators after a layer are four consecutive neurons
+// of one query per 16-neuron tile, which is what the same lane supplies as a B operand of the next layer once
+// the next layer's input columns are permuted to match. Activations never leave the registers; LDS holds the
+// weights (and biases) only. A wave works on kInferTiles tiles of 16 queries, so each A operand read from LDS
+// feeds kInferTiles MFMAs.
+//
+// Physical orders in LDS. Input column k = 32 s + 8 g + e is element e of what lane group g supplies at MFMA
+// k-step s. Layer 0: each group encodes fixed coordinates, so a lane builds its features without selecting
+// among the six: groups 0..2 take position coordinate g at frequencies 4s .. 4s + 3 (s < 3) and direction
+// coordinate g at frequencies 4(s - 3) .. 4(s - 3) + 3 (s >= 3); group 3 takes direction coordinate s at
+// frequencies 8..11 (s < 3), the six raw coordinates and two zeros (s = 3) and zeros (s = 4). Layer l >= 1:
+// logical input neuron 16 (2 s + e / 4) + 4 g + e % 4, since the lane of group g holds, for the query of its
+// column, neurons 16 t + 4 g + r (tile t, row r), and its k-step s uses tiles 2 s and 2 s + 1. The output layer's
+// rows are permuted so that lane group g holds lobes 2 g and 2 g + 1: physical row 16 t + 4 g + r with
+// e = 4 t + r is logical output 10 g + e for e < 10 (lobe 2 g + e / 5, field e % 5) and 40 + 6 g + e - 10 (unused)
+// otherwise. Every other order is the identity.
+constexpr int kInferBlock = 256;       // 4 waves
+constexpr int kInferTiles = 4;         // query tiles of 16 per wave iteration
+constexpr int kInferQueries = 16 * kInferTiles;
+// Weights in LDS: physical rows padded by 8 bf16, so that the 16 rows an A-operand read touches start on 16
 // different bank groups (a dense row of 64 or 160 bf16 would put 8 or 4 of them on the same banks).
 constexpr int kWeightPad = 8;
 __host__ __device__ constexpr int guide_lds_w_offset(int l) {
@@ -162,175 +178,206 @@ __host__ __device__ constexpr int guide_lds_w_offset(int l) {
     return off;
 }
 constexpr int kGuideLdsWeights = guide_lds_w_offset(kGuideLayers);
-constexpr int kInferLds = kGuideLdsWeights * 2 + (kInferBlock / 64) * kWaveScratchBytes;
+constexpr int kInferLds = kGuideLdsWeights * 2 + kGuideLayers * kGuideHid * 4;  // bf16 weights, fp32 biases
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Logical input neuron of physical input column k of layer l (see above).
+__host__ __device__ constexpr int guide_phys_in(int l, int k) {
+    const int s = k >> 5, g = (k >> 3) & 3, e = k & 7;
+    if (l != 0) return 16 * (2 * s + (e >> 2)) + 4 * g + (e & 3);
+    if (g < 3) return s < 3 ? 24 * g + 8 * s + e : 24 * (3 + g) + 8 * (s - 3) + e;
+    return s < 3 ? 24 * (3 + s) + 16 + e : 144 + 8 * (s - 3) + e;
 }
 
-// One hidden or output layer for a wave's two 16-row tiles. A operands (activations) come from `act` (row
-// stride kActStride bf16), B operands (W[out][in], in-major) from the block's LDS copy.
-template <int K, int N>
-__device__ __forceinline__ void guide_layer_mfma(const bf16x8 (&a)[2][K / 32], const __bf16* w, const float (&bias)[N / 16],
-                                                 f32x4 (&acc)[2][N / 16]) {
-    const int l = __lane_id(), g = l >> 4, c = l & 15;
+// The eight layer-0 features lane group g supplies at k-step S (physical order above). cp and cd are position
+// and direction coordinate min(g, 2); v holds all six. The same arithmetic as guide_features8 (training).
+template <int S>
+__device__ __forceinline__ void guide_features_phys(const float v[6], float cp, float cd, int g, float out[8]) {
+    const float base = S < 3 ? (g < 3 ? cp : v[3 + (S < 3 ? S : 0)]) : cd;
+    const float scale = S < 3 ? (g < 3 ? static_cast<float>(1 << (4 * S)) * 0.5f : 128.0f)
+                              : static_cast<float>(1 << (4 * (S - 3))) * 0.5f;
+    const float r = base * scale;  // exact: a power of two
+    const float fr = r - floorf(r);
+    float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
 #pragma unroll
-    for (int t = 0; t < N / 16; ++t) {
-        // The accumulators start at the bias of their column (every D element of a lane is in column 16t + c).
-        acc[0][t] = f32x4{bias[t], bias[t], bias[t], bias[t]};
-        acc[1][t] = acc[0][t];
+    for (int j = 0; j < 4; ++j) {
+        out[2 * j] = sn;
+        out[2 * j + 1] = cs;
+        const float s2 = 2.0f * sn * cs, c2 = 1.0f - 2.0f * sn * sn;
+        sn = s2;
+        cs = c2;
+    }
+    if constexpr (S >= 3) {
 #pragma unroll
-        for (int s = 0; s < K / 32; ++s) {
-            const bf16x8 b = *reinterpret_cast<const bf16x8*>(w + (16 * t + c) * (K + kWeightPad) + 32 * s + 8 * g);
-            acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][s], b, acc[0][t], 0, 0, 0);
-            acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][s], b, acc[1][t], 0, 0, 0);
-        }
+        for (int e = 0; e < 8; ++e) out[e] = g < 3 ? out[e] : (S == 3 && e < 6 ? v[e] : 0.0f);
+    }
+}
+// Logical output of physical row p of layer l (see above).
+__host__ __device__ constexpr int guide_phys_out(int l, int p) {
+    if (l != kGuideLayers - 1) return p;
+    const int t = p >> 4, g = (p >> 2) & 3, r = p & 3, e = 4 * t + r;
+    return e < 10 ? 10 * g + e : 40 + 6 * g + (e - 10);
+}
+
+static_assert(kGuideOut == kGuideHid, "the output layer has the hidden layers' tile shape");
+static_assert(kGuideIn % 32 == 0 && kGuideHid == 64, "k-steps of 32, four 16-neuron tiles per layer");
+
+// Layer-0 k-step S for the wave's query tiles: features into B operands, one A operand read per neuron tile.
+template <int S>
+__device__ __forceinline__ void guide_layer0_step(const float (&v)[kInferTiles][6], const float (&cp)[kInferTiles],
+                                                  const float (&cd)[kInferTiles], int g, const __bf16* wl, int c,
+                                                  f32x4 (&acc)[kInferTiles][kGuideHid / 16]) {
+    bf16x8 xb[kInferTiles];
+#pragma unroll
+    for (int qt = 0; qt < kInferTiles; ++qt) {
+        float f8[8];
+        guide_features_phys<S>(v[qt], cp[qt], cd[qt], g, f8);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xb[qt][e] = static_cast<__bf16>(f8[e]);
+    }
+#pragma unroll
+    for (int t = 0; t < kGuideHid / 16; ++t) {
+        const bf16x8 w = *reinterpret_cast<const bf16x8*>(wl + (16 * t + c) * (kGuideIn + kWeightPad) + 32 * S + 8 * g);
+#pragma unroll
+        for (int qt = 0; qt < kInferTiles; ++qt)
+            acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[qt], acc[qt][t], 0, 0, 0);
     }
 }
 
+// Q supplies queries: QueueSegs segs(n) (the segments of slots to infer; a wave works on segment w % segs, so the
+// grid needs at least segs waves), bool fetch(i, V3& pos, V3& dir) (false: no inference needed) and
+// store_part(i, part, lobe, lobe, cdf, coef) (see guide_store_part).
 // Mixture parameters from the raw outputs o[5j .. 5j + 4]: mean direction (normalised; +z when degenerate),
 // concentration exp(clamp(o3, -4, 7)), softmax weights over the eight lobes.
-// Q supplies queries: uint32_t count(n) (how many of the n slots are live), bool fetch(i, V3& pos, V3& dir)
-// (false: no inference needed) and store_part(i, part, lobe, lobe, cdf, coef) (see guide_store_part).
 template <class Q>
-__global__ void __launch_bounds__(kInferBlock) k_guide_infer(GuideView gv, Q q, uint32_t n) {
+__global__ void __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_eu(2, 2))) k_guide_infer(GuideView gv, Q q, uint32_t n) {
     __shared__ __align__(16) unsigned char lds_raw[kInferLds];
     __bf16* wl = reinterpret_cast<__bf16*>(lds_raw);
+    float* bl = reinterpret_cast<float*>(lds_raw + kGuideLdsWeights * 2);
+    // Weights into their physical order, in runs of four bf16 (8 bytes): a run of four physical columns is four
+    // consecutive logical ones in every layer.
 #pragma unroll
-    for (int layer = 0; layer < kGuideLayers; ++layer) {  // 16-byte chunks, into padded rows
-        const int in = guide_layer_in(layer), chunks = guide_layer_out(layer) * in / 8;
-        const bf16x8* src = reinterpret_cast<const bf16x8*>(gv.wbf + guide_wbf_offset(layer));
-        for (int i = threadIdx.x; i < chunks; i += blockDim.x) {
-            const int row = i / (in / 8), col = i % (in / 8);
-            *reinterpret_cast<bf16x8*>(wl + guide_lds_w_offset(layer) + row * (in + kWeightPad) + 8 * col) = src[i];
+    for (int layer = 0; layer < kGuideLayers; ++layer) {
+        const int in = guide_layer_in(layer), runs = guide_layer_out(layer) * in / 4;
+        const __bf16* src = gv.wbf + guide_wbf_offset(layer);
+        for (int i = threadIdx.x; i < runs; i += blockDim.x) {
+            const int row = i / (in / 4), k = 4 * (i % (in / 4));
+            const uint2 v = *reinterpret_cast<const uint2*>(src + guide_phys_out(layer, row) * in + guide_phys_in(layer, k));
+            *reinterpret_cast<uint2*>(wl + guide_lds_w_offset(layer) + row * (in + kWeightPad) + k) = v;
         }
+        for (int i = threadIdx.x; i < kGuideHid; i += blockDim.x)
+            bl[layer * kGuideHid + i] = gv.params[guide_b_offset(layer) + guide_phys_out(layer, i)];
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, l = __lane_id(), g = l >> 4, c = l & 15;
-    unsigned char* scratch = lds_raw + kGuideLdsWeights * 2 + wave * kWaveScratchBytes;
-    __bf16* act = reinterpret_cast<__bf16*>(scratch);
-    float* out = reinterpret_cast<float*>(scratch);
-    // This lane's bias of each layer and output tile (column 16 t + c).
-    float bias[kGuideLayers][kGuideHid / 16];
+    const QueueSegs qs = q.segs(n);
+    const SegShare sh = seg_share(blockIdx.x * (blockDim.x >> 6) + wave, gridDim.x * (blockDim.x >> 6), qs.segs);
+    const uint32_t nk = qs.count(sh.seg), base = sh.seg * qs.seg_cap;
+    const uint32_t tiles = (nk + kInferQueries - 1u) / kInferQueries;
+    uint32_t inferred = 0;  // live queries of this wave's tiles, added to the counter once at the end
+    for (uint32_t tile = sh.rank; tile < tiles; tile += sh.workers) {
+        float v[kInferTiles][6];
+        bool ok[kInferTiles];
+        bool any_ok = false;
 #pragma unroll
-    for (int layer = 0; layer < kGuideLayers; ++layer)
-#pragma unroll
-        for (int t = 0; t < kGuideHid / 16; ++t) bias[layer][t] = gv.params[guide_b_offset(layer) + 16 * t + c];
-    n = q.count(n);
-    const uint32_t waves = gridDim.x * (blockDim.x >> 6);
-    const uint32_t tiles = (n + 31u) >> 5;
-    for (uint32_t tile = blockIdx.x * (blockDim.x >> 6) + wave; tile < tiles; tile += waves) {
-        // Layer 0: this lane's 8 features of each k-step for rows c and 16 + c, built one k-step at a time
-        // straight into A operands.
-        float v[2][6];
-        bool ok[2];
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
-            const uint32_t i = tile * 32u + static_cast<uint32_t>(16 * rt + c);
+        for (int qt = 0; qt < kInferTiles; ++qt) {
+            const uint32_t j = tile * kInferQueries + static_cast<uint32_t>(16 * qt + c);
             V3 p = mk(0.0f, 0.0f, 0.0f), d = mk(0.0f, 0.0f, 1.0f);
-            ok[rt] = i < n && q.fetch(i, p, d);
-            for (int k = 0; k < 6; ++k) v[rt][k] = guide_coord(gv, p, d, k);
+            ok[qt] = j < nk && q.fetch(base + j, p, d);
+            for (int k = 0; k < 6; ++k) v[qt][k] = guide_coord(gv, p, d, k);
+            any_ok = any_ok || ok[qt];
         }
-        const uint64_t live0 = __ballot(ok[0]), live1 = __ballot(ok[1]);
-        if ((live0 | live1) == 0) continue;
-        if (gv.inferred && l == 0)  // each query row is held by four lanes
-            atomicAdd(gv.inferred, static_cast<unsigned long long>((__popcll(live0) + __popcll(live1)) >> 2));
-        f32x4 acc[2][kGuideHid / 16];
+        if (__ballot(any_ok) == 0) continue;
 #pragma unroll
-        for (int t = 0; t < kGuideHid / 16; ++t) acc[0][t] = acc[1][t] = f32x4{bias[0][t], bias[0][t], bias[0][t], bias[0][t]};
-        for (int s = 0; s < kGuideIn / 32; ++s) {
-            bf16x8 a0[2];
+        for (int qt = 0; qt < kInferTiles; ++qt) inferred += __popcll(__ballot(ok[qt]) & 0xFFFFull);  // group 0's lanes
+        f32x4 acc[kInferTiles][kGuideHid / 16];
+        // Layer 0: the features of k-step s for this lane's queries, built straight into B operands.
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt) {
-                float f8[8];
-                guide_features8(v[rt], 32 * s + 8 * g, f8);
+        for (int t = 0; t < kGuideHid / 16; ++t) {
+            const float4 b = *reinterpret_cast<const float4*>(bl + 16 * t + 4 * g);
 #pragma unroll
-                for (int e = 0; e < 8; ++e) a0[rt][e] = static_cast<__bf16>(ok[rt] ? f8[e] : 0.0f);
-            }
-#pragma unroll
-            for (int t = 0; t < kGuideHid / 16; ++t) {
-                const bf16x8 bw =
-                    *reinterpret_cast<const bf16x8*>(wl + (16 * t + c) * (kGuideIn + kWeightPad) + 32 * s + 8 * g);
-                acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[0], bw, acc[0][t], 0, 0, 0);
-                acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0[1], bw, acc[1][t], 0, 0, 0);
-            }
+            for (int qt = 0; qt < kInferTiles; ++qt) acc[qt][t] = f32x4{b.x, b.y, b.z, b.w};
         }
-        const __bf16* w_layer = wl;
+        float cp[kInferTiles], cd[kInferTiles];
+#pragma unroll
+        for (int qt = 0; qt < kInferTiles; ++qt) {
+            cp[qt] = g == 0 ? v[qt][0] : g == 1 ? v[qt][1] : v[qt][2];
+            cd[qt] = g == 0 ? v[qt][3] : g == 1 ? v[qt][4] : v[qt][5];
+        }
+        guide_layer0_step<0>(v, cp, cd, g, wl, c, acc);
+        guide_layer0_step<1>(v, cp, cd, g, wl, c, acc);
+        guide_layer0_step<2>(v, cp, cd, g, wl, c, acc);
+        guide_layer0_step<3>(v, cp, cd, g, wl, c, acc);
+        guide_layer0_step<4>(v, cp, cd, g, wl, c, acc);
 #pragma unroll
         for (int layer = 1; layer < kGuideLayers; ++layer) {
-            // Epilogue of the previous layer: bias, ReLU, bf16, into the wave's activation tile.
+            // ReLU and bf16 of the previous layer, in the order the permuted input columns expect. ReLU as an
+            // integer max of the bit pattern (a negative float is a negative integer): one instruction, where
+            // fmaxf adds a NaN-quieting one.
+            bf16x8 xb[kInferTiles][kGuideHid / 32];
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
+            for (int s = 0; s < kGuideHid / 32; ++s)
 #pragma unroll
-                for (int t = 0; t < kGuideHid / 16; ++t)
+                for (int qt = 0; qt < kInferTiles; ++qt)
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const float h = fmaxf(0.0f, acc[rt][t][r]);
-                        act[(16 * rt + 4 * g + r) * kActStride + 16 * t + c] = static_cast<__bf16>(h);
-                    }
-            wave_lds_sync();
-            bf16x8 a[2][kGuideHid / 32];
+                    for (int e = 0; e < 8; ++e) xb[qt][s][e] = static_cast<__bf16>(
+                        __int_as_float(max(__float_as_int(acc[qt][2 * s + (e >> 2)][e & 3]), 0)));
+            const __bf16* w_layer = wl + guide_lds_w_offset(layer);
+            const float* b_layer = bl + layer * kGuideHid;
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
+            for (int t = 0; t < kGuideHid / 16; ++t) {
+                const float4 b = *reinterpret_cast<const float4*>(b_layer + 16 * t + 4 * g);
 #pragma unroll
-                for (int s = 0; s < kGuideHid / 32; ++s)
-                    a[rt][s] = *reinterpret_cast<const bf16x8*>(act + (16 * rt + c) * kActStride + 32 * s + 8 * g);
-            wave_lds_sync();
-            w_layer = wl + guide_lds_w_offset(layer);
-            guide_layer_mfma<kGuideHid, kGuideHid>(a, w_layer, bias[layer], acc);  // kGuideOut == kGuideHid
+                for (int qt = 0; qt < kInferTiles; ++qt) acc[qt][t] = f32x4{b.x, b.y, b.z, b.w};
+#pragma unroll
+                for (int s = 0; s < kGuideHid / 32; ++s) {
+                    const bf16x8 w = *reinterpret_cast<const bf16x8*>(w_layer + (16 * t + c) * (kGuideHid + kWeightPad) + 32 * s + 8 * g);
+#pragma unroll
+                    for (int qt = 0; qt < kInferTiles; ++qt)
+                        acc[qt][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(w, xb[qt][s], acc[qt][t], 0, 0, 0);
+                }
+            }
         }
-        // Output layer epilogue: bias, fp32 rows in LDS (one 16-row tile at a time); then four lanes per query
-        // build its mixture, two lobes each (softmax and CDF across the four lanes by shuffles).
+        // Output epilogue: the four lanes of groups 0..3 in column c hold one query's lobes 2g and 2g + 1 (raw
+        // outputs e = 0..9 at tile e / 4, row e % 4); softmax and CDF across the four lanes by shuffles.
+        // Hardware reciprocal, reciprocal square root and exp (about 1 ulp): the stored mixture is the
+        // distribution the renderer samples and evaluates, whatever its last bits.
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
+        for (int qt = 0; qt < kInferTiles; ++qt) {
+            float o[10];
 #pragma unroll
-            for (int t = 0; t < kGuideOut / 16; ++t)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) out[(4 * g + r) * kOutStride + 16 * t + c] = acc[rt][t][r];
-            wave_lds_sync();
-            const int row = l >> 2, part = l & 3;
-            const uint32_t i = tile * 32u + static_cast<uint32_t>(16 * rt + row);
-            V3 p, d;
-            const bool mine = i < n && q.fetch(i, p, d);
-            const float* o = out + row * kOutStride;
+            for (int e = 0; e < 10; ++e) o[e] = acc[qt][e >> 2][e & 3];
             float4 lobe[2];
             float logit[2], coefk[2];
-            // Hardware reciprocal, reciprocal square root and exp (about 1 ulp): the stored mixture is the
-            // distribution the renderer samples and evaluates, whatever its last bits.
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
-                const int j = 2 * part + h;
-                const V3 u = mk(o[5 * j], o[5 * j + 1], o[5 * j + 2]);
+                const V3 u = mk(o[5 * h], o[5 * h + 1], o[5 * h + 2]);
                 const float len2 = dot(u, u);
                 const V3 mu = len2 > 1e-12f ? u * __builtin_amdgcn_rsqf(len2) : mk(0.0f, 0.0f, 1.0f);
-                const float kappa = __expf(clampf(o[5 * j + 3], kGuideMinLogK, kGuideMaxLogK));
+                const float kappa = __expf(clampf(o[5 * h + 3], kGuideMinLogK, kGuideMaxLogK));
                 lobe[h] = make_float4(mu.x, mu.y, mu.z, kappa);
-                logit[h] = o[5 * j + 4];
+                logit[h] = o[5 * h + 4];
                 coefk[h] = kappa * __builtin_amdgcn_rcpf(kTwoPi * expm1_neg2k(kappa));
             }
             float mx = fmaxf(logit[0], logit[1]);
-            mx = fmaxf(mx, __shfl_xor(mx, 1));
-            mx = fmaxf(mx, __shfl_xor(mx, 2));
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
             const float e0 = __expf(logit[0] - mx), e1 = __expf(logit[1] - mx);
             float sum = e0 + e1;
-            sum += __shfl_xor(sum, 1);
-            sum += __shfl_xor(sum, 2);
+            sum += __shfl_xor(sum, 16);
+            sum += __shfl_xor(sum, 32);
             const float inv = __builtin_amdgcn_rcpf(sum);
             const float w0 = e0 * inv, w1 = e1 * inv;
-            // Exclusive prefix of the parts' weight sums (parts 0..3 of this query).
+            // Exclusive prefix of the groups' weight sums (groups 0..3 of this query's column).
             const float pw = w0 + w1;
-            const float n1 = __shfl_up(pw, 1), n2 = __shfl_up(pw, 2), n3 = __shfl_up(pw, 3);
-            const float before = (part >= 1 ? n1 : 0.0f) + (part >= 2 ? n2 : 0.0f) + (part >= 3 ? n3 : 0.0f);
+            const float n1 = __shfl_up(pw, 16), n2 = __shfl_up(pw, 32), n3 = __shfl_up(pw, 48);
+            const float before = (g >= 1 ? n1 : 0.0f) + (g >= 2 ? n2 : 0.0f) + (g >= 3 ? n3 : 0.0f);
             const float c0 = before + w0;
-            const float c1 = part == 3 ? 1.0f : before + pw;
-            if (mine) q.store_part(i, part, lobe[0], lobe[1], make_float2(c0, c1), make_float2(w0 * coefk[0], w1 * coefk[1]));
-            wave_lds_sync();
+            const float c1 = g == 3 ? 1.0f : before + pw;
+            const uint32_t i = base + tile * kInferQueries + static_cast<uint32_t>(16 * qt + c);
+            if (ok[qt]) q.store_part(i, g, lobe[0], lobe[1], make_float2(c0, c1), make_float2(w0 * coefk[0], w1 * coefk[1]));
         }
     }
+    if (gv.inferred && l == 0 && inferred) atomicAdd(gv.inferred, static_cast<unsigned long long>(inferred));
 }
-
-static_assert(kGuideOut == kGuideHid, "the output layer reuses the hidden-layer tile shape");
 
 }  // namespace pe

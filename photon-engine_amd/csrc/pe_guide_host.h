@@ -20,7 +20,7 @@ struct pe_guide {
     uint32_t record_slots = 0;
     pe::DevBuf records;
     // Per-render buffers: the mixture of every queue slot and two open training records per path.
-    uint32_t path_capacity = 0;
+    uint32_t path_capacity = 0, slot_capacity = 0;
     pe::DevBuf mix, path_records;
     uint64_t steps = 0;
     uint64_t launches = 0;           // render launches seen
@@ -43,8 +43,8 @@ struct pe_guide {
 };
 
 namespace pe {
-// Sizes the per-render buffers for `paths` paths in flight.
-int guide_reserve_paths(pe_guide* g, uint32_t paths);
+// Sizes the per-render buffers for `paths` paths in flight and `slots` ray-queue slots.
+int guide_reserve_paths(pe_guide* g, uint32_t paths, uint32_t slots);
 // Clears the record buffer before a launch collects into it.
 int guide_clear_records(pe_guide* g, hipStream_t s);
 // cfg.steps_per_launch Adam steps on batches of the collected records, on stream s.

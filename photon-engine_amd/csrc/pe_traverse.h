@@ -19,9 +19,9 @@
 // ordered by entry distance (a four-element sorting network) and all but the nearest are pushed, farthest
 // first. A lane whose ray has ended takes the next ray of its wave's share at the top of the next step (a
 // ballot ranks the idle lanes), and the new ray's record is loaded together with the root node, so lanes stay
-// busy until the wave's share is used up. The share of wave w is every W-th 64-ray chunk of the queue from
-// chunk w (W = waves in the grid): cheap and expensive regions of the queue are spread over all waves without
-// an atomic counter.
+// busy until the wave's share is used up. A queue is one or more segments (QueueSegs); wave w works on segment
+// w % segs, and its share is every Wk-th 64-ray chunk of that segment from chunk w / segs (Wk = the waves of
+// the segment): cheap and expensive regions of a segment are spread over its waves without an atomic counter.
 //
 // Item encoding (stack entries are items too): an interior node is its index; a leaf run is
 // kLeafItem | (remaining triangles - 1) << 27 | first triangle, so a leaf holds at most 16 triangles and
@@ -47,6 +47,30 @@ constexpr uint32_t kMaxLeafItemTris = 16;
 constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 constexpr uint32_t kNoRay = 0xFFFFFFFFu;
 constexpr uint64_t kMaxTraceTriangles = uint64_t{1} << 27;
+
+// A ray queue of `segs` segments: segment k holds count(k) items at [k * seg_cap, k * seg_cap + count(k)). The
+// wavefront queues have kQueueSegs segments, each appended to through a counter of its own (kSegStride uint32
+// apart, so no two share a cache line): the appends of the whole grid would otherwise serialise on one address.
+// A plain array of n rays is one segment (counts null).
+constexpr uint32_t kQueueSegs = 64;
+constexpr uint32_t kSegStride = 64;
+struct QueueSegs {
+    const uint32_t* counts;  // counts[k * kSegStride]; null: one segment of n items
+    uint32_t n;
+    uint32_t seg_cap;
+    uint32_t segs;
+    __device__ __forceinline__ uint32_t count(uint32_t k) const { return counts ? counts[k * kSegStride] : n; }
+};
+
+// The segment a worker (wave or block) w of W works on, its rank among that segment's workers and their number.
+// Every segment has a worker when W >= segs.
+struct SegShare {
+    uint32_t seg, rank, workers;
+};
+__device__ __forceinline__ SegShare seg_share(uint32_t w, uint32_t W, uint32_t segs) {
+    const uint32_t k = w % segs;
+    return SegShare{k, w / segs, (W - 1u - k) / segs + 1u};
+}
 
 struct TraceStack {
     uint32_t* lds;      // this lane's column: entry e at lds[e * kTraceBlock]
@@ -153,12 +177,15 @@ __device__ __forceinline__ uint32_t child_item(int32_t ref, int32_t count) {
 //   any:     job.any(i, ray, occluded)
 // nv / tv accumulate the node and triangle items this lane handled (kCount only).
 template <bool kAny, bool kCount, class Job>
-__device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const BvhNode4* __restrict__ nodes,
+__device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, const BvhNode4* __restrict__ nodes,
                                             const TriAccel* __restrict__ tris, TraceStack& st, uint32_t& nv,
                                             uint32_t& tv) {
     const uint32_t wpb = blockDim.x >> 6;
-    const uint32_t W = gridDim.x * wpb;
-    const uint32_t w = __builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6));
+    const SegShare sh = seg_share(__builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6)),
+                                  gridDim.x * wpb, q.segs);
+    const uint32_t w = sh.rank, W = sh.workers;
+    const uint32_t n = __builtin_amdgcn_readfirstlane(q.count(sh.seg));
+    const uint32_t seg_base = sh.seg * q.seg_cap;
     const uint32_t nchunks = (n + 63u) >> 6;
     const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
     uint32_t taken = 0;  // rays of this wave's share handed out so far (wave-uniform)
@@ -179,8 +206,8 @@ __device__ __forceinline__ void trace_queue(const Job& job, uint32_t n, const Bv
                 const uint32_t c = w + (j >> 6) * W;
                 const uint32_t idx = (c << 6) | (j & 63u);
                 if (c < nchunks && idx < n) {
-                    slot = idx;
-                    job.fetch(idx, ra, rb);
+                    slot = seg_base + idx;
+                    job.fetch(slot, ra, rb);
                     fresh = true;
                     item = 0;
                     st.sp = 0;
