@@ -199,7 +199,14 @@ typedef struct pe_settings {
     int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together; 0 = sized
                                    by the library (about 2^26 paths, within a quarter of free memory) */
     int32_t count_visits;       /* 1: also count BVH node / triangle visits (measurement runs; slower) */
-    int32_t reserved[6];
+    /* Adaptive sampling (RenderSettings::adaptiveSampling / noiseThreshold / minAdaptiveSamples,
+       engine/render_settings.h:30-32; renderer.cpp:438-505): a batch skips every pixel whose 3x3 neighbourhood
+       is already below the threshold, and within a batch a pixel stops after sample s when s + 1 is a multiple
+       of 8, at least min_adaptive_samples and not the batch's end, and its own error is below the threshold. */
+    int32_t adaptive;           /* 0 = off */
+    float noise_threshold;      /* relative standard error target; clamped to >= 1e-4 (reference default 0.02) */
+    int32_t min_adaptive_samples; /* at least 2 (reference default 16) */
+    int32_t reserved[3];
 } pe_settings;
 
 /* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray and traversal counters. */

@@ -81,6 +81,8 @@ _REF_SIGS = {
     "render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                          C.c_float, C.c_int, C.c_int, C.c_int, _F, _U, _P(C.c_double)]),
     "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
+    "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
+                                  C.c_float, C.c_int, _F, _U, _F, _P(C.c_uint64)]),
 }
 
 _cache: dict = {}
@@ -244,6 +246,26 @@ class RefScene:
         return mean, cnt, secs.value
 
 
+def ref_render_adaptive(scene_h, lib, view, batches, threshold, min_spp):
+    """The reference's Renderer::renderSamples with adaptive sampling over `batches` [(first, count), ...] into
+    one film -> (mean (h, w, 3), counts (h, w), relative errors (h, w), samples traced)."""
+    w, h = view.width, view.height
+    mean = np.zeros((h, w, 3), np.float32)
+    cnt = np.zeros((h, w), np.uint32)
+    rel = np.zeros((h, w), np.float32)
+    traced = C.c_uint64()
+    firsts = np.array([b[0] for b in batches], np.int32)
+    counts = np.array([b[1] for b in batches], np.int32)
+    f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+    rc = lib.ref_render_adaptive(scene_h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
+                                 w, h, view.max_bounces, len(batches), capi.ptr(firsts, C.c_int32),
+                                 capi.ptr(counts, C.c_int32), threshold, min_spp, _fp(mean), capi.ptr(cnt, C.c_uint32),
+                                 _fp(rel), C.byref(traced))
+    if rc != 0:
+        raise RuntimeError("ref_render_adaptive failed")
+    return mean, cnt, rel, traced.value
+
+
 def bsdf_eval(lib_kind: str, material, normal, tangent, front, wo, wi):
     """lib_kind 'oracle' or 'ref': Material::evalPdf on n tuples -> (f (n,3), pdf (n,))."""
     n = normal.shape[0]
@@ -318,6 +340,9 @@ _HIPDEV_SIGS = {
                                         C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, _F, _U, _I, C.c_char_p,
                                         C.c_int]),
     "hipdev_enumerate": (C.c_int, [C.c_char_p, C.c_int]),
+    "hipdev_session_render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I, C.c_char_p,
+                                                 C.c_int]),
 }
 
 
@@ -357,6 +382,19 @@ class HipDeviceSession:
             view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, ao_strength,
             _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
+        return rc, mean, counts, batches.value, err.value.decode(errors="replace")
+
+    def render_adaptive(self, view, spp, max_batch, threshold, min_spp):
+        """RenderSession with adaptive sampling -> (rc, mean (H,W,3), counts (H,W), batches, error text)."""
+        f3 = C.c_float * 3
+        mean = np.zeros((view.height, view.width, 3), np.float32)
+        counts = np.zeros((view.height, view.width), np.uint32)
+        batches = C.c_int32()
+        err = C.create_string_buffer(512)
+        rc = self._lib.hipdev_session_render_adaptive(
+            self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
+            view.height, view.max_bounces, spp, max_batch, threshold, min_spp, _fp(mean), capi.ptr(counts, C.c_uint32),
+            C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
 
 

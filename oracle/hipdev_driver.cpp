@@ -89,6 +89,65 @@ HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* 
     return 0;
 }
 
+// RenderSession(Final, targetSpp = spp, maxBatch) with adaptive sampling on, on a HipDevice: writes the sample
+// counts, the resolved mean and the number of batches (the session ends early once a batch traces nothing).
+HIPDEV_API int hipdev_session_render_adaptive(void* h, const float* camPos, const float* camTarget,
+                                              const float* camUp, float fov, float aspect, int width, int height,
+                                              int maxBounces, int spp, int maxBatch, float threshold, int minSpp,
+                                              float* mean, uint32_t* counts, int* batches, char* err, int errlen) {
+    std::string e;
+    std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
+    if (!dev) {
+        copy_error(e, err, errlen);
+        return 2;
+    }
+    dev->setScene(ref_scene_shared(h));
+    if (!dev->hasScene()) {
+        copy_error(dev->lastError(), err, errlen);
+        return 1;
+    }
+    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
+                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
+                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = spp;
+    s.maxBounces = maxBounces;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    s.adaptiveSampling = true;
+    s.noiseThreshold = threshold;
+    s.minAdaptiveSamples = minSpp;
+    SessionConfig cfg;
+    cfg.mode = RenderMode::Final;
+    cfg.targetSpp = spp;
+    cfg.maxBatch = maxBatch;
+    RenderSession session(*dev);
+    session.begin(cam, s, cfg);
+    int n = 0;
+    while (!session.finished()) {
+        if (!session.advance(nullptr)) {
+            copy_error(dev->lastError().empty() ? "RenderSession::advance returned false" : dev->lastError(), err,
+                       errlen);
+            return 1;
+        }
+        ++n;
+    }
+    if (batches) *batches = n;
+    const Film& film = session.film();
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * static_cast<size_t>(width) + static_cast<size_t>(x);
+            const Color3f c = film.resolvedPixel(x, y);
+            mean[3 * k] = c.r;
+            mean[3 * k + 1] = c.g;
+            mean[3 * k + 2] = c.b;
+            counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+        }
+    return 0;
+}
+
 // The HIP devices as the adapter lists them: writes "id|name|usable|note" lines into out.
 HIPDEV_API int hipdev_enumerate(char* out, int outlen) {
     std::string s;

@@ -353,6 +353,48 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
     return r.completed ? 0 : 1;
 }
 
+// Renderer::renderSamples with adaptive sampling over consecutive batches [firsts[b], firsts[b] + counts[b])
+// into one Film, as a RenderSession drives it (renderer.cpp:438-505); writes the film's resolved mean, sample
+// counts and relative errors (Film::resolvedPixel / sampleCount / relativeError) and the samples traced.
+REF_API int ref_render_adaptive(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
+                                float aspect, int width, int height, int maxBounces, int nbatch, const int* firsts,
+                                const int* counts, float threshold, int minSpp, float* mean, uint32_t* countOut,
+                                float* relErr, uint64_t* traced) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.maxBounces = maxBounces;
+    s.numThreads = 1;
+    s.filter.type = static_cast<PixelFilterType>(0);  // box, width 1: the filter the oracle's tests use
+    s.filter.width = 1.0f;
+    s.adaptiveSampling = true;
+    s.noiseThreshold = threshold;
+    s.minAdaptiveSamples = minSpp;
+    Film film(width, height);
+    Renderer renderer;
+    uint64_t total = 0;
+    for (int b = 0; b < nbatch; ++b) {
+        s.samplesPerPixel = firsts[b] + counts[b];
+        const SampleBatchResult r = renderer.renderSamples(*rs->scene, cam, s, film, firsts[b], counts[b], nullptr);
+        if (!r.completed) return 1;
+        total += r.samplesTraced;
+    }
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * width + x;
+            const Color3f c = film.resolvedPixel(x, y);
+            mean[3 * k] = c.r;
+            mean[3 * k + 1] = c.g;
+            mean[3 * k + 2] = c.b;
+            countOut[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+            relErr[k] = film.relativeError(x, y);
+        }
+    *traced = total;
+    return 0;
+}
+
 }  // extern "C"
 
 // The reference Scene behind a handle of ref_scene_build (for oracle/hipdev_driver.cpp, linked beside it).

@@ -258,7 +258,7 @@ int pe_device_features_get(const pe_device* dev, pe_device_features* out) {
     }
     out->aovs = 0;
     out->light_passes = 0;
-    out->adaptive_sampling = 0;
+    out->adaptive_sampling = 1;  // pe_settings.adaptive (renderer.cpp:438-505)
     return PE_OK;
 }
 

@@ -149,7 +149,14 @@ struct LaunchParams {
     int width, height;
     int first_sample;
     int spl;              // samples per pixel in this launch
-    uint32_t num_paths;   // width * height * spl
+    uint32_t num_pixels;  // pixels rendered by this launch
+    const uint32_t* pixel_list;  // their film indices in increasing order; null: every pixel (num_pixels = w * h)
+    uint32_t num_paths;   // num_pixels * spl
+    // Adaptive sampling: after adding this launch's samples k_film clears active[pixel] of every pixel whose
+    // relative error is below noise_threshold (adapt_check: the launch ends where the reference checks).
+    uint32_t* active;
+    int adapt_check;
+    float noise_threshold;
     int max_depth, rr_depth, shadow_samples;
     float clamp_indirect;
     uint64_t seed;
@@ -205,11 +212,11 @@ __device__ __forceinline__ float power_heuristic(int nf, float f_pdf, int ng, fl
 // ---------------------------------------------------------------------------------------------- kernels
 
 __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffers pb) {
-    const uint32_t pixels = static_cast<uint32_t>(lp.width) * static_cast<uint32_t>(lp.height);
+    const uint32_t m = lp.num_pixels;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lp.num_paths; i += stride) {
-        const uint32_t pix = i % pixels;
-        const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + i / pixels;
+        const uint32_t pix = lp.pixel_list ? lp.pixel_list[i % m] : i % m;
+        const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + i / m;
         const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
         const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
         SobolState ss{sobol_pixel_hash(lp.seed, x, y), rev_bits(sample), 0u};
@@ -872,19 +879,38 @@ __global__ void k_zero_stats(PathBuffers pb) {
 }
 
 // Film::addSample for every sample of every pixel, in sample order (film.cpp:56-71).
+// Film::relativeError (film.cpp:100-109) without contraction: sqrt(var / n) / (mean + 0.01) of the luminance.
+__device__ __forceinline__ float film_relative_error(float s0, float s1, float s2, uint32_t n, float l2) {
+#pragma clang fp contract(off)
+    if (n < 2) return 1e30f;
+    const float fn = static_cast<float>(n);
+    const float mean = (0.212671f * s0 + 0.715160f * s1 + 0.072169f * s2) / fn;
+    const float mean_sq = l2 / fn;
+    const float var = fmaxf(0.0f, mean_sq - mean * mean) * fn / static_cast<float>(n - 1);
+    return sqrtf(var / fn) / (fmaxf(mean, 0.0f) + 0.01f);
+}
+
+// Luminance as Film::addSample accumulates it (spectrum.h:68-70), without contraction.
+__device__ __forceinline__ float film_luminance(V3 c) {
+#pragma clang fp contract(off)
+    return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z;
+}
+
 __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb, float* __restrict__ sum,
                                                  uint32_t* __restrict__ count, float* __restrict__ lum2,
                                                  unsigned long long* __restrict__ rejected) {
-    const uint32_t pixels = static_cast<uint32_t>(lp.width) * static_cast<uint32_t>(lp.height);
+#pragma clang fp contract(off)
+    const uint32_t pixels = lp.num_pixels;
     const uint32_t stride = gridDim.x * blockDim.x;
     const int S = lp.shadow_samples;
-    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += stride) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < pixels; q += stride) {
+        const uint32_t p = lp.pixel_list ? lp.pixel_list[q] : q;
         float sr = sum[3 * static_cast<size_t>(p)], sg = sum[3 * static_cast<size_t>(p) + 1], sb = sum[3 * static_cast<size_t>(p) + 2];
         uint32_t c = count[p];
         float l2 = lum2[p];
         uint32_t rej = 0;
         for (int k = 0; k < lp.spl; ++k) {
-            const uint32_t pid = p + static_cast<uint32_t>(k) * pixels;
+            const uint32_t pid = q + static_cast<uint32_t>(k) * pixels;
             const float4 L4 = pb.L[pid];
             V3 L = mk(L4.x, L4.y, L4.z);
             for (int s = 0; s < S; ++s) {
@@ -900,7 +926,7 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
             sr += L.x;
             sg += L.y;
             sb += L.z;
-            const float lum = luminance(L);
+            const float lum = film_luminance(L);
             l2 += lum * lum;
         }
         sum[3 * static_cast<size_t>(p)] = sr;
@@ -909,6 +935,104 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
         count[p] = c;
         lum2[p] = l2;
         if (rej) atomicAdd(rejected, static_cast<unsigned long long>(rej));
+        // renderer.cpp:500-502: the pixel stops for the rest of the batch once its own error is low enough.
+        if (lp.adapt_check && film_relative_error(sr, sg, sb, c, l2) < lp.noise_threshold) lp.active[p] = 0u;
+    }
+}
+
+// ------------------------------------------------------------------------------ adaptive sampling
+// The batch's skip mask (renderer.cpp:442-467): a pixel is clean when its error is below the threshold and
+// is skipped for the whole batch when it and its eight neighbours (clamped at the border) are clean.
+__global__ void __launch_bounds__(kBlock) k_adapt_clean(const float* __restrict__ sum, const uint32_t* __restrict__ count,
+                                                        const float* __restrict__ lum2, uint32_t pixels, float threshold,
+                                                        uint32_t* __restrict__ clean) {
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += gridDim.x * blockDim.x)
+        clean[p] = film_relative_error(sum[3 * static_cast<size_t>(p)], sum[3 * static_cast<size_t>(p) + 1],
+                                       sum[3 * static_cast<size_t>(p) + 2], count[p], lum2[p]) < threshold ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(kBlock) k_adapt_mask(const uint32_t* __restrict__ clean, int w, int h,
+                                                       uint32_t* __restrict__ active) {
+    const uint32_t pixels = static_cast<uint32_t>(w) * static_cast<uint32_t>(h);
+    for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < pixels; p += gridDim.x * blockDim.x) {
+        const int x = static_cast<int>(p % static_cast<uint32_t>(w)), y = static_cast<int>(p / static_cast<uint32_t>(w));
+        bool all = true;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int nx = min(max(x + dx, 0), w - 1), ny = min(max(y + dy, 0), h - 1);
+                all = all && clean[static_cast<size_t>(ny) * w + nx] != 0u;
+            }
+        active[p] = all ? 0u : 1u;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u32(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// Ordered compaction of the active pixels into a list: per-block counts (1024 pixels a block, four rounds of
+// 256), one block scanning the counts, then each block writing its pixels at its offset in pixel order.
+constexpr uint32_t kCompactSpan = 1024;
+
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t v, uint32_t* s_wave, uint32_t& total) {
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, waves = blockDim.x >> 6;
+    uint32_t incl = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off);
+        if (lane >= static_cast<uint32_t>(off)) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t before = 0;
+    total = 0;
+    for (uint32_t k = 0; k < waves; ++k) {
+        if (k < wave) before += s_wave[k];
+        total += s_wave[k];
+    }
+    __syncthreads();
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_count_active(const uint32_t* __restrict__ active, uint32_t pixels,
+                                                         uint32_t* __restrict__ counts) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    uint32_t mine = 0;
+    for (uint32_t r = 0; r < kCompactSpan / kBlock; ++r) {
+        const uint32_t p = blockIdx.x * kCompactSpan + r * kBlock + threadIdx.x;
+        mine += p < pixels && active[p] ? 1u : 0u;
+    }
+    uint32_t total;
+    (void)block_exclusive(mine, s_wave, total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+// One block: exclusive offsets of nb counts in place; counts[nb] = the total.
+__global__ void __launch_bounds__(kBlock) k_scan_counts(uint32_t* __restrict__ counts, uint32_t nb) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nb; base += kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? counts[i] : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive(v, s_wave, total);
+        if (i < nb) counts[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) counts[nb] = carry;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scatter_active(const uint32_t* __restrict__ active, uint32_t pixels,
+                                                           const uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ list) {
+    __shared__ uint32_t s_wave[kBlock / 64];
+    uint32_t base = offsets[blockIdx.x];
+    for (uint32_t r = 0; r < kCompactSpan / kBlock; ++r) {
+        const uint32_t p = blockIdx.x * kCompactSpan + r * kBlock + threadIdx.x;
+        const uint32_t a = p < pixels && active[p] ? 1u : 0u;
+        uint32_t total;
+        const uint32_t ex = block_exclusive(a, s_wave, total);
+        if (a) list[base + ex] = p;
+        base += total;
     }
 }
 
@@ -994,6 +1118,9 @@ struct Wavefront {
     uint32_t seg_cap = 0;    // queue slots per segment (kQueueSegs segments)
     int shadow_samples = 0;
     DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, ctr, stats, filter;
+    // Adaptive sampling: per-pixel active flags and clean flags, the active pixel list, compaction counts.
+    DevBuf adapt_active, adapt_clean, adapt_list, adapt_counts;
+    uint32_t adapt_pixels = 0;
     int filter_type = -1;
     float filter_width = 0.0f;
 
@@ -1001,6 +1128,17 @@ struct Wavefront {
 
     // Device bytes of the wavefront buffers per path in flight.
     static uint64_t bytes_per_path(int S) { return 4 * 16 + 7 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
+
+    int ensure_adaptive(uint32_t pixels, uint32_t blocks) {
+        if (pixels <= adapt_pixels) return PE_OK;
+        if (!adapt_active.alloc(static_cast<size_t>(pixels) * 4) || !adapt_clean.alloc(static_cast<size_t>(pixels) * 4) ||
+            !adapt_list.alloc(static_cast<size_t>(pixels) * 4) || !adapt_counts.alloc((static_cast<size_t>(blocks) + 1) * 4)) {
+            adapt_pixels = 0;
+            return PE_ERR_OOM;
+        }
+        adapt_pixels = pixels;
+        return PE_OK;
+    }
 
     // Queue slots: kQueueSegs segments of seg_cap (a little above the path count).
     uint32_t slots() const { return kQueueSegs * seg_cap; }
@@ -1144,7 +1282,13 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             return PE_ERR_INVALID;
         }
         const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(path_limit, uint64_t{pixels} * spl_req));
-        const int spl_cap = static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
+        // Adaptive sampling renders in groups that end where the reference checks a pixel's error (every 8th
+        // sample), so a launch holds at most 8 samples of each pixel still rendering.
+        const bool adaptive = settings->adaptive != 0;
+        const float threshold = std::max(1e-4f, settings->noise_threshold);
+        const int min_spp = std::max(2, settings->min_adaptive_samples);
+        const int spl_cap = adaptive ? std::min(8, static_cast<int>(std::max<uint64_t>(1, max_paths / pixels)))
+                                     : static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
         int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
         if (rc != PE_OK) return rc;
         if (guide) {
@@ -1239,15 +1383,55 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         };
         uint64_t traced = 0;
         bool cancelled = false;
+        const uint32_t cblocks = (pixels + kCompactSpan - 1) / kCompactSpan;
+        const int pix_blocks = static_cast<int>(std::min<uint32_t>((pixels + kBlock - 1) / kBlock, 65535u));
+        lp.num_pixels = pixels;
+        lp.pixel_list = nullptr;
+        lp.active = nullptr;
+        lp.adapt_check = 0;
+        lp.noise_threshold = threshold;
+        if (adaptive) {
+            rc = wf->ensure_adaptive(pixels, cblocks);
+            if (rc != PE_OK) return rc;
+            lp.active = wf->adapt_active.as<uint32_t>();
+            if (first_sample >= min_spp) {
+                hipLaunchKernelGGL(k_adapt_clean, dim3(pix_blocks), dim3(kBlock), 0, s, film->sum.as<float>(),
+                                   film->count.as<uint32_t>(), film->lum2.as<float>(), pixels, threshold,
+                                   wf->adapt_clean.as<uint32_t>());
+                hipLaunchKernelGGL(k_adapt_mask, dim3(pix_blocks), dim3(kBlock), 0, s, wf->adapt_clean.as<uint32_t>(),
+                                   film->width, film->height, lp.active);
+            } else {
+                hipLaunchKernelGGL(k_fill_u32, dim3(pix_blocks), dim3(kBlock), 0, s, lp.active, pixels, 1u);
+            }
+        }
         for (int done = 0; done < sample_count;) {
             if (cancel && *cancel) {
                 cancelled = true;
                 break;
             }
-            const int spl = std::min(spl_cap, sample_count - done);
+            int spl = std::min(spl_cap, sample_count - done);
+            if (adaptive) {
+                // This group: up to the next multiple of 8; its end is checked unless it ends the batch
+                // (renderer.cpp:500: s + 1 < chunkEnd, s + 1 >= minSpp, (s + 1) % 8 == 0).
+                const int s0 = first_sample + done, end = first_sample + sample_count;
+                const int launch_end = std::min({end, (s0 / 8 + 1) * 8, s0 + spl_cap});
+                spl = launch_end - s0;
+                lp.adapt_check = launch_end < end && launch_end >= min_spp && launch_end % 8 == 0 ? 1 : 0;
+                uint32_t* counts = wf->adapt_counts.as<uint32_t>();
+                hipLaunchKernelGGL(k_count_active, dim3(cblocks), dim3(kBlock), 0, s, lp.active, pixels, counts);
+                hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, s, counts, cblocks);
+                hipLaunchKernelGGL(k_scatter_active, dim3(cblocks), dim3(kBlock), 0, s, lp.active, pixels, counts,
+                                   wf->adapt_list.as<uint32_t>());
+                uint32_t m = 0;
+                PE_HIP_CHECK(hipMemcpyAsync(&m, counts + cblocks, 4, hipMemcpyDeviceToHost, s));
+                PE_HIP_CHECK(hipStreamSynchronize(s));
+                if (m == 0) break;  // every pixel has stopped: the batch is over
+                lp.num_pixels = m;
+                lp.pixel_list = wf->adapt_list.as<uint32_t>();
+            }
             lp.first_sample = first_sample + done;
             lp.spl = spl;
-            lp.num_paths = pixels * static_cast<uint32_t>(spl);
+            lp.num_paths = lp.num_pixels * static_cast<uint32_t>(spl);
             // Guiding: the first launch samples the BSDF only and collects; later ones mix in the trained guide.
             lp.guide_fraction = guide && guide->steps > 0 ? guide->cfg.guide_fraction : 0.0f;
             lp.guide_records = guide && guide->training() ? 1 : 0;
@@ -1283,7 +1467,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 hipLaunchKernelGGL(k_guide_collect, dim3(cb), dim3(kBlock), 0, s, lp, pb, guide->records.as<float4>(),
                                    guide->record_slots, static_cast<uint64_t>(guide->launches) * 0x9e3779b97f4a7c15ull);
             }
-            const int film_blocks = static_cast<int>(std::min<uint32_t>((pixels + kBlock - 1) / kBlock, 65535u));
+            const int film_blocks = static_cast<int>(std::min<uint32_t>((lp.num_pixels + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKFilm, s);
             hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, film->sum.as<float>(),
                                film->count.as<uint32_t>(), film->lum2.as<float>(),
@@ -1300,7 +1484,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 ++guide->launches;
             }
             done += spl;
-            traced += static_cast<uint64_t>(pixels) * spl;
+            traced += static_cast<uint64_t>(lp.num_pixels) * spl;
         }
         unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         PE_HIP_CHECK(hipMemcpyAsync(st, pb.stats, sizeof(st), hipMemcpyDeviceToHost, s));

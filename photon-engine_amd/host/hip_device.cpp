@@ -373,6 +373,10 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     st.clamp_indirect = settings.clampIndirect;
     st.seed = 12345;                                   // createRenderSampler default (renderer.h:37)
     st.samples_per_launch = 0;                         // the library sizes launches for the device
+    st.adaptive = settings.adaptiveSampling ? 1 : 0;   // renderer.cpp:438-505
+    st.noise_threshold = settings.noiseThreshold;
+    st.min_adaptive_samples = settings.minAdaptiveSamples;
+    const bool adaptive = settings.adaptiveSampling;
 
     const int w = film.width(), h = film.height();
     if (!m_film || m_filmW != w || m_filmH != h) {
@@ -384,13 +388,22 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         }
         m_filmW = w;
         m_filmH = h;
+        m_accumulating = false;
     }
-    if (pe_film_clear(m_film) != PE_OK) {
-        m_error = "HipDevice::render: " + lastHipError();
-        return res;
+    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
+    if (!adaptive || firstSample == 0 || !m_accumulating) {
+        if (pe_film_clear(m_film) != PE_OK) {
+            m_error = "HipDevice::render: " + lastHipError();
+            return res;
+        }
+        m_prevSum.assign(3 * pixels, 0.0f);
+        m_prevCount.assign(pixels, 0u);
+        m_accumulating = adaptive;
     }
-    // Cancellation is polled between chunks; one chunk when nobody can cancel.
-    const int chunk = cancel ? std::max(1, std::min(count, 8)) : std::max(1, count);
+    // Cancellation is polled between chunks; one chunk when nobody can cancel. An adaptive batch is one call:
+    // its skip mask and stops belong to the batch as a whole.
+    const int chunk = cancel && !adaptive ? std::max(1, std::min(count, 8)) : std::max(1, count);
+    uint64_t traced = 0;
     int done = 0;
     bool cancelled = false;
     while (done < count) {
@@ -405,9 +418,9 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
             return res;
         }
         m_lastRays += br.extension_rays + br.shadow_rays;
+        traced += br.samples_traced;
         done += n;
     }
-    const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
     std::vector<float> sum(3 * pixels), lum2(pixels);
     std::vector<uint32_t> cnt(pixels);
     uint64_t rejected = 0;
@@ -415,13 +428,21 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         m_error = "HipDevice::render: " + lastHipError();
         return res;
     }
+    // Merge what this call added (the whole film unless it accumulates across calls) with Film::addBatch.
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {
             const size_t p = static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x);
-            film.addBatch(x, y, Color3f(sum[3 * p], sum[3 * p + 1], sum[3 * p + 2]), cnt[p]);
+            film.addBatch(x, y,
+                          Color3f(sum[3 * p] - m_prevSum[3 * p], sum[3 * p + 1] - m_prevSum[3 * p + 1],
+                                  sum[3 * p + 2] - m_prevSum[3 * p + 2]),
+                          cnt[p] - m_prevCount[p]);
         }
+    if (m_accumulating) {
+        m_prevSum = std::move(sum);
+        m_prevCount = std::move(cnt);
+    }
     res.completed = !cancelled;
-    res.samplesTraced = static_cast<uint64_t>(pixels) * static_cast<uint64_t>(done);
+    res.samplesTraced = traced;
     return res;
 }
 

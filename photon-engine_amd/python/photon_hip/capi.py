@@ -94,7 +94,8 @@ class Settings(C.Structure):
     _fields_ = [("max_bounces", C.c_int32), ("rr_depth", C.c_int32), ("shadow_quality", C.c_int32),
                 ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
                 ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("count_visits", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("adaptive", C.c_int32), ("noise_threshold", C.c_float), ("min_adaptive_samples", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class BatchResult(C.Structure):
