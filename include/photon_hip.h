@@ -248,6 +248,25 @@ int pe_camera_perspective(const float position[3], const float target[3], const 
 int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
               int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result);
 
+/* RenderTargets (engine/render_device.h:58-62) with AovFilms (engine/renderer.h:42-53): the beauty film and,
+ * each optional, the AOV films of the same size. Per sample, as tracePixel records them (renderer.cpp:175-205,
+ * environment background): the first surface's albedo clamped to [0, 1] (DisneyMaterial baseColor, Lambertian
+ * albedo, Dielectric tint) with Film::addSample, its shading normal with Film::addSampleSigned and coverage 1
+ * in r, g, b; a camera ray that misses adds the clamped environment radiance along it, a zero normal and
+ * coverage 0. Light passes and cryptomatte (RenderPassFilms) are not implemented. */
+typedef struct pe_targets {
+    pe_film* beauty;   /* required */
+    pe_film* albedo;   /* AovFilms::albedo, or NULL */
+    pe_film* normal;   /* AovFilms::normal, or NULL */
+    pe_film* alpha;    /* AovFilms::alpha, or NULL */
+    void* reserved[4];
+} pe_targets;
+
+/* pe_render into every film of `targets` (RenderDevice::render with RenderTargets). */
+int pe_render_targets(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings,
+                      const pe_targets* targets, int first_sample, int sample_count, const volatile int32_t* cancel,
+                      pe_batch_result* result);
+
 int pe_profile_enable(pe_device* dev, int enable);
 /* Accumulated times since profiling was enabled (or last read); resets them. */
 int pe_profile_read(pe_device* dev, pe_kernel_stats* out, int capacity, int* count);

@@ -63,6 +63,8 @@ _ORACLE_SIGS = {
     "light_select": (C.c_int, [_VP, C.c_size_t, _F, _F, _F, _F, _I, _F, _I]),
     "spectral": (C.c_int, [C.c_size_t, _F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
     "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
+    "trace_aovs": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I, _F,
+                             _F]),
 }
 
 _REF_SIGS = {
@@ -81,6 +83,8 @@ _REF_SIGS = {
     "render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                          C.c_float, C.c_int, C.c_int, C.c_int, _F, _U, _P(C.c_double)]),
     "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
+    "render_aovs": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                              _F, _F, _F, _F]),
     "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
                                   C.c_float, C.c_int, _F, _U, _F, _P(C.c_uint64)]),
 }
@@ -171,6 +175,20 @@ class OracleScene:
             raise RuntimeError("oracle_trace_samples failed (feature outside the restated scope)")
         return L, (rays[0], rays[1])
 
+    def trace_aovs(self, camera, settings, width, height, px, py, s):
+        """Per-sample radiance and AOVs -> (L (n, 3), albedo (n, 3), normal (n, 3), alpha (n,))."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        aov = np.zeros((px.shape[0], 7), np.float32)
+        rc = self._lib.oracle_trace_aovs(self.h, C.byref(camera), C.byref(settings), width, height, px.shape[0],
+                                         capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32), capi.ptr(s, C.c_int32),
+                                         _fp(L), _fp(aov))
+        if rc != 0:
+            raise RuntimeError("oracle_trace_aovs failed (feature outside the restated scope)")
+        return L, aov[:, 0:3], aov[:, 3:6], aov[:, 6]
+
     def render(self, camera, settings, width, height, first, count, threads=1, rows=None, film=None):
         """Accumulates samples [first, first+count) of every pixel (rows [r0, r1) if given) into film
         (sum (H,W,3), count (H,W), lum2 (H,W)); returns (film, seconds, (extension, shadow) rays)."""
@@ -233,6 +251,17 @@ class RefScene:
                                     filter_width, seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
                                     capi.ptr(s, C.c_int32), _fp(L))
         return L
+
+    def render_aovs(self, view, first, count):
+        """Renderer::renderSamples with AovFilms -> resolved (beauty, albedo, normal, alpha), each (h, w, 3)."""
+        w, h = view.width, view.height
+        out = [np.zeros((h, w, 3), np.float32) for _ in range(4)]
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        rc = self._lib.ref_render_aovs(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov,
+                                       view.aspect, w, h, view.max_bounces, first, count, *[_fp(a) for a in out])
+        if rc != 0:
+            raise RuntimeError("ref_render_aovs failed")
+        return out
 
     def render(self, view, first, count, threads, shadow_quality=1, filter_type=0, filter_width=1.0):
         w, h = view.width, view.height
@@ -340,6 +369,8 @@ _HIPDEV_SIGS = {
                                         C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, _F, _U, _I, C.c_char_p,
                                         C.c_int]),
     "hipdev_enumerate": (C.c_int, [C.c_char_p, C.c_int]),
+    "hipdev_session_render_aovs": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, C.c_int, _F, _F, _F, _F, _U, C.c_char_p, C.c_int]),
     "hipdev_session_render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                                  C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I, C.c_char_p,
                                                  C.c_int]),
@@ -383,6 +414,18 @@ class HipDeviceSession:
             _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
+
+    def render_aovs(self, view, spp, max_batch):
+        """RenderSession with AOVs -> (rc, [beauty, albedo, normal, alpha] each (H,W,3), counts (H,W), error)."""
+        f3 = C.c_float * 3
+        imgs = [np.zeros((view.height, view.width, 3), np.float32) for _ in range(4)]
+        counts = np.zeros((view.height, view.width), np.uint32)
+        err = C.create_string_buffer(512)
+        rc = self._lib.hipdev_session_render_aovs(
+            self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
+            view.height, view.max_bounces, spp, max_batch, *[_fp(a) for a in imgs], capi.ptr(counts, C.c_uint32),
+            err, 512)
+        return rc, imgs, counts, err.value.decode(errors="replace")
 
     def render_adaptive(self, view, spp, max_batch, threshold, min_spp):
         """RenderSession with adaptive sampling -> (rc, mean (H,W,3), counts (H,W), batches, error text)."""

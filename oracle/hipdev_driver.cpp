@@ -148,6 +148,69 @@ HIPDEV_API int hipdev_session_render_adaptive(void* h, const float* camPos, cons
     return 0;
 }
 
+// RenderSession(Final, targetSpp = spp, maxBatch) with AOVs on (SessionConfig::aovs), on a HipDevice: writes the
+// resolved beauty, albedo, normal and alpha images (3 floats/px each) and the beauty's sample counts.
+HIPDEV_API int hipdev_session_render_aovs(void* h, const float* camPos, const float* camTarget, const float* camUp,
+                                          float fov, float aspect, int width, int height, int maxBounces, int spp,
+                                          int maxBatch, float* mean, float* albedo, float* normal, float* alpha,
+                                          uint32_t* counts, char* err, int errlen) {
+    std::string e;
+    std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
+    if (!dev) {
+        copy_error(e, err, errlen);
+        return 2;
+    }
+    dev->setScene(ref_scene_shared(h));
+    if (!dev->hasScene()) {
+        copy_error(dev->lastError(), err, errlen);
+        return 1;
+    }
+    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
+                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
+                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = spp;
+    s.maxBounces = maxBounces;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    SessionConfig cfg;
+    cfg.mode = RenderMode::Final;
+    cfg.targetSpp = spp;
+    cfg.maxBatch = maxBatch;
+    cfg.aovs = true;
+    RenderSession session(*dev);
+    session.begin(cam, s, cfg);
+    while (!session.finished()) {
+        if (!session.advance(nullptr)) {
+            copy_error(dev->lastError().empty() ? "RenderSession::advance returned false" : dev->lastError(), err,
+                       errlen);
+            return 1;
+        }
+    }
+    const Film& film = session.film();
+    const AovFilms* aovs = session.aovs();
+    if (!aovs) {
+        copy_error("the session kept no AOV films", err, errlen);
+        return 1;
+    }
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * static_cast<size_t>(width) + static_cast<size_t>(x);
+            const Color3f c[4] = {film.resolvedPixel(x, y), aovs->albedo.resolvedPixel(x, y),
+                                  aovs->normal.resolvedPixel(x, y), aovs->alpha.resolvedPixel(x, y)};
+            float* outs[4] = {mean, albedo, normal, alpha};
+            for (int j = 0; j < 4; ++j) {
+                outs[j][3 * k] = c[j].r;
+                outs[j][3 * k + 1] = c[j].g;
+                outs[j][3 * k + 2] = c[j].b;
+            }
+            counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+        }
+    return 0;
+}
+
 // The HIP devices as the adapter lists them: writes "id|name|usable|note" lines into out.
 HIPDEV_API int hipdev_enumerate(char* out, int outlen) {
     std::string s;

@@ -1173,8 +1173,16 @@ struct MediumStack {  // path_tracer.cpp:63-83 (entries are material indices; -1
     }
 };
 
-V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& ray, Sobol& sampler, RayCounters& rc) {
+// PathTracer::PrimaryHit (path_tracer.cpp:441-443): the first segment's surface, after camera-invisible skips.
+struct PrimaryHit {
+    bool hit = false;
+    SI isect;
+};
+
+V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& ray, Sobol& sampler, RayCounters& rc,
+      PrimaryHit* primary = nullptr) {
     V3 L(0.0f), throughput(1.0f);  // path_tracer.cpp:321-890
+    bool firstSegment = true;
     Ray cur = ray;
     bool specularBounce = true;
     float lastBsdfPdf = 0.0f;
@@ -1238,6 +1246,11 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                 hit = intersect(scene, cur, h);
                 if (hit) fillHit(scene, cur, h, isect);
             }
+            if (firstSegment && primary) {
+                primary->hit = hit;
+                if (hit) primary->isect = isect;
+            }
+            firstSegment = false;
             // Inside a volume: Beer-Lambert absorption along the segment (:457-479; no scattering media here).
             const int medOwner = media.current();
             V3 sigmaA;
@@ -1436,8 +1449,11 @@ struct Tracer {
     int width, height;
     uint64_t seed;
 
-    // tracePixel (renderer.cpp:139-162) without AOVs: filter offset (dim 0), lens sample (dim 1), Li.
-    V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc) const {
+    // tracePixel (renderer.cpp:139-162): filter offset (dim 0), lens sample (dim 1), Li; with `aov`, the AOVs of
+    // the environment-background branch (:175-205): the primary hit's clamped albedo (primaryAlbedo, :47-58:
+    // Disney baseColor, Lambert albedo, Dielectric tint), its shading normal and coverage 1, or on a miss the
+    // clamped environment radiance along the camera ray, a zero normal and coverage 0.
+    V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* aov = nullptr) const {
         sampler.startPixel(x, y);
         sampler.startSample(s);
         float fu, fv;
@@ -1452,7 +1468,25 @@ struct Tracer {
         Ray ray;
         ray.o = pos;
         ray.d = normalized(target - pos);
-        return Li(scene, it, st, ray, sampler, rc);
+        if (!aov) return Li(scene, it, st, ray, sampler, rc);
+        PrimaryHit primary;
+        const V3 L = Li(scene, it, st, ray, sampler, rc, &primary);
+        auto clamp01 = [](V3 c) {
+            return V3(std::clamp(c.x, 0.0f, 1.0f), std::clamp(c.y, 0.0f, 1.0f), std::clamp(c.z, 0.0f, 1.0f));
+        };
+        V3 albedo(0.0f), normal(0.0f);
+        float alpha = 0.0f;
+        if (primary.hit) {
+            const int m = primary.isect.material;
+            if (m >= 0) albedo = clamp01(c3(scene.materials[static_cast<size_t>(m)].base_color));
+            normal = primary.isect.normal;
+            alpha = 1.0f;
+        } else if (it.hasEnv) {
+            albedo = clamp01(it.env.eval(ray.d));
+        }
+        const float v[7] = {albedo.x, albedo.y, albedo.z, normal.x, normal.y, normal.z, alpha};
+        std::copy(v, v + 7, aov);
+        return L;
     }
 };
 
@@ -1561,6 +1595,33 @@ ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_set
         if (rays) {
             rays[0] = rc.extension;
             rays[1] = rc.shadow;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Per-sample radiance and AOVs (7 floats: albedo rgb, normal xyz, alpha) for n (px, py, sample) triples.
+ORACLE_API int oracle_trace_aovs(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,
+                                 size_t n, const int32_t* px, const int32_t* py, const int32_t* s, float* L,
+                                 float* aov) {
+    try {
+        const Scene& scene = *static_cast<Scene*>(sp);
+        const Integrator& it = integratorOf(scene);
+        Settings st;
+        st.maxDepth = ps->max_bounces;
+        st.rrDepth = ps->rr_depth;
+        st.shadowSamples = std::max(1, ps->shadow_quality);
+        st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
+        Sobol sampler(ps->seed);
+        RayCounters rc;
+        for (size_t i = 0; i < n; ++i) {
+            const V3 c = tr.trace(px[i], py[i], s[i], sampler, rc, aov + 7 * i);
+            L[3 * i] = c.x;
+            L[3 * i + 1] = c.y;
+            L[3 * i + 2] = c.z;
         }
         return 0;
     } catch (const std::exception&) {

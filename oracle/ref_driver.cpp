@@ -23,6 +23,7 @@
 
 #include "photon_hip.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <memory>
@@ -351,6 +352,41 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
             if (counts) counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
         }
     return r.completed ? 0 : 1;
+}
+
+// Renderer::renderSamples of samples [first, first + count) with AovFilms (box filter of width 1): writes the
+// resolved beauty, albedo, normal and alpha images (Film::resolvedPixel: sum * (1 / n)).
+REF_API int ref_render_aovs(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
+                            float aspect, int width, int height, int maxBounces, int firstSample, int count,
+                            float* beauty, float* albedo, float* normal, float* alpha) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = firstSample + count;
+    s.maxBounces = maxBounces;
+    s.numThreads = 1;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    Film film(width, height);
+    AovFilms aovs;
+    aovs.resize(width, height);
+    Renderer renderer;
+    const SampleBatchResult r = renderer.renderSamples(*rs->scene, cam, s, film, firstSample, count, nullptr, &aovs);
+    if (!r.completed) return 1;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * width + x;
+            const Color3f b = film.resolvedPixel(x, y), a = aovs.albedo.resolvedPixel(x, y);
+            const Color3f n = aovs.normal.resolvedPixel(x, y), c = aovs.alpha.resolvedPixel(x, y);
+            const float v[12] = {b.r, b.g, b.b, a.r, a.g, a.b, n.r, n.g, n.b, c.r, c.g, c.b};
+            std::copy(v, v + 3, beauty + 3 * k);
+            std::copy(v + 3, v + 6, albedo + 3 * k);
+            std::copy(v + 6, v + 9, normal + 3 * k);
+            std::copy(v + 9, v + 12, alpha + 3 * k);
+        }
+    return 0;
 }
 
 // Renderer::renderSamples with adaptive sampling over consecutive batches [firsts[b], firsts[b] + counts[b])

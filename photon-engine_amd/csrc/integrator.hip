@@ -51,6 +51,8 @@ struct PathBuffers {
     float4* mix;      // guiding: the mixture of every queue slot (kGuideMixFloat4 float4 each)
     float4* grec;     // guiding: two training records per path, 4 float4 each (see k_guide_collect)
     float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
+    float4* aov;      // AOVs of the camera vertex, two per path: albedo rgb | coverage, shading normal xyz | state
+                      // (-1 until the first segment has been resolved, then 0)
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
     float4* sq_d;     //               direction xyz, w = pend index pid * S + s (uint bits)
@@ -157,6 +159,7 @@ struct LaunchParams {
     uint32_t* active;
     int adapt_check;
     float noise_threshold;
+    int want_aov;         // record the camera vertex's AOVs (pb.aov)
     int max_depth, rr_depth, shadow_samples;
     float clamp_indirect;
     uint64_t seed;
@@ -168,6 +171,11 @@ struct LaunchParams {
     const float* filter_cdf;  // 513 entries when not a box
     V3 cam_pos, cam_ll, cam_h, cam_v;
 };
+
+// clamp01 (renderer.cpp:41-43) of c, with w.
+__device__ __forceinline__ float4 clamped4(V3 c, float w) {
+    return make_float4(fminf(fmaxf(c.x, 0.0f), 1.0f), fminf(fmaxf(c.y, 0.0f), 1.0f), fminf(fmaxf(c.z, 0.0f), 1.0f), w);
+}
 
 // Wave-aggregated append: returns this lane's slot in the queue behind `counter`.
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
@@ -237,6 +245,10 @@ __global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffer
         pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
         pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
         if (lp.transport || lp.guide_records) pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (lp.want_aov) {
+            pb.aov[2 * static_cast<size_t>(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            pb.aov[2 * static_cast<size_t>(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        }
         if (lp.guide_records)
             for (int r = 0; r < kGuideRecords; ++r) pb.grec[static_cast<size_t>(i) * 8 + 4 * r + 3] = make_float4(0, 0, 0, 0);
         for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
@@ -612,6 +624,12 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     }
                 }
             } else if (tri < 0) {
+                // The camera ray's first segment misses: the AOVs of the background (renderer.cpp:196-204).
+                if (lp.want_aov && camera && pb.aov[2 * static_cast<size_t>(pid) + 1].w < 0.0f) {
+                    pb.aov[2 * static_cast<size_t>(pid)] =
+                        sc.has_env ? clamped4(env_eval(sc.env, dir), 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    pb.aov[2 * static_cast<size_t>(pid) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
                 // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
                 // environment with MIS against NEE (:482-514).
                 const int top = kTransport ? ext_media_top(ext) : -1;
@@ -638,6 +656,13 @@ __global__ void __launch_bounds__(kBlock) k_shade(SceneView sc, LaunchParams lp,
                     new_d = dir;
                     extend = true;
                 } else {
+                    // The camera ray's first surface: its AOVs (renderer.cpp:183-194, primaryAlbedo :47-58). A camera
+                    // path stays one through delta refractions, but only its first segment is the primary hit
+                    // (path_tracer.cpp:441-444).
+                    if (lp.want_aov && camera && pb.aov[2 * static_cast<size_t>(pid) + 1].w < 0.0f) {
+                        pb.aov[2 * static_cast<size_t>(pid)] = clamped4(sc.materials[si.material].base_color, 1.0f);
+                        pb.aov[2 * static_cast<size_t>(pid) + 1] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                    }
                     // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
                     const int top = kTransport ? ext_media_top(ext) : -1;
                     if (top >= 0 && (sc.materials[top].flags & kMatFlagMedium)) {
@@ -896,9 +921,52 @@ __device__ __forceinline__ float film_luminance(V3 c) {
     return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z;
 }
 
+// A device film's arrays (pe_film).
+struct FilmView {
+    float* sum;
+    uint32_t* count;
+    float* lum2;
+    unsigned long long* rejected;
+};
+
+// Film::addSample (film.cpp:56-71) or, signed, Film::addSampleSigned (:73-86) of the samples of one pixel.
+template <bool kSigned>
+__device__ __forceinline__ void film_add_aov(const FilmView& f, uint32_t p, const float4* __restrict__ aov, int which,
+                                             uint32_t q, uint32_t pixels, int spl, bool splat_w) {
+#pragma clang fp contract(off)
+    if (!f.sum) return;
+    float sr = f.sum[3 * static_cast<size_t>(p)], sg = f.sum[3 * static_cast<size_t>(p) + 1],
+          sb = f.sum[3 * static_cast<size_t>(p) + 2];
+    float l2 = kSigned ? 0.0f : f.lum2[p];
+    uint32_t rej = 0;
+    for (int k = 0; k < spl; ++k) {
+        const float4 a = aov[2 * static_cast<size_t>(q + static_cast<uint32_t>(k) * pixels) + which];
+        const V3 v = splat_w ? splat(a.w) : mk(a.x, a.y, a.z);
+        const bool ok = finite3(v) && (kSigned || (v.x >= 0.0f && v.y >= 0.0f && v.z >= 0.0f));
+        if (!ok) {
+            ++rej;
+            continue;
+        }
+        sr += v.x;
+        sg += v.y;
+        sb += v.z;
+        if (!kSigned) {
+            const float lum = film_luminance(v);
+            l2 += lum * lum;
+        }
+    }
+    f.sum[3 * static_cast<size_t>(p)] = sr;
+    f.sum[3 * static_cast<size_t>(p) + 1] = sg;
+    f.sum[3 * static_cast<size_t>(p) + 2] = sb;
+    f.count[p] += static_cast<uint32_t>(spl);
+    if (!kSigned) f.lum2[p] = l2;
+    if (rej) atomicAdd(f.rejected, static_cast<unsigned long long>(rej));
+}
+
 __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb, float* __restrict__ sum,
                                                  uint32_t* __restrict__ count, float* __restrict__ lum2,
-                                                 unsigned long long* __restrict__ rejected) {
+                                                 unsigned long long* __restrict__ rejected, FilmView f_albedo,
+                                                 FilmView f_normal, FilmView f_alpha) {
 #pragma clang fp contract(off)
     const uint32_t pixels = lp.num_pixels;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -935,6 +1003,11 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
         count[p] = c;
         lum2[p] = l2;
         if (rej) atomicAdd(rejected, static_cast<unsigned long long>(rej));
+        if (lp.want_aov) {  // addAovs (renderer.cpp:222-227)
+            film_add_aov<false>(f_albedo, p, pb.aov, 0, q, pixels, lp.spl, false);
+            film_add_aov<true>(f_normal, p, pb.aov, 1, q, pixels, lp.spl, false);
+            film_add_aov<false>(f_alpha, p, pb.aov, 0, q, pixels, lp.spl, true);
+        }
         // renderer.cpp:500-502: the pixel stops for the rest of the batch once its own error is low enough.
         if (lp.adapt_check && film_relative_error(sr, sg, sb, c, l2) < lp.noise_threshold) lp.active[p] = 0u;
     }
@@ -1120,6 +1193,7 @@ struct Wavefront {
     DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, ctr, stats, filter;
     // Adaptive sampling: per-pixel active flags and clean flags, the active pixel list, compaction counts.
     DevBuf adapt_active, adapt_clean, adapt_list, adapt_counts;
+    DevBuf aov;  // AOV records, two float4 per path, allocated by the first render that asks for AOVs
     uint32_t adapt_pixels = 0;
     int filter_type = -1;
     float filter_width = 0.0f;
@@ -1222,7 +1296,7 @@ uint64_t wavefront_bytes_per_path(int shadow_quality) {
 // traversal-stack spill, so lanes on different streams never share device scratch.
 int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
                    const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
-                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide) {
+                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide, const AovTargets* aovs) {
     if (result) {
         result->completed = 0;
         result->samples_traced = 0;
@@ -1244,6 +1318,21 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
     if (guide && guide->dev != dev) {
         set_error("pe_render: the guide belongs to another device");
         return PE_ERR_INVALID;
+    }
+    FilmView aov_view[3] = {};
+    bool want_aov = false;
+    if (aovs) {
+        pe_film* const fs[3] = {aovs->albedo, aovs->normal, aovs->alpha};
+        for (int k = 0; k < 3; ++k) {
+            if (!fs[k]) continue;
+            if (fs[k]->dev != dev || fs[k]->width != film->width || fs[k]->height != film->height) {
+                set_error("pe_render: an AOV film of another device or size");
+                return PE_ERR_INVALID;
+            }
+            aov_view[k] = FilmView{fs[k]->sum.as<float>(), fs[k]->count.as<uint32_t>(), fs[k]->lum2.as<float>(),
+                                   fs[k]->rejected.as<unsigned long long>()};
+            want_aov = true;
+        }
     }
     if (first_sample < 0 || sample_count < 0) {
         set_error("pe_render: negative sample range");
@@ -1295,6 +1384,9 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             rc = guide_reserve_paths(guide, wf->capacity, wf->slots());
             if (rc != PE_OK) return rc;
         }
+        if (want_aov && wf->aov.bytes() < static_cast<size_t>(wf->capacity) * 32 &&
+            !wf->aov.alloc(static_cast<size_t>(wf->capacity) * 32))
+            return PE_ERR_OOM;
 
         LaunchParams lp{};
         lp.width = film->width;
@@ -1328,6 +1420,8 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
 
         PathBuffers pb = wf->view();
+        pb.aov = want_aov ? wf->aov.as<float4>() : nullptr;
+        lp.want_aov = want_aov ? 1 : 0;
         pb.mix = guide ? guide->mix.as<float4>() : nullptr;
         pb.grec = guide ? guide->path_records.as<float4>() : nullptr;
         hipStream_t s = lane.stream;
@@ -1471,7 +1565,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             wf->prof.begin(kKFilm, s);
             hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, film->sum.as<float>(),
                                film->count.as<uint32_t>(), film->lum2.as<float>(),
-                               film->rejected.as<unsigned long long>());
+                               film->rejected.as<unsigned long long>(), aov_view[0], aov_view[1], aov_view[2]);
             wf->prof.end(s);
             PE_HIP_CHECK(hipGetLastError());
             if (guide) {
@@ -1534,6 +1628,32 @@ extern "C" int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camer
                               nullptr);
     } catch (const std::exception& e) {
         set_error(std::string("pe_render: ") + e.what());
+        return PE_ERR_INVALID;
+    }
+}
+
+extern "C" int pe_render_targets(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings,
+                                 const pe_targets* targets, int first_sample, int sample_count,
+                                 const volatile int32_t* cancel, pe_batch_result* result) {
+    if (!dev || !targets) {
+        if (result) *result = pe_batch_result{};
+        set_error("pe_render_targets: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    try {
+        PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+        int blocks = 0;
+        const int rc = ensure_trace_resources(dev, blocks);
+        if (rc != PE_OK) return rc;
+        const RenderLane lane{dev->stream, wavefront_of(dev), dev->spill.as<uint32_t>(), blocks};
+        AovTargets aovs;
+        aovs.albedo = targets->albedo;
+        aovs.normal = targets->normal;
+        aovs.alpha = targets->alpha;
+        return render_samples(dev, lane, scene, camera, settings, targets->beauty, first_sample, sample_count, cancel,
+                              result, nullptr, &aovs);
+    } catch (const std::exception& e) {
+        set_error(std::string("pe_render_targets: ") + e.what());
         return PE_ERR_INVALID;
     }
 }

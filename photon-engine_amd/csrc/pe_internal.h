@@ -116,9 +116,16 @@ struct RenderLane {
     uint32_t* spill;
     int grid_blocks;
 };
+// AOV films of a render (any may be null).
+struct AovTargets {
+    pe_film* albedo = nullptr;
+    pe_film* normal = nullptr;
+    pe_film* alpha = nullptr;
+};
 int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
                    const pe_settings* settings, pe_film* film, int first_sample, int sample_count,
-                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide = nullptr);
+                   const volatile int32_t* cancel, pe_batch_result* result, pe_guide* guide = nullptr,
+                   const AovTargets* aovs = nullptr);
 // Bytes of traversal-stack spill a lane of `grid_blocks` blocks needs (capi.hip).
 size_t trace_spill_bytes(int grid_blocks);
 }  // namespace pe

@@ -293,7 +293,8 @@ std::unique_ptr<HipDevice> HipDevice::create(int ordinal, std::string* error) {
 }
 
 HipDevice::~HipDevice() {
-    if (m_film) pe_film_destroy(m_film);
+    for (DeviceFilm& f : m_films)
+        if (f.film) pe_film_destroy(f.film);
     if (m_scene) pe_scene_release(m_scene);
     if (m_dev) pe_device_destroy(m_dev);
 }
@@ -379,26 +380,55 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     const bool adaptive = settings.adaptiveSampling;
 
     const int w = film.width(), h = film.height();
-    if (!m_film || m_filmW != w || m_filmH != h) {
-        if (m_film) pe_film_destroy(m_film);
-        m_film = nullptr;
-        if (pe_film_create(m_dev, w, h, &m_film) != PE_OK) {
-            m_error = "HipDevice::render: " + lastHipError();
-            return res;
+    // The caller's films: beauty and, when it asks for them, the AOVs (renderer.cpp:222-227).
+    Film* targets_of[4] = {&film, nullptr, nullptr, nullptr};
+    if (targets.aovs) {
+        AovFilms& a = *targets.aovs;
+        if (a.albedo.width() != w || a.albedo.height() != h) a.resize(w, h);
+        targets_of[1] = &a.albedo;
+        targets_of[2] = &a.normal;
+        targets_of[3] = &a.alpha;
+    }
+    if (m_filmW != w || m_filmH != h) {
+        for (DeviceFilm& f : m_films) {
+            if (f.film) pe_film_destroy(f.film);
+            f.film = nullptr;
         }
         m_filmW = w;
         m_filmH = h;
         m_accumulating = false;
     }
     const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
-    if (!adaptive || firstSample == 0 || !m_accumulating) {
-        if (pe_film_clear(m_film) != PE_OK) {
-            m_error = "HipDevice::render: " + lastHipError();
-            return res;
+    const bool restart = !adaptive || firstSample == 0 || !m_accumulating;
+    for (int k = 0; k < 4; ++k) {
+        if (!targets_of[k]) continue;
+        DeviceFilm& f = m_films[k];
+        if (!f.film) {
+            if (pe_film_create(m_dev, w, h, &f.film) != PE_OK) {
+                m_error = "HipDevice::render: " + lastHipError();
+                return res;
+            }
+            if (!restart) {  // an AOV film joining an accumulating session starts from this call
+                f.prevSum.assign(3 * pixels, 0.0f);
+                f.prevCount.assign(pixels, 0u);
+            }
         }
-        m_prevSum.assign(3 * pixels, 0.0f);
-        m_prevCount.assign(pixels, 0u);
-        m_accumulating = adaptive;
+        if (restart) {
+            if (pe_film_clear(f.film) != PE_OK) {
+                m_error = "HipDevice::render: " + lastHipError();
+                return res;
+            }
+            f.prevSum.assign(3 * pixels, 0.0f);
+            f.prevCount.assign(pixels, 0u);
+        }
+    }
+    if (restart) m_accumulating = adaptive;
+    pe_targets tg{};
+    tg.beauty = m_films[0].film;
+    if (targets_of[1]) {
+        tg.albedo = m_films[1].film;
+        tg.normal = m_films[2].film;
+        tg.alpha = m_films[3].film;
     }
     // Cancellation is polled between chunks; one chunk when nobody can cancel. An adaptive batch is one call:
     // its skip mask and stops belong to the batch as a whole.
@@ -413,7 +443,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         }
         const int n = std::min(chunk, count - done);
         pe_batch_result br{};
-        if (pe_render(m_dev, m_scene, &cam, &st, m_film, firstSample + done, n, nullptr, &br) != PE_OK) {
+        if (pe_render_targets(m_dev, m_scene, &cam, &st, &tg, firstSample + done, n, nullptr, &br) != PE_OK) {
             m_error = "HipDevice::render: " + lastHipError();
             return res;
         }
@@ -421,25 +451,31 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         traced += br.samples_traced;
         done += n;
     }
+    // Merge what this call added (the whole film unless it accumulates across calls) with Film::addBatch; the
+    // normal AOV keeps its sign (addSampleSigned, film.cpp:73-86).
     std::vector<float> sum(3 * pixels), lum2(pixels);
     std::vector<uint32_t> cnt(pixels);
-    uint64_t rejected = 0;
-    if (pe_film_download(m_film, sum.data(), cnt.data(), lum2.data(), &rejected) != PE_OK) {
-        m_error = "HipDevice::render: " + lastHipError();
-        return res;
-    }
-    // Merge what this call added (the whole film unless it accumulates across calls) with Film::addBatch.
-    for (int y = 0; y < h; ++y)
-        for (int x = 0; x < w; ++x) {
-            const size_t p = static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x);
-            film.addBatch(x, y,
-                          Color3f(sum[3 * p] - m_prevSum[3 * p], sum[3 * p + 1] - m_prevSum[3 * p + 1],
-                                  sum[3 * p + 2] - m_prevSum[3 * p + 2]),
-                          cnt[p] - m_prevCount[p]);
+    for (int k = 0; k < 4; ++k) {
+        if (!targets_of[k]) continue;
+        DeviceFilm& f = m_films[k];
+        uint64_t rejected = 0;
+        if (pe_film_download(f.film, sum.data(), cnt.data(), lum2.data(), &rejected) != PE_OK) {
+            m_error = "HipDevice::render: " + lastHipError();
+            return res;
         }
-    if (m_accumulating) {
-        m_prevSum = std::move(sum);
-        m_prevCount = std::move(cnt);
+        Film& dst = *targets_of[k];
+        for (int y = 0; y < h; ++y)
+            for (int x = 0; x < w; ++x) {
+                const size_t p = static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x);
+                dst.addBatch(x, y,
+                             Color3f(sum[3 * p] - f.prevSum[3 * p], sum[3 * p + 1] - f.prevSum[3 * p + 1],
+                                     sum[3 * p + 2] - f.prevSum[3 * p + 2]),
+                             cnt[p] - f.prevCount[p], k == 2);
+            }
+        if (m_accumulating) {
+            f.prevSum = sum;
+            f.prevCount = cnt;
+        }
     }
     res.completed = !cancelled;
     res.samplesTraced = traced;

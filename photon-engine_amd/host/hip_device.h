@@ -23,9 +23,9 @@ public:
     ~HipDevice() override;
 
     const DeviceInfo& info() const override { return m_info; }
-    // Adaptive sampling is supported; AOVs and light passes are not, and RenderSession turns them off with a
-    // warning (render_session.cpp:23-35).
-    DeviceFeatures features() const override { return DeviceFeatures{false, false, true}; }
+    // AOVs (albedo / normal / alpha) and adaptive sampling are supported; light passes are not, and
+    // RenderSession turns them off with a warning (render_session.cpp:23-35).
+    DeviceFeatures features() const override { return DeviceFeatures{true, false, true}; }
     void setScene(std::shared_ptr<const Scene> scene) override;
     void updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind kind) override;
     bool hasScene() const override { return m_scene != nullptr; }
@@ -43,14 +43,17 @@ private:
     DeviceInfo m_info;
     pe_device* m_dev = nullptr;
     pe_scene* m_scene = nullptr;
-    pe_film* m_film = nullptr;
+    // Device films: beauty, then the albedo, normal and alpha AOVs. With adaptive sampling they keep every sample
+    // this device rendered since the last render that started at sample 0 (the beauty's per-sample statistics
+    // drive the skip mask and the stops); prevSum / prevCount is the part already merged into the caller's films.
+    struct DeviceFilm {
+        pe_film* film = nullptr;
+        std::vector<float> prevSum;
+        std::vector<uint32_t> prevCount;
+    };
+    DeviceFilm m_films[4];
     int m_filmW = 0, m_filmH = 0;
-    // Adaptive sampling: the device film keeps every sample this device rendered since the last render that
-    // started at sample 0 (its per-sample statistics drive the skip mask and the stops); m_prev* is the part
-    // already merged into the caller's Film.
     bool m_accumulating = false;
-    std::vector<float> m_prevSum;
-    std::vector<uint32_t> m_prevCount;
     std::shared_ptr<const Scene> m_sceneRef;
     std::string m_error;
     uint64_t m_lastRays = 0;

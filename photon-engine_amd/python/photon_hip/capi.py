@@ -104,6 +104,11 @@ class BatchResult(C.Structure):
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_visits", C.c_uint64)]
 
 
+class Targets(C.Structure):
+    _fields_ = [("beauty", C.c_void_p), ("albedo", C.c_void_p), ("normal", C.c_void_p), ("alpha", C.c_void_p),
+                ("reserved", C.c_void_p * 4)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -164,6 +169,8 @@ SIGNATURES = {
     "pe_film_destroy": (C.c_int, [_VP]),
     "pe_film_clear": (C.c_int, [_VP]),
     "pe_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_float), _P(C.c_uint64)]),
+    "pe_render_targets": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _P(Targets), C.c_int, C.c_int,
+                                    _P(C.c_int32), _P(BatchResult)]),
     "pe_render": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _VP, C.c_int, C.c_int, _P(C.c_int32),
                             _P(BatchResult)]),
     "pe_profile_enable": (C.c_int, [_VP, C.c_int]),

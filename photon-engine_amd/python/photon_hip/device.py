@@ -159,6 +159,22 @@ class GpuFilm:
         return s / np.maximum(c, 1)[..., None]
 
 
+def render_targets(dev: Device, scene: GpuScene, camera: capi.Camera, st: capi.Settings, film: GpuFilm, first: int,
+                   count: int, albedo: GpuFilm | None = None, normal: GpuFilm | None = None,
+                   alpha: GpuFilm | None = None, cancel=None) -> capi.BatchResult:
+    """pe_render_targets: the beauty film plus the AOV films given (RenderTargets with AovFilms)."""
+    res = capi.BatchResult()
+    t = capi.Targets()
+    t.beauty = film.h
+    t.albedo, t.normal, t.alpha = (f.h if f is not None else None for f in (albedo, normal, alpha))
+    cptr = ptr(cancel, C.c_int32) if cancel is not None else C.cast(None, C.POINTER(C.c_int32))
+    rc = load().pe_render_targets(dev.h, scene.h, C.byref(camera), C.byref(st), C.byref(t), first, count, cptr,
+                                  C.byref(res))
+    if rc not in (capi.PE_OK, capi.PE_ERR_CANCELLED):
+        check("pe_render_targets", rc)
+    return res
+
+
 def render(dev: Device, scene: GpuScene, camera: capi.Camera, st: capi.Settings, film: GpuFilm, first: int,
            count: int, cancel=None) -> capi.BatchResult:
     res = capi.BatchResult()
