@@ -267,6 +267,38 @@ int pe_render_targets(pe_device* dev, pe_scene* scene, const pe_camera* camera, 
                       const pe_targets* targets, int first_sample, int sample_count, const volatile int32_t* cancel,
                       pe_batch_result* result);
 
+/* ---- display (film -> imager -> tone map -> 8-bit), on the device -------------------------------------
+ * Film::resolve (film.cpp:139-147), applyImager (core/image/imager.cpp, ImagerSettings imager.h:12-36: chromatic
+ * aberration, bloom, glare, white balance, contrast / saturation, vignette), toneMap (tone_mapping.cpp:146-160:
+ * exposure, operator, clamp, sRGB encode) and, for 8-bit output, quantizeUnorm8 with saveImagePNG's
+ * deterministic TPDF dither (transfer.cpp:28-52, image_io.cpp:61-80). */
+#define PE_TONEMAP_REINHARD 0           /* ToneMapOperator (tone_mapping.h:14-22), same order */
+#define PE_TONEMAP_REINHARD_EXTENDED 1
+#define PE_TONEMAP_ACES 2
+#define PE_TONEMAP_FILMIC 3
+#define PE_TONEMAP_AGX 4
+#define PE_TONEMAP_PBR_NEUTRAL 5
+#define PE_TONEMAP_LINEAR 6
+
+typedef struct pe_display {
+    int32_t tonemap;            /* PE_TONEMAP_* */
+    float exposure_ev;          /* also the imager's exposure (bloom threshold, contrast pivot) */
+    int32_t dither;             /* 8-bit output: 1 = TPDF dither (saveImagePNG), 0 = plain rounding */
+    float bloom_intensity, bloom_radius, bloom_threshold;
+    float glare_intensity;
+    int32_t glare_blades;
+    float glare_angle_deg, glare_length;
+    float white_balance_k, tint, saturation, contrast;
+    float vignette, chromatic_aberration;
+    float reserved[4];
+} pe_display;
+
+/* The reference's defaults: ACES, 0 EV, dither on, ImagerSettings{} (identity). */
+int pe_display_defaults(pe_display* out);
+/* The film's display image: display_rgb (3 * w * h floats in [0, 1], sRGB-encoded) and / or rgb8 (3 * w * h
+ * bytes); either may be NULL. Rows top first, as the film. */
+int pe_film_display(pe_film* film, const pe_display* settings, float* display_rgb, uint8_t* rgb8);
+
 int pe_profile_enable(pe_device* dev, int enable);
 /* Accumulated times since profiling was enabled (or last read); resets them. */
 int pe_profile_read(pe_device* dev, pe_kernel_stats* out, int capacity, int* count);
@@ -398,6 +430,8 @@ int pe_job_wait(pe_server* server, uint64_t id, int32_t timeout_ms, pe_job_statu
 int pe_job_cancel(pe_server* server, uint64_t id);
 /* Copies the job's film (whole chunks only) into host arrays, as pe_film_download; any pointer may be NULL. */
 int pe_job_download(pe_server* server, uint64_t id, float* sum_rgb, uint32_t* count, float* sum_lum2);
+/* The job's display image (whole chunks only), as pe_film_display of its film: the preview a client shows. */
+int pe_job_display(pe_server* server, uint64_t id, const pe_display* settings, float* display_rgb, uint8_t* rgb8);
 /* Forgets an ended job and frees its film (a running job is cancelled first). */
 int pe_job_release(pe_server* server, uint64_t id);
 

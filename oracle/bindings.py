@@ -63,6 +63,7 @@ _ORACLE_SIGS = {
     "light_select": (C.c_int, [_VP, C.c_size_t, _F, _F, _F, _F, _I, _F, _I]),
     "spectral": (C.c_int, [C.c_size_t, _F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
     "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
+    "display": (C.c_int, [C.c_int, C.c_int, _F, _U, _P(capi.Display), _F, _P(C.c_uint8)]),
     "trace_aovs": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I, _F,
                              _F]),
 }
@@ -83,6 +84,7 @@ _REF_SIGS = {
     "render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                          C.c_float, C.c_int, C.c_int, C.c_int, _F, _U, _P(C.c_double)]),
     "thin_film": (C.c_int, [C.c_size_t, _F, C.c_float, C.c_float, C.c_float, C.c_float, _F, _F, _F]),
+    "display": (C.c_int, [C.c_int, C.c_int, _F, _U, _P(capi.Display), _F, _P(C.c_uint8)]),
     "render_aovs": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                               _F, _F, _F, _F]),
     "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
@@ -273,6 +275,24 @@ class RefScene:
                              view.max_bounces, shadow_quality, filter_type, filter_width, first, count, threads,
                              _fp(mean), capi.ptr(cnt, C.c_uint32), C.byref(secs))
         return mean, cnt, secs.value
+
+
+def display(lib_kind: str, sum_rgb, count, settings):
+    """The display path of a film (sums (h, w, 3), counts (h, w)) by the oracle ('oracle') or the reference
+    ('ref' / 'ref_strict') -> (display (h, w, 3) float32, rgb8 (h, w, 3) uint8)."""
+    sum_rgb = np.ascontiguousarray(sum_rgb, np.float32)
+    count = np.ascontiguousarray(count, np.uint32)
+    h, w = count.shape
+    out = np.zeros((h, w, 3), np.float32)
+    rgb8 = np.zeros((h, w, 3), np.uint8)
+    if lib_kind == "oracle":
+        fn = oracle().oracle_display
+    else:
+        fn = reference(strict=(lib_kind == "ref_strict")).ref_display
+    rc = fn(w, h, _fp(sum_rgb), capi.ptr(count, C.c_uint32), C.byref(settings), _fp(out), capi.ptr(rgb8, C.c_uint8))
+    if rc != 0:
+        raise RuntimeError("display failed")
+    return out, rgb8
 
 
 def ref_render_adaptive(scene_h, lib, view, batches, threshold, min_spp):

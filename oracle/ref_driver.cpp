@@ -10,6 +10,9 @@
 #include "engine/scene.h"
 #include "camera/perspective_camera.h"
 #include "core/image/pixel_filter.h"
+#include "core/image/imager.h"
+#include "core/image/tone_mapping.h"
+#include "core/color/transfer.h"
 #include "geometry/mesh.h"
 #include "integrators/path_tracer.h"
 #include "lights/area_light.h"
@@ -352,6 +355,46 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
             if (counts) counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
         }
     return r.completed ? 0 : 1;
+}
+
+// The reference's display path for a film given as sums and counts (Film::addBatch of each pixel into an empty
+// Film): Film::resolve, applyImager with the settings of `ds`, then per pixel toneMap and quantizeUnorm8 with
+// tpdfDither, as saveImagePNG does (image_io.cpp:61-80). display and rgb8 may be NULL.
+REF_API int ref_display(int w, int h, const float* sum, const uint32_t* count, const pe_display* ds, float* display,
+                        uint8_t* rgb8) {
+    Film film(w, h);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t k = static_cast<size_t>(y) * w + x;
+            film.addBatch(x, y, Color3f(sum[3 * k], sum[3 * k + 1], sum[3 * k + 2]), count[k]);
+        }
+    Image img = film.resolve();
+    ImagerSettings is;
+    is.bloomIntensity = ds->bloom_intensity;
+    is.bloomRadius = ds->bloom_radius;
+    is.bloomThreshold = ds->bloom_threshold;
+    is.glareIntensity = ds->glare_intensity;
+    is.glareBlades = ds->glare_blades;
+    is.glareAngleDeg = ds->glare_angle_deg;
+    is.glareLength = ds->glare_length;
+    is.whiteBalanceK = ds->white_balance_k;
+    is.tint = ds->tint;
+    is.saturation = ds->saturation;
+    is.contrast = ds->contrast;
+    is.vignette = ds->vignette;
+    is.chromaticAberration = ds->chromatic_aberration;
+    applyImager(img, is, ds->exposure_ev);
+    const auto op = static_cast<ToneMapOperator>(ds->tonemap);
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < w; ++x) {
+            const size_t k = static_cast<size_t>(y) * w + x;
+            const Color3f d = toneMap(img.getPixel(x, y), op, ds->exposure_ev);
+            for (int c = 0; c < 3; ++c) {
+                if (display) display[3 * k + c] = d[c];
+                if (rgb8) rgb8[3 * k + c] = quantizeUnorm8(d[c], ds->dither ? tpdfDither(x, y, c) : 0.0f);
+            }
+        }
+    return 0;
 }
 
 // Renderer::renderSamples of samples [first, first + count) with AovFilms (box filter of width 1): writes the

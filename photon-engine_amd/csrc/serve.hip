@@ -462,6 +462,29 @@ int pe_job_download(pe_server* s, uint64_t id, float* sum_rgb, uint32_t* count, 
     return PE_OK;
 }
 
+int pe_job_display(pe_server* s, uint64_t id, const pe_display* settings, float* display_rgb, uint8_t* rgb8) {
+    if (!s || !settings) {
+        set_error("pe_job_display: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    Job* j = find(s, id);
+    if (!j) return PE_ERR_INVALID;
+    s->cv.wait(lk, [&] { return !j->running; });  // whole chunks only
+    if (!j->film) {
+        set_error("pe_job_display: the job holds no film (queued, or failed before admission)");
+        return PE_ERR_INVALID;
+    }
+    j->copying = true;
+    pe_film* f = j->film.get();
+    lk.unlock();
+    const int rc = pe_film_display(f, settings, display_rgb, rgb8);
+    lk.lock();
+    j->copying = false;
+    s->cv.notify_all();
+    return rc;
+}
+
 int pe_job_release(pe_server* s, uint64_t id) {
     if (!s) {
         set_error("pe_job_release: NULL server");

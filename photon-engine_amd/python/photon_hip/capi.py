@@ -109,6 +109,19 @@ class Targets(C.Structure):
                 ("reserved", C.c_void_p * 4)]
 
 
+PE_TONEMAP_REINHARD, PE_TONEMAP_REINHARD_EXTENDED, PE_TONEMAP_ACES, PE_TONEMAP_FILMIC = 0, 1, 2, 3
+PE_TONEMAP_AGX, PE_TONEMAP_PBR_NEUTRAL, PE_TONEMAP_LINEAR = 4, 5, 6
+
+
+class Display(C.Structure):
+    _fields_ = [("tonemap", C.c_int32), ("exposure_ev", C.c_float), ("dither", C.c_int32),
+                ("bloom_intensity", C.c_float), ("bloom_radius", C.c_float), ("bloom_threshold", C.c_float),
+                ("glare_intensity", C.c_float), ("glare_blades", C.c_int32), ("glare_angle_deg", C.c_float),
+                ("glare_length", C.c_float), ("white_balance_k", C.c_float), ("tint", C.c_float),
+                ("saturation", C.c_float), ("contrast", C.c_float), ("vignette", C.c_float),
+                ("chromatic_aberration", C.c_float), ("reserved", C.c_float * 4)]
+
+
 class KernelStats(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -169,6 +182,9 @@ SIGNATURES = {
     "pe_film_destroy": (C.c_int, [_VP]),
     "pe_film_clear": (C.c_int, [_VP]),
     "pe_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_float), _P(C.c_uint64)]),
+    "pe_display_defaults": (C.c_int, [_P(Display)]),
+    "pe_film_display": (C.c_int, [_VP, _P(Display), _P(C.c_float), _P(C.c_uint8)]),
+    "pe_job_display": (C.c_int, [_VP, C.c_uint64, _P(Display), _P(C.c_float), _P(C.c_uint8)]),
     "pe_render_targets": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _P(Targets), C.c_int, C.c_int,
                                     _P(C.c_int32), _P(BatchResult)]),
     "pe_render": (C.c_int, [_VP, _VP, _P(Camera), _P(Settings), _VP, C.c_int, C.c_int, _P(C.c_int32),

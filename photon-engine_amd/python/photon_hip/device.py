@@ -124,6 +124,15 @@ class GpuScene:
         return occ
 
 
+def display_settings(**overrides) -> capi.Display:
+    """pe_display with the reference's defaults (pe_display_defaults), fields overridden by name."""
+    d = capi.Display()
+    check("pe_display_defaults", load().pe_display_defaults(C.byref(d)))
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
 class GpuFilm:
     def __init__(self, dev: Device, width: int, height: int):
         self.dev = dev
@@ -153,6 +162,14 @@ class GpuFilm:
         check("pe_film_download", self._lib.pe_film_download(self.h, ptr(s, C.c_float), ptr(c, C.c_uint32),
                                                              ptr(l2, C.c_float), C.byref(rej)))
         return s, c, l2, rej.value
+
+    def display(self, settings: capi.Display | None = None):
+        """pe_film_display -> (display (H, W, 3) float32 in [0, 1], rgb8 (H, W, 3) uint8)."""
+        d = settings if settings is not None else display_settings()
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        rgb8 = np.zeros((self.height, self.width, 3), np.uint8)
+        check("pe_film_display", self._lib.pe_film_display(self.h, C.byref(d), ptr(out, C.c_float), ptr(rgb8, C.c_uint8)))
+        return out, rgb8
 
     def mean(self) -> np.ndarray:
         s, c, _, _ = self.download()

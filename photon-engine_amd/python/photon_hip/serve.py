@@ -94,6 +94,18 @@ class Server:
                                                            ptr(l2, C.c_float)))
         return s, c, l2
 
+    def display(self, jid: int, settings=None):
+        """pe_job_display -> (display (H, W, 3) float32, rgb8 (H, W, 3) uint8)."""
+        from .device import display_settings
+
+        w, h = self._sizes[jid]
+        d = settings if settings is not None else display_settings()
+        out = np.zeros((h, w, 3), np.float32)
+        rgb8 = np.zeros((h, w, 3), np.uint8)
+        check("pe_job_display", self._lib.pe_job_display(self.h, jid, C.byref(d), ptr(out, C.c_float),
+                                                         ptr(rgb8, C.c_uint8)))
+        return out, rgb8
+
     def release(self, jid: int) -> None:
         check("pe_job_release", self._lib.pe_job_release(self.h, jid))
         self._sizes.pop(jid, None)

@@ -137,3 +137,32 @@ def test_cancel_release_and_errors(dev):
         assert srv.wait(j2, 60000).samples_done == 3
     finally:
         srv.close()
+
+
+@pytest.mark.gpu
+def test_job_display_is_the_display_of_its_film(dev):
+    """pe_job_display of a finished job equals pe_film_display of the same samples rendered directly (the job's
+    film equals pe_render over its chunk boundaries), with an imager and tone map that exercise every stage."""
+    from photon_hip.device import display_settings
+
+    (sc, view) = configs.product_sphere(width=64, height=36, lon=64, lat=48, env_width=128, env_height=64)
+    gs = dev.upload(sc)
+    d = display_settings(tonemap=capi.PE_TONEMAP_AGX, exposure_ev=0.5, bloom_intensity=0.4, glare_intensity=0.2,
+                         white_balance_k=5000.0, contrast=1.2, vignette=0.3)
+    srv = Server(dev, slots=2, chunk_spp=4, max_chunk_paths=CHUNK_PATHS)
+    try:
+        jid = _submit(srv, gs, view, spp=12)
+        assert srv.wait(jid, 60000).state == capi.PE_JOB_DONE
+        job_disp, job8 = srv.display(jid, d)
+    finally:
+        srv.close()
+    film = dev.film(view.width, view.height)
+    first = 0
+    for n in (8, 4):
+        render(dev, gs, _cam(view), settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=n), film,
+               first, n)
+        first += n
+    disp, rgb8 = film.display(d)
+    np.testing.assert_array_equal(job_disp, disp)
+    np.testing.assert_array_equal(job8, rgb8)
+    assert 0.05 < disp.mean() < 0.95
