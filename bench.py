@@ -202,7 +202,7 @@ def guiding_run(dev, budget_spp: int, ref_spp: int, spl: int = 16, seed: int = 1
     guided_film = dev.film(view.width, view.height)
     dev.synchronize()
     t0 = time.perf_counter()
-    guide = Guide(dev, gs, seed=seed, learning_rate=2e-3, steps_per_launch=16, train_launches=6)
+    guide = Guide(dev, gs, seed=seed, learning_rate=2e-3, steps_per_launch=16, train_launches=3, guided_bounces=1)
     spp = 0
     while time.perf_counter() - t0 < budget:
         render_guided(dev, gs, guide, cam, st, guided_film, spp, spl)
