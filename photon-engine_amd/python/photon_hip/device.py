@@ -95,9 +95,11 @@ class GpuScene:
         return rc
 
     def close(self):
-        if getattr(self, "h", None):
+        # A scene outliving its device (interpreter shutdown destroys objects in any order) is left to the
+        # process's end: its handle died with the device.
+        if getattr(self, "h", None) and getattr(self.dev, "h", None):
             self._lib.pe_scene_release(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
@@ -143,9 +145,9 @@ class GpuFilm:
         self.h = h
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.dev, "h", None):  # as GpuScene.close
             self._lib.pe_film_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()

@@ -64,9 +64,10 @@ class Guide:
         return {"steps": s.value, "last_loss": l.value, "records": r.value, "inferred": q.value}
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        dev = getattr(self, "dev", None)
+        if getattr(self, "h", None) and (dev is None or getattr(dev, "h", None)):  # as GpuScene.close
             self._lib.pe_guide_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()

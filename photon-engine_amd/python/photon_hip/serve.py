@@ -111,9 +111,10 @@ class Server:
         self._sizes.pop(jid, None)
 
     def close(self) -> None:
-        if getattr(self, "h", None):
+        dev = getattr(self, "dev", None)
+        if getattr(self, "h", None) and (dev is None or getattr(dev, "h", None)):  # as GpuScene.close
             self._lib.pe_server_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         self.close()
