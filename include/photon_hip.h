@@ -80,6 +80,7 @@ int pe_device_synchronize(pe_device* dev);
 #define PE_MAT_DISNEY 0      /* materials/disney.h DisneyMaterial */
 #define PE_MAT_LAMBERT 1     /* materials/lambertian.h Lambertian */
 #define PE_MAT_DIELECTRIC 2  /* materials/dielectric.h Dielectric */
+#define PE_MAT_MIRROR 3      /* materials/mirror.h Mirror(reflectance = base_color); emits nothing */
 
 typedef struct pe_material_desc {
     int32_t type;

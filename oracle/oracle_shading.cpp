@@ -986,10 +986,10 @@ bool subsurfaceCoeffs(const pe_material_desc& m, const ShadingPoint& sp, V3& sig
 
 bool materialIsDelta(const pe_material_desc& m) {
     if (m.type == PE_MAT_DIELECTRIC) return m.thin_walled != 0 || m.roughness <= 1e-3f;  // dielectric.h:37
-    return false;
+    return m.type == PE_MAT_MIRROR;  // mirror.h:17
 }
 
-V3 materialEmitted(const pe_material_desc& m) { return c3(m.emission); }
+V3 materialEmitted(const pe_material_desc& m) { return m.type == PE_MAT_MIRROR ? V3(0.0f) : c3(m.emission); }
 
 V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, float& pdf) {
     if (m.type == PE_MAT_LAMBERT) {  // lambertian.cpp:38-60
@@ -1001,6 +1001,10 @@ V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, floa
         }
         pdf = cosineHemispherePdf(ci);
         return c3(m.base_color) * INV_PI;
+    }
+    if (m.type == PE_MAT_MIRROR) {  // mirror.cpp:36-44 (evalPdf = eval + pdf, material.h)
+        pdf = 0.0f;
+        return V3(0.0f);
     }
     if (m.type == PE_MAT_DIELECTRIC) {  // dielectric.cpp:262-292
         const Glass g(m);
@@ -1033,6 +1037,19 @@ bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float 
         if (wiL.z <= 0.0f) return false;
         out.f = c3(m.base_color) * INV_PI;
         out.pdf = cosineHemispherePdf(wiL.z);
+        return true;
+    }
+    if (m.type == PE_MAT_MIRROR) {  // material.h:91-96 -> mirror.cpp:7-33 (uc and u unused)
+        out.delta = true;
+        out.subsurface = false;
+        out.passThrough = false;
+        const Frame f(sp.normal);
+        const V3 woL = f.toLocal(wo);
+        if (woL.z <= 0.0f) return false;
+        const V3 wiL(-woL.x, -woL.y, woL.z);
+        out.wi = normalized(f.toWorld(wiL));
+        out.pdf = 1.0f;
+        out.f = c3(m.base_color) / wiL.z;
         return true;
     }
     if (m.type == PE_MAT_DIELECTRIC) return glassSampleFull(Glass(m), wo, sp, uc, u0, u1, out);

@@ -21,6 +21,7 @@
 #include "materials/fresnel.h"
 #include "materials/disney.h"
 #include "materials/lambertian.h"
+#include "materials/mirror.h"
 #include "materials/microfacet.h"
 #include "samplers/sobol_sampler.h"
 
@@ -42,6 +43,7 @@ Vec3f v3(const float* v) { return Vec3f(v[0], v[1], v[2]); }
 
 std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
     if (d.type == PE_MAT_LAMBERT) return std::make_shared<Lambertian>(c3(d.base_color), c3(d.emission));
+    if (d.type == PE_MAT_MIRROR) return std::make_shared<Mirror>(c3(d.base_color));
     if (d.type == PE_MAT_DIELECTRIC) {
         auto m = std::make_shared<Dielectric>(d.ior, c3(d.base_color), d.roughness);
         if (d.abbe > 0.0f) m->setAbbe(d.abbe);

@@ -25,7 +25,7 @@ struct EnergyTables {
 
 // A material with its parameters clamped as the reference's setters clamp them and every quantity that depends
 // on the material alone precomputed on the host (same operations the reference performs per shading call).
-enum : int32_t { kMatDisney = 0, kMatLambert = 1, kMatDielectric = 2 };
+enum : int32_t { kMatDisney = 0, kMatLambert = 1, kMatDielectric = 2, kMatMirror = 3 };
 struct alignas(16) GpuMaterial {
     int32_t type;
     uint32_t flags;  // kMatFlag*

@@ -559,6 +559,10 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
         pdf = ci * kInvPi;
         return m.base_color * kInvPi;
     }
+    if (m.type == kMatMirror) {  // mirror.cpp:36-44: a Dirac delta has no finite value
+        pdf = 0.0f;
+        return splat(0.0f);
+    }
     if (kTransport && m.type == kMatDielectric) {  // dielectric.cpp:262-292
         if (m.flags & kMatFlagDelta) {  // smooth or thin: no finite BSDF value
             pdf = 0.0f;
@@ -647,6 +651,17 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
         if (wil.z <= 0.0f) return false;
         out.f = m.base_color * kInvPi;
         out.pdf = wil.z * kInvPi;
+        return true;
+    }
+    if (m.type == kMatMirror) {  // mirror.cpp:7-33: wi = the reflection of wo, f = R / cos, pdf = 1
+        out.delta = true;
+        const Frame f = frame_from_normal(si.normal);
+        const V3 wol = to_local(f, wo);
+        if (wol.z <= 0.0f) return false;
+        const V3 wil = mk(-wol.x, -wol.y, wol.z);
+        out.wi = normalized(to_world(f, wil));
+        out.pdf = 1.0f;
+        out.f = m.base_color / wil.z;
         return true;
     }
     if (kTransport && m.type == kMatDielectric) {

@@ -37,6 +37,13 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
         g.base_color = c3(d.base_color);
         return PE_OK;
     }
+    if (d.type == PE_MAT_MIRROR) {  // Mirror(reflectance) (mirror.h:14): a delta reflection; Material::emitted is black
+        g.type = kMatMirror;
+        g.base_color = c3(d.base_color);
+        g.emission = mk(0.0f, 0.0f, 0.0f);
+        g.flags = kMatFlagDelta;
+        return PE_OK;
+    }
     if (d.type == PE_MAT_DIELECTRIC) {
         // Dielectric(ior, tint, roughness) + setAbbe / setAbsorption / setThinWalled (dielectric.h:18-78,
         // dielectric.cpp:123-144). The interior medium only absorbs: the description carries no scattering.

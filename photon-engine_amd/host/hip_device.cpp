@@ -24,6 +24,7 @@
 #include "materials/dielectric.h"
 #include "materials/disney.h"
 #include "materials/lambertian.h"
+#include "materials/mirror.h"
 
 #include <algorithm>
 #include <unordered_map>
@@ -116,6 +117,11 @@ bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
     if (auto* l = dynamic_cast<const Lambertian*>(m)) {
         o.type = PE_MAT_LAMBERT;
         putc(o.base_color, l->albedo());
+        return true;
+    }
+    if (auto* r = dynamic_cast<const Mirror*>(m)) {
+        o.type = PE_MAT_MIRROR;
+        putc(o.base_color, r->reflectance());
         return true;
     }
     if (auto* g = dynamic_cast<const Dielectric*>(m)) {

@@ -24,7 +24,7 @@ PE_ERR_CANCELLED = -6
 PE_ERR_TIMEOUT = -7
 PE_JOB_QUEUED, PE_JOB_ADMITTED, PE_JOB_DONE, PE_JOB_CANCELLED, PE_JOB_FAILED = 0, 1, 2, 3, 4
 
-PE_MAT_DISNEY, PE_MAT_LAMBERT, PE_MAT_DIELECTRIC = 0, 1, 2
+PE_MAT_DISNEY, PE_MAT_LAMBERT, PE_MAT_DIELECTRIC, PE_MAT_MIRROR = 0, 1, 2, 3
 PE_LIGHT_AREA, PE_LIGHT_TRIANGLE = 1, 2
 PE_SHAPE_CAMERA_INVISIBLE = 1
 PE_FILTER_BOX, PE_FILTER_TRIANGLE, PE_FILTER_GAUSSIAN, PE_FILTER_BLACKMAN_HARRIS = 0, 1, 2, 3

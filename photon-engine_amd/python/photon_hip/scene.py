@@ -70,6 +70,13 @@ def lambert(albedo, emission=(0.0, 0.0, 0.0)) -> capi.MaterialDesc:
     return m
 
 
+def mirror(color=(0.95, 0.95, 0.95)) -> capi.MaterialDesc:
+    """Mirror(reflectance) (mirror.h:14; the project format's default colour, project_io.cpp:167-168)."""
+    m = disney(color)
+    m.type = capi.PE_MAT_MIRROR
+    return m
+
+
 @dataclass
 class Mesh:
     positions: np.ndarray                 # (V, 3) float32
