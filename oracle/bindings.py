@@ -394,6 +394,9 @@ _HIPDEV_SIGS = {
     "hipdev_session_render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                                  C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I, C.c_char_p,
                                                  C.c_int]),
+    "hipdev_session_interactive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                             C.c_int, _F, _U, C.POINTER(C.c_double), C.c_int, C.c_int,
+                                             C.POINTER(C.c_double), _I, _I, C.c_char_p, C.c_int]),
 }
 
 
@@ -459,6 +462,24 @@ class HipDeviceSession:
             view.height, view.max_bounces, spp, max_batch, threshold, min_spp, _fp(mean), capi.ptr(counts, C.c_uint32),
             C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
+
+    def interactive(self, view, spp, cancel_spp, cancel_after_ms):
+        """Interactive one-sample passes, then a cancelled Final batch -> dict(rc, mean, counts, pass_seconds,
+        cancel_latency, cancel_completed, cancel_samples, error)."""
+        f3 = C.c_float * 3
+        mean = np.zeros((view.height, view.width, 3), np.float32)
+        counts = np.zeros((view.height, view.width), np.uint32)
+        passes = np.zeros(spp, np.float64)
+        lat, done, samples = C.c_double(), C.c_int32(), C.c_int32()
+        err = C.create_string_buffer(512)
+        rc = self._lib.hipdev_session_interactive(
+            self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
+            view.height, view.max_bounces, spp, _fp(mean), capi.ptr(counts, C.c_uint32),
+            passes.ctypes.data_as(C.POINTER(C.c_double)), cancel_spp, cancel_after_ms, C.byref(lat), C.byref(done),
+            C.byref(samples), err, 512)
+        return dict(rc=rc, mean=mean, counts=counts, pass_seconds=passes, cancel_latency=lat.value,
+                    cancel_completed=done.value, cancel_samples=samples.value,
+                    error=err.value.decode(errors="replace"))
 
 
 def spectral(u, nd=1.5, abbe=30.0, wavelengths=()):

@@ -8,9 +8,12 @@
 #include "camera/perspective_camera.h"
 #include "hip_device.h"
 
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 
 using namespace photon;
 
@@ -217,6 +220,95 @@ HIPDEV_API int hipdev_enumerate(char* out, int outlen) {
     for (const DeviceInfo& d : enumerateHipDevices())
         s += d.id + "|" + d.name + "|" + (d.usable ? "1" : "0") + "|" + d.note + "\n";
     copy_error(s, out, outlen);
+    return 0;
+}
+
+// RenderSession(Interactive, targetSpp = spp) on a HipDevice: one-sample passes (render_session.cpp:69-71), the
+// wall time of each advance() in passSeconds[spp], the resolved mean and counts. Then the cancel path: a Final
+// session of cancelSpp samples whose second batch (sized from the first one's speed, batchSeconds 1e3) is cancelled
+// by another thread cancelAfterMs into it; cancelLatency = seconds from the flag's store to advance()'s return,
+// cancelCompleted = that batch's SampleBatchResult::completed, cancelSamples = the session's samples after it.
+HIPDEV_API int hipdev_session_interactive(void* h, const float* camPos, const float* camTarget, const float* camUp,
+                                          float fov, float aspect, int width, int height, int maxBounces, int spp,
+                                          float* mean, uint32_t* counts, double* passSeconds, int cancelSpp,
+                                          int cancelAfterMs, double* cancelLatency, int* cancelCompleted,
+                                          int* cancelSamples, char* err, int errlen) {
+    using Clock = std::chrono::steady_clock;
+    std::string e;
+    std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
+    if (!dev) {
+        copy_error(e, err, errlen);
+        return 2;
+    }
+    dev->setScene(ref_scene_shared(h));
+    if (!dev->hasScene()) {
+        copy_error(dev->lastError(), err, errlen);
+        return 1;
+    }
+    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
+                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
+                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = spp;
+    s.maxBounces = maxBounces;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    {
+        SessionConfig cfg;
+        cfg.mode = RenderMode::Interactive;
+        cfg.targetSpp = spp;
+        RenderSession session(*dev);
+        session.begin(cam, s, cfg);
+        for (int i = 0; i < spp; ++i) {
+            const auto t0 = Clock::now();
+            if (!session.advance(nullptr)) {
+                copy_error("interactive pass failed: " + dev->lastError(), err, errlen);
+                return 1;
+            }
+            passSeconds[i] = std::chrono::duration<double>(Clock::now() - t0).count();
+        }
+        if (!session.finished()) {
+            copy_error("interactive session not finished after targetSpp passes", err, errlen);
+            return 1;
+        }
+        const Film& film = session.film();
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) {
+                const size_t k = static_cast<size_t>(y) * static_cast<size_t>(width) + static_cast<size_t>(x);
+                const Color3f c = film.resolvedPixel(x, y);
+                mean[3 * k] = c.r;
+                mean[3 * k + 1] = c.g;
+                mean[3 * k + 2] = c.b;
+                counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+            }
+    }
+    SessionConfig cfg;
+    cfg.mode = RenderMode::Final;
+    cfg.targetSpp = cancelSpp;
+    cfg.maxBatch = cancelSpp;
+    cfg.batchSeconds = 1e3;
+    s.samplesPerPixel = cancelSpp;
+    RenderSession session(*dev);
+    session.begin(cam, s, cfg);
+    if (!session.advance(nullptr)) {  // the first batch is one sample (nextBatch, render_session.cpp:69)
+        copy_error("first batch failed: " + dev->lastError(), err, errlen);
+        return 1;
+    }
+    std::atomic<bool> cancel{false};
+    Clock::time_point stored;
+    std::thread canceller([&] {
+        std::this_thread::sleep_for(std::chrono::milliseconds(cancelAfterMs));
+        stored = Clock::now();
+        cancel.store(true, std::memory_order_relaxed);
+    });
+    const bool ok = session.advance(&cancel);
+    const auto back = Clock::now();
+    canceller.join();
+    *cancelLatency = std::chrono::duration<double>(back - stored).count();
+    *cancelCompleted = ok ? 1 : 0;
+    *cancelSamples = session.samples();
     return 0;
 }
 

@@ -30,7 +30,8 @@ def relmse(x, ref):
 def test_adapter_library_loads_and_exports():
     _harness_or_skip()
     lib = bindings.hip_device_harness()
-    for name in ("hipdev_session_render", "hipdev_enumerate", "ref_scene_build", "ref_scene_free"):
+    for name in ("hipdev_session_render", "hipdev_session_interactive", "hipdev_enumerate", "ref_scene_build",
+                 "ref_scene_free"):
         assert hasattr(lib.lib, name), name
 
 
@@ -79,3 +80,26 @@ def test_hip_device_refuses_what_it_does_not_implement(dev):
     rc, _, _, _, err = bindings.HipDeviceSession(sc).render(view, 1, ao_strength=0.5)
     assert rc == 1
     assert "not implemented" in err, err
+
+
+@pytest.mark.gpu
+def test_interactive_passes_and_cancel_through_hip_device(dev):
+    """RenderSession's Interactive mode (one-sample passes, render_session.cpp:69-71) through HipDevice gives the
+    image of a direct render of the same samples, pass by pass at viewport latency; a cancelled Final batch
+    returns within one chunk of the flag (HipDevice polls between chunks of 8 samples), reports the batch
+    incomplete, and the session keeps only its completed samples."""
+    _harness_or_skip()
+    sc, view = configs.product_sphere(width=320, height=180, lon=128, lat=96, env_width=512, env_height=256)
+    spp = 8
+    r = bindings.HipDeviceSession(sc).interactive(view, spp, cancel_spp=100000, cancel_after_ms=40)
+    assert r["rc"] == 0, r["error"]
+    assert (r["counts"] == spp).all()
+    direct, c = _direct_mean(dev, sc, view, spp, spl=1)
+    np.testing.assert_allclose(r["mean"], direct, rtol=2e-5, atol=1e-6)
+    # The first pass carries the scene upload's first launches; the steady passes are viewport-fast.
+    assert np.median(r["pass_seconds"][1:]) < 0.05, r["pass_seconds"]
+    assert r["cancel_completed"] == 0
+    assert r["cancel_samples"] == 1
+    assert r["cancel_latency"] < 0.25, r["cancel_latency"]
+    print(f"interactive pass median {np.median(r['pass_seconds'][1:]) * 1e3:.2f} ms, cancel latency "
+          f"{r['cancel_latency'] * 1e3:.2f} ms")
