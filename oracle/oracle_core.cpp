@@ -313,6 +313,12 @@ bool hitTriangleLean(const TriAccel& T, const Ray& r, float& t, float& u, float&
 
 void buildBvh(Scene& s) {
     const size_t n = s.tris.size();
+    s.ordered.clear();
+    if (n == 0) {  // an empty scene: no nodes, every ray misses (bvh.cpp build of no primitives)
+        s.nodes.clear();
+        s.accel.clear();
+        return;
+    }
     std::vector<BuildPrim> prims;
     prims.reserve(n);
     for (size_t i = 0; i < n; ++i) {

@@ -73,10 +73,6 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             sc.max_emitter_ref = std::max(sc.max_emitter_ref, static_cast<int>(last));
         }
     }
-    if (nt == 0) {
-        set_error("scene: no triangles");
-        return PE_ERR_INVALID;
-    }
     if (nt > kMaxTraceTriangles) {
         set_error("scene: more than 2^27 triangles (the traversal's leaf items address 2^27)");
         return PE_ERR_UNSUPPORTED;
@@ -150,7 +146,25 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     BvhBuildOptions opt;
     const BvhBuildResult bvh = build_bvh(lo.data(), hi.data(), nt, opt);
     int depth4 = 0;
-    const std::vector<BvhNode4> nodes4 = collapse_bvh4(bvh.nodes, depth4);
+    std::vector<BvhNode4> nodes4 = collapse_bvh4(bvh.nodes, depth4);
+    if (nt == 0) {
+        // A scene of no triangles (an environment alone): one root whose four slots are empty boxes, so every
+        // ray misses; the shading buffers get one zero record each so that no device pointer is null.
+        BvhNode4 root{};
+        root.lox = root.loy = root.loz = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        root.hix = root.hiy = root.hiz = make_float4(-1.0f, -1.0f, -1.0f, -1.0f);
+        root.item = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+        nodes4.assign(1, root);
+        depth4 = 1;
+        shade.resize(1);
+        shade[0].material = 0;
+        shade[0].emitter = -1;
+        if (pos.empty()) {
+            pos.resize(1);
+            nrm.resize(1);
+            uv.resize(1);
+        }
+    }
 
     // One zero record of tail padding: traversal reads every triangle as a 64-byte item.
     std::vector<TriAccel> acc(nt + 1);

@@ -195,3 +195,67 @@ def test_gpu_converged_image_matches_reference_renderer(dev, name):
     assert (cnt_r == spp).all()
     e = relmse(gpu, mean_r)
     assert e <= 1e-3, e
+
+
+def _env_only():
+    """An environment and no geometry: BVH::build of no primitives, every camera ray sees the environment."""
+    from photon_hip.scene import Scene
+    sph, view = configs.product_sphere(width=9, height=5, lon=32, lat=16, env_width=128, env_height=64)
+    sc = Scene()
+    sc.environment = sph.environment
+    return sc, view
+
+
+def test_oracle_empty_scene_equals_reference():
+    if bindings.reference(strict=True) is None:
+        pytest.skip("oracle/_ref not built here (reference tree absent)")
+    sc, view = _env_only()
+    py, px = np.divmod(np.arange(view.width * view.height), view.width)
+    for s in (0, 3):
+        st = settings(max_bounces=view.max_bounces, filter_type=capi.PE_FILTER_BOX, filter_width=1.0, seed=12345)
+        Lo, _ = bindings.OracleScene(sc).trace_samples(_cam(view), st, view.width, view.height, px, py,
+                                                       np.full(px.size, s))
+        Lr = bindings.RefScene(sc, strict=True).trace_samples(view, px, py, np.full(px.size, s))
+        np.testing.assert_array_equal(Lo, Lr)
+        assert Lo.min() > 0
+
+
+@pytest.mark.gpu
+def test_gpu_edge_cases(dev):
+    """An environment with no geometry, a 1x1 film, a zero-sample call and depth 0 (emission only), each against
+    the oracle."""
+    sc, view = _env_only()
+    gs = dev.upload(sc)
+    film = dev.film(view.width, view.height)
+    res = render(dev, gs, _cam(view), settings(max_bounces=view.max_bounces, seed=12345), film, 2, 1)
+    assert res.completed == 1 and res.shadow_rays == 0
+    s, c, _, _ = film.download()
+    py, px = np.divmod(np.arange(view.width * view.height), view.width)
+    Lo, _ = bindings.OracleScene(sc).trace_samples(_cam(view), settings(max_bounces=view.max_bounces, seed=12345),
+                                                   view.width, view.height, px, py, np.full(px.size, 2))
+    np.testing.assert_allclose(s.reshape(-1, 3), Lo, rtol=1e-4)
+    assert (c == 1).all()
+
+    box, bview = configs.box_icosphere(level=2, width=1, height=1)
+    gb = dev.upload(box)
+    one = dev.film(1, 1)
+    res = render(dev, gb, _cam(bview), settings(max_bounces=bview.max_bounces, seed=12345, samples_per_launch=16),
+                 one, 0, 16)
+    assert res.completed == 1 and res.samples_traced == 16
+    (so, co, _), _, _ = bindings.OracleScene(box).render(_cam(bview), settings(max_bounces=bview.max_bounces), 1, 1, 0,
+                                                          16)
+    s1, c1, _, _ = one.download()
+    assert (c1 == 16).all()
+    np.testing.assert_allclose(s1, so, rtol=1e-3, atol=1e-5)
+    res = render(dev, gb, _cam(bview), settings(), one, 16, 0)    # no samples: a no-op
+    assert res.samples_traced == 0
+    assert (one.download()[1] == 16).all()
+
+    small, sview = configs.box_icosphere(level=2, width=12, height=10)
+    gsm = dev.upload(small)
+    f0 = dev.film(12, 10)
+    render(dev, gsm, _cam(sview), settings(max_bounces=0, seed=12345), f0, 0, 1)
+    py, px = np.divmod(np.arange(120), 12)
+    L0, _ = bindings.OracleScene(small).trace_samples(_cam(sview), settings(max_bounces=0, seed=12345), 12, 10, px, py,
+                                                      np.zeros(120, int))
+    np.testing.assert_allclose(f0.download()[0].reshape(-1, 3), L0, rtol=1e-4, atol=1e-6)
