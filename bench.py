@@ -3,8 +3,9 @@
 displaced sphere in 16 Disney parts under a 2048 x 1024 sky with a 0.5-degree sun at 1e4 x; 1920 x 1080,
 256 spp, max depth 12), rendered by the gfx950 wavefront path tracer through the C-ABI.
 
-A step is one wavefront launch of --spl samples for every pixel of the 1080p frame (default 32, the library's
-own launch size at 1080p, so the default 8 timed steps are exactly the workload's 256 spp). Prints one JSON line:
+A step is one wavefront launch of --spl samples for every pixel of the 1080p frame (default 128: 265 M paths,
+about 60 GB of path state, in flight at once; deeper launches amortise the short tail bounces), so the default 2
+timed steps are exactly the workload's 256 spp. Prints one JSON line:
   value            Mrays/s: extension + shadow rays the kernels traced, over the timed wall time
   spp_per_s        1080p frames' worth of samples per second;  msamples_per_s: the reference's "M samples/s"
   roofline         closest-hit kernel: algorithmic bytes per launch (ray in/out + node and triangle bytes of the
@@ -241,9 +242,9 @@ def guiding_run(dev, budget_spp: int, ref_spp: int, spl: int = 16, seed: int = 1
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--spl", type=int, default=32, help="samples per pixel per step (one wavefront launch)")
+    ap.add_argument("--spl", type=int, default=128, help="samples per pixel per step (one wavefront launch)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target duration of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="reduced scene (smoke runs only; not the workload)")
