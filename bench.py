@@ -10,6 +10,7 @@ timed steps are exactly the workload's 256 spp. Prints one JSON line:
   spp_per_s        1080p frames' worth of samples per second;  msamples_per_s: the reference's "M samples/s"
   roofline         closest-hit kernel: algorithmic bytes per launch (ray in/out + node and triangle bytes of the
                    measured visits) over its HIP-event time, against 8 TB/s HBM3E
+  roofline_shade   the dominant kernel (shade): rocprof's calibrated HBM bytes per launch over its HIP-event time
   cpu_baseline     the reference's own CPU renderer (oracle/_ref, its own compiler flags) on this host's cores,
                    on a bounded sample of the same workload
 """
@@ -371,6 +372,18 @@ def main() -> int:
                      "launches": isect["launches"],
                      "avg_launch_ms": round(isect["ms"] / max(1, isect["launches"]), 4)},
     }
+    # The dominant kernel, shade, has no single algorithmic byte count (its gathers depend on the branch each
+    # path takes); its line gives the HBM traffic rocprof measured per launch over the HIP-event time per launch.
+    shade = kt.get("shade", {"launches": 0, "ms": 0.0})
+    sfile = ROOT / "profiles" / "shade_traffic.json"
+    if shade["launches"] > 0 and shade["ms"] > 0 and sfile.exists():
+        sbytes = json.loads(sfile.read_text()).get("bytes_per_launch")
+        shade_ms = shade["ms"] / shade["launches"]
+        gbs = sbytes / (shade_ms * 1e-3) / 1e9
+        result["roofline_shade"] = {"kernel": "shade", "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": sbytes,
+                                    "basis": "measured HBM bytes per launch (rocprof, calibrated) / HIP-event time",
+                                    "launches": shade["launches"], "avg_launch_ms": round(shade_ms, 4)}
     if args.serving_seconds > 0:
         log("serving stream ...")
         result["serving"] = serving_run(dev, (gs, view), args.serving_seconds, args.serving_rate)
