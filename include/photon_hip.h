@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define PE_ABI_VERSION 1
+#define PE_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------------------------------------- */
 #define PE_OK 0
@@ -197,8 +197,9 @@ typedef struct pe_settings {
     float filter_width;
     float clamp_indirect;       /* 0 = off */
     uint64_t seed;              /* Sobol seed; the reference renderer hard-codes 12345 */
-    int32_t samples_per_launch; /* wavefront batch: samples of every pixel kept in flight together; 0 = sized
-                                   by the library (about 2^26 paths, within a quarter of free memory) */
+    int32_t samples_per_launch; /* window: samples of every pixel whose results the film adds together, in
+                                   sample order, at the window's end (16 B of device memory per sample); 0 = sized
+                                   by the library (up to 2^28 samples within an eighth of free memory) */
     int32_t count_visits;       /* 1: also count BVH node / triangle visits (measurement runs; slower) */
     /* Adaptive sampling (RenderSettings::adaptiveSampling / noiseThreshold / minAdaptiveSamples,
        engine/render_settings.h:30-32; renderer.cpp:438-505): a batch skips every pixel whose 3x3 neighbourhood
@@ -207,7 +208,9 @@ typedef struct pe_settings {
     int32_t adaptive;           /* 0 = off */
     float noise_threshold;      /* relative standard error target; clamped to >= 1e-4 (reference default 0.02) */
     int32_t min_adaptive_samples; /* at least 2 (reference default 16) */
-    int32_t reserved[3];
+    int32_t paths_in_flight;    /* path pool: paths traced together, a finished path's id taking the window's next
+                                   sample at once; 0 = 2^24 (never more than the window's samples) */
+    int32_t reserved[2];
 } pe_settings;
 
 /* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray and traversal counters. */
@@ -220,6 +223,8 @@ typedef struct pe_batch_result {
     uint64_t tri_visits;
     uint64_t shadow_node_visits; /* any-hit traversal, when settings.count_visits */
     uint64_t shadow_tri_visits;
+    uint32_t max_stack_depth;    /* deepest traversal stack of any ray, when settings.count_visits */
+    uint32_t stack_overflows;    /* traversals whose stack overflowed (then the call fails with PE_ERR_INVALID) */
 } pe_batch_result;
 
 /* Per-kernel device time of the wavefront stages, from HIP events recorded on the device's stream around
@@ -244,8 +249,8 @@ int pe_camera_perspective(const float position[3], const float target[3], const 
                           float aspect, pe_camera* out);
 
 /* RenderDevice::render: adds samples [first_sample, first_sample + sample_count) of every pixel into the
- * film. cancel (may be NULL) is polled between wavefront launches; a set flag ends the call with
- * completed = 0 and a valid partial film. */
+ * film. cancel (may be NULL) is polled between iterations of the wavefront (a few milliseconds apart); a set
+ * flag ends the call with completed = 0, PE_ERR_CANCELLED and a valid partial film holding whole windows. */
 int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe_settings* settings, pe_film* film,
               int first_sample, int sample_count, const volatile int32_t* cancel, pe_batch_result* result);
 
@@ -317,6 +322,11 @@ int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, c
 int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary);
 /* Any hit (shadow rays): occluded[i] = 1 when something lies within [tmin, tmax]. */
 int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded);
+/* Traversal diagnostics of the device's last pe_intersect_batch / pe_occluded_batch: the deepest stack any of
+ * its rays needed (the first 16 entries live in LDS, deeper ones in a global spill column) and the number of
+ * traversals whose stack overflowed (a batch with any fails with PE_ERR_INVALID). No counterpart in the
+ * reference (its traversal stack is a growing std::vector, bvh.cpp:287-296). */
+int pe_trace_diag(pe_device* dev, uint32_t* max_stack_depth, uint64_t* overflows);
 
 /* ---- learned path guiding ------------------------------------------------------------------------------
  * The reference guides with an SD-tree (GuidingField, integrators/path_guiding.cpp:11-388) trained in the first

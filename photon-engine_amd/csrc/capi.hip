@@ -106,27 +106,32 @@ struct BatchAny : BatchRays {
 
 __global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode4* __restrict__ nodes,
                                                                  const TriAccel* __restrict__ tris, BatchClosest job,
-                                                                 uint32_t n, uint32_t* __restrict__ spill) {
+                                                                 uint32_t n, uint32_t* __restrict__ spill,
+                                                                 unsigned long long* __restrict__ diag) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, false>(job, QueueSegs{nullptr, n, n, 1u}, nodes, tris, st, nv, tv);
+    trace_queue<false, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv);
+    report_stack(diag, st, true);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* __restrict__ nodes,
                                                                 const TriAccel* __restrict__ tris, BatchAny job,
-                                                                uint32_t n, uint32_t* __restrict__ spill) {
+                                                                uint32_t n, uint32_t* __restrict__ spill,
+                                                                unsigned long long* __restrict__ diag) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<true, false>(job, QueueSegs{nullptr, n, n, 1u}, nodes, tris, st, nv, tv);
+    trace_queue<true, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv);
+    report_stack(diag, st, true);
 }
 
 // Persistent grid for 256-lane blocks and the spill buffer sized for it.
 int ensure_trace_resources(pe_device* dev, int& blocks) {
     blocks = dev->max_grid_blocks;
+    if (!dev->diag.get() && !dev->diag.alloc(kDiagBytes)) return PE_ERR_OOM;
     const uint32_t lanes = static_cast<uint32_t>(blocks) * kTraceBlock;
     if (dev->spill_lanes < lanes) {
         if (!dev->spill.alloc(static_cast<size_t>(lanes) * (kStackDepth - kLdsDepth) * sizeof(uint32_t))) return PE_ERR_OOM;
@@ -137,6 +142,20 @@ int ensure_trace_resources(pe_device* dev, int& blocks) {
 
 size_t trace_spill_bytes(int grid_blocks) {
     return static_cast<size_t>(grid_blocks) * kTraceBlock * (kStackDepth - kLdsDepth) * sizeof(uint32_t);
+}
+
+// Copies a batch's traversal diagnostics to the device (after the stream's other copies) and fails the batch
+// when a stack overflowed.
+int finish_batch(pe_device* dev, const char* what) {
+    static_assert(sizeof(dev->diag_host) == kDiagBytes, "diagnostics copy size");
+    PE_HIP_CHECK(hipMemcpyAsync(dev->diag_host, dev->diag.get(), kDiagBytes, hipMemcpyDeviceToHost, dev->stream));
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    if (dev->diag_host[1] != 0) {
+        set_error(std::string(what) + ": " + std::to_string(dev->diag_host[1]) + " traversals overflowed the " +
+                  std::to_string(kStackDepth) + "-entry stack");
+        return PE_ERR_INVALID;
+    }
+    return PE_OK;
 }
 
 int grid_for(pe_device* dev, size_t n) {
@@ -360,6 +379,16 @@ int pe_scene_stats(const pe_scene* scene, uint64_t* triangles, uint64_t* bvh_nod
     return PE_OK;
 }
 
+int pe_trace_diag(pe_device* dev, uint32_t* max_stack_depth, uint64_t* overflows) {
+    if (!dev) {
+        set_error("pe_trace_diag: NULL device");
+        return PE_ERR_INVALID;
+    }
+    if (max_stack_depth) *max_stack_depth = static_cast<uint32_t>(dev->diag_host[0]);
+    if (overflows) *overflows = dev->diag_host[1];
+    return PE_OK;
+}
+
 int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
                    const uint32_t* index, const uint32_t* dim, uint32_t* out) {
     PE_GUARD_BEGIN
@@ -406,13 +435,14 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
     job.shade = scene->tri_shade.as<TriShade>();
     job.hits = dhits.as<int4>();
     job.bary = bary ? dbary.as<float2>() : nullptr;
+    PE_HIP_CHECK(hipMemsetAsync(dev->diag.get(), 0, kDiagBytes, dev->stream));
     hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
-                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>(),
+                       dev->diag.as<unsigned long long>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
     if (bary) PE_HIP_CHECK(hipMemcpyAsync(bary, dbary.get(), n * 8, hipMemcpyDeviceToHost, dev->stream));
-    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
-    return PE_OK;
+    return finish_batch(dev, "pe_intersect_batch");
     PE_GUARD_END
 }
 
@@ -437,12 +467,13 @@ int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* ra
     BatchAny job{};
     job.r = drays.as<float4>();
     job.occ = docc.as<uint8_t>();
+    PE_HIP_CHECK(hipMemsetAsync(dev->diag.get(), 0, kDiagBytes, dev->stream));
     hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
-                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>());
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>(),
+                       dev->diag.as<unsigned long long>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(occluded, docc.get(), n, hipMemcpyDeviceToHost, dev->stream));
-    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
-    return PE_OK;
+    return finish_batch(dev, "pe_occluded_batch");
     PE_GUARD_END
 }
 

@@ -36,57 +36,95 @@ namespace pe {
 
 namespace {
 
+#ifndef PE_SHADE_PLAIN_ATTR
+#define PE_SHADE_PLAIN_ATTR
+#endif
 constexpr int kBlock = 256;
 constexpr int kMaxShadowSamples = 4;
+constexpr int kStatSlots = 16;  // 64-bit device counters of a render (PathBuffers::stats)
+constexpr uint64_t kAutoWindowSamples = uint64_t{1} << 28;  // default result buffer: 4 GB
+constexpr uint64_t kMaxWindowSamples = (uint64_t{1} << 31) - 64;  // window sample indices are 31-bit
+constexpr uint64_t kDefaultPoolIds = uint64_t{1} << 24;  // default path ids in flight
+constexpr uint64_t kMaxPoolIds = uint64_t{1} << 30;
+constexpr int kChunkIterations = 2;  // iterations between two reads of the remaining-work counter
 
 struct PathBuffers {
     float4* ro[2];    // ray queues (ping-pong), per slot: origin xyz, w = path id (uint bits)
-    float4* rd[2];    //                                   direction xyz
+    float4* rd[2];    //                                   direction xyz, w = entry tag (kCamEntry | g, or the depth)
     float4* thr;      // throughput rgb, w = last BSDF pdf
     float4* L;        // radiance rgb
     float4* last_p;   // previous vertex point (MIS)
     float4* last_n;   // previous vertex normal (MIS)
-    uint4* ids;       // pixel, sample, Sobol pixel hash, packed bits (see pack_bits)
+    uint4* ids;       // sample index g in the window, sample, Sobol pixel hash, packed bits (see pack_bits)
     uint4* ext;       // transport state beyond the surface bounce (see the kExt* fields)
     float4* mix;      // guiding: the mixture of every queue slot (kGuideMixFloat4 float4 each)
-    float4* grec;     // guiding: two training records per path, 4 float4 each (see k_guide_collect)
+    float4* grec;     // guiding: two training records per path, 4 float4 each (see guide_collect)
     float4* pend;     // held-back NEE contribution per shadow sample, [pid * S + s]
-    float4* aov;      // AOVs of the camera vertex, two per path: albedo rgb | coverage, shading normal xyz | state
-                      // (-1 until the first segment has been resolved, then 0)
+    float4* res;      // per sample of the window: its radiance (L plus the held-back NEE), read by k_film
+    float4* aov_res;  // per sample of the window, two float4: albedo rgb | coverage, shading normal xyz | state
+                      // (-1 until the first segment has been resolved, then 0); null without AOV films
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
     float4* sq_d;     //               direction xyz, w = pend index pid * S + s (uint bits)
+    uint32_t* fl[2];  // free path ids (ping-pong): pid | kDeferred when the path still has to be finalised
     uint32_t* ctr;    // segment counters (kCtr* below)
-    uint32_t seg_cap; // slots per segment of the ray queues (S * seg_cap per shadow queue segment)
-    unsigned long long* stats;  // [0] extension rays, [1] shadow rays
+    uint32_t seg_cap; // slots per segment of the ray queues (S * seg_cap per shadow queue segment); segment k
+                      // owns the path ids [k * seg_cap, (k + 1) * seg_cap)
+    unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2..5] visits, [6] guide records,
+                                // [7] guide queries, [8] deepest traversal stack, [9] stack overflows
 };
 
-// Queue counters: the current, next and shadow queues have kQueueSegs segments each, counter of segment k of
-// queue q at ctr[(q * kQueueSegs + k) * kSegStride]; ctr[kCtrTotal] = paths in the current queue (read by the host).
-constexpr uint32_t kCtrCur = 0, kCtrNext = 1, kCtrShadow = 2;
-constexpr uint32_t kCtrTotal = 3 * kQueueSegs * kSegStride;
+// Path regeneration. The wavefront keeps a pool of path ids; segment k of every queue owns the ids
+// [k * seg_cap, (k + 1) * seg_cap), so a segment's queues never hold more entries than it has ids. An id is
+// either a path in flight (its ray is in the current queue) or free (in the segment's free list). Each
+// iteration:
+//   k_regen      finalises the free ids whose path ended with a shadow ray outstanding, then gives free ids the
+//                segment's next samples: their camera rays go to queue slots [base, base + cam), after the
+//                continuation rays [0, base)
+//   k_intersect  closest hit of every ray of the queue
+//   k_shade      every entry: continuation rays to the next queue, shadow rays to the shadow queue, ended
+//                paths to the next free list (finalised at once unless a shadow ray is outstanding)
+//   k_shadow     any hit of the shadow queue
+//   k_rotate     counters: next queue -> current, next free list -> current, samples handed out
+// Samples are dealt to the segments in 64-sample chunks (chunk c to segment c mod 64); a path's result goes to
+// its sample's slot of the window's result buffer, and k_film adds the results of every pixel in sample order,
+// so the film does not depend on which id or iteration rendered a sample.
+constexpr uint32_t kCamEntry = 0x80000000u;  // rd.w of a camera entry: kCamEntry | g
+constexpr uint32_t kDeferred = 0x80000000u;  // free-list entry of a path with a shadow ray outstanding
+constexpr uint32_t kPidMask = 0x7FFFFFFFu;
+constexpr uint32_t kCtrBase = 0, kCtrCur = 1, kCtrCam = 2, kCtrNext = 3, kCtrShadow = 4, kCtrFree = 5,
+                   kCtrFreeNext = 6, kCtrGen = 7, kCtrChunkIsect = 8, kCtrChunkShadow = 9, kCtrQueues = 10;
+constexpr uint32_t kCtrTotal = kCtrQueues * kQueueSegs * kSegStride;  // paths and samples still to go
 constexpr size_t kCtrBytes = (kCtrTotal + kSegStride) * sizeof(uint32_t);
 __host__ __device__ __forceinline__ uint32_t* seg_counter(uint32_t* ctr, uint32_t q, uint32_t k) {
     return ctr + (q * kQueueSegs + k) * kSegStride;
 }
-__device__ __forceinline__ QueueSegs ray_queue(const PathBuffers& pb) {
-    return QueueSegs{seg_counter(pb.ctr, kCtrCur, 0), 0u, pb.seg_cap, kQueueSegs};
+__device__ __forceinline__ QueueSegs seg_queue(const PathBuffers& pb, uint32_t which) {
+    return QueueSegs{seg_counter(pb.ctr, which, 0), 0u, pb.seg_cap, kQueueSegs};
+}
+// The current ray queue as k_intersect traces it (chunks handed out through their own counters).
+__device__ __forceinline__ QueueSegs trace_ray_queue(const PathBuffers& pb) {
+    return QueueSegs{seg_counter(pb.ctr, kCtrCur, 0), 0u, pb.seg_cap, kQueueSegs, seg_counter(pb.ctr, kCtrChunkIsect, 0)};
 }
 __device__ __forceinline__ QueueSegs shadow_queue(const PathBuffers& pb, int S) {
-    return QueueSegs{seg_counter(pb.ctr, kCtrShadow, 0), 0u, pb.seg_cap * static_cast<uint32_t>(S), kQueueSegs};
+    return QueueSegs{seg_counter(pb.ctr, kCtrShadow, 0), 0u, pb.seg_cap * static_cast<uint32_t>(S), kQueueSegs,
+                     seg_counter(pb.ctr, kCtrChunkShadow, 0)};
 }
-// Slot of camera path i in the first queue: 64-path chunks dealt to the segments in turn, so that every segment
-// holds paths from all over the image (segments keep their paths for the whole launch) while a chunk stays 64
-// neighbouring pixels.
-__host__ __device__ __forceinline__ uint32_t first_slot(uint32_t i, uint32_t seg_cap) {
-    const uint32_t q = i >> 6;
-    return (q % kQueueSegs) * seg_cap + (q / kQueueSegs) * 64u + (i & 63u);
-}
-// Slots per segment for n paths: the chunks dealt to one segment, rounded up to whole 256-slot shade rounds.
-__host__ __device__ __forceinline__ uint32_t seg_capacity(uint32_t n) {
+// Samples of a window of n that segment k renders: the 64-sample chunks c with c mod 64 = k, the last partial.
+__host__ __device__ __forceinline__ uint32_t seg_samples(uint32_t n, uint32_t k) {
     const uint32_t chunks = (n + 63u) / 64u;
-    const uint32_t per = (chunks + kQueueSegs - 1u) / kQueueSegs;
-    return (per * 64u + 255u) / 256u * 256u;
+    uint32_t cnt = chunks > k ? ((chunks - 1u - k) / kQueueSegs + 1u) * 64u : 0u;
+    if (chunks > 0u && (chunks - 1u) % kQueueSegs == k && (n & 63u) != 0u) cnt -= 64u - (n & 63u);
+    return cnt;
+}
+// The window sample of the j-th sample segment k hands out.
+__host__ __device__ __forceinline__ uint32_t seg_sample_index(uint32_t k, uint32_t j) {
+    return ((k + kQueueSegs * (j >> 6)) << 6) | (j & 63u);
+}
+// Slots per segment for a pool of n path ids, rounded up to whole 256-slot shade rounds.
+__host__ __device__ __forceinline__ uint32_t seg_capacity(uint32_t n) {
+    const uint32_t per = (n + kQueueSegs - 1u) / kQueueSegs;
+    return (per + 255u) / 256u * 256u;
 }
 
 // packed: dim (bits 0-15), depth (16-22), diffuse bounces (23-29), specular bounce (30), camera ray (31)
@@ -150,16 +188,21 @@ __device__ __forceinline__ void ext_media_leave(uint4& e, int mat) {  // MediumS
 struct LaunchParams {
     int width, height;
     int first_sample;
-    int spl;              // samples per pixel in this launch
-    uint32_t num_pixels;  // pixels rendered by this launch
+    int spl;              // samples per pixel in this window
+    uint32_t num_pixels;  // pixels rendered by this window
     const uint32_t* pixel_list;  // their film indices in increasing order; null: every pixel (num_pixels = w * h)
-    uint32_t num_paths;   // num_pixels * spl
-    // Adaptive sampling: after adding this launch's samples k_film clears active[pixel] of every pixel whose
-    // relative error is below noise_threshold (adapt_check: the launch ends where the reference checks).
+    uint32_t num_paths;   // samples of the window: num_pixels * spl; sample g is pixel g % num_pixels, sample
+                          // first_sample + g / num_pixels
+    // Adaptive sampling: after adding this window's samples k_film clears active[pixel] of every pixel whose
+    // relative error is below noise_threshold (adapt_check: the window ends where the reference checks).
     uint32_t* active;
     int adapt_check;
     float noise_threshold;
-    int want_aov;         // record the camera vertex's AOVs (pb.aov)
+    int want_aov;         // record the camera vertex's AOVs (pb.aov_res)
+    int guided_bounces;   // guiding: vertices of a path below this depth are guided
+    uint32_t grec_slots;  // guiding: slots of the training-record buffer
+    float4* grec_out;     //           the buffer (3 float4 per record)
+    uint64_t grec_salt;   //           per-window salt of the slot hash
     int max_depth, rr_depth, shadow_samples;
     float clamp_indirect;
     uint64_t seed;
@@ -219,48 +262,67 @@ __device__ __forceinline__ float power_heuristic(int nf, float f_pdf, int ng, fl
 
 // ---------------------------------------------------------------------------------------------- kernels
 
-__global__ void __launch_bounds__(kBlock) k_generate(LaunchParams lp, PathBuffers pb) {
+// tracePixel's camera ray (renderer.cpp:139-162) for window sample g: Sobol dimensions 0-1 for the pixel filter,
+// 2-3 drawn for the lens and unused by the pinhole camera.
+__device__ __forceinline__ V3 camera_dir(const LaunchParams& lp, uint32_t g) {
     const uint32_t m = lp.num_pixels;
+    const uint32_t q = g % m;
+    const uint32_t pix = lp.pixel_list ? lp.pixel_list[q] : q;
+    const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + g / m;
+    const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
+    const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
+    SobolState ss{sobol_pixel_hash(lp.seed, x, y), rev_bits(sample), 0u};
+    float fu, fv, lu, lv;
+    sobol_get2d(ss, fu, fv);  // pixel filter (renderer.cpp:148)
+    sobol_get2d(ss, lu, lv);  // lens sample, unused by the pinhole camera but consumed (renderer.cpp:151)
+    const float ox = filter_sample(lp, fu), oy = filter_sample(lp, fv);
+    const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(lp.width);
+    const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
+    const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
+    return normalized(target - lp.cam_pos);
+}
+constexpr uint32_t kCameraDims = 2;  // Sobol dimensions camera_dir consumes
+
+// The state a camera path starts with (tracePixel, renderer.cpp:139-162; PathTracer::Li, path_tracer.cpp:321-360).
+__device__ __forceinline__ uint4 camera_ids(const LaunchParams& lp, uint32_t g) {
+    const uint32_t m = lp.num_pixels;
+    const uint32_t q = g % m;
+    const uint32_t pix = lp.pixel_list ? lp.pixel_list[q] : q;
+    const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + g / m;
+    const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
+    const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
+    return make_uint4(g, sample, sobol_pixel_hash(lp.seed, x, y), pack_bits(kCameraDims, 0, 0, true, true));
+}
+
+// Start of a window: every path id free, nothing handed out; held-back slots cleared (a cancelled window may
+// leave contributions behind).
+// Segment k uses the first ids_per_seg of its ids in this window (the window's pool), fewer when it renders
+// fewer samples.
+__global__ void __launch_bounds__(kBlock) k_window_init(LaunchParams lp, PathBuffers pb, uint32_t ids_per_seg) {
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < lp.num_paths; i += stride) {
-        const uint32_t pix = lp.pixel_list ? lp.pixel_list[i % m] : i % m;
-        const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + i / m;
-        const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
-        const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
-        SobolState ss{sobol_pixel_hash(lp.seed, x, y), rev_bits(sample), 0u};
-        float fu, fv, lu, lv;
-        sobol_get2d(ss, fu, fv);  // pixel filter (renderer.cpp:148)
-        sobol_get2d(ss, lu, lv);  // lens sample, unused by the pinhole camera but consumed (renderer.cpp:151)
-        const float ox = filter_sample(lp, fu), oy = filter_sample(lp, fv);
-        const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(lp.width);
-        const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
-        const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
-        const V3 dir = normalized(target - lp.cam_pos);
-        const uint32_t slot = first_slot(i, pb.seg_cap);
-        pb.ro[0][slot] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(i));
-        pb.rd[0][slot] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-        pb.thr[i] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        pb.L[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        pb.last_p[i] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
-        pb.last_n[i] = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
-        pb.ids[i] = make_uint4(pix, sample, ss.pixel_hash, pack_bits(ss.dim, 0, 0, true, true));
-        if (lp.transport || lp.guide_records) pb.ext[i] = make_uint4(0u, 0u, 0u, 0u);
-        if (lp.want_aov) {
-            pb.aov[2 * static_cast<size_t>(i)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            pb.aov[2 * static_cast<size_t>(i) + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
-        }
-        if (lp.guide_records)
-            for (int r = 0; r < kGuideRecords; ++r) pb.grec[static_cast<size_t>(i) * 8 + 4 * r + 3] = make_float4(0, 0, 0, 0);
-        for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(i) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
+    const uint32_t ids = kQueueSegs * ids_per_seg;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ids; i += stride) {
+        const uint32_t k = i / ids_per_seg, e = i - k * ids_per_seg;
+        if (e >= seg_samples(lp.num_paths, k)) continue;
+        const uint32_t pid = k * pb.seg_cap + e;
+        pb.fl[0][pid] = pid;
+        for (int s = 0; s < lp.shadow_samples; ++s) pb.pend[static_cast<size_t>(pid) * lp.shadow_samples + s] = make_float4(0, 0, 0, 0);
     }
-    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) {  // the paths of segment k: its chunks, the last one partial
-        const uint32_t k = threadIdx.x, n = lp.num_paths, chunks = (n + 63u) / 64u;
-        uint32_t cnt = chunks > k ? ((chunks - 1u - k) / kQueueSegs + 1u) * 64u : 0u;
-        if (chunks > 0u && (chunks - 1u) % kQueueSegs == k && (n & 63u) != 0u) cnt -= 64u - (n & 63u);
-        *seg_counter(pb.ctr, kCtrCur, k) = cnt;
+    if (blockIdx.x == 0 && threadIdx.x < kQueueSegs) {
+        const uint32_t k = threadIdx.x;
+        const uint32_t total = seg_samples(lp.num_paths, k);
+        const uint32_t cam = min(ids_per_seg, total);
+        *seg_counter(pb.ctr, kCtrBase, k) = 0u;
+        *seg_counter(pb.ctr, kCtrCam, k) = cam;
+        *seg_counter(pb.ctr, kCtrCur, k) = cam;
         *seg_counter(pb.ctr, kCtrNext, k) = 0u;
         *seg_counter(pb.ctr, kCtrShadow, k) = 0u;
-        if (k == 0) pb.ctr[kCtrTotal] = n;
+        *seg_counter(pb.ctr, kCtrFree, k) = cam;
+        *seg_counter(pb.ctr, kCtrFreeNext, k) = 0u;
+        *seg_counter(pb.ctr, kCtrGen, k) = 0u;
+        *seg_counter(pb.ctr, kCtrChunkIsect, k) = 0u;
+        *seg_counter(pb.ctr, kCtrChunkShadow, k) = 0u;
+        if (k == 0) pb.ctr[kCtrTotal] = lp.num_paths;
     }
 }
 
@@ -380,11 +442,102 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuf
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, ray_queue(pb), sc.nodes, sc.tris, st, nv, tv);
+    trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, trace_ray_queue(pb), sc.nodes, sc.tris,
+                               st, nv, tv);
+    report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[2], nv);
         wave_count(&pb.stats[3], tv);
     }
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// A path's result: its radiance plus the held-back NEE of its last vertex, into its sample's slot
+// (the sum Film::addSample receives, path_tracer.cpp:355-359 and the end of Li).
+__device__ __forceinline__ V3 finish_radiance(const PathBuffers& pb, uint32_t pid, V3 L, int S, bool clear) {
+    for (int s = 0; s < S; ++s) {
+        const size_t k = static_cast<size_t>(pid) * S + s;
+        const float4 e = pb.pend[k];
+        L += mk(e.x, e.y, e.z);
+        if (clear) pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    return L;
+}
+
+// Training records of an ended path (guiding). A record of a guided vertex holds the point, the outgoing and
+// the sampled direction, the throughput before the vertex and the path radiance once the vertex's own NEE had
+// entered it (written at the start of the next vertex: a record is closed once that baseline is in); the
+// radiance the path then brought back along the sampled direction, divided by that throughput, is
+// f cos L_i / pdf for the sampled direction: its luminance weights the record. Records go to hash-chosen
+// slots of the guide's buffer (later ones overwrite), so any contiguous batch of the buffer is a spatially
+// unbiased sample. Returns the number of records written.
+__device__ __forceinline__ uint32_t guide_collect(const LaunchParams& lp, const PathBuffers& pb, uint32_t pid,
+                                                  uint32_t g, V3 L, uint32_t extw) {
+    uint32_t closed = (extw & kExtRecMask) >> kExtRecShift;
+    if (extw & kExtRecOpen) --closed;
+    uint32_t written = 0;
+    for (uint32_t r = 0; r < closed; ++r) {
+        const float4* rec = pb.grec + static_cast<size_t>(pid) * 8 + 4 * r;
+        const float4 base = rec[3];
+        const float4 a = rec[0], b = rec[1], c = rec[2];
+        const V3 inc = L - mk(base.x, base.y, base.z);
+        const V3 li = mk(c.y > 0.0f ? inc.x / c.y : 0.0f, c.z > 0.0f ? inc.y / c.z : 0.0f, c.w > 0.0f ? inc.z / c.w : 0.0f);
+        const float w = luminance(li);
+        if (!(w > 0.0f) || !finite_f(w)) continue;
+        const uint32_t slot = static_cast<uint32_t>(mix64((static_cast<uint64_t>(g) << 1 | r) ^ lp.grec_salt) % lp.grec_slots);
+        lp.grec_out[3 * static_cast<size_t>(slot)] = make_float4(a.x, a.y, a.z, w);
+        lp.grec_out[3 * static_cast<size_t>(slot) + 1] = make_float4(a.w, b.x, b.y, 0.0f);
+        lp.grec_out[3 * static_cast<size_t>(slot) + 2] = make_float4(b.z, b.w, c.x, 0.0f);
+        ++written;
+    }
+    return written;
+}
+
+// Free ids of the current free list: the deferred ones are finalised (their shadow rays have been traced), and
+// the first cam of every segment start the segment's next samples: their camera rays go to the current queue
+// after the continuation rays, tagged with the sample index, for k_intersect to trace with the others.
+template <bool kGuide>
+__global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers pb, int cur) {
+    const SegShare sh = seg_share(blockIdx.x, gridDim.x, kQueueSegs);
+    const uint32_t k = sh.seg, seg_base = k * pb.seg_cap;
+    const uint32_t n = *seg_counter(pb.ctr, kCtrFree, k);
+    const uint32_t cam = *seg_counter(pb.ctr, kCtrCam, k);
+    const uint32_t base = *seg_counter(pb.ctr, kCtrBase, k);
+    const uint32_t gen = *seg_counter(pb.ctr, kCtrGen, k);
+    const uint32_t* fl = pb.fl[cur] + seg_base;
+    float4* ro = pb.ro[cur] + seg_base + base;
+    float4* rd = pb.rd[cur] + seg_base + base;
+    const int S = lp.shadow_samples;
+    uint32_t written = 0;
+    for (uint32_t e = sh.rank * blockDim.x + threadIdx.x; e < n; e += sh.workers * blockDim.x) {
+        const uint32_t entry = fl[e];
+        const uint32_t pid = entry & kPidMask;
+        if (entry & kDeferred) {
+            const uint4 ids = pb.ids[pid];
+            const float4 L4 = pb.L[pid];
+            const V3 L = finish_radiance(pb, pid, mk(L4.x, L4.y, L4.z), S, true);
+            pb.res[ids.x] = make_float4(L.x, L.y, L.z, 0.0f);
+            if (kGuide && lp.guide_records) written += guide_collect(lp, pb, pid, ids.x, L, pb.ext[pid].w);
+        }
+        if (e < cam) {
+            const uint32_t g = seg_sample_index(k, gen + e);
+            const V3 d = camera_dir(lp, g);
+            ro[e] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(pid));
+            rd[e] = make_float4(d.x, d.y, d.z, __uint_as_float(kCamEntry | g));
+            if (pb.aov_res) {
+                pb.aov_res[2 * static_cast<size_t>(g)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            }
+        }
+    }
+    if (kGuide) wave_count(&pb.stats[6], written);
 }
 
 template <bool kCount>
@@ -394,6 +547,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffer
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
     trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, shadow_queue(pb, S), sc.nodes, sc.tris, st, nv, tv);
+    report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
         wave_count(&pb.stats[5], tv);
@@ -409,6 +563,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, P
     uint32_t nv = 0, tv = 0;
     trace_queue<false, kCount>(TransmitShadowJob{pb.sq_o, pb.sq_d, pb.pend, sc, clamp_limit}, shadow_queue(pb, S),
                                sc.nodes, sc.tris, st, nv, tv);
+    report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
         wave_count(&pb.stats[5], tv);
@@ -423,10 +578,13 @@ struct RenderQueries {
     const uint32_t* counts;   // segment counters of the current queue
     uint32_t seg_cap;
     float4* mix;
+    uint32_t guided_bounces;  // entries of paths at this depth or deeper are not guided
     __device__ __forceinline__ bool fetch(uint32_t i, V3& p, V3& d) const {
         const float4 h = hits[i];
         if (__float_as_int(h.y) < 0) return false;
         const float4 o = ro[i], r = rd[i];
+        const uint32_t tag = __float_as_uint(r.w);
+        if (!(tag & kCamEntry) && tag >= guided_bounces) return false;
         const V3 dir = mk(r.x, r.y, r.z);
         p = mk(o.x, o.y, o.z) + dir * h.x;
         d = -dir;
@@ -438,49 +596,22 @@ struct RenderQueries {
     }
 };
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
-    x ^= x >> 30;
-    x *= 0xbf58476d1ce4e5b9ull;
-    x ^= x >> 27;
-    x *= 0x94d049bb133111ebull;
-    return x ^ (x >> 31);
-}
 
-// Training records of every ended path. A record of a guided vertex holds the point, the outgoing and the
-// sampled direction, the throughput before the vertex and the path radiance once the vertex's own NEE had
-// entered it (written at the start of the next vertex); the radiance the path then brought back along the
-// sampled direction, divided by that throughput, is f cos L_i / pdf for the sampled direction: its luminance
-// weights the record. Records go to hash-chosen slots of the guide's buffer (later ones overwrite), so any
-// contiguous batch of the buffer is a spatially unbiased sample.
-__global__ void __launch_bounds__(kBlock) k_guide_collect(LaunchParams lp, PathBuffers pb, float4* __restrict__ out,
-                                                          uint32_t slots, uint64_t salt) {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const int S = lp.shadow_samples;
-    for (uint32_t pid = blockIdx.x * blockDim.x + threadIdx.x; pid < lp.num_paths; pid += stride) {
-        const float4 L4 = pb.L[pid];
-        V3 L = mk(L4.x, L4.y, L4.z);
-        for (int s = 0; s < S; ++s) {
-            const float4 e = pb.pend[static_cast<size_t>(pid) * S + s];
-            L += mk(e.x, e.y, e.z);
-        }
-        uint32_t written = 0;
-        for (int r = 0; r < kGuideRecords; ++r) {
-            const float4* rec = pb.grec + static_cast<size_t>(pid) * 8 + 4 * r;
-            const float4 base = rec[3];
-            if (base.w != 1.0f) continue;
-            const float4 a = rec[0], b = rec[1], c = rec[2];
-            const V3 inc = L - mk(base.x, base.y, base.z);
-            const V3 li = mk(c.y > 0.0f ? inc.x / c.y : 0.0f, c.z > 0.0f ? inc.y / c.z : 0.0f, c.w > 0.0f ? inc.z / c.w : 0.0f);
-            const float w = luminance(li);
-            if (!(w > 0.0f) || !finite_f(w)) continue;
-            const uint32_t slot = static_cast<uint32_t>(mix64((static_cast<uint64_t>(pid) << 1 | r) ^ salt) % slots);
-            out[3 * static_cast<size_t>(slot)] = make_float4(a.x, a.y, a.z, w);
-            out[3 * static_cast<size_t>(slot) + 1] = make_float4(a.w, b.x, b.y, 0.0f);
-            out[3 * static_cast<size_t>(slot) + 2] = make_float4(b.z, b.w, c.x, 0.0f);
-            ++written;
-        }
-        wave_count(&pb.stats[6], written);
+// The environment seen by a ray that left the scene, with MIS against NEE unless the last bounce was specular
+// (path_tracer.cpp:482-514), added to the path radiance.
+__device__ __forceinline__ V3 add_miss_radiance(const SceneView& sc, const LaunchParams& lp, V3 L, V3 dir, V3 thr,
+                                                bool specular, float last_pdf, V3 last_p, V3 last_n, uint32_t diffuse) {
+    const V3 le = env_eval(sc.env, dir);
+    float weight = 1.0f;
+    if (!specular) {
+        const Picker pk = picker_prepare(sc, last_p, last_n);
+        const float light_pdf = picker_pmf(sc, pk, sc.num_lights, last_p, last_n) * env_pdf(sc.env, dir);
+        weight = power_heuristic(1, last_pdf, lp.shadow_samples, light_pdf);
     }
+    V3 c = thr * (le * weight);
+    if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
+    if (finite3(c)) L += c;
+    return L;
 }
 
 // kTransport = false: the scene has no Dielectric and no transmitting or subsurface material, so no path
@@ -488,15 +619,20 @@ __global__ void __launch_bounds__(kBlock) k_guide_collect(LaunchParams lp, PathB
 // kGuide: the render has a guide; a guided vertex samples the mixture of its queue slot or the BSDF and weighs
 // by the mixture density (path_tracer.cpp:700-735 with the MLP in place of the SD-tree), and records its
 // sample for training.
-template <bool kTransport, bool kGuide>
-__device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBuffers pb, int cur, int guide_bounce) {
+// kHitsOnly: the misses are left to k_shade_miss.
+template <bool kTransport, bool kGuide, bool kHitsOnly>
+__device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
     // Block b works on segment b % kQueueSegs of the current queue and appends to the same segment of the next
-    // and shadow queues (whose segments therefore never receive more than this one holds).
+    // queue, the shadow queue and the next free list (whose segments therefore never receive more than the
+    // segment's path ids).
     const SegShare sh = seg_share(blockIdx.x, gridDim.x, kQueueSegs);
     const uint32_t n = *seg_counter(pb.ctr, kCtrCur, sh.seg);
     const uint32_t seg_base = sh.seg * pb.seg_cap;
     uint32_t* next_count = seg_counter(pb.ctr, kCtrNext, sh.seg);
     uint32_t* shadow_count = seg_counter(pb.ctr, kCtrShadow, sh.seg);
+    uint32_t* free_count = seg_counter(pb.ctr, kCtrFreeNext, sh.seg);
+    uint32_t* fl_next = pb.fl[cur ^ 1] + seg_base;
+    uint32_t collected = 0;
     const float4* ro = pb.ro[cur];
     const float4* rd = pb.rd[cur];
     float4* ro_next = pb.ro[cur ^ 1] + seg_base;
@@ -517,7 +653,8 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
         const uint32_t mine = seg_base + base + threadIdx.x;
         const bool valid = base + threadIdx.x < n;
         const bool hit_here = valid && __float_as_int(pb.hits[mine].y) >= 0;
-        const uint64_t hm = __ballot(hit_here), mm = __ballot(valid && !hit_here);
+        const bool miss_here = !kHitsOnly && valid && !hit_here;
+        const uint64_t hm = __ballot(hit_here), mm = __ballot(miss_here);
         if (__lane_id() == 0) {
             s_count[0][wave] = static_cast<uint32_t>(__popcll(hm));
             s_count[1][wave] = static_cast<uint32_t>(__popcll(mm));
@@ -532,7 +669,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 misses_before += s_count[1][k];
             }
         }
-        if (valid) {
+        if (hit_here || miss_here) {
             const uint32_t pos = hit_here ? hits_before + static_cast<uint32_t>(__popcll(hm & below))
                                           : total_hits + misses_before + static_cast<uint32_t>(__popcll(mm & below));
             s_slot[pos] = mine;
@@ -542,33 +679,51 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
         const uint32_t i = active ? s_slot[threadIdx.x] : 0u;
         __syncthreads();  // s_slot and s_count are rewritten by the next round
         bool extend = false;
-        uint32_t pid = 0;
+        bool ended = false;     // the path ended at this entry
+        bool deferred = false;  // ... with a shadow ray of its last vertex still to be traced
+        uint32_t pid = 0, new_depth_tag = 0;
         V3 new_o, new_d;
         if (active) {
-            pid = __float_as_uint(ro[i].w);
             const float4 o4 = ro[i];
             const float4 d4 = rd[i];
+            pid = __float_as_uint(o4.w);
             const V3 dir = mk(d4.x, d4.y, d4.z);
-            const float4 t4 = pb.thr[pid];
-            V3 thr = mk(t4.x, t4.y, t4.z);
-            float last_pdf = t4.w;
-            const float4 L4 = pb.L[pid];
-            V3 L = mk(L4.x, L4.y, L4.z);
-            const uint4 ids = pb.ids[pid];
+            // A camera entry starts its path here (k_regen made its ray); its state is the camera's, not read.
+            const bool cam_entry = (__float_as_uint(d4.w) & kCamEntry) != 0u;
+            V3 thr = mk(1.0f, 1.0f, 1.0f), L = mk(0.0f, 0.0f, 0.0f);
+            float last_pdf = 0.0f;
+            uint4 ids;
             uint4 ext = make_uint4(0u, 0u, 0u, 0u);
-            if constexpr (kTransport || kGuide) ext = pb.ext[pid];
+            float4 lp4 = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
+            float4 ln4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+            if (cam_entry) {
+                ids = camera_ids(lp, __float_as_uint(d4.w) & ~kCamEntry);
+            } else {
+                const float4 t4 = pb.thr[pid];
+                thr = mk(t4.x, t4.y, t4.z);
+                last_pdf = t4.w;
+                const float4 L4 = pb.L[pid];
+                L = mk(L4.x, L4.y, L4.z);
+                ids = pb.ids[pid];
+                if constexpr (kTransport || kGuide) ext = pb.ext[pid];
+                lp4 = pb.last_p[pid];
+                ln4 = pb.last_n[pid];
+                // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
+                for (int s = 0; s < S; ++s) {
+                    const size_t k = static_cast<size_t>(pid) * S + s;
+                    const float4 p = pb.pend[k];
+                    L += mk(p.x, p.y, p.z);
+                    pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                }
+            }
+            const uint32_t g = ids.x;
+            bool state_moved = cam_entry;  // last_p / last_n must be written if the path goes on
             float lambda = __uint_as_float(ext.x);
             SobolState ss{ids.z, rev_bits(ids.y), get_dim(ids.w)};
             uint32_t depth = get_depth(ids.w), diffuse = get_diffuse(ids.w);
             bool specular = get_specular(ids.w), camera = get_camera(ids.w);
             bool alive = true;
-            // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
-            for (int s = 0; s < S; ++s) {
-                const size_t k = static_cast<size_t>(pid) * S + s;
-                const float4 p = pb.pend[k];
-                L += mk(p.x, p.y, p.z);
-                pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
+            bool issued = false;  // a shadow ray of this vertex was appended
             if (kGuide && (ext.w & kExtRecOpen)) {  // the radiance baseline of the previous vertex's record
                 const uint32_t r = ((ext.w & kExtRecMask) >> kExtRecShift) - 1u;
                 pb.grec[static_cast<size_t>(pid) * 8 + 4 * r + 3] = make_float4(L.x, L.y, L.z, 1.0f);
@@ -576,7 +731,6 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             }
             const float4 h = pb.hits[i];
             const int tri = __float_as_int(h.y);
-            const float4 lp4 = pb.last_p[pid], ln4 = pb.last_n[pid];
             const V3 last_p = mk(lp4.x, lp4.y, lp4.z), last_n = mk(ln4.x, ln4.y, ln4.z);
             Surface si;
             V3 wo = -dir;
@@ -624,27 +778,16 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 }
             } else if (tri < 0) {
                 // The camera ray's first segment misses: the AOVs of the background (renderer.cpp:196-204).
-                if (lp.want_aov && camera && pb.aov[2 * static_cast<size_t>(pid) + 1].w < 0.0f) {
-                    pb.aov[2 * static_cast<size_t>(pid)] =
+                if (lp.want_aov && camera && pb.aov_res[2 * static_cast<size_t>(g) + 1].w < 0.0f) {
+                    pb.aov_res[2 * static_cast<size_t>(g)] =
                         sc.has_env ? clamped4(env_eval(sc.env, dir), 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    pb.aov[2 * static_cast<size_t>(pid) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
                 // environment with MIS against NEE (:482-514).
                 const int top = kTransport ? ext_media_top(ext) : -1;
                 const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
-                if (sc.has_env && !absorbed) {
-                    const V3 le = env_eval(sc.env, dir);
-                    float weight = 1.0f;
-                    if (!specular) {
-                        const Picker pk = picker_prepare(sc, last_p, last_n);
-                        const float light_pdf = picker_pmf(sc, pk, sc.num_lights, last_p, last_n) * env_pdf(sc.env, dir);
-                        weight = power_heuristic(1, last_pdf, S, light_pdf);
-                    }
-                    V3 c = thr * (le * weight);
-                    if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
-                    if (finite3(c)) L += c;
-                }
+                if (sc.has_env && !absorbed) L = add_miss_radiance(sc, lp, L, dir, thr, specular, last_pdf, last_p, last_n, diffuse);
                 alive = false;
             } else {
                 fill_hit(sc, tri, dir, h.z, h.w, si);
@@ -658,9 +801,9 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     // The camera ray's first surface: its AOVs (renderer.cpp:183-194, primaryAlbedo :47-58). A camera
                     // path stays one through delta refractions, but only its first segment is the primary hit
                     // (path_tracer.cpp:441-444).
-                    if (lp.want_aov && camera && pb.aov[2 * static_cast<size_t>(pid) + 1].w < 0.0f) {
-                        pb.aov[2 * static_cast<size_t>(pid)] = clamped4(sc.materials[si.material].base_color, 1.0f);
-                        pb.aov[2 * static_cast<size_t>(pid) + 1] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                    if (lp.want_aov && camera && pb.aov_res[2 * static_cast<size_t>(g) + 1].w < 0.0f) {
+                        pb.aov_res[2 * static_cast<size_t>(g)] = clamped4(sc.materials[si.material].base_color, 1.0f);
+                        pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
                     }
                     // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
                     const int top = kTransport ? ext_media_top(ext) : -1;
@@ -706,8 +849,9 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                         ext.x = __float_as_uint(lambda);
                     }
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
-                    // Guided: the bounce had its mixtures inferred (the first guided_bounces bounces of a launch).
-                    const bool guided = kGuide && guide_bounce && lp.guide_fraction > 0.0f && !is_delta && !exit_vertex;
+                    // Guided: the slot's mixture was inferred (vertices below depth guided_bounces).
+                    const bool guided = kGuide && static_cast<int>(depth) < lp.guided_bounces && lp.guide_fraction > 0.0f &&
+                                        !is_delta && !exit_vertex;
                     const float alpha = lp.guide_fraction;
                     // The slot's mixture is read where it is used (L1/L2), not held in registers across the NEE loop.
                     const float4* gmix = pb.mix + static_cast<size_t>(i) * kGuideMixFloat4;
@@ -759,6 +903,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                 ok = true;
                             } while (false);
                             const uint32_t ss_slot = wave_append(shadow_count, ok);
+                            issued = issued || ok;
                             if (ok) {
                                 sq_o[ss_slot] = make_float4(so.x, so.y, so.z, st);
                                 sq_d[ss_slot] = make_float4(sd.x, sd.y, sd.z,
@@ -785,9 +930,14 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     } else {
                         const float ub = guided ? (uc - alpha) / (1.0f - alpha) : uc;
                         sampled = bsdf_sample<kTransport>(m, T, si, wo, ub, u0, u1, lambda, bs);
-                        if (guided && sampled && !bs.subsurface && !bs.pass_through)
-                            bs.pdf = bs.delta ? (1.0f - alpha) * bs.pdf
-                                              : (1.0f - alpha) * bs.pdf + alpha * guide_pdf(gmix, bs.wi);
+                        // The BSDF branch is taken with probability 1 - alpha (path_tracer.cpp:721-730): a
+                        // subsurface entry, which the mixture cannot produce, carries its weight / (1 - alpha);
+                        // delta and pass-through samples their pdf x (1 - alpha); the rest the mixture density.
+                        if (guided && sampled) {
+                            if (bs.subsurface) bs.weight = bs.weight * (1.0f / (1.0f - alpha));
+                            else if (bs.delta || bs.pass_through) bs.pdf = (1.0f - alpha) * bs.pdf;
+                            else bs.pdf = (1.0f - alpha) * bs.pdf + alpha * guide_pdf(gmix, bs.wi);
+                        }
                     }
                     if (!sampled) {
                         alive = false;
@@ -796,7 +946,8 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                         new_d = normalized(bs.wi);
                         new_o = spawn_origin(si, new_d);
                         ext.w = (ext.w & ~kExtWalkStepMask) | kExtWalk;
-                        pb.last_n[pid] = make_float4(ln4.x, ln4.y, ln4.z, __uint_as_float(static_cast<uint32_t>(mat)));
+                        ln4.w = __uint_as_float(static_cast<uint32_t>(mat));
+                        state_moved = true;
                         extend = true;
                     } else if (bs.pdf <= 0.0f || is_black(bs.f)) {
                         alive = false;
@@ -827,8 +978,9 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             alive = false;
                         } else {
                             last_pdf = bs.pdf;
-                            pb.last_p[pid] = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
-                            pb.last_n[pid] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                            lp4 = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
+                            ln4 = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                            state_moved = true;
                             specular = bs.delta;
                             if (!bs.delta) ++diffuse;
                             camera = camera && bs.delta && dot(bs.wi, si.normal) < 0.0f;
@@ -859,19 +1011,37 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 }
             }
             if (!alive) extend = false;
-            pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
-            if (extend) {
-                pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+            ended = !extend;
+            // An ended path is finalised here unless a shadow ray of this vertex is still to be traced: then
+            // its state stays for k_regen to finalise in the next iteration.
+            deferred = ended && issued;
+            if (extend || deferred) {
+                pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
                 if constexpr (kTransport || kGuide) pb.ext[pid] = ext;
             }
+            if (extend) {
+                pb.thr[pid] = make_float4(thr.x, thr.y, thr.z, last_pdf);
+                if (state_moved) {
+                    pb.last_p[pid] = lp4;
+                    pb.last_n[pid] = ln4;
+                }
+            }
+            if (ended && !deferred) {
+                pb.res[g] = make_float4(L.x, L.y, L.z, 0.0f);
+                if (kGuide && lp.guide_records) collected += guide_collect(lp, pb, pid, g, L, ext.w);
+            }
+            new_depth_tag = depth;
         }
         const uint32_t slot = wave_append(next_count, extend);
         if (extend) {
             ro_next[slot] = make_float4(new_o.x, new_o.y, new_o.z, __uint_as_float(pid));
-            rd_next[slot] = make_float4(new_d.x, new_d.y, new_d.z, 0.0f);
+            rd_next[slot] = make_float4(new_d.x, new_d.y, new_d.z, __uint_as_float(new_depth_tag));
         }
+        const uint32_t fslot = wave_append(free_count, ended);
+        if (ended) fl_next[fslot] = pid | (deferred ? kDeferred : 0u);
     }
+    if (kGuide) wave_count(&pb.stats[6], collected);
 }
 
 // Two waves per SIMD for the guide and transport variants: they sit a few registers above the 256 that allow
@@ -880,26 +1050,45 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
 // own schedule, which the occupancy request would change.
 template <bool kTransport, bool kGuide>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
-k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur, int guide_bounce) {
-    shade_body<kTransport, kGuide>(sc, lp, pb, cur, guide_bounce);
+k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
+    shade_body<kTransport, kGuide, false>(sc, lp, pb, cur);
 }
 
-__global__ void __launch_bounds__(kBlock) k_shade_plain(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
-    shade_body<false, false>(sc, lp, pb, cur, 0);
+__global__ void __launch_bounds__(kBlock) PE_SHADE_PLAIN_ATTR k_shade_plain(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
+    shade_body<false, false, false>(sc, lp, pb, cur);
 }
 
-// After a bounce: the next queue becomes current, the shadow queue is emptied, ray counts accumulate.
-// One wave, one lane per segment.
-__global__ void k_rotate(PathBuffers pb) {
+// After an iteration: the next queue becomes current, the next free list current, the samples given out are
+// counted, and the new free ids' share of the segment's remaining samples is set. One wave, one lane per
+// segment. ctr[kCtrTotal] = continuation rays + free ids + samples not yet handed out: zero when the window is
+// done (its last free ids finalised).
+__global__ void k_rotate(PathBuffers pb, uint32_t num_samples) {
     const uint32_t k = threadIdx.x;
     if (blockIdx.x != 0 || k >= kQueueSegs) return;
+    uint32_t* base = seg_counter(pb.ctr, kCtrBase, k);
     uint32_t* cur = seg_counter(pb.ctr, kCtrCur, k);
+    uint32_t* cam = seg_counter(pb.ctr, kCtrCam, k);
     uint32_t* next = seg_counter(pb.ctr, kCtrNext, k);
     uint32_t* shadow = seg_counter(pb.ctr, kCtrShadow, k);
-    unsigned long long traced = *cur, shadows = *shadow, live = *next;
-    *cur = *next;
+    uint32_t* fr = seg_counter(pb.ctr, kCtrFree, k);
+    uint32_t* fr_next = seg_counter(pb.ctr, kCtrFreeNext, k);
+    uint32_t* gen = seg_counter(pb.ctr, kCtrGen, k);
+    const uint32_t total = seg_samples(num_samples, k);
+    unsigned long long traced = *cur, shadows = *shadow;
+    const uint32_t given = *gen + *cam;
+    const uint32_t b = *next, f = *fr_next;
+    const uint32_t c = min(f, total - given);
+    *gen = given;
+    *seg_counter(pb.ctr, kCtrChunkIsect, k) = 0u;
+    *seg_counter(pb.ctr, kCtrChunkShadow, k) = 0u;
+    *base = b;
+    *cam = c;
+    *cur = b + c;
     *next = 0u;
     *shadow = 0u;
+    *fr = f;
+    *fr_next = 0u;
+    unsigned long long live = static_cast<unsigned long long>(b) + f + (total - given);
     for (int off = 32; off > 0; off >>= 1) {
         traced += __shfl_xor(traced, off);
         shadows += __shfl_xor(shadows, off);
@@ -908,12 +1097,12 @@ __global__ void k_rotate(PathBuffers pb) {
     if (k == 0) {
         pb.stats[0] += traced;
         pb.stats[1] += shadows;
-        pb.ctr[kCtrTotal] = static_cast<uint32_t>(live);
+        pb.ctr[kCtrTotal] = static_cast<uint32_t>(min(live, 0xFFFFFFFFull));
     }
 }
 
 __global__ void k_zero_stats(PathBuffers pb) {
-    if (blockIdx.x == 0 && threadIdx.x < 8) pb.stats[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x < kStatSlots) pb.stats[threadIdx.x] = 0;
 }
 
 // Film::addSample for every sample of every pixel, in sample order (film.cpp:56-71).
@@ -983,7 +1172,6 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
 #pragma clang fp contract(off)
     const uint32_t pixels = lp.num_pixels;
     const uint32_t stride = gridDim.x * blockDim.x;
-    const int S = lp.shadow_samples;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < pixels; q += stride) {
         const uint32_t p = lp.pixel_list ? lp.pixel_list[q] : q;
         float sr = sum[3 * static_cast<size_t>(p)], sg = sum[3 * static_cast<size_t>(p) + 1], sb = sum[3 * static_cast<size_t>(p) + 2];
@@ -991,13 +1179,8 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
         float l2 = lum2[p];
         uint32_t rej = 0;
         for (int k = 0; k < lp.spl; ++k) {
-            const uint32_t pid = q + static_cast<uint32_t>(k) * pixels;
-            const float4 L4 = pb.L[pid];
-            V3 L = mk(L4.x, L4.y, L4.z);
-            for (int s = 0; s < S; ++s) {
-                const float4 e = pb.pend[static_cast<size_t>(pid) * S + s];
-                L += mk(e.x, e.y, e.z);
-            }
+            const float4 L4 = pb.res[q + static_cast<uint32_t>(k) * pixels];
+            const V3 L = mk(L4.x, L4.y, L4.z);
             ++c;
             const bool ok = finite3(L) && L.x >= 0.0f && L.y >= 0.0f && L.z >= 0.0f;
             if (!ok) {
@@ -1017,9 +1200,9 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
         lum2[p] = l2;
         if (rej) atomicAdd(rejected, static_cast<unsigned long long>(rej));
         if (lp.want_aov) {  // addAovs (renderer.cpp:222-227)
-            film_add_aov<false>(f_albedo, p, pb.aov, 0, q, pixels, lp.spl, false);
-            film_add_aov<true>(f_normal, p, pb.aov, 1, q, pixels, lp.spl, false);
-            film_add_aov<false>(f_alpha, p, pb.aov, 0, q, pixels, lp.spl, true);
+            film_add_aov<false>(f_albedo, p, pb.aov_res, 0, q, pixels, lp.spl, false);
+            film_add_aov<true>(f_normal, p, pb.aov_res, 1, q, pixels, lp.spl, false);
+            film_add_aov<false>(f_alpha, p, pb.aov_res, 0, q, pixels, lp.spl, true);
         }
         // renderer.cpp:500-502: the pixel stops for the rest of the batch once its own error is low enough.
         if (lp.adapt_check && film_relative_error(sr, sg, sb, c, l2) < lp.noise_threshold) lp.active[p] = 0u;
@@ -1157,9 +1340,9 @@ std::vector<float> filter_cdf(int type, float width) {
 
 }  // namespace
 
-enum KernelKind { kKGenerate = 0, kKIntersect, kKShade, kKShadow, kKFilm, kKGuide, kKTrain, kKindCount };
+enum KernelKind { kKGenerate = 0, kKIntersect, kKShade, kKShadow, kKFilm, kKGuide, kKTrain, kKRegen, kKindCount };
 const char* const kKernelNames[kKindCount] = {"generate", "intersect", "shade", "shadow", "film", "guide_infer",
-                                              "guide_train"};
+                                              "guide_train", "regen"};
 
 // HIP events around each wavefront launch on the device's stream while profiling is on.
 struct Profiler {
@@ -1197,24 +1380,38 @@ struct Profiler {
     }
 };
 
-// Per-device wavefront buffers, grown on demand and kept between renders.
+// Per-stream wavefront buffers, grown on demand and kept between renders: the path pool (state of the ids in
+// flight, ray / shadow queues, free lists) and the window's result buffer.
 struct Wavefront {
     Profiler prof;
-    uint32_t capacity = 0;   // paths
-    uint32_t seg_cap = 0;    // queue slots per segment (kQueueSegs segments)
+    uint32_t capacity = 0;   // path ids: kQueueSegs segments of seg_cap
+    uint32_t seg_cap = 0;    // ids (and queue slots) per segment
     int shadow_samples = 0;
-    DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, ctr, stats, filter;
+    uint64_t result_cap = 0; // samples of the result buffer
+    uint64_t aov_cap = 0;    // samples of the AOV result buffer
+    DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, fl0, fl1, ctr, stats, filter;
+    DevBuf res, aov_res;
     // Adaptive sampling: per-pixel active flags and clean flags, the active pixel list, compaction counts.
     DevBuf adapt_active, adapt_clean, adapt_list, adapt_counts;
-    DevBuf aov;  // AOV records, two float4 per path, allocated by the first render that asks for AOVs
     uint32_t adapt_pixels = 0;
     int filter_type = -1;
     float filter_width = 0.0f;
+    // Progress of the iterations in flight: the remaining-work counter copied after every chunk of iterations
+    // into pinned host memory, one slot and one event per chunk (two chunks in flight).
+    uint32_t* host_live = nullptr;
+    hipEvent_t chunk_done[2] = {nullptr, nullptr};
 
-    ~Wavefront() { prof.drain(); }
+    ~Wavefront() {
+        prof.drain();
+        for (hipEvent_t& e : chunk_done)
+            if (e) (void)hipEventDestroy(e);
+        if (host_live) (void)hipHostFree(host_live);
+    }
 
-    // Device bytes of the wavefront buffers per path in flight.
-    static uint64_t bytes_per_path(int S) { return 4 * 16 + 7 * 16 + static_cast<uint64_t>(S) * 3 * 16; }
+    // Device bytes per path id in flight: two ray queues, hits, six state columns, two free lists, S held-back
+    // contributions and S shadow-queue records.
+    static uint64_t bytes_per_path(int S) { return 4 * 16 + 16 + 6 * 16 + 8 + static_cast<uint64_t>(S) * 3 * 16; }
+    static constexpr uint64_t kResultBytes = 16, kAovResultBytes = 32;  // per window sample
 
     int ensure_adaptive(uint32_t pixels, uint32_t blocks) {
         if (pixels <= adapt_pixels) return PE_OK;
@@ -1227,30 +1424,61 @@ struct Wavefront {
         return PE_OK;
     }
 
-    // Queue slots: kQueueSegs segments of seg_cap (a little above the path count).
     uint32_t slots() const { return kQueueSegs * seg_cap; }
 
+    // A pool of at least n path ids.
     int ensure(uint32_t n, int S) {
-        if (n <= capacity && S == shadow_samples) return PE_OK;
+        if (!host_live) {
+            if (hipHostMalloc(reinterpret_cast<void**>(&host_live), 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+                host_live = nullptr;
+                set_error("pinned host allocation failed");
+                return PE_ERR_HIP;
+            }
+            for (hipEvent_t& e : chunk_done)
+                if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+                    e = nullptr;
+                    set_error("event creation failed");
+                    return PE_ERR_HIP;
+                }
+        }
         const uint32_t cap = seg_capacity(n);
-        const size_t N = n, NS = static_cast<size_t>(n) * S;
+        if (cap <= seg_cap && S == shadow_samples) return PE_OK;
         const size_t Q = static_cast<size_t>(kQueueSegs) * cap, QS = Q * S;
-        if (!ro0.alloc(Q * 16) || !rd0.alloc(Q * 16) || !ro1.alloc(Q * 16) || !rd1.alloc(Q * 16) ||
-            !thr.alloc(N * 16) || !L.alloc(N * 16) || !last_p.alloc(N * 16) || !last_n.alloc(N * 16) ||
-            !ids.alloc(N * 16) || !ext.alloc(N * 16) || !pend.alloc(NS * 16) || !hits.alloc(Q * 16) || !sq_o.alloc(QS * 16) ||
-            !sq_d.alloc(QS * 16) || !ctr.alloc(kCtrBytes) || !stats.alloc(64)) {
+        if (!ro0.alloc(Q * 16) || !rd0.alloc(Q * 16) || !ro1.alloc(Q * 16) || !rd1.alloc(Q * 16) || !thr.alloc(Q * 16) ||
+            !L.alloc(Q * 16) || !last_p.alloc(Q * 16) || !last_n.alloc(Q * 16) || !ids.alloc(Q * 16) || !ext.alloc(Q * 16) ||
+            !pend.alloc(QS * 16) || !hits.alloc(Q * 16) || !sq_o.alloc(QS * 16) || !sq_d.alloc(QS * 16) ||
+            !fl0.alloc(Q * 4) || !fl1.alloc(Q * 4) || !ctr.alloc(kCtrBytes) || !stats.alloc(kStatSlots * 8)) {
             capacity = 0;
             seg_cap = 0;
             return PE_ERR_OOM;
         }
-        capacity = n;
+        capacity = static_cast<uint32_t>(Q);
         seg_cap = cap;
         shadow_samples = S;
         return PE_OK;
     }
 
+    // A result buffer of at least n samples (and AOV results when asked for).
+    int ensure_results(uint64_t n, bool aov) {
+        if (n > result_cap) {
+            if (!res.alloc(n * kResultBytes)) {
+                result_cap = 0;
+                return PE_ERR_OOM;
+            }
+            result_cap = n;
+        }
+        if (aov && n > aov_cap) {
+            if (!aov_res.alloc(n * kAovResultBytes)) {
+                aov_cap = 0;
+                return PE_ERR_OOM;
+            }
+            aov_cap = n;
+        }
+        return PE_OK;
+    }
+
     PathBuffers view() {
-        PathBuffers pb;
+        PathBuffers pb{};
         pb.ro[0] = ro0.as<float4>();
         pb.rd[0] = rd0.as<float4>();
         pb.ro[1] = ro1.as<float4>();
@@ -1262,9 +1490,13 @@ struct Wavefront {
         pb.ids = ids.as<uint4>();
         pb.ext = ext.as<uint4>();
         pb.pend = pend.as<float4>();
+        pb.res = res.as<float4>();
+        pb.aov_res = nullptr;
         pb.hits = hits.as<float4>();
         pb.sq_o = sq_o.as<float4>();
         pb.sq_d = sq_d.as<float4>();
+        pb.fl[0] = fl0.as<uint32_t>();
+        pb.fl[1] = fl1.as<uint32_t>();
         pb.ctr = ctr.as<uint32_t>();
         pb.seg_cap = seg_cap;
         pb.stats = stats.as<unsigned long long>();
@@ -1297,11 +1529,16 @@ namespace pe {
 
 Wavefront* wavefront_new() { return new Wavefront(); }
 void wavefront_delete(Wavefront* wf) { delete wf; }
-int wavefront_reserve(Wavefront* wf, uint32_t paths, int shadow_quality) {
-    return wf->ensure(paths, std::clamp(shadow_quality, 1, kMaxShadowSamples));
+int wavefront_reserve(Wavefront* wf, uint32_t paths, uint64_t results, int shadow_quality) {
+    const int rc = wf->ensure(paths, std::clamp(shadow_quality, 1, kMaxShadowSamples));
+    return rc != PE_OK ? rc : wf->ensure_results(results, false);
 }
 uint64_t wavefront_bytes_per_path(int shadow_quality) {
     return Wavefront::bytes_per_path(std::clamp(shadow_quality, 1, kMaxShadowSamples));
+}
+uint64_t wavefront_device_bytes(const Wavefront* wf) {
+    return static_cast<uint64_t>(wf->capacity) * Wavefront::bytes_per_path(std::max(1, wf->shadow_samples)) +
+           wf->result_cap * Wavefront::kResultBytes + wf->aov_cap * Wavefront::kAovResultBytes;
 }
 
 // Renders samples [first_sample, first_sample + sample_count) of every pixel into `film` on `lane`: the
@@ -1366,40 +1603,43 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         Wavefront* wf = lane.wf;
         const int S = std::clamp(settings->shadow_quality, 1, kMaxShadowSamples);  // PathTracer ctor (path_tracer.h:43)
         const uint32_t pixels = static_cast<uint32_t>(film->width) * static_cast<uint32_t>(film->height);
-        // Samples per launch: as asked, or (0) as many as keep about 2^26 paths in flight within a quarter of
-        // the free device memory. Deep launches amortise the short tail bounces, whose time is set by the
-        // longest traversal rather than by the number of rays.
+        // The window: samples of every pixel whose results are held until the film pass adds them in sample
+        // order; as asked, or (0) up to 2^28 samples (4 GB of results) within an eighth of the free device
+        // memory. The path pool (ids in flight, regenerated as paths end) is sized independently: as asked, or
+        // (0) 2^24 ids, and never above the window.
+        const bool adaptive = settings->adaptive != 0;
         int spl_req = settings->samples_per_launch;
         if (spl_req <= 0) {
             size_t free_b = 0, total_b = 0;
             if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = size_t{1} << 32;
-            const uint64_t per_path = Wavefront::bytes_per_path(S) + (guide ? (kGuideMixFloat4 + 8) * 16 : 0);
-            const uint64_t target = std::min<uint64_t>(uint64_t{1} << 26, free_b / 4 / per_path);
+            const uint64_t per_sample = Wavefront::kResultBytes + (want_aov ? Wavefront::kAovResultBytes : 0);
+            const uint64_t target = std::min<uint64_t>(kAutoWindowSamples, free_b / 8 / per_sample);
             spl_req = static_cast<int>(std::clamp<uint64_t>(target / std::max<uint32_t>(1, pixels), 1, 1024));
         }
-        // Path ids and held-back slots (pid * S + s) are 32-bit.
-        const uint64_t path_limit = std::min<uint64_t>(uint64_t{1} << 31, 0xFFFFFFFFull / static_cast<uint64_t>(S));
-        if (pixels > path_limit) {
-            set_error("pe_render: film too large for 32-bit path ids");
+        // Window sample indices carry a flag bit in the queue (31 bits); path ids and held-back slots
+        // (pid * S + s) are 32-bit.
+        if (pixels > kMaxWindowSamples) {
+            set_error("pe_render: film too large for 31-bit sample indices");
             return PE_ERR_INVALID;
         }
-        const uint64_t max_paths = std::max<uint64_t>(pixels, std::min<uint64_t>(path_limit, uint64_t{pixels} * spl_req));
+        const uint64_t max_window = std::max<uint64_t>(pixels, std::min<uint64_t>(kMaxWindowSamples, uint64_t{pixels} * spl_req));
         // Adaptive sampling renders in groups that end where the reference checks a pixel's error (every 8th
-        // sample), so a launch holds at most 8 samples of each pixel still rendering.
-        const bool adaptive = settings->adaptive != 0;
+        // sample), so a window holds at most 8 samples of each pixel still rendering.
         const float threshold = std::max(1e-4f, settings->noise_threshold);
         const int min_spp = std::max(2, settings->min_adaptive_samples);
-        const int spl_cap = adaptive ? std::min(8, static_cast<int>(std::max<uint64_t>(1, max_paths / pixels)))
-                                     : static_cast<int>(std::max<uint64_t>(1, max_paths / pixels));
-        int rc = wf->ensure(static_cast<uint32_t>(uint64_t{pixels} * std::min(spl_cap, sample_count)), S);
+        const int spl_cap = adaptive ? std::min(8, static_cast<int>(std::max<uint64_t>(1, max_window / pixels)))
+                                     : static_cast<int>(std::max<uint64_t>(1, max_window / pixels));
+        const uint64_t window = uint64_t{pixels} * std::min(spl_cap, sample_count);
+        const uint64_t pool_limit = std::min<uint64_t>(kMaxPoolIds, 0xFFFFFFFFull / static_cast<uint64_t>(S) - kQueueSegs * 256u);
+        const uint64_t pool_req = settings->paths_in_flight > 0 ? static_cast<uint64_t>(settings->paths_in_flight) : kDefaultPoolIds;
+        const uint32_t pool = static_cast<uint32_t>(std::min({pool_req, window, pool_limit}));
+        int rc = wf->ensure(pool, S);
+        if (rc == PE_OK) rc = wf->ensure_results(window, want_aov);
         if (rc != PE_OK) return rc;
         if (guide) {
             rc = guide_reserve_paths(guide, wf->capacity, wf->slots());
             if (rc != PE_OK) return rc;
         }
-        if (want_aov && wf->aov.bytes() < static_cast<size_t>(wf->capacity) * 32 &&
-            !wf->aov.alloc(static_cast<size_t>(wf->capacity) * 32))
-            return PE_ERR_OOM;
 
         LaunchParams lp{};
         lp.width = film->width;
@@ -1433,10 +1673,13 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
 
         PathBuffers pb = wf->view();
-        pb.aov = want_aov ? wf->aov.as<float4>() : nullptr;
+        pb.aov_res = want_aov ? wf->aov_res.as<float4>() : nullptr;
         lp.want_aov = want_aov ? 1 : 0;
         pb.mix = guide ? guide->mix.as<float4>() : nullptr;
         pb.grec = guide ? guide->path_records.as<float4>() : nullptr;
+        lp.guided_bounces = guide ? guide->cfg.guided_bounces : 0;
+        lp.grec_out = guide ? guide->records.as<float4>() : nullptr;
+        lp.grec_slots = guide ? std::max<uint32_t>(1, guide->record_slots) : 1u;
         hipStream_t s = lane.stream;
         const int gblocks = lane.grid_blocks;
         if (gblocks < static_cast<int>(kQueueSegs)) {  // every queue segment needs a block of its own
@@ -1447,16 +1690,21 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         const bool count = settings->count_visits != 0;
         uint32_t* spill = lane.spill;
         const SceneView view = scene->view;
-        // One path vertex for every path in flight: closest hit, shade, shadow rays, queue rotation.
-        auto bounce = [&](int cur, int it) {
-            const int guide_bounce = guide && lp.guide_fraction > 0.0f && it < guide->cfg.guided_bounces ? 1 : 0;
+        // One iteration: closest hit of the continuation rays, new samples on the free ids, shade, shadow rays,
+        // counter rotation.
+        auto iterate = [&](int cur) {
+            wf->prof.begin(kKRegen, s);
+            if (guide) hipLaunchKernelGGL(k_regen<true>, dim3(gblocks), dim3(kBlock), 0, s, lp, pb, cur);
+            else hipLaunchKernelGGL(k_regen<false>, dim3(gblocks), dim3(kBlock), 0, s, lp, pb, cur);
+            wf->prof.end(s);
             wf->prof.begin(kKIntersect, s);
             if (count) hipLaunchKernelGGL(k_intersect<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
             else hipLaunchKernelGGL(k_intersect<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, cur, spill);
             wf->prof.end(s);
-            if (guide_bounce) {
+            if (guide && lp.guide_fraction > 0.0f) {
                 wf->prof.begin(kKGuide, s);
-                const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, seg_counter(pb.ctr, kCtrCur, 0), pb.seg_cap, pb.mix};
+                const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, seg_counter(pb.ctr, kCtrCur, 0), pb.seg_cap, pb.mix,
+                                      static_cast<uint32_t>(std::max(0, lp.guided_bounces))};
                 GuideView gv = guide->view(lp.guide_fraction);
                 gv.inferred = pb.stats + 7;
                 hipLaunchKernelGGL(k_guide_infer<RenderQueries>, dim3(std::max(16, dev->cu_count * 2)),
@@ -1465,11 +1713,14 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             }
             wf->prof.begin(kKShade, s);
             if (lp.transport) {
-                if (guide) hipLaunchKernelGGL((k_shade<true, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, guide_bounce);
-                else hipLaunchKernelGGL((k_shade<true, false>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, 0);
+                if (guide) hipLaunchKernelGGL((k_shade<true, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+                else hipLaunchKernelGGL((k_shade<true, false>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
             } else {
-                if (guide) hipLaunchKernelGGL((k_shade<false, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur, guide_bounce);
-                else hipLaunchKernelGGL(k_shade_plain, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+                if (guide) {
+                    hipLaunchKernelGGL((k_shade<false, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+                } else {
+                    hipLaunchKernelGGL(k_shade_plain, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+                }
             }
             wf->prof.end(s);
             wf->prof.begin(kKShadow, s);
@@ -1486,7 +1737,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S, spill);
             }
             wf->prof.end(s);
-            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kQueueSegs), 0, s, pb);
+            hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kQueueSegs), 0, s, pb, lp.num_paths);
         };
         uint64_t traced = 0;
         bool cancelled = false;
@@ -1539,41 +1790,54 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             lp.first_sample = first_sample + done;
             lp.spl = spl;
             lp.num_paths = lp.num_pixels * static_cast<uint32_t>(spl);
-            // Guiding: the first launch samples the BSDF only and collects; later ones mix in the trained guide.
+            // Guiding: the first window samples the BSDF only and collects; later ones mix in the trained guide.
             lp.guide_fraction = guide && guide->steps > 0 ? guide->cfg.guide_fraction : 0.0f;
             lp.guide_records = guide && guide->training() ? 1 : 0;
+            lp.grec_salt = guide ? static_cast<uint64_t>(guide->launches) * 0x9e3779b97f4a7c15ull : 0ull;
             if (lp.guide_records) {
                 rc = guide_clear_records(guide, s);
                 if (rc != PE_OK) return rc;
             }
-            const int gen_blocks = static_cast<int>(std::min<uint32_t>((lp.num_paths + kBlock - 1) / kBlock, 65535u));
+            // The window's pool: as many ids per segment as the pool size allows, and no more than the segment
+            // has samples.
+            const uint32_t ids_per_seg = std::min<uint32_t>(wf->seg_cap, (pool + kQueueSegs - 1) / kQueueSegs);
+            const int init_blocks =
+                static_cast<int>(std::min<uint64_t>((uint64_t{ids_per_seg} * kQueueSegs + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKGenerate, s);
-            hipLaunchKernelGGL(k_generate, dim3(gen_blocks), dim3(kBlock), 0, s, lp, pb);
+            hipLaunchKernelGGL(k_window_init, dim3(init_blocks), dim3(kBlock), 0, s, lp, pb, ids_per_seg);
             wf->prof.end(s);
-            // max_depth + 1 vertices per path. Continuations that are not vertices (past a camera-invisible
-            // light, through a thin wall, the steps of a subsurface walk) lengthen a path, so the loop runs on,
-            // checking the queue on the host every few bounces, until every path has ended. The bound is what
-            // the reference's own limits allow: 256 walk steps per vertex, 64 pass-throughs, 64 skips.
-            int it = 0;
-            const int planned = lp.max_depth + 1;
-            const int limit = planned * static_cast<int>(kMaxWalkSteps + 1) + 2 * 64 + 8;
-            for (;;) {
-                for (; it < planned; ++it) bounce(it & 1, it);
-                uint32_t left = 0;
-                PE_HIP_CHECK(hipMemcpyAsync(&left, pb.ctr + kCtrTotal, 4, hipMemcpyDeviceToHost, s));
-                PE_HIP_CHECK(hipStreamSynchronize(s));
-                if (left == 0) break;
+            // Iterations in chunks; after each chunk the remaining-work counter is copied to pinned memory and an
+            // event recorded. The host reads a chunk's counter while the next chunk runs, so the device never
+            // waits for the host; the chunk queued after the last one needed runs as empty kernels. Every path
+            // ends within the reference's own limits (max_bounces + 1 vertices, 256 walk steps per vertex, 64
+            // pass-throughs, 64 camera skips), so the iteration count is bounded.
+            const int per_path = (lp.max_depth + 1) * static_cast<int>(kMaxWalkSteps + 1) + 2 * 64 + 8;
+            const uint64_t generations = lp.num_paths / std::max<uint32_t>(1, wf->capacity) + 2;
+            const uint64_t limit = static_cast<uint64_t>(per_path) * generations;
+            uint64_t it = 0;
+            auto enqueue_chunk = [&](int slot, int iters) -> int {
+                for (int k = 0; k < iters; ++k, ++it) iterate(static_cast<int>(it & 1));
+                PE_HIP_CHECK(hipMemcpyAsync(&wf->host_live[slot], pb.ctr + kCtrTotal, 4, hipMemcpyDeviceToHost, s));
+                PE_HIP_CHECK(hipEventRecord(wf->chunk_done[slot], s));
+                return PE_OK;
+            };
+            rc = enqueue_chunk(0, kChunkIterations);
+            if (rc != PE_OK) return rc;
+            for (int n = 1;; ++n) {
+                rc = enqueue_chunk(n & 1, kChunkIterations);
+                if (rc != PE_OK) return rc;
+                PE_HIP_CHECK(hipEventSynchronize(wf->chunk_done[(n - 1) & 1]));
+                if (wf->host_live[(n - 1) & 1] == 0) break;
+                if (cancel && *cancel) {
+                    cancelled = true;
+                    break;
+                }
                 if (it > limit) {
                     set_error("pe_render: paths still alive past the bounce, walk and pass-through limits");
                     return PE_ERR_INVALID;
                 }
-                for (int k = 0; k < 4; ++k, ++it) bounce(it & 1, it);
             }
-            if (lp.guide_records) {
-                const int cb = static_cast<int>(std::min<uint32_t>((lp.num_paths + kBlock - 1) / kBlock, 65535u));
-                hipLaunchKernelGGL(k_guide_collect, dim3(cb), dim3(kBlock), 0, s, lp, pb, guide->records.as<float4>(),
-                                   guide->record_slots, static_cast<uint64_t>(guide->launches) * 0x9e3779b97f4a7c15ull);
-            }
+            if (cancelled) break;  // the window's samples are dropped: the film keeps whole windows only
             const int film_blocks = static_cast<int>(std::min<uint32_t>((lp.num_pixels + kBlock - 1) / kBlock, 65535u));
             wf->prof.begin(kKFilm, s);
             hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, film->sum.as<float>(),
@@ -1593,7 +1857,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             done += spl;
             traced += static_cast<uint64_t>(lp.num_pixels) * spl;
         }
-        unsigned long long st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned long long st[kStatSlots] = {};
         PE_HIP_CHECK(hipMemcpyAsync(st, pb.stats, sizeof(st), hipMemcpyDeviceToHost, s));
         PE_HIP_CHECK(hipStreamSynchronize(s));
         if (result) {
@@ -1605,6 +1869,14 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             result->tri_visits = st[3];
             result->shadow_node_visits = st[4];
             result->shadow_tri_visits = st[5];
+            result->max_stack_depth = static_cast<uint32_t>(st[8]);
+            result->stack_overflows = static_cast<uint32_t>(std::min<unsigned long long>(st[9], 0xFFFFFFFFull));
+        }
+        if (st[9] != 0) {
+            set_error("pe_render: " + std::to_string(st[9]) + " traversals overflowed the " +
+                      std::to_string(kStackDepth) + "-entry stack (a BVH deeper than the build allows); the film "
+                      "holds wrong hits");
+            return PE_ERR_INVALID;
         }
         if (guide) {
             guide->collected += st[6];

@@ -64,6 +64,8 @@ struct pe_device {
     pe::DevBuf spill;            // traversal stack spill, (kStackDepth - kLdsDepth) entries per lane
     uint32_t spill_lanes = 0;    // lanes the spill buffer is sized for
     pe::DevBuf scratch;          // small per-call staging (counters)
+    pe::DevBuf diag;             // traversal diagnostics of the batch entry points: deepest stack, overflows
+    unsigned long long diag_host[4] = {0, 0, 0, 0};  // ... as copied back after the last batch
     int max_grid_blocks = 0;     // persistent-grid size for 256-lane blocks
     void* wavefront = nullptr;   // integrator buffers (integrator.hip), grown on demand
 };
@@ -97,6 +99,7 @@ struct pe_film {
 };
 
 namespace pe {
+constexpr size_t kDiagBytes = 32;  // deepest stack, overflows, the batch queue's chunk counter
 // Persistent grid size and the traversal-stack spill buffer sized for it (capi.hip).
 int ensure_trace_resources(pe_device* dev, int& blocks);
 // Releases the integrator's buffers of a device (integrator.hip).
@@ -106,8 +109,10 @@ void wavefront_release(pe_device* dev);
 struct Wavefront;
 Wavefront* wavefront_new();
 void wavefront_delete(Wavefront* wf);
-int wavefront_reserve(Wavefront* wf, uint32_t paths, int shadow_quality);
+// A pool of `paths` path ids and a result buffer of `results` samples.
+int wavefront_reserve(Wavefront* wf, uint32_t paths, uint64_t results, int shadow_quality);
 uint64_t wavefront_bytes_per_path(int shadow_quality);
+uint64_t wavefront_device_bytes(const Wavefront* wf);
 
 // One render stream: a HIP stream with its own path state and traversal-stack spill (grid_blocks x 256 lanes).
 struct RenderLane {

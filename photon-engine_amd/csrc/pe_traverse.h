@@ -19,9 +19,11 @@
 // ordered by entry distance (a four-element sorting network) and all but the nearest are pushed, farthest
 // first. A lane whose ray has ended takes the next ray of its wave's share at the top of the next step (a
 // ballot ranks the idle lanes), and the new ray's record is loaded together with the root node, so lanes stay
-// busy until the wave's share is used up. A queue is one or more segments (QueueSegs); wave w works on segment
-// w % segs, and its share is every Wk-th 64-ray chunk of that segment from chunk w / segs (Wk = the waves of
-// the segment): cheap and expensive regions of a segment are spread over its waves without an atomic counter.
+// busy until the wave's share is used up. A queue is one or more segments (QueueSegs); wave w starts on segment
+// w % segs. With chunk counters the waves take the segment's 64-ray chunks one at a time (one atomic per chunk,
+// each segment's counter on a line of its own) and, once it is used up, the next segments' chunks: the kernel
+// ends when the last chunk anywhere is done, not when the slowest segment is. Without them a wave's share is
+// every Wk-th chunk of its segment from chunk w / segs (Wk = the waves of the segment), with no atomics.
 //
 // Item encoding (stack entries are items too): an interior node is its index; a leaf run is
 // kLeafItem | (remaining triangles - 1) << 27 | first triangle, so a leaf holds at most 16 triangles and
@@ -53,12 +55,17 @@ constexpr uint64_t kMaxTraceTriangles = uint64_t{1} << 27;
 // apart, so no two share a cache line): the appends of the whole grid would otherwise serialise on one address.
 // A plain array of n rays is one segment (counts null).
 constexpr uint32_t kQueueSegs = 64;
+static_assert(kQueueSegs <= 64, "traversal checks every segment with one lane of a wave");
 constexpr uint32_t kSegStride = 64;
 struct QueueSegs {
     const uint32_t* counts;  // counts[k * kSegStride]; null: one segment of n items
     uint32_t n;
     uint32_t seg_cap;
     uint32_t segs;
+    // Chunk counters (next[k * kSegStride], zero before the kernel): traversal hands out 64-ray chunks of each
+    // segment through them, and a wave whose segment is used up moves on to the others. Null: every wave of a
+    // segment takes a fixed interleaved share.
+    uint32_t* next = nullptr;
     __device__ __forceinline__ uint32_t count(uint32_t k) const { return counts ? counts[k * kSegStride] : n; }
 };
 
@@ -77,13 +84,21 @@ struct TraceStack {
     uint32_t* spill;    // this lane's column in the spill buffer: entry e at spill[(e - kLdsDepth) * spill_stride]
     uint32_t spill_stride;
     int sp;
+    int max_sp = 0;         // deepest stack of this lane's traversals (kept by the counting variants)
+    bool overflow = false;  // a push found the stack full and dropped its entry
 
     // The build bounds the tree depth below kStackDepth, so a push never finds the stack full; were it to,
-    // the entry is dropped rather than written out of bounds.
+    // the entry is dropped rather than written out of bounds, and the lane's overflow flag reports it (the
+    // kernels count flagged lanes into a device counter that the host turns into an error).
     __device__ __forceinline__ void push(uint32_t v) {
-        if (sp < kLdsDepth) lds[sp * kTraceBlock] = v;
-        else if (sp < kStackDepth) spill[static_cast<size_t>(sp - kLdsDepth) * spill_stride] = v;
-        else return;
+        if (sp < kLdsDepth) {
+            lds[sp * kTraceBlock] = v;
+        } else if (sp < kStackDepth) {
+            spill[static_cast<size_t>(sp - kLdsDepth) * spill_stride] = v;
+        } else {
+            overflow = true;
+            return;
+        }
         ++sp;
     }
     __device__ __forceinline__ uint32_t pop() {
@@ -92,6 +107,19 @@ struct TraceStack {
         return spill[static_cast<size_t>(sp - kLdsDepth) * spill_stride];
     }
 };
+
+// Traversal diagnostics of a kernel, two 64-bit device counters: [0] the deepest stack any lane reached
+// (counting variants only), [1] lanes whose stack overflowed. One atomic per wave.
+__device__ __forceinline__ void report_stack(unsigned long long* diag, const TraceStack& st, bool with_depth) {
+    if (!diag) return;
+    const unsigned long long over = __ballot(st.overflow);
+    if (with_depth) {
+        int m = st.max_sp;
+        for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off));
+        if (__lane_id() == 0 && m > 0) atomicMax(diag, static_cast<unsigned long long>(m));
+    }
+    if (__lane_id() == 0 && over) atomicAdd(diag + 1, static_cast<unsigned long long>(__popcll(over)));
+}
 
 // One ray of a queue, decoded by the queue's job.
 struct TraceRay {
@@ -189,6 +217,10 @@ __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, 
     const uint32_t nchunks = (n + 63u) >> 6;
     const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
     uint32_t taken = 0;  // rays of this wave's share handed out so far (wave-uniform)
+    // Chunk hand-out (q.next): the segment the wave draws from, the current chunk's next slot and rays left in
+    // it (all wave-uniform). At most 64 segments.
+    uint32_t cseg = sh.seg, cbase = 0, cleft = 0;
+    bool drained = false;
     uint32_t slot = kNoRay;
     uint32_t item = 0;
     bool fresh = false;
@@ -199,8 +231,52 @@ __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, 
     int32_t htri = -1;
     for (;;) {
         // Idle lanes take the next rays of the wave's share, in lane order.
-        const uint64_t idle = __ballot(slot == kNoRay);
-        if (idle != 0 && w + (taken >> 6) * W < nchunks) {
+        uint64_t idle = __ballot(slot == kNoRay);
+        if (q.next) {
+            while (idle != 0 && !drained) {
+                if (cleft == 0) {
+                    // A chunk of the current segment; once that is used up, one round trip tells which segments
+                    // still have chunks (lane k looks at segment k) and the wave moves to the first of them from
+                    // its current one on, or ends when none has.
+                    const uint32_t lane = __lane_id();
+                    uint32_t nk = q.count(cseg), nck = (nk + 63u) >> 6;
+                    uint32_t c = nck;
+                    if (lane == 0 && *reinterpret_cast<volatile uint32_t*>(q.next + cseg * kSegStride) < nck)
+                        c = atomicAdd(q.next + cseg * kSegStride, 1u);
+                    c = __builtin_amdgcn_readfirstlane(__shfl(c, 0));
+                    if (c < nck) {
+                        cbase = cseg * q.seg_cap + (c << 6);
+                        cleft = min(64u, nk - (c << 6));
+                        continue;
+                    }
+                    bool open = false;
+                    if (lane < q.segs) {
+                        const uint32_t nl = (q.count(lane) + 63u) >> 6;
+                        open = *reinterpret_cast<volatile uint32_t*>(q.next + lane * kSegStride) < nl;
+                    }
+                    const uint64_t avail = __ballot(open);
+                    if (avail == 0) {
+                        drained = true;
+                        continue;
+                    }
+                    const uint64_t rot = cseg == 0 ? avail : (avail >> cseg) | (avail << (64u - cseg));
+                    cseg = __builtin_amdgcn_readfirstlane((cseg + static_cast<uint32_t>(__ffsll(static_cast<long long>(rot)) - 1)) & 63u);
+                    continue;
+                }
+                const uint32_t rank = static_cast<uint32_t>(__popcll(idle & below));
+                if (slot == kNoRay && rank < cleft) {
+                    slot = cbase + rank;
+                    job.fetch(slot, ra, rb);
+                    fresh = true;
+                    item = 0;
+                    st.sp = 0;
+                }
+                const uint32_t given = min(cleft, static_cast<uint32_t>(__popcll(idle)));
+                cbase += given;
+                cleft -= given;
+                idle = __ballot(slot == kNoRay);
+            }
+        } else if (idle != 0 && w + (taken >> 6) * W < nchunks) {
             if (slot == kNoRay) {
                 const uint32_t j = taken + static_cast<uint32_t>(__popcll(idle & below));
                 const uint32_t c = w + (j >> 6) * W;
@@ -285,6 +361,7 @@ __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, 
             if (cnt > 2) st.push(ci[2]);
             if (cnt > 1) st.push(ci[1]);
             if (cnt > 0) next = ci[0];
+            if (kCount) st.max_sp = max(st.max_sp, st.sp);
         }
         if (next == kNoItem && !occluded && st.sp > 0) next = st.pop();
         if (occluded || next == kNoItem) {

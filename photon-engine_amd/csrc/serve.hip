@@ -295,7 +295,8 @@ int pe_server_create(pe_device* dev, const pe_server_config* config, pe_server**
             lane.wf = wavefront_new();
             if (!lane.spill.alloc(trace_spill_bytes(dev->max_grid_blocks))) return fail(PE_ERR_OOM);
             // Path state for the largest chunk, reserved up front so lanes do not allocate while others render.
-            const int rc = wavefront_reserve(lane.wf, static_cast<uint32_t>(c.max_chunk_paths), 1);
+            const int rc = wavefront_reserve(lane.wf, static_cast<uint32_t>(std::min<uint64_t>(c.max_chunk_paths, 1u << 24)),
+                                             c.max_chunk_paths, 1);
             if (rc != PE_OK) return fail(rc);
         }
         for (auto& lane : s->lanes) lane->thread = std::thread(lane_main, s.get(), lane.get());

@@ -95,13 +95,14 @@ class Settings(C.Structure):
                 ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
                 ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("count_visits", C.c_int32),
                 ("adaptive", C.c_int32), ("noise_threshold", C.c_float), ("min_adaptive_samples", C.c_int32),
-                ("reserved", C.c_int32 * 3)]
+                ("paths_in_flight", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class BatchResult(C.Structure):
     _fields_ = [("completed", C.c_int32), ("samples_traced", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_visits", C.c_uint64),
-                ("shadow_node_visits", C.c_uint64), ("shadow_tri_visits", C.c_uint64)]
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_visits", C.c_uint64),
+                ("max_stack_depth", C.c_uint32), ("stack_overflows", C.c_uint32)]
 
 
 class Targets(C.Structure):
@@ -195,6 +196,7 @@ SIGNATURES = {
                                  _P(C.c_uint32), _P(C.c_uint32)]),
     "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
     "pe_occluded_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_uint8)]),
+    "pe_trace_diag": (C.c_int, [_VP, _P(C.c_uint32), _P(C.c_uint64)]),
     "pe_guide_create": (C.c_int, [_VP, _VP, _P(GuideConfig), _P(_VP)]),
     "pe_guide_destroy": (C.c_int, [_VP]),
     "pe_guide_get_params": (C.c_int, [_VP, _P(C.c_float)]),

@@ -25,12 +25,12 @@ def camera_perspective(position, target, up, vfov, aspect) -> capi.Camera:
 
 def settings(max_bounces=8, rr_depth=3, shadow_quality=1, filter_type=capi.PE_FILTER_BOX, filter_width=1.0,
              seed=12345, samples_per_launch=1, clamp_indirect=0.0, adaptive=False, noise_threshold=0.02,
-             min_adaptive_samples=16) -> capi.Settings:
+             min_adaptive_samples=16, paths_in_flight=0) -> capi.Settings:
     """RenderSettings as pe_settings (the reference's defaults for the adaptive fields, render_settings.h:30-32)."""
     s = capi.Settings()
     s.max_bounces, s.rr_depth, s.shadow_quality = max_bounces, rr_depth, shadow_quality
     s.filter_type, s.filter_width, s.clamp_indirect = filter_type, filter_width, clamp_indirect
-    s.seed, s.samples_per_launch = seed, samples_per_launch
+    s.seed, s.samples_per_launch, s.paths_in_flight = seed, samples_per_launch, paths_in_flight
     s.adaptive, s.noise_threshold, s.min_adaptive_samples = int(bool(adaptive)), noise_threshold, min_adaptive_samples
     return s
 
@@ -69,6 +69,12 @@ class Device:
                                                          ptr(py, C.c_int32), ptr(index, C.c_uint32),
                                                          ptr(dim, C.c_uint32), ptr(out, C.c_uint32)))
         return out
+
+    def trace_diag(self) -> tuple[int, int]:
+        """(deepest traversal stack, stack overflows) of the device's last closest-hit / any-hit batch."""
+        depth, over = C.c_uint32(), C.c_uint64()
+        check("pe_trace_diag", self._lib.pe_trace_diag(self.h, C.byref(depth), C.byref(over)))
+        return depth.value, over.value
 
     def upload(self, scene) -> "GpuScene":
         return GpuScene(self, scene)

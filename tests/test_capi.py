@@ -61,7 +61,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
 
 def test_abi_version_and_device_registry_without_gpu():
     lib = capi.load()
-    assert lib.pe_abi_version() == 1
+    assert lib.pe_abi_version() == 2
     n = C.c_int(-1)
     assert lib.pe_device_count(C.byref(n)) == capi.PE_OK
     assert n.value >= 0
