@@ -320,6 +320,15 @@ int pe_sobol_batch(pe_device* dev, uint64_t seed, size_t n, const int32_t* px, c
  * each = t (float), shape index (int, -1 = miss), prim index (int, -1 for standalone shapes), and the
  * global triangle index (int). bary (optional, may be NULL): 2 floats u, v per ray. */
 int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, int32_t* hits, float* bary);
+/* pe_intersect_batch that allocates nothing: rays_dev, hits_dev and bary_dev (optional) are device buffers the
+ * caller owns (same layouts as pe_intersect_batch), and the traversal's stack spill and counters come from a
+ * fixed device scratch of 29,360,384 bytes per GPU, reserved for the process's lifetime; the grid is capped at
+ * 256 blocks of 256 lanes so that the spill fits it (a batch that would need more is refused). No counterpart
+ * in the reference: it is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). Calls that overlap on one GPU share its scratch and race.
+ * Synchronous on the device's stream; at most 2^31 - 1 rays. */
+int pe_intersect_batch_noalloc(pe_device* dev, pe_scene* scene, size_t n, const float* rays_dev, int32_t* hits_dev,
+                               float* bary_dev);
 /* Any hit (shadow rays): occluded[i] = 1 when something lies within [tmin, tmax]. */
 int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* rays, uint8_t* occluded);
 /* Traversal diagnostics of the device's last pe_intersect_batch / pe_occluded_batch: the deepest stack any of

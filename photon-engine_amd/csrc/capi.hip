@@ -128,6 +128,23 @@ __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* 
     report_stack(diag, st, true);
 }
 
+// ---------------------------------------------------------------------------------------------------------
+// Device scratch of pe_intersect_batch_noalloc: its traversal diagnostics (the first 256 bytes: deepest stack,
+// overflows, chunk counter) and the traversal-stack spill of a grid of at most kNoAllocBlocks blocks. One copy
+// per GPU, held until the process ends. Host code reaches it only through the address hipGetSymbolAddress gave
+// for the device (pe_device::noalloc_scratch), never through the symbol's name.
+constexpr uint32_t kNoAllocBlocks = 256;
+constexpr size_t kNoAllocDiagBytes = 256;
+constexpr size_t kNoAllocSpillBytes = size_t{kNoAllocBlocks} * kTraceBlock * (kStackDepth - kLdsDepth) * sizeof(uint32_t);
+constexpr size_t kScratchBytes = kNoAllocDiagBytes + kNoAllocSpillBytes;  // 29,360,384 bytes
+static_assert(kDiagBytes <= kNoAllocDiagBytes, "diagnostics fit the scratch's head");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Clears the diagnostics counters (a kernel, not a memset node, so that a caller may capture the call).
+__global__ void k_clear_diag(unsigned long long* __restrict__ diag) {
+    if (threadIdx.x < kDiagBytes / sizeof(unsigned long long)) diag[threadIdx.x] = 0ull;
+}
+
 // Persistent grid for 256-lane blocks and the spill buffer sized for it.
 int ensure_trace_resources(pe_device* dev, int& blocks) {
     blocks = dev->max_grid_blocks;
@@ -253,6 +270,15 @@ int pe_device_create(int ordinal, pe_device** out) {
     dev->cu_count = info.compute_units;
     dev->max_grid_blocks = std::max(1, dev->cu_count) * 8;
     PE_HIP_CHECK(hipStreamCreateWithFlags(&dev->stream, hipStreamNonBlocking));
+    // This device's copy of the no-allocation scratch (with the device current).
+    void* scratch_ptr = nullptr;
+    const hipError_t se = hipGetSymbolAddress(&scratch_ptr, HIP_SYMBOL(scratch));
+    if (se != hipSuccess || !scratch_ptr) {
+        (void)hipStreamDestroy(dev->stream);
+        set_error(std::string("pe_device_create: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(se));
+        return PE_ERR_HIP;
+    }
+    dev->noalloc_scratch = scratch_ptr;
     *out = dev.release();
     return PE_OK;
     PE_GUARD_END
@@ -443,6 +469,56 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
     PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
     if (bary) PE_HIP_CHECK(hipMemcpyAsync(bary, dbary.get(), n * 8, hipMemcpyDeviceToHost, dev->stream));
     return finish_batch(dev, "pe_intersect_batch");
+    PE_GUARD_END
+}
+
+// pe_intersect_batch on caller-owned device buffers, with no device allocation: the stack spill and the
+// counters come from this device's copy of `scratch`, and the grid is capped so that the spill fits it. Calls
+// that overlap on one device share that scratch and race.
+int pe_intersect_batch_noalloc(pe_device* dev, pe_scene* scene, size_t n, const float* rays_dev, int32_t* hits_dev,
+                               float* bary_dev) {
+    PE_GUARD_BEGIN
+    if (!dev || !scene || scene->dev != dev || (n > 0 && (!rays_dev || !hits_dev))) {
+        set_error("pe_intersect_batch_noalloc: NULL argument or scene of another device");
+        return PE_ERR_INVALID;
+    }
+    if (n == 0) return PE_OK;
+    if (n >= (size_t{1} << 31)) {
+        set_error("pe_intersect_batch_noalloc: at most 2^31 - 1 rays per batch");
+        return PE_ERR_INVALID;
+    }
+    if (!dev->noalloc_scratch) {
+        set_error("pe_intersect_batch_noalloc: the device has no scratch address");
+        return PE_ERR_INVALID;
+    }
+    const int blocks = std::min(grid_for(dev, n), static_cast<int>(kNoAllocBlocks));
+    const size_t need = kNoAllocDiagBytes + static_cast<size_t>(blocks) * kTraceBlock * (kStackDepth - kLdsDepth) * sizeof(uint32_t);
+    if (need > kScratchBytes) {
+        set_error("pe_intersect_batch_noalloc: the traversal needs " + std::to_string(need) + " bytes of scratch, above the " +
+                  std::to_string(kScratchBytes) + "-byte maximum");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    unsigned char* base = static_cast<unsigned char*>(dev->noalloc_scratch);
+    unsigned long long* diag = reinterpret_cast<unsigned long long*>(base);
+    uint32_t* spill = reinterpret_cast<uint32_t*>(base + kNoAllocDiagBytes);
+    hipLaunchKernelGGL(k_clear_diag, dim3(1), dim3(64), 0, dev->stream, diag);
+    BatchClosest job{};
+    job.r = reinterpret_cast<const float4*>(rays_dev);
+    job.shade = scene->tri_shade.as<TriShade>();
+    job.hits = reinterpret_cast<int4*>(hits_dev);
+    job.bary = reinterpret_cast<float2*>(bary_dev);
+    hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), spill, diag);
+    PE_HIP_CHECK(hipGetLastError());
+    PE_HIP_CHECK(hipMemcpyAsync(dev->diag_host, diag, kDiagBytes, hipMemcpyDeviceToHost, dev->stream));
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    if (dev->diag_host[1] != 0) {
+        set_error("pe_intersect_batch_noalloc: " + std::to_string(dev->diag_host[1]) + " traversals overflowed the " +
+                  std::to_string(kStackDepth) + "-entry stack");
+        return PE_ERR_INVALID;
+    }
+    return PE_OK;
     PE_GUARD_END
 }
 

@@ -66,6 +66,7 @@ struct pe_device {
     pe::DevBuf scratch;          // small per-call staging (counters)
     pe::DevBuf diag;             // traversal diagnostics of the batch entry points: deepest stack, overflows
     unsigned long long diag_host[4] = {0, 0, 0, 0};  // ... as copied back after the last batch
+    void* noalloc_scratch = nullptr;  // this device's address of the no-allocation scratch (capi.hip)
     int max_grid_blocks = 0;     // persistent-grid size for 256-lane blocks
     void* wavefront = nullptr;   // integrator buffers (integrator.hip), grown on demand
 };

@@ -197,6 +197,7 @@ SIGNATURES = {
     "pe_intersect_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_int32), _P(C.c_float)]),
     "pe_occluded_batch": (C.c_int, [_VP, _VP, C.c_size_t, _P(C.c_float), _P(C.c_uint8)]),
     "pe_trace_diag": (C.c_int, [_VP, _P(C.c_uint32), _P(C.c_uint64)]),
+    "pe_intersect_batch_noalloc": (C.c_int, [_VP, _VP, C.c_size_t, _VP, _VP, _VP]),
     "pe_guide_create": (C.c_int, [_VP, _VP, _P(GuideConfig), _P(_VP)]),
     "pe_guide_destroy": (C.c_int, [_VP]),
     "pe_guide_get_params": (C.c_int, [_VP, _P(C.c_float)]),
