@@ -162,7 +162,7 @@ def prism_mesh(center, size, height, subdiv: int = 1) -> Mesh:
 def glass_studio(width: int = 64, height: int = 48, spp: int = 16, max_bounces: int = 16, led_quads: int = 0,
                  prism_subdiv: int = 2) -> tuple[Scene, View]:
     """Transmissive studio after the spectral/volume workload: a dispersive prism (Dielectric 1.5, Abbe 30), an
-    absorbing liquid block (Dielectric 1.33 with setAbsorption), a subsurface block (Disney subsurface 1,
+    absorbing liquid block (Dielectric 1.33, sigma_a U[0.1, 2] / cm), a subsurface block (Disney subsurface 1,
     mean free path 1-5 mm), a thin-walled pane, a rough glass sphere and a Disney material with specular and
     diffuse transmission, on a Lambert floor under an area light (plus an LED spiral of `led_quads` emissive
     quads when asked)."""
@@ -176,7 +176,10 @@ def glass_studio(width: int = 64, height: int = 48, spp: int = 16, max_bounces: 
     sc.add_shape(_quad((-4, 0, -1.5), (4, 0, -1.5), (4, 3, -1.5), (-4, 3, -1.5)), back)
     prism = sc.add_material(dielectric(1.5, abbe=30.0))
     sc.add_shape(prism_mesh((-1.1, 0.002, 0.0), 0.45, 0.9, prism_subdiv), prism)
-    liquid = sc.add_material(dielectric(1.33, absorption_color=rng.uniform(0.2, 0.9, 3), absorption_distance=0.3))
+    # Tinted liquid: sigma_a ~ U[0.1, 2] per cm per channel (U[10, 200] per m; scene units are metres), given as
+    # Dielectric::setAbsorption(exp(-sigma_a * 1 cm), 1 cm) (dielectric.cpp:123-144 recovers sigma_a).
+    sigma_a = rng.uniform(10.0, 200.0, 3)
+    liquid = sc.add_material(dielectric(1.33, absorption_color=np.exp(-sigma_a * 0.01), absorption_distance=0.01))
     sc.add_shape(box_mesh((-0.45, 0.002, -0.2), (0.05, 0.6, 0.3)), liquid)
     sss = sc.add_material(disney((0.85, 0.55, 0.4), 0.0, 0.4, 0.5, subsurface=1.0,
                                  subsurface_radius=rng.uniform(1.0, 5.0, 3), subsurface_scale=1e-3))

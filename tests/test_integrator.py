@@ -121,7 +121,8 @@ def test_gpu_radiance_matches_oracle_per_sample(dev, name):
                             np.full(xx.size, sample))
     Lg = s.reshape(-1, 3)
     close = np.all(np.abs(Lg - Lo) <= 1e-4 * np.maximum(1.0, np.abs(Lo)), axis=1)
-    assert close.mean() >= 0.97, close.mean()
+    print(f"{name}: per-sample agreement with the oracle {close.mean():.5f} of {close.size} samples", flush=True)
+    assert close.mean() >= 0.999, close.mean()
     assert np.abs(Lg.mean(0) - Lo.mean(0)).max() <= 0.02 * max(1e-3, Lo.mean())
 
 
@@ -194,6 +195,25 @@ def test_gpu_converged_image_matches_reference_renderer(dev, name):
     mean_r, cnt_r, _ = bindings.RefScene(sc, strict=False).render(view, 0, spp, threads=16)
     assert (cnt_r == spp).all()
     e = relmse(gpu, mean_r)
+    assert e <= 1e-3, e
+
+
+@pytest.mark.gpu
+def test_gpu_full_size_box_icosphere_matches_reference_renderer(dev):
+    """The small plain workload at its full size: 512 x 512, 64 spp, depth 8, the level-5 icosphere (20,480
+    triangles) in the area-lit box, against the reference's own renderer on the same samples (seed 12345,
+    Renderer::renderSamples with the reference's -O3 -ffast-math build): relMSE <= 1e-3."""
+    if bindings.reference(strict=False) is None:
+        pytest.skip("oracle/_ref not built (reference tree absent when this tree was built)")
+    sc, view = configs.box_icosphere(level=5)
+    assert (view.width, view.height, view.spp, view.max_bounces) == (512, 512, 64, 8)
+    s, c, _, rej, res = _gpu_image(dev, sc, view, view.spp, spl=64)
+    assert (c == view.spp).all() and rej == 0
+    gpu = s / c[..., None]
+    mean_r, cnt_r, _ = bindings.RefScene(sc, strict=False).render(view, 0, view.spp, threads=16)
+    assert (cnt_r == view.spp).all()
+    e = relmse(gpu, mean_r)
+    print(f"512x512x64 box + icosphere: relMSE vs the reference renderer {e:.3e}", flush=True)
     assert e <= 1e-3, e
 
 

@@ -34,9 +34,11 @@ VIEW = View((0.2, 1.3, 3.4), (0.0, 0.45, 0.0), (0.0, 1.0, 0.0), 45.0, 64, 48, 16
 FEATURES = {
     "dispersive_prism": lambda sc: sc.add_shape(prism_mesh((-1.1, 0.002, 0.0), 0.45, 0.9, 2),
                                                 sc.add_material(dielectric(1.5, abbe=30.0))),
+    # sigma_a = (0.1, 0.45, 1.2) per cm, inside the lights + media workload's U[0.1, 2] / cm (scene units: metres),
+    # as setAbsorption(exp(-sigma_a * 1 cm), 1 cm).
     "absorbing_liquid": lambda sc: sc.add_shape(box_mesh((-0.45, 0.002, -0.2), (0.05, 0.6, 0.3)),
-                                                sc.add_material(dielectric(1.33, absorption_color=(0.3, 0.6, 0.8),
-                                                                           absorption_distance=0.3))),
+                                                sc.add_material(dielectric(1.33, absorption_color=np.exp(
+                                                    -np.array([10.0, 45.0, 120.0]) * 0.01), absorption_distance=0.01))),
     "subsurface_block": lambda sc: sc.add_shape(box_mesh((0.3, 0.002, -0.25), (0.8, 0.5, 0.25)),
                                                 sc.add_material(disney((0.85, 0.55, 0.4), 0.0, 0.4, 0.5, subsurface=1.0,
                                                                        subsurface_radius=(2, 3, 4),
@@ -182,6 +184,7 @@ def _gpu_per_sample(dev, sc, view, sample=3):
     # Same Sobol samples, same paths: a sample differs only where a device transcendental (exp, log, sin) one
     # ulp away from the host's flips a later decision.
     close = np.all(np.abs(Lg - Lo) <= 1e-4 * np.maximum(1.0, np.abs(Lo)), axis=1)
+    print(f"per-sample agreement with the oracle: {close.mean():.5f} of {close.size} samples", flush=True)
     return close.mean(), Lg, Lo
 
 
@@ -189,7 +192,7 @@ def _gpu_per_sample(dev, sc, view, sample=3):
 @pytest.mark.parametrize("name", list(FEATURES))
 def test_gpu_radiance_matches_oracle_per_sample(dev, name):
     frac, Lg, Lo = _gpu_per_sample(dev, feature_scene(name), VIEW)
-    assert frac >= 0.95, frac
+    assert frac >= 0.998, frac
     assert Lo.max() > 0
 
 
@@ -197,7 +200,7 @@ def test_gpu_radiance_matches_oracle_per_sample(dev, name):
 def test_gpu_glass_studio_matches_oracle_per_sample(dev):
     sc, view = configs.glass_studio(led_quads=100)
     frac, _, _ = _gpu_per_sample(dev, sc, view)
-    assert frac >= 0.99, frac
+    assert frac >= 0.998, frac
 
 
 @pytest.mark.gpu
@@ -218,4 +221,5 @@ def test_gpu_image_matches_oracle_relmse(dev, name):
     st = settings(max_bounces=view.max_bounces, seed=12345)
     (so, co, _), _, _ = bindings.OracleScene(sc).render(_cam(view), st, view.width, view.height, 0, spp, threads=8)
     ora = so / co[..., None]
+    print(f"{name}: relMSE vs oracle at {spp} spp = {relmse(gpu, ora):.3e}", flush=True)
     assert relmse(gpu, ora) <= 1e-3, relmse(gpu, ora)
