@@ -399,20 +399,39 @@ int pe_render_guided(pe_device* dev, pe_scene* scene, pe_guide* guide, const pe_
  * run on the device together and a job that arrives while others render starts as soon as a lane frees up.
  * A job's chunks run one after another in sample order and their boundaries depend on the job and the
  * server configuration only, so a job's film is the same bit for bit whether it ran alone or among others.
- * Scenes are uploaded by the caller (pe_scene_upload) and stay resident across jobs. */
+ * Device memory: the lanes hold their path pools, result buffers and traversal-stack spill for the server's
+ * life; the memory budget covers the rest, the films of admitted jobs and the scenes kept resident. Scenes
+ * are uploaded by the caller (pe_scene_upload) and stay resident across jobs; when the head of the queue does
+ * not fit, the server parks idle scenes (no admitted job renders them; least recently used first) in host
+ * memory, and restores a parked scene when a job that needs it is admitted. A scene cannot be released while
+ * a job that references it is not released (PE_ERR_INVALID); pe_scene_update waits for the renders of the
+ * scene in progress and renders wait for it. */
 typedef struct pe_server pe_server;
 
 typedef struct pe_server_config {
     int32_t slots;              /* render lanes (streams) working at once; 0 = 4 */
     int32_t chunk_spp;          /* samples of every pixel per chunk after the preview; 0 = 16 */
     uint64_t max_chunk_paths;   /* paths one chunk keeps in flight (splits chunks of large films); 0 = 2^25 */
-    uint64_t film_budget;       /* device bytes for the films of admitted jobs; 0 = an eighth of free memory */
+    uint64_t memory_budget;     /* device bytes for the films of admitted jobs and the resident scenes; 0 = three
+                                   quarters of the device's memory less what the lanes hold */
     int32_t max_admitted;       /* jobs holding a film at once; 0 = 64 */
-    int32_t reserved[5];
+    int32_t lane_paths;         /* path ids each lane keeps in flight (pe_settings.paths_in_flight); 0 = 2^24 */
+    int32_t reserved[4];
 } pe_server_config;
 
+/* Server memory and residency counters. */
+typedef struct pe_server_stats {
+    uint64_t memory_budget;     /* as configured or defaulted */
+    uint64_t memory_used;       /* films of admitted jobs and resident scenes */
+    uint64_t lane_bytes;        /* path pools, result buffers and stack spill of the lanes */
+    uint64_t scenes_parked;     /* scenes moved to host memory so far */
+    uint64_t scenes_restored;   /* ... and brought back */
+    int32_t admitted, queued;   /* jobs holding a film / waiting for admission */
+    int32_t scenes_tracked, scenes_resident;
+} pe_server_stats;
+
 typedef struct pe_job_desc {
-    pe_scene* scene;            /* an uploaded scene; it must outlive the job's last chunk */
+    pe_scene* scene;            /* an uploaded scene; it cannot be released before the job is */
     pe_camera camera;
     pe_settings settings;       /* samples_per_launch is ignored: a chunk is one launch */
     int32_t width, height;      /* film size */
@@ -442,6 +461,7 @@ int pe_server_create(pe_device* dev, const pe_server_config* config, pe_server**
 int pe_server_destroy(pe_server* server);
 /* Seconds since the server was created (the clock of pe_job_status). */
 double pe_server_clock(const pe_server* server);
+int pe_server_stats_get(pe_server* server, pe_server_stats* out);
 int pe_job_submit(pe_server* server, const pe_job_desc* job, uint64_t* id);
 int pe_job_poll(pe_server* server, uint64_t id, pe_job_status* status);
 /* Waits until the job is done, cancelled or failed (timeout_ms < 0: no limit); PE_ERR_TIMEOUT otherwise. */

@@ -182,6 +182,80 @@ int grid_for(pe_device* dev, size_t n) {
 
 }  // namespace pe
 
+namespace pe {
+
+int scene_park(pe_scene& sc) {
+    if (!sc.resident) return PE_OK;
+    PE_HIP_CHECK(hipSetDevice(sc.dev->ordinal));
+    std::vector<DevBuf*> bufs = sc.buffers();
+    std::vector<std::vector<unsigned char>> host(bufs.size());
+    for (size_t k = 0; k < bufs.size(); ++k) {
+        host[k].resize(bufs[k]->bytes());
+        if (!host[k].empty())
+            PE_HIP_CHECK(hipMemcpyAsync(host[k].data(), bufs[k]->get(), host[k].size(), hipMemcpyDeviceToHost, sc.dev->stream));
+    }
+    PE_HIP_CHECK(hipStreamSynchronize(sc.dev->stream));
+    for (DevBuf* b : bufs) b->release();
+    sc.parked = std::move(host);
+    sc.resident = false;
+    return PE_OK;
+}
+
+// The view's device pointers, from the scene's buffers (its other fields do not depend on where they live).
+static void rebind_view(pe_scene& sc) {
+    SceneView& v = sc.view;
+    v.nodes = sc.nodes.as<BvhNode4>();
+    v.tris = sc.tris.as<TriAccel>();
+    v.shade = sc.tri_shade.as<TriShade>();
+    v.pos = sc.positions.as<float4>();
+    v.nrm = sc.normals.as<float4>();
+    v.uv = sc.uvs.as<float2>();
+    v.materials = sc.materials.as<GpuMaterial>();
+    v.lights = sc.lights.as<GpuLight>();
+    v.tables = sc.tables.as<EnergyTables>();
+    v.light_nodes = sc.light_nodes.as<GpuLightNode>();
+    v.light_paths = sc.light_paths.as<LightPath>();
+    v.env.rgb = sc.env_rgb.as<float>();
+    v.env.func = sc.env_func.as<float>();
+    v.env.marginal = sc.env_marginal.as<float>();
+    v.env.cond = sc.env_cond.as<float>();
+    v.env.marginal_guide = sc.env_mguide.as<uint32_t>();
+    v.env.cond_guide = sc.env_cguide.as<uint32_t>();
+}
+
+int scene_restore(pe_scene& sc) {
+    if (sc.resident) return PE_OK;
+    PE_HIP_CHECK(hipSetDevice(sc.dev->ordinal));
+    std::vector<DevBuf*> bufs = sc.buffers();
+    for (size_t k = 0; k < bufs.size(); ++k) {
+        if (!bufs[k]->upload(sc.parked[k].data(), sc.parked[k].size(), sc.dev->stream)) {
+            for (size_t j = 0; j <= k; ++j) bufs[j]->release();
+            return PE_ERR_OOM;
+        }
+    }
+    PE_HIP_CHECK(hipStreamSynchronize(sc.dev->stream));
+    rebind_view(sc);
+    sc.parked.clear();
+    sc.resident = true;
+    return PE_OK;
+}
+
+void scene_swap_contents(pe_scene& a, pe_scene& b) {
+    std::vector<DevBuf*> ba = a.buffers(), bb = b.buffers();
+    for (size_t k = 0; k < ba.size(); ++k) std::swap(*ba[k], *bb[k]);
+    std::swap(a.num_tris, b.num_tris);
+    std::swap(a.num_verts, b.num_verts);
+    std::swap(a.num_shapes, b.num_shapes);
+    std::swap(a.bvh_depth, b.bvh_depth);
+    std::swap(a.view, b.view);
+    std::swap(a.num_nodes, b.num_nodes);
+    std::swap(a.max_material_ref, b.max_material_ref);
+    std::swap(a.max_emitter_ref, b.max_emitter_ref);
+    std::swap(a.broken, b.broken);
+}
+
+}  // namespace pe
+
 using namespace pe;
 
 #define PE_GUARD_BEGIN try {
@@ -346,7 +420,9 @@ int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
     }
     pe_device* dev = scene->dev;
     PE_HIP_CHECK(hipSetDevice(dev->ordinal));
-    // Renders already queued on the device's stream finish with the old data.
+    // Renders in progress on any stream (pe_render, server lanes) finish with the old data first; none starts
+    // until the update is done.
+    std::unique_lock<std::shared_mutex> lk(scene->use);
     PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
     if (kind == PE_UPDATE_FULL || kind == PE_UPDATE_TRANSFORMS) {
         auto fresh = std::make_unique<pe_scene>();
@@ -354,8 +430,14 @@ int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
         const int rc = upload_scene(dev, *desc, *fresh);
         if (rc != PE_OK) return rc;  // the old scene is untouched
         PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
-        std::swap(*scene, *fresh);
+        scene_swap_contents(*scene, *fresh);
+        scene->resident = true;
+        scene->parked.clear();
         return PE_OK;
+    }
+    if (!scene->resident) {  // a parked scene comes back before a partial rebuild
+        const int rc = scene_restore(*scene);
+        if (rc != PE_OK) return rc;
     }
     if (kind != PE_UPDATE_MATERIALS && kind != PE_UPDATE_LIGHTS) {
         set_error("pe_scene_update: unknown update kind " + std::to_string(kind));
@@ -388,8 +470,21 @@ int pe_scene_update(pe_scene* scene, int kind, const pe_scene_desc* desc) {
 
 int pe_scene_release(pe_scene* scene) {
     if (!scene) return PE_OK;
-    (void)hipSetDevice(scene->dev->ordinal);
-    (void)hipStreamSynchronize(scene->dev->stream);
+    if (const int refs = scene->job_refs.load(); refs > 0) {
+        set_error("pe_scene_release: " + std::to_string(refs) + " server job(s) still reference the scene; release them first");
+        return PE_ERR_INVALID;
+    }
+    std::vector<pe_server*> trackers;
+    {
+        std::lock_guard<std::mutex> g(scene->trackers_mu);
+        trackers.swap(scene->trackers);
+    }
+    for (pe_server* s : trackers) server_forget_scene(s, scene);
+    {
+        std::unique_lock<std::shared_mutex> lk(scene->use);  // no render of it is in progress
+        (void)hipSetDevice(scene->dev->ordinal);
+        (void)hipStreamSynchronize(scene->dev->stream);
+    }
     delete scene;
     return PE_OK;
 }

@@ -1561,6 +1561,22 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         set_error("pe_render: scene or film belongs to another device");
         return PE_ERR_INVALID;
     }
+    // The scene stays as it is for the whole render: updates, parking and release wait for it (and a scene a
+    // render-job server has parked in host memory comes back first).
+    std::shared_lock<std::shared_mutex> scene_use(scene->use);
+    for (int attempt = 0; !scene->resident; ++attempt) {
+        scene_use.unlock();
+        {
+            std::unique_lock<std::shared_mutex> ex(scene->use);
+            const int rc = scene_restore(*scene);
+            if (rc != PE_OK) return rc;
+        }
+        scene_use.lock();
+        if (attempt > 8) {
+            set_error("pe_render: the scene keeps being parked");
+            return PE_ERR_INVALID;
+        }
+    }
     if (scene->broken) {
         set_error("pe_render: the scene's last update failed; upload it again");
         return PE_ERR_INVALID;

@@ -7,8 +7,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -73,6 +76,13 @@ struct pe_device {
 
 struct pe_scene {
     pe_device* dev = nullptr;
+    // Use by renders and servers (not part of the scene's contents: kept across Full / Transforms updates).
+    std::shared_mutex use;                 // renders hold it shared; updates, parking and release exclusively
+    std::atomic<int> job_refs{0};          // server jobs that reference the scene and are not released
+    std::mutex trackers_mu;
+    std::vector<pe_server*> trackers;      // servers that account for the scene's residency
+    bool resident = true;                  // device buffers present (false: parked in host memory)
+    std::vector<std::vector<unsigned char>> parked;  // the device buffers' bytes while parked
     uint64_t num_tris = 0;
     uint64_t num_verts = 0;
     int num_shapes = 0;
@@ -85,6 +95,11 @@ struct pe_scene {
     int max_material_ref = -1;  // largest material index a mesh uses
     int max_emitter_ref = -1;   // largest light index a mesh's emitter range reaches
     bool broken = false;        // an update failed part-way: the scene must be uploaded again
+    // The device buffers, in a fixed order (parking and restoring walk them).
+    std::vector<pe::DevBuf*> buffers() {
+        return {&nodes, &tris, &tri_shade, &positions, &normals, &uvs, &materials, &lights, &tables, &env_rgb,
+                &env_func, &env_marginal, &env_cond, &env_mguide, &env_cguide, &light_nodes, &light_paths};
+    }
     uint64_t device_bytes() const {
         return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes() +
                materials.bytes() + lights.bytes() + tables.bytes() + env_rgb.bytes() + env_func.bytes() +
@@ -93,6 +108,8 @@ struct pe_scene {
     }
 };
 
+struct pe_server;
+
 struct pe_film {
     pe_device* dev = nullptr;
     int width = 0, height = 0;
@@ -100,6 +117,14 @@ struct pe_film {
 };
 
 namespace pe {
+// Scene residency (capi.hip): parking copies a scene's device buffers to host memory and frees them; restoring
+// uploads them again and points the scene's view at the new buffers. Callers hold scene.use exclusively.
+int scene_park(pe_scene& sc);
+int scene_restore(pe_scene& sc);
+// Device data of a + b exchanged (a Full / Transforms update swaps in a freshly built scene).
+void scene_swap_contents(pe_scene& a, pe_scene& b);
+// A server stops accounting for a scene (serve.hip; called when the scene is released).
+void server_forget_scene(pe_server* s, pe_scene* sc);
 constexpr size_t kDiagBytes = 32;  // deepest stack, overflows, the batch queue's chunk counter
 // Persistent grid size and the traversal-stack spill buffer sized for it (capi.hip).
 int ensure_trace_resources(pe_device* dev, int& blocks);

@@ -9,6 +9,7 @@
 // lane instead of behind whole jobs. A job never has two chunks in flight, and its chunk boundaries depend
 // only on the job and the server configuration: its film is the same bit for bit alone and in any mix.
 #include "photon_hip.h"
+#include "pe_admission.h"
 #include "pe_internal.h"
 
 #include <algorithm>
@@ -20,6 +21,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 #include <vector>
@@ -44,7 +46,8 @@ struct Job {
     double t_submit = -1.0, t_admit = -1.0, t_preview = -1.0, t_done = -1.0;
     uint64_t extension_rays = 0, shadow_rays = 0;
     bool running = false;       // a lane renders one of its chunks
-    bool copying = false;       // pe_job_download is reading the film
+    int copying = 0;            // pe_job_download / pe_job_display calls reading the film
+    bool uses_scene = false;    // counted as a user of its scene by the admission planner
     volatile int32_t cancel = 0;  // polled by render_samples between launches
 
     bool ended() const { return state == PE_JOB_DONE || state == PE_JOB_CANCELLED || state == PE_JOB_FAILED; }
@@ -70,7 +73,10 @@ struct pe_server {
     std::vector<std::unique_ptr<Lane>> lanes;
     hipStream_t copy_stream = nullptr;
     uint64_t next_id = 1;
-    uint64_t film_used = 0;
+    pe::ResidencyPlanner planner;  // films of admitted jobs and resident scenes within the memory budget
+    std::set<pe_scene*> tracked;   // scenes whose residency the server accounts for
+    uint64_t lane_bytes = 0;       // device bytes the lanes hold (path pools, results, stack spill)
+    uint64_t parks = 0, restores = 0;
     int admitted = 0;            // jobs holding a film
     bool stop = false;
 
@@ -82,8 +88,36 @@ namespace {
 
 uint64_t film_bytes_of(int w, int h) { return static_cast<uint64_t>(w) * static_cast<uint64_t>(h) * 20 + 8; }
 
-// Admits queued jobs in submission order while the head fits; the head is never overtaken, so a large film
-// waits for memory instead of starving. Called with the lock held.
+uint64_t scene_key(const pe_scene* sc) { return reinterpret_cast<uintptr_t>(sc); }
+
+// Device bytes of a scene, resident or parked.
+uint64_t scene_bytes(pe_scene& sc) {
+    if (sc.resident) return sc.device_bytes();
+    uint64_t b = 0;
+    for (const auto& v : sc.parked) b += v.size();
+    return b;
+}
+
+// The scene's current size and residency into the planner (an update or a direct render may have changed them).
+void refresh(pe_server& s, pe_scene* sc) {
+    std::shared_lock<std::shared_mutex> lk(sc->use);
+    s.planner.track(scene_key(sc), scene_bytes(*sc), sc->resident);
+}
+
+// Starts accounting for a scene (with the lock held).
+void track_scene(pe_server& s, pe_scene* sc) {
+    if (s.tracked.insert(sc).second) {
+        std::lock_guard<std::mutex> g(sc->trackers_mu);
+        sc->trackers.push_back(&s);
+    }
+    refresh(s, sc);
+}
+
+void end_scene_use(pe_server& s, Job& j);
+
+// Admits queued jobs in submission order while the head fits the memory budget, parking idle scenes (least
+// recently used first) to make room and restoring the head's scene if it was parked; the head is never
+// overtaken, so a large job waits for memory instead of starving. Called with the lock held.
 void admit(pe_server& s) {
     while (!s.queue.empty()) {
         Job& j = *s.jobs.at(s.queue.front());
@@ -93,13 +127,31 @@ void admit(pe_server& s) {
             s.queue.pop_front();
             continue;
         }
-        if (s.admitted >= s.cfg.max_admitted || s.film_used + j.film_bytes > s.cfg.film_budget) return;
+        if (s.admitted >= s.cfg.max_admitted) return;
+        for (pe_scene* sc : s.tracked) refresh(s, sc);
+        pe_scene* scene = j.desc.scene;
+        const ResidencyPlanner::Plan plan = s.planner.plan(j.film_bytes, scene_key(scene));
+        if (!plan.admit) return;
+        int rc = PE_OK;
+        for (uint64_t key : plan.evict) {
+            pe_scene* victim = reinterpret_cast<pe_scene*>(static_cast<uintptr_t>(key));
+            std::unique_lock<std::shared_mutex> lk(victim->use);
+            rc = scene_park(*victim);
+            if (rc != PE_OK) break;
+            ++s.parks;
+        }
+        if (rc == PE_OK && plan.restore) {
+            std::unique_lock<std::shared_mutex> lk(scene->use);
+            rc = scene_restore(*scene);
+            if (rc == PE_OK) ++s.restores;
+        }
         auto f = std::make_unique<pe_film>();
         f->dev = s.dev;
         f->width = j.desc.width;
         f->height = j.desc.height;
         const size_t px = static_cast<size_t>(f->width) * f->height;
-        bool ok = f->sum.alloc(px * 12) && f->count.alloc(px * 4) && f->lum2.alloc(px * 4) && f->rejected.alloc(8);
+        bool ok = rc == PE_OK && f->sum.alloc(px * 12) && f->count.alloc(px * 4) && f->lum2.alloc(px * 4) &&
+                  f->rejected.alloc(8);
         if (ok) {
             ok = hipMemsetAsync(f->sum.get(), 0, px * 12, s.copy_stream) == hipSuccess &&
                  hipMemsetAsync(f->count.get(), 0, px * 4, s.copy_stream) == hipSuccess &&
@@ -109,17 +161,20 @@ void admit(pe_server& s) {
             if (!ok) set_error("film clear failed");
         }
         s.queue.pop_front();
+        // The planner follows what happened on the device, whatever the outcome.
+        for (pe_scene* sc : s.tracked) refresh(s, sc);
         if (!ok) {
             j.state = PE_JOB_FAILED;
-            j.error = PE_ERR_OOM;
+            j.error = rc != PE_OK ? rc : PE_ERR_OOM;
             j.message = pe_last_error();
             j.t_done = s.now();
             continue;
         }
+        s.planner.commit(ResidencyPlanner::Plan{true, false, false, {}}, j.film_bytes, scene_key(scene));
+        j.uses_scene = true;
         j.film = std::move(f);
         j.state = PE_JOB_ADMITTED;
         j.t_admit = s.now();
-        s.film_used += j.film_bytes;
         ++s.admitted;
     }
 }
@@ -138,10 +193,11 @@ Job* pick(pe_server& s) {
     Job* best = nullptr;
     for (auto& kv : s.jobs) {
         Job& j = *kv.second;
-        if (j.state != PE_JOB_ADMITTED || j.running || j.copying) continue;
+        if (j.state != PE_JOB_ADMITTED || j.running || j.copying > 0) continue;
         if (j.cancel) {
             j.state = PE_JOB_CANCELLED;
             j.t_done = s.now();
+            end_scene_use(s, j);
             s.cv.notify_all();
             continue;
         }
@@ -201,6 +257,10 @@ void lane_main(pe_server* s, Lane* lane) {
             j->message = msg;
             j->t_done = t;
         }
+        if (j->ended()) {
+            end_scene_use(*s, *j);
+            admit(*s);  // its scene may now be parked for a queued job
+        }
         s->cv.notify_all();
     }
 }
@@ -230,14 +290,29 @@ Job* find(pe_server* s, uint64_t id) {
     return it->second.get();
 }
 
+// An admitted job stopped rendering: its scene may be parked once no other admitted job renders it.
+void end_scene_use(pe_server& s, Job& j) {
+    if (!j.uses_scene) return;
+    j.uses_scene = false;
+    s.planner.end_use(scene_key(j.desc.scene));
+}
+
 void release_film(pe_server& s, Job& j) {
+    end_scene_use(s, j);
     if (!j.film) return;
     j.film.reset();  // frees after the device is idle (hipFree waits for it)
-    s.film_used -= j.film_bytes;
+    s.planner.free_film(j.film_bytes);
     --s.admitted;
 }
 
 }  // namespace
+
+void server_forget_scene(pe_server* s, pe_scene* sc) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->planner.forget(scene_key(sc));
+    s->tracked.erase(sc);
+}
+
 }  // namespace pe
 
 using namespace pe;
@@ -264,12 +339,8 @@ int pe_server_create(pe_device* dev, const pe_server_config* config, pe_server**
         if (c.max_chunk_paths == 0) c.max_chunk_paths = uint64_t{1} << 25;
         c.max_chunk_paths = std::min<uint64_t>(c.max_chunk_paths, uint64_t{1} << 30);
         if (c.max_admitted == 0) c.max_admitted = 64;
+        if (c.lane_paths <= 0) c.lane_paths = 1 << 24;
         PE_HIP_CHECK(hipSetDevice(dev->ordinal));
-        if (c.film_budget == 0) {
-            size_t free_b = 0, total_b = 0;
-            PE_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
-            c.film_budget = free_b / 8;
-        }
         s->cfg = c;
         // Undoes a partial construction (no lane thread has started yet).
         auto fail = [&](int rc) {
@@ -294,11 +365,24 @@ int pe_server_create(pe_device* dev, const pe_server_config* config, pe_server**
             }
             lane.wf = wavefront_new();
             if (!lane.spill.alloc(trace_spill_bytes(dev->max_grid_blocks))) return fail(PE_ERR_OOM);
-            // Path state for the largest chunk, reserved up front so lanes do not allocate while others render.
-            const int rc = wavefront_reserve(lane.wf, static_cast<uint32_t>(std::min<uint64_t>(c.max_chunk_paths, 1u << 24)),
-                                             c.max_chunk_paths, 1);
+            // The lane's path pool and the results of its largest chunk, reserved up front so lanes do not
+            // allocate while others render.
+            const int rc = wavefront_reserve(
+                lane.wf, static_cast<uint32_t>(std::min<uint64_t>(c.max_chunk_paths, static_cast<uint64_t>(c.lane_paths))),
+                c.max_chunk_paths, 1);
             if (rc != PE_OK) return fail(rc);
+            s->lane_bytes += wavefront_device_bytes(lane.wf) + lane.spill.bytes();
         }
+        // The memory budget of films and resident scenes: as given, or three quarters of the device's memory
+        // less what the lanes hold.
+        if (c.memory_budget == 0) {
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return fail(PE_ERR_HIP);
+            const uint64_t share = total_b / 4 * 3;
+            c.memory_budget = share > s->lane_bytes ? share - s->lane_bytes : 0;
+        }
+        s->cfg = c;
+        s->planner = ResidencyPlanner(c.memory_budget);
         for (auto& lane : s->lanes) lane->thread = std::thread(lane_main, s.get(), lane.get());
         *out = s.release();
         return PE_OK;
@@ -325,13 +409,38 @@ int pe_server_destroy(pe_server* s) {
         lane->spill.release();
         (void)hipStreamDestroy(lane->stream);
     }
+    for (auto& kv : s->jobs) kv.second->desc.scene->job_refs.fetch_sub(1);
     s->jobs.clear();
+    for (pe_scene* sc : s->tracked) {
+        std::lock_guard<std::mutex> g(sc->trackers_mu);
+        sc->trackers.erase(std::remove(sc->trackers.begin(), sc->trackers.end(), s), sc->trackers.end());
+    }
+    s->tracked.clear();
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     delete s;
     return PE_OK;
 }
 
 double pe_server_clock(const pe_server* s) { return s ? s->now() : -1.0; }
+
+int pe_server_stats_get(pe_server* s, pe_server_stats* out) {
+    if (!s || !out) {
+        set_error("pe_server_stats_get: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = pe_server_stats{};
+    out->memory_budget = s->planner.budget();
+    out->memory_used = s->planner.used();
+    out->lane_bytes = s->lane_bytes;
+    out->scenes_parked = s->parks;
+    out->scenes_restored = s->restores;
+    out->admitted = s->admitted;
+    out->queued = static_cast<int32_t>(s->queue.size());
+    out->scenes_tracked = static_cast<int32_t>(s->tracked.size());
+    for (pe_scene* sc : s->tracked) out->scenes_resident += sc->resident ? 1 : 0;
+    return PE_OK;
+}
 
 int pe_job_submit(pe_server* s, const pe_job_desc* job, uint64_t* id) {
     if (!s || !job || !id) {
@@ -357,16 +466,18 @@ int pe_job_submit(pe_server* s, const pe_job_desc* job, uint64_t* id) {
             set_error("pe_job_submit: server is shutting down");
             return PE_ERR_INVALID;
         }
-        if (j->film_bytes > s->cfg.film_budget) {
-            set_error("pe_job_submit: the film is larger than the server's film budget");
+        (void)hipSetDevice(s->dev->ordinal);
+        track_scene(*s, job->scene);
+        if (s->planner.plan(j->film_bytes, scene_key(job->scene)).never) {
+            set_error("pe_job_submit: the film and the scene together exceed the server's memory budget");
             return PE_ERR_OOM;
         }
+        job->scene->job_refs.fetch_add(1);
         j->id = s->next_id++;
         j->t_submit = s->now();
         *id = j->id;
         s->queue.push_back(j->id);
         s->jobs.emplace(j->id, std::move(j));
-        (void)hipSetDevice(s->dev->ordinal);
         admit(*s);
         s->cv.notify_all();
         return PE_OK;
@@ -425,6 +536,7 @@ int pe_job_cancel(pe_server* s, uint64_t id) {
         // Not on a lane: it ends now; a running chunk ends it when the lane returns.
         j->state = PE_JOB_CANCELLED;
         j->t_done = s->now();
+        end_scene_use(*s, *j);
         if (!s->queue.empty()) s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), id), s->queue.end());
     }
     s->cv.notify_all();
@@ -444,7 +556,7 @@ int pe_job_download(pe_server* s, uint64_t id, float* sum_rgb, uint32_t* count, 
         set_error("pe_job_download: the job holds no film (queued, or failed before admission)");
         return PE_ERR_INVALID;
     }
-    j->copying = true;
+    ++j->copying;
     pe_film* f = j->film.get();
     lk.unlock();
     const size_t px = static_cast<size_t>(f->width) * f->height;
@@ -454,7 +566,7 @@ int pe_job_download(pe_server* s, uint64_t id, float* sum_rgb, uint32_t* count, 
     if (e == hipSuccess && sum_lum2) e = hipMemcpyAsync(sum_lum2, f->lum2.get(), px * 4, hipMemcpyDeviceToHost, s->copy_stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
     lk.lock();
-    j->copying = false;
+    --j->copying;
     s->cv.notify_all();
     if (e != hipSuccess) {
         set_error(std::string("pe_job_download: ") + hipGetErrorString(e));
@@ -476,12 +588,12 @@ int pe_job_display(pe_server* s, uint64_t id, const pe_display* settings, float*
         set_error("pe_job_display: the job holds no film (queued, or failed before admission)");
         return PE_ERR_INVALID;
     }
-    j->copying = true;
+    ++j->copying;
     pe_film* f = j->film.get();
     lk.unlock();
     const int rc = pe_film_display(f, settings, display_rgb, rgb8);
     lk.lock();
-    j->copying = false;
+    --j->copying;
     s->cv.notify_all();
     return rc;
 }
@@ -495,11 +607,12 @@ int pe_job_release(pe_server* s, uint64_t id) {
     Job* j = find(s, id);
     if (!j) return PE_ERR_INVALID;
     j->cancel = 1;
-    s->cv.wait(lk, [&] { return !j->running && !j->copying; });
+    s->cv.wait(lk, [&] { return !j->running && j->copying == 0; });
     if (j->state == PE_JOB_QUEUED)
         s->queue.erase(std::remove(s->queue.begin(), s->queue.end(), id), s->queue.end());
     (void)hipSetDevice(s->dev->ordinal);
     release_film(*s, *j);
+    j->desc.scene->job_refs.fetch_sub(1);
     s->jobs.erase(id);
     admit(*s);  // the freed film memory may admit the next queued job
     s->cv.notify_all();

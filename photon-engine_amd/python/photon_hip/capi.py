@@ -140,7 +140,14 @@ PE_GUIDE_PARAMS = 26944
 
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_int32), ("chunk_spp", C.c_int32), ("max_chunk_paths", C.c_uint64),
-                ("film_budget", C.c_uint64), ("max_admitted", C.c_int32), ("reserved", C.c_int32 * 5)]
+                ("memory_budget", C.c_uint64), ("max_admitted", C.c_int32), ("lane_paths", C.c_int32),
+                ("reserved", C.c_int32 * 4)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("memory_used", C.c_uint64), ("lane_bytes", C.c_uint64),
+                ("scenes_parked", C.c_uint64), ("scenes_restored", C.c_uint64), ("admitted", C.c_int32),
+                ("queued", C.c_int32), ("scenes_tracked", C.c_int32), ("scenes_resident", C.c_int32)]
 
 
 class JobDesc(C.Structure):
@@ -216,6 +223,7 @@ SIGNATURES = {
     "pe_job_cancel": (C.c_int, [_VP, C.c_uint64]),
     "pe_job_download": (C.c_int, [_VP, C.c_uint64, _P(C.c_float), _P(C.c_uint32), _P(C.c_float)]),
     "pe_job_release": (C.c_int, [_VP, C.c_uint64]),
+    "pe_server_stats_get": (C.c_int, [_VP, _VP]),
 }
 
 _lib = None

@@ -35,14 +35,15 @@ def _status(st: capi.JobStatus) -> JobStatus:
 
 
 class Server:
-    """Admits render jobs by film memory and interleaves their chunks on `slots` render lanes."""
+    """Admits render jobs by device memory (films and resident scenes) and interleaves their chunks on `slots`
+    render lanes."""
 
-    def __init__(self, dev, slots=0, chunk_spp=0, max_chunk_paths=0, film_budget=0, max_admitted=0):
+    def __init__(self, dev, slots=0, chunk_spp=0, max_chunk_paths=0, memory_budget=0, max_admitted=0, lane_paths=0):
         self.dev = dev
         self._lib = dev._lib
         cfg = capi.ServerConfig()
         cfg.slots, cfg.chunk_spp, cfg.max_chunk_paths = slots, chunk_spp, max_chunk_paths
-        cfg.film_budget, cfg.max_admitted = film_budget, max_admitted
+        cfg.memory_budget, cfg.max_admitted, cfg.lane_paths = memory_budget, max_admitted, lane_paths
         h = C.c_void_p()
         check("pe_server_create", self._lib.pe_server_create(dev.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -50,6 +51,11 @@ class Server:
 
     def clock(self) -> float:
         return self._lib.pe_server_clock(self.h)
+
+    def stats(self) -> dict:
+        st = capi.ServerStats()
+        check("pe_server_stats_get", self._lib.pe_server_stats_get(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in capi.ServerStats._fields_}
 
     def submit(self, gscene, camera: capi.Camera, st: capi.Settings, width: int, height: int, spp: int,
                preview_spp: int = 0) -> int:

@@ -3,8 +3,9 @@ film sizes and sample counts on one device. Its model is RenderSession's batch l
 render_session.cpp:69-101) run for many sessions at once.
 
 What is checked: a job's film is bit for bit the same alone and in a mix of concurrent jobs, and the same as
-pe_render over the job's chunk boundaries; admission waits for film memory; a short job's preview overtakes
-a long job's later chunks; cancel, release and the error paths.
+pe_render over the job's chunk boundaries; admission waits for device memory (films and resident scenes) and parks idle scenes in host memory to make
+room, restoring them bit for bit; a short job's preview overtakes a long job's later chunks; a scene cannot be
+released while jobs reference it; cancel, release and the error paths.
 """
 from __future__ import annotations
 
@@ -83,7 +84,9 @@ def test_admission_waits_for_film_memory(dev):
     (sc, view) = _scenes()[0]
     gs = dev.upload(sc)
     one_film = view.width * view.height * 20 + 8
-    srv = Server(dev, slots=2, chunk_spp=4, max_chunk_paths=CHUNK_PATHS, film_budget=one_film)
+    # The budget covers the (resident) scene and one film.
+    srv = Server(dev, slots=2, chunk_spp=4, max_chunk_paths=CHUNK_PATHS,
+                 memory_budget=gs.stats()["device_bytes"] + one_film)
     try:
         j1 = _submit(srv, gs, view, spp=8)
         j2 = _submit(srv, gs, view, spp=8)
@@ -94,6 +97,7 @@ def test_admission_waits_for_film_memory(dev):
         assert s2.state == capi.PE_JOB_DONE and s2.t_admit >= 0
         big = srv.try_submit(gs, _cam(view), settings(), view.width * 2, view.height, 4)
         assert big[0] == capi.PE_ERR_OOM                      # can never fit the budget
+        srv.release(j2)
     finally:
         srv.close()
 
@@ -166,3 +170,89 @@ def test_job_display_is_the_display_of_its_film(dev):
     np.testing.assert_array_equal(job_disp, disp)
     np.testing.assert_array_equal(job8, rgb8)
     assert 0.05 < disp.mean() < 0.95
+
+
+@pytest.mark.gpu
+def test_idle_scene_is_parked_for_room_and_restored_bit_for_bit(dev):
+    """A budget that holds one scene besides a film: a job on the second scene parks the first (idle), a later
+    job on the first restores it, and its film equals the first job's bit for bit."""
+    (sc_a, view_a), (sc_b, view_b) = _scenes()[1], _scenes()[0]
+    ga, gb = dev.upload(sc_a), dev.upload(sc_b)
+    bytes_a, bytes_b = ga.stats()["device_bytes"], gb.stats()["device_bytes"]
+    film = max(view_a.width * view_a.height, view_b.width * view_b.height) * 20 + 8
+    srv = Server(dev, slots=2, chunk_spp=4, max_chunk_paths=CHUNK_PATHS,
+                 memory_budget=max(bytes_a, bytes_b) + min(bytes_a, bytes_b) // 2 + film)
+    try:
+        j1 = _submit(srv, ga, view_a, spp=8)
+        assert srv.wait(j1, 60000).state == capi.PE_JOB_DONE
+        first = srv.download(j1)
+        srv.release(j1)
+        j2 = _submit(srv, gb, view_b, spp=8)                  # room for scene b: scene a (idle) is parked
+        assert srv.wait(j2, 60000).state == capi.PE_JOB_DONE
+        st = srv.stats()
+        assert st["scenes_parked"] == 1 and st["scenes_resident"] == 1, st
+        assert st["memory_used"] <= st["memory_budget"], st
+        srv.release(j2)
+        j3 = _submit(srv, ga, view_a, spp=8)                  # scene a back: b is parked in turn
+        assert srv.wait(j3, 60000).state == capi.PE_JOB_DONE
+        again = srv.download(j3)
+        st = srv.stats()
+        assert st["scenes_restored"] == 1 and st["scenes_parked"] == 2, st
+        for x, y in zip(first, again):
+            np.testing.assert_array_equal(x, y)
+        # A scene referenced by a job that is not released cannot be released; pe_render restores a parked scene.
+        assert capi.load().pe_scene_release(ga.h) == capi.PE_ERR_INVALID
+        srv.release(j3)
+        film_b = dev.film(view_b.width, view_b.height)
+        render(dev, gb, _cam(view_b), settings(max_bounces=view_b.max_bounces, seed=12345, samples_per_launch=8),
+               film_b, 0, 8)
+        assert (film_b.download()[1] == 8).all()
+    finally:
+        srv.close()
+
+
+@pytest.mark.gpu
+def test_serving_stream_at_workload_sizes(dev):
+    """The serving workload's shapes on the device: a seeded Poisson stream that holds a 2048 x 2048 film, a
+    512-spp job and the 5.1M-triangle displaced sphere beside the small plain scene. Every job completes, every
+    preview lands before its job completes, and each of three jobs re-rendered alone afterwards (the 5M-sphere
+    job among them) gives the same film bit for bit as in the mix."""
+    rng = np.random.default_rng(3)
+    big_sc, big_view = configs.product_sphere(lon=1600, lat=1600, env_width=512, env_height=256)
+    assert big_sc.num_triangles > 5_000_000
+    small_sc, small_view = configs.box_icosphere(level=5)
+    g_big, g_small = dev.upload(big_sc), dev.upload(small_sc)
+    jobs = [(g_big, big_view, 2048, 2048, 24), (g_small, small_view, 512, 512, 512), (g_big, big_view, 256, 256, 64),
+            (g_small, small_view, 256, 256, 16), (g_big, big_view, 1920, 1080, 32), (g_small, small_view, 512, 512, 40)]
+    arrivals = np.cumsum(rng.exponential(1.0 / 20.0, len(jobs)))
+    srv = Server(dev, slots=4, chunk_spp=16)
+    try:
+        import time
+
+        t0 = time.perf_counter()
+        ids = []
+        for (gs, view, w, h, spp), at in zip(jobs, arrivals):
+            while time.perf_counter() - t0 < at:
+                time.sleep(0.001)
+            st = settings(max_bounces=view.max_bounces, seed=12345)
+            cam = camera_perspective(view.position, view.target, view.up, view.vfov, w / h)
+            ids.append(srv.submit(gs, cam, st, w, h, spp, 16))
+        stats = [srv.wait(j, 300000) for j in ids]
+        for s, (_, _, w, h, spp) in zip(stats, jobs):
+            assert s.state == capi.PE_JOB_DONE and s.samples_done == spp, s
+            assert 0 <= s.t_preview <= s.t_done
+        films = {k: srv.download(ids[k]) for k in (0, 1, 4)}
+        for j in ids:
+            srv.release(j)
+        for k in (0, 1, 4):                                   # the same jobs alone
+            gs, view, w, h, spp = jobs[k]
+            cam = camera_perspective(view.position, view.target, view.up, view.vfov, w / h)
+            j = srv.submit(gs, cam, settings(max_bounces=view.max_bounces, seed=12345), w, h, spp, 16)
+            assert srv.wait(j, 300000).state == capi.PE_JOB_DONE
+            alone = srv.download(j)
+            srv.release(j)
+            for x, y in zip(films[k], alone):
+                np.testing.assert_array_equal(x, y)
+            print(f"job {k}: {w}x{h} {spp} spp alone == in the mix", flush=True)
+    finally:
+        srv.close()
