@@ -209,7 +209,7 @@ typedef struct pe_settings {
     float noise_threshold;      /* relative standard error target; clamped to >= 1e-4 (reference default 0.02) */
     int32_t min_adaptive_samples; /* at least 2 (reference default 16) */
     int32_t paths_in_flight;    /* path pool: paths traced together, a finished path's id taking the window's next
-                                   sample at once; 0 = 2^24 (never more than the window's samples) */
+                                   sample at once; 0 = 2^26 (never more than the window's samples) */
     int32_t reserved[2];
 } pe_settings;
 
@@ -225,6 +225,7 @@ typedef struct pe_batch_result {
     uint64_t shadow_tri_visits;
     uint32_t max_stack_depth;    /* deepest traversal stack of any ray, when settings.count_visits */
     uint32_t stack_overflows;    /* traversals whose stack overflowed (then the call fails with PE_ERR_INVALID) */
+    uint64_t closest_hits;       /* extension rays that hit a triangle, when settings.count_visits */
 } pe_batch_result;
 
 /* Per-kernel device time of the wavefront stages, from HIP events recorded on the device's stream around
@@ -265,7 +266,12 @@ typedef struct pe_targets {
     pe_film* albedo;   /* AovFilms::albedo, or NULL */
     pe_film* normal;   /* AovFilms::normal, or NULL */
     pe_film* alpha;    /* AovFilms::alpha, or NULL */
-    void* reserved[4];
+    pe_guide* guide;   /* a learned guide of the scene (RenderSettings::pathGuiding), or NULL */
+    /* Cancellation for callers whose flag is not an int32 (RenderDevice::render's std::atomic<bool>): called
+     * between iterations like the cancel flag is read; non-zero cancels. NULL: the cancel argument only. */
+    int (*cancel_poll)(void* user);
+    void* cancel_user;
+    void* reserved[1];
 } pe_targets;
 
 /* pe_render into every film of `targets` (RenderDevice::render with RenderTargets). */

@@ -386,8 +386,8 @@ _HIPDEV_SIGS = {
     "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
     "ref_scene_free": (None, [_VP]),
     "hipdev_session_render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
-                                        C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, _F, _U, _I, C.c_char_p,
-                                        C.c_int]),
+                                        C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I,
+                                        C.c_char_p, C.c_int]),
     "hipdev_enumerate": (C.c_int, [C.c_char_p, C.c_int]),
     "hipdev_session_render_aovs": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                              C.c_int, C.c_int, _F, _F, _F, _F, _U, C.c_char_p, C.c_int]),
@@ -424,7 +424,8 @@ class HipDeviceSession:
             self._lib.ref_scene_free(self.h)
             self.h = None
 
-    def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0, ao_strength=0.0):
+    def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0, ao_strength=0.0,
+               path_guiding=False):
         """Returns (rc, mean (H,W,3), counts (H,W), batches, error text)."""
         f3 = C.c_float * 3
         mean = np.zeros((view.height, view.width, 3), np.float32)
@@ -434,7 +435,7 @@ class HipDeviceSession:
         rc = self._lib.hipdev_session_render(
             self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
             view.height, view.max_bounces, shadow_quality, filter_type, filter_width, spp, max_batch, ao_strength,
-            _fp(mean),
+            1 if path_guiding else 0, _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
 

@@ -38,8 +38,8 @@ extern "C" {
 HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* camTarget, const float* camUp,
                                      float fov, float aspect, int width, int height, int maxBounces,
                                      int shadowQuality, int filterType, float filterWidth, int spp, int maxBatch,
-                                     float aoStrength, float* mean, uint32_t* counts, int* batches, char* err,
-                                     int errlen) {
+                                     float aoStrength, int pathGuiding, float* mean, uint32_t* counts,
+                                     int* batches, char* err, int errlen) {
     std::string e;
     std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
     if (!dev) {
@@ -63,6 +63,7 @@ HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* 
     s.filter.type = static_cast<PixelFilterType>(filterType);
     s.filter.width = filterWidth;
     s.aoStrength = aoStrength;
+    s.pathGuiding = pathGuiding != 0;
     SessionConfig cfg;
     cfg.mode = RenderMode::Final;
     cfg.targetSpp = spp;

@@ -44,7 +44,7 @@ constexpr int kMaxShadowSamples = 4;
 constexpr int kStatSlots = 16;  // 64-bit device counters of a render (PathBuffers::stats)
 constexpr uint64_t kAutoWindowSamples = uint64_t{1} << 28;  // default result buffer: 4 GB
 constexpr uint64_t kMaxWindowSamples = (uint64_t{1} << 31) - 64;  // window sample indices are 31-bit
-constexpr uint64_t kDefaultPoolIds = uint64_t{1} << 24;  // default path ids in flight
+constexpr uint64_t kDefaultPoolIds = uint64_t{1} << 26;  // default path ids in flight (about 15.6 GB)
 constexpr uint64_t kMaxPoolIds = uint64_t{1} << 30;
 constexpr int kChunkIterations = 2;  // iterations between two reads of the remaining-work counter
 
@@ -71,7 +71,8 @@ struct PathBuffers {
     uint32_t seg_cap; // slots per segment of the ray queues (S * seg_cap per shadow queue segment); segment k
                       // owns the path ids [k * seg_cap, (k + 1) * seg_cap)
     unsigned long long* stats;  // [0] extension rays, [1] shadow rays, [2..5] visits, [6] guide records,
-                                // [7] guide queries, [8] deepest traversal stack, [9] stack overflows
+                                // [7] guide queries, [8] deepest traversal stack, [9] stack overflows,
+                                // [10] extension rays that hit (counting variant)
 };
 
 // Path regeneration. The wavefront keeps a pool of path ids; segment k of every queue owns the ids
@@ -448,6 +449,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuf
     if (kCount) {
         wave_count(&pb.stats[2], nv);
         wave_count(&pb.stats[3], tv);
+        wave_count(&pb.stats[10], st.hits);
     }
 }
 
@@ -1622,7 +1624,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         // The window: samples of every pixel whose results are held until the film pass adds them in sample
         // order; as asked, or (0) up to 2^28 samples (4 GB of results) within an eighth of the free device
         // memory. The path pool (ids in flight, regenerated as paths end) is sized independently: as asked, or
-        // (0) 2^24 ids, and never above the window.
+        // (0) 2^26 ids, and never above the window.
         const bool adaptive = settings->adaptive != 0;
         int spl_req = settings->samples_per_launch;
         if (spl_req <= 0) {
@@ -1778,8 +1780,11 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 hipLaunchKernelGGL(k_fill_u32, dim3(pix_blocks), dim3(kBlock), 0, s, lp.active, pixels, 1u);
             }
         }
+        auto cancel_requested = [&]() {
+            return (cancel && *cancel) || (aovs && aovs->cancel_poll && aovs->cancel_poll(aovs->cancel_user) != 0);
+        };
         for (int done = 0; done < sample_count;) {
-            if (cancel && *cancel) {
+            if (cancel_requested()) {
                 cancelled = true;
                 break;
             }
@@ -1844,7 +1849,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 if (rc != PE_OK) return rc;
                 PE_HIP_CHECK(hipEventSynchronize(wf->chunk_done[(n - 1) & 1]));
                 if (wf->host_live[(n - 1) & 1] == 0) break;
-                if (cancel && *cancel) {
+                if (cancel_requested()) {
                     cancelled = true;
                     break;
                 }
@@ -1885,6 +1890,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             result->tri_visits = st[3];
             result->shadow_node_visits = st[4];
             result->shadow_tri_visits = st[5];
+            result->closest_hits = st[10];
             result->max_stack_depth = static_cast<uint32_t>(st[8]);
             result->stack_overflows = static_cast<uint32_t>(std::min<unsigned long long>(st[9], 0xFFFFFFFFull));
         }
@@ -1951,8 +1957,10 @@ extern "C" int pe_render_targets(pe_device* dev, pe_scene* scene, const pe_camer
         aovs.albedo = targets->albedo;
         aovs.normal = targets->normal;
         aovs.alpha = targets->alpha;
+        aovs.cancel_poll = targets->cancel_poll;
+        aovs.cancel_user = targets->cancel_user;
         return render_samples(dev, lane, scene, camera, settings, targets->beauty, first_sample, sample_count, cancel,
-                              result, nullptr, &aovs);
+                              result, targets->guide, &aovs);
     } catch (const std::exception& e) {
         set_error(std::string("pe_render_targets: ") + e.what());
         return PE_ERR_INVALID;

@@ -147,11 +147,13 @@ struct RenderLane {
     uint32_t* spill;
     int grid_blocks;
 };
-// AOV films of a render (any may be null).
+// AOV films of a render (any may be null), and a cancellation callback polled with the cancel flag.
 struct AovTargets {
     pe_film* albedo = nullptr;
     pe_film* normal = nullptr;
     pe_film* alpha = nullptr;
+    int (*cancel_poll)(void*) = nullptr;
+    void* cancel_user = nullptr;
 };
 int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, const pe_camera* camera,
                    const pe_settings* settings, pe_film* film, int first_sample, int sample_count,

@@ -85,6 +85,7 @@ struct TraceStack {
     uint32_t spill_stride;
     int sp;
     int max_sp = 0;         // deepest stack of this lane's traversals (kept by the counting variants)
+    uint32_t hits = 0;      // closest-hit queries of this lane that found a triangle (counting variants)
     bool overflow = false;  // a push found the stack full and dropped its entry
 
     // The build bounds the tree depth below kStackDepth, so a push never finds the stack full; were it to,
@@ -368,7 +369,7 @@ __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, 
             if constexpr (kAny) {
                 job.any(slot, ray, occluded);
                 slot = kNoRay;
-            } else if (job.closest(slot, ray, ht, htri, hu, hv, ra, rb)) {
+            } else if ((kCount && htri >= 0 ? ++st.hits : 0u), job.closest(slot, ray, ht, htri, hu, hv, ra, rb)) {
                 // The job continues the query with a new ray in (ra, rb): restart at the root.
                 fresh = true;
                 item = 0;

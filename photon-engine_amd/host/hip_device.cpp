@@ -9,8 +9,8 @@
 // What the HIP path does not implement is refused, never approximated: setScene leaves the device without a
 // scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle, lights other
 // than AreaLight / TriangleLight, textured or flaked Disney materials and materials other than Disney /
-// Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO,
-// motion blur and path guiding.
+// Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO and
+// motion blur. Path guiding uses the library's learned guide in place of the reference's SD-tree.
 #include "hip_device.h"
 
 #include "camera/camera.h"
@@ -301,11 +301,17 @@ std::unique_ptr<HipDevice> HipDevice::create(int ordinal, std::string* error) {
 HipDevice::~HipDevice() {
     for (DeviceFilm& f : m_films)
         if (f.film) pe_film_destroy(f.film);
+    if (m_guide) pe_guide_destroy(m_guide);
     if (m_scene) pe_scene_release(m_scene);
     if (m_dev) pe_device_destroy(m_dev);
 }
 
 void HipDevice::setScene(std::shared_ptr<const Scene> scene) {
+    if (m_guide) {  // a guide belongs to its scene
+        pe_guide_destroy(m_guide);
+        m_guide = nullptr;
+        m_guideScene = nullptr;
+    }
     if (m_scene) {
         pe_scene_release(m_scene);
         m_scene = nullptr;
@@ -361,9 +367,30 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         m_error = "HipDevice::render: only the pinhole perspective camera is implemented";
         return res;
     }
-    if (settings.aoStrength > 0.0f || settings.motionBlur || settings.pathGuiding) {
-        m_error = "HipDevice::render: AO, motion blur and path guiding are not implemented";
+    if (settings.aoStrength > 0.0f || settings.motionBlur) {
+        m_error = "HipDevice::render: AO and motion blur are not implemented";
         return res;
+    }
+    // Path guiding (renderer.cpp:413-429): a learned guide per scene, made afresh when a render starts at sample
+    // 0 or the scene changed, trained during its first launches (the reference trains its SD-tree over the
+    // first 32 samples, then freezes it), dropped when guiding is switched off.
+    if (settings.pathGuiding) {
+        if (!m_guide || m_guideScene != m_scene || firstSample == 0) {
+            if (m_guide) pe_guide_destroy(m_guide);
+            m_guide = nullptr;
+            pe_guide_config gc{};  // the library's defaults (learning rate 1e-3, mixture fraction 0.5)
+            gc.train_launches = 4;
+            if (pe_guide_create(m_dev, m_scene, &gc, &m_guide) != PE_OK) {
+                m_error = "HipDevice::render: " + lastHipError();
+                m_guide = nullptr;
+                return res;
+            }
+            m_guideScene = m_scene;
+        }
+    } else if (m_guide) {
+        pe_guide_destroy(m_guide);
+        m_guide = nullptr;
+        m_guideScene = nullptr;
     }
     pe_camera cam{};
     put3(cam.position, cd.position);
@@ -436,27 +463,24 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         tg.normal = m_films[2].film;
         tg.alpha = m_films[3].film;
     }
-    // Cancellation is polled between chunks; one chunk when nobody can cancel. An adaptive batch is one call:
-    // its skip mask and stops belong to the batch as a whole.
-    const int chunk = cancel && !adaptive ? std::max(1, std::min(count, 8)) : std::max(1, count);
-    uint64_t traced = 0;
-    int done = 0;
-    bool cancelled = false;
-    while (done < count) {
-        if (cancel && cancel->load(std::memory_order_relaxed)) {
-            cancelled = true;
-            break;
-        }
-        const int n = std::min(chunk, count - done);
-        pe_batch_result br{};
-        if (pe_render_targets(m_dev, m_scene, &cam, &st, &tg, firstSample + done, n, nullptr, &br) != PE_OK) {
-            m_error = "HipDevice::render: " + lastHipError();
-            return res;
-        }
-        m_lastRays += br.extension_rays + br.shadow_rays;
-        traced += br.samples_traced;
-        done += n;
+    // The whole batch is one call: the library polls the caller's cancel flag between its iterations (a few
+    // milliseconds apart) and keeps the windows of samples it completed.
+    tg.guide = settings.pathGuiding ? m_guide : nullptr;
+    if (cancel) {
+        tg.cancel_poll = [](void* flag) -> int {
+            return static_cast<const std::atomic<bool>*>(flag)->load(std::memory_order_relaxed) ? 1 : 0;
+        };
+        tg.cancel_user = const_cast<std::atomic<bool>*>(cancel);
     }
+    pe_batch_result br{};
+    const int rrc = pe_render_targets(m_dev, m_scene, &cam, &st, &tg, firstSample, count, nullptr, &br);
+    if (rrc != PE_OK && rrc != PE_ERR_CANCELLED) {
+        m_error = "HipDevice::render: " + lastHipError();
+        return res;
+    }
+    m_lastRays += br.extension_rays + br.shadow_rays;
+    const uint64_t traced = br.samples_traced;
+    const bool cancelled = rrc == PE_ERR_CANCELLED;
     // Merge what this call added (the whole film unless it accumulates across calls) with Film::addBatch; the
     // normal AOV keeps its sign (addSampleSigned, film.cpp:73-86).
     std::vector<float> sum(3 * pixels), lum2(pixels);

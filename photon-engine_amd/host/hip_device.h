@@ -43,6 +43,8 @@ private:
     DeviceInfo m_info;
     pe_device* m_dev = nullptr;
     pe_scene* m_scene = nullptr;
+    pe_guide* m_guide = nullptr;          // RenderSettings::pathGuiding: the learned guide of m_guideScene
+    const pe_scene* m_guideScene = nullptr;
     // Device films: beauty, then the albedo, normal and alpha AOVs. With adaptive sampling they keep every sample
     // this device rendered since the last render that started at sample 0 (the beauty's per-sample statistics
     // drive the skip mask and the stops); prevSum / prevCount is the part already merged into the caller's films.

@@ -102,12 +102,16 @@ class BatchResult(C.Structure):
     _fields_ = [("completed", C.c_int32), ("samples_traced", C.c_uint64), ("extension_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("node_visits", C.c_uint64), ("tri_visits", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_visits", C.c_uint64),
-                ("max_stack_depth", C.c_uint32), ("stack_overflows", C.c_uint32)]
+                ("max_stack_depth", C.c_uint32), ("stack_overflows", C.c_uint32), ("closest_hits", C.c_uint64)]
+
+
+CANCEL_POLL = C.CFUNCTYPE(C.c_int, C.c_void_p)
 
 
 class Targets(C.Structure):
     _fields_ = [("beauty", C.c_void_p), ("albedo", C.c_void_p), ("normal", C.c_void_p), ("alpha", C.c_void_p),
-                ("reserved", C.c_void_p * 4)]
+                ("guide", C.c_void_p), ("cancel_poll", CANCEL_POLL), ("cancel_user", C.c_void_p),
+                ("reserved", C.c_void_p * 1)]
 
 
 PE_TONEMAP_REINHARD, PE_TONEMAP_REINHARD_EXTENDED, PE_TONEMAP_ACES, PE_TONEMAP_FILMIC = 0, 1, 2, 3

@@ -35,12 +35,12 @@ def test_adapter_library_loads_and_exports():
         assert hasattr(lib.lib, name), name
 
 
-def _direct_mean(dev, sc, view, spp, spl):
+def _direct_mean(dev, sc, view, spp, spl, first=0):
     gs = dev.upload(sc)
     film = dev.film(view.width, view.height)
     cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
     st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=spl)
-    res = render(dev, gs, cam, st, film, 0, spp)
+    res = render(dev, gs, cam, st, film, first, spp)
     assert res.completed == 1
     s, c, _, rej = film.download()
     assert rej == 0
@@ -86,7 +86,7 @@ def test_hip_device_refuses_what_it_does_not_implement(dev):
 def test_interactive_passes_and_cancel_through_hip_device(dev):
     """RenderSession's Interactive mode (one-sample passes, render_session.cpp:69-71) through HipDevice gives the
     image of a direct render of the same samples, pass by pass at viewport latency; a cancelled Final batch
-    returns within one chunk of the flag (HipDevice polls between chunks of 8 samples), reports the batch
+    returns soon after the flag (the library polls it between wavefront iterations), reports the batch
     incomplete, and the session keeps only its completed samples."""
     _harness_or_skip()
     sc, view = configs.product_sphere(width=320, height=180, lon=128, lat=96, env_width=512, env_height=256)
@@ -103,3 +103,26 @@ def test_interactive_passes_and_cancel_through_hip_device(dev):
     assert r["cancel_latency"] < 0.25, r["cancel_latency"]
     print(f"interactive pass median {np.median(r['pass_seconds'][1:]) * 1e3:.2f} ms, cancel latency "
           f"{r['cancel_latency'] * 1e3:.2f} ms")
+
+
+@pytest.mark.gpu
+def test_path_guiding_through_hip_device_converges_to_plain(dev):
+    """RenderSettings::pathGuiding through HipDevice (renderer.cpp:413-429): RenderSession's batches train the
+    learned guide on the first launches and then sample it with one-sample MIS; the estimate stays unbiased, so
+    the guided image of the door-gap room converges to the plain one (relMSE against a plain 1024-spp image of
+    the same frame no worse than the plain render's own)."""
+    _harness_or_skip()
+    sc, view = configs.slit_room(width=96, height=54, max_bounces=6)
+    spp = 256
+    rc, guided, counts, batches, err = bindings.HipDeviceSession(sc).render(view, spp, max_batch=32,
+                                                                         path_guiding=True)
+    assert rc == 0, err
+    assert (counts == spp).all() and batches > 1
+    rc, plain, _, _, err = bindings.HipDeviceSession(sc).render(view, spp, max_batch=32)
+    assert rc == 0, err
+    ref, _ = _direct_mean(dev, sc, view, 1024, spl=64, first=1 << 20)  # independent samples
+    e_g, e_p = relmse(guided, ref), relmse(plain, ref)
+    print(f"guided relMSE {e_g:.4f}, plain {e_p:.4f} against plain 1024 spp", flush=True)
+    assert np.isfinite(guided).all()
+    assert abs(guided.mean() / plain.mean() - 1.0) < 0.05
+    assert e_g <= 1.5 * e_p
