@@ -338,7 +338,35 @@ std::vector<BvhNode4> collapse_bvh4(const std::vector<BvhNode>& nodes, int& dept
     if (nodes.empty()) return out;
     out.reserve(nodes.size() / 2 + 1);
     convert4(nodes, 0, out, 1, depth);
-    return out;
+    // The top of the tree first: the first kBvhTileNodes nodes in breadth-first order from the root (the
+    // traversal keeps them in LDS), the others after them in their depth-first order.
+    const size_t n = out.size();
+    std::vector<uint32_t> order;
+    order.reserve(n);
+    std::vector<uint8_t> placed(n, 0);
+    order.push_back(0u);
+    placed[0] = 1;
+    for (size_t head = 0; head < order.size() && order.size() < kBvhTileNodes; ++head) {
+        const uint4 it = out[order[head]].item;
+        for (uint32_t c : {it.x, it.y, it.z, it.w}) {
+            if ((c & 0x80000000u) || order.size() >= kBvhTileNodes) continue;
+            order.push_back(c);
+            placed[c] = 1;
+        }
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (!placed[i]) order.push_back(i);
+    std::vector<uint32_t> where(n);
+    for (uint32_t k = 0; k < n; ++k) where[order[k]] = k;
+    std::vector<BvhNode4> re(n);
+    for (uint32_t k = 0; k < n; ++k) {
+        BvhNode4 b = out[order[k]];
+        uint32_t* it = &b.item.x;
+        for (int j = 0; j < 4; ++j)
+            if (!(it[j] & 0x80000000u)) it[j] = where[it[j]];
+        re[k] = b;
+    }
+    return re;
 }
 
 }  // namespace pe

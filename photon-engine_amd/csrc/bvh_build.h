@@ -32,7 +32,8 @@ BvhBuildResult build_bvh(const float* tri_lo, const float* tri_hi, size_t n, con
 
 // The four-wide tree the traversal reads, collapsed from the binary one: a node takes its two children and
 // puts each interior child's two children in its place, so every four-wide level spans two binary levels
-// (a leaf child stays as it is). Depth-first, root at 0; `depth` receives the deepest level (root = 1).
+// (a leaf child stays as it is). Root at 0, the first kBvhTileNodes nodes in breadth-first order from the root
+// (the traversal keeps them in LDS), the rest depth-first; `depth` receives the deepest level (root = 1).
 std::vector<BvhNode4> collapse_bvh4(const std::vector<BvhNode>& nodes, int& depth);
 
 }  // namespace pe

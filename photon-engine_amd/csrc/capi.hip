@@ -106,25 +106,31 @@ struct BatchAny : BatchRays {
 
 __global__ void __launch_bounds__(kTraceBlock) k_intersect_batch(const BvhNode4* __restrict__ nodes,
                                                                  const TriAccel* __restrict__ tris, BatchClosest job,
-                                                                 uint32_t n, uint32_t* __restrict__ spill,
+                                                                 uint32_t n, uint32_t num_nodes, uint32_t* __restrict__ spill,
                                                                  unsigned long long* __restrict__ diag) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    __shared__ float4 node_tile[kBvhTileNodes * 8];
+    load_node_tile(node_tile, nodes, num_nodes);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<false, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv);
+    trace_queue<false, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv, node_tile,
+                             min(num_nodes, kBvhTileNodes));
     report_stack(diag, st, true);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_occluded_batch(const BvhNode4* __restrict__ nodes,
                                                                 const TriAccel* __restrict__ tris, BatchAny job,
-                                                                uint32_t n, uint32_t* __restrict__ spill,
+                                                                uint32_t n, uint32_t num_nodes, uint32_t* __restrict__ spill,
                                                                 unsigned long long* __restrict__ diag) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    __shared__ float4 node_tile[kBvhTileNodes * 8];
+    load_node_tile(node_tile, nodes, num_nodes);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<true, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv);
+    trace_queue<true, true>(job, QueueSegs{nullptr, n, n, 1u, reinterpret_cast<uint32_t*>(diag + 2)}, nodes, tris, st, nv, tv, node_tile,
+                             min(num_nodes, kBvhTileNodes));
     report_stack(diag, st, true);
 }
 
@@ -205,6 +211,7 @@ int scene_park(pe_scene& sc) {
 static void rebind_view(pe_scene& sc) {
     SceneView& v = sc.view;
     v.nodes = sc.nodes.as<BvhNode4>();
+    v.num_nodes = static_cast<uint32_t>(sc.num_nodes);
     v.tris = sc.tris.as<TriAccel>();
     v.shade = sc.tri_shade.as<TriShade>();
     v.pos = sc.positions.as<float4>();
@@ -558,7 +565,7 @@ int pe_intersect_batch(pe_device* dev, pe_scene* scene, size_t n, const float* r
     job.bary = bary ? dbary.as<float2>() : nullptr;
     PE_HIP_CHECK(hipMemsetAsync(dev->diag.get(), 0, kDiagBytes, dev->stream));
     hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
-                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>(),
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), scene->view.num_nodes, dev->spill.as<uint32_t>(),
                        dev->diag.as<unsigned long long>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(hits, dhits.get(), n * 16, hipMemcpyDeviceToHost, dev->stream));
@@ -604,7 +611,7 @@ int pe_intersect_batch_noalloc(pe_device* dev, pe_scene* scene, size_t n, const 
     job.hits = reinterpret_cast<int4*>(hits_dev);
     job.bary = reinterpret_cast<float2*>(bary_dev);
     hipLaunchKernelGGL(k_intersect_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
-                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), spill, diag);
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), scene->view.num_nodes, spill, diag);
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(dev->diag_host, diag, kDiagBytes, hipMemcpyDeviceToHost, dev->stream));
     PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
@@ -640,7 +647,7 @@ int pe_occluded_batch(pe_device* dev, pe_scene* scene, size_t n, const float* ra
     job.occ = docc.as<uint8_t>();
     PE_HIP_CHECK(hipMemsetAsync(dev->diag.get(), 0, kDiagBytes, dev->stream));
     hipLaunchKernelGGL(k_occluded_batch, dim3(blocks), dim3(kTraceBlock), 0, dev->stream, scene->nodes.as<BvhNode4>(),
-                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), dev->spill.as<uint32_t>(),
+                       scene->tris.as<TriAccel>(), job, static_cast<uint32_t>(n), scene->view.num_nodes, dev->spill.as<uint32_t>(),
                        dev->diag.as<unsigned long long>());
     PE_HIP_CHECK(hipGetLastError());
     PE_HIP_CHECK(hipMemcpyAsync(occluded, docc.get(), n, hipMemcpyDeviceToHost, dev->stream));

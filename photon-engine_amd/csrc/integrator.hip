@@ -42,6 +42,15 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxShadowSamples = 4;
 constexpr int kStatSlots = 16;  // 64-bit device counters of a render (PathBuffers::stats)
+#ifndef PE_NODE_TILE
+#define PE_NODE_TILE 1
+#endif
+constexpr bool kNodeTile = PE_NODE_TILE != 0;  // traversal reads the top of the BVH from an LDS tile
+#ifndef PE_MAT_BUCKETS
+#define PE_MAT_BUCKETS 4
+#endif
+constexpr uint32_t kMatBuckets = PE_MAT_BUCKETS;  // shade's sort keys for hits (1: hits / misses only)
+static_assert(kMatBuckets == 1 || kMatBuckets == static_cast<uint32_t>(kSortClasses), "one key per shading class");
 constexpr uint64_t kAutoWindowSamples = uint64_t{1} << 28;  // default result buffer: 4 GB
 constexpr uint64_t kMaxWindowSamples = (uint64_t{1} << 31) - 64;  // window sample indices are 31-bit
 constexpr uint64_t kDefaultPoolIds = uint64_t{1} << 26;  // default path ids in flight (about 15.6 GB)
@@ -440,11 +449,14 @@ struct TransmitShadowJob {
 template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(SceneView sc, PathBuffers pb, int cur, uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    __shared__ float4 node_tile[kBvhTileNodes * 8];
+    load_node_tile(node_tile, sc.nodes, kNodeTile ? sc.num_nodes : 0u);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
     trace_queue<false, kCount>(ExtensionJob{pb.ro[cur], pb.rd[cur], pb.hits}, trace_ray_queue(pb), sc.nodes, sc.tris,
-                               st, nv, tv);
+                               st, nv, tv,
+                               node_tile, kNodeTile ? min(sc.num_nodes, kBvhTileNodes) : 0u);
     report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[2], nv);
@@ -545,10 +557,13 @@ __global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers p
 template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(SceneView sc, PathBuffers pb, int S, uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    __shared__ float4 node_tile[kBvhTileNodes * 8];
+    load_node_tile(node_tile, sc.nodes, kNodeTile ? sc.num_nodes : 0u);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
-    trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, shadow_queue(pb, S), sc.nodes, sc.tris, st, nv, tv);
+    trace_queue<true, kCount>(ShadowJob{pb.sq_o, pb.sq_d, pb.pend}, shadow_queue(pb, S), sc.nodes, sc.tris, st, nv, tv,
+                               node_tile, kNodeTile ? min(sc.num_nodes, kBvhTileNodes) : 0u);
     report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
@@ -560,11 +575,14 @@ template <bool kCount>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow_transmit(SceneView sc, PathBuffers pb, int S, float clamp_limit,
                                                                  uint32_t* spill) {
     __shared__ uint32_t lds_stack[kLdsDepth * kTraceBlock];
+    __shared__ float4 node_tile[kBvhTileNodes * 8];
+    load_node_tile(node_tile, sc.nodes, kNodeTile ? sc.num_nodes : 0u);
     const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
     TraceStack st{lds_stack + threadIdx.x, spill + gid, gridDim.x * blockDim.x, 0};
     uint32_t nv = 0, tv = 0;
     trace_queue<false, kCount>(TransmitShadowJob{pb.sq_o, pb.sq_d, pb.pend, sc, clamp_limit}, shadow_queue(pb, S),
-                               sc.nodes, sc.tris, st, nv, tv);
+                               sc.nodes, sc.tris, st, nv, tv,
+                               node_tile, kNodeTile ? min(sc.num_nodes, kBvhTileNodes) : 0u);
     report_stack(pb.stats + 8, st, kCount);
     if (kCount) {
         wave_count(&pb.stats[4], nv);
@@ -644,38 +662,46 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
     float4* sq_o = pb.sq_o + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
     float4* sq_d = pb.sq_d + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
     const uint32_t stride = sh.workers * blockDim.x;
-    // Each block takes 256 queue slots per round and reorders them in LDS, hits first and misses after, so a
-    // wave runs either the long surface path or the short environment path, not both with half its lanes.
+    // Each block takes 256 queue slots per round and reorders them in LDS by a sort key: in scenes that mix
+    // shading code paths, the hit material's path (GpuMaterial::sort_class: opaque Disney, Disney with lobes
+    // below the surface, Dielectric, Lambert / Mirror), then the misses, so that a wave runs one path's code,
+    // or the short environment path, rather than several with a share of its lanes each (a counting sort: one
+    // ballot per key and wave, a prefix over keys and waves). Where every material takes the same path the key
+    // is hit / miss only (measured on the product-sphere workload: sorting its 16 Disney materials by index
+    // cost 2%).
+    constexpr uint32_t kKeys = kMatBuckets + 1, kNoKey = 0xFFFFFFFFu;
     __shared__ uint32_t s_slot[kBlock];
-    __shared__ uint32_t s_count[2][kBlock / 64];
+    __shared__ uint32_t s_count[kKeys][kBlock / 64];
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
     // The loop bound is uniform over the block, so every lane takes part in the barriers and queue appends.
+    const bool sort_hits = kMatBuckets > 1 && sc.sort_classes > 1;
+    const uint32_t keys = sort_hits ? kKeys : 2u;  // hit classes, then the miss key
     for (uint32_t base = sh.rank * blockDim.x; base < n; base += stride) {
         const uint32_t mine = seg_base + base + threadIdx.x;
-        const bool valid = base + threadIdx.x < n;
-        const bool hit_here = valid && __float_as_int(pb.hits[mine].y) >= 0;
-        const bool miss_here = !kHitsOnly && valid && !hit_here;
-        const uint64_t hm = __ballot(hit_here), mm = __ballot(miss_here);
-        if (__lane_id() == 0) {
-            s_count[0][wave] = static_cast<uint32_t>(__popcll(hm));
-            s_count[1][wave] = static_cast<uint32_t>(__popcll(mm));
+        uint32_t key = kNoKey;
+        if (base + threadIdx.x < n) {
+            const int tri = __float_as_int(pb.hits[mine].y);
+            if (tri >= 0)
+                key = sort_hits ? static_cast<uint32_t>(sc.materials[sc.shade[tri].material].sort_class) : 0u;
+            else if (!kHitsOnly)
+                key = keys - 1u;
+        }
+        uint32_t rank = 0;
+        for (uint32_t b = 0; b < keys; ++b) {
+            const uint64_t m = __ballot(key == b);
+            if (key == b) rank = static_cast<uint32_t>(__popcll(m & below));
+            if (__lane_id() == 0) s_count[b][wave] = static_cast<uint32_t>(__popcll(m));
         }
         __syncthreads();
-        uint32_t hits_before = 0, misses_before = 0, total_hits = 0, total = 0;
-        for (uint32_t k = 0; k < kBlock / 64; ++k) {
-            total_hits += s_count[0][k];
-            total += s_count[0][k] + s_count[1][k];
-            if (k < wave) {
-                hits_before += s_count[0][k];
-                misses_before += s_count[1][k];
+        uint32_t pos = 0, total = 0;
+        for (uint32_t b = 0; b < keys; ++b)
+            for (uint32_t k = 0; k < kBlock / 64; ++k) {
+                const uint32_t c = s_count[b][k];
+                total += c;
+                if (b < key || (b == key && k < wave)) pos += c;
             }
-        }
-        if (hit_here || miss_here) {
-            const uint32_t pos = hit_here ? hits_before + static_cast<uint32_t>(__popcll(hm & below))
-                                          : total_hits + misses_before + static_cast<uint32_t>(__popcll(mm & below));
-            s_slot[pos] = mine;
-        }
+        if (key != kNoKey) s_slot[pos + rank] = mine;
         __syncthreads();
         const bool active = threadIdx.x < total;
         const uint32_t i = active ? s_slot[threadIdx.x] : 0u;

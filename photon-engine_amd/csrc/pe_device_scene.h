@@ -41,7 +41,8 @@ struct alignas(16) GpuMaterial {
     V3 edge_tint;
     float sheen_roughness;
     float ior, coat_ior, specular_transmission, anisotropy;
-    float diffuse_roughness, diffuse_transmission, subsurface, pad0;
+    float diffuse_roughness, diffuse_transmission, subsurface;
+    int32_t sort_class;   // shade's sort key: the shading code path (kSortClass*)
     // derived
     float ax, ay, alpha;             // anisoAlpha and sqrt(ax * ay) (disney.cpp:55-60, :133-135)
     float f0d, wd;                   // dielectricF0, dielectricWeight (:34-42)
@@ -70,6 +71,10 @@ struct alignas(16) GpuMaterial {
     V3 film_metal_k;
     float pad8;
 };
+
+// Shading code paths the shade kernel groups a round's hits by (when a scene has more than one).
+enum : int32_t { kSortClassDisney = 0, kSortClassDisneyTransport = 1, kSortClassDielectric = 2, kSortClassSimple = 3,
+                 kSortClasses = 4 };
 
 enum : uint32_t {
     kMatFlagEmissive = 1u,
@@ -136,7 +141,8 @@ struct EnvView {
 enum : int32_t { kPickUniform = 0, kPickExhaustive = 1, kPickBvh = 2 };
 
 struct SceneView {
-    const BvhNode4* nodes;  // four-wide, root at 0
+    const BvhNode4* nodes;  // four-wide, root at 0, the top kBvhTileNodes first (breadth-first)
+    uint32_t num_nodes;
     const TriAccel* tris;
     const TriShade* shade;
     const float4* pos;
@@ -153,6 +159,7 @@ struct SceneView {
     int32_t exit_material;   // white Lambert shading a subsurface walk's exit (one past the scene's materials)
     int32_t has_transmitters;  // a thin-walled surface passes shadow rays on with its transmittance
     int32_t transport;         // a Dielectric or a transmitting / subsurface material is present
+    int32_t sort_classes;      // distinct GpuMaterial::sort_class values of the scene's materials
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;
     const LightPath* light_paths;  // light_count entries

@@ -40,6 +40,10 @@ struct alignas(16) TriAccel {
 static_assert(sizeof(TriAccel) == 48, "TriAccel must be 48 bytes");
 
 // Per-triangle shading record in scene order (32 B).
+// Four-wide nodes at the top of the tree (root, its children, theirs: 1 + 4 + 16) that the traversal kernels
+// hold in LDS; the BVH lays them out first.
+constexpr uint32_t kBvhTileNodes = 21;
+
 struct TriShade {
     uint32_t v0, v1, v2;  // global vertex indices
     int32_t material;

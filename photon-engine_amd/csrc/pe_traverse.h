@@ -205,10 +205,19 @@ __device__ __forceinline__ uint32_t child_item(int32_t ref, int32_t count) {
 //            job continues the query with the new ray record it wrote into (a, b)
 //   any:     job.any(i, ray, occluded)
 // nv / tv accumulate the node and triangle items this lane handled (kCount only).
+// The top kBvhTileNodes nodes of the tree, loaded into a block's LDS tile (8 float4 per node: the seven the
+// traversal reads and the pad) by all of its threads before traversal starts.
+__device__ __forceinline__ void load_node_tile(float4* tile, const BvhNode4* __restrict__ nodes, uint32_t num_nodes) {
+    const uint32_t n = min(num_nodes, kBvhTileNodes) * 8u;
+    const float4* src = reinterpret_cast<const float4*>(nodes);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) tile[i] = src[i];
+    __syncthreads();
+}
+
 template <bool kAny, bool kCount, class Job>
 __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, const BvhNode4* __restrict__ nodes,
                                             const TriAccel* __restrict__ tris, TraceStack& st, uint32_t& nv,
-                                            uint32_t& tv) {
+                                            uint32_t& tv, const float4* tile = nullptr, uint32_t tile_nodes = 0) {
     const uint32_t wpb = blockDim.x >> 6;
     const SegShare sh = seg_share(__builtin_amdgcn_readfirstlane(blockIdx.x * wpb + (threadIdx.x >> 6)),
                                   gridDim.x * wpb, q.segs);
@@ -295,16 +304,31 @@ __device__ __forceinline__ void trace_queue(const Job& job, const QueueSegs& q, 
         if (__ballot(slot != kNoRay) == 0) break;
         if (slot == kNoRay) continue;
 
-        // One item: a triangle of a leaf (the first 48 of 64 bytes read) or a four-wide node (128 bytes).
+        // One item: a triangle of a leaf (the first 48 of 64 bytes read) or a four-wide node (128 bytes), the
+        // top nodes from the block's LDS tile.
         const bool leaf = (item & kLeafItem) != 0u;
-        const float4* p = leaf ? reinterpret_cast<const float4*>(tris + (item & kLeafFirstMask))
-                               : reinterpret_cast<const float4*>(nodes + item);
-        const float4 x0 = p[0], x1 = p[1], x2 = p[2], x3 = p[3];
-        float4 x4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x5 = x4, x6 = x4;
-        if (!leaf) {
-            x4 = p[4];
-            x5 = p[5];
-            x6 = p[6];
+        float4 x0, x1, x2, x3, x4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), x5 = x4, x6 = x4;
+        if (!leaf && item < tile_nodes) {
+            const float4* t = tile + item * 8u;
+            x0 = t[0];
+            x1 = t[1];
+            x2 = t[2];
+            x3 = t[3];
+            x4 = t[4];
+            x5 = t[5];
+            x6 = t[6];
+        } else {
+            const float4* p = leaf ? reinterpret_cast<const float4*>(tris + (item & kLeafFirstMask))
+                                   : reinterpret_cast<const float4*>(nodes + item);
+            x0 = p[0];
+            x1 = p[1];
+            x2 = p[2];
+            x3 = p[3];
+            if (!leaf) {
+                x4 = p[4];
+                x5 = p[5];
+                x6 = p[6];
+            }
         }
         if (fresh) {
             ray = job.decode(ra, rb);

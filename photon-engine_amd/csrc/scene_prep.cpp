@@ -429,6 +429,7 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     const int num_mats = desc.num_materials;
     std::vector<GpuMaterial> mats(static_cast<size_t>(num_mats) + 1);
     bool transmitters = false, transport = false;
+    uint32_t classes = 0;  // sort classes of the scene's own materials (not the walk-exit Lambert)
     for (int i = 0; i <= num_mats; ++i) {
         pe_material_desc exit_desc{};
         if (i == num_mats) {
@@ -447,12 +448,18 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             set_error("material " + std::to_string(i) + ": " + err);
             return rc;
         }
-        const GpuMaterial& g = mats[static_cast<size_t>(i)];
+        GpuMaterial& g = mats[static_cast<size_t>(i)];
         transmitters = transmitters || (g.flags & kMatFlagThinWall) != 0;
-        transport = transport || g.type == kMatDielectric || g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f ||
-                    g.p_spec_trans > 0.0f || g.p_sss > 0.0f ||
+        const bool lobes_below = g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f || g.p_spec_trans > 0.0f ||
+                                 g.p_sss > 0.0f;
+        transport = transport || g.type == kMatDielectric || lobes_below ||
                     (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium |
                                 kMatFlagFilm)) != 0;
+        g.sort_class = g.type == kMatDielectric ? kSortClassDielectric
+                       : g.type != kMatDisney   ? kSortClassSimple
+                       : (lobes_below || (g.flags & kMatFlagFilm)) ? kSortClassDisneyTransport
+                                                                   : kSortClassDisney;
+        if (i < num_mats) classes |= 1u << g.sort_class;
     }
     hipStream_t s = dev->stream;
     if (!sc.materials.upload(mats.data(), mats.size() * sizeof(GpuMaterial), s) ||
@@ -469,6 +476,7 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     v.exit_material = num_mats;
     v.has_transmitters = transmitters ? 1 : 0;
     v.transport = transport ? 1 : 0;
+    v.sort_classes = __builtin_popcount(classes);
     return PE_OK;
 }
 
@@ -540,6 +548,7 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     if (rc != PE_OK) return rc;
     SceneView& v = sc.view;
     v.nodes = sc.nodes.as<BvhNode4>();
+    v.num_nodes = static_cast<uint32_t>(sc.num_nodes);
     v.tris = sc.tris.as<TriAccel>();
     v.shade = sc.tri_shade.as<TriShade>();
     v.pos = sc.positions.as<float4>();

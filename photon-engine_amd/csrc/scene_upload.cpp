@@ -200,12 +200,12 @@ int upload_scene(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         set_error(std::string("scene upload: ") + hipGetErrorString(e));
         return PE_ERR_HIP;
     }
+    sc.num_nodes = nodes4.size();
     const int rc = prepare_shading(dev, desc, sc);
     if (rc != PE_OK) return rc;
     sc.num_tris = nt;
     sc.num_verts = nv;
     sc.num_shapes = desc.num_meshes;
-    sc.num_nodes = nodes4.size();
     sc.bvh_depth = depth4;
     return PE_OK;
 }
