@@ -153,7 +153,19 @@ typedef struct pe_scene_desc {
     const pe_light_desc* lights;
     int32_t num_lights;
     const pe_env_desc* environment;   /* NULL: no environment light */
+    /* Light groups (engine/scene.cpp:144-173), resolved by the caller: light i's group index
+     * (Scene::lightGroupIndex(i), lights in Scene::lights() order) and the environment's
+     * (Scene::envLightGroupIndex), of num_light_groups (Scene::lightGroupNames().size(), at most
+     * PE_MAX_LIGHT_GROUPS). light_groups == NULL or num_light_groups == 0: the scene has no groups. */
+    const int32_t* light_groups;
+    int32_t env_light_group;
+    int32_t num_light_groups;
+    /* Per mesh: the scene-graph node that owns it (Scene::tagShape / nodeUidOf, the object-ID pass), or NULL
+     * (0 for every mesh: untagged shapes). */
+    const uint64_t* node_uids;
 } pe_scene_desc;
+
+#define PE_MAX_LIGHT_GROUPS 16  /* LightPasses::kMaxLightGroups (integrators/path_tracer.h:27) */
 
 #define PE_UPDATE_FULL 0         /* SceneUpdateKind::Full (render_device.h:33) */
 #define PE_UPDATE_MATERIALS 1    /* SceneUpdateKind::Materials */
@@ -260,7 +272,39 @@ int pe_render(pe_device* dev, pe_scene* scene, const pe_camera* camera, const pe
  * environment background): the first surface's albedo clamped to [0, 1] (DisneyMaterial baseColor, Lambertian
  * albedo, Dielectric tint) with Film::addSample, its shading normal with Film::addSampleSigned and coverage 1
  * in r, g, b; a camera ray that misses adds the clamped environment radiance along it, a zero normal and
- * coverage 0. Light passes and cryptomatte (RenderPassFilms) are not implemented. */
+ * coverage 0. Light passes, light groups and the data passes come with `passes` (pe_pass_targets). */
+
+/* LightPass (integrators/path_tracer.h:21-24), same order. */
+#define PE_PASS_DIFFUSE_DIRECT 0
+#define PE_PASS_DIFFUSE_INDIRECT 1
+#define PE_PASS_GLOSSY_DIRECT 2
+#define PE_PASS_GLOSSY_INDIRECT 3
+#define PE_PASS_TRANSMISSION_DIRECT 4
+#define PE_PASS_TRANSMISSION_INDIRECT 5
+#define PE_PASS_EMISSION 6
+#define PE_PASS_BACKGROUND 7
+#define PE_LIGHT_PASSES 8
+
+/* RenderPassFilms (engine/renderer.h:59-100) without cryptomatte: films of the beauty film's size, each
+ * optional. Per sample, as PathTracer::Li credits them (path_tracer.cpp:372-395, :482-698, :790-812) and
+ * tracePixel / addPasses add them (renderer.cpp:162-171, :207-219):
+ *   light[k]         LightPass k; the eight passes of a sample sum to its beauty value
+ *   light_groups[g]  what lights of group g added (the scene descriptor's groups; g < num_light_groups)
+ *   depth            distance from the camera to the first surface, in r, g, b (0 for a miss)
+ *   position         the first surface's point (signed, Film::addSampleSigned)
+ *   object_id        idColor(Scene::nodeUidOf) of the first surface (the descriptor's node_uids)
+ *   material_id      idColor(material index + 1) of the first surface; the reference hashes the material's
+ *                    address, which no other process can reproduce, so the colours differ from its own
+ * Light-pass and group films use Film::addSample (a negative or non-finite value is rejected for that film). */
+typedef struct pe_pass_targets {
+    pe_film* light[PE_LIGHT_PASSES];
+    pe_film* light_groups[PE_MAX_LIGHT_GROUPS];
+    pe_film* depth;
+    pe_film* position;
+    pe_film* object_id;
+    pe_film* material_id;
+} pe_pass_targets;
+
 typedef struct pe_targets {
     pe_film* beauty;   /* required */
     pe_film* albedo;   /* AovFilms::albedo, or NULL */
@@ -271,7 +315,7 @@ typedef struct pe_targets {
      * between iterations like the cancel flag is read; non-zero cancels. NULL: the cancel argument only. */
     int (*cancel_poll)(void* user);
     void* cancel_user;
-    void* reserved[1];
+    const pe_pass_targets* passes;  /* render passes, or NULL */
 } pe_targets;
 
 /* pe_render into every film of `targets` (RenderDevice::render with RenderTargets). */

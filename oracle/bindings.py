@@ -66,6 +66,8 @@ _ORACLE_SIGS = {
     "display": (C.c_int, [C.c_int, C.c_int, _F, _U, _P(capi.Display), _F, _P(C.c_uint8)]),
     "trace_aovs": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I, _F,
                              _F]),
+    "trace_passes": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I,
+                               _F, _F]),
 }
 
 _REF_SIGS = {
@@ -89,7 +91,20 @@ _REF_SIGS = {
                               _F, _F, _F, _F]),
     "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
                                   C.c_float, C.c_int, _F, _U, _F, _P(C.c_uint64)]),
+    "scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p]),
+    "scene_light_groups": (C.c_int, [_VP, _I, _I, C.c_char_p, C.c_size_t]),
+    "trace_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.c_float, C.c_uint64, C.c_size_t, _I, _I, _I, _F, _F]),
 }
+
+PASS_FLOATS = 82  # per sample: 8 light passes, 16 light groups (rgb each), depth, position, object id, material id
+
+
+def split_passes(out: np.ndarray) -> dict:
+    """The per-sample pass record (n, 82) as named arrays."""
+    return {"light": out[:, :24].reshape(-1, 8, 3), "groups": out[:, 24:72].reshape(-1, 16, 3),
+            "depth": out[:, 72], "position": out[:, 73:76], "object_id": out[:, 76:79],
+            "material_id": out[:, 79:82]}
 
 _cache: dict = {}
 
@@ -191,6 +206,20 @@ class OracleScene:
             raise RuntimeError("oracle_trace_aovs failed (feature outside the restated scope)")
         return L, aov[:, 0:3], aov[:, 3:6], aov[:, 6]
 
+    def trace_passes(self, camera, settings, width, height, px, py, s):
+        """Per-sample radiance and render passes -> (L (n, 3), split_passes dict)."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        out = np.zeros((px.shape[0], PASS_FLOATS), np.float32)
+        rc = self._lib.oracle_trace_passes(self.h, C.byref(camera), C.byref(settings), width, height, px.shape[0],
+                                           capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32), capi.ptr(s, C.c_int32),
+                                           _fp(L), _fp(out))
+        if rc != 0:
+            raise RuntimeError("oracle_trace_passes failed (feature outside the restated scope)")
+        return L, split_passes(out)
+
     def render(self, camera, settings, width, height, first, count, threads=1, rows=None, film=None):
         """Accumulates samples [first, first+count) of every pixel (rows [r0, r1) if given) into film
         (sum (H,W,3), count (H,W), lum2 (H,W)); returns (film, seconds, (extension, shadow) rays)."""
@@ -216,12 +245,40 @@ class RefScene:
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
         desc, self._keep = scene.pack()
-        self.h = self._lib.ref_scene_build(C.byref(desc))
+        # Light-group names as the lights and the environment carry them; the reference resolves them itself.
+        names = (C.c_char_p * max(1, len(scene.lights)))(*[g.encode() for g in scene.light_group_of])
+        env_name = scene.environment.light_group.encode() if scene.environment is not None else None
+        self._keep += [names]
+        self.h = self._lib.ref_scene_build_groups(C.byref(desc), names, env_name)
 
     def __del__(self):
         if getattr(self, "h", None):
             self._lib.ref_scene_free(self.h)
             self.h = None
+
+    def light_groups(self):
+        """Scene::lightGroupNames, lightGroupIndex per light, envLightGroupIndex."""
+        nl = self.counts()[1]
+        per = np.zeros(max(1, nl), np.int32)
+        env = C.c_int32()
+        buf = C.create_string_buffer(4096)
+        n = self._lib.ref_scene_light_groups(self.h, capi.ptr(per, C.c_int32), C.byref(env), buf, 4096)
+        names = buf.value.decode().split("\n")[:n]
+        return names, per[:nl].tolist(), env.value
+
+    def trace_passes(self, view, px, py, s, shadow_quality=1, filter_type=0, filter_width=1.0, seed=12345):
+        """Per-sample radiance and render passes along tracePixel's path -> (L (n, 3), split_passes dict)."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        out = np.zeros((px.shape[0], PASS_FLOATS), np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self._lib.ref_trace_passes(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
+                                   view.width, view.height, view.max_bounces, shadow_quality, filter_type,
+                                   filter_width, seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
+                                   capi.ptr(s, C.c_int32), _fp(L), _fp(out))
+        return L, split_passes(out)
 
     def counts(self):
         s, l = C.c_int32(), C.c_int32()
@@ -417,12 +474,40 @@ class HipDeviceSession:
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref/libphoton_hipdevice.so was not built (the reference tree is absent)")
         desc, self._keep = scene.pack()
-        self.h = self._lib.ref_scene_build(C.byref(desc))
+        # Light-group names as the lights and the environment carry them; the reference resolves them itself.
+        names = (C.c_char_p * max(1, len(scene.lights)))(*[g.encode() for g in scene.light_group_of])
+        env_name = scene.environment.light_group.encode() if scene.environment is not None else None
+        self._keep += [names]
+        self.h = self._lib.ref_scene_build_groups(C.byref(desc), names, env_name)
 
     def __del__(self):
         if getattr(self, "h", None):
             self._lib.ref_scene_free(self.h)
             self.h = None
+
+    def light_groups(self):
+        """Scene::lightGroupNames, lightGroupIndex per light, envLightGroupIndex."""
+        nl = self.counts()[1]
+        per = np.zeros(max(1, nl), np.int32)
+        env = C.c_int32()
+        buf = C.create_string_buffer(4096)
+        n = self._lib.ref_scene_light_groups(self.h, capi.ptr(per, C.c_int32), C.byref(env), buf, 4096)
+        names = buf.value.decode().split("\n")[:n]
+        return names, per[:nl].tolist(), env.value
+
+    def trace_passes(self, view, px, py, s, shadow_quality=1, filter_type=0, filter_width=1.0, seed=12345):
+        """Per-sample radiance and render passes along tracePixel's path -> (L (n, 3), split_passes dict)."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        s = np.ascontiguousarray(s, np.int32)
+        L = np.zeros((px.shape[0], 3), np.float32)
+        out = np.zeros((px.shape[0], PASS_FLOATS), np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self._lib.ref_trace_passes(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
+                                   view.width, view.height, view.max_bounces, shadow_quality, filter_type,
+                                   filter_width, seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
+                                   capi.ptr(s, C.c_int32), _fp(L), _fp(out))
+        return L, split_passes(out)
 
     def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0, ao_strength=0.0,
                path_guiding=False):

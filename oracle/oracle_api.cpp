@@ -57,6 +57,13 @@ Scene* buildScene(const pe_scene_desc& d) {
         }
         s->env.rgb = nullptr;
     }
+    if (d.light_groups && d.num_light_groups > 0) {
+        s->lightGroup.assign(d.light_groups, d.light_groups + d.num_lights);
+        s->envLightGroup = d.env_light_group;
+        s->numLightGroups = d.num_light_groups;
+    }
+    s->nodeUid.assign(static_cast<size_t>(d.num_meshes), 0);
+    if (d.node_uids) s->nodeUid.assign(d.node_uids, d.node_uids + d.num_meshes);
     buildBvh(*s);
     return s;
 }

@@ -1179,8 +1179,17 @@ struct PrimaryHit {
     SI isect;
 };
 
+// LightPasses (path_tracer.h:21-30): the eight light passes (LightPass order) and the light groups.
+constexpr int kLightPasses = 8, kMaxLightGroups = 16;
+enum : int { kDiffuseDirect = 0, kDiffuseIndirect, kGlossyDirect, kGlossyIndirect, kTransmissionDirect,
+             kTransmissionIndirect, kEmission, kBackground };
+struct LightPasses {
+    V3 c[kLightPasses];
+    V3 groups[kMaxLightGroups];
+};
+
 V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& ray, Sobol& sampler, RayCounters& rc,
-      PrimaryHit* primary = nullptr) {
+      PrimaryHit* primary = nullptr, LightPasses* passes = nullptr) {
     V3 L(0.0f), throughput(1.0f);  // path_tracer.cpp:321-890
     bool firstSegment = true;
     Ray cur = ray;
@@ -1207,19 +1216,43 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
     auto materialOf = [&](int index) -> const pe_material_desc& {
         return index < 0 ? exitMaterial : scene.materials[static_cast<size_t>(index)];
     };
-    auto accumulate = [&L](V3 c) {
-        if (!isFinite(c)) return;
+    // accumulate returns what it added (black for a non-finite contribution), the value the passes receive.
+    auto accumulate = [&L](V3 c) -> V3 {
+        if (!isFinite(c)) return V3(0.0f);
         L += c;
+        return c;
     };
-    auto accumulateIndirect = [&](V3 c) {
+    auto accumulateIndirect = [&](V3 c) -> V3 {
         if (st.clampIndirect > 0.0f) {
             const float m = std::max({c.x, c.y, c.z});
-            if (m > st.clampIndirect) {
-                accumulate(c * (st.clampIndirect / m));
-                return;
-            }
+            if (m > st.clampIndirect) return accumulate(c * (st.clampIndirect / m));
         }
-        accumulate(c);
+        return accumulate(c);
+    };
+    // Render passes and light groups (path_tracer.cpp:372-395): the lobe shares of the direction sampled at
+    // the first surface (per channel) route everything found after it; light found by the first bounce is
+    // direct, later light indirect.
+    V3 firstLobe[3] = {V3(1.0f), V3(0.0f), V3(0.0f)};
+    const int numLightGroups = scene.numLightGroups;
+    auto lightGroupIndex = [&](int li) {  // Scene::lightGroupIndex (scene.cpp:258-261)
+        return li >= 0 && static_cast<size_t>(li) < scene.lightGroup.size() ? scene.lightGroup[static_cast<size_t>(li)] : -1;
+    };
+    auto passAt = [](int lobe, bool direct) {
+        return lobe == 0 ? (direct ? kDiffuseDirect : kDiffuseIndirect)
+                         : lobe == 1 ? (direct ? kGlossyDirect : kGlossyIndirect)
+                                     : (direct ? kTransmissionDirect : kTransmissionIndirect);
+    };
+    auto credit = [&](int pass, V3 added) {
+        if (passes) passes->c[pass] += added;
+    };
+    auto creditGroup = [&](int groupIdx, V3 added) {
+        if (!passes || numLightGroups <= 0 || isBlack(added)) return;
+        const int g = groupIdx >= 0 && groupIdx < numLightGroups ? groupIdx : 0;
+        if (g < kMaxLightGroups) passes->groups[g] += added;
+    };
+    auto creditPath = [&](V3 added, bool direct) {
+        if (!passes) return;
+        for (int k = 0; k < 3; ++k) passes->c[passAt(k, direct)] += added * firstLobe[k];
     };
     for (;;) {
         V3 wo;
@@ -1270,8 +1303,11 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                         weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
                     }
                     const V3 envLocal = Le * weight;
-                    if (diffuseBounces >= 2) accumulateIndirect(throughput * envLocal);
-                    else accumulate(throughput * envLocal);
+                    const V3 added = diffuseBounces >= 2 ? accumulateIndirect(throughput * envLocal)
+                                                         : accumulate(throughput * envLocal);
+                    if (depth == 0) credit(kBackground, added);
+                    else creditPath(added, depth == 1);
+                    creditGroup(scene.envLightGroup, added);
                 }
                 break;
             }
@@ -1285,8 +1321,11 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                     if (lightPdf > 0.0f && lastBsdfPdf > 0.0f) weight = powerHeuristic(1, lastBsdfPdf, shadowSamples, lightPdf);
                 }
                 const V3 emitLocal = emitted * weight;
-                if (diffuseBounces >= 2) accumulateIndirect(throughput * emitLocal);
-                else accumulate(throughput * emitLocal);
+                const V3 added = diffuseBounces >= 2 ? accumulateIndirect(throughput * emitLocal)
+                                                     : accumulate(throughput * emitLocal);
+                if (depth == 0) credit(kEmission, added);
+                else creditPath(added, depth == 1);
+                creditGroup(li >= 0 ? lightGroupIndex(li) : 0, added);
             }
         }
         if (depth >= st.maxDepth) break;  // :596-599
@@ -1327,8 +1366,24 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                         const V3 neeLocal = f * ls.Li * vis *
                                             (cosTheta * weight / (lightPdf * static_cast<float>(shadowSamples)));
                         const V3 c = throughput * neeLocal;
-                        if (diffuseBounces >= 1) accumulateIndirect(c);
-                        else accumulate(c);
+                        const V3 added = diffuseBounces >= 1 ? accumulateIndirect(c) : accumulate(c);
+                        if (passes && !isBlack(added)) {  // :679-693
+                            creditGroup(fromEnv ? scene.envLightGroup : lightGroupIndex(lightIndex), added);
+                            if (depth == 0) {
+                                // The first surface: this light sample's value split over the lobe groups.
+                                const LobeSplit sp3 = evalSplit(mat, wo, ls.wi, sp);
+                                const V3 parts[3] = {sp3.diffuse, sp3.glossy, sp3.transmission};
+                                const V3 sum = sp3.diffuse + sp3.glossy + sp3.transmission;
+                                for (int k = 0; k < 3; ++k) {
+                                    V3 frac;
+                                    for (int ch = 0; ch < 3; ++ch)
+                                        frac[ch] = sum[ch] > 0.0f ? parts[k][ch] / sum[ch] : (k == 0 ? 1.0f : 0.0f);
+                                    passes->c[passAt(k, true)] += added * frac;
+                                }
+                            } else {
+                                creditPath(added, false);
+                            }
+                        }
                     }
                 }
             }
@@ -1361,6 +1416,23 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
             continue;
         }
         const V3 wi = bs.wi;
+        if (passes && depth == 0) {  // :789-812: the first surface's lobe shares of the sampled direction
+            const bool refracted = dot(wo, isect.ng) * dot(wi, isect.ng) < 0.0f;
+            if (bs.delta) {
+                firstLobe[0] = V3(0.0f);
+                firstLobe[1] = refracted ? V3(0.0f) : V3(1.0f);
+                firstLobe[2] = refracted ? V3(1.0f) : V3(0.0f);
+            } else {
+                const LobeSplit b = evalSplit(mat, wo, wi, sp);
+                const V3 sum = b.diffuse + b.glossy + b.transmission;
+                for (int ch = 0; ch < 3; ++ch) {
+                    const bool ok = sum[ch] > 0.0f;
+                    firstLobe[0][ch] = ok ? b.diffuse[ch] / sum[ch] : 1.0f;
+                    firstLobe[1][ch] = ok ? b.glossy[ch] / sum[ch] : 0.0f;
+                    firstLobe[2][ch] = ok ? b.transmission[ch] / sum[ch] : 0.0f;
+                }
+            }
+        }
         const float cosTheta = std::abs(dot(wi, isect.normal));  // :814-816
         throughput *= bs.f * (cosTheta / bs.pdf);
         if (!isFinite(throughput)) break;
@@ -1440,6 +1512,28 @@ struct FilterSampler {  // pixel_filter.cpp:37-87
     }
 };
 
+// idColor (renderer.cpp:252-273): an id's stable colour, splitmix64 to a hue stepped by the golden ratio, HSV
+// with value 1 and saturation 0.75; id 0 is black.
+V3 idColor(uint64_t id) {
+    if (id == 0) return V3(0.0f);
+    uint64_t z = id + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    const float h = std::fmod(static_cast<float>(z >> 40) * 0x1p-24f + 0.618034f, 1.0f) * 6.0f;
+    const int i = static_cast<int>(h) % 6;
+    const float f = h - std::floor(h);
+    const float v = 1.0f, sat = 0.75f, p0 = v * (1.0f - sat), q = v * (1.0f - sat * f), t = v * (1.0f - sat * (1.0f - f));
+    switch (i) {
+        case 0: return V3(v, t, p0);
+        case 1: return V3(q, v, p0);
+        case 2: return V3(p0, v, t);
+        case 3: return V3(p0, q, v);
+        case 4: return V3(t, p0, v);
+        default: return V3(v, p0, q);
+    }
+}
+
 struct Tracer {
     const Scene& scene;
     const Integrator& it;
@@ -1486,6 +1580,50 @@ struct Tracer {
         }
         const float v[7] = {albedo.x, albedo.y, albedo.z, normal.x, normal.y, normal.z, alpha};
         std::copy(v, v + 7, aov);
+        return L;
+    }
+
+    // tracePixel with render passes (renderer.cpp:162-171): the light passes and groups Li credits, and the
+    // primary hit's data passes (depth from the camera, position, idColor of its node uid and of its material
+    // index + 1; all zero for a miss). out: 24 light-pass floats, 48 group floats, depth, position xyz, object
+    // id rgb, material id rgb (82 floats).
+    V3 tracePasses(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* out) const {
+        sampler.startPixel(x, y);
+        sampler.startSample(s);
+        float fu, fv;
+        sampler.get2D(fu, fv);
+        const float ox = filter.sample1D(fu), oy = filter.sample1D(fv);
+        const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(width);
+        const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(height);
+        float lx, ly;
+        sampler.get2D(lx, ly);
+        const V3 pos = c3(cam.position), ll = c3(cam.lower_left), hor = c3(cam.horizontal), ver = c3(cam.vertical);
+        const V3 target = ll + sx * hor + sy * ver;
+        Ray ray;
+        ray.o = pos;
+        ray.d = normalized(target - pos);
+        PrimaryHit primary;
+        LightPasses lp;
+        const V3 L = Li(scene, it, st, ray, sampler, rc, &primary, &lp);
+        for (int k = 0; k < kLightPasses; ++k)
+            for (int c = 0; c < 3; ++c) out[3 * k + c] = lp.c[k][c];
+        for (int g = 0; g < kMaxLightGroups; ++g)
+            for (int c = 0; c < 3; ++c) out[24 + 3 * g + c] = lp.groups[g][c];
+        std::fill(out + 72, out + 82, 0.0f);
+        if (primary.hit) {
+            const SI& h = primary.isect;
+            out[72] = length(h.point - ray.o);
+            out[73] = h.point.x;
+            out[74] = h.point.y;
+            out[75] = h.point.z;
+            const int shape = h.tri >= 0 ? scene.tris[static_cast<size_t>(h.tri)].shape : -1;
+            const V3 obj = idColor(shape >= 0 ? scene.nodeUid[static_cast<size_t>(shape)] : 0);
+            const V3 mat = idColor(h.material >= 0 ? static_cast<uint64_t>(h.material) + 1 : 0);
+            for (int c = 0; c < 3; ++c) {
+                out[76 + c] = obj[c];
+                out[79 + c] = mat[c];
+            }
+        }
         return L;
     }
 };
@@ -1619,6 +1757,33 @@ ORACLE_API int oracle_trace_aovs(void* sp, const pe_camera* cam, const pe_settin
         RayCounters rc;
         for (size_t i = 0; i < n; ++i) {
             const V3 c = tr.trace(px[i], py[i], s[i], sampler, rc, aov + 7 * i);
+            L[3 * i] = c.x;
+            L[3 * i + 1] = c.y;
+            L[3 * i + 2] = c.z;
+        }
+        return 0;
+    } catch (const std::exception&) {
+        return -1;
+    }
+}
+
+// Per-sample radiance and render passes (82 floats each, Tracer::tracePasses) for n (px, py, sample) triples.
+ORACLE_API int oracle_trace_passes(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,
+                                   size_t n, const int32_t* px, const int32_t* py, const int32_t* s, float* L,
+                                   float* out) {
+    try {
+        const Scene& scene = *static_cast<Scene*>(sp);
+        const Integrator& it = integratorOf(scene);
+        Settings st;
+        st.maxDepth = ps->max_bounces;
+        st.rrDepth = ps->rr_depth;
+        st.shadowSamples = std::max(1, ps->shadow_quality);
+        st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
+        Sobol sampler(ps->seed);
+        RayCounters rc;
+        for (size_t i = 0; i < n; ++i) {
+            const V3 c = tr.tracePasses(px[i], py[i], s[i], sampler, rc, out + 82 * i);
             L[3 * i] = c.x;
             L[3 * i + 1] = c.y;
             L[3 * i + 2] = c.z;

@@ -35,6 +35,11 @@ struct Scene {
     bool hasEnv = false;
     pe_env_desc env{};
     std::vector<float> envRgb;
+    // Light groups (scene.cpp:144-173) as the descriptor resolves them, and per-shape node uids.
+    std::vector<int> lightGroup;  // per light; empty: no groups
+    int envLightGroup = -1;
+    int numLightGroups = 0;
+    std::vector<uint64_t> nodeUid;  // per shape (mesh of the descriptor)
 
     // BVH
     std::vector<LinearNode> nodes;

@@ -540,7 +540,7 @@ V3 eonEval(V3 rho, float r, V3 wi, V3 wo) {  // eon.h:37-51
     return fss + rhoMs * (INV_PI * ms);
 }
 
-V3 evalLobes(V3 wo, V3 wi, const Params& p) {  // :223-311 (no flakes)
+V3 evalLobes(V3 wo, V3 wi, const Params& p, LobeSplit* split = nullptr) {  // :223-311 (no flakes)
     const Frame frame = shadingFrame(p.normal, p.tangent);
     const V3 woL = frame.toLocal(wo), wiL = frame.toLocal(wi);
     if (woL.z <= 0.0f) return V3(0.0f);
@@ -560,12 +560,13 @@ V3 evalLobes(V3 wo, V3 wi, const Params& p) {  // :223-311 (no flakes)
             ft += p.baseColor * (trans * INV_PI * coatO * dl.factor(cosThetaO, muI) * sh) * cTint;
         }
         if (stW > 0.0f) ft += evalSpecTransmission(woL, wiL, p) * coatO * cTint;
+        if (split) split->transmission = ft;
         return ft;
     }
     const V3 hL = normalized(woL + wiL);
     const float cosThetaI = wiL.z;
     const float cosThetaD = dot(woL, hL);
-    V3 base;
+    V3 base, baseDiffuse;
     {
         const DielectricLayer dl(p);
         const float diffW = opaqueDielectric * (1.0f - clampf(p.diffuseTransmission, 0.0f, 1.0f)) * (1.0f - p.subsurface);
@@ -583,13 +584,21 @@ V3 evalLobes(V3 wo, V3 wi, const Params& p) {  // :223-311 (no flakes)
         }
         const V3 fSpecular = (F * D * G) / (4.0f * cosThetaO * cosThetaI) * multiScatterFactor(p, cosThetaO, ax, ay);
         base = fDiffuse + fSpecular;
+        baseDiffuse = fDiffuse;
         const V3 sc = resolvedSheenColor(p);
         if (!isBlack(sc)) {
             const float s = sheenScale(p, std::max({sc.x, sc.y, sc.z}), cosThetaO, cosThetaI);
             base = base * s + evalSheen(sc, p, cosThetaO, cosThetaI, hL.z);
+            baseDiffuse = baseDiffuse * s;
         }
     }
-    if (p.clearCoat <= 0.0f) return base;
+    if (p.clearCoat <= 0.0f) {  // :290-296: the diffuse lobe, the rest (specular, sheen) glossy
+        if (split) {
+            split->diffuse = baseDiffuse;
+            split->glossy = base - baseDiffuse;
+        }
+        return base;
+    }
     const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
     const float Dc = ggxD(hL, alphaC, alphaC);
     const float Gc = ggxG(woL, wiL, alphaC, alphaC);
@@ -598,6 +607,10 @@ V3 evalLobes(V3 wo, V3 wi, const Params& p) {  // :223-311 (no flakes)
     const float msC = 1.0f + f0c * (1.0f / ggxDirectionalAlbedo(cosThetaO, alphaC) - 1.0f);
     const V3 fCoat(p.clearCoat * Fc * Dc * Gc / (4.0f * cosThetaO * cosThetaI) * msC);
     const V3 coatAtt = cTint * (coatO * coatTransmit(p, cosThetaI));
+    if (split) {  // :306-309
+        split->diffuse = baseDiffuse * coatAtt;
+        split->glossy = (base - baseDiffuse) * coatAtt + fCoat;
+    }
     return base * coatAtt + fCoat;
 }
 
@@ -1022,6 +1035,27 @@ V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, floa
     const Params p = resolve(m, sp);  // disney.cpp:664-674
     pdf = pdfLobes(wo, wi, p);
     return evalLobes(wo, wi, p);
+}
+
+LobeSplit evalSplit(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp) {
+    LobeSplit s;
+    if (m.type == PE_MAT_DISNEY) {  // disney.cpp:654-658
+        evalLobes(wo, wi, resolve(m, sp), &s);
+        return s;
+    }
+    // Material::evalSplit (material.h:154-161) for Lambert and Mirror, Dielectric::evalSplit
+    // (dielectric.cpp:279-286): the value on the geometric normal's side of wo is diffuse (glossy for glass),
+    // across it transmission.
+    float pdf = 0.0f;
+    const V3 f = evalPdf(m, wo, wi, sp, pdf);
+    const V3 n = lengthSq(sp.ng) > 0.0f ? sp.ng : sp.normal;
+    if (dot(wo, n) * dot(wi, n) >= 0.0f) {
+        if (m.type == PE_MAT_DIELECTRIC) s.glossy = f;
+        else s.diffuse = f;
+    } else {
+        s.transmission = f;
+    }
+    return s;
 }
 
 bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1, BsdfSample& out) {

@@ -96,6 +96,11 @@ float sampleWavelength(float u, V3& weight);
 float cauchyIor(float nd, float abbe, float lambdaNm);
 V3 materialEmitted(const pe_material_desc& m);
 V3 evalPdf(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp, float& pdf);
+// Material::evalSplit (material.h:51-57, :154-161): evalPdf's value split into diffuse, glossy and transmission.
+struct LobeSplit {
+    V3 diffuse{0.0f}, glossy{0.0f}, transmission{0.0f};
+};
+LobeSplit evalSplit(const pe_material_desc& m, V3 wo, V3 wi, const ShadingPoint& sp);
 bool sampleFull(const pe_material_desc& m, V3 wo, const ShadingPoint& sp, float uc, float u0, float u1, BsdfSample& out);
 
 

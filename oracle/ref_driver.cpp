@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <string>
 #include <memory>
 #include <unordered_map>
 #include <vector>
@@ -79,6 +80,7 @@ struct RefScene {
     std::shared_ptr<Scene> scene;
     std::vector<std::shared_ptr<Material>> materials;
     std::unordered_map<const void*, int> shapeIndex;
+    std::vector<int> meshMaterial;  // per descriptor mesh (= Scene::shapes() index): its material index
 };
 
 bool toRay(const float* r, Ray& ray) {
@@ -117,7 +119,11 @@ REF_API int ref_sobol_batch(uint64_t seed, size_t n, const int32_t* px, const in
     return 0;
 }
 
-REF_API void* ref_scene_build(const pe_scene_desc* desc) {
+// The scene of a descriptor; with names, area light i's group name (Light::setLightGroup, "" = none; the
+// triangle lights Scene::addShape makes for emissive meshes have none) and the environment's, which
+// Scene::buildAccelerator resolves into group indices (scene.cpp:144-173). Meshes are tagged with the
+// descriptor's node uids (Scene::tagShape).
+void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, const char* envGroupName) {
     Scene::setPreferOwnBvh(true);
     auto rs = new RefScene();
     rs->scene = std::make_shared<Scene>();
@@ -133,6 +139,7 @@ REF_API void* ref_scene_build(const pe_scene_desc* desc) {
             const pe_light_desc& L = desc->lights[li];
             auto area = std::make_shared<AreaLight>(v3(L.p0), v3(L.p1), v3(L.p2), c3(L.radiance));
             area->setVisibleToCamera(L.visible_to_camera != 0);
+            if (lightGroupNames && lightGroupNames[li]) area->setLightGroup(lightGroupNames[li]);
             rs->scene->addLight(area);
             continue;
         }
@@ -148,8 +155,11 @@ REF_API void* ref_scene_build(const pe_scene_desc* desc) {
             for (uint32_t v = 0; v < m.num_vertices; ++v) uv[v] = Vec2f(m.uvs[2 * v], m.uvs[2 * v + 1]);
         }
         std::vector<uint32_t> idx(m.indices, m.indices + 3 * static_cast<size_t>(m.num_triangles));
-        rs->scene->addShape(std::make_shared<TriangleMesh>(pos, nrm, uv, idx, rs->materials[static_cast<size_t>(m.material)].get()));
+        auto shape = std::make_shared<TriangleMesh>(pos, nrm, uv, idx, rs->materials[static_cast<size_t>(m.material)].get());
+        if (desc->node_uids && desc->node_uids[i] != 0) rs->scene->tagShape(shape.get(), desc->node_uids[i]);
+        rs->scene->addShape(shape);
     }
+    for (int i = 0; i < desc->num_meshes; ++i) rs->meshMaterial.push_back(desc->meshes[i].material);
     for (auto& mat : rs->materials) rs->scene->retainMaterial(mat);
     if (desc->environment) {
         const pe_env_desc& e = *desc->environment;
@@ -164,6 +174,7 @@ REF_API void* ref_scene_build(const pe_scene_desc* desc) {
         }
         auto env = std::make_shared<EnvironmentLight>(img, e.rotation, e.intensity);
         if (e.has_sun) env->setSun(v3(e.sun_direction), e.sun_cos_max, c3(e.sun_radiance));
+        if (envGroupName) env->setLightGroup(envGroupName);
         rs->scene->setEnvironment(env);
     }
     rs->scene->buildAccelerator();
@@ -172,7 +183,26 @@ REF_API void* ref_scene_build(const pe_scene_desc* desc) {
     return rs;
 }
 
+REF_API void* ref_scene_build(const pe_scene_desc* desc) { return buildRef(desc, nullptr, nullptr); }
+REF_API void* ref_scene_build_groups(const pe_scene_desc* desc, const char* const* lightGroupNames,
+                                     const char* envGroupName) {
+    return buildRef(desc, lightGroupNames, envGroupName);
+}
+
 REF_API void ref_scene_free(void* h) { delete static_cast<RefScene*>(h); }
+
+// The scene's resolved light groups: Scene::lightGroupIndex(i) per light, envLightGroupIndex and the group
+// names ('\n'-separated into `names`). Returns the number of groups, or -1 if `names` is too small.
+REF_API int ref_scene_light_groups(void* h, int32_t* lightToGroup, int32_t* envGroup, char* names, size_t namesCap) {
+    const Scene& sc = *static_cast<RefScene*>(h)->scene;
+    for (size_t i = 0; i < sc.lights().size(); ++i) lightToGroup[i] = sc.lightGroupIndex(static_cast<int>(i));
+    *envGroup = sc.envLightGroupIndex();
+    std::string joined;
+    for (const std::string& n : sc.lightGroupNames()) joined += n + "\n";
+    if (joined.size() + 1 > namesCap) return -1;
+    std::memcpy(names, joined.c_str(), joined.size() + 1);
+    return static_cast<int>(sc.lightGroupNames().size());
+}
 
 // The reference's own thinFilmReflectance (materials/fresnel.h) for n queries.
 REF_API int ref_thin_film(size_t n, const float* cos1, float n1, float n2, float n3re, float n3im, const float* thickness,
@@ -323,6 +353,61 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
         ray.coneSpread = cam.pixelConeSpread(height);
         const Color3f c = integrator.Li(ray, *rs->scene, sampler);
         L[3 * i] = c.r; L[3 * i + 1] = c.g; L[3 * i + 2] = c.b;
+    }
+    return 0;
+}
+
+// Per-sample radiance and render passes along tracePixel's path with wantPasses (renderer.cpp:162-171):
+// PathTracer::Li with a LightPasses, then the primary hit's data passes. out, 82 floats per sample: the eight
+// light passes, the sixteen light groups, depth, position, idColor(nodeUidOf) and idColor(material index + 1)
+// (the descriptor's index of the hit mesh's material, where the reference hashes its address).
+REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
+                             float aspect, int width, int height, int maxBounces, int shadowQuality, int filterType,
+                             float filterWidth, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
+                             const int32_t* sampleIndex, float* L, float* out) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    PixelFilter pf;
+    pf.type = static_cast<PixelFilterType>(filterType);
+    pf.width = filterWidth;
+    const PixelFilterSampler filter(pf);
+    PathTracer integrator(maxBounces, 3, 0.0f, shadowQuality);
+    SobolSampler sampler(seed);
+    for (size_t i = 0; i < n; ++i) {
+        sampler.startPixel(px[i], py[i]);
+        sampler.startSample(sampleIndex[i]);
+        const Vec2f offset = filter.sample(sampler.get2D());
+        const float sx = (static_cast<float>(px[i]) + 0.5f + offset.x) / static_cast<float>(width);
+        const float sy = 1.0f - (static_cast<float>(py[i]) + 0.5f + offset.y) / static_cast<float>(height);
+        const Vec2f lens = sampler.get2D();
+        Ray ray = cam.generateRay(sx, sy, lens);
+        ray.coneWidth = cam.pixelConeWidth(height);
+        ray.coneSpread = cam.pixelConeSpread(height);
+        PathTracer::PrimaryHit primary;
+        LightPasses lp;
+        for (auto& c : lp.c) c = Color3f(0.0f);
+        for (auto& c : lp.lightGroups) c = Color3f(0.0f);
+        const Color3f c = integrator.Li(ray, *rs->scene, sampler, &primary, &lp);
+        L[3 * i] = c.r; L[3 * i + 1] = c.g; L[3 * i + 2] = c.b;
+        float* o = out + 82 * i;
+        for (int k = 0; k < static_cast<int>(LightPass::Count); ++k) {
+            o[3 * k] = lp.c[k].r; o[3 * k + 1] = lp.c[k].g; o[3 * k + 2] = lp.c[k].b;
+        }
+        for (int g = 0; g < LightPasses::kMaxLightGroups; ++g) {
+            o[24 + 3 * g] = lp.lightGroups[g].r; o[25 + 3 * g] = lp.lightGroups[g].g; o[26 + 3 * g] = lp.lightGroups[g].b;
+        }
+        std::fill(o + 72, o + 82, 0.0f);
+        if (primary.hit) {
+            o[72] = (primary.isect.point - ray.origin).length();
+            o[73] = primary.isect.point.x; o[74] = primary.isect.point.y; o[75] = primary.isect.point.z;
+            const Color3f obj = idColor(rs->scene->nodeUidOf(primary.isect));
+            const auto it = rs->shapeIndex.find(primary.isect.hitObject);
+            const int shape = it == rs->shapeIndex.end() ? -1 : it->second;
+            const int mat = shape >= 0 && shape < static_cast<int>(rs->meshMaterial.size()) ? rs->meshMaterial[shape] : -1;
+            const Color3f mc = idColor(mat >= 0 ? static_cast<uint64_t>(mat) + 1 : 0);
+            o[76] = obj.r; o[77] = obj.g; o[78] = obj.b;
+            o[79] = mc.r; o[80] = mc.g; o[81] = mc.b;
+        }
     }
     return 0;
 }

@@ -82,7 +82,24 @@ class SceneDesc(C.Structure):
     _fields_ = [("meshes", C.POINTER(MeshDesc)), ("num_meshes", C.c_int32),
                 ("materials", C.POINTER(MaterialDesc)), ("num_materials", C.c_int32),
                 ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_int32),
-                ("environment", C.POINTER(EnvDesc))]
+                ("environment", C.POINTER(EnvDesc)), ("light_groups", C.POINTER(C.c_int32)),
+                ("env_light_group", C.c_int32), ("num_light_groups", C.c_int32),
+                ("node_uids", C.POINTER(C.c_uint64))]
+
+
+PE_MAX_LIGHT_GROUPS = 16
+# LightPass order (integrators/path_tracer.h:21-24)
+PE_PASS_DIFFUSE_DIRECT, PE_PASS_DIFFUSE_INDIRECT, PE_PASS_GLOSSY_DIRECT, PE_PASS_GLOSSY_INDIRECT = 0, 1, 2, 3
+PE_PASS_TRANSMISSION_DIRECT, PE_PASS_TRANSMISSION_INDIRECT, PE_PASS_EMISSION, PE_PASS_BACKGROUND = 4, 5, 6, 7
+PE_LIGHT_PASSES = 8
+LIGHT_PASS_NAMES = ("DiffuseDirect", "DiffuseIndirect", "GlossyDirect", "GlossyIndirect", "TransmissionDirect",
+                    "TransmissionIndirect", "Emission", "Background")
+
+
+class PassTargets(C.Structure):
+    _fields_ = [("light", C.c_void_p * PE_LIGHT_PASSES), ("light_groups", C.c_void_p * PE_MAX_LIGHT_GROUPS),
+                ("depth", C.c_void_p), ("position", C.c_void_p), ("object_id", C.c_void_p),
+                ("material_id", C.c_void_p)]
 
 
 class Camera(C.Structure):
@@ -111,7 +128,7 @@ CANCEL_POLL = C.CFUNCTYPE(C.c_int, C.c_void_p)
 class Targets(C.Structure):
     _fields_ = [("beauty", C.c_void_p), ("albedo", C.c_void_p), ("normal", C.c_void_p), ("alpha", C.c_void_p),
                 ("guide", C.c_void_p), ("cancel_poll", CANCEL_POLL), ("cancel_user", C.c_void_p),
-                ("reserved", C.c_void_p * 1)]
+                ("passes", C.POINTER(PassTargets))]
 
 
 PE_TONEMAP_REINHARD, PE_TONEMAP_REINHARD_EXTENDED, PE_TONEMAP_ACES, PE_TONEMAP_FILMIC = 0, 1, 2, 3

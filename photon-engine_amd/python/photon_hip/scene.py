@@ -105,6 +105,7 @@ class AreaLight:
     v: tuple
     radiance: tuple
     visible_to_camera: bool = True
+    light_group: str = ""      # Light::setLightGroup (light.h:85): the EXR group LG_<name>; "" = none
 
 
 @dataclass
@@ -116,6 +117,7 @@ class Environment:
     sun_direction: tuple | None = None
     sun_cos_max: float = 1.0
     sun_radiance: tuple = (0.0, 0.0, 0.0)
+    light_group: str = ""
 
 
 @dataclass
@@ -125,6 +127,7 @@ class _Shape:
     emitter_base: int
     standalone: bool
     flags: int = 0
+    node_uid: int = 0          # Scene::tagShape: the owning scene-graph node (object-ID pass), 0 = untagged
 
 
 @dataclass
@@ -133,12 +136,13 @@ class Scene:
     shapes: list = field(default_factory=list)
     lights: list = field(default_factory=list)     # LightDesc in Scene::lights() order
     environment: Environment | None = None
+    light_group_of: list = field(default_factory=list)  # per light: its group name ("" = none)
 
     def add_material(self, m: capi.MaterialDesc) -> int:
         self.materials.append(m)
         return len(self.materials) - 1
 
-    def add_shape(self, mesh: Mesh, material: int) -> None:
+    def add_shape(self, mesh: Mesh, material: int, node_uid: int = 0) -> None:
         mat = self.materials[material]
         emissive = any(float(x) != 0.0 for x in mat.emission)
         base = -1
@@ -152,7 +156,8 @@ class Scene:
                 L.radiance[:] = list(mat.emission)
                 L.visible_to_camera = 1
                 self.lights.append(L)
-        self.shapes.append(_Shape(mesh, material, base, False))
+                self.light_group_of.append("")
+        self.shapes.append(_Shape(mesh, material, base, False, 0, int(node_uid)))
 
     def add_light(self, light: AreaLight) -> None:
         index = len(self.lights)
@@ -164,6 +169,7 @@ class Scene:
         L.radiance[:] = list(map(float, light.radiance))
         L.visible_to_camera = 1 if light.visible_to_camera else 0
         self.lights.append(L)
+        self.light_group_of.append(light.light_group)
         # The quad inserted into the BVH as two black emissive Lambert triangles (scene.cpp:77-98).
         mat = self.add_material(lambert((0.0, 0.0, 0.0), light.radiance))
         p0 = np.asarray(light.position, dtype=np.float32)
@@ -190,6 +196,29 @@ class Scene:
     @property
     def num_triangles(self) -> int:
         return sum(s.mesh.num_triangles for s in self.shapes)
+
+    def light_groups(self):
+        """Scene::buildAccelerator's light-group tables (engine/scene.cpp:144-173): when any light or the
+        environment names a group, every light belongs to its group or "Default" and the environment to its
+        group or "Env", groups numbered in first-seen order (lights first) and capped at 16, a name past the cap
+        falling into group 0. Returns (names, per-light index, environment index); no groups: ([], [-1, ...], -1)."""
+        env = self.environment
+        explicit = (env is not None and env.light_group != "") or any(g != "" for g in self.light_group_of)
+        if not explicit:
+            return [], [-1] * len(self.lights), -1
+        names: list = []
+
+        def group(name):
+            if name in names:
+                return names.index(name)
+            if len(names) >= capi.PE_MAX_LIGHT_GROUPS:
+                return 0
+            names.append(name)
+            return len(names) - 1
+
+        per_light = [group(g if g else "Default") for g in self.light_group_of]
+        env_index = group(env.light_group if env.light_group else "Env") if env is not None else -1
+        return names, per_light, env_index
 
     def pack(self):
         """Returns (SceneDesc, keepalive). keepalive owns every buffer the descriptor points into."""
@@ -218,6 +247,20 @@ class Scene:
         desc.lights = C.cast(lights, C.POINTER(capi.LightDesc))
         desc.num_lights = len(self.lights)
         keep += [meshes, mats, lights]
+        names, per_light, env_index = self.light_groups()
+        if names:
+            groups = np.ascontiguousarray(per_light + [0], dtype=np.int32)
+            keep.append(groups)
+            desc.light_groups = capi.ptr(groups, C.c_int32)
+            desc.env_light_group = env_index
+            desc.num_light_groups = len(names)
+        else:
+            desc.light_groups = C.cast(None, C.POINTER(C.c_int32))
+            desc.env_light_group = -1
+            desc.num_light_groups = 0
+        uids = np.array([s.node_uid for s in self.shapes] + [0], dtype=np.uint64)
+        keep.append(uids)
+        desc.node_uids = capi.ptr(uids, C.c_uint64)
         if self.environment is not None:
             e = self.environment
             env = capi.EnvDesc()

@@ -1,0 +1,128 @@
+"""Render passes: the eight light passes, light groups and the data passes (RenderPassFilms, engine/renderer.h:
+59-100; PathTracer::Li's crediting, integrators/path_tracer.cpp:372-395, :482-698, :790-812; tracePixel /
+addPasses, engine/renderer.cpp:162-171, :207-219).
+
+CPU: the oracle's restated crediting equals the reference's own strict build sample for sample (every pass,
+every group, the data passes) on the scene of the reference's own pass test (tests/test_render_passes.cpp:33-53)
+with light groups; Scene::buildAccelerator's group tables equal the scene description's; the passes sum to the
+beauty value of every sample and each lobe lands where the reference's test expects it.
+GPU: the wavefront kernels' pass films against the oracle per sample (one sample per pixel, so a film pixel is
+one sample), the passes summing to the beauty film, and light groups summing to it.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import bindings
+from photon_hip import capi, configs
+from photon_hip.device import camera_perspective, settings
+from photon_hip.scene import AreaLight, Environment, Scene, lambert
+
+
+def _cam(view):
+    return camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+
+
+def _need_ref():
+    if bindings.reference(strict=True) is None:
+        pytest.skip("oracle/_ref not built here (reference tree absent)")
+
+
+def _samples(view, n, seed, max_sample=64):
+    rng = np.random.default_rng(seed)
+    return (rng.integers(0, view.width, n), rng.integers(0, view.height, n), rng.integers(0, max_sample, n))
+
+
+def _group_scene(env_group="", emitter_rows=2):
+    """Area lights in named groups, an emissive mesh (no group) and an environment."""
+    sc = Scene()
+    m = sc.add_material(lambert((0.5, 0.5, 0.5)))
+    e = sc.add_material(lambert((0.0, 0.0, 0.0), (3.0, 3.0, 3.0)))
+    sc.add_shape(configs.icosphere(1, 0.5, (0.0, 0.0, 0.0)), m)
+    sc.add_shape(configs.grid_quad((-1, 2, -1), (2, 0, 0), (0, 0, 2), emitter_rows, 1), e)
+    sc.add_light(AreaLight((1, 1, 1), (0.3, 0, 0), (0, 0, 0.3), (5, 5, 5), light_group="Key"))
+    sc.add_light(AreaLight((-1, 1, 1), (0.3, 0, 0), (0, 0, 0.3), (5, 5, 5)))
+    sc.add_light(AreaLight((0, 1, -1), (0.3, 0, 0), (0, 0, 0.3), (5, 5, 5), light_group="Rim"))
+    sc.add_light(AreaLight((0, -1, 1), (0.3, 0, 0), (0, 0, 0.3), (5, 5, 5), light_group="Key"))
+    sc.set_environment(Environment(None, 0.0, 0.5, light_group=env_group))
+    return sc
+
+
+@pytest.mark.parametrize("env_group", ["", "Sky", "Key"])
+def test_light_group_tables_equal_reference(env_group):
+    _need_ref()
+    sc = _group_scene(env_group)
+    names, per, env = sc.light_groups()
+    rn, rp, re = bindings.RefScene(sc, strict=True).light_groups()
+    assert (names, per, env) == (rn, rp, re)
+    assert names[:2] == ["Default", "Key"]
+
+
+def test_light_group_tables_without_explicit_groups_and_past_the_cap():
+    plain = Scene()
+    plain.add_material(lambert((0.5, 0.5, 0.5)))
+    plain.add_light(AreaLight((0, 1, 0), (1, 0, 0), (0, 0, 1), (1, 1, 1)))
+    plain.set_environment(Environment(None))
+    assert plain.light_groups() == ([], [-1], -1)
+    # 20 named groups: the first 16 names get indices, later ones fall into group 0 (scene.cpp:160).
+    many = Scene()
+    many.add_material(lambert((0.5, 0.5, 0.5)))
+    for k in range(20):
+        many.add_light(AreaLight((k, 1, 0), (1, 0, 0), (0, 0, 1), (1, 1, 1), light_group=f"G{k}"))
+    names, per, env = many.light_groups()
+    assert len(names) == capi.PE_MAX_LIGHT_GROUPS and per[:16] == list(range(16)) and per[16:] == [0] * 4
+    assert env == -1
+    if bindings.reference(strict=True) is not None:
+        assert bindings.RefScene(many, strict=True).light_groups() == (names, per, env)
+
+
+@pytest.mark.parametrize("groups", [True, False])
+def test_oracle_passes_equal_reference_per_sample(groups):
+    _need_ref()
+    sc, view = configs.pass_studio(groups=groups)
+    o = bindings.OracleScene(sc)
+    ref = bindings.RefScene(sc, strict=True)
+    px, py, s = _samples(view, 2500, 5)
+    st = settings(max_bounces=view.max_bounces, filter_type=capi.PE_FILTER_BOX, filter_width=1.0, seed=12345)
+    Lo, po = o.trace_passes(_cam(view), st, view.width, view.height, px, py, s)
+    Lr, pr = ref.trace_passes(view, px, py, s, filter_type=0, filter_width=1.0, seed=12345)
+    np.testing.assert_array_equal(Lo, Lr)
+    for key in po:
+        np.testing.assert_array_equal(po[key], pr[key], err_msg=key)
+    # Every pass is exercised by these samples (the reference test's expectations, test_render_passes.cpp:88-93).
+    tot = po["light"].sum(axis=(0, 2))
+    for k in (capi.PE_PASS_DIFFUSE_DIRECT, capi.PE_PASS_DIFFUSE_INDIRECT, capi.PE_PASS_GLOSSY_DIRECT,
+              capi.PE_PASS_GLOSSY_INDIRECT, capi.PE_PASS_TRANSMISSION_INDIRECT, capi.PE_PASS_EMISSION,
+              capi.PE_PASS_BACKGROUND):
+        assert tot[k] > 0.0, capi.LIGHT_PASS_NAMES[k]
+    if groups:
+        assert (po["groups"][:, :3].sum(axis=(0, 2)) > 0).all() and not po["groups"][:, 3:].any()
+        assert (po["object_id"].max(axis=1) > 0).mean() > 0.5
+    else:
+        assert not po["groups"].any() and not po["object_id"].any()
+
+
+def test_oracle_passes_sum_to_beauty_and_land_in_place():
+    """Light passes and light groups each sum to the beauty value (test_render_passes.cpp:71-86: relative
+    1e-4); the metal sphere's centre is glossy, the floor diffuse, the glass transmission (:95-103)."""
+    sc, view = configs.pass_studio(width=64, height=48)
+    o = bindings.OracleScene(sc)
+    yy, xx = np.mgrid[0:view.height, 0:view.width]
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    for sample in (0, 7):
+        L, p = o.trace_passes(_cam(view), st, view.width, view.height, xx.ravel(), yy.ravel(),
+                              np.full(xx.size, sample))
+        for part in (p["light"].sum(axis=1), p["groups"].sum(axis=1)):
+            assert (np.abs(part - L) <= 1e-4 * np.maximum(1.0, np.abs(L))).all()
+    light = p["light"].reshape(view.height, view.width, 8, 3)
+    mx, my = int(0.37 * view.width), int(0.52 * view.height)
+    lum = light[my, mx] @ np.array([0.212671, 0.715160, 0.072169])
+    assert lum[2] + lum[3] > 10 * (lum[0] + lum[1]) + 1e-4
+    depth = p["depth"].reshape(view.height, view.width)
+    assert abs(depth[my, mx] - 4.0) < 0.6
+    floor_px = light[view.height - 2, 4]
+    assert floor_px[0].sum() > 0 and floor_px[2:6].sum() == 0
+
+
+# ------------------------------------------------------------------------------------------------- GPU
