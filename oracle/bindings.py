@@ -91,6 +91,8 @@ _REF_SIGS = {
                               _F, _F, _F, _F]),
     "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
                                   C.c_float, C.c_int, _F, _U, _F, _P(C.c_uint64)]),
+    "render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                C.c_int, _F, _F]),
     "scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p]),
     "scene_light_groups": (C.c_int, [_VP, _I, _I, C.c_char_p, C.c_size_t]),
     "trace_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -333,6 +335,19 @@ class RefScene:
                              _fp(mean), capi.ptr(cnt, C.c_uint32), C.byref(secs))
         return mean, cnt, secs.value
 
+    def render_passes(self, view, first, count, threads=4):
+        """Renderer::renderSamples with RenderPassFilms -> (beauty (H,W,3), films (28, H,W,3), groups)."""
+        w, h = view.width, view.height
+        beauty = np.zeros((h, w, 3), np.float32)
+        films = np.zeros((28, h, w, 3), np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        n = self._lib.ref_render_passes(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov,
+                                        view.aspect, w, h, view.max_bounces, first, count, threads, _fp(beauty),
+                                        _fp(films))
+        if n < 0:
+            raise RuntimeError("ref_render_passes failed")
+        return beauty, films, n
+
 
 def display(lib_kind: str, sum_rgb, count, settings):
     """The display path of a film (sums (h, w, 3), counts (h, w)) by the oracle ('oracle') or the reference
@@ -441,6 +456,9 @@ def sobol_reference(seed, px, py, idx, dim):
 
 _HIPDEV_SIGS = {
     "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
+    "ref_scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p]),
+    "hipdev_session_render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                               C.c_int, C.c_int, _F, _F, _U, _I, C.c_char_p, C.c_int]),
     "ref_scene_free": (None, [_VP]),
     "hipdev_session_render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I,
@@ -485,30 +503,6 @@ class HipDeviceSession:
             self._lib.ref_scene_free(self.h)
             self.h = None
 
-    def light_groups(self):
-        """Scene::lightGroupNames, lightGroupIndex per light, envLightGroupIndex."""
-        nl = self.counts()[1]
-        per = np.zeros(max(1, nl), np.int32)
-        env = C.c_int32()
-        buf = C.create_string_buffer(4096)
-        n = self._lib.ref_scene_light_groups(self.h, capi.ptr(per, C.c_int32), C.byref(env), buf, 4096)
-        names = buf.value.decode().split("\n")[:n]
-        return names, per[:nl].tolist(), env.value
-
-    def trace_passes(self, view, px, py, s, shadow_quality=1, filter_type=0, filter_width=1.0, seed=12345):
-        """Per-sample radiance and render passes along tracePixel's path -> (L (n, 3), split_passes dict)."""
-        px = np.ascontiguousarray(px, np.int32)
-        py = np.ascontiguousarray(py, np.int32)
-        s = np.ascontiguousarray(s, np.int32)
-        L = np.zeros((px.shape[0], 3), np.float32)
-        out = np.zeros((px.shape[0], PASS_FLOATS), np.float32)
-        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
-        self._lib.ref_trace_passes(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
-                                   view.width, view.height, view.max_bounces, shadow_quality, filter_type,
-                                   filter_width, seed, px.shape[0], capi.ptr(px, C.c_int32), capi.ptr(py, C.c_int32),
-                                   capi.ptr(s, C.c_int32), _fp(L), _fp(out))
-        return L, split_passes(out)
-
     def render(self, view, spp, max_batch=64, shadow_quality=1, filter_type=0, filter_width=1.0, ao_strength=0.0,
                path_guiding=False):
         """Returns (rc, mean (H,W,3), counts (H,W), batches, error text)."""
@@ -523,6 +517,21 @@ class HipDeviceSession:
             1 if path_guiding else 0, _fp(mean),
             capi.ptr(counts, C.c_uint32), C.byref(batches), err, 512)
         return rc, mean, counts, batches.value, err.value.decode(errors="replace")
+
+    def render_passes(self, view, spp, max_batch):
+        """RenderSession with SessionConfig::passes -> (rc, beauty (H,W,3), films (28, H,W,3), counts, groups,
+        error)."""
+        f3 = C.c_float * 3
+        beauty = np.zeros((view.height, view.width, 3), np.float32)
+        films = np.zeros((28, view.height, view.width, 3), np.float32)
+        counts = np.zeros((view.height, view.width), np.uint32)
+        groups = C.c_int32()
+        err = C.create_string_buffer(512)
+        rc = self._lib.hipdev_session_render_passes(
+            self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
+            view.height, view.max_bounces, spp, max_batch, _fp(beauty), _fp(films), capi.ptr(counts, C.c_uint32),
+            C.byref(groups), err, 512)
+        return rc, beauty, films, counts, groups.value, err.value.decode(errors="replace")
 
     def render_aovs(self, view, spp, max_batch):
         """RenderSession with AOVs -> (rc, [beauty, albedo, normal, alpha] each (H,W,3), counts (H,W), error)."""

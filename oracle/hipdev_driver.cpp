@@ -18,6 +18,7 @@
 using namespace photon;
 
 std::shared_ptr<const Scene> ref_scene_shared(void* h);
+void ref_copy_passes(const RenderPassFilms& pf, int width, int height, float* out);
 
 #define HIPDEV_API __attribute__((visibility("default")))
 
@@ -212,6 +213,71 @@ HIPDEV_API int hipdev_session_render_aovs(void* h, const float* camPos, const fl
             }
             counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
         }
+    return 0;
+}
+
+// RenderSession(Final, targetSpp = spp, maxBatch) with SessionConfig::passes on a HipDevice: the resolved
+// beauty, the pass films (ref_copy_passes layout), the sample counts and the number of light groups it kept.
+HIPDEV_API int hipdev_session_render_passes(void* h, const float* camPos, const float* camTarget, const float* camUp,
+                                            float fov, float aspect, int width, int height, int maxBounces, int spp,
+                                            int maxBatch, float* mean, float* passes, uint32_t* counts, int* groups,
+                                            char* err, int errlen) {
+    std::string e;
+    std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
+    if (!dev) {
+        copy_error(e, err, errlen);
+        return 2;
+    }
+    dev->setScene(ref_scene_shared(h));
+    if (!dev->hasScene()) {
+        copy_error(dev->lastError(), err, errlen);
+        return 1;
+    }
+    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
+                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
+                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = spp;
+    s.maxBounces = maxBounces;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    SessionConfig cfg;
+    cfg.mode = RenderMode::Final;
+    cfg.targetSpp = spp;
+    cfg.maxBatch = maxBatch;
+    cfg.passes = true;
+    RenderSession session(*dev);
+    session.begin(cam, s, cfg);
+    if (!session.warnings().empty()) {
+        copy_error(session.warnings().front(), err, errlen);
+        return 1;
+    }
+    while (!session.finished()) {
+        if (!session.advance(nullptr)) {
+            copy_error(dev->lastError().empty() ? "RenderSession::advance returned false" : dev->lastError(), err,
+                       errlen);
+            return 1;
+        }
+    }
+    const Film& film = session.film();
+    const RenderPassFilms* pf = session.passes();
+    if (!pf) {
+        copy_error("the session kept no pass films", err, errlen);
+        return 1;
+    }
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * static_cast<size_t>(width) + static_cast<size_t>(x);
+            const Color3f c = film.resolvedPixel(x, y);
+            mean[3 * k] = c.r;
+            mean[3 * k + 1] = c.g;
+            mean[3 * k + 2] = c.b;
+            counts[k] = static_cast<uint32_t>(film.sampleCount(x, y));
+        }
+    ref_copy_passes(*pf, width, height, passes);
+    *groups = static_cast<int>(pf->lightGroups.size());
     return 0;
 }
 

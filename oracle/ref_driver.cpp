@@ -565,3 +565,64 @@ REF_API int ref_render_adaptive(void* h, const float* camPos, const float* camTa
 
 // The reference Scene behind a handle of ref_scene_build (for oracle/hipdev_driver.cpp, linked beside it).
 std::shared_ptr<const Scene> ref_scene_shared(void* h) { return static_cast<RefScene*>(h)->scene; }
+
+// The pass films of a RenderPassFilms in pe_pass_targets order (8 light passes, 16 groups, depth, position,
+// object id, material id): film f's resolved pixel k at out[(f * pixels + k) * 3]; absent films stay zero.
+void ref_copy_passes(const RenderPassFilms& pf, int width, int height, float* out) {
+    const size_t pixels = static_cast<size_t>(width) * static_cast<size_t>(height);
+    std::fill(out, out + 28 * pixels * 3, 0.0f);
+    const Film* films[28] = {};
+    for (int k = 0; k < 8; ++k) films[k] = &pf.light[k];
+    for (size_t g = 0; g < pf.lightGroups.size() && g < 16; ++g) films[8 + g] = &pf.lightGroups[g].second;
+    films[24] = &pf.depth;
+    films[25] = &pf.position;
+    films[26] = &pf.objectId;
+    films[27] = &pf.materialId;
+    for (int f = 0; f < 28; ++f) {
+        if (!films[f] || films[f]->width() != width || films[f]->height() != height) continue;
+        for (int y = 0; y < height; ++y)
+            for (int x = 0; x < width; ++x) {
+                const Color3f c = films[f]->resolvedPixel(x, y);
+                float* o = out + (static_cast<size_t>(f) * pixels + static_cast<size_t>(y) * width + x) * 3;
+                o[0] = c.r;
+                o[1] = c.g;
+                o[2] = c.b;
+            }
+    }
+}
+
+extern "C" {
+// Renderer::renderSamples of samples [first, first + count) with RenderPassFilms (box filter of width 1,
+// `threads` workers): the resolved beauty and the pass films (ref_copy_passes layout); returns the number of
+// light groups the renderer made, or -1.
+REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
+                              float aspect, int width, int height, int maxBounces, int firstSample, int count,
+                              int threads, float* beauty, float* passes) {
+    auto* rs = static_cast<RefScene*>(h);
+    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    RenderSettings s;
+    s.width = width;
+    s.height = height;
+    s.samplesPerPixel = firstSample + count;
+    s.maxBounces = maxBounces;
+    s.numThreads = threads;
+    s.filter.type = static_cast<PixelFilterType>(0);
+    s.filter.width = 1.0f;
+    Film film(width, height);
+    RenderPassFilms pf;
+    pf.resize(width, height);
+    Renderer renderer;
+    const SampleBatchResult r = renderer.renderSamples(*rs->scene, cam, s, film, firstSample, count, nullptr, nullptr, &pf);
+    if (!r.completed) return -1;
+    for (int y = 0; y < height; ++y)
+        for (int x = 0; x < width; ++x) {
+            const size_t k = static_cast<size_t>(y) * width + x;
+            const Color3f b = film.resolvedPixel(x, y);
+            beauty[3 * k] = b.r;
+            beauty[3 * k + 1] = b.g;
+            beauty[3 * k + 2] = b.b;
+        }
+    ref_copy_passes(pf, width, height, passes);
+    return static_cast<int>(pf.lightGroups.size());
+}
+}  // extern "C"

@@ -52,6 +52,8 @@ constexpr bool kNodeTile = PE_NODE_TILE != 0;  // traversal reads the top of the
 constexpr uint32_t kMatBuckets = PE_MAT_BUCKETS;  // shade's sort keys for hits (1: hits / misses only)
 static_assert(kMatBuckets == 1 || kMatBuckets == static_cast<uint32_t>(kSortClasses), "one key per shading class");
 constexpr uint64_t kAutoWindowSamples = uint64_t{1} << 28;  // default result buffer: 4 GB
+constexpr uint64_t kPassWindowSamples = uint64_t{1} << 23;  // window limit with pass films (26 records a sample
+                                                            // with 16 groups: 3.5 GB)
 constexpr uint64_t kMaxWindowSamples = (uint64_t{1} << 31) - 64;  // window sample indices are 31-bit
 constexpr uint64_t kDefaultPoolIds = uint64_t{1} << 26;  // default path ids in flight (about 15.6 GB)
 constexpr uint64_t kMaxPoolIds = uint64_t{1} << 30;
@@ -72,6 +74,11 @@ struct PathBuffers {
     float4* res;      // per sample of the window: its radiance (L plus the held-back NEE), read by k_film
     float4* aov_res;  // per sample of the window, two float4: albedo rgb | coverage, shading normal xyz | state
                       // (-1 until the first segment has been resolved, then 0); null without AOV films
+    float4* pass_res; // render passes: per window sample, LaunchParams::pass_count records in pass-major order
+                      // (see pass_add); null without pass films
+    float4* lobe;     // render passes: per path id, the first surface's lobe shares (3 float4: diffuse, glossy,
+                      // transmission rgb)
+    float4* pfrac;    // render passes: per held-back NEE record, its routing (3 float4, see credit_nee)
     float4* hits;     // per queue slot: t, triangle (int bits), u, v
     float4* sq_o;     // shadow queue: origin xyz, w = tmax
     float4* sq_d;     //               direction xyz, w = pend index pid * S + s (uint bits)
@@ -209,6 +216,9 @@ struct LaunchParams {
     int adapt_check;
     float noise_threshold;
     int want_aov;         // record the camera vertex's AOVs (pb.aov_res)
+    int want_passes;      // credit the render passes (pb.pass_res, pb.lobe, pb.pfrac)
+    int num_groups;       // light groups of the scene (0: none)
+    uint32_t pass_count;  // records per sample of pb.pass_res: 8 light passes, num_groups groups, 2 data
     int guided_bounces;   // guiding: vertices of a path below this depth are guided
     uint32_t grec_slots;  // guiding: slots of the training-record buffer
     float4* grec_out;     //           the buffer (3 float4 per record)
@@ -387,6 +397,80 @@ __device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
     return c;
 }
 
+// ------------------------------------------------------------------------------------------ render passes
+// PathTracer::Li's crediting (path_tracer.cpp:372-395): a contribution found at the first surface goes to the
+// Emission / Background pass or, for NEE, to the direct passes by the lobe shares of its light sample; one found
+// after the first bounce to the lobe shares of the direction the first surface sampled (the path's lobe
+// column), direct when found by the first bounce and indirect later; every light contribution also to its light's
+// group. A sample's records live at pass_res[k * num_paths + g]: k < 8 the LightPass order, then the groups, then
+// two data records: the first surface's point | distance from the camera, and its shape | material (int bits) |
+// 0 | state (-1 until the camera ray's first segment is resolved). Paths own their sample, so the read-modify-
+// write needs no atomics, and a pass receives its terms in the reference's order.
+constexpr uint32_t kLightPasses = 8;
+enum : uint32_t { kPassDiffuseDirect = 0, kPassGlossyDirect = 2, kPassTransmissionDirect = 4, kPassEmission = 6,
+                  kPassBackground = 7 };
+
+__device__ __forceinline__ void pass_add(const PathBuffers& pb, const LaunchParams& lp, uint32_t k, uint32_t g, V3 v) {
+    float4* r = pb.pass_res + static_cast<size_t>(k) * lp.num_paths + g;
+    const float4 a = *r;
+    *r = make_float4(a.x + v.x, a.y + v.y, a.z + v.z, a.w);
+}
+// creditGroup (path_tracer.cpp:386-390): a group index out of range counts as group 0.
+__device__ __forceinline__ void credit_group(const PathBuffers& pb, const LaunchParams& lp, uint32_t g, int group,
+                                             V3 added) {
+    if (lp.num_groups <= 0 || is_black(added)) return;
+    const int k = group >= 0 && group < lp.num_groups ? group : 0;
+    pass_add(pb, lp, kLightPasses + static_cast<uint32_t>(k), g, added);
+}
+// creditPath (path_tracer.cpp:392-395): passAt(lobe, direct) is pass 2 lobe + (direct ? 0 : 1).
+__device__ __forceinline__ void credit_path(const PathBuffers& pb, const LaunchParams& lp, uint32_t pid, uint32_t g,
+                                            V3 added, bool direct) {
+    for (uint32_t k = 0; k < 3; ++k) {
+        const float4 f = pb.lobe[3 * static_cast<size_t>(pid) + k];
+        pass_add(pb, lp, 2 * k + (direct ? 0u : 1u), g, added * mk(f.x, f.y, f.z));
+    }
+}
+// Per-channel lobe shares of a split (path_tracer.cpp:683-690, :805-810): part / sum, or all diffuse where the
+// sum is not positive.
+__device__ __forceinline__ void lobe_shares(const LobeSplit& sp, V3 out[3]) {
+    const V3 sum = sp.diffuse + sp.glossy + sp.transmission;
+    const bool ox = sum.x > 0.0f, oy = sum.y > 0.0f, oz = sum.z > 0.0f;
+    out[0] = mk(ox ? sp.diffuse.x / sum.x : 1.0f, oy ? sp.diffuse.y / sum.y : 1.0f, oz ? sp.diffuse.z / sum.z : 1.0f);
+    out[1] = mk(ox ? sp.glossy.x / sum.x : 0.0f, oy ? sp.glossy.y / sum.y : 0.0f, oz ? sp.glossy.z / sum.z : 0.0f);
+    out[2] = mk(ox ? sp.transmission.x / sum.x : 0.0f, oy ? sp.transmission.y / sum.y : 0.0f,
+                oz ? sp.transmission.z / sum.z : 0.0f);
+}
+// A held-back NEE record's routing, written when its shadow ray is issued: pfrac[3 k + j] = the lobe shares of
+// its light sample (first-surface records) in xyz, and in the first record's w (uint bits) bit 0 = first
+// surface, bits 1.. = light group + 1.
+__device__ __forceinline__ uint32_t nee_route(bool first_surface, int group) {
+    return (first_surface ? 1u : 0u) | (static_cast<uint32_t>(group + 1) << 1);
+}
+// The NEE record k of path pid, resolved to c by its shadow ray, into the passes (path_tracer.cpp:679-693).
+__device__ __forceinline__ void credit_nee(const PathBuffers& pb, const LaunchParams& lp, uint32_t pid, uint32_t g,
+                                           size_t k, V3 c) {
+    if (is_black(c)) return;
+    const float4 f0 = pb.pfrac[3 * k];
+    const uint32_t route = __float_as_uint(f0.w);
+    credit_group(pb, lp, g, static_cast<int>(route >> 1) - 1, c);
+    if (route & 1u) {
+        const float4 f1 = pb.pfrac[3 * k + 1], f2 = pb.pfrac[3 * k + 2];
+        pass_add(pb, lp, kPassDiffuseDirect, g, c * mk(f0.x, f0.y, f0.z));
+        pass_add(pb, lp, kPassGlossyDirect, g, c * mk(f1.x, f1.y, f1.z));
+        pass_add(pb, lp, kPassTransmissionDirect, g, c * mk(f2.x, f2.y, f2.z));
+    } else {
+        credit_path(pb, lp, pid, g, c, false);
+    }
+}
+// A contribution found on the path (emission, environment): the first surface's own pass at depth 0, else the
+// path's lobe shares; and its group.
+__device__ __forceinline__ void credit_found(const PathBuffers& pb, const LaunchParams& lp, uint32_t pid, uint32_t g,
+                                             uint32_t depth, uint32_t first_pass, int group, V3 added) {
+    if (depth == 0) pass_add(pb, lp, first_pass, g, added);
+    else credit_path(pb, lp, pid, g, added, depth == 1);
+    credit_group(pb, lp, g, group, added);
+}
+
 // Held-back NEE records in scenes with thin walls: pend.w carries, as uint bits, the deferred indirect clamp
 // (bit 0) and the number of closest-hit queries the shadow ray has made (bits 1-7).
 constexpr uint32_t kPendClamp = 1u;
@@ -475,12 +559,14 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
 
 // A path's result: its radiance plus the held-back NEE of its last vertex, into its sample's slot
 // (the sum Film::addSample receives, path_tracer.cpp:355-359 and the end of Li).
-__device__ __forceinline__ V3 finish_radiance(const PathBuffers& pb, uint32_t pid, V3 L, int S, bool clear) {
+__device__ __forceinline__ V3 finish_radiance(const LaunchParams& lp, const PathBuffers& pb, uint32_t pid, uint32_t g,
+                                              V3 L, int S) {
     for (int s = 0; s < S; ++s) {
         const size_t k = static_cast<size_t>(pid) * S + s;
         const float4 e = pb.pend[k];
         L += mk(e.x, e.y, e.z);
-        if (clear) pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (lp.want_passes) credit_nee(pb, lp, pid, g, k, mk(e.x, e.y, e.z));
+        pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     return L;
 }
@@ -536,7 +622,7 @@ __global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers p
         if (entry & kDeferred) {
             const uint4 ids = pb.ids[pid];
             const float4 L4 = pb.L[pid];
-            const V3 L = finish_radiance(pb, pid, mk(L4.x, L4.y, L4.z), S, true);
+            const V3 L = finish_radiance(lp, pb, pid, ids.x, mk(L4.x, L4.y, L4.z), S);
             pb.res[ids.x] = make_float4(L.x, L.y, L.z, 0.0f);
             if (kGuide && lp.guide_records) written += guide_collect(lp, pb, pid, ids.x, L, pb.ext[pid].w);
         }
@@ -548,6 +634,11 @@ __global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers p
             if (pb.aov_res) {
                 pb.aov_res[2 * static_cast<size_t>(g)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+            }
+            if (lp.want_passes) {
+                for (uint32_t j = 0; j < lp.pass_count; ++j)
+                    pb.pass_res[static_cast<size_t>(j) * lp.num_paths + g] =
+                        make_float4(0.0f, 0.0f, 0.0f, j + 1 == lp.pass_count ? -1.0f : 0.0f);
             }
         }
     }
@@ -618,9 +709,9 @@ struct RenderQueries {
 
 
 // The environment seen by a ray that left the scene, with MIS against NEE unless the last bounce was specular
-// (path_tracer.cpp:482-514), added to the path radiance.
-__device__ __forceinline__ V3 add_miss_radiance(const SceneView& sc, const LaunchParams& lp, V3 L, V3 dir, V3 thr,
-                                                bool specular, float last_pdf, V3 last_p, V3 last_n, uint32_t diffuse) {
+// (path_tracer.cpp:482-514): what the path radiance receives (black for a non-finite value).
+__device__ __forceinline__ V3 miss_radiance(const SceneView& sc, const LaunchParams& lp, V3 dir, V3 thr, bool specular,
+                                            float last_pdf, V3 last_p, V3 last_n, uint32_t diffuse) {
     const V3 le = env_eval(sc.env, dir);
     float weight = 1.0f;
     if (!specular) {
@@ -630,8 +721,7 @@ __device__ __forceinline__ V3 add_miss_radiance(const SceneView& sc, const Launc
     }
     V3 c = thr * (le * weight);
     if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
-    if (finite3(c)) L += c;
-    return L;
+    return finite3(c) ? c : splat(0.0f);
 }
 
 // kTransport = false: the scene has no Dielectric and no transmitting or subsurface material, so no path
@@ -639,8 +729,8 @@ __device__ __forceinline__ V3 add_miss_radiance(const SceneView& sc, const Launc
 // kGuide: the render has a guide; a guided vertex samples the mixture of its queue slot or the BSDF and weighs
 // by the mixture density (path_tracer.cpp:700-735 with the MLP in place of the SD-tree), and records its
 // sample for training.
-// kHitsOnly: the misses are left to k_shade_miss.
-template <bool kTransport, bool kGuide, bool kHitsOnly>
+// kPasses: the render credits the light passes, light groups and data passes (pb.pass_res).
+template <bool kTransport, bool kGuide, bool kPasses>
 __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
     // Block b works on segment b % kQueueSegs of the current queue and appends to the same segment of the next
     // queue, the shadow queue and the next free list (whose segments therefore never receive more than the
@@ -684,7 +774,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             const int tri = __float_as_int(pb.hits[mine].y);
             if (tri >= 0)
                 key = sort_hits ? static_cast<uint32_t>(sc.materials[sc.shade[tri].material].sort_class) : 0u;
-            else if (!kHitsOnly)
+            else
                 key = keys - 1u;
         }
         uint32_t rank = 0;
@@ -741,6 +831,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     const size_t k = static_cast<size_t>(pid) * S + s;
                     const float4 p = pb.pend[k];
                     L += mk(p.x, p.y, p.z);
+                    if (kPasses) credit_nee(pb, lp, pid, ids.x, k, mk(p.x, p.y, p.z));
                     pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
             }
@@ -811,11 +902,19 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                         sc.has_env ? clamped4(env_eval(sc.env, dir), 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                     pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
+                if (kPasses && camera) {  // no primary hit: the data passes stay zero (renderer.cpp:163)
+                    float4* d1 = pb.pass_res + static_cast<size_t>(lp.pass_count - 1) * lp.num_paths + g;
+                    if (d1->w < 0.0f) *d1 = make_float4(__int_as_float(-1), __int_as_float(-1), 0.0f, 0.0f);
+                }
                 // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
                 // environment with MIS against NEE (:482-514).
                 const int top = kTransport ? ext_media_top(ext) : -1;
                 const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
-                if (sc.has_env && !absorbed) L = add_miss_radiance(sc, lp, L, dir, thr, specular, last_pdf, last_p, last_n, diffuse);
+                if (sc.has_env && !absorbed) {
+                    const V3 added = miss_radiance(sc, lp, dir, thr, specular, last_pdf, last_p, last_n, diffuse);
+                    L += added;
+                    if (kPasses) credit_found(pb, lp, pid, g, depth, kPassBackground, sc.env_group, added);
+                }
                 alive = false;
             } else {
                 fill_hit(sc, tri, dir, h.z, h.w, si);
@@ -832,6 +931,14 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     if (lp.want_aov && camera && pb.aov_res[2 * static_cast<size_t>(g) + 1].w < 0.0f) {
                         pb.aov_res[2 * static_cast<size_t>(g)] = clamped4(sc.materials[si.material].base_color, 1.0f);
                         pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
+                    }
+                    if (kPasses && camera) {  // the data passes of the primary hit (renderer.cpp:163-167)
+                        float4* d1 = pb.pass_res + static_cast<size_t>(lp.pass_count - 1) * lp.num_paths + g;
+                        if (d1->w < 0.0f) {
+                            pb.pass_res[static_cast<size_t>(lp.pass_count - 2) * lp.num_paths + g] =
+                                make_float4(si.point.x, si.point.y, si.point.z, length(si.point - lp.cam_pos));
+                            *d1 = make_float4(__int_as_float(sc.shade[tri].shape), __int_as_float(si.material), 0.0f, 0.0f);
+                        }
                     }
                     // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
                     const int top = kTransport ? ext_media_top(ext) : -1;
@@ -858,6 +965,8 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             V3 c = thr * (m.emission * weight);
                             if (diffuse >= 2) c = clamp_indirect(c, lp.clamp_indirect);
                             if (finite3(c)) L += c;
+                            if (kPasses && finite3(c))
+                                credit_found(pb, lp, pid, g, depth, kPassEmission, li >= 0 ? sc.lights[li].group : 0, c);
                         }
                         vertex = true;
                     }
@@ -928,6 +1037,19 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                 if (!finite3(c)) break;
                                 pb.pend[static_cast<size_t>(pid) * S + s] =
                                     make_float4(c.x, c.y, c.z, __uint_as_float(pend_bits));
+                                if (kPasses) {
+                                    const int group = from_env ? sc.env_group : sc.lights[li].group;
+                                    float4* fr = pb.pfrac + 3 * (static_cast<size_t>(pid) * S + s);
+                                    if (depth == 0) {  // the first surface: this sample's lobe shares (:682-690)
+                                        V3 sh[3];
+                                        lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, ls.wi, lambda), sh);
+                                        fr[0] = make_float4(sh[0].x, sh[0].y, sh[0].z, __uint_as_float(nee_route(true, group)));
+                                        fr[1] = make_float4(sh[1].x, sh[1].y, sh[1].z, 0.0f);
+                                        fr[2] = make_float4(sh[2].x, sh[2].y, sh[2].z, 0.0f);
+                                    } else {
+                                        fr[0] = make_float4(0.0f, 0.0f, 0.0f, __uint_as_float(nee_route(false, group)));
+                                    }
+                                }
                                 ok = true;
                             } while (false);
                             const uint32_t ss_slot = wave_append(shadow_count, ok);
@@ -991,6 +1113,19 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             extend = true;
                         }
                     } else {
+                        if (kPasses && depth == 0) {  // the first surface's lobe shares of the sampled direction (:789-812)
+                            V3 sh[3];
+                            if (bs.delta) {
+                                const bool refracted = dot(wo, si.ng) * dot(bs.wi, si.ng) < 0.0f;
+                                sh[0] = splat(0.0f);
+                                sh[1] = splat(refracted ? 0.0f : 1.0f);
+                                sh[2] = splat(refracted ? 1.0f : 0.0f);
+                            } else {
+                                lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, bs.wi, lambda), sh);
+                            }
+                            for (int k = 0; k < 3; ++k)
+                                pb.lobe[3 * static_cast<size_t>(pid) + k] = make_float4(sh[k].x, sh[k].y, sh[k].z, 0.0f);
+                        }
                         const float cos_t = fabsf(dot(bs.wi, si.normal));
                         const V3 thr_before = thr;
                         thr *= bs.f * (cos_t / bs.pdf);
@@ -1080,6 +1215,13 @@ template <bool kTransport, bool kGuide>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
 k_shade(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
     shade_body<kTransport, kGuide, false>(sc, lp, pb, cur);
+}
+
+// Renders with pass films: the transport variant (it shades every scene) with the pass crediting.
+template <bool kGuide>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2)))
+k_shade_passes(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
+    shade_body<true, kGuide, true>(sc, lp, pb, cur);
 }
 
 __global__ void __launch_bounds__(kBlock) PE_SHADE_PLAIN_ATTR k_shade_plain(SceneView sc, LaunchParams lp, PathBuffers pb, int cur) {
@@ -1234,6 +1376,96 @@ __global__ void __launch_bounds__(kBlock) k_film(LaunchParams lp, PathBuffers pb
         }
         // renderer.cpp:500-502: the pixel stops for the rest of the batch once its own error is low enough.
         if (lp.adapt_check && film_relative_error(sr, sg, sb, c, l2) < lp.noise_threshold) lp.active[p] = 0u;
+    }
+}
+
+// idColor (renderer.cpp:252-273): an id's stable colour, splitmix64 to a hue stepped by the golden ratio, HSV
+// with value 1 and saturation 0.75; id 0 is black.
+__device__ __forceinline__ V3 id_color(uint64_t id) {
+#pragma clang fp contract(off)
+    if (id == 0) return splat(0.0f);
+    uint64_t z = id + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    const float h = fmodf(static_cast<float>(z >> 40) * 0x1p-24f + 0.618034f, 1.0f) * 6.0f;
+    const int i = static_cast<int>(h) % 6;
+    const float f = h - floorf(h);
+    const float p0 = 0.25f, q = 1.0f - 0.75f * f, t = 1.0f - 0.75f * (1.0f - f);
+    switch (i) {
+        case 0: return mk(1.0f, t, p0);
+        case 1: return mk(q, 1.0f, p0);
+        case 2: return mk(p0, 1.0f, t);
+        case 3: return mk(p0, q, 1.0f);
+        case 4: return mk(t, p0, 1.0f);
+        default: return mk(1.0f, p0, q);
+    }
+}
+
+// Pass films of a render (pe_pass_targets): 8 light passes, 16 groups, depth, position, object and material id.
+constexpr int kPassFilms = 28;
+struct PassFilms {
+    FilmView f[kPassFilms];
+};
+
+// addPasses (renderer.cpp:207-219) for every pass film and pixel: one thread per (film, pixel), adding the
+// pixel's samples of this window in sample order. Light passes, groups, depth and the id colours with
+// Film::addSample, the position with addSampleSigned.
+__global__ void __launch_bounds__(kBlock) k_film_passes(LaunchParams lp, PathBuffers pb, PassFilms films,
+                                                        const uint64_t* __restrict__ node_uids) {
+#pragma clang fp contract(off)
+    const uint32_t pixels = lp.num_pixels;
+    const uint64_t total = static_cast<uint64_t>(pixels) * kPassFilms;
+    const uint32_t data0 = lp.pass_count - 2, data1 = lp.pass_count - 1;
+    for (uint64_t idx = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; idx < total;
+         idx += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+        const int fi = static_cast<int>(idx / pixels);
+        const uint32_t q = static_cast<uint32_t>(idx - static_cast<uint64_t>(fi) * pixels);
+        const FilmView& f = films.f[fi];
+        if (!f.sum) continue;
+        if (fi >= static_cast<int>(kLightPasses) && fi < static_cast<int>(kLightPasses) + 16 &&
+            fi - static_cast<int>(kLightPasses) >= lp.num_groups)
+            continue;
+        const bool is_signed = fi == 25;
+        const uint32_t p = lp.pixel_list ? lp.pixel_list[q] : q;
+        float sr = f.sum[3 * static_cast<size_t>(p)], sg = f.sum[3 * static_cast<size_t>(p) + 1],
+              sb = f.sum[3 * static_cast<size_t>(p) + 2];
+        float l2 = is_signed ? 0.0f : f.lum2[p];
+        uint32_t rej = 0;
+        for (int k = 0; k < lp.spl; ++k) {
+            const uint32_t g = q + static_cast<uint32_t>(k) * pixels;
+            V3 v;
+            if (fi < 24) {
+                const float4 a = pb.pass_res[static_cast<size_t>(fi) * lp.num_paths + g];
+                v = mk(a.x, a.y, a.z);
+            } else if (fi == 24 || fi == 25) {
+                const float4 a = pb.pass_res[static_cast<size_t>(data0) * lp.num_paths + g];
+                v = fi == 24 ? splat(a.w) : mk(a.x, a.y, a.z);
+            } else {
+                const float4 a = pb.pass_res[static_cast<size_t>(data1) * lp.num_paths + g];
+                const int shape = __float_as_int(a.x), mat = __float_as_int(a.y);
+                if (fi == 26) v = id_color(shape >= 0 ? node_uids[shape] : 0ull);
+                else v = id_color(mat >= 0 ? static_cast<uint64_t>(mat) + 1u : 0ull);
+            }
+            const bool ok = finite3(v) && (is_signed || (v.x >= 0.0f && v.y >= 0.0f && v.z >= 0.0f));
+            if (!ok) {
+                ++rej;
+                continue;
+            }
+            sr += v.x;
+            sg += v.y;
+            sb += v.z;
+            if (!is_signed) {
+                const float lum = film_luminance(v);
+                l2 += lum * lum;
+            }
+        }
+        f.sum[3 * static_cast<size_t>(p)] = sr;
+        f.sum[3 * static_cast<size_t>(p) + 1] = sg;
+        f.sum[3 * static_cast<size_t>(p) + 2] = sb;
+        f.count[p] += static_cast<uint32_t>(lp.spl);
+        if (!is_signed) f.lum2[p] = l2;
+        if (rej) atomicAdd(f.rejected, static_cast<unsigned long long>(rej));
     }
 }
 
@@ -1419,6 +1651,10 @@ struct Wavefront {
     uint64_t aov_cap = 0;    // samples of the AOV result buffer
     DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, fl0, fl1, ctr, stats, filter;
     DevBuf res, aov_res;
+    // Render passes: per-sample records (pass_cap float4), per-id lobe shares and per-record NEE routing.
+    DevBuf pass_res, lobe, pfrac;
+    uint64_t pass_cap = 0;
+    uint32_t lobe_cap = 0;  // path ids the lobe / routing columns hold (at the current shadow sample count)
     // Adaptive sampling: per-pixel active flags and clean flags, the active pixel list, compaction counts.
     DevBuf adapt_active, adapt_clean, adapt_list, adapt_counts;
     uint32_t adapt_pixels = 0;
@@ -1483,6 +1719,7 @@ struct Wavefront {
         capacity = static_cast<uint32_t>(Q);
         seg_cap = cap;
         shadow_samples = S;
+        lobe_cap = 0;  // the pass columns follow the pool
         return PE_OK;
     }
 
@@ -1505,6 +1742,27 @@ struct Wavefront {
         return PE_OK;
     }
 
+    // Pass buffers for a window of n samples of `records` float4 each, over the current pool.
+    int ensure_passes(uint64_t n, uint32_t records) {
+        const uint64_t need = n * records;
+        if (need > pass_cap) {
+            if (!pass_res.alloc(need * 16)) {
+                pass_cap = 0;
+                return PE_ERR_OOM;
+            }
+            pass_cap = need;
+        }
+        if (lobe_cap < capacity) {
+            if (!lobe.alloc(static_cast<size_t>(capacity) * 48) ||
+                !pfrac.alloc(static_cast<size_t>(capacity) * static_cast<size_t>(shadow_samples) * 48)) {
+                lobe_cap = 0;
+                return PE_ERR_OOM;
+            }
+            lobe_cap = capacity;
+        }
+        return PE_OK;
+    }
+
     PathBuffers view() {
         PathBuffers pb{};
         pb.ro[0] = ro0.as<float4>();
@@ -1520,6 +1778,9 @@ struct Wavefront {
         pb.pend = pend.as<float4>();
         pb.res = res.as<float4>();
         pb.aov_res = nullptr;
+        pb.pass_res = nullptr;
+        pb.lobe = lobe.as<float4>();
+        pb.pfrac = pfrac.as<float4>();
         pb.hits = hits.as<float4>();
         pb.sq_o = sq_o.as<float4>();
         pb.sq_d = sq_d.as<float4>();
@@ -1566,7 +1827,8 @@ uint64_t wavefront_bytes_per_path(int shadow_quality) {
 }
 uint64_t wavefront_device_bytes(const Wavefront* wf) {
     return static_cast<uint64_t>(wf->capacity) * Wavefront::bytes_per_path(std::max(1, wf->shadow_samples)) +
-           wf->result_cap * Wavefront::kResultBytes + wf->aov_cap * Wavefront::kAovResultBytes;
+           wf->result_cap * Wavefront::kResultBytes + wf->aov_cap * Wavefront::kAovResultBytes +
+           wf->pass_cap * 16 + static_cast<uint64_t>(wf->lobe_cap) * 48 * (1 + std::max(1, wf->shadow_samples));
 }
 
 // Renders samples [first_sample, first_sample + sample_count) of every pixel into `film` on `lane`: the
@@ -1628,6 +1890,29 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             want_aov = true;
         }
     }
+    // Pass films (RenderPassFilms): every one optional, each of the beauty film's device and size.
+    PassFilms pass_films{};
+    bool want_passes = false;
+    if (aovs && aovs->passes) {
+        const pe_pass_targets& pt = *aovs->passes;
+        pe_film* fs[kPassFilms];
+        for (int k = 0; k < 8; ++k) fs[k] = pt.light[k];
+        for (int k = 0; k < 16; ++k) fs[8 + k] = pt.light_groups[k];
+        fs[24] = pt.depth;
+        fs[25] = pt.position;
+        fs[26] = pt.object_id;
+        fs[27] = pt.material_id;
+        for (int k = 0; k < kPassFilms; ++k) {
+            if (!fs[k]) continue;
+            if (fs[k]->dev != dev || fs[k]->width != film->width || fs[k]->height != film->height) {
+                set_error("pe_render: a pass film of another device or size");
+                return PE_ERR_INVALID;
+            }
+            pass_films.f[k] = FilmView{fs[k]->sum.as<float>(), fs[k]->count.as<uint32_t>(), fs[k]->lum2.as<float>(),
+                                       fs[k]->rejected.as<unsigned long long>()};
+            want_passes = true;
+        }
+    }
     if (first_sample < 0 || sample_count < 0) {
         set_error("pe_render: negative sample range");
         return PE_ERR_INVALID;
@@ -1652,11 +1937,13 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         // memory. The path pool (ids in flight, regenerated as paths end) is sized independently: as asked, or
         // (0) 2^26 ids, and never above the window.
         const bool adaptive = settings->adaptive != 0;
+        const uint32_t pass_count = kLightPasses + static_cast<uint32_t>(scene->view.num_groups) + 2u;
         int spl_req = settings->samples_per_launch;
         if (spl_req <= 0) {
             size_t free_b = 0, total_b = 0;
             if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = size_t{1} << 32;
-            const uint64_t per_sample = Wavefront::kResultBytes + (want_aov ? Wavefront::kAovResultBytes : 0);
+            const uint64_t per_sample = Wavefront::kResultBytes + (want_aov ? Wavefront::kAovResultBytes : 0) +
+                                        (want_passes ? 16ull * pass_count : 0);
             const uint64_t target = std::min<uint64_t>(kAutoWindowSamples, free_b / 8 / per_sample);
             spl_req = static_cast<int>(std::clamp<uint64_t>(target / std::max<uint32_t>(1, pixels), 1, 1024));
         }
@@ -1666,7 +1953,8 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             set_error("pe_render: film too large for 31-bit sample indices");
             return PE_ERR_INVALID;
         }
-        const uint64_t max_window = std::max<uint64_t>(pixels, std::min<uint64_t>(kMaxWindowSamples, uint64_t{pixels} * spl_req));
+        const uint64_t max_window = std::max<uint64_t>(
+            pixels, std::min<uint64_t>(want_passes ? kPassWindowSamples : kMaxWindowSamples, uint64_t{pixels} * spl_req));
         // Adaptive sampling renders in groups that end where the reference checks a pixel's error (every 8th
         // sample), so a window holds at most 8 samples of each pixel still rendering.
         const float threshold = std::max(1e-4f, settings->noise_threshold);
@@ -1679,6 +1967,7 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         const uint32_t pool = static_cast<uint32_t>(std::min({pool_req, window, pool_limit}));
         int rc = wf->ensure(pool, S);
         if (rc == PE_OK) rc = wf->ensure_results(window, want_aov);
+        if (rc == PE_OK && want_passes) rc = wf->ensure_passes(window, pass_count);
         if (rc != PE_OK) return rc;
         if (guide) {
             rc = guide_reserve_paths(guide, wf->capacity, wf->slots());
@@ -1719,6 +2008,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         PathBuffers pb = wf->view();
         pb.aov_res = want_aov ? wf->aov_res.as<float4>() : nullptr;
         lp.want_aov = want_aov ? 1 : 0;
+        pb.pass_res = want_passes ? wf->pass_res.as<float4>() : nullptr;
+        lp.want_passes = want_passes ? 1 : 0;
+        lp.num_groups = scene->view.num_groups;
+        lp.pass_count = pass_count;
         pb.mix = guide ? guide->mix.as<float4>() : nullptr;
         pb.grec = guide ? guide->path_records.as<float4>() : nullptr;
         lp.guided_bounces = guide ? guide->cfg.guided_bounces : 0;
@@ -1756,7 +2049,10 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                 wf->prof.end(s);
             }
             wf->prof.begin(kKShade, s);
-            if (lp.transport) {
+            if (want_passes) {
+                if (guide) hipLaunchKernelGGL(k_shade_passes<true>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+                else hipLaunchKernelGGL(k_shade_passes<false>, dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
+            } else if (lp.transport) {
                 if (guide) hipLaunchKernelGGL((k_shade<true, true>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
                 else hipLaunchKernelGGL((k_shade<true, false>), dim3(gblocks), dim3(kBlock), 0, s, view, lp, pb, cur);
             } else {
@@ -1890,6 +2186,12 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             hipLaunchKernelGGL(k_film, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, film->sum.as<float>(),
                                film->count.as<uint32_t>(), film->lum2.as<float>(),
                                film->rejected.as<unsigned long long>(), aov_view[0], aov_view[1], aov_view[2]);
+            if (want_passes) {
+                const int pass_blocks = static_cast<int>(
+                    std::min<uint64_t>((uint64_t{lp.num_pixels} * kPassFilms + kBlock - 1) / kBlock, 65535u));
+                hipLaunchKernelGGL(k_film_passes, dim3(pass_blocks), dim3(kBlock), 0, s, lp, pb, pass_films,
+                                   view.node_uids);
+            }
             wf->prof.end(s);
             PE_HIP_CHECK(hipGetLastError());
             if (guide) {
@@ -1985,6 +2287,7 @@ extern "C" int pe_render_targets(pe_device* dev, pe_scene* scene, const pe_camer
         aovs.alpha = targets->alpha;
         aovs.cancel_poll = targets->cancel_poll;
         aovs.cancel_user = targets->cancel_user;
+        aovs.passes = targets->passes;
         return render_samples(dev, lane, scene, camera, settings, targets->beauty, first_sample, sample_count, cancel,
                               result, targets->guide, &aovs);
     } catch (const std::exception& e) {

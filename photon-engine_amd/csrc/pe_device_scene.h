@@ -99,6 +99,7 @@ struct alignas(16) GpuLight {
     V3 normal;
     V3 center;            // importance reference point
     float len_u, len_v;   // |u|, |v| (AREA)
+    int32_t group;        // Scene::lightGroupIndex (scene.cpp:258-261), -1 without light groups
 };
 
 // Light-BVH node (LightSampler::Node + LightBounds, light_sampler.h:64-69, light_bounds.h:15-25), 64 B. Nodes
@@ -160,6 +161,9 @@ struct SceneView {
     int32_t has_transmitters;  // a thin-walled surface passes shadow rays on with its transmittance
     int32_t transport;         // a Dielectric or a transmitting / subsurface material is present
     int32_t sort_classes;      // distinct GpuMaterial::sort_class values of the scene's materials
+    int32_t num_groups;        // light groups (Scene::lightGroupNames().size()), 0 = none
+    int32_t env_group;         // Scene::envLightGroupIndex
+    const uint64_t* node_uids; // per shape: Scene::nodeUidOf (object-ID pass)
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;
     const LightPath* light_paths;  // light_count entries

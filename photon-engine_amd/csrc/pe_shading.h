@@ -374,11 +374,18 @@ __device__ __forceinline__ V3 film_specular_fresnel(const GpuMaterial& m, float 
     return lerp3(fd, thin_film_rgb(cos_d, m.film_ior, metal, m.film_thickness), m.metallic);
 }
 
+// Material::evalSplit's value (material.h:51-57): a BSDF value split into the lobe groups of the light passes.
+struct LobeSplit {
+    V3 diffuse, glossy, transmission;
+};
+
 // evalLobes (disney.cpp:223-311) in the local frame: reflection above the surface; diffuse and specular
 // transmission below it. kTransport = false is the variant for scenes whose materials all have zero
-// transmission weights, where the lobes below the surface are exactly zero.
+// transmission weights, where the lobes below the surface are exactly zero. With `split` (the light passes),
+// the value's diffuse, glossy (specular, sheen, coat) and transmission parts as well (disney.cpp:250, :290-309).
 template <bool kTransport>
-__device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, bool front, V3 wo, V3 wi) {
+__device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, bool front, V3 wo, V3 wi,
+                                                LobeSplit* split = nullptr) {
     if (wo.z <= 0.0f) return splat(0.0f);
     const float cos_o = wo.z;
     if (wi.z <= 0.0f) {
@@ -393,6 +400,7 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
             ft += m.base_color * (m.trans_w * kInvPi * coat_o * factor * sh) * m.coat_tint_factor;
         }
         if (m.st_w > 0.0f) ft += eval_spec_trans(m, front, wo, wi) * coat_o * m.coat_tint_factor;
+        if (split) split->transmission = ft;
         return ft;
     }
     const float cos_i = wi.z;
@@ -420,20 +428,32 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
     const V3 ms = splat(1.0f) + f0 * (1.0f / E - 1.0f);
     const V3 f_spec = (F * D * G) / (4.0f * cos_o * cos_i) * ms;
     V3 base = f_diffuse + f_spec;
+    V3 base_diffuse = f_diffuse;
     if (m.flags & kMatFlagSheen) {
         const float s = sheen_scale(m, T, cos_o, cos_i);
         const float norm = fmaxf(1.0f, fmaxf(charlie_sheen_albedo(T, cos_o, m.sheen_roughness),
                                              charlie_sheen_albedo(T, cos_i, m.sheen_roughness)));
         const V3 sheen = m.sheen_color * (charlie_d(h.z, m.sheen_roughness) * charlie_v(cos_i, cos_o) / norm);
         base = base * s + sheen;
+        base_diffuse = base_diffuse * s;
     }
-    if (m.clearcoat <= 0.0f) return base;
+    if (m.clearcoat <= 0.0f) {
+        if (split) {
+            split->diffuse = base_diffuse;
+            split->glossy = base - base_diffuse;
+        }
+        return base;
+    }
     const float dc = ggx_d(h, m.alpha_c, m.alpha_c);
     const float gc = ggx_g(wo, wi, m.alpha_c, m.alpha_c);
     const float fc = m.f0c + (1.0f - m.f0c) * pow5(1.0f - clampf(cos_d, 0.0f, 1.0f));
     const float ms_c = 1.0f + m.f0c * (1.0f / ggx_directional_albedo(T, cos_o, m.alpha_c) - 1.0f);
     const V3 f_coat = splat(m.clearcoat * fc * dc * gc / (4.0f * cos_o * cos_i) * ms_c);
     const V3 coat_att = m.coat_tint_factor * (coat_transmit(m, T, cos_o) * coat_transmit(m, T, cos_i));
+    if (split) {
+        split->diffuse = base_diffuse * coat_att;
+        split->glossy = (base - base_diffuse) * coat_att + f_coat;
+    }
     return base * coat_att + f_coat;
 }
 
@@ -578,6 +598,30 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
     const V3 wol = to_local(f, wo), wil = to_local(f, wi);
     pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil);
     return disney_eval_local<kTransport>(m, T, si.front_face, wol, wil);
+}
+
+// Material::evalSplit (material.h:154-161; DisneyMaterial::evalSplit, disney.cpp:654-658; Dielectric::evalSplit,
+// dielectric.cpp:279-286): Disney's lobes as evalLobes splits them; for the other materials the BSDF value on the
+// geometric normal's side of wo is diffuse (glossy for glass), across it transmission.
+template <bool kTransport>
+__device__ __forceinline__ LobeSplit bsdf_eval_split(const GpuMaterial& m, const EnergyTables& T, const Surface& si,
+                                                     V3 wo, V3 wi, float lambda) {
+    LobeSplit sp{splat(0.0f), splat(0.0f), splat(0.0f)};
+    if (m.type == kMatDisney) {
+        const Frame f = shading_frame(si.normal, si.tangent);
+        (void)disney_eval_local<kTransport>(m, T, si.front_face, to_local(f, wo), to_local(f, wi), &sp);
+        return sp;
+    }
+    float pdf = 0.0f;
+    const V3 v = bsdf_eval_pdf<kTransport>(m, T, si, wo, wi, lambda, pdf);
+    const V3 n = length_sq(si.ng) > 0.0f ? si.ng : si.normal;
+    if (dot(wo, n) * dot(wi, n) >= 0.0f) {
+        if (m.type == kMatDielectric) sp.glossy = v;
+        else sp.diffuse = v;
+    } else {
+        sp.transmission = v;
+    }
+    return sp;
 }
 
 struct BsdfSample {

@@ -494,6 +494,27 @@ int prepare_lights(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             return PE_ERR_INVALID;
         }
         lights[static_cast<size_t>(i)] = prepare_light(L);
+        lights[static_cast<size_t>(i)].group = -1;
+    }
+    // Light groups as the caller resolved them (Scene::buildAccelerator, scene.cpp:144-173).
+    const bool grouped = desc.light_groups && desc.num_light_groups > 0;
+    if (grouped) {
+        if (desc.num_light_groups > PE_MAX_LIGHT_GROUPS) {
+            set_error("scene: more than 16 light groups");
+            return PE_ERR_INVALID;
+        }
+        for (int i = 0; i < desc.num_lights; ++i) {
+            const int g = desc.light_groups[i];
+            if (g < -1 || g >= desc.num_light_groups) {
+                set_error("light " + std::to_string(i) + ": light group out of range");
+                return PE_ERR_INVALID;
+            }
+            lights[static_cast<size_t>(i)].group = g;
+        }
+        if (desc.env_light_group < -1 || desc.env_light_group >= desc.num_light_groups) {
+            set_error("scene: environment light group out of range");
+            return PE_ERR_INVALID;
+        }
     }
     // LightSampler::build (light_sampler.cpp:42-77): up to 33 candidates are weighed exhaustively, more go into
     // the light BVH with the unbounded ones on a top level.
@@ -535,6 +556,8 @@ int prepare_lights(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     v.light_paths = sc.light_paths.as<LightPath>();
     v.num_light_nodes = lb.mode == kPickBvh ? static_cast<int32_t>(lb.nodes.size()) : 0;
     v.num_top = lb.mode == kPickBvh ? static_cast<int32_t>(lb.top.size()) : 0;
+    v.num_groups = grouped ? desc.num_light_groups : 0;
+    v.env_group = grouped ? desc.env_light_group : -1;
     for (int k = 0; k < kMaxTopCandidates; ++k)
         v.top[k] = (lb.mode == kPickBvh && k < static_cast<int>(lb.top.size())) ? lb.top[static_cast<size_t>(k)] : 0;
     return PE_OK;
@@ -546,7 +569,18 @@ int prepare_shading(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     if (rc != PE_OK) return rc;
     rc = prepare_lights(dev, desc, sc);
     if (rc != PE_OK) return rc;
+    // Node uids of the shapes (the object-ID pass): 0 for untagged shapes.
+    std::vector<uint64_t> uids(static_cast<size_t>(std::max(1, desc.num_meshes)), 0);
+    if (desc.node_uids)
+        for (int i = 0; i < desc.num_meshes; ++i) uids[static_cast<size_t>(i)] = desc.node_uids[i];
+    if (!sc.node_uids.upload(uids.data(), uids.size() * sizeof(uint64_t), dev->stream)) return PE_ERR_OOM;
+    const hipError_t eu = hipStreamSynchronize(dev->stream);
+    if (eu != hipSuccess) {
+        set_error(std::string("node uid upload: ") + hipGetErrorString(eu));
+        return PE_ERR_HIP;
+    }
     SceneView& v = sc.view;
+    v.node_uids = sc.node_uids.as<uint64_t>();
     v.nodes = sc.nodes.as<BvhNode4>();
     v.num_nodes = static_cast<uint32_t>(sc.num_nodes);
     v.tris = sc.tris.as<TriAccel>();

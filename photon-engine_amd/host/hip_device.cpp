@@ -59,6 +59,10 @@ struct PackedScene {
     std::vector<std::vector<uint32_t>> indices;
     pe_env_desc env{};
     bool hasEnv = false;
+    std::vector<int32_t> lightGroups;  // Scene::lightGroupIndex per light (empty: the scene has no groups)
+    int32_t envGroup = -1;
+    int32_t numGroups = 0;
+    std::vector<uint64_t> nodeUids;    // per shape: Scene::nodeUidOf
 
     const float* keep(std::vector<float> v) {
         floats.push_back(std::move(v));
@@ -73,6 +77,10 @@ struct PackedScene {
         d.lights = lights.data();
         d.num_lights = static_cast<int32_t>(lights.size());
         d.environment = hasEnv ? &env : nullptr;
+        d.light_groups = lightGroups.empty() ? nullptr : lightGroups.data();
+        d.env_light_group = envGroup;
+        d.num_light_groups = lightGroups.empty() ? 0 : numGroups;
+        d.node_uids = nodeUids.data();
         return d;
     }
 };
@@ -177,6 +185,12 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
         }
         p.lights.push_back(d);
     }
+    // Light groups as Scene::buildAccelerator resolved them (scene.cpp:144-173).
+    if (!scene.lightGroupNames().empty()) {
+        p.numGroups = static_cast<int32_t>(std::min<size_t>(scene.lightGroupNames().size(), PE_MAX_LIGHT_GROUPS));
+        for (size_t i = 0; i < p.lights.size(); ++i) p.lightGroups.push_back(scene.lightGroupIndex(static_cast<int>(i)));
+        p.envGroup = scene.envLightGroupIndex();
+    }
 
     for (const auto& shape : scene.shapes()) {
         SurfaceInteraction probe;
@@ -225,6 +239,7 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
             return false;
         }
         p.meshes.push_back(md);
+        p.nodeUids.push_back(scene.nodeUidOf(probe));
     }
 
     if (const EnvironmentLight* env = scene.environment()) {
@@ -413,14 +428,33 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     const bool adaptive = settings.adaptiveSampling;
 
     const int w = film.width(), h = film.height();
-    // The caller's films: beauty and, when it asks for them, the AOVs (renderer.cpp:222-227).
-    Film* targets_of[4] = {&film, nullptr, nullptr, nullptr};
+    // The caller's films: beauty and, when it asks for them, the AOVs (renderer.cpp:222-227) and the render
+    // passes (light passes, the scene's light groups, depth, position, object and material id; renderer.cpp:
+    // 207-219, with the groups named as the scene's, :400-405). Cryptomatte films are left as they are.
+    Film* targets_of[kDeviceFilms] = {&film};
     if (targets.aovs) {
         AovFilms& a = *targets.aovs;
         if (a.albedo.width() != w || a.albedo.height() != h) a.resize(w, h);
         targets_of[1] = &a.albedo;
         targets_of[2] = &a.normal;
         targets_of[3] = &a.alpha;
+    }
+    if (targets.passes) {
+        RenderPassFilms& pf = *targets.passes;
+        if (pf.depth.width() != w || pf.depth.height() != h) pf.resize(w, h);
+        const std::vector<std::string>& names = m_sceneRef->lightGroupNames();
+        bool same = pf.lightGroups.size() == names.size();
+        for (size_t i = 0; same && i < names.size(); ++i)
+            same = pf.lightGroups[i].first == names[i] && pf.lightGroups[i].second.width() == w &&
+                   pf.lightGroups[i].second.height() == h;
+        if (!same) pf.initLightGroups(w, h, names);
+        for (int k = 0; k < PE_LIGHT_PASSES; ++k) targets_of[kPassFilm0 + k] = &pf.light[k];
+        for (size_t g = 0; g < pf.lightGroups.size() && g < PE_MAX_LIGHT_GROUPS; ++g)
+            targets_of[kPassFilm0 + PE_LIGHT_PASSES + g] = &pf.lightGroups[g].second;
+        targets_of[kPassFilm0 + 24] = &pf.depth;
+        targets_of[kPassFilm0 + 25] = &pf.position;
+        targets_of[kPassFilm0 + 26] = &pf.objectId;
+        targets_of[kPassFilm0 + 27] = &pf.materialId;
     }
     if (m_filmW != w || m_filmH != h) {
         for (DeviceFilm& f : m_films) {
@@ -433,7 +467,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     }
     const size_t pixels = static_cast<size_t>(w) * static_cast<size_t>(h);
     const bool restart = !adaptive || firstSample == 0 || !m_accumulating;
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kDeviceFilms; ++k) {
         if (!targets_of[k]) continue;
         DeviceFilm& f = m_films[k];
         if (!f.film) {
@@ -463,6 +497,18 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         tg.normal = m_films[2].film;
         tg.alpha = m_films[3].film;
     }
+    pe_pass_targets pt{};
+    if (targets.passes) {
+        for (int k = 0; k < PE_LIGHT_PASSES; ++k) pt.light[k] = m_films[kPassFilm0 + k].film;
+        for (int g = 0; g < PE_MAX_LIGHT_GROUPS; ++g)
+            pt.light_groups[g] = targets_of[kPassFilm0 + PE_LIGHT_PASSES + g] ? m_films[kPassFilm0 + PE_LIGHT_PASSES + g].film
+                                                                               : nullptr;
+        pt.depth = m_films[kPassFilm0 + 24].film;
+        pt.position = m_films[kPassFilm0 + 25].film;
+        pt.object_id = m_films[kPassFilm0 + 26].film;
+        pt.material_id = m_films[kPassFilm0 + 27].film;
+        tg.passes = &pt;
+    }
     // The whole batch is one call: the library polls the caller's cancel flag between its iterations (a few
     // milliseconds apart) and keeps the windows of samples it completed.
     tg.guide = settings.pathGuiding ? m_guide : nullptr;
@@ -482,10 +528,10 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     const uint64_t traced = br.samples_traced;
     const bool cancelled = rrc == PE_ERR_CANCELLED;
     // Merge what this call added (the whole film unless it accumulates across calls) with Film::addBatch; the
-    // normal AOV keeps its sign (addSampleSigned, film.cpp:73-86).
+    // normal AOV and the position pass keep their sign (addSampleSigned, film.cpp:73-86).
     std::vector<float> sum(3 * pixels), lum2(pixels);
     std::vector<uint32_t> cnt(pixels);
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < kDeviceFilms; ++k) {
         if (!targets_of[k]) continue;
         DeviceFilm& f = m_films[k];
         uint64_t rejected = 0;
@@ -500,7 +546,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
                 dst.addBatch(x, y,
                              Color3f(sum[3 * p] - f.prevSum[3 * p], sum[3 * p + 1] - f.prevSum[3 * p + 1],
                                      sum[3 * p + 2] - f.prevSum[3 * p + 2]),
-                             cnt[p] - f.prevCount[p], k == 2);
+                             cnt[p] - f.prevCount[p], k == 2 || k == kPassFilm0 + 25);
             }
         if (m_accumulating) {
             f.prevSum = sum;

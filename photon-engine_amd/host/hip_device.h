@@ -23,9 +23,9 @@ public:
     ~HipDevice() override;
 
     const DeviceInfo& info() const override { return m_info; }
-    // AOVs (albedo / normal / alpha) and adaptive sampling are supported; light passes are not, and
-    // RenderSession turns them off with a warning (render_session.cpp:23-35).
-    DeviceFeatures features() const override { return DeviceFeatures{true, false, true}; }
+    // AOVs (albedo / normal / alpha), render passes (light passes, light groups, data passes; no cryptomatte)
+    // and adaptive sampling.
+    DeviceFeatures features() const override { return DeviceFeatures{true, true, true}; }
     void setScene(std::shared_ptr<const Scene> scene) override;
     void updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind kind) override;
     bool hasScene() const override { return m_scene != nullptr; }
@@ -45,7 +45,8 @@ private:
     pe_scene* m_scene = nullptr;
     pe_guide* m_guide = nullptr;          // RenderSettings::pathGuiding: the learned guide of m_guideScene
     const pe_scene* m_guideScene = nullptr;
-    // Device films: beauty, then the albedo, normal and alpha AOVs. With adaptive sampling they keep every sample
+    // Device films: beauty, then the albedo, normal and alpha AOVs, then the pass films (8 light passes, 16 light
+    // groups, depth, position, object id, material id). With adaptive sampling they keep every sample
     // this device rendered since the last render that started at sample 0 (the beauty's per-sample statistics
     // drive the skip mask and the stops); prevSum / prevCount is the part already merged into the caller's films.
     struct DeviceFilm {
@@ -53,7 +54,8 @@ private:
         std::vector<float> prevSum;
         std::vector<uint32_t> prevCount;
     };
-    DeviceFilm m_films[4];
+    static constexpr int kPassFilm0 = 4, kDeviceFilms = kPassFilm0 + 28;
+    DeviceFilm m_films[kDeviceFilms];
     int m_filmW = 0, m_filmH = 0;
     bool m_accumulating = false;
     std::shared_ptr<const Scene> m_sceneRef;

@@ -184,14 +184,48 @@ class GpuFilm:
         return s / np.maximum(c, 1)[..., None]
 
 
+class PassFilms:
+    """RenderPassFilms without cryptomatte: a film per light pass (LightPass order), per light group of the
+    scene (up to 16) and per data pass, each of the beauty film's size."""
+
+    def __init__(self, dev: Device, width: int, height: int, groups: int = 0, light=True, data=True):
+        mk = lambda: GpuFilm(dev, width, height)  # noqa: E731
+        self.light = [mk() for _ in range(capi.PE_LIGHT_PASSES)] if light else []
+        self.groups = [mk() for _ in range(min(groups, capi.PE_MAX_LIGHT_GROUPS))]
+        self.depth, self.position, self.object_id, self.material_id = (mk() for _ in range(4)) if data else (None,) * 4
+
+    def targets(self) -> capi.PassTargets:
+        pt = capi.PassTargets()
+        for k, f in enumerate(self.light):
+            pt.light[k] = f.h
+        for k, f in enumerate(self.groups):
+            pt.light_groups[k] = f.h
+        pt.depth, pt.position, pt.object_id, pt.material_id = (
+            f.h if f is not None else None for f in (self.depth, self.position, self.object_id, self.material_id))
+        return pt
+
+    def means(self) -> dict:
+        """Film::resolve of every film: {"light": (8, H, W, 3), "groups": (G, H, W, 3), "depth", ...}."""
+        out = {"light": np.stack([f.mean() for f in self.light]) if self.light else None,
+               "groups": np.stack([f.mean() for f in self.groups]) if self.groups else None}
+        for name in ("depth", "position", "object_id", "material_id"):
+            f = getattr(self, name)
+            out[name] = f.mean() if f is not None else None
+        return out
+
+
 def render_targets(dev: Device, scene: GpuScene, camera: capi.Camera, st: capi.Settings, film: GpuFilm, first: int,
                    count: int, albedo: GpuFilm | None = None, normal: GpuFilm | None = None,
-                   alpha: GpuFilm | None = None, cancel=None) -> capi.BatchResult:
-    """pe_render_targets: the beauty film plus the AOV films given (RenderTargets with AovFilms)."""
+                   alpha: GpuFilm | None = None, cancel=None, passes: PassFilms | None = None) -> capi.BatchResult:
+    """pe_render_targets: the beauty film plus the AOV and pass films given (RenderTargets with AovFilms and
+    RenderPassFilms)."""
     res = capi.BatchResult()
     t = capi.Targets()
     t.beauty = film.h
     t.albedo, t.normal, t.alpha = (f.h if f is not None else None for f in (albedo, normal, alpha))
+    if passes is not None:
+        pt = passes.targets()
+        t.passes = C.pointer(pt)
     cptr = ptr(cancel, C.c_int32) if cancel is not None else C.cast(None, C.POINTER(C.c_int32))
     rc = load().pe_render_targets(dev.h, scene.h, C.byref(camera), C.byref(st), C.byref(t), first, count, cptr,
                                   C.byref(res))

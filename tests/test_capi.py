@@ -40,7 +40,8 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
                "pe_settings": capi.Settings, "pe_batch_result": capi.BatchResult,
                "pe_kernel_stats": capi.KernelStats, "pe_server_config": capi.ServerConfig,
                "pe_job_desc": capi.JobDesc, "pe_job_status": capi.JobStatus,
-               "pe_guide_config": capi.GuideConfig, "pe_server_stats": capi.ServerStats}
+               "pe_guide_config": capi.GuideConfig, "pe_server_stats": capi.ServerStats,
+               "pe_targets": capi.Targets, "pe_pass_targets": capi.PassTargets}
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "photon_hip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
