@@ -39,6 +39,7 @@ extern "C" {
 typedef struct pe_device pe_device;
 typedef struct pe_scene pe_scene;
 typedef struct pe_film pe_film;
+typedef struct pe_crypto_film pe_crypto_film;
 typedef struct pe_guide pe_guide;
 
 /* ---- device registry --------------------------------------------------------------------------------
@@ -303,7 +304,28 @@ typedef struct pe_pass_targets {
     pe_film* position;
     pe_film* object_id;
     pe_film* material_id;
+    /* Cryptomatte (RenderPassFilms::cryptoObject / cryptoMaterial): per sample, the id of the first surface's
+     * object and material, looked up in the caller's tables (buildCryptoLookup, renderer.cpp:65-118: the
+     * cryptoHashToFloat of each registered name, cryptomatte.cpp:99-112; 0 = no id), counted per pixel. */
+    pe_crypto_film* crypto_object;    /* or NULL */
+    pe_crypto_film* crypto_material;  /* or NULL */
+    const float* object_ids;          /* num_object_ids entries, one per mesh of the scene description */
+    const float* material_ids;        /* num_material_ids entries, one per material */
+    int32_t num_object_ids;
+    int32_t num_material_ids;
 } pe_pass_targets;
+
+/* A device CryptoFilm (core/image/cryptomatte.h:38-88): per pixel the samples counted (PixelBucket::total) and up
+ * to PE_CRYPTO_SLOTS distinct ids with their sample counts, in the order they first appeared (the entries
+ * vector). A sample whose id finds no free slot is counted in the total only and in `dropped`; the reference's
+ * bucket grows without bound, so a pixel that sees more than PE_CRYPTO_SLOTS objects differs from it there. */
+#define PE_CRYPTO_SLOTS 8
+int pe_crypto_film_create(pe_device* dev, int width, int height, pe_crypto_film** out);
+int pe_crypto_film_destroy(pe_crypto_film* film);
+int pe_crypto_film_clear(pe_crypto_film* film);
+/* ids, counts: PE_CRYPTO_SLOTS * w * h (slot j of pixel p at p * PE_CRYPTO_SLOTS + j; count 0 = unused);
+ * totals: w * h; any may be NULL. */
+int pe_crypto_film_download(pe_crypto_film* film, float* ids, uint32_t* counts, uint32_t* totals, uint64_t* dropped);
 
 typedef struct pe_targets {
     pe_film* beauty;   /* required */

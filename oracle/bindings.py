@@ -92,21 +92,45 @@ _REF_SIGS = {
     "render_adaptive": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, _I, _I,
                                   C.c_float, C.c_int, _F, _U, _F, _P(C.c_uint64)]),
     "render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                C.c_int, _F, _F]),
-    "scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p]),
+                                C.c_int, _F, _F, _F]),
+    "scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p), _P(C.c_char_p)]),
+    "crypto_hash": (C.c_int, [C.c_size_t, _P(C.c_char_p), _U, _F]),
     "scene_light_groups": (C.c_int, [_VP, _I, _I, C.c_char_p, C.c_size_t]),
     "trace_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_float, C.c_uint64, C.c_size_t, _I, _I, _I, _F, _F]),
 }
 
-PASS_FLOATS = 82  # per sample: 8 light passes, 16 light groups (rgb each), depth, position, object id, material id
+def _scene_names(scene):
+    """The names the reference's Scene carries and resolves itself: light-group names of the lights and of the
+    environment, shape names and material names (C arrays for ref_scene_build_groups)."""
+    groups = (C.c_char_p * max(1, len(scene.lights)))(*[g.encode() for g in scene.light_group_of])
+    env = scene.environment.light_group.encode() if scene.environment is not None else None
+    shapes = (C.c_char_p * max(1, len(scene.shapes)))(*[sh.name.encode() for sh in scene.shapes])
+    mats = (C.c_char_p * max(1, len(scene.materials)))(
+        *[scene.material_names.get(i, "").encode() for i in range(len(scene.materials))])
+    return groups, env, shapes, mats
+
+
+def crypto_hash_reference(names):
+    """The reference's cryptoHashName / cryptoHashToFloat of each name -> (uint32 hashes, float32 ids)."""
+    lib = reference(strict=True)
+    arr = (C.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    h = np.zeros(len(names), np.uint32)
+    f = np.zeros(len(names), np.float32)
+    lib.ref_crypto_hash(len(names), arr, capi.ptr(h, C.c_uint32), _fp(f))
+    return h, f
+
+
+PASS_FLOATS = 84  # per sample: 8 light passes, 16 light groups (rgb each), depth, position, object id, material id,
+#                   then the first surface's shape and material index (-1: miss)
 
 
 def split_passes(out: np.ndarray) -> dict:
-    """The per-sample pass record (n, 82) as named arrays."""
+    """The per-sample pass record (n, 84) as named arrays."""
     return {"light": out[:, :24].reshape(-1, 8, 3), "groups": out[:, 24:72].reshape(-1, 16, 3),
             "depth": out[:, 72], "position": out[:, 73:76], "object_id": out[:, 76:79],
-            "material_id": out[:, 79:82]}
+            "material_id": out[:, 79:82], "shape": out[:, 82].astype(np.int32),
+            "material": out[:, 83].astype(np.int32)}
 
 _cache: dict = {}
 
@@ -247,11 +271,9 @@ class RefScene:
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
         desc, self._keep = scene.pack()
-        # Light-group names as the lights and the environment carry them; the reference resolves them itself.
-        names = (C.c_char_p * max(1, len(scene.lights)))(*[g.encode() for g in scene.light_group_of])
-        env_name = scene.environment.light_group.encode() if scene.environment is not None else None
-        self._keep += [names]
-        self.h = self._lib.ref_scene_build_groups(C.byref(desc), names, env_name)
+        args = _scene_names(scene)
+        self._keep += list(args)
+        self.h = self._lib.ref_scene_build_groups(C.byref(desc), *args)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -336,17 +358,19 @@ class RefScene:
         return mean, cnt, secs.value
 
     def render_passes(self, view, first, count, threads=4):
-        """Renderer::renderSamples with RenderPassFilms -> (beauty (H,W,3), films (28, H,W,3), groups)."""
+        """Renderer::renderSamples with RenderPassFilms -> (beauty (H,W,3), films (28, H,W,3), groups, crypto
+        ranks (2, 3, H, W, 4): CryptoFilm::resolveRanks(3) of objects and of materials)."""
         w, h = view.width, view.height
         beauty = np.zeros((h, w, 3), np.float32)
         films = np.zeros((28, h, w, 3), np.float32)
+        crypto = np.zeros((2, 3, h, w, 4), np.float32)
         f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
         n = self._lib.ref_render_passes(self.h, f3(view.position), f3(view.target), f3(view.up), view.vfov,
                                         view.aspect, w, h, view.max_bounces, first, count, threads, _fp(beauty),
-                                        _fp(films))
+                                        _fp(films), _fp(crypto))
         if n < 0:
             raise RuntimeError("ref_render_passes failed")
-        return beauty, films, n
+        return beauty, films, n, crypto
 
 
 def display(lib_kind: str, sum_rgb, count, settings):
@@ -456,9 +480,10 @@ def sobol_reference(seed, px, py, idx, dim):
 
 _HIPDEV_SIGS = {
     "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
-    "ref_scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p]),
+    "ref_scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p),
+                                     _P(C.c_char_p)]),
     "hipdev_session_render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
-                                               C.c_int, C.c_int, _F, _F, _U, _I, C.c_char_p, C.c_int]),
+                                               C.c_int, C.c_int, _F, _F, _U, _I, _F, C.c_char_p, C.c_int]),
     "ref_scene_free": (None, [_VP]),
     "hipdev_session_render": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int,
                                         C.c_int, C.c_float, C.c_int, C.c_int, C.c_float, C.c_int, _F, _U, _I,
@@ -492,11 +517,9 @@ class HipDeviceSession:
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref/libphoton_hipdevice.so was not built (the reference tree is absent)")
         desc, self._keep = scene.pack()
-        # Light-group names as the lights and the environment carry them; the reference resolves them itself.
-        names = (C.c_char_p * max(1, len(scene.lights)))(*[g.encode() for g in scene.light_group_of])
-        env_name = scene.environment.light_group.encode() if scene.environment is not None else None
-        self._keep += [names]
-        self.h = self._lib.ref_scene_build_groups(C.byref(desc), names, env_name)
+        args = _scene_names(scene)
+        self._keep += list(args)
+        self.h = self._lib.ref_scene_build_groups(C.byref(desc), *args)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -520,18 +543,19 @@ class HipDeviceSession:
 
     def render_passes(self, view, spp, max_batch):
         """RenderSession with SessionConfig::passes -> (rc, beauty (H,W,3), films (28, H,W,3), counts, groups,
-        error)."""
+        error, crypto ranks (2, 3, H, W, 4))."""
         f3 = C.c_float * 3
         beauty = np.zeros((view.height, view.width, 3), np.float32)
         films = np.zeros((28, view.height, view.width, 3), np.float32)
         counts = np.zeros((view.height, view.width), np.uint32)
+        crypto = np.zeros((2, 3, view.height, view.width, 4), np.float32)
         groups = C.c_int32()
         err = C.create_string_buffer(512)
         rc = self._lib.hipdev_session_render_passes(
             self.h, f3(*view.position), f3(*view.target), f3(*view.up), view.vfov, view.aspect, view.width,
             view.height, view.max_bounces, spp, max_batch, _fp(beauty), _fp(films), capi.ptr(counts, C.c_uint32),
-            C.byref(groups), err, 512)
-        return rc, beauty, films, counts, groups.value, err.value.decode(errors="replace")
+            C.byref(groups), _fp(crypto), err, 512)
+        return rc, beauty, films, counts, groups.value, err.value.decode(errors="replace"), crypto
 
     def render_aovs(self, view, spp, max_batch):
         """RenderSession with AOVs -> (rc, [beauty, albedo, normal, alpha] each (H,W,3), counts (H,W), error)."""

@@ -4,6 +4,7 @@
 // Part of the oracle (test infrastructure): compiled with the reference sources and the adapter into
 // oracle/_ref/libphoton_hipdevice.so (oracle/Makefile.ref). The scene is the reference Scene that
 // ref_scene_build (ref_driver.cpp) makes from a pe_scene_desc, so the adapter packs a genuine reference scene.
+#include "core/image/cryptomatte.h"
 #include "engine/render_session.h"
 #include "camera/perspective_camera.h"
 #include "hip_device.h"
@@ -217,11 +218,12 @@ HIPDEV_API int hipdev_session_render_aovs(void* h, const float* camPos, const fl
 }
 
 // RenderSession(Final, targetSpp = spp, maxBatch) with SessionConfig::passes on a HipDevice: the resolved
-// beauty, the pass films (ref_copy_passes layout), the sample counts and the number of light groups it kept.
+// beauty, the pass films (ref_copy_passes layout), the sample counts, the number of light groups it kept and,
+// with `crypto`, the cryptomatte ranks (ref_render_passes layout).
 HIPDEV_API int hipdev_session_render_passes(void* h, const float* camPos, const float* camTarget, const float* camUp,
                                             float fov, float aspect, int width, int height, int maxBounces, int spp,
                                             int maxBatch, float* mean, float* passes, uint32_t* counts, int* groups,
-                                            char* err, int errlen) {
+                                            float* crypto, char* err, int errlen) {
     std::string e;
     std::unique_ptr<HipDevice> dev = HipDevice::create(0, &e);
     if (!dev) {
@@ -278,6 +280,13 @@ HIPDEV_API int hipdev_session_render_passes(void* h, const float* camPos, const 
         }
     ref_copy_passes(*pf, width, height, passes);
     *groups = static_cast<int>(pf->lightGroups.size());
+    if (crypto) {  // CryptoFilm::resolveRanks(3) of objects, then materials (w * h RGBA per layer)
+        const size_t layer = static_cast<size_t>(width) * height * 4;
+        const std::vector<CryptoRankImage> ranks[2] = {pf->cryptoObject.resolveRanks(3), pf->cryptoMaterial.resolveRanks(3)};
+        for (int k = 0; k < 2; ++k)
+            for (size_t j = 0; j < ranks[k].size() && j < 3; ++j)
+                std::copy(ranks[k][j].rgba.begin(), ranks[k][j].rgba.end(), crypto + (3 * k + j) * layer);
+    }
     return 0;
 }
 

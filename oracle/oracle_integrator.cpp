@@ -1586,7 +1586,8 @@ struct Tracer {
     // tracePixel with render passes (renderer.cpp:162-171): the light passes and groups Li credits, and the
     // primary hit's data passes (depth from the camera, position, idColor of its node uid and of its material
     // index + 1; all zero for a miss). out: 24 light-pass floats, 48 group floats, depth, position xyz, object
-    // id rgb, material id rgb (82 floats).
+    // id rgb, material id rgb, then the hit's shape and material index (-1 for a miss; the cryptomatte lookup
+    // keys): 84 floats.
     V3 tracePasses(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* out) const {
         sampler.startPixel(x, y);
         sampler.startSample(s);
@@ -1610,6 +1611,7 @@ struct Tracer {
         for (int g = 0; g < kMaxLightGroups; ++g)
             for (int c = 0; c < 3; ++c) out[24 + 3 * g + c] = lp.groups[g][c];
         std::fill(out + 72, out + 82, 0.0f);
+        out[82] = out[83] = -1.0f;
         if (primary.hit) {
             const SI& h = primary.isect;
             out[72] = length(h.point - ray.o);
@@ -1623,6 +1625,8 @@ struct Tracer {
                 out[76 + c] = obj[c];
                 out[79 + c] = mat[c];
             }
+            out[82] = static_cast<float>(shape);
+            out[83] = static_cast<float>(h.material);
         }
         return L;
     }
@@ -1767,7 +1771,7 @@ ORACLE_API int oracle_trace_aovs(void* sp, const pe_camera* cam, const pe_settin
     }
 }
 
-// Per-sample radiance and render passes (82 floats each, Tracer::tracePasses) for n (px, py, sample) triples.
+// Per-sample radiance and render passes (84 floats each, Tracer::tracePasses) for n (px, py, sample) triples.
 ORACLE_API int oracle_trace_passes(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,
                                    size_t n, const int32_t* px, const int32_t* py, const int32_t* s, float* L,
                                    float* out) {
@@ -1783,7 +1787,7 @@ ORACLE_API int oracle_trace_passes(void* sp, const pe_camera* cam, const pe_sett
         Sobol sampler(ps->seed);
         RayCounters rc;
         for (size_t i = 0; i < n; ++i) {
-            const V3 c = tr.tracePasses(px[i], py[i], s[i], sampler, rc, out + 82 * i);
+            const V3 c = tr.tracePasses(px[i], py[i], s[i], sampler, rc, out + 84 * i);
             L[3 * i] = c.x;
             L[3 * i + 1] = c.y;
             L[3 * i + 2] = c.z;

@@ -10,6 +10,7 @@
 #include "engine/scene.h"
 #include "camera/perspective_camera.h"
 #include "core/image/pixel_filter.h"
+#include "core/image/cryptomatte.h"
 #include "core/image/imager.h"
 #include "core/image/tone_mapping.h"
 #include "core/color/transfer.h"
@@ -123,7 +124,8 @@ REF_API int ref_sobol_batch(uint64_t seed, size_t n, const int32_t* px, const in
 // triangle lights Scene::addShape makes for emissive meshes have none) and the environment's, which
 // Scene::buildAccelerator resolves into group indices (scene.cpp:144-173). Meshes are tagged with the
 // descriptor's node uids (Scene::tagShape).
-void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, const char* envGroupName) {
+void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, const char* envGroupName,
+               const char* const* shapeNames, const char* const* materialNames) {
     Scene::setPreferOwnBvh(true);
     auto rs = new RefScene();
     rs->scene = std::make_shared<Scene>();
@@ -157,10 +159,14 @@ void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, co
         std::vector<uint32_t> idx(m.indices, m.indices + 3 * static_cast<size_t>(m.num_triangles));
         auto shape = std::make_shared<TriangleMesh>(pos, nrm, uv, idx, rs->materials[static_cast<size_t>(m.material)].get());
         if (desc->node_uids && desc->node_uids[i] != 0) rs->scene->tagShape(shape.get(), desc->node_uids[i]);
+        if (shapeNames && shapeNames[i] && shapeNames[i][0]) rs->scene->setShapeName(shape.get(), shapeNames[i]);
         rs->scene->addShape(shape);
     }
     for (int i = 0; i < desc->num_meshes; ++i) rs->meshMaterial.push_back(desc->meshes[i].material);
     for (auto& mat : rs->materials) rs->scene->retainMaterial(mat);
+    if (materialNames)
+        for (int i = 0; i < desc->num_materials; ++i)
+            if (materialNames[i] && materialNames[i][0]) rs->scene->setMaterialName(rs->materials[static_cast<size_t>(i)].get(), materialNames[i]);
     if (desc->environment) {
         const pe_env_desc& e = *desc->environment;
         std::shared_ptr<Image> img;
@@ -183,10 +189,22 @@ void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, co
     return rs;
 }
 
-REF_API void* ref_scene_build(const pe_scene_desc* desc) { return buildRef(desc, nullptr, nullptr); }
+REF_API void* ref_scene_build(const pe_scene_desc* desc) { return buildRef(desc, nullptr, nullptr, nullptr, nullptr); }
+// With light-group names, shape names (Scene::setShapeName, per descriptor mesh; area-light quads cannot be
+// named) and material names (Scene::setMaterialName); any array may be NULL and any entry NULL or "".
 REF_API void* ref_scene_build_groups(const pe_scene_desc* desc, const char* const* lightGroupNames,
-                                     const char* envGroupName) {
-    return buildRef(desc, lightGroupNames, envGroupName);
+                                     const char* envGroupName, const char* const* shapeNames,
+                                     const char* const* materialNames) {
+    return buildRef(desc, lightGroupNames, envGroupName, shapeNames, materialNames);
+}
+
+// cryptoHashName and cryptoHashToFloat (cryptomatte.cpp:78-112) of n names.
+REF_API int ref_crypto_hash(size_t n, const char* const* names, uint32_t* hashes, float* floats) {
+    for (size_t i = 0; i < n; ++i) {
+        hashes[i] = cryptoHashName(names[i]);
+        floats[i] = cryptoHashToFloat(hashes[i]);
+    }
+    return 0;
 }
 
 REF_API void ref_scene_free(void* h) { delete static_cast<RefScene*>(h); }
@@ -358,9 +376,10 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
 }
 
 // Per-sample radiance and render passes along tracePixel's path with wantPasses (renderer.cpp:162-171):
-// PathTracer::Li with a LightPasses, then the primary hit's data passes. out, 82 floats per sample: the eight
+// PathTracer::Li with a LightPasses, then the primary hit's data passes. out, per sample: the eight
 // light passes, the sixteen light groups, depth, position, idColor(nodeUidOf) and idColor(material index + 1)
-// (the descriptor's index of the hit mesh's material, where the reference hashes its address).
+// (the descriptor's index of the hit mesh's material, where the reference hashes its address), then the hit's
+// shape and material index (-1 for a miss): 84 floats.
 REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
                              float aspect, int width, int height, int maxBounces, int shadowQuality, int filterType,
                              float filterWidth, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
@@ -389,7 +408,7 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
         for (auto& c : lp.lightGroups) c = Color3f(0.0f);
         const Color3f c = integrator.Li(ray, *rs->scene, sampler, &primary, &lp);
         L[3 * i] = c.r; L[3 * i + 1] = c.g; L[3 * i + 2] = c.b;
-        float* o = out + 82 * i;
+        float* o = out + 84 * i;
         for (int k = 0; k < static_cast<int>(LightPass::Count); ++k) {
             o[3 * k] = lp.c[k].r; o[3 * k + 1] = lp.c[k].g; o[3 * k + 2] = lp.c[k].b;
         }
@@ -397,6 +416,7 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
             o[24 + 3 * g] = lp.lightGroups[g].r; o[25 + 3 * g] = lp.lightGroups[g].g; o[26 + 3 * g] = lp.lightGroups[g].b;
         }
         std::fill(o + 72, o + 82, 0.0f);
+        o[82] = o[83] = -1.0f;
         if (primary.hit) {
             o[72] = (primary.isect.point - ray.origin).length();
             o[73] = primary.isect.point.x; o[74] = primary.isect.point.y; o[75] = primary.isect.point.z;
@@ -407,6 +427,8 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
             const Color3f mc = idColor(mat >= 0 ? static_cast<uint64_t>(mat) + 1 : 0);
             o[76] = obj.r; o[77] = obj.g; o[78] = obj.b;
             o[79] = mc.r; o[80] = mc.g; o[81] = mc.b;
+            o[82] = static_cast<float>(shape);
+            o[83] = static_cast<float>(mat);
         }
     }
     return 0;
@@ -593,11 +615,12 @@ void ref_copy_passes(const RenderPassFilms& pf, int width, int height, float* ou
 
 extern "C" {
 // Renderer::renderSamples of samples [first, first + count) with RenderPassFilms (box filter of width 1,
-// `threads` workers): the resolved beauty and the pass films (ref_copy_passes layout); returns the number of
-// light groups the renderer made, or -1.
+// `threads` workers): the resolved beauty, the pass films (ref_copy_passes layout) and, with `crypto`, the
+// cryptomatte ranks CryptoFilm::resolveRanks(3) gives (object layers 0-2, then material layers 0-2, each
+// w * h RGBA); returns the number of light groups the renderer made, or -1.
 REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarget, const float* camUp, float fov,
                               float aspect, int width, int height, int maxBounces, int firstSample, int count,
-                              int threads, float* beauty, float* passes) {
+                              int threads, float* beauty, float* passes, float* crypto) {
     auto* rs = static_cast<RefScene*>(h);
     PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
     RenderSettings s;
@@ -623,6 +646,13 @@ REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarg
             beauty[3 * k + 2] = b.b;
         }
     ref_copy_passes(pf, width, height, passes);
+    if (crypto) {
+        const size_t layer = static_cast<size_t>(width) * height * 4;
+        const std::vector<CryptoRankImage> ranks[2] = {pf.cryptoObject.resolveRanks(3), pf.cryptoMaterial.resolveRanks(3)};
+        for (int k = 0; k < 2; ++k)
+            for (size_t j = 0; j < ranks[k].size() && j < 3; ++j)
+                std::copy(ranks[k][j].rgba.begin(), ranks[k][j].rgba.end(), crypto + (3 * k + j) * layer);
+    }
     return static_cast<int>(pf.lightGroups.size());
 }
 }  // extern "C"

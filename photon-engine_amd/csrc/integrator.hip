@@ -1469,6 +1469,62 @@ __global__ void __launch_bounds__(kBlock) k_film_passes(LaunchParams lp, PathBuf
     }
 }
 
+// A device CryptoFilm's arrays (pe_crypto_film).
+struct CryptoView {
+    float* ids;
+    uint32_t* counts;
+    uint32_t* totals;
+    unsigned long long* dropped;
+};
+
+// CryptoFilm::addSample (cryptomatte.cpp:121-133) of a pixel's samples in sample order: the total counts every
+// sample; a non-zero id increments its entry or takes the first free slot. `which` selects the first surface's
+// shape (0) or material (1) from the data record, mapped through `table` (n entries; outside it: no id).
+__device__ __forceinline__ void crypto_add(const CryptoView& f, const LaunchParams& lp, const PathBuffers& pb,
+                                           uint32_t q, uint32_t p, int which, const float* table, int n) {
+    if (!f.ids) return;
+    const uint32_t pixels = lp.num_pixels, data1 = lp.pass_count - 1;
+    float* ids = f.ids + static_cast<size_t>(p) * PE_CRYPTO_SLOTS;
+    uint32_t* counts = f.counts + static_cast<size_t>(p) * PE_CRYPTO_SLOTS;
+    uint32_t dropped = 0;
+    for (int k = 0; k < lp.spl; ++k) {
+        const float4 a = pb.pass_res[static_cast<size_t>(data1) * lp.num_paths + q + static_cast<uint32_t>(k) * pixels];
+        const int index = __float_as_int(which == 0 ? a.x : a.y);
+        const float id = index >= 0 && index < n ? table[index] : 0.0f;
+        if (id == 0.0f) continue;
+        int slot = -1;
+        for (int j = 0; j < PE_CRYPTO_SLOTS; ++j) {
+            const uint32_t c = counts[j];
+            if (c == 0u) {
+                if (slot < 0) slot = j;
+                break;
+            }
+            if (ids[j] == id) {
+                slot = j;
+                break;
+            }
+        }
+        if (slot < 0) {
+            ++dropped;
+        } else {
+            if (counts[slot] == 0u) ids[slot] = id;
+            counts[slot] += 1u;
+        }
+    }
+    f.totals[p] += static_cast<uint32_t>(lp.spl);
+    if (dropped) atomicAdd(f.dropped, static_cast<unsigned long long>(dropped));
+}
+
+__global__ void __launch_bounds__(kBlock) k_film_crypto(LaunchParams lp, PathBuffers pb, CryptoView obj, CryptoView mat,
+                                                        const float* __restrict__ obj_ids, int num_obj,
+                                                        const float* __restrict__ mat_ids, int num_mat) {
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < lp.num_pixels; q += gridDim.x * blockDim.x) {
+        const uint32_t p = lp.pixel_list ? lp.pixel_list[q] : q;
+        crypto_add(obj, lp, pb, q, p, 0, obj_ids, num_obj);
+        crypto_add(mat, lp, pb, q, p, 1, mat_ids, num_mat);
+    }
+}
+
 // ------------------------------------------------------------------------------ adaptive sampling
 // The batch's skip mask (renderer.cpp:442-467): a pixel is clean when its error is below the threshold and
 // is skipped for the whole batch when it and its eight neighbours (clamped at the border) are clean.
@@ -1651,8 +1707,9 @@ struct Wavefront {
     uint64_t aov_cap = 0;    // samples of the AOV result buffer
     DevBuf ro0, rd0, ro1, rd1, thr, L, last_p, last_n, ids, ext, pend, hits, sq_o, sq_d, fl0, fl1, ctr, stats, filter;
     DevBuf res, aov_res;
-    // Render passes: per-sample records (pass_cap float4), per-id lobe shares and per-record NEE routing.
-    DevBuf pass_res, lobe, pfrac;
+    // Render passes: per-sample records (pass_cap float4), per-id lobe shares and per-record NEE routing; the
+    // cryptomatte id tables of the render (objects, then materials).
+    DevBuf pass_res, lobe, pfrac, crypto_ids;
     uint64_t pass_cap = 0;
     uint32_t lobe_cap = 0;  // path ids the lobe / routing columns hold (at the current shadow sample count)
     // Adaptive sampling: per-pixel active flags and clean flags, the active pixel list, compaction counts.
@@ -1913,6 +1970,34 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             want_passes = true;
         }
     }
+    CryptoView crypto[2] = {};
+    std::vector<float> crypto_table;
+    int crypto_n[2] = {0, 0};
+    if (aovs && aovs->passes) {
+        const pe_pass_targets& pt = *aovs->passes;
+        pe_crypto_film* cf[2] = {pt.crypto_object, pt.crypto_material};
+        const float* tables[2] = {pt.object_ids, pt.material_ids};
+        const int counts[2] = {pt.num_object_ids, pt.num_material_ids};
+        for (int k = 0; k < 2; ++k) {
+            if (!cf[k]) continue;
+            if (cf[k]->dev != dev || cf[k]->width != film->width || cf[k]->height != film->height) {
+                set_error("pe_render: a cryptomatte film of another device or size");
+                return PE_ERR_INVALID;
+            }
+            if (counts[k] < 0 || (counts[k] > 0 && !tables[k])) {
+                set_error("pe_render: a cryptomatte id table without entries");
+                return PE_ERR_INVALID;
+            }
+            crypto[k] = CryptoView{cf[k]->ids.as<float>(), cf[k]->counts.as<uint32_t>(), cf[k]->totals.as<uint32_t>(),
+                                   cf[k]->dropped.as<unsigned long long>()};
+            crypto_n[k] = counts[k];
+            want_passes = true;
+        }
+        crypto_table.assign(1 + static_cast<size_t>(crypto_n[0]) + static_cast<size_t>(crypto_n[1]), 0.0f);
+        if (crypto_n[0] > 0) std::copy(pt.object_ids, pt.object_ids + crypto_n[0], crypto_table.begin());
+        if (crypto_n[1] > 0) std::copy(pt.material_ids, pt.material_ids + crypto_n[1], crypto_table.begin() + crypto_n[0]);
+    }
+    const bool want_crypto = crypto[0].ids || crypto[1].ids;
     if (first_sample < 0 || sample_count < 0) {
         set_error("pe_render: negative sample range");
         return PE_ERR_INVALID;
@@ -1969,6 +2054,11 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         if (rc == PE_OK) rc = wf->ensure_results(window, want_aov);
         if (rc == PE_OK && want_passes) rc = wf->ensure_passes(window, pass_count);
         if (rc != PE_OK) return rc;
+        if (want_crypto) {
+            if (!wf->crypto_ids.upload(crypto_table.data(), crypto_table.size() * sizeof(float), lane.stream))
+                return PE_ERR_OOM;
+            PE_HIP_CHECK(hipStreamSynchronize(lane.stream));
+        }
         if (guide) {
             rc = guide_reserve_paths(guide, wf->capacity, wf->slots());
             if (rc != PE_OK) return rc;
@@ -2191,6 +2281,11 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
                     std::min<uint64_t>((uint64_t{lp.num_pixels} * kPassFilms + kBlock - 1) / kBlock, 65535u));
                 hipLaunchKernelGGL(k_film_passes, dim3(pass_blocks), dim3(kBlock), 0, s, lp, pb, pass_films,
                                    view.node_uids);
+            }
+            if (want_crypto) {
+                const float* tbl = wf->crypto_ids.as<float>();
+                hipLaunchKernelGGL(k_film_crypto, dim3(film_blocks), dim3(kBlock), 0, s, lp, pb, crypto[0], crypto[1],
+                                   tbl, crypto_n[0], tbl + crypto_n[0], crypto_n[1]);
             }
             wf->prof.end(s);
             PE_HIP_CHECK(hipGetLastError());

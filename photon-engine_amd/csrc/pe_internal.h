@@ -108,6 +108,13 @@ struct pe_scene {
     }
 };
 
+// A device CryptoFilm: PE_CRYPTO_SLOTS (id, count) entries and a sample total per pixel.
+struct pe_crypto_film {
+    pe_device* dev = nullptr;
+    int width = 0, height = 0;
+    pe::DevBuf ids, counts, totals, dropped;
+};
+
 struct pe_server;
 
 struct pe_film {

@@ -127,4 +127,81 @@ int pe_film_download(pe_film* film, float* sum_rgb, uint32_t* count, float* sum_
     return PE_OK;
 }
 
+namespace {
+int crypto_clear(pe_crypto_film* f) {
+    const size_t px = static_cast<size_t>(f->width) * f->height;
+    const size_t words[4] = {PE_CRYPTO_SLOTS * px, PE_CRYPTO_SLOTS * px, px, 2};
+    void* bufs[4] = {f->ids.get(), f->counts.get(), f->totals.get(), f->dropped.get()};
+    for (int k = 0; k < 4; ++k) {
+        const int grid = static_cast<int>(std::min<size_t>((words[k] + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_zero_u32, dim3(grid), dim3(256), 0, f->dev->stream, static_cast<uint32_t*>(bufs[k]),
+                           words[k]);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error(std::string("crypto film clear: ") + hipGetErrorString(e));
+        return PE_ERR_HIP;
+    }
+    return PE_OK;
+}
+}  // namespace
+
+int pe_crypto_film_create(pe_device* dev, int width, int height, pe_crypto_film** out) {
+    if (!dev || !out || width <= 0 || height <= 0) {
+        set_error("pe_crypto_film_create: bad argument");
+        return PE_ERR_INVALID;
+    }
+    *out = nullptr;
+    PE_HIP_CHECK(hipSetDevice(dev->ordinal));
+    auto f = std::make_unique<pe_crypto_film>();
+    f->dev = dev;
+    f->width = width;
+    f->height = height;
+    const size_t px = static_cast<size_t>(width) * height;
+    if (!f->ids.alloc(px * PE_CRYPTO_SLOTS * 4) || !f->counts.alloc(px * PE_CRYPTO_SLOTS * 4) ||
+        !f->totals.alloc(px * 4) || !f->dropped.alloc(8))
+        return PE_ERR_OOM;
+    int rc = crypto_clear(f.get());
+    if (rc != PE_OK) return rc;
+    PE_HIP_CHECK(hipStreamSynchronize(dev->stream));
+    *out = f.release();
+    return PE_OK;
+}
+
+int pe_crypto_film_destroy(pe_crypto_film* film) {
+    if (!film) return PE_OK;
+    (void)hipSetDevice(film->dev->ordinal);
+    (void)hipStreamSynchronize(film->dev->stream);
+    delete film;
+    return PE_OK;
+}
+
+int pe_crypto_film_clear(pe_crypto_film* film) {
+    if (!film) {
+        set_error("pe_crypto_film_clear: NULL film");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipSetDevice(film->dev->ordinal));
+    int rc = crypto_clear(film);
+    if (rc != PE_OK) return rc;
+    PE_HIP_CHECK(hipStreamSynchronize(film->dev->stream));
+    return PE_OK;
+}
+
+int pe_crypto_film_download(pe_crypto_film* film, float* ids, uint32_t* counts, uint32_t* totals, uint64_t* dropped) {
+    if (!film) {
+        set_error("pe_crypto_film_download: NULL film");
+        return PE_ERR_INVALID;
+    }
+    PE_HIP_CHECK(hipSetDevice(film->dev->ordinal));
+    hipStream_t s = film->dev->stream;
+    const size_t px = static_cast<size_t>(film->width) * film->height;
+    if (ids) PE_HIP_CHECK(hipMemcpyAsync(ids, film->ids.get(), px * PE_CRYPTO_SLOTS * 4, hipMemcpyDeviceToHost, s));
+    if (counts) PE_HIP_CHECK(hipMemcpyAsync(counts, film->counts.get(), px * PE_CRYPTO_SLOTS * 4, hipMemcpyDeviceToHost, s));
+    if (totals) PE_HIP_CHECK(hipMemcpyAsync(totals, film->totals.get(), px * 4, hipMemcpyDeviceToHost, s));
+    if (dropped) PE_HIP_CHECK(hipMemcpyAsync(dropped, film->dropped.get(), 8, hipMemcpyDeviceToHost, s));
+    PE_HIP_CHECK(hipStreamSynchronize(s));
+    return PE_OK;
+}
+
 }  // extern "C"

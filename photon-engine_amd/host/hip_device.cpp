@@ -14,8 +14,10 @@
 #include "hip_device.h"
 
 #include "camera/camera.h"
+#include "core/image/cryptomatte.h"
 #include "core/image/film.h"
 #include "core/image/image.h"
+#include "engine/renderer.h"
 #include "engine/scene.h"
 #include "geometry/mesh.h"
 #include "geometry/triangle.h"
@@ -63,6 +65,7 @@ struct PackedScene {
     int32_t envGroup = -1;
     int32_t numGroups = 0;
     std::vector<uint64_t> nodeUids;    // per shape: Scene::nodeUidOf
+    std::vector<const Material*> materialPtrs;  // per packed material: the scene's material
 
     const float* keep(std::vector<float> v) {
         floats.push_back(std::move(v));
@@ -157,6 +160,7 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
         pe_material_desc d;
         if (!packMaterial(m, d, err)) return -1;
         p.materials.push_back(d);
+        p.materialPtrs.push_back(m);
         const int k = static_cast<int>(p.materials.size()) - 1;
         matIndex.emplace(m, k);
         return k;
@@ -274,6 +278,52 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
     return true;
 }
 
+// The material of a shape as the cryptomatte lookup registers it (renderer.cpp:101-109).
+const Material* shapeMaterial(const Shape* shape) {
+    if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape)) return mesh->material();
+    if (auto* tri = dynamic_cast<const Triangle*>(shape)) return tri->material();
+    return nullptr;
+}
+
+// buildCryptoLookup (renderer.cpp:65-118) for the packed scene: names registered into the pass films'
+// manifests; the id of every shape (scene.shapes() order = the packed meshes) and of every packed material.
+void cryptoTables(const Scene& scene, const std::vector<const Material*>& materials, RenderPassFilms& passes,
+                  std::vector<float>& objectIds, std::vector<float>& materialIds) {
+    std::unordered_map<const Material*, float> mats;
+    size_t anonShape = 0, anonMat = 0;
+    auto registerMat = [&](const Material* m) {
+        if (!m || mats.count(m)) return;
+        const auto it = scene.materialNames().find(m);
+        const std::string name = it != scene.materialNames().end() && !it->second.empty()
+                                     ? it->second
+                                     : "material_" + std::to_string(anonMat++);
+        mats[m] = passes.cryptoMaterial.registerName(name);
+    };
+    for (const auto& [m, name] : scene.materialNames())
+        if (m && !name.empty()) mats[m] = passes.cryptoMaterial.registerName(name);
+    objectIds.clear();
+    std::unordered_map<const void*, bool> seen;
+    for (const auto& shape : scene.shapes()) {
+        const void* ptr = shape.get();
+        std::string name;
+        const auto nit = scene.shapeNames().find(ptr);
+        const auto oit = scene.shapeOwners().find(ptr);
+        if (nit != scene.shapeNames().end() && !nit->second.empty()) name = nit->second;
+        else if (oit != scene.shapeOwners().end() && oit->second != 0) name = "object_" + std::to_string(oit->second);
+        else name = "shape_" + std::to_string(anonShape++);
+        objectIds.push_back(passes.cryptoObject.registerName(name));
+        seen[ptr] = true;
+        registerMat(shapeMaterial(shape.get()));
+    }
+    for (const auto& [ptr, name] : scene.shapeNames())
+        if (ptr && !name.empty() && !seen.count(ptr)) passes.cryptoObject.registerName(name);
+    materialIds.assign(materials.size(), 0.0f);
+    for (size_t i = 0; i < materials.size(); ++i) {
+        const auto it = mats.find(materials[i]);
+        if (it != mats.end()) materialIds[i] = it->second;
+    }
+}
+
 }  // namespace
 
 std::vector<DeviceInfo> enumerateHipDevices() {
@@ -316,6 +366,8 @@ std::unique_ptr<HipDevice> HipDevice::create(int ordinal, std::string* error) {
 HipDevice::~HipDevice() {
     for (DeviceFilm& f : m_films)
         if (f.film) pe_film_destroy(f.film);
+    for (DeviceCrypto& c : m_crypto)
+        if (c.film) pe_crypto_film_destroy(c.film);
     if (m_guide) pe_guide_destroy(m_guide);
     if (m_scene) pe_scene_release(m_scene);
     if (m_dev) pe_device_destroy(m_dev);
@@ -341,6 +393,7 @@ void HipDevice::setScene(std::shared_ptr<const Scene> scene) {
         return;
     }
     const pe_scene_desc desc = packed.desc();
+    m_materialPtrs = packed.materialPtrs;
     if (pe_scene_upload(m_dev, &desc, &m_scene) != PE_OK) {
         m_error = "HipDevice::setScene: " + lastHipError();
         m_scene = nullptr;
@@ -368,6 +421,7 @@ void HipDevice::updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind 
         setScene(std::move(scene));
         return;
     }
+    m_materialPtrs = packed.materialPtrs;
     m_sceneRef = std::move(scene);
 }
 
@@ -430,7 +484,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     const int w = film.width(), h = film.height();
     // The caller's films: beauty and, when it asks for them, the AOVs (renderer.cpp:222-227) and the render
     // passes (light passes, the scene's light groups, depth, position, object and material id; renderer.cpp:
-    // 207-219, with the groups named as the scene's, :400-405). Cryptomatte films are left as they are.
+    // 207-219, with the groups named as the scene's, :400-405); cryptomatte below.
     Film* targets_of[kDeviceFilms] = {&film};
     if (targets.aovs) {
         AovFilms& a = *targets.aovs;
@@ -461,6 +515,10 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
             if (f.film) pe_film_destroy(f.film);
             f.film = nullptr;
         }
+        for (DeviceCrypto& c : m_crypto) {
+            if (c.film) pe_crypto_film_destroy(c.film);
+            c.film = nullptr;
+        }
         m_filmW = w;
         m_filmH = h;
         m_accumulating = false;
@@ -489,6 +547,34 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
             f.prevCount.assign(pixels, 0u);
         }
     }
+    // Cryptomatte (renderer.cpp:408-411, :215-216): the ids of this scene's names, counted on the device.
+    std::vector<float> objectIds, materialIds;
+    if (targets.passes) {
+        RenderPassFilms& pf = *targets.passes;
+        if (pf.cryptoObject.width() != w || pf.cryptoObject.height() != h) {
+            pf.cryptoObject.resize(w, h);
+            pf.cryptoMaterial.resize(w, h);
+        }
+        cryptoTables(*m_sceneRef, m_materialPtrs, pf, objectIds, materialIds);
+        for (DeviceCrypto& c : m_crypto) {
+            if (!c.film) {
+                if (pe_crypto_film_create(m_dev, w, h, &c.film) != PE_OK) {
+                    m_error = "HipDevice::render: " + lastHipError();
+                    return res;
+                }
+                c.prevCounts.assign(PE_CRYPTO_SLOTS * pixels, 0u);
+                c.prevTotals.assign(pixels, 0u);
+            }
+            if (restart) {
+                if (pe_crypto_film_clear(c.film) != PE_OK) {
+                    m_error = "HipDevice::render: " + lastHipError();
+                    return res;
+                }
+                c.prevCounts.assign(PE_CRYPTO_SLOTS * pixels, 0u);
+                c.prevTotals.assign(pixels, 0u);
+            }
+        }
+    }
     if (restart) m_accumulating = adaptive;
     pe_targets tg{};
     tg.beauty = m_films[0].film;
@@ -507,6 +593,12 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         pt.position = m_films[kPassFilm0 + 25].film;
         pt.object_id = m_films[kPassFilm0 + 26].film;
         pt.material_id = m_films[kPassFilm0 + 27].film;
+        pt.crypto_object = m_crypto[0].film;
+        pt.crypto_material = m_crypto[1].film;
+        pt.object_ids = objectIds.data();
+        pt.num_object_ids = static_cast<int32_t>(objectIds.size());
+        pt.material_ids = materialIds.data();
+        pt.num_material_ids = static_cast<int32_t>(materialIds.size());
         tg.passes = &pt;
     }
     // The whole batch is one call: the library polls the caller's cancel flag between its iterations (a few
@@ -551,6 +643,35 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         if (m_accumulating) {
             f.prevSum = sum;
             f.prevCount = cnt;
+        }
+    }
+    // Cryptomatte: what this call counted, into the CryptoFilms (CryptoFilm::addSamples / addTotalSamples, the
+    // merge path the reference keeps for GPU read-back). A slot keeps its id once taken, so its new count is
+    // the difference from the last merge.
+    if (targets.passes) {
+        std::vector<float> ids(PE_CRYPTO_SLOTS * pixels);
+        std::vector<uint32_t> counts(PE_CRYPTO_SLOTS * pixels), totals(pixels);
+        CryptoFilm* dsts[2] = {&targets.passes->cryptoObject, &targets.passes->cryptoMaterial};
+        for (int k = 0; k < 2; ++k) {
+            DeviceCrypto& c = m_crypto[k];
+            uint64_t dropped = 0;
+            if (pe_crypto_film_download(c.film, ids.data(), counts.data(), totals.data(), &dropped) != PE_OK) {
+                m_error = "HipDevice::render: " + lastHipError();
+                return res;
+            }
+            for (int y = 0; y < h; ++y)
+                for (int x = 0; x < w; ++x) {
+                    const size_t p = static_cast<size_t>(y) * static_cast<size_t>(w) + static_cast<size_t>(x);
+                    dsts[k]->addTotalSamples(x, y, totals[p] - c.prevTotals[p]);
+                    for (int j = 0; j < PE_CRYPTO_SLOTS; ++j) {
+                        const size_t q = p * PE_CRYPTO_SLOTS + static_cast<size_t>(j);
+                        if (counts[q] > c.prevCounts[q]) dsts[k]->addSamples(x, y, ids[q], counts[q] - c.prevCounts[q]);
+                    }
+                }
+            if (m_accumulating) {
+                c.prevCounts = counts;
+                c.prevTotals = totals;
+            }
         }
     }
     res.completed = !cancelled;

@@ -16,6 +16,8 @@
 
 namespace photon {
 
+class Material;
+
 class HipDevice final : public RenderDevice {
 public:
     // nullptr and *error set when the ordinal has no usable gfx950 device (createDevice contract).
@@ -23,8 +25,8 @@ public:
     ~HipDevice() override;
 
     const DeviceInfo& info() const override { return m_info; }
-    // AOVs (albedo / normal / alpha), render passes (light passes, light groups, data passes; no cryptomatte)
-    // and adaptive sampling.
+    // AOVs (albedo / normal / alpha), render passes (light passes, light groups, data passes, cryptomatte) and
+    // adaptive sampling.
     DeviceFeatures features() const override { return DeviceFeatures{true, true, true}; }
     void setScene(std::shared_ptr<const Scene> scene) override;
     void updateScene(std::shared_ptr<const Scene> scene, SceneUpdateKind kind) override;
@@ -56,6 +58,13 @@ private:
     };
     static constexpr int kPassFilm0 = 4, kDeviceFilms = kPassFilm0 + 28;
     DeviceFilm m_films[kDeviceFilms];
+    // Device CryptoFilms (objects, materials) and the per-slot counts and totals already merged.
+    struct DeviceCrypto {
+        pe_crypto_film* film = nullptr;
+        std::vector<uint32_t> prevCounts, prevTotals;
+    };
+    DeviceCrypto m_crypto[2];
+    std::vector<const Material*> m_materialPtrs;  // per packed material: the scene's material (cryptomatte ids)
     int m_filmW = 0, m_filmH = 0;
     bool m_accumulating = false;
     std::shared_ptr<const Scene> m_sceneRef;

@@ -99,7 +99,12 @@ LIGHT_PASS_NAMES = ("DiffuseDirect", "DiffuseIndirect", "GlossyDirect", "GlossyI
 class PassTargets(C.Structure):
     _fields_ = [("light", C.c_void_p * PE_LIGHT_PASSES), ("light_groups", C.c_void_p * PE_MAX_LIGHT_GROUPS),
                 ("depth", C.c_void_p), ("position", C.c_void_p), ("object_id", C.c_void_p),
-                ("material_id", C.c_void_p)]
+                ("material_id", C.c_void_p), ("crypto_object", C.c_void_p), ("crypto_material", C.c_void_p),
+                ("object_ids", C.POINTER(C.c_float)), ("material_ids", C.POINTER(C.c_float)),
+                ("num_object_ids", C.c_int32), ("num_material_ids", C.c_int32)]
+
+
+PE_CRYPTO_SLOTS = 8
 
 
 class Camera(C.Structure):
@@ -211,6 +216,10 @@ SIGNATURES = {
     "pe_film_destroy": (C.c_int, [_VP]),
     "pe_film_clear": (C.c_int, [_VP]),
     "pe_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_float), _P(C.c_uint64)]),
+    "pe_crypto_film_create": (C.c_int, [_VP, C.c_int, C.c_int, _P(_VP)]),
+    "pe_crypto_film_destroy": (C.c_int, [_VP]),
+    "pe_crypto_film_clear": (C.c_int, [_VP]),
+    "pe_crypto_film_download": (C.c_int, [_VP, _P(C.c_float), _P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint64)]),
     "pe_display_defaults": (C.c_int, [_P(Display)]),
     "pe_film_display": (C.c_int, [_VP, _P(Display), _P(C.c_float), _P(C.c_uint8)]),
     "pe_job_display": (C.c_int, [_VP, C.c_uint64, _P(Display), _P(C.c_float), _P(C.c_uint8)]),

@@ -398,16 +398,19 @@ def pass_studio(width: int = 64, height: int = 48, spp: int = 16, max_bounces: i
     """The render-pass test scene (tests/test_render_passes.cpp:33-53): a Lambert floor, a rough gold metal
     sphere, a smooth glass sphere, an emissive quad seen by the camera, an area light and a flat grey
     environment; spheres as icospheres. With groups the area light is in light group "Key" (the quad's mesh
-    lights fall into "Default", the environment into "Env"), and every shape carries a node uid."""
+    lights fall into "Default", the environment into "Env"), every shape carries a node uid, and the floor and
+    the metal sphere and their materials carry names (cryptomatte: named, uid-named and anonymous objects)."""
     sc = Scene()
     up = np.array([[0.0, 1.0, 0.0]] * 4, dtype=np.float32)
-    floor = sc.add_material(lambert((0.6, 0.6, 0.6)))
-    metal = sc.add_material(disney((0.95, 0.8, 0.5), 1.0, 0.25, 0.5))
+    floor = sc.add_material(lambert((0.6, 0.6, 0.6)), name="FloorMat" if groups else "")
+    metal = sc.add_material(disney((0.95, 0.8, 0.5), 1.0, 0.25, 0.5), name="Gold" if groups else "")
     glass = sc.add_material(dielectric(1.5, (1.0, 1.0, 1.0), 0.0))
     emitter = sc.add_material(lambert((0.0, 0.0, 0.0), (6.0, 6.0, 6.0)))
     fp = np.array([(-4, 0, -4), (4, 0, -4), (4, 0, 4), (-4, 0, 4)], dtype=np.float32)
-    sc.add_shape(Mesh(fp, np.array([[0, 2, 1], [0, 3, 2]], dtype=np.uint32), up), floor, node_uid=11 if groups else 0)
-    sc.add_shape(icosphere(level, 0.6, (-0.8, 0.6, 0.0)), metal, node_uid=12 if groups else 0)
+    sc.add_shape(Mesh(fp, np.array([[0, 2, 1], [0, 3, 2]], dtype=np.uint32), up), floor, node_uid=11 if groups else 0,
+                 name="Floor" if groups else "")
+    sc.add_shape(icosphere(level, 0.6, (-0.8, 0.6, 0.0)), metal, node_uid=12 if groups else 0,
+                 name="Hero_Metal" if groups else "")
     sc.add_shape(icosphere(level, 0.6, (0.8, 0.6, 0.0)), glass, node_uid=13 if groups else 0)
     ep = np.array([(-0.4, 1.8, -1.5), (0.4, 1.8, -1.5), (0.4, 2.4, -1.5), (-0.4, 2.4, -1.5)], dtype=np.float32)
     sc.add_shape(Mesh(ep, np.array([[0, 1, 2], [0, 2, 3]], dtype=np.uint32)), emitter, node_uid=14 if groups else 0)

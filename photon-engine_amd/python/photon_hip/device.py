@@ -184,18 +184,73 @@ class GpuFilm:
         return s / np.maximum(c, 1)[..., None]
 
 
+class CryptoFilm:
+    """pe_crypto_film: per pixel up to PE_CRYPTO_SLOTS (id, count) entries in first-seen order and a total."""
+
+    def __init__(self, dev: Device, width: int, height: int):
+        self.dev, self.width, self.height = dev, width, height
+        self._lib = load()
+        h = C.c_void_p()
+        check("pe_crypto_film_create", self._lib.pe_crypto_film_create(dev.h, width, height, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and getattr(self.dev, "h", None):
+            self._lib.pe_crypto_film_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def clear(self):
+        check("pe_crypto_film_clear", self._lib.pe_crypto_film_clear(self.h))
+
+    def download(self):
+        """Returns (ids (H, W, S), counts (H, W, S), totals (H, W), dropped)."""
+        S = capi.PE_CRYPTO_SLOTS
+        ids = np.zeros((self.height, self.width, S), np.float32)
+        counts = np.zeros((self.height, self.width, S), np.uint32)
+        totals = np.zeros((self.height, self.width), np.uint32)
+        dropped = C.c_uint64()
+        check("pe_crypto_film_download", self._lib.pe_crypto_film_download(
+            self.h, ptr(ids, C.c_float), ptr(counts, C.c_uint32), ptr(totals, C.c_uint32), C.byref(dropped)))
+        return ids, counts, totals, dropped.value
+
+    def pixel_ranks(self, x: int, y: int, data=None):
+        """CryptoFilm::pixelRanks (cryptomatte.cpp:195-205): (id, coverage) by decreasing count."""
+        ids, counts, totals, _ = data if data is not None else self.download()
+        c, t = counts[y, x], totals[y, x]
+        if t == 0:
+            return []
+        order = sorted((j for j in range(len(c)) if c[j] > 0), key=lambda j: -int(c[j]))
+        inv = np.float32(1.0) / np.float32(t)
+        return [(float(ids[y, x, j]), float(np.float32(c[j]) * inv)) for j in order]
+
+
 class PassFilms:
     """RenderPassFilms without cryptomatte: a film per light pass (LightPass order), per light group of the
     scene (up to 16) and per data pass, each of the beauty film's size."""
 
-    def __init__(self, dev: Device, width: int, height: int, groups: int = 0, light=True, data=True):
+    def __init__(self, dev: Device, width: int, height: int, groups: int = 0, light=True, data=True,
+                 crypto_tables=None):
         mk = lambda: GpuFilm(dev, width, height)  # noqa: E731
         self.light = [mk() for _ in range(capi.PE_LIGHT_PASSES)] if light else []
         self.groups = [mk() for _ in range(min(groups, capi.PE_MAX_LIGHT_GROUPS))]
         self.depth, self.position, self.object_id, self.material_id = (mk() for _ in range(4)) if data else (None,) * 4
+        # Cryptomatte: (object ids per mesh, material ids per material), as Scene.crypto_tables gives them.
+        self.crypto_object = self.crypto_material = None
+        self._tables = None
+        if crypto_tables is not None:
+            self.crypto_object = CryptoFilm(dev, width, height)
+            self.crypto_material = CryptoFilm(dev, width, height)
+            self._tables = tuple(np.ascontiguousarray(t, np.float32) for t in crypto_tables[:2])
 
     def targets(self) -> capi.PassTargets:
         pt = capi.PassTargets()
+        if self.crypto_object is not None:
+            pt.crypto_object, pt.crypto_material = self.crypto_object.h, self.crypto_material.h
+            pt.object_ids, pt.material_ids = (ptr(t, C.c_float) for t in self._tables)
+            pt.num_object_ids, pt.num_material_ids = (int(t.size) for t in self._tables)
         for k, f in enumerate(self.light):
             pt.light[k] = f.h
         for k, f in enumerate(self.groups):

@@ -120,6 +120,41 @@ class Environment:
     light_group: str = ""
 
 
+def crypto_hash(name: str) -> int:
+    """cryptoHashName: MurmurHash3_x86_32 of the UTF-8 name with seed 0 (Psyop Cryptomatte 1.2.0;
+    core/image/cryptomatte.cpp:28-76)."""
+    data = name.encode("utf-8")
+    c1, c2, mask = 0xCC9E2D51, 0x1B873593, 0xFFFFFFFF
+    rotl = lambda x, r: ((x << r) | (x >> (32 - r))) & mask  # noqa: E731
+    h = 0
+    n4 = len(data) // 4
+    for i in range(n4):
+        k = int.from_bytes(data[4 * i:4 * i + 4], "little")
+        k = rotl((k * c1) & mask, 15)
+        h ^= (k * c2) & mask
+        h = (rotl(h, 13) * 5 + 0xE6546B64) & mask
+    tail = data[4 * n4:]
+    if tail:
+        k = int.from_bytes(tail, "little")
+        k = rotl((k * c1) & mask, 15)
+        h ^= (k * c2) & mask
+    h ^= len(data)
+    h ^= h >> 16
+    h = (h * 0x85EBCA6B) & mask
+    h ^= h >> 13
+    h = (h * 0xC2B2AE35) & mask
+    return h ^ (h >> 16)
+
+
+def crypto_hash_to_float(h: int) -> float:
+    """cryptoHashToFloat (cryptomatte.cpp:99-109): an exponent of 0 or 255 has its lowest bit flipped, so every
+    id is a finite, normal float32."""
+    exp = (h >> 23) & 0xFF
+    if exp in (0, 0xFF):
+        h = (h & 0x80000000) | ((exp ^ 1) << 23) | (h & 0x7FFFFF)
+    return float(np.array([h], np.uint32).view(np.float32)[0])
+
+
 @dataclass
 class _Shape:
     mesh: Mesh
@@ -128,6 +163,7 @@ class _Shape:
     standalone: bool
     flags: int = 0
     node_uid: int = 0          # Scene::tagShape: the owning scene-graph node (object-ID pass), 0 = untagged
+    name: str = ""             # Scene::setShapeName (cryptomatte object name), "" = unnamed
 
 
 @dataclass
@@ -137,12 +173,15 @@ class Scene:
     lights: list = field(default_factory=list)     # LightDesc in Scene::lights() order
     environment: Environment | None = None
     light_group_of: list = field(default_factory=list)  # per light: its group name ("" = none)
+    material_names: dict = field(default_factory=dict)  # material index -> Scene::setMaterialName name
 
-    def add_material(self, m: capi.MaterialDesc) -> int:
+    def add_material(self, m: capi.MaterialDesc, name: str = "") -> int:
         self.materials.append(m)
+        if name:
+            self.material_names[len(self.materials) - 1] = name
         return len(self.materials) - 1
 
-    def add_shape(self, mesh: Mesh, material: int, node_uid: int = 0) -> None:
+    def add_shape(self, mesh: Mesh, material: int, node_uid: int = 0, name: str = "") -> None:
         mat = self.materials[material]
         emissive = any(float(x) != 0.0 for x in mat.emission)
         base = -1
@@ -157,7 +196,7 @@ class Scene:
                 L.visible_to_camera = 1
                 self.lights.append(L)
                 self.light_group_of.append("")
-        self.shapes.append(_Shape(mesh, material, base, False, 0, int(node_uid)))
+        self.shapes.append(_Shape(mesh, material, base, False, 0, int(node_uid), name))
 
     def add_light(self, light: AreaLight) -> None:
         index = len(self.lights)
@@ -196,6 +235,40 @@ class Scene:
     @property
     def num_triangles(self) -> int:
         return sum(s.mesh.num_triangles for s in self.shapes)
+
+    def crypto_tables(self):
+        """Renderer's cryptomatte lookup (buildCryptoLookup, engine/renderer.cpp:65-118): every named material
+        registers its name; then each shape in order registers its name, or object_<node uid> when tagged, or
+        shape_<k> (k counting the anonymous shapes), and its material, unless registered, as
+        material_<k> (k counting the anonymous materials). Returns (object id per shape, material id per
+        material (0: never registered), object manifest {name: hash}, material manifest)."""
+        obj = np.zeros(len(self.shapes), np.float32)
+        mat = np.zeros(len(self.materials), np.float32)
+        obj_manifest, mat_manifest = {}, {}
+        done = set()
+        for m, name in self.material_names.items():
+            if name:
+                mat_manifest[name] = crypto_hash(name)
+                mat[m] = crypto_hash_to_float(mat_manifest[name])
+                done.add(m)
+        anon_shape = anon_mat = 0
+        for i, sh in enumerate(self.shapes):
+            if sh.name:
+                name = sh.name
+            elif sh.node_uid != 0:
+                name = f"object_{sh.node_uid}"
+            else:
+                name = f"shape_{anon_shape}"
+                anon_shape += 1
+            obj_manifest[name] = crypto_hash(name)
+            obj[i] = crypto_hash_to_float(obj_manifest[name])
+            if sh.material not in done:
+                name = f"material_{anon_mat}"
+                anon_mat += 1
+                mat_manifest[name] = crypto_hash(name)
+                mat[sh.material] = crypto_hash_to_float(mat_manifest[name])
+                done.add(sh.material)
+        return obj, mat, obj_manifest, mat_manifest
 
     def light_groups(self):
         """Scene::buildAccelerator's light-group tables (engine/scene.cpp:144-173): when any light or the

@@ -246,16 +246,139 @@ def test_gpu_session_passes_through_hip_device(dev):
     sc, view = configs.pass_studio()
     names, _, _ = sc.light_groups()
     spp = 16
-    rc, beauty, films, counts, groups, err = bindings.HipDeviceSession(sc).render_passes(view, spp, 4)
+    rc, beauty, films, counts, groups, err, _ = bindings.HipDeviceSession(sc).render_passes(view, spp, 4)
     assert rc == 0, err
     assert (counts == spp).all() and groups == len(names)
     film, pf, _ = _gpu_passes(dev, sc, view, 0, spp, spl=spp, groups=len(names))
     direct = _films_of(pf, len(names))
     np.testing.assert_allclose(beauty, film.mean(), rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(films, direct, rtol=1e-5, atol=1e-6)
-    rb, rfilms, rgroups = bindings.RefScene(sc).render_passes(view, 0, spp, threads=8)
+    rb, rfilms, rgroups, _ = bindings.RefScene(sc).render_passes(view, 0, spp, threads=8)
     assert rgroups == len(names)
     for k in list(range(8 + rgroups)) + [24, 25, 26]:
         err_k = float(np.mean((films[k] - rfilms[k]) ** 2 / (rfilms[k] ** 2 + 0.01)))
         assert err_k <= 1e-3, (k, err_k)
     assert float(np.mean((beauty - rb) ** 2 / (rb ** 2 + 0.01))) <= 1e-3
+
+
+# ------------------------------------------------------------------------------------------------- cryptomatte
+
+def _crypto_counts(ids_per_sample, width, height, px, py):
+    """CryptoFilm::addSample over samples in order: per pixel {id: count} and the total (cryptomatte.cpp:121-133)."""
+    buckets = {}
+    for x, y, i in zip(px, py, ids_per_sample):
+        b = buckets.setdefault((int(x), int(y)), [0, {}])
+        b[0] += 1
+        if i != 0.0:
+            b[1][float(i)] = b[1].get(float(i), 0) + 1
+    return buckets
+
+
+def _pairs(bucket):
+    """A bucket's (id, coverage) pairs as pixelRanks computes coverage (count * (1 / total), float32)."""
+    total, entries = bucket
+    inv = np.float32(1.0) / np.float32(total)
+    return sorted((i, float(np.float32(c) * inv)) for i, c in entries.items())
+
+
+def _rank_pairs(ranks, x, y):
+    """(id, coverage) pairs of a pixel from resolveRanks(3) layers (3, H, W, 4); zero coverage = no entry."""
+    out = []
+    for layer in ranks:
+        r, g, b, a = layer[y, x]
+        out += [(float(r), float(g))] if g > 0 else []
+        out += [(float(b), float(a))] if a > 0 else []
+    return sorted(out)
+
+
+def test_crypto_hash_equals_reference_and_known_answers():
+    """cryptoHashName (MurmurHash3_x86_32, seed 0) against the published vectors and the reference's own build;
+    cryptoHashToFloat keeps every id finite and normal (test_v04_production.cpp:146-153)."""
+    from photon_hip.scene import crypto_hash, crypto_hash_to_float
+
+    assert crypto_hash("") == 0 and crypto_hash("hello") == 0x248BFA47 and crypto_hash("test") == 0xBA6BD213
+    tricky = [0x00000000, 0x00000001, 0x007FFFFF, 0x7F800000, 0x7FC00000, 0x7FFFFFFF, 0x80000000, 0xFF800000,
+              0xFFFFFFFF]
+    for h in tricky:
+        f = np.float32(crypto_hash_to_float(h))
+        assert np.isfinite(f) and f != 0 and abs(f) >= np.finfo(np.float32).tiny
+    _need_ref()
+    names = ["", "a", "ab", "abc", "abcd", "Hero_A", "Hero_B", "object_12", "shape_0", "material_3",
+             "çok katmanlı", "x" * 37]
+    h, f = bindings.crypto_hash_reference(names)
+    assert [crypto_hash(n) for n in names] == h.tolist()
+    np.testing.assert_array_equal(np.array([crypto_hash_to_float(int(v)) for v in h], np.float32), f)
+
+
+def test_crypto_lookup_and_counts_equal_reference_renderer():
+    """The scene description's cryptomatte lookup (named, uid-named and anonymous objects and materials) and
+    CryptoFilm's counting, applied to the oracle's per-sample first surfaces, give the (id, coverage) ranks the
+    reference's Renderer::renderSamples with RenderPassFilms writes for the same samples."""
+    _need_ref()
+    sc, view = configs.pass_studio(width=24, height=18)
+    obj_ids, mat_ids, _, _ = sc.crypto_tables()
+    spp = 4
+    _, _, _, ranks = bindings.RefScene(sc, strict=True).render_passes(view, 0, spp, threads=4)
+    o = bindings.OracleScene(sc)
+    yy, xx = np.mgrid[0:view.height, 0:view.width]
+    st = settings(max_bounces=view.max_bounces, seed=12345)
+    px = np.tile(xx.ravel(), spp)
+    py = np.tile(yy.ravel(), spp)
+    s = np.repeat(np.arange(spp), xx.size)
+    _, p = o.trace_passes(_cam(view), st, view.width, view.height, px, py, s)
+    order = np.lexsort((s, px, py))  # per pixel, in sample order
+    for k, (key, table) in enumerate((("shape", obj_ids), ("material", mat_ids))):
+        idx = p[key][order]
+        ids = np.where(idx >= 0, table[np.clip(idx, 0, None)], 0.0)
+        buckets = _crypto_counts(ids, view.width, view.height, px[order], py[order])
+        for (x, y), b in buckets.items():
+            assert _pairs(b) == _rank_pairs(ranks[k], x, y), (key, x, y)
+    names = set(sc.crypto_tables()[2])
+    assert {"Floor", "Hero_Metal", "object_13", "object_14", "shape_0", "shape_1"} <= names
+
+
+@pytest.mark.gpu
+def test_gpu_cryptomatte_matches_oracle_and_reference(dev):
+    """Device CryptoFilms against the oracle's per-sample first surfaces counted as CryptoFilm counts them, and
+    against the reference renderer's ranks for the same samples; then through HipDevice (RenderSession with
+    passes), whose merged CryptoFilms give the same ranks."""
+    from photon_hip.device import PassFilms, render_targets
+
+    sc, view = configs.pass_studio(width=32, height=24)
+    tables = sc.crypto_tables()
+    spp = 8
+    gs = dev.upload(sc)
+    film = dev.film(view.width, view.height)
+    pf = PassFilms(dev, view.width, view.height, light=False, data=False, crypto_tables=tables)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=spp)
+    assert render_targets(dev, gs, _cam(view), st, film, 0, spp, passes=pf).completed == 1
+    o = bindings.OracleScene(sc)
+    yy, xx = np.mgrid[0:view.height, 0:view.width]
+    px, py = np.tile(xx.ravel(), spp), np.tile(yy.ravel(), spp)
+    s = np.repeat(np.arange(spp), xx.size)
+    _, p = o.trace_passes(_cam(view), settings(max_bounces=view.max_bounces, seed=12345), view.width, view.height,
+                          px, py, s)
+    order = np.lexsort((s, px, py))
+    _, _, _, ranks = bindings.RefScene(sc, strict=True).render_passes(view, 0, spp, threads=8)
+    agree = []
+    for k, (key, table, cf) in enumerate((("shape", tables[0], pf.crypto_object),
+                                          ("material", tables[1], pf.crypto_material))):
+        data = cf.download()
+        assert data[3] == 0  # no pixel saw more than PE_CRYPTO_SLOTS ids
+        assert (data[2] == spp).all()
+        idx = p[key][order]
+        ids = np.where(idx >= 0, table[np.clip(idx, 0, None)], 0.0)
+        buckets = _crypto_counts(ids, view.width, view.height, px[order], py[order])
+        for (x, y), b in buckets.items():
+            mine = sorted(cf.pixel_ranks(x, y, data))
+            agree.append(mine == _pairs(b) and mine == _rank_pairs(ranks[k], x, y))
+    print(f"cryptomatte: pixels agreeing with the oracle and the reference renderer {np.mean(agree):.4f}", flush=True)
+    assert np.mean(agree) >= 0.995
+    if bindings.hip_device_harness() is not None:
+        rc, _, _, _, _, err, hranks = bindings.HipDeviceSession(sc).render_passes(view, spp, 4)
+        assert rc == 0, err
+        same = [_rank_pairs(hranks[k], x, y) == _rank_pairs(ranks[k], x, y)
+                for k in range(2) for y in range(view.height) for x in range(view.width)]
+        print(f"cryptomatte through HipDevice: pixels agreeing with the reference renderer {np.mean(same):.4f}",
+              flush=True)
+        assert np.mean(same) >= 0.995
