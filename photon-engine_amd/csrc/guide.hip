@@ -307,6 +307,7 @@ struct ArrayQueries {
         return true;
     }
     __device__ __forceinline__ QueueSegs segs(uint32_t n) const { return QueueSegs{nullptr, n, n, 1u}; }
+    __device__ __forceinline__ uint32_t begin(uint32_t) const { return 0u; }
     __device__ __forceinline__ void store_part(uint32_t i, int part, float4 a, float4 b, float2 cdf, float2 coef) const {
         guide_store_part(mix + static_cast<size_t>(i) * kGuideMixFloat4, part, a, b, cdf, coef);
     }

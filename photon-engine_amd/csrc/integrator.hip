@@ -690,6 +690,8 @@ struct RenderQueries {
     uint32_t seg_cap;
     float4* mix;
     uint32_t guided_bounces;  // entries of paths at this depth or deeper are not guided
+    const uint32_t* bases;    // segment counters of the first camera entry (k_regen's kCtrBase)
+    bool camera_only;         // only camera entries are guided: a segment's queries start at its camera entries
     __device__ __forceinline__ bool fetch(uint32_t i, V3& p, V3& d) const {
         const float4 h = hits[i];
         if (__float_as_int(h.y) < 0) return false;
@@ -702,6 +704,7 @@ struct RenderQueries {
         return true;
     }
     __device__ __forceinline__ QueueSegs segs(uint32_t) const { return QueueSegs{counts, 0u, seg_cap, kQueueSegs}; }
+    __device__ __forceinline__ uint32_t begin(uint32_t k) const { return camera_only ? bases[k * kSegStride] : 0u; }
     __device__ __forceinline__ void store_part(uint32_t i, int part, float4 a, float4 b, float2 cdf, float2 coef) const {
         guide_store_part(mix + static_cast<size_t>(i) * kGuideMixFloat4, part, a, b, cdf, coef);
     }
@@ -987,8 +990,10 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     }
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
                     // Guided: the slot's mixture was inferred (vertices below depth guided_bounces).
+                    // With one guided bounce only camera entries were inferred: a depth-0 continuation (past a
+                    // camera-invisible light or a thin wall) samples the BSDF alone.
                     const bool guided = kGuide && static_cast<int>(depth) < lp.guided_bounces && lp.guide_fraction > 0.0f &&
-                                        !is_delta && !exit_vertex;
+                                        !is_delta && !exit_vertex && (lp.guided_bounces > 1 || cam_entry);
                     const float alpha = lp.guide_fraction;
                     // The slot's mixture is read where it is used (L1/L2), not held in registers across the NEE loop.
                     const float4* gmix = pb.mix + static_cast<size_t>(i) * kGuideMixFloat4;
@@ -2131,7 +2136,8 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             if (guide && lp.guide_fraction > 0.0f) {
                 wf->prof.begin(kKGuide, s);
                 const RenderQueries q{pb.ro[cur], pb.rd[cur], pb.hits, seg_counter(pb.ctr, kCtrCur, 0), pb.seg_cap, pb.mix,
-                                      static_cast<uint32_t>(std::max(0, lp.guided_bounces))};
+                                      static_cast<uint32_t>(std::max(0, lp.guided_bounces)),
+                                      seg_counter(pb.ctr, kCtrBase, 0), lp.guided_bounces <= 1};
                 GuideView gv = guide->view(lp.guide_fraction);
                 gv.inferred = pb.stats + 7;
                 hipLaunchKernelGGL(k_guide_infer<RenderQueries>, dim3(std::max(16, dev->cu_count * 2)),

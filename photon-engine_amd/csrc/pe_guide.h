@@ -271,7 +271,9 @@ __global__ void __launch_bounds__(kInferBlock) __attribute__((amdgpu_waves_per_e
     const int wave = threadIdx.x >> 6, l = __lane_id(), g = l >> 4, c = l & 15;
     const QueueSegs qs = q.segs(n);
     const SegShare sh = seg_share(blockIdx.x * (blockDim.x >> 6) + wave, gridDim.x * (blockDim.x >> 6), qs.segs);
-    const uint32_t nk = qs.count(sh.seg), base = sh.seg * qs.seg_cap;
+    // The segment's queries are items [first, count) (a source may know that the rest are never guided).
+    const uint32_t first = q.begin(sh.seg), base = sh.seg * qs.seg_cap + first;
+    const uint32_t nk = qs.count(sh.seg) > first ? qs.count(sh.seg) - first : 0u;
     const uint32_t tiles = (nk + kInferQueries - 1u) / kInferQueries;
     uint32_t inferred = 0;  // live queries of this wave's tiles, added to the counter once at the end
     for (uint32_t tile = sh.rank; tile < tiles; tile += sh.workers) {
