@@ -50,6 +50,10 @@ constexpr bool kNodeTile = PE_NODE_TILE != 0;  // traversal reads the top of the
 #define PE_MAT_BUCKETS 4
 #endif
 constexpr uint32_t kMatBuckets = PE_MAT_BUCKETS;  // shade's sort keys for hits (1: hits / misses only)
+#ifndef PE_SHADE_STAGE
+#define PE_SHADE_STAGE 1
+#endif
+constexpr bool kShadeStage = PE_SHADE_STAGE != 0;  // shade loads a round's entries before sorting (LDS staging)
 static_assert(kMatBuckets == 1 || kMatBuckets == static_cast<uint32_t>(kSortClasses), "one key per shading class");
 constexpr uint64_t kAutoWindowSamples = uint64_t{1} << 28;  // default result buffer: 4 GB
 constexpr uint64_t kPassWindowSamples = uint64_t{1} << 23;  // window limit with pass films (26 records a sample
@@ -765,16 +769,54 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
     constexpr uint32_t kKeys = kMatBuckets + 1, kNoKey = 0xFFFFFFFFu;
     __shared__ uint32_t s_slot[kBlock];
     __shared__ uint32_t s_count[kKeys][kBlock / 64];
+    // Staging (kShadeStage): each thread loads the entry of its own, unsorted slot (the slot's ray and hit,
+    // coalesced, then the path's state by path id: throughput, radiance, ids, last point and normal, and the
+    // first held-back NEE record, or the ext column in the transport and guide variants) into LDS before the
+    // sort; after the sort a thread reads the entry it shades from LDS where it uses it. The loads are issued
+    // together with the sort key's, so the sorted shading starts one memory round trip earlier. Two buffers
+    // (rounds alternate), 9 float4 a slot: 72 KB a block, two blocks a CU.
+    // Only the plain variant stages: the others hold more state across the vertex and spill more with it
+    // (measured on the lights + media studio: 3% slower staged, the product sphere 2.5% faster).
+    constexpr bool kStage = kShadeStage && !kTransport && !kGuide && !kPasses;
+    constexpr int kStageVecs = kStage ? 9 : 1;
+    constexpr bool kStagePend = !(kTransport || kGuide);  // vector 8 holds pend[pid * S], else ext
+    __shared__ float4 s_stage[2][kStageVecs][kBlock];
+    uint32_t round = 0;
     const uint32_t wave = threadIdx.x >> 6;
     const uint64_t below = (uint64_t{1} << __lane_id()) - 1u;
     // The loop bound is uniform over the block, so every lane takes part in the barriers and queue appends.
     const bool sort_hits = kMatBuckets > 1 && sc.sort_classes > 1;
     const uint32_t keys = sort_hits ? kKeys : 2u;  // hit classes, then the miss key
-    for (uint32_t base = sh.rank * blockDim.x; base < n; base += stride) {
+    for (uint32_t base = sh.rank * blockDim.x; base < n; base += stride, ++round) {
         const uint32_t mine = seg_base + base + threadIdx.x;
+        float4 (*stage)[kBlock] = s_stage[round & 1u];
         uint32_t key = kNoKey;
         if (base + threadIdx.x < n) {
-            const int tri = __float_as_int(pb.hits[mine].y);
+            const float4 h_mine = pb.hits[mine];
+            const int tri = __float_as_int(h_mine.y);
+            if constexpr (kStage) {
+                const float4 o4 = ro[mine], d4 = rd[mine];
+                stage[0][threadIdx.x] = o4;
+                stage[1][threadIdx.x] = d4;
+                stage[2][threadIdx.x] = h_mine;
+                if (!(__float_as_uint(d4.w) & kCamEntry)) {
+                    const uint32_t p = __float_as_uint(o4.w);
+                    stage[3][threadIdx.x] = pb.thr[p];
+                    stage[4][threadIdx.x] = pb.L[p];
+                    const uint4 id4 = pb.ids[p];
+                    stage[5][threadIdx.x] = make_float4(__uint_as_float(id4.x), __uint_as_float(id4.y),
+                                                        __uint_as_float(id4.z), __uint_as_float(id4.w));
+                    stage[6][threadIdx.x] = pb.last_p[p];
+                    stage[7][threadIdx.x] = pb.last_n[p];
+                    if constexpr (kStagePend) {
+                        stage[8][threadIdx.x] = pb.pend[static_cast<size_t>(p) * S];
+                    } else {
+                        const uint4 e4 = pb.ext[p];
+                        stage[8][threadIdx.x] = make_float4(__uint_as_float(e4.x), __uint_as_float(e4.y),
+                                                            __uint_as_float(e4.z), __uint_as_float(e4.w));
+                    }
+                }
+            }
             if (tri >= 0)
                 key = sort_hits ? static_cast<uint32_t>(sc.materials[sc.shade[tri].material].sort_class) : 0u;
             else
@@ -794,19 +836,20 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 total += c;
                 if (b < key || (b == key && k < wave)) pos += c;
             }
-        if (key != kNoKey) s_slot[pos + rank] = mine;
+        if (key != kNoKey) s_slot[pos + rank] = kStage ? threadIdx.x : mine;
         __syncthreads();
         const bool active = threadIdx.x < total;
-        const uint32_t i = active ? s_slot[threadIdx.x] : 0u;
-        __syncthreads();  // s_slot and s_count are rewritten by the next round
+        const uint32_t src = active ? s_slot[threadIdx.x] : 0u;  // staging: the thread that loaded the entry
+        const uint32_t i = kStage ? seg_base + base + src : src;
+        __syncthreads();  // s_slot and s_count are rewritten by the next round (the staging two rounds on)
         bool extend = false;
         bool ended = false;     // the path ended at this entry
         bool deferred = false;  // ... with a shadow ray of its last vertex still to be traced
         uint32_t pid = 0, new_depth_tag = 0;
         V3 new_o, new_d;
         if (active) {
-            const float4 o4 = ro[i];
-            const float4 d4 = rd[i];
+            const float4 o4 = kStage ? stage[0][src] : ro[i];
+            const float4 d4 = kStage ? stage[1][src] : rd[i];
             pid = __float_as_uint(o4.w);
             const V3 dir = mk(d4.x, d4.y, d4.z);
             // A camera entry starts its path here (k_regen made its ray); its state is the camera's, not read.
@@ -820,19 +863,33 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             if (cam_entry) {
                 ids = camera_ids(lp, __float_as_uint(d4.w) & ~kCamEntry);
             } else {
-                const float4 t4 = pb.thr[pid];
+                const float4 t4 = kStage ? stage[3][src] : pb.thr[pid];
                 thr = mk(t4.x, t4.y, t4.z);
                 last_pdf = t4.w;
-                const float4 L4 = pb.L[pid];
+                const float4 L4 = kStage ? stage[4][src] : pb.L[pid];
                 L = mk(L4.x, L4.y, L4.z);
-                ids = pb.ids[pid];
-                if constexpr (kTransport || kGuide) ext = pb.ext[pid];
-                lp4 = pb.last_p[pid];
-                ln4 = pb.last_n[pid];
+                if constexpr (kStage) {
+                    const float4 id4 = stage[5][src];
+                    ids = make_uint4(__float_as_uint(id4.x), __float_as_uint(id4.y), __float_as_uint(id4.z),
+                                     __float_as_uint(id4.w));
+                } else {
+                    ids = pb.ids[pid];
+                }
+                if constexpr (kTransport || kGuide) {
+                    if constexpr (kStage) {
+                        const float4 e4 = stage[8][src];
+                        ext = make_uint4(__float_as_uint(e4.x), __float_as_uint(e4.y), __float_as_uint(e4.z),
+                                         __float_as_uint(e4.w));
+                    } else {
+                        ext = pb.ext[pid];
+                    }
+                }
+                lp4 = kStage ? stage[6][src] : pb.last_p[pid];
+                ln4 = kStage ? stage[7][src] : pb.last_n[pid];
                 // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
                 for (int s = 0; s < S; ++s) {
                     const size_t k = static_cast<size_t>(pid) * S + s;
-                    const float4 p = pb.pend[k];
+                    const float4 p = kStage && kStagePend && s == 0 ? stage[8][src] : pb.pend[k];
                     L += mk(p.x, p.y, p.z);
                     if (kPasses) credit_nee(pb, lp, pid, ids.x, k, mk(p.x, p.y, p.z));
                     pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -851,7 +908,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 pb.grec[static_cast<size_t>(pid) * 8 + 4 * r + 3] = make_float4(L.x, L.y, L.z, 1.0f);
                 ext.w &= ~kExtRecOpen;
             }
-            const float4 h = pb.hits[i];
+            const float4 h = kStage ? stage[2][src] : pb.hits[i];
             const int tri = __float_as_int(h.y);
             const V3 last_p = mk(lp4.x, lp4.y, lp4.z), last_n = mk(ln4.x, ln4.y, ln4.z);
             Surface si;
