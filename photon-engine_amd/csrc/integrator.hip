@@ -1047,10 +1047,13 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     }
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
                     // Guided: the slot's mixture was inferred (vertices below depth guided_bounces).
-                    // With one guided bounce only camera entries were inferred: a depth-0 continuation (past a
-                    // camera-invisible light or a thin wall) samples the BSDF alone.
+                    // Guided vertices as the reference picks them (path_tracer.cpp:624-627): not delta, not a
+                    // volume boundary (whose refraction moves the medium stack), not a walk's exit. With one guided
+                    // bounce only camera entries were inferred: a depth-0 continuation (past a camera-invisible
+                    // light or a thin wall) samples the BSDF alone.
                     const bool guided = kGuide && static_cast<int>(depth) < lp.guided_bounces && lp.guide_fraction > 0.0f &&
-                                        !is_delta && !exit_vertex && (lp.guided_bounces > 1 || cam_entry);
+                                        !is_delta && !(m.flags & kMatFlagVolume) && !exit_vertex &&
+                                        (lp.guided_bounces > 1 || cam_entry);
                     const float alpha = lp.guide_fraction;
                     // The slot's mixture is read where it is used (L1/L2), not held in registers across the NEE loop.
                     const float4* gmix = pb.mix + static_cast<size_t>(i) * kGuideMixFloat4;

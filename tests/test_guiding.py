@@ -241,3 +241,40 @@ def test_guiding_reduces_error_on_the_slit_room(dev):
     plain = _render(dev, gs, view, 128)
     e_g, e_p = relmse(guided, ref), relmse(plain, ref)
     assert e_g < 0.95 * e_p, (e_g, e_p)
+
+
+@pytest.mark.gpu
+def test_guided_render_is_unbiased_through_subsurface(dev):
+    """A guided vertex takes the BSDF branch with probability 1 - alpha: a subsurface entry, which the mixture
+    cannot produce, must carry weight / (1 - alpha) (path_tracer.cpp:721-726). Without it the light through the
+    block would come out scaled by 1 - alpha (half). Guided and plain renders of a close view of a subsurface
+    block (the rough glass sphere beside it is a volume boundary, which is not guided) must agree in their mean
+    to a few per cent."""
+    from photon_hip.guide import render_guided
+    from photon_hip.scene import AreaLight, Mesh, Scene, dielectric, disney, lambert
+
+    sc = Scene()
+    up = np.array([0.0, 1.0, 0.0], np.float32)
+    sc.add_shape(Mesh(np.array([[-4, 0, -4], [4, 0, -4], [4, 0, 4], [-4, 0, 4]], np.float32),
+                      np.array([[0, 2, 1], [0, 3, 2]], np.uint32), np.stack([up] * 4)),
+                 sc.add_material(lambert((0.6, 0.6, 0.6))))
+    sc.add_shape(configs.box_mesh((0.3, 0.002, -0.25), (0.8, 0.5, 0.25)),
+                 sc.add_material(disney((0.85, 0.55, 0.4), 0.0, 0.4, 0.5, subsurface=1.0, subsurface_radius=(2, 3, 4),
+                                        subsurface_scale=1e-3)))
+    sc.add_shape(configs.icosphere(2, 0.15, (0.15, 0.15, 0.2)),
+                 sc.add_material(dielectric(1.45, tint=(1.0, 0.97, 0.9), roughness=0.25)))
+    sc.add_light(AreaLight((-0.6, 2.5, -0.2), (1.2, 0.0, 0.0), (0.0, 0.0, 0.8), (12.0, 12.0, 12.0)))
+    view = configs.View((0.55, 0.6, 1.6), (0.55, 0.25, 0.0), (0.0, 1.0, 0.0), 40.0, 64, 48, 16, 8)
+    gs = dev.upload(sc)
+    cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=16)
+    g = _guide(dev, gs, seed=5, batch=16384, steps_per_launch=4, train_launches=1, guided_bounces=3)
+    guided, plain = dev.film(view.width, view.height), dev.film(view.width, view.height)
+    render_guided(dev, gs, g, cam, st, dev.film(view.width, view.height), 1 << 20, 16)   # trains: guide on
+    assert g.stats()["steps"] > 0
+    render_guided(dev, gs, g, cam, st, guided, 0, 512)
+    render(dev, gs, cam, st, plain, 1 << 21, 512)
+    mg, mp = guided.mean(), plain.mean()
+    shift = float(mg.mean() / mp.mean() - 1.0)
+    print(f"guided / plain mean through subsurface: {1 + shift:.4f}", flush=True)
+    assert abs(shift) < 0.03, shift
