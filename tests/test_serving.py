@@ -5,7 +5,8 @@ render_session.cpp:69-101) run for many sessions at once.
 What is checked: a job's film is bit for bit the same alone and in a mix of concurrent jobs, and the same as
 pe_render over the job's chunk boundaries; admission waits for device memory (films and resident scenes) and parks idle scenes in host memory to make
 room, restoring them bit for bit; a short job's preview overtakes a long job's later chunks; a scene cannot be
-released while jobs reference it; cancel, release and the error paths.
+released while jobs reference it, and scene updates wait for the chunks rendering it; cancel, release and the
+error paths.
 """
 from __future__ import annotations
 
@@ -209,6 +210,46 @@ def test_idle_scene_is_parked_for_room_and_restored_bit_for_bit(dev):
         assert (film_b.download()[1] == 8).all()
     finally:
         srv.close()
+
+
+@pytest.mark.gpu
+def test_scene_updates_wait_for_running_chunks(dev):
+    """pe_scene_update may be called while server lanes render the scene: it takes the scene's lock, which every
+    chunk holds shared, so it waits for the chunks in progress and no chunk starts on half-rebuilt buffers (the
+    Full rebuild frees the old ones). The job still finishes every sample with finite values, and afterwards the
+    scene renders bit for bit what a fresh upload of the last description renders."""
+    import threading
+
+    sc, view = configs.box_icosphere(level=3, width=48, height=40, max_bounces=6)
+    gs = dev.upload(sc)
+    srv = Server(dev, slots=2, chunk_spp=2, max_chunk_paths=CHUNK_PATHS)
+    try:
+        jobs = [_submit(srv, gs, view, spp=96), _submit(srv, gs, view, spp=96)]
+        while srv.poll(jobs[0]).samples_done == 0:          # a lane is rendering the scene
+            threading.Event().wait(0.001)
+        for m in sc.materials[:3]:
+            m.base_color[:] = [0.8, 0.2, 0.1]
+        assert gs.update(sc, capi.PE_UPDATE_MATERIALS) == capi.PE_OK
+        sc.lights[0].radiance[:] = [4.0, 4.0, 9.0]
+        assert gs.update(sc, capi.PE_UPDATE_LIGHTS) == capi.PE_OK
+        moved, _ = configs.box_icosphere(level=3, width=48, height=40, max_bounces=6)
+        moved.shapes[6].mesh.positions = moved.shapes[6].mesh.positions + np.float32([0.15, 0.1, 0.0])
+        assert gs.update(moved, capi.PE_UPDATE_FULL) == capi.PE_OK
+        assert capi.load().pe_scene_release(gs.h) == capi.PE_ERR_INVALID   # jobs still reference it
+        for j in jobs:
+            st = srv.wait(j, 120000)
+            assert st.state == capi.PE_JOB_DONE and st.samples_done == 96, st
+            s, c, *_ = srv.download(j)
+            assert (c == 96).all() and np.isfinite(s).all()
+            srv.release(j)
+    finally:
+        srv.close()
+    film = dev.film(view.width, view.height)
+    st = settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=4)
+    render(dev, gs, _cam(view), st, film, 0, 4)
+    fresh = dev.film(view.width, view.height)
+    render(dev, dev.upload(moved), _cam(view), st, fresh, 0, 4)
+    np.testing.assert_array_equal(film.download()[0], fresh.download()[0])
 
 
 @pytest.mark.gpu
