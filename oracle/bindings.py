@@ -94,6 +94,8 @@ _REF_SIGS = {
     "render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                 C.c_int, _F, _F, _F]),
     "scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p), _P(C.c_char_p)]),
+    "scene_build_instanced": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p),
+                                    _P(C.c_char_p), _F, _P(C.c_uint8)]),
     "crypto_hash": (C.c_int, [C.c_size_t, _P(C.c_char_p), _U, _F]),
     "scene_light_groups": (C.c_int, [_VP, _I, _I, C.c_char_p, C.c_size_t]),
     "trace_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -109,6 +111,31 @@ def _scene_names(scene):
     mats = (C.c_char_p * max(1, len(scene.materials)))(
         *[scene.material_names.get(i, "").encode() for i in range(len(scene.materials))])
     return groups, env, shapes, mats
+
+
+def _instancing(scene, instances, visibility):
+    """Per-mesh local-to-world matrices (16 floats row-major, zeros where a mesh is not instanced) and visibility
+    masks for ref_scene_build_instanced, from {mesh index: 4x4 matrix} and {mesh index: mask}."""
+    n = max(1, len(scene.shapes))
+    xf = np.zeros((n, 16), np.float32)
+    for i, m in (instances or {}).items():
+        xf[i] = np.asarray(m, np.float32).reshape(16)
+    vis = np.full(n, 0x1F, np.uint8)
+    for i, v in (visibility or {}).items():
+        vis[i] = v
+    return xf, vis
+
+
+def _build(lib, prefix, scene, instances, visibility, keep):
+    desc, k = scene.pack()
+    keep += k
+    args = _scene_names(scene)
+    keep += list(args)
+    if instances is None and visibility is None:
+        return getattr(lib, prefix + "scene_build_groups")(C.byref(desc), *args)
+    xf, vis = _instancing(scene, instances, visibility)
+    keep += [xf, vis]
+    return getattr(lib, prefix + "scene_build_instanced")(C.byref(desc), *args, _fp(xf), capi.ptr(vis, C.c_uint8))
 
 
 def crypto_hash_reference(names):
@@ -266,14 +293,14 @@ class OracleScene:
 class RefScene:
     """engine::Scene built by the reference's own code from the same descriptor."""
 
-    def __init__(self, scene, strict: bool = False):
+    def __init__(self, scene, strict: bool = False, instances=None, visibility=None):
+        """instances: {mesh index: 4x4 local-to-world matrix} makes those meshes MeshInstances of the (local)
+        mesh; visibility: {mesh index: Shape visibility mask}."""
         self._lib = reference(strict)
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
-        desc, self._keep = scene.pack()
-        args = _scene_names(scene)
-        self._keep += list(args)
-        self.h = self._lib.ref_scene_build_groups(C.byref(desc), *args)
+        self._keep = []
+        self.h = _build(self._lib, "ref_", scene, instances, visibility, self._keep)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -482,6 +509,8 @@ _HIPDEV_SIGS = {
     "ref_scene_build": (_VP, [_P(capi.SceneDesc)]),
     "ref_scene_build_groups": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p),
                                      _P(C.c_char_p)]),
+    "ref_scene_build_instanced": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p),
+                                        _P(C.c_char_p), _F, _P(C.c_uint8)]),
     "hipdev_session_render_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                                C.c_int, C.c_int, _F, _F, _U, _I, _F, C.c_char_p, C.c_int]),
     "ref_scene_free": (None, [_VP]),
@@ -512,14 +541,13 @@ def hip_device_harness() -> _Lib | None:
 class HipDeviceSession:
     """A reference Scene built from the descriptor, rendered by HipDevice under RenderSession (Final mode)."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, instances=None, visibility=None):
+        """instances / visibility as RefScene takes them."""
         self._lib = hip_device_harness()
         if self._lib is None:
             raise FileNotFoundError("oracle/_ref/libphoton_hipdevice.so was not built (the reference tree is absent)")
-        desc, self._keep = scene.pack()
-        args = _scene_names(scene)
-        self._keep += list(args)
-        self.h = self._lib.ref_scene_build_groups(C.byref(desc), *args)
+        self._keep = []
+        self.h = _build(self._lib, "ref_", scene, instances, visibility, self._keep)
 
     def __del__(self):
         if getattr(self, "h", None):

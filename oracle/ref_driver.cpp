@@ -15,6 +15,7 @@
 #include "core/image/tone_mapping.h"
 #include "core/color/transfer.h"
 #include "geometry/mesh.h"
+#include "geometry/mesh_instance.h"
 #include "integrators/path_tracer.h"
 #include "lights/area_light.h"
 #include "lights/environment_light.h"
@@ -123,9 +124,13 @@ REF_API int ref_sobol_batch(uint64_t seed, size_t n, const int32_t* px, const in
 // The scene of a descriptor; with names, area light i's group name (Light::setLightGroup, "" = none; the
 // triangle lights Scene::addShape makes for emissive meshes have none) and the environment's, which
 // Scene::buildAccelerator resolves into group indices (scene.cpp:144-173). Meshes are tagged with the
-// descriptor's node uids (Scene::tagShape).
+// descriptor's node uids (Scene::tagShape). With `xforms` (16 floats a mesh, row-major local-to-world; a
+// matrix whose last entry is 0 means none), a mesh becomes what SceneGraph::compileNode makes of a mesh node
+// (scene_graph.cpp:189-202): a MeshInstance of the descriptor's (local) mesh under the matrix; `vis` gives
+// each shape's visibility mask (Shape::setVisibilityMask).
 void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, const char* envGroupName,
-               const char* const* shapeNames, const char* const* materialNames) {
+               const char* const* shapeNames, const char* const* materialNames, const float* xforms = nullptr,
+               const uint8_t* vis = nullptr) {
     Scene::setPreferOwnBvh(true);
     auto rs = new RefScene();
     rs->scene = std::make_shared<Scene>();
@@ -157,7 +162,18 @@ void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, co
             for (uint32_t v = 0; v < m.num_vertices; ++v) uv[v] = Vec2f(m.uvs[2 * v], m.uvs[2 * v + 1]);
         }
         std::vector<uint32_t> idx(m.indices, m.indices + 3 * static_cast<size_t>(m.num_triangles));
-        auto shape = std::make_shared<TriangleMesh>(pos, nrm, uv, idx, rs->materials[static_cast<size_t>(m.material)].get());
+        const Material* mat = rs->materials[static_cast<size_t>(m.material)].get();
+        std::shared_ptr<Shape> shape;
+        const float* X = xforms ? xforms + 16 * static_cast<size_t>(i) : nullptr;
+        if (X && X[15] != 0.0f) {
+            const Mat4f M(X[0], X[1], X[2], X[3], X[4], X[5], X[6], X[7], X[8], X[9], X[10], X[11], X[12], X[13],
+                          X[14], X[15]);
+            shape = std::make_shared<MeshInstance>(std::make_shared<TriangleMesh>(pos, nrm, uv, idx, mat), Transform(M),
+                                                   mat);
+        } else {
+            shape = std::make_shared<TriangleMesh>(pos, nrm, uv, idx, mat);
+        }
+        if (vis) shape->setVisibilityMask(vis[i]);
         if (desc->node_uids && desc->node_uids[i] != 0) rs->scene->tagShape(shape.get(), desc->node_uids[i]);
         if (shapeNames && shapeNames[i] && shapeNames[i][0]) rs->scene->setShapeName(shape.get(), shapeNames[i]);
         rs->scene->addShape(shape);
@@ -196,6 +212,13 @@ REF_API void* ref_scene_build_groups(const pe_scene_desc* desc, const char* cons
                                      const char* envGroupName, const char* const* shapeNames,
                                      const char* const* materialNames) {
     return buildRef(desc, lightGroupNames, envGroupName, shapeNames, materialNames);
+}
+// As ref_scene_build_groups, with meshes made MeshInstances under `xforms` and visibility masks `vis` (either
+// may be NULL; see buildRef).
+REF_API void* ref_scene_build_instanced(const pe_scene_desc* desc, const char* const* lightGroupNames,
+                                        const char* envGroupName, const char* const* shapeNames,
+                                        const char* const* materialNames, const float* xforms, const uint8_t* vis) {
+    return buildRef(desc, lightGroupNames, envGroupName, shapeNames, materialNames, xforms, vis);
 }
 
 // cryptoHashName and cryptoHashToFloat (cryptomatte.cpp:78-112) of n names.

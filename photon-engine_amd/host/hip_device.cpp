@@ -7,7 +7,8 @@
 // device merges its batches.
 //
 // What the HIP path does not implement is refused, never approximated: setScene leaves the device without a
-// scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle, lights other
+// scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle / static
+// MeshInstance, visibility masks other than all rays or all but camera rays, shadow catchers, lights other
 // than AreaLight / TriangleLight, textured or flaked Disney materials and materials other than Disney /
 // Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO and
 // motion blur. Path guiding uses the library's learned guide in place of the reference's SD-tree.
@@ -20,6 +21,7 @@
 #include "engine/renderer.h"
 #include "engine/scene.h"
 #include "geometry/mesh.h"
+#include "geometry/mesh_instance.h"
 #include "geometry/triangle.h"
 #include "lights/area_light.h"
 #include "lights/environment_light.h"
@@ -201,7 +203,67 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
         probe.hitObject = shape.get();
         probe.primIndex = 0;
         pe_mesh_desc md{};
-        if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape.get())) {
+        // Ray visibility: every ray, or every ray but camera rays (what a camera-invisible area light's quads
+        // carry, scene.cpp:93-96); the per-bounce masks of path_tracer.cpp:326-467 are not implemented.
+        const uint8_t vis = shape->visibilityMask();
+        if (vis != VisAll && vis != static_cast<uint8_t>(VisAll & ~VisCamera)) {
+            err = "shape visibility masks other than all rays or all but camera rays are not implemented on the "
+                  "HIP device";
+            return false;
+        }
+        if (scene.isShadowCatcher(probe)) {
+            err = "shadow catchers are not implemented on the HIP device";
+            return false;
+        }
+        if (vis != VisAll) md.flags |= PE_SHAPE_CAMERA_INVISIBLE;
+        if (auto* inst = dynamic_cast<const MeshInstance*>(shape.get())) {
+            // A static instance (SceneGraph::compileNode, scene_graph.cpp:189-202, makes one per mesh node) is
+            // baked into world-space triangles, as the Vulkan device bakes it (vulkan_device.cpp:1409-1460):
+            // points through transformPoint, normals through transformNormal and normalised. A mirroring
+            // transform (negative determinant) turns the winding, so two indices of every triangle swap and the
+            // edge cross product keeps pointing along the transformed outward normal, which is the geometric
+            // normal MeshInstance reports (mesh_instance.cpp:57-70). The instance's emissive triangles are the
+            // world-space TriangleLights Scene::addShape made from the same transformPoint values.
+            if (inst->hasMotion()) {
+                err = "a MeshInstance with a motion transform is not implemented on the HIP device";
+                return false;
+            }
+            const TriangleMesh* mesh = inst->mesh().get();
+            const Transform& xf = inst->transform0();
+            std::vector<float> pos, nrm, uv;
+            std::vector<uint32_t> idx;
+            if (mesh) {
+                for (const Vec3f& v : mesh->positions()) {
+                    const Vec3f w = xf.transformPoint(v);
+                    pos.insert(pos.end(), {w.x, w.y, w.z});
+                }
+                for (const Vec3f& v : mesh->normals()) {
+                    Vec3f n = xf.transformNormal(v);
+                    if (n.lengthSquared() > 0.0f) n = n.normalized();
+                    nrm.insert(nrm.end(), {n.x, n.y, n.z});
+                }
+                for (const Vec2f& v : mesh->uvs()) uv.insert(uv.end(), {v.x, v.y});
+                idx = mesh->indices();
+                const Mat4f& m = xf.matrix();
+                const float det = m(0, 0) * (m(1, 1) * m(2, 2) - m(1, 2) * m(2, 1)) -
+                                  m(0, 1) * (m(1, 0) * m(2, 2) - m(1, 2) * m(2, 0)) +
+                                  m(0, 2) * (m(1, 0) * m(2, 1) - m(1, 1) * m(2, 0));
+                if (det < 0.0f)
+                    for (size_t i = 0; i + 2 < idx.size(); i += 3) std::swap(idx[i + 1], idx[i + 2]);
+            }
+            if (!nrm.empty() && nrm.size() != pos.size()) nrm.clear();
+            if (!uv.empty() && uv.size() / 2 != pos.size() / 3) uv.clear();
+            p.indices.push_back(std::move(idx));
+            md.num_vertices = static_cast<uint32_t>(pos.size() / 3);
+            md.num_triangles = static_cast<uint32_t>(p.indices.back().size() / 3);
+            md.positions = p.keep(std::move(pos));
+            md.normals = nrm.empty() ? nullptr : p.keep(std::move(nrm));
+            md.uvs = uv.empty() ? nullptr : p.keep(std::move(uv));
+            md.indices = p.indices.back().data();
+            md.material = material(inst->material());
+            md.emitter_base = scene.emitterIndex(probe);
+            md.standalone = 0;
+        } else if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape.get())) {
             std::vector<float> pos, nrm, uv;
             for (const Vec3f& v : mesh->positions()) pos.insert(pos.end(), {v.x, v.y, v.z});
             for (const Vec3f& v : mesh->normals()) nrm.insert(nrm.end(), {v.x, v.y, v.z});
@@ -233,7 +295,7 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
             probe.primIndex = -1;
             md.emitter_base = scene.emitterIndex(probe);
             md.standalone = 1;
-            md.flags = scene.cameraInvisible(probe) ? PE_SHAPE_CAMERA_INVISIBLE : 0u;
+            if (scene.cameraInvisible(probe)) md.flags |= PE_SHAPE_CAMERA_INVISIBLE;
         } else {
             err = "shape type not implemented on the HIP device (TriangleMesh and Triangle are)";
             return false;
@@ -278,11 +340,14 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
     return true;
 }
 
-// The material of a shape as the cryptomatte lookup registers it (renderer.cpp:101-109).
-const Material* shapeMaterial(const Shape* shape) {
-    if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape)) return mesh->material();
-    if (auto* tri = dynamic_cast<const Triangle*>(shape)) return tri->material();
-    return nullptr;
+// The materials of a shape in the order the cryptomatte lookup registers them (renderer.cpp:101-109): an
+// instance's own (or its mesh's), then its mesh's.
+std::vector<const Material*> shapeMaterials(const Shape* shape) {
+    if (auto* inst = dynamic_cast<const MeshInstance*>(shape))
+        return {inst->material(), inst->mesh() ? inst->mesh()->material() : nullptr};
+    if (auto* mesh = dynamic_cast<const TriangleMesh*>(shape)) return {mesh->material()};
+    if (auto* tri = dynamic_cast<const Triangle*>(shape)) return {tri->material()};
+    return {};
 }
 
 // buildCryptoLookup (renderer.cpp:65-118) for the packed scene: names registered into the pass films'
@@ -313,7 +378,7 @@ void cryptoTables(const Scene& scene, const std::vector<const Material*>& materi
         else name = "shape_" + std::to_string(anonShape++);
         objectIds.push_back(passes.cryptoObject.registerName(name));
         seen[ptr] = true;
-        registerMat(shapeMaterial(shape.get()));
+        for (const Material* m : shapeMaterials(shape.get())) registerMat(m);
     }
     for (const auto& [ptr, name] : scene.shapeNames())
         if (ptr && !name.empty() && !seen.count(ptr)) passes.cryptoObject.registerName(name);
