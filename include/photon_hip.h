@@ -164,7 +164,19 @@ typedef struct pe_scene_desc {
     /* Per mesh: the scene-graph node that owns it (Scene::tagShape / nodeUidOf, the object-ID pass), or NULL
      * (0 for every mesh: untagged shapes). */
     const uint64_t* node_uids;
+    /* Scene::background() (engine/scene.h:24-28): what a camera path that leaves the scene before its first
+     * vertex sees (path_tracer.cpp:483-495; it may have passed thin walls and camera-invisible lights):
+     * PE_BACKGROUND_ENVIRONMENT (0, the environment light with MIS, as every later miss), PE_BACKGROUND_COLOR
+     * (background_color, credited to the Background pass and the environment's light group) or
+     * PE_BACKGROUND_TRANSPARENT (nothing; a camera ray that passed thin walls gives the alpha AOV their
+     * coverage, 1 - luminance of its throughput). A zeroed descriptor has the environment background. */
+    int32_t background_mode;
+    float background_color[3];
 } pe_scene_desc;
+
+#define PE_BACKGROUND_ENVIRONMENT 0
+#define PE_BACKGROUND_COLOR 1
+#define PE_BACKGROUND_TRANSPARENT 2
 
 #define PE_MAX_LIGHT_GROUPS 16  /* LightPasses::kMaxLightGroups (integrators/path_tracer.h:27) */
 

@@ -64,6 +64,8 @@ Scene* buildScene(const pe_scene_desc& d) {
     }
     s->nodeUid.assign(static_cast<size_t>(d.num_meshes), 0);
     if (d.node_uids) s->nodeUid.assign(d.node_uids, d.node_uids + d.num_meshes);
+    s->bgMode = d.background_mode;
+    s->bgColor = V3(d.background_color[0], d.background_color[1], d.background_color[2]);
     buildBvh(*s);
     return s;
 }

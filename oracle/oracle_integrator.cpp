@@ -1177,6 +1177,7 @@ struct MediumStack {  // path_tracer.cpp:63-83 (entries are material indices; -1
 struct PrimaryHit {
     bool hit = false;
     SI isect;
+    float coverage = -1.0f;  // >= 0: the camera ray passed thin walls to a transparent background (path_tracer.h:54-56)
 };
 
 // LightPasses (path_tracer.h:21-30): the eight light passes (LightPass order) and the light groups.
@@ -1295,6 +1296,19 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
             }
             wo = -cur.d;
             if (!hit) {  // :482-514
+                // Seen straight from the camera (past thin walls and hidden lights), a background other than
+                // the environment (:487-495). `primary` is always given then: tracePixel asks for it whenever
+                // the background is not the environment (renderer.cpp:158-161).
+                if (depth == 0 && scene.bgMode != PE_BACKGROUND_ENVIRONMENT) {
+                    if (scene.bgMode == PE_BACKGROUND_COLOR) {
+                        const V3 added = accumulate(throughput * scene.bgColor);
+                        credit(kBackground, added);
+                        creditGroup(scene.envLightGroup, added);
+                    } else if (passThroughs > 0 && primary) {
+                        primary->coverage = std::max(0.0f, 1.0f - luminance(throughput));
+                    }
+                    break;
+                }
                 if (it.hasEnv) {
                     const V3 Le = it.env.eval(cur.d);
                     float weight = 1.0f;
@@ -1543,10 +1557,11 @@ struct Tracer {
     int width, height;
     uint64_t seed;
 
-    // tracePixel (renderer.cpp:139-162): filter offset (dim 0), lens sample (dim 1), Li; with `aov`, the AOVs of
-    // the environment-background branch (:175-205): the primary hit's clamped albedo (primaryAlbedo, :47-58:
-    // Disney baseColor, Lambert albedo, Dielectric tint), its shading normal and coverage 1, or on a miss the
-    // clamped environment radiance along the camera ray, a zero normal and coverage 0.
+    // tracePixel (renderer.cpp:139-162): filter offset (dim 0), lens sample (dim 1), Li; with `aov`, the AOVs
+    // (:175-205): the primary hit's clamped albedo (primaryAlbedo, :47-58: Disney baseColor, Lambert albedo,
+    // Dielectric tint), its shading normal and coverage 1 (the thin walls' coverage where the camera ray passed
+    // them to a transparent background), or on a miss the clamped Color background or environment radiance
+    // along the camera ray, a zero normal and coverage 0.
     V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* aov = nullptr) const {
         sampler.startPixel(x, y);
         sampler.startSample(s);
@@ -1574,7 +1589,9 @@ struct Tracer {
             const int m = primary.isect.material;
             if (m >= 0) albedo = clamp01(c3(scene.materials[static_cast<size_t>(m)].base_color));
             normal = primary.isect.normal;
-            alpha = 1.0f;
+            alpha = primary.coverage >= 0.0f ? primary.coverage : 1.0f;  // :184-189
+        } else if (scene.bgMode == PE_BACKGROUND_COLOR) {
+            albedo = clamp01(scene.bgColor);
         } else if (it.hasEnv) {
             albedo = clamp01(it.env.eval(ray.d));
         }

@@ -40,6 +40,9 @@ struct Scene {
     int envLightGroup = -1;
     int numLightGroups = 0;
     std::vector<uint64_t> nodeUid;  // per shape (mesh of the descriptor)
+    // Scene::background() (scene.h:24-28): PE_BACKGROUND_* and the Color mode's colour.
+    int bgMode = PE_BACKGROUND_ENVIRONMENT;
+    V3 bgColor{0.18f, 0.18f, 0.18f};
 
     // BVH
     std::vector<LinearNode> nodes;

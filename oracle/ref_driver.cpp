@@ -199,6 +199,12 @@ void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, co
         if (envGroupName) env->setLightGroup(envGroupName);
         rs->scene->setEnvironment(env);
     }
+    Background bg;
+    bg.mode = desc->background_mode == PE_BACKGROUND_COLOR         ? Background::Mode::Color
+              : desc->background_mode == PE_BACKGROUND_TRANSPARENT ? Background::Mode::Transparent
+                                                                   : Background::Mode::Environment;
+    bg.color = c3(desc->background_color);
+    rs->scene->setBackground(bg);
     rs->scene->buildAccelerator();
     const auto& shapes = rs->scene->shapes();
     for (size_t k = 0; k < shapes.size(); ++k) rs->shapeIndex[shapes[k].get()] = static_cast<int>(k);
@@ -392,7 +398,12 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
         Ray ray = cam.generateRay(sx, sy, lens);
         ray.coneWidth = cam.pixelConeWidth(height);
         ray.coneSpread = cam.pixelConeSpread(height);
-        const Color3f c = integrator.Li(ray, *rs->scene, sampler);
+        // tracePixel hands Li a primary-hit record whenever the background is not the environment
+        // (renderer.cpp:158-161), which is what lets the background branch run (path_tracer.cpp:487).
+        PathTracer::PrimaryHit primary;
+        const bool wantPrimary = rs->scene->background().mode != Background::Mode::Environment;
+        const Color3f c = wantPrimary ? integrator.Li(ray, *rs->scene, sampler, &primary, nullptr)
+                                      : integrator.Li(ray, *rs->scene, sampler);
         L[3 * i] = c.r; L[3 * i + 1] = c.g; L[3 * i + 2] = c.b;
     }
     return 0;

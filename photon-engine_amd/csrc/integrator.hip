@@ -175,6 +175,12 @@ constexpr uint32_t kMaxPassThroughs = 64;  // path_tracer.cpp:783
 constexpr uint32_t kMaxShadowQueries = 64;  // shadowTransmittance (path_tracer.cpp:180)
 
 __device__ __forceinline__ uint32_t ext_pass_throughs(uint32_t w) { return (w & kExtPassMask) >> kExtPassShift; }
+// The coverage of the thin walls a camera ray passed to a transparent background: 1 - the luminance of its
+// throughput (Color3f::luminance, spectrum.h:68-70), uncontracted as the reference computes it.
+__device__ __forceinline__ float wall_coverage(V3 t) {
+#pragma clang fp contract(off)
+    return fmaxf(0.0f, 1.0f - (0.212671f * t.x + 0.715160f * t.y + 0.072169f * t.z));
+}
 __device__ __forceinline__ uint32_t ext_walk_step(uint32_t w) { return (w & kExtWalkStepMask) >> kExtWalkStepShift; }
 __device__ __forceinline__ uint64_t ext_stack(const uint4& e) { return e.y | (static_cast<uint64_t>(e.z) << 32); }
 __device__ __forceinline__ void ext_set_stack(uint4& e, uint64_t st, uint32_t depth) {
@@ -956,10 +962,13 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     }
                 }
             } else if (tri < 0) {
-                // The camera ray's first segment misses: the AOVs of the background (renderer.cpp:196-204).
+                // The camera ray's first segment misses: the AOVs of the background (renderer.cpp:196-204), the
+                // Color background's colour or the environment.
                 if (lp.want_aov && camera && pb.aov_res[2 * static_cast<size_t>(g) + 1].w < 0.0f) {
                     pb.aov_res[2 * static_cast<size_t>(g)] =
-                        sc.has_env ? clamped4(env_eval(sc.env, dir), 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        sc.bg_mode == PE_BACKGROUND_COLOR
+                            ? clamped4(mk(sc.bg_color.x, sc.bg_color.y, sc.bg_color.z), 0.0f)
+                            : (sc.has_env ? clamped4(env_eval(sc.env, dir), 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f));
                     pb.aov_res[2 * static_cast<size_t>(g) + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
                 if (kPasses && camera) {  // no primary hit: the data passes stay zero (renderer.cpp:163)
@@ -970,7 +979,22 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 // environment with MIS against NEE (:482-514).
                 const int top = kTransport ? ext_media_top(ext) : -1;
                 const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
-                if (sc.has_env && !absorbed) {
+                if (absorbed) {
+                } else if (depth == 0 && sc.bg_mode != PE_BACKGROUND_ENVIRONMENT) {
+                    // A camera path leaving before its first vertex (past thin walls and hidden lights) sees the
+                    // background (path_tracer.cpp:487-495): the Color one credited to the Background pass and
+                    // the environment's group; the Transparent one nothing, and the thin walls it passed set the
+                    // sample's coverage.
+                    if (sc.bg_mode == PE_BACKGROUND_COLOR) {
+                        const V3 c = thr * mk(sc.bg_color.x, sc.bg_color.y, sc.bg_color.z);
+                        const V3 added = finite3(c) ? c : splat(0.0f);
+                        L += added;
+                        if (kPasses) credit_found(pb, lp, pid, g, depth, kPassBackground, sc.env_group, added);
+                    } else if (kTransport && lp.want_aov && ext_pass_throughs(ext.w) > 0) {
+                        float4* a = pb.aov_res + 2 * static_cast<size_t>(g);
+                        a->w = wall_coverage(thr);
+                    }
+                } else if (sc.has_env) {
                     const V3 added = miss_radiance(sc, lp, dir, thr, specular, last_pdf, last_p, last_n, diffuse);
                     L += added;
                     if (kPasses) credit_found(pb, lp, pid, g, depth, kPassBackground, sc.env_group, added);

@@ -163,6 +163,8 @@ struct SceneView {
     int32_t sort_classes;      // distinct GpuMaterial::sort_class values of the scene's materials
     int32_t num_groups;        // light groups (Scene::lightGroupNames().size()), 0 = none
     int32_t env_group;         // Scene::envLightGroupIndex
+    int32_t bg_mode;           // PE_BACKGROUND_*: what a camera path leaving before its first vertex sees
+    float3 bg_color;           // the Color background
     const uint64_t* node_uids; // per shape: Scene::nodeUidOf (object-ID pass)
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;

@@ -486,6 +486,10 @@ int prepare_lights(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
         set_error("scene: bad light list");
         return PE_ERR_INVALID;
     }
+    if (desc.background_mode < PE_BACKGROUND_ENVIRONMENT || desc.background_mode > PE_BACKGROUND_TRANSPARENT) {
+        set_error("scene: unknown background mode " + std::to_string(desc.background_mode));
+        return PE_ERR_INVALID;
+    }
     std::vector<GpuLight> lights(static_cast<size_t>(std::max(1, desc.num_lights)));
     for (int i = 0; i < desc.num_lights; ++i) {
         const pe_light_desc& L = desc.lights[i];
@@ -558,6 +562,9 @@ int prepare_lights(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
     v.num_top = lb.mode == kPickBvh ? static_cast<int32_t>(lb.top.size()) : 0;
     v.num_groups = grouped ? desc.num_light_groups : 0;
     v.env_group = grouped ? desc.env_light_group : -1;
+    // The background goes with the environment (a Lights update carries both).
+    v.bg_mode = desc.background_mode;
+    v.bg_color = make_float3(desc.background_color[0], desc.background_color[1], desc.background_color[2]);
     for (int k = 0; k < kMaxTopCandidates; ++k)
         v.top[k] = (lb.mode == kPickBvh && k < static_cast<int>(lb.top.size())) ? lb.top[static_cast<size_t>(k)] : 0;
     return PE_OK;

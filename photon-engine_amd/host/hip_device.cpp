@@ -68,6 +68,8 @@ struct PackedScene {
     int32_t numGroups = 0;
     std::vector<uint64_t> nodeUids;    // per shape: Scene::nodeUidOf
     std::vector<const Material*> materialPtrs;  // per packed material: the scene's material
+    int32_t backgroundMode = PE_BACKGROUND_ENVIRONMENT;
+    float backgroundColor[3] = {0.18f, 0.18f, 0.18f};
 
     const float* keep(std::vector<float> v) {
         floats.push_back(std::move(v));
@@ -86,6 +88,8 @@ struct PackedScene {
         d.env_light_group = envGroup;
         d.num_light_groups = lightGroups.empty() ? 0 : numGroups;
         d.node_uids = nodeUids.data();
+        d.background_mode = backgroundMode;
+        std::copy(backgroundColor, backgroundColor + 3, d.background_color);
         return d;
     }
 };
@@ -307,6 +311,13 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
         p.meshes.push_back(md);
         p.nodeUids.push_back(scene.nodeUidOf(probe));
     }
+
+    // Scene::background(): Environment, Color or Transparent (scene.h:24-28).
+    const Background& bg = scene.background();
+    p.backgroundMode = bg.mode == Background::Mode::Color         ? PE_BACKGROUND_COLOR
+                       : bg.mode == Background::Mode::Transparent ? PE_BACKGROUND_TRANSPARENT
+                                                                  : PE_BACKGROUND_ENVIRONMENT;
+    putc(p.backgroundColor, bg.color);
 
     if (const EnvironmentLight* env = scene.environment()) {
         p.hasEnv = true;

@@ -84,7 +84,12 @@ class SceneDesc(C.Structure):
                 ("lights", C.POINTER(LightDesc)), ("num_lights", C.c_int32),
                 ("environment", C.POINTER(EnvDesc)), ("light_groups", C.POINTER(C.c_int32)),
                 ("env_light_group", C.c_int32), ("num_light_groups", C.c_int32),
-                ("node_uids", C.POINTER(C.c_uint64))]
+                ("node_uids", C.POINTER(C.c_uint64)), ("background_mode", C.c_int32),
+                ("background_color", C.c_float * 3)]
+
+
+# Background::Mode (engine/scene.h:24-28)
+PE_BACKGROUND_ENVIRONMENT, PE_BACKGROUND_COLOR, PE_BACKGROUND_TRANSPARENT = 0, 1, 2
 
 
 PE_MAX_LIGHT_GROUPS = 16

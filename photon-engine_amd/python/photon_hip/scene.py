@@ -174,6 +174,9 @@ class Scene:
     environment: Environment | None = None
     light_group_of: list = field(default_factory=list)  # per light: its group name ("" = none)
     material_names: dict = field(default_factory=dict)  # material index -> Scene::setMaterialName name
+    # Scene::setBackground (engine/scene.h:24-28): capi.PE_BACKGROUND_* and the colour of the Color mode
+    background_mode: int = 0
+    background_color: tuple = (0.18, 0.18, 0.18)
 
     def add_material(self, m: capi.MaterialDesc, name: str = "") -> int:
         self.materials.append(m)
@@ -334,6 +337,8 @@ class Scene:
         uids = np.array([s.node_uid for s in self.shapes] + [0], dtype=np.uint64)
         keep.append(uids)
         desc.node_uids = capi.ptr(uids, C.c_uint64)
+        desc.background_mode = int(self.background_mode)
+        desc.background_color[:] = [float(x) for x in self.background_color]
         if self.environment is not None:
             e = self.environment
             env = capi.EnvDesc()
