@@ -97,6 +97,9 @@ _REF_SIGS = {
     "scene_build_instanced": (_VP, [_P(capi.SceneDesc), _P(C.c_char_p), C.c_char_p, _P(C.c_char_p),
                                     _P(C.c_char_p), _F, _P(C.c_uint8)]),
     "crypto_hash": (C.c_int, [C.c_size_t, _P(C.c_char_p), _U, _F]),
+    "mesh_encode": (C.c_size_t, [_F, _F, _F, C.c_uint32, _U, C.c_uint32, _P(C.c_uint8), C.c_size_t]),
+    "mesh_decode": (C.c_int, [_P(C.c_uint8), C.c_size_t, _F, _F, _U, _U]),
+    "temperature_color": (C.c_int, [C.c_size_t, _F, _F]),
     "scene_light_groups": (C.c_int, [_VP, _I, _I, C.c_char_p, C.c_size_t]),
     "trace_passes": (C.c_int, [_VP, _F, _F, _F, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                C.c_float, C.c_uint64, C.c_size_t, _I, _I, _I, _F, _F]),

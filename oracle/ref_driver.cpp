@@ -26,6 +26,7 @@
 #include "materials/mirror.h"
 #include "materials/microfacet.h"
 #include "samplers/sobol_sampler.h"
+#include "scene/mesh_binary.h"
 
 #include "photon_hip.h"
 
@@ -225,6 +226,63 @@ REF_API void* ref_scene_build_instanced(const pe_scene_desc* desc, const char* c
                                         const char* envGroupName, const char* const* shapeNames,
                                         const char* const* materialNames, const float* xforms, const uint8_t* vis) {
     return buildRef(desc, lightGroupNames, envGroupName, shapeNames, materialNames, xforms, vis);
+}
+
+// encodeMeshBinary (scene/mesh_binary.cpp) of a mesh (nrm / uv may be NULL): the encoded size, the bytes
+// written into `out` when `cap` holds them.
+REF_API size_t ref_mesh_encode(const float* pos, const float* nrm, const float* uv, uint32_t nv, const uint32_t* idx,
+                               uint32_t ni, uint8_t* out, size_t cap) {
+    std::vector<Vec3f> p(nv), n;
+    std::vector<Vec2f> t;
+    for (uint32_t i = 0; i < nv; ++i) p[i] = v3(pos + 3 * i);
+    if (nrm) {
+        n.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i) n[i] = v3(nrm + 3 * i);
+    }
+    if (uv) {
+        t.resize(nv);
+        for (uint32_t i = 0; i < nv; ++i) t[i] = Vec2f(uv[2 * i], uv[2 * i + 1]);
+    }
+    const TriangleMesh mesh(p, n, t, std::vector<uint32_t>(idx, idx + ni), nullptr);
+    const std::vector<uint8_t> bytes = encodeMeshBinary(mesh);
+    if (out && cap >= bytes.size()) std::memcpy(out, bytes.data(), bytes.size());
+    return bytes.size();
+}
+
+// decodeMeshBinary of `size` bytes: 0 and counts = (vertices, indices, has uvs) on success, 1 when the
+// reference refuses the bytes. pos / uv / idx receive the mesh when they are non-NULL and `counts` on entry
+// holds their capacities (vertices, indices).
+REF_API int ref_mesh_decode(const uint8_t* data, size_t size, float* pos, float* uv, uint32_t* idx,
+                            uint32_t* counts) {
+    std::string err;
+    const auto mesh = decodeMeshBinary(data, size, nullptr, &err);
+    if (!mesh) return 1;
+    const uint32_t capV = counts[0], capI = counts[1];
+    const auto& p = mesh->positions();
+    const auto& t = mesh->uvs();
+    const auto& ix = mesh->indices();
+    counts[0] = static_cast<uint32_t>(p.size());
+    counts[1] = static_cast<uint32_t>(ix.size());
+    counts[2] = t.empty() ? 0u : 1u;
+    if (pos && p.size() <= capV)
+        for (size_t i = 0; i < p.size(); ++i) {
+            pos[3 * i] = p[i].x; pos[3 * i + 1] = p[i].y; pos[3 * i + 2] = p[i].z;
+        }
+    if (uv && t.size() <= capV)
+        for (size_t i = 0; i < t.size(); ++i) {
+            uv[2 * i] = t[i].x; uv[2 * i + 1] = t[i].y;
+        }
+    if (idx && ix.size() <= capI) std::copy(ix.begin(), ix.end(), idx);
+    return 0;
+}
+
+// temperatureColor (core/image/imager.cpp:204-213) of n temperatures.
+REF_API int ref_temperature_color(size_t n, const float* kelvin, float* rgb) {
+    for (size_t i = 0; i < n; ++i) {
+        const Color3f c = temperatureColor(kelvin[i]);
+        rgb[3 * i] = c.r; rgb[3 * i + 1] = c.g; rgb[3 * i + 2] = c.b;
+    }
+    return 0;
 }
 
 // cryptoHashName and cryptoHashToFloat (cryptomatte.cpp:78-112) of n names.

@@ -9,12 +9,13 @@ rendered at the Mitsuba image's size with 256 samples, maxBounces 64 and a width
   2. the median and the 95th percentile of the relative difference of 8x8 block means (grey = (r+g+b)/3; blocks
      below 2% of the brightest reference block skipped) under the case's bounds,
 
-with the reference's thresholds (test_golden_reference.cpp:195-209). The scene documents are read as
-loadProject + buildRenderScene read them (scene/project_io.cpp:148-182, scene/scene_graph.cpp:170-216,
-scene/document.cpp:330-352; the orbit camera of test_golden_reference.cpp:44-51). The analytic spheres of the
-documents are tessellated (128 x 64 UV sphere with smooth normals), as the reference's USD arm of the same test
-renders them (test_golden_reference.cpp:211-215); the environment map's RGBE bytes are decoded as stb_image
-decodes Radiance files (a byte m with exponent e gives m * 2^(e - 136), e = 0 gives black).
+with the reference's thresholds (test_golden_reference.cpp:195-209). The scene documents go through the
+product's document loader, photon_hip.document (loadProject + buildRenderScene, scene/project_io.cpp,
+scene/scene_graph.cpp:170-216, scene/document.cpp:315-352; the orbit camera of test_golden_reference.cpp:44-51),
+so these images pin the loader as well. The analytic spheres of the documents are tessellated (128 x 64 UV
+sphere with smooth normals), as the reference's USD arm of the same test renders them
+(test_golden_reference.cpp:211-215); the environment map's RGBE bytes are decoded as stb_image decodes Radiance
+files (a byte m with exponent e gives m * 2^(e - 136), e = 0 gives black).
 
 CPU: the oracle meets the thresholds (the Mitsuba images pin the oracle independently of the reference's code).
 GPU: pe_render meets them, and at 1024 samples its image is the oracle's image to relMSE 1e-3.
@@ -23,15 +24,15 @@ The fixture tests/golden/mitsuba_golden.npz is written by tests/golden/make_mits
 from __future__ import annotations
 
 import json
-import math
+import tempfile
 from pathlib import Path
 
 import numpy as np
 import pytest
 
 from oracle import bindings
+from photon_hip import document
 from photon_hip.device import camera_perspective, render, settings
-from photon_hip.scene import Environment, Mesh, Scene, dielectric, disney, lambert, mirror
 
 FIXTURE = Path(__file__).resolve().parent / "golden" / "mitsuba_golden.npz"
 SPP = 256
@@ -44,44 +45,6 @@ CASES = {
 }
 
 
-def _col(j, key, default):
-    return tuple(float(x) for x in j.get(key, default))
-
-
-def material(j):
-    """materialFromJson (project_io.cpp:148-182) for the keys the golden documents use."""
-    t = j.get("type", "disney")
-    if t == "glass":
-        return dielectric(j.get("ior", 1.5), _col(j, "tint", (1, 1, 1)), j.get("roughness", 0.0))
-    if t == "lambert":
-        return lambert(_col(j, "albedo", (0.8, 0.8, 0.8)), _col(j, "emission", (0, 0, 0)))
-    if t == "mirror":
-        return mirror(_col(j, "color", (0.95, 0.95, 0.95)))
-    return disney(_col(j, "baseColor", (0.8, 0.8, 0.8)), j.get("metallic", 0.0), j.get("roughness", 0.5),
-                  j.get("specular", 0.5), clearcoat=j.get("clearCoat", 0.0),
-                  clearcoat_roughness=j.get("clearCoatRoughness", 0.03), anisotropy=j.get("anisotropy", 0.0),
-                  sheen=j.get("sheen", 0.0), diffuse_transmission=j.get("diffuseTransmission", 0.0),
-                  emission=_col(j, "emission", (0, 0, 0)))
-
-
-def uv_sphere(center, radius, lon=128, lat=64) -> Mesh:
-    th = np.linspace(0.0, np.pi, lat + 1)
-    ph = np.linspace(0.0, 2.0 * np.pi, lon + 1)
-    T, P = np.meshgrid(th, ph, indexing="ij")
-    n = np.stack([np.sin(T) * np.cos(P), np.cos(T), np.sin(T) * np.sin(P)], -1).reshape(-1, 3)
-    pos = (np.asarray(center, np.float64) + radius * n).astype(np.float32)
-    uv = np.stack([P / (2 * np.pi), T / np.pi], -1).reshape(-1, 2).astype(np.float32)
-    tris = []
-    for i in range(lat):
-        for k in range(lon):
-            a, b = i * (lon + 1) + k, (i + 1) * (lon + 1) + k
-            if i > 0:
-                tris.append((a, a + 1, b))          # counter-clockwise seen from outside
-            if i < lat - 1:
-                tris.append((a + 1, b + 1, b))
-    return Mesh(pos, np.asarray(tris, np.uint32), n.astype(np.float32), uv)
-
-
 def decode_rgbe(rgbe: np.ndarray) -> np.ndarray:
     e = rgbe[..., 3].astype(np.int32)
     scale = np.where(e > 0, np.ldexp(np.float32(1.0), e - 136), 0.0).astype(np.float32)
@@ -89,45 +52,26 @@ def decode_rgbe(rgbe: np.ndarray) -> np.ndarray:
 
 
 def load(name):
-    """The scene, the camera and the Mitsuba image of one golden case."""
+    """The scene, the camera and the Mitsuba image of one golden case, read by the product's document loader
+    (photon_hip.document): the fixture's document (given back the format and version keys the fixture leaves
+    out) and its environment map, written as the Radiance file the document names."""
     z = np.load(FIXTURE)
     doc = json.loads(bytes(z[f"{name}_doc"]).decode())
+    doc.setdefault("format", "photon-project")
+    doc.setdefault("version", 2)
     ref = z[f"{name}_image"].astype(np.float32)
     h, w = ref.shape[:2]
-    sc = Scene()
-    mats = [material(m) for m in doc["materials"]]
-    mat_ids = {}
-    for node in doc["nodes"]:
-        if not node.get("visible", True):
-            continue
-        mi = node["material"]
-        if mi not in mat_ids:
-            mat_ids[mi] = sc.add_material(mats[mi])
-        xf = np.asarray(node["xf"], np.float64).reshape(4, 4)   # row-major, translation in the last column
-        if node["type"] == "mesh":
-            g = node["geometry"]
-            p = np.asarray(g["p"], np.float64).reshape(-1, 3)
-            p = (p @ xf[:3, :3].T + xf[:3, 3]).astype(np.float32)
-            nrm = None
-            if g.get("n"):
-                nrm = np.asarray(g["n"], np.float64).reshape(-1, 3) @ np.linalg.inv(xf[:3, :3])
-                nrm = (nrm / np.linalg.norm(nrm, axis=1, keepdims=True)).astype(np.float32)
-            uv = np.asarray(g["uv"], np.float32).reshape(-1, 2) if g.get("uv") else None
-            sc.add_shape(Mesh(p, np.asarray(g["i"], np.uint32).reshape(-1, 3), nrm, uv), mat_ids[mi])
-        elif node["type"] == "sphere":
-            s = max(np.linalg.norm(xf[:3, k]) for k in range(3))
-            sc.add_shape(uv_sphere(xf[:3, 3], node["radius"] * s), mat_ids[mi])
-    assert not doc["lights"]
-    env = doc["environment"]
-    if env.get("hdr") and env.get("intensity", 1.0) > 0.0:   # a zero-intensity environment is left out
-        sc.environment = Environment(decode_rgbe(z["env_rgbe"]), float(env.get("rotation", 0.0)),
-                                     float(env.get("intensity", 1.0)))
-    c = doc["camera"]
-    f32 = np.float32
-    th, ph, r = f32(c["theta"]), f32(c["phi"]), f32(c["radius"])
-    target = np.asarray(c["target"], f32)
-    eye = target + np.array([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)], f32) * r
-    cam = camera_perspective(tuple(eye), tuple(target), (0.0, 1.0, 0.0), float(c["fov"]), w / h)
+    with tempfile.TemporaryDirectory() as d:
+        hdr = doc["environment"].get("hdr")
+        if hdr:
+            rgbe = z["env_rgbe"]
+            head = f"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y {rgbe.shape[0]} +X {rgbe.shape[1]}\n".encode()
+            (Path(d) / hdr).write_bytes(head + rgbe.tobytes())
+        pr = document.load_project_json(json.dumps(doc), d)
+        sc = document.build_render_scene(pr, sphere_segments=(128, 64))
+    assert not pr.lights and not pr.warnings, pr.warnings
+    v = document.orbit_view(pr, w, h, max_bounces=64)
+    cam = camera_perspective(v.position, v.target, v.up, v.vfov, w / h)
     return sc, cam, ref
 
 
