@@ -380,10 +380,14 @@ typedef struct pe_display {
     float glare_angle_deg, glare_length;
     float white_balance_k, tint, saturation, contrast;
     float vignette, chromatic_aberration;
-    float reserved[4];
+    /* 1 = denoiseImage on the resolved image, before the imager (main.cpp:277-289). Without OIDN, which this
+     * image lacks (as does any reference build without PHOTON_ENABLE_OIDN), that is softBlurDenoise
+     * (engine/denoiser.cpp:17-43, :140-143): each pixel the 3 x 3 average weighted by 1 / (1 + luminance^2). */
+    int32_t denoise;
+    float reserved[3];
 } pe_display;
 
-/* The reference's defaults: ACES, 0 EV, dither on, ImagerSettings{} (identity). */
+/* The reference's defaults: ACES, 0 EV, dither on, ImagerSettings{} (identity), no denoising. */
 int pe_display_defaults(pe_display* out);
 /* The film's display image: display_rgb (3 * w * h floats in [0, 1], sRGB-encoded) and / or rgb8 (3 * w * h
  * bytes); either may be NULL. Rows top first, as the film. */

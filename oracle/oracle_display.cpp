@@ -412,6 +412,27 @@ void apply_imager(Img& hdr, const pe_display& s, float ev) {  // imager.cpp:235-
         }
 }
 
+// softBlurDenoise (engine/denoiser.cpp:17-43), what denoiseImage does without OIDN (:140-143): each pixel the
+// 3 x 3 average (edges clamped) weighted by 1 / (1 + luminance^2), the colour scaled by 1 / the weight sum.
+void soft_blur(Img& img) {
+    const Img src = img;
+    for (int y = 0; y < img.h; ++y)
+        for (int x = 0; x < img.w; ++x) {
+            C3 sum(0.0f);
+            float wsum = 0.0f;
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const C3 c = src.get(std::clamp(x + dx, 0, img.w - 1), std::clamp(y + dy, 0, img.h - 1));
+                    const float lum = 0.2126f * c.r + 0.7152f * c.g + 0.0722f * c.b;
+                    const float weight = 1.0f / (1.0f + lum * lum);
+                    sum = sum + c * weight;
+                    wsum += weight;
+                }
+            const float inv = 1.0f / std::max(wsum, 1e-6f);
+            img.set(x, y, sum * inv);
+        }
+}
+
 }  // namespace
 
 // toneMap (tone_mapping.cpp:157-160) of n linear colours.
@@ -446,6 +467,7 @@ ORACLE_API int oracle_display(int w, int h, const float* sum, const uint32_t* co
             const float inv = 1.0f / static_cast<float>(n);
             img.set(x, y, C3(sum[3 * k] * inv, sum[3 * k + 1] * inv, sum[3 * k + 2] * inv));
         }
+    if (ds->denoise) soft_blur(img);  // before the imager (main.cpp:277-289)
     apply_imager(img, *ds, ds->exposure_ev);
     for (int y = 0; y < h; ++y)
         for (int x = 0; x < w; ++x) {

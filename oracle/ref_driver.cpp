@@ -6,6 +6,7 @@
 // tests and bench.py's CPU-baseline leg call through ctypes: Sobol values, scene build from the same
 // flattened pe_scene_desc the HIP library takes, closest/any hit on ray batches, the microfacet energy
 // tables, Disney BSDF eval/sample, per-sample radiance and Renderer::renderSamples itself.
+#include "engine/denoiser.h"
 #include "engine/renderer.h"
 #include "engine/scene.h"
 #include "camera/perspective_camera.h"
@@ -570,6 +571,7 @@ REF_API int ref_display(int w, int h, const float* sum, const uint32_t* count, c
             film.addBatch(x, y, Color3f(sum[3 * k], sum[3 * k + 1], sum[3 * k + 2]), count[k]);
         }
     Image img = film.resolve();
+    if (ds->denoise) denoiseImage(img);  // main.cpp:277-289; softBlurDenoise without OIDN
     ImagerSettings is;
     is.bloomIntensity = ds->bloom_intensity;
     is.bloomRadius = ds->bloom_radius;

@@ -68,6 +68,30 @@ __global__ void __launch_bounds__(kDispBlock) k_resolve(const float* __restrict_
     }
 }
 
+// denoiseImage without OIDN: softBlurDenoise (engine/denoiser.cpp:17-43). Each pixel becomes the average of its
+// 3 x 3 neighbourhood (coordinates clamped to the image), each neighbour weighted by 1 / (1 + luminance^2), which
+// pulls single bright pixels down; sums in the reference's order, uncontracted, the colour scaled by 1 / the
+// weight sum as Color3f::operator/ does.
+__global__ void __launch_bounds__(kDispBlock) k_soft_blur(Img4 src, Img4 dst) {
+#pragma clang fp contract(off)
+    PIXELS(dst) {
+        const int x = i % dst.w, y = i / dst.w;
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f, ws = 0.0f;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const float3 c = ld(src, min(max(x + dx, 0), src.w - 1), min(max(y + dy, 0), src.h - 1));
+                const float lum = 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z;
+                const float wgt = 1.0f / (1.0f + lum * lum);
+                sr += c.x * wgt;
+                sg += c.y * wgt;
+                sb += c.z * wgt;
+                ws += wgt;
+            }
+        const float inv = 1.0f / fmaxf(ws, 1e-6f);
+        st(dst, x, y, make_float3(sr * inv, sg * inv, sb * inv));
+    }
+}
+
 __global__ void __launch_bounds__(kDispBlock) k_chroma(Img4 src, Img4 dst, float k) {  // imager.cpp:244-256
     const float cx = 0.5f * src.w, cy = 0.5f * src.h;
     PIXELS(dst) {
@@ -436,6 +460,12 @@ extern "C" int pe_film_display(pe_film* film, const pe_display* ds, float* displ
         if (!hdr.p) return PE_ERR_OOM;
         hipLaunchKernelGGL(k_resolve, dim3(grid_for(n)), dim3(kDispBlock), 0, s, film->sum.as<float>(),
                            film->count.as<uint32_t>(), hdr);
+        if (ds->denoise) {
+            Img4 resolved = hdr;
+            hdr = pool.make(w, h);
+            if (!hdr.p) return PE_ERR_OOM;
+            hipLaunchKernelGGL(k_soft_blur, dim3(grid_for(n)), dim3(kDispBlock), 0, s, resolved, hdr);
+        }
         const pe_display& c = *ds;
         const bool identity = c.bloom_intensity <= 0.0f && c.glare_intensity <= 0.0f && c.white_balance_k == 6500.0f &&
                               c.tint == 0.0f && c.saturation == 1.0f && c.contrast == 1.0f && c.vignette <= 0.0f &&

@@ -151,7 +151,7 @@ class Display(C.Structure):
                 ("glare_intensity", C.c_float), ("glare_blades", C.c_int32), ("glare_angle_deg", C.c_float),
                 ("glare_length", C.c_float), ("white_balance_k", C.c_float), ("tint", C.c_float),
                 ("saturation", C.c_float), ("contrast", C.c_float), ("vignette", C.c_float),
-                ("chromatic_aberration", C.c_float), ("reserved", C.c_float * 4)]
+                ("chromatic_aberration", C.c_float), ("denoise", C.c_int32), ("reserved", C.c_float * 3)]
 
 
 class KernelStats(C.Structure):

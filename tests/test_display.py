@@ -4,8 +4,10 @@ the 8-bit loop of saveImagePNG, core/image/image_io.cpp:61-80).
 
 CPU: the oracle's restatement (oracle/oracle_display.cpp) equals the reference's own functions (IEEE build)
 float for float and byte for byte: all seven tone-map operators, exposure, and each imager stage alone and all
-together (chromatic aberration, bloom, glare, white balance with tint, contrast, saturation, vignette), on a film
-with empty, dim and very bright pixels; closed forms for the operators that have one.
+together (chromatic aberration, bloom, glare, white balance with tint, contrast, saturation, vignette), and the
+denoiser of a build without OIDN (denoiseImage = softBlurDenoise, engine/denoiser.cpp:17-43, before the imager as
+main.cpp:277-289 orders them), on a film with empty, dim and very bright pixels; closed forms for the operators
+that have one, and for the denoiser: a flat image stays flat, an isolated bright pixel is pulled down.
 GPU: pe_film_display on a rendered film against the oracle on the same sums (float tolerance: the device's pow,
 log2 and exp differ from the host's in the last bits; 8-bit values equal or one apart).
 """
@@ -23,6 +25,8 @@ OPS = [capi.PE_TONEMAP_REINHARD, capi.PE_TONEMAP_REINHARD_EXTENDED, capi.PE_TONE
 
 IMAGERS = {
     "identity": {},
+    "denoise": {"denoise": 1},
+    "denoise_all": {"denoise": 1, "bloom_intensity": 0.5, "glare_intensity": 0.3, "white_balance_k": 9000.0},
     "chromatic": {"chromatic_aberration": 3.0},
     "bloom": {"bloom_intensity": 0.6, "bloom_radius": 0.08, "bloom_threshold": 0.8},
     "glare": {"glare_intensity": 0.4, "glare_blades": 6, "glare_angle_deg": 20.0, "glare_length": 0.15},
@@ -100,8 +104,31 @@ def test_tonemap_closed_forms():
 
 # ------------------------------------------------------------------------------------------------- GPU
 
+def test_denoise_closed_forms():
+    """softBlurDenoise: a flat image is a fixed point (every weight equal); a lone grey pixel c of luminance L on
+    black has weight k = 1 / (1 + L^2) against its eight black neighbours' weights of 1, so it becomes
+    c k / (8 + k)."""
+    d = display_settings(tonemap=capi.PE_TONEMAP_LINEAR, denoise=1, dither=0)
+    flat = np.full((5, 7, 3), 0.3, np.float32)
+    cnt = np.ones((5, 7), np.uint32)
+    a, _ = bindings.display("oracle", flat, cnt, d)
+    b, _ = bindings.display("oracle", flat, cnt, display_settings(tonemap=capi.PE_TONEMAP_LINEAR, dither=0))
+    np.testing.assert_array_equal(a, b)
+    spike = np.zeros((5, 5, 3), np.float32)
+    spike[2, 2] = 0.9                                   # luminance 0.9 (grey), within the linear operator's range
+    raw = np.zeros((5, 5, 3), np.float32)
+    k = 1.0 / (1.0 + 0.9 * 0.9)
+    raw[2, 2] = 0.9 * k / (8.0 + k)                     # the centre: its own term over the nine weights
+    assert raw[2, 2, 0] < 0.9 / 5                       # pulled far down
+    out, _ = bindings.display("oracle", spike, np.ones((5, 5), np.uint32),
+                              display_settings(tonemap=capi.PE_TONEMAP_LINEAR, denoise=1, dither=0))
+    lin, _ = bindings.display("oracle", raw, np.ones((5, 5), np.uint32),
+                              display_settings(tonemap=capi.PE_TONEMAP_LINEAR, dither=0))
+    np.testing.assert_allclose(out[2, 2], lin[2, 2], rtol=1e-6)
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["identity", "bloom", "glare", "all"])
+@pytest.mark.parametrize("name", ["identity", "bloom", "glare", "all", "denoise", "denoise_all"])
 def test_gpu_display_matches_oracle(dev, name):
     sc, view = configs.product_sphere(width=96, height=54, lon=96, lat=64, env_width=256, env_height=128)
     gs = dev.upload(sc)
