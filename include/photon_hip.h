@@ -106,7 +106,13 @@ typedef struct pe_material_desc {
     float absorption_color[3];
     float absorption_distance;
     int32_t thin_walled;
-    float reserved[5];
+    /* Disney only: DisneyMaterial::setOpacity (disney.h:147-158, clamped to [0, 1]) when has_opacity != 0
+     * (a zeroed descriptor is opaque). Below 1 the surface is a stochastic cutout: a ray passes it with
+     * probability 1 - opacity (passesCutout, path_tracer.cpp:159-167, one sampler dimension) and a shadow ray
+     * is attenuated by 1 - opacity (shadowTransmittance, :172-202). */
+    float opacity;
+    int32_t has_opacity;
+    float reserved[3];
 } pe_material_desc;
 
 #define PE_SHAPE_CAMERA_INVISIBLE 1u

@@ -1125,8 +1125,17 @@ bool randomWalkSubsurface(const Scene& scene, const SI& entry, V3 dir, V3 sigmaT
     return false;
 }
 
-// shadowTransmittance (path_tracer.cpp:172-202), without shadow linking: thin-walled surfaces pass the shadow
-// ray on with their transmittance, anything else blocks it.
+// passesCutout (path_tracer.cpp:159-167): opacity 1 never passes, 0 always, in between a sampler draw (one
+// dimension) at or above the opacity passes.
+bool passesCutout(const pe_material_desc& m, Sobol& sampler) {
+    const float alpha = materialOpacity(m);
+    if (alpha >= 1.0f) return false;
+    if (alpha <= 0.0f) return true;
+    return sampler.get1D() >= alpha;
+}
+
+// shadowTransmittance (path_tracer.cpp:172-202), without shadow linking: thin-walled surfaces and cutouts pass
+// the shadow ray on with their transmittance, anything else blocks it.
 V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow, RayCounters& rc) {
     if (!transmitters) return intersectAny(scene, shadow) ? V3(0.0f) : V3(1.0f);
     V3 T(1.0f);
@@ -1267,8 +1276,10 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
             ++rc.extension;
             bool hit = intersect(scene, cur, h);
             if (hit) fillHit(scene, cur, h, isect);
-            for (int skip = 0; hit && skip < 64; ++skip) {  // :423-441 (no cutouts: opacity is 1)
-                if (!(cameraRay && isect.cameraInvisible)) break;
+            for (int skip = 0; hit && skip < 64; ++skip) {  // :423-441
+                if (!(cameraRay && isect.cameraInvisible) &&
+                    !passesCutout(scene.materials[static_cast<size_t>(isect.material)], sampler))
+                    break;
                 Ray next;
                 next.o = offsetRayOrigin(isect.point, isect.pError, isect.ng, cur.d);
                 next.d = cur.d;

@@ -954,7 +954,16 @@ bool materialIsVolumeBoundary(const pe_material_desc& m) {
     return false;
 }
 
-bool materialTransmitsShadows(const pe_material_desc& m) { return m.type == PE_MAT_DIELECTRIC && m.thin_walled != 0; }
+// Material::opacity: DisneyMaterial's setOpacity value, clamped to [0, 1] (disney.h:147-158); 1 for the others.
+float materialOpacity(const pe_material_desc& m) {
+    return m.type == PE_MAT_DISNEY && m.has_opacity ? std::clamp(m.opacity, 0.0f, 1.0f) : 1.0f;
+}
+
+// transmitsShadows: a thin-walled Dielectric (dielectric.h:48) or a Disney material with opacity below 1
+// (disney.h:158).
+bool materialTransmitsShadows(const pe_material_desc& m) {
+    return (m.type == PE_MAT_DIELECTRIC && m.thin_walled != 0) || materialOpacity(m) < 1.0f;
+}
 
 bool materialIsDispersive(const pe_material_desc& m) { return m.type == PE_MAT_DIELECTRIC && m.abbe > 0.0f; }
 
@@ -968,6 +977,7 @@ bool materialInteriorAbsorption(const pe_material_desc& m, V3& sigmaA) {
 
 V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp, V3 dir) {
     if (!materialTransmitsShadows(m)) return V3(0.0f);
+    if (m.type == PE_MAT_DISNEY) return V3(1.0f - materialOpacity(m));  // disney.cpp:511-513
     const Glass g(m);
     const V3 n = lengthSq(sp.ng) > 0.0f ? normalized(sp.ng) : sp.normal;
     return (V3(1.0f) - g.thinWallReflectance(dot(dir, n))) * g.tint;

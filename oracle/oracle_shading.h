@@ -82,6 +82,7 @@ bool materialIsDelta(const pe_material_desc& m);
 // Material::isVolumeBoundary / transmitsShadows / isDispersive (material.h:109-122; dielectric.h:48-78;
 // disney.h:76).
 bool materialIsVolumeBoundary(const pe_material_desc& m);
+float materialOpacity(const pe_material_desc& m);
 bool materialTransmitsShadows(const pe_material_desc& m);
 bool materialIsDispersive(const pe_material_desc& m);
 // Dielectric interior medium (Dielectric::rebuildMedium, dielectric.cpp:136-144): absorption only (the scene

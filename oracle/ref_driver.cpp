@@ -77,6 +77,7 @@ std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
     m->setEdgeTint(c3(d.edge_tint));
     m->setThinFilm(d.thin_film_thickness, d.thin_film_ior);
     m->setEmission(c3(d.emission));
+    if (d.has_opacity) m->setOpacity(d.opacity);
     return m;
 }
 

@@ -513,10 +513,12 @@ struct TransmitShadowJob {
             fill_hit(sc, tri, r.d, u, v, si);
             const GpuMaterial& m = sc.materials[si.material];
             const uint32_t queries = (bits >> 1) + 1u;
-            if (!(m.flags & kMatFlagThinWall)) {
+            // Thin walls pass the shadow ray on with their transmittance, cutouts with 1 - opacity
+            // (shadowTransmittance, dielectric.cpp / disney.cpp:511-513); anything else blocks it.
+            if (!(m.flags & (kMatFlagThinWall | kMatFlagCutout))) {
                 c = splat(0.0f);
             } else {
-                c = c * thin_wall_transmittance(m, si, r.d);
+                c = c * ((m.flags & kMatFlagThinWall) ? thin_wall_transmittance(m, si, r.d) : splat(1.0f - m.opacity));
                 const float remaining = r.tmax - t;
                 if (!is_black(c) && remaining > 0.0f) {
                     if (queries >= kMaxShadowQueries) {
@@ -1002,9 +1004,16 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 alive = false;
             } else {
                 fill_hit(sc, tri, dir, h.z, h.w, si);
-                if (camera && (si.flags & kTriCameraInvisible)) {
-                    // Camera rays pass through lights hidden from the camera (path_tracer.cpp:423-441): the ray
-                    // continues from the far side, and the continuation is not a new vertex.
+                // Camera rays pass through lights hidden from the camera, and any ray through a cutout surface
+                // with probability 1 - opacity, one sampler dimension drawn when the opacity is strictly between 0
+                // and 1 (passesCutout, path_tracer.cpp:159-167, :423-441): the ray continues from the far side,
+                // and the continuation is not a new vertex.
+                bool pass = camera && (si.flags & kTriCameraInvisible);
+                if (kTransport && !pass && (sc.materials[si.material].flags & kMatFlagCutout)) {
+                    const float op = sc.materials[si.material].opacity;
+                    pass = op <= 0.0f || sobol_get1d(ss) >= op;
+                }
+                if (pass) {
                     new_o = offset_ray_origin(si.point, si.p_error, si.ng, dir);
                     new_d = dir;
                     extend = true;
@@ -1072,11 +1081,12 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
                     // Guided: the slot's mixture was inferred (vertices below depth guided_bounces).
                     // Guided vertices as the reference picks them (path_tracer.cpp:624-627): not delta, not a
-                    // volume boundary (whose refraction moves the medium stack), not a walk's exit. With one guided
+                    // volume boundary (whose refraction moves the medium stack), not a surface that transmits
+                    // shadows (a cutout; thin walls are delta), not a walk's exit. With one guided
                     // bounce only camera entries were inferred: a depth-0 continuation (past a camera-invisible
                     // light or a thin wall) samples the BSDF alone.
                     const bool guided = kGuide && static_cast<int>(depth) < lp.guided_bounces && lp.guide_fraction > 0.0f &&
-                                        !is_delta && !(m.flags & kMatFlagVolume) && !exit_vertex &&
+                                        !is_delta && !(m.flags & (kMatFlagVolume | kMatFlagCutout)) && !exit_vertex &&
                                         (lp.guided_bounces > 1 || cam_entry);
                     const float alpha = lp.guide_fraction;
                     // The slot's mixture is read where it is used (L1/L2), not held in registers across the NEE loop.

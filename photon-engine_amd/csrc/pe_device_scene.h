@@ -56,7 +56,8 @@ struct alignas(16) GpuMaterial {
     V3 sigma_a;                      // interior Beer-Lambert absorption (rebuildMedium, dielectric.cpp:136-144)
     float abbe;                      // > 0: Cauchy dispersion (setAbbe, dielectric.h:74)
     float glass_alpha;               // ggxClampAlpha(roughness^2) (dielectric.cpp:21)
-    float pad2, pad3, pad4;
+    float opacity;                   // Disney setOpacity (kMatFlagCutout when below 1)
+    float pad3, pad4;
     // Disney subsurface random walk (subsurfaceCoeffs, disney.cpp:738-757)
     V3 sss_sigma_t;
     float sss_g;
@@ -86,6 +87,7 @@ enum : uint32_t {
     kMatFlagDispersive = 64u,        // isDispersive
     kMatFlagMedium = 128u,           // interior medium absorbs (sigma_a != 0)
     kMatFlagFilm = 256u,             // thin-film interference on the specular Fresnel / the thin wall
+    kMatFlagCutout = 512u,           // Disney opacity below 1: stochastic cutout, shadow transmittance 1 - opacity
 };
 
 // AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.

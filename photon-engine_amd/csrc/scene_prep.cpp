@@ -30,6 +30,7 @@ float f0_from_ior(float ior) {  // disney.cpp:25-28
 // DisneyMaterial setters (disney.h:22-126), then the per-material terms of disney.cpp.
 int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMaterial& g, std::string& err) {
     g = GpuMaterial{};
+    g.opacity = 1.0f;
     g.emission = c3(d.emission);
     g.flags = is_black(g.emission) ? 0u : kMatFlagEmissive;
     if (d.type == PE_MAT_LAMBERT) {
@@ -95,6 +96,10 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
     g.subsurface = clampf(d.subsurface, 0.0f, 1.0f);
     g.edge_tint = clamp01v(c3(d.edge_tint));
     if (g.specular_transmission > 0.0f) g.flags |= kMatFlagVolume;  // isVolumeBoundary (disney.h:76)
+    if (d.has_opacity) {  // setOpacity clamps (disney.h:148); transmitsShadows below 1 (:158)
+        g.opacity = clampf(d.opacity, 0.0f, 1.0f);
+        if (g.opacity < 1.0f) g.flags |= kMatFlagCutout;
+    }
 
     // resolvedSheenColor (disney.cpp:160-166)
     V3 sc = splat(0.0f);
@@ -449,15 +454,15 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
             return rc;
         }
         GpuMaterial& g = mats[static_cast<size_t>(i)];
-        transmitters = transmitters || (g.flags & kMatFlagThinWall) != 0;
+        transmitters = transmitters || (g.flags & (kMatFlagThinWall | kMatFlagCutout)) != 0;
         const bool lobes_below = g.trans_w > 0.0f || g.st_w > 0.0f || g.p_trans > 0.0f || g.p_spec_trans > 0.0f ||
                                  g.p_sss > 0.0f;
         transport = transport || g.type == kMatDielectric || lobes_below ||
                     (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium |
-                                kMatFlagFilm)) != 0;
+                                kMatFlagFilm | kMatFlagCutout)) != 0;
         g.sort_class = g.type == kMatDielectric ? kSortClassDielectric
                        : g.type != kMatDisney   ? kSortClassSimple
-                       : (lobes_below || (g.flags & kMatFlagFilm)) ? kSortClassDisneyTransport
+                       : (lobes_below || (g.flags & (kMatFlagFilm | kMatFlagCutout))) ? kSortClassDisneyTransport
                                                                    : kSortClassDisney;
         if (i < num_mats) classes |= 1u << g.sort_class;
     }

@@ -104,6 +104,10 @@ bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
             err = "DisneyMaterial with texture maps or flakes is not implemented on the HIP device";
             return false;
         }
+        if (d->baseOpacity() < 1.0f) {  // a cutout (setOpacity, disney.h:147-158)
+            o.opacity = d->baseOpacity();
+            o.has_opacity = 1;
+        }
         o.type = PE_MAT_DISNEY;
         putc(o.base_color, d->baseColor());
         o.metallic = d->metallic();

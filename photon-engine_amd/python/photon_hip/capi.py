@@ -56,7 +56,8 @@ class MaterialDesc(C.Structure):
         ("subsurface_scale", C.c_float), ("subsurface_anisotropy", C.c_float),
         ("edge_tint", F3), ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
         ("abbe", C.c_float), ("absorption_color", F3), ("absorption_distance", C.c_float),
-        ("thin_walled", C.c_int32), ("reserved", C.c_float * 5),
+        ("thin_walled", C.c_int32), ("opacity", C.c_float), ("has_opacity", C.c_int32),
+        ("reserved", C.c_float * 3),
     ]
 
 

@@ -18,7 +18,7 @@ What each part follows:
   * read_hdr: the Radiance RGBE format (flat and run-length scanlines) decoded as stb_image decodes it.
 
 What the HIP path does not render is refused with DocumentError, not approximated: texture maps, flakes,
-opacity below 1, rounded edges and scattering glass; point, spot and directional lights and light linking;
+rounded edges and scattering glass; point, spot and directional lights and light linking;
 the physical sky; shadow catchers; OpenEXR, glTF, FBX, Alembic and USD files. Analytic sphere nodes are refused
 unless the caller asks for them tessellated (sphere_segments), as the reference's USD arm of its golden test
 renders them.
@@ -463,8 +463,9 @@ def _material(j: dict, base: Path):
     m.emission[:] = list(_vec3(j, "emission", (0, 0, 0)))
     if isinstance(j.get("maps"), dict) and j["maps"]:
         why.append("texture maps")
-    if _f(j, "opacity", 1.0) < 1.0:
-        why.append("opacity below 1")
+    if "opacity" in j:                                   # setOpacity, clamped by the material (disney.h:148)
+        m.opacity = _f(j, "opacity", 1.0)
+        m.has_opacity = 1
     m.ior = _f(j, "ior", 1.5)
     m.specular_transmission = _f(j, "specularTransmission", 0.0)
     m.coat_ior = _f(j, "coatIor", 1.5)

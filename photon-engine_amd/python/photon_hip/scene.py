@@ -38,6 +38,8 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
     m.thin_film_thickness, m.thin_film_ior = 0.0, 1.4
     m.absorption_color[:] = [1.0, 1.0, 1.0]
     m.absorption_distance = 0.0
+    if "opacity" in kw:                      # setOpacity (disney.h:148)
+        m.has_opacity = 1
     for k, v in kw.items():
         cur = getattr(m, k)
         if isinstance(cur, C.Array):
