@@ -112,6 +112,13 @@ typedef struct pe_material_desc {
      * is attenuated by 1 - opacity (shadowTransmittance, :172-202). */
     float opacity;
     int32_t has_opacity;
+    /* Dielectric only: Dielectric::setScattering(density, color, anisotropy) (dielectric.h:61,
+     * dielectric.cpp:129-144): the interior medium scatters with sigma_s = color * density, adds
+     * (1 - color) * density to sigma_a, and has Henyey-Greenstein asymmetry g (clamped to [-0.95, 0.95]); paths
+     * inside sample free flights (sampleFreeFlight, path_tracer.cpp:97-129, :457-479). Density 0: no scattering. */
+    float scatter_density;
+    float scatter_color[3];
+    float scatter_anisotropy;
     float reserved[3];
 } pe_material_desc;
 

@@ -1165,6 +1165,8 @@ V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow,
     return V3(0.0f);
 }
 
+constexpr int kMaxMediumEvents = 1024;  // path_tracer.cpp:51
+
 struct MediumStack {  // path_tracer.cpp:63-83 (entries are material indices; -1 = the walk's exit material)
     int owner[4] = {0, 0, 0, 0};
     int n = 0;
@@ -1207,6 +1209,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
     float lastBsdfPdf = 0.0f;
     V3 lastPoint = ray.o;
     MediumStack media;
+    int mediumEvents = 0;  // :51, :463
     float lambda = 0.0f;
     bool pendingExit = false;
     SI isect;
@@ -1296,13 +1299,37 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                 if (hit) primary->isect = isect;
             }
             firstSegment = false;
-            // Inside a volume: Beer-Lambert absorption along the segment (:457-479; no scattering media here).
+            // Inside a volume (:457-479): a free flight in a scattering medium, which may scatter before the
+            // surface (a new ray from there, no vertex, no NEE inside), or Beer-Lambert absorption along the segment.
             const int medOwner = media.current();
-            V3 sigmaA;
-            if (medOwner >= 0 && materialInteriorAbsorption(scene.materials[static_cast<size_t>(medOwner)], sigmaA)) {
+            V3 sigmaA, sigmaS;
+            float g = 0.0f;
+            if (medOwner >= 0 && materialInteriorMedium(scene.materials[static_cast<size_t>(medOwner)], sigmaA, sigmaS, g)) {
+                const float dirLen = length(cur.d);
                 const float segment = hit ? length(isect.point - segmentStart) : INFINITY_F;
-                if (hit) throughput *= expNeg(sigmaA, segment);
-                else if (!isBlack(sigmaA)) break;
+                if (!isBlack(sigmaS)) {
+                    float t = 0.0f;
+                    if (sampleFreeFlight(sigmaA + sigmaS, sigmaS, segment, hit, throughput, sampler, throughput, t)) {
+                        if (++mediumEvents > kMaxMediumEvents || !isFinite(throughput)) break;
+                        const V3 d = cur.d / dirLen;
+                        float u0, u1;
+                        sampler.get2D(u0, u1);
+                        Ray next;
+                        next.o = segmentStart + d * t;
+                        next.d = sampleHenyeyGreenstein(d, g, u0, u1);
+                        next.tMin = 0.0f;
+                        next.tMax = 1e30f;
+                        cur = next;
+                        cameraRay = false;
+                        specularBounce = true;
+                        if (mediumEvents > 8 && !russianRoulette(throughput, sampler)) break;
+                        continue;
+                    }
+                } else if (hit) {
+                    throughput *= expNeg(sigmaA, segment);
+                } else if (!isBlack(sigmaA)) {
+                    break;
+                }
                 if (isBlack(throughput) || !isFinite(throughput)) break;
             }
             wo = -cur.d;

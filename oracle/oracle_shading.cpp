@@ -967,12 +967,33 @@ bool materialTransmitsShadows(const pe_material_desc& m) {
 
 bool materialIsDispersive(const pe_material_desc& m) { return m.type == PE_MAT_DIELECTRIC && m.abbe > 0.0f; }
 
+bool materialInteriorMedium(const pe_material_desc& m, V3& sigmaA, V3& sigmaS, float& g) {
+    // interiorMedium (dielectric.h:69-71) of rebuildMedium (dielectric.cpp:136-144): sigma_a from the absorption
+    // colour at its distance (setAbsorption, :123-127), plus (1 - colour) * density of the scattering
+    // (setScattering, :129-134), sigma_s = colour * density; none for thin walls or an empty medium.
+    if (m.type != PE_MAT_DIELECTRIC || m.thin_walled != 0) return false;
+    sigmaA = V3(0.0f);
+    sigmaS = V3(0.0f);
+    g = 0.0f;
+    if (m.absorption_distance > 0.0f) {
+        const V3 color = clamp01(c3(m.absorption_color));
+        const float dist = std::max(m.absorption_distance, 1e-5f);
+        for (int c = 0; c < 3; ++c) sigmaA[c] = -std::log(std::max(color[c], 1e-4f)) / dist;
+    }
+    if (m.scatter_density > 0.0f) {
+        const float dens = m.scatter_density;
+        const V3 sc = clamp01(c3(m.scatter_color));
+        sigmaS = sc * dens;
+        sigmaA = sigmaA + (V3(1.0f) - sc) * dens;
+        g = std::clamp(m.scatter_anisotropy, -0.95f, 0.95f);
+    }
+    return !(isBlack(sigmaA) && isBlack(sigmaS));  // Medium::empty (material.h:25)
+}
+
 bool materialInteriorAbsorption(const pe_material_desc& m, V3& sigmaA) {
-    if (m.type != PE_MAT_DIELECTRIC || m.thin_walled != 0 || !(m.absorption_distance > 0.0f)) return false;
-    const V3 color = clamp01(c3(m.absorption_color));  // setAbsorption (dielectric.cpp:123-127)
-    const float dist = std::max(m.absorption_distance, 1e-5f);
-    for (int c = 0; c < 3; ++c) sigmaA[c] = -std::log(std::max(color[c], 1e-4f)) / dist;
-    return !isBlack(sigmaA);  // Medium::empty (material.h:25)
+    V3 sigmaS;
+    float g;
+    return materialInteriorMedium(m, sigmaA, sigmaS, g) && !isBlack(sigmaA);
 }
 
 V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp, V3 dir) {

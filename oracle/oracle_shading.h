@@ -87,6 +87,7 @@ bool materialTransmitsShadows(const pe_material_desc& m);
 bool materialIsDispersive(const pe_material_desc& m);
 // Dielectric interior medium (Dielectric::rebuildMedium, dielectric.cpp:136-144): absorption only (the scene
 // description carries no scattering). false = none (interiorMedium() == nullptr).
+bool materialInteriorMedium(const pe_material_desc& m, V3& sigmaA, V3& sigmaS, float& g);
 bool materialInteriorAbsorption(const pe_material_desc& m, V3& sigmaA);
 // Dielectric::shadowTransmittance (dielectric.cpp:163-167).
 V3 materialShadowTransmittance(const pe_material_desc& m, const ShadingPoint& sp, V3 dir);

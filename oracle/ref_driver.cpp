@@ -55,6 +55,7 @@ std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
         if (d.absorption_distance > 0.0f) m->setAbsorption(c3(d.absorption_color), d.absorption_distance);
         m->setThinWalled(d.thin_walled != 0);
         if (d.thin_film_thickness > 0.0f) m->setThinFilm(d.thin_film_thickness);
+        if (d.scatter_density > 0.0f) m->setScattering(d.scatter_density, c3(d.scatter_color), d.scatter_anisotropy);
         return m;
     }
     auto m = std::make_shared<DisneyMaterial>(c3(d.base_color), d.metallic, d.roughness, d.specular);

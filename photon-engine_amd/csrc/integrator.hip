@@ -173,6 +173,7 @@ constexpr int kGuideRecords = 2;
 constexpr uint32_t kMaxWalkSteps = 256;   // randomWalkSubsurface's step limit (path_tracer.cpp:270)
 constexpr uint32_t kMaxPassThroughs = 64;  // path_tracer.cpp:783
 constexpr uint32_t kMaxShadowQueries = 64;  // shadowTransmittance (path_tracer.cpp:180)
+constexpr uint32_t kMaxMediumEvents = 1024;  // path_tracer.cpp:51
 
 __device__ __forceinline__ uint32_t ext_pass_throughs(uint32_t w) { return (w & kExtPassMask) >> kExtPassShift; }
 // The coverage of the thin walls a camera ray passed to a transparent background: 1 - the luminance of its
@@ -583,6 +584,40 @@ __device__ __forceinline__ V3 finish_radiance(const LaunchParams& lp, const Path
     return L;
 }
 
+// A segment inside a scattering medium (path_tracer.cpp:457-479): a free flight with spectral MIS towards the
+// next surface at `seg` (or infinity); a scattering event there restarts the path at that point in a
+// Henyey-Greenstein direction about the unit ray direction, counts against the 1024 events a path may have and,
+// past 8 of them, faces Russian roulette. Returns true when the path scattered (and still lives: new_o /
+// new_d hold its ray); false when the segment reached its end, with the throughput attenuated (alive false if
+// it went black or non-finite). One 1D draw for the channel and one for the distance, then a 2D draw for the
+// direction and, past 8 events, one for the roulette.
+__device__ __forceinline__ bool medium_scatter(const GpuMaterial& m, V3 origin, V3 dir, float seg, bool hit,
+                                               SobolState& ss, V3& thr, uint32_t& events, bool& alive, V3& new_o,
+                                               V3& new_d) {
+    float t = 0.0f;
+    if (free_flight(m.sigma_a + m.sigma_s, m.sigma_s, seg, hit, thr, ss, thr, t)) {
+        if (++events > kMaxMediumEvents || !finite3(thr)) {
+            alive = false;
+            return false;
+        }
+        const V3 d = dir / length(dir);
+        float u0, u1;
+        sobol_get2d(ss, u0, u1);
+        new_o = origin + d * t;
+        new_d = sample_hg(d, m.medium_g, u0, u1);
+        if (events > 8) {
+            const float mx = fmaxf(thr.x, fmaxf(thr.y, thr.z));
+            if (mx < 1.0f) {
+                if (!(mx > 0.0f) || sobol_get1d(ss) >= mx) alive = false;
+                else thr = thr / mx;
+            }
+        }
+        return alive;
+    }
+    if (is_black(thr) || !finite3(thr)) alive = false;
+    return false;
+}
+
 // Training records of an ended path (guiding). A record of a guided vertex holds the point, the outgoing and
 // the sampled direction, the throughput before the vertex and the path radiance once the vertex's own NEE had
 // entered it (written at the start of the next vertex: a record is closed once that baseline is in); the
@@ -868,6 +903,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             uint4 ext = make_uint4(0u, 0u, 0u, 0u);
             float4 lp4 = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
             float4 ln4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
+            uint32_t media_events = 0;  // scattering events in media so far (path_tracer.cpp:51, :463), in L.w
             if (cam_entry) {
                 ids = camera_ids(lp, __float_as_uint(d4.w) & ~kCamEntry);
             } else {
@@ -876,6 +912,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 last_pdf = t4.w;
                 const float4 L4 = kStage ? stage[4][src] : pb.L[pid];
                 L = mk(L4.x, L4.y, L4.z);
+                if constexpr (kTransport) media_events = __float_as_uint(L4.w);
                 if constexpr (kStage) {
                     const float4 id4 = stage[5][src];
                     ids = make_uint4(__float_as_uint(id4.x), __float_as_uint(id4.y), __float_as_uint(id4.z),
@@ -977,11 +1014,21 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     float4* d1 = pb.pass_res + static_cast<size_t>(lp.pass_count - 1) * lp.num_paths + g;
                     if (d1->w < 0.0f) *d1 = make_float4(__int_as_float(-1), __int_as_float(-1), 0.0f, 0.0f);
                 }
-                // Miss. Inside an absorbing medium the open path ends (path_tracer.cpp:476-477); otherwise the
+                // Miss. Inside a scattering medium the open segment is a free flight first (path_tracer.cpp:
+                // 457-470); inside a medium that only absorbs the open path ends (:476-477); otherwise the
                 // environment with MIS against NEE (:482-514).
                 const int top = kTransport ? ext_media_top(ext) : -1;
-                const bool absorbed = top >= 0 && (sc.materials[top].flags & kMatFlagMedium);
-                if (absorbed) {
+                bool scattered = false;
+                if (kTransport && top >= 0 && (sc.materials[top].flags & kMatFlagScatter))
+                    scattered = medium_scatter(sc.materials[top], mk(o4.x, o4.y, o4.z), dir, kFloatMax, false, ss, thr,
+                                               media_events, alive, new_o, new_d);
+                const bool absorbed = top >= 0 && !(sc.materials[top].flags & kMatFlagScatter) &&
+                                      (sc.materials[top].flags & kMatFlagMedium);
+                if (scattered || !alive) {
+                    camera = false;
+                    specular = true;
+                    extend = scattered && alive;
+                } else if (absorbed) {
                 } else if (depth == 0 && sc.bg_mode != PE_BACKGROUND_ENVIRONMENT) {
                     // A camera path leaving before its first vertex (past thin walls and hidden lights) sees the
                     // background (path_tracer.cpp:487-495): the Color one credited to the Background pass and
@@ -1001,7 +1048,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     L += added;
                     if (kPasses) credit_found(pb, lp, pid, g, depth, kPassBackground, sc.env_group, added);
                 }
-                alive = false;
+                if (!extend) alive = false;
             } else {
                 fill_hit(sc, tri, dir, h.z, h.w, si);
                 // Camera rays pass through lights hidden from the camera, and any ray through a cutout surface
@@ -1033,15 +1080,26 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             *d1 = make_float4(__int_as_float(sc.shade[tri].shape), __int_as_float(si.material), 0.0f, 0.0f);
                         }
                     }
-                    // Beer-Lambert absorption along the segment inside the current medium (path_tracer.cpp:457-479).
+                    // Inside the current medium (path_tracer.cpp:457-479): a free flight in a scattering one, which
+                    // may scatter before the surface, or Beer-Lambert absorption along the segment.
                     const int top = kTransport ? ext_media_top(ext) : -1;
-                    if (top >= 0 && (sc.materials[top].flags & kMatFlagMedium)) {
+                    bool scattered = false;
+                    if (kTransport && top >= 0 && (sc.materials[top].flags & kMatFlagScatter)) {
+                        const V3 o = mk(o4.x, o4.y, o4.z);
+                        scattered = medium_scatter(sc.materials[top], o, dir, length(si.point - o), true, ss, thr,
+                                                   media_events, alive, new_o, new_d);
+                        if (scattered || !alive) {
+                            camera = false;
+                            specular = true;
+                            extend = scattered && alive;
+                        }
+                    } else if (top >= 0 && (sc.materials[top].flags & kMatFlagMedium)) {
                         const V3 sa = sc.materials[top].sigma_a;
                         const float seg = length(si.point - mk(o4.x, o4.y, o4.z));
                         thr *= mk(expf(-sa.x * seg), expf(-sa.y * seg), expf(-sa.z * seg));
                         if (is_black(thr) || !finite3(thr)) alive = false;
                     }
-                    if (alive) {
+                    if (alive && !scattered) {
                         const GpuMaterial& m = sc.materials[si.material];
                         if (m.flags & kMatFlagEmissive) {  // path_tracer.cpp:565-593
                             const int li = si.emitter;
@@ -1278,7 +1336,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             // its state stays for k_regen to finalise in the next iteration.
             deferred = ended && issued;
             if (extend || deferred) {
-                pb.L[pid] = make_float4(L.x, L.y, L.z, 0.0f);
+                pb.L[pid] = make_float4(L.x, L.y, L.z, kTransport ? __uint_as_float(media_events) : 0.0f);
                 pb.ids[pid] = make_uint4(ids.x, ids.y, ids.z, pack_bits(ss.dim, depth, diffuse, specular, camera));
                 if constexpr (kTransport || kGuide) pb.ext[pid] = ext;
             }

@@ -71,6 +71,9 @@ struct alignas(16) GpuMaterial {
     float pad7;
     V3 film_metal_k;
     float pad8;
+    // Dielectric interior scattering (setScattering, dielectric.cpp:129-144): sigma_s and the Henyey-Greenstein g
+    V3 sigma_s;
+    float medium_g;
 };
 
 // Shading code paths the shade kernel groups a round's hits by (when a scene has more than one).
@@ -88,6 +91,7 @@ enum : uint32_t {
     kMatFlagMedium = 128u,           // interior medium absorbs (sigma_a != 0)
     kMatFlagFilm = 256u,             // thin-film interference on the specular Fresnel / the thin wall
     kMatFlagCutout = 512u,           // Disney opacity below 1: stochastic cutout, shadow transmittance 1 - opacity
+    kMatFlagScatter = 1024u,         // interior medium scatters (sigma_s != 0): free flights inside
 };
 
 // AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.

@@ -69,8 +69,16 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
             const float dist = std::fmax(d.absorption_distance, 1e-5f);
             g.sigma_a = mk(-std::log(std::fmax(color.x, 1e-4f)) / dist, -std::log(std::fmax(color.y, 1e-4f)) / dist,
                            -std::log(std::fmax(color.z, 1e-4f)) / dist);
-            if (!is_black(g.sigma_a)) g.flags |= kMatFlagMedium;
         }
+        if (!thin && d.scatter_density > 0.0f) {  // rebuildMedium's scattering terms (dielectric.cpp:141-143)
+            const float dens = d.scatter_density;
+            const V3 sc = clamp01v(c3(d.scatter_color));
+            g.sigma_s = sc * dens;
+            g.sigma_a = g.sigma_a + (splat(1.0f) - sc) * dens;
+            g.medium_g = clampf(d.scatter_anisotropy, -0.95f, 0.95f);
+            if (!is_black(g.sigma_s)) g.flags |= kMatFlagScatter;
+        }
+        if (!thin && !is_black(g.sigma_a)) g.flags |= kMatFlagMedium;
         return PE_OK;
     }
     if (d.type != PE_MAT_DISNEY) {
@@ -459,7 +467,7 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
                                  g.p_sss > 0.0f;
         transport = transport || g.type == kMatDielectric || lobes_below ||
                     (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium |
-                                kMatFlagFilm | kMatFlagCutout)) != 0;
+                                kMatFlagFilm | kMatFlagCutout | kMatFlagScatter)) != 0;
         g.sort_class = g.type == kMatDielectric ? kSortClassDielectric
                        : g.type != kMatDisney   ? kSortClassSimple
                        : (lobes_below || (g.flags & (kMatFlagFilm | kMatFlagCutout))) ? kSortClassDisneyTransport

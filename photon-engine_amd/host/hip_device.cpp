@@ -155,6 +155,9 @@ bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
         o.absorption_distance = g->absorptionDistance();
         o.thin_walled = g->thinWalled() ? 1 : 0;
         o.thin_film_thickness = g->thinFilmThickness();
+        o.scatter_density = g->scatterDensity();  // setScattering (dielectric.h:61-66)
+        putc(o.scatter_color, g->scatterColor());
+        o.scatter_anisotropy = g->scatterAnisotropy();
         return true;
     }
     err = "material type not implemented on the HIP device";

@@ -57,6 +57,7 @@ class MaterialDesc(C.Structure):
         ("edge_tint", F3), ("thin_film_thickness", C.c_float), ("thin_film_ior", C.c_float),
         ("abbe", C.c_float), ("absorption_color", F3), ("absorption_distance", C.c_float),
         ("thin_walled", C.c_int32), ("opacity", C.c_float), ("has_opacity", C.c_int32),
+        ("scatter_density", C.c_float), ("scatter_color", C.c_float * 3), ("scatter_anisotropy", C.c_float),
         ("reserved", C.c_float * 3),
     ]
 

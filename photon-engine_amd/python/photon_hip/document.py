@@ -17,8 +17,8 @@ What each part follows:
   * read_mesh_binary / write_mesh_binary: scene/mesh_binary.cpp (PHMESH01);
   * read_hdr: the Radiance RGBE format (flat and run-length scanlines) decoded as stb_image decodes it.
 
-What the HIP path does not render is refused with DocumentError, not approximated: texture maps, flakes,
-rounded edges and scattering glass; point, spot and directional lights and light linking;
+What the HIP path does not render is refused with DocumentError, not approximated: texture maps, flakes and
+rounded edges; point, spot and directional lights and light linking;
 the physical sky; shadow catchers; OpenEXR, glTF, FBX, Alembic and USD files. Analytic sphere nodes are refused
 unless the caller asks for them tessellated (sphere_segments), as the reference's USD arm of its golden test
 renders them.
@@ -445,8 +445,10 @@ def _material(j: dict, base: Path):
             m.absorption_color[:] = list(_vec3(ab, "color", (1, 1, 1)))
             m.absorption_distance = _f(ab, "distance", 1.0)
         sc = j.get("scattering")
-        if isinstance(sc, dict) and _f(sc, "density", 0.0) > 0.0:
-            why.append("a scattering medium")
+        if isinstance(sc, dict):
+            m.scatter_density = _f(sc, "density", 0.0)
+            m.scatter_color[:] = list(_vec3(sc, "color", (1, 1, 1)))
+            m.scatter_anisotropy = _f(sc, "anisotropy", 0.0)
         m.abbe = _f(j, "abbe", 0.0)
         return m, ", ".join(why) or None
     if t == "lambert":

@@ -38,6 +38,7 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
     m.thin_film_thickness, m.thin_film_ior = 0.0, 1.4
     m.absorption_color[:] = [1.0, 1.0, 1.0]
     m.absorption_distance = 0.0
+    m.scatter_color[:] = [1.0, 1.0, 1.0]
     if "opacity" in kw:                      # setOpacity (disney.h:148)
         m.has_opacity = 1
     for k, v in kw.items():
@@ -50,10 +51,15 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
 
 
 def dielectric(ior=1.5, tint=(1.0, 1.0, 1.0), roughness=0.0, abbe=0.0, absorption_color=None,
-               absorption_distance=0.0, thin_walled=False, thin_film_thickness=0.0) -> capi.MaterialDesc:
+               absorption_distance=0.0, thin_walled=False, thin_film_thickness=0.0, scatter_density=0.0,
+               scatter_color=(1.0, 1.0, 1.0), scatter_anisotropy=0.0) -> capi.MaterialDesc:
     """Dielectric(ior, tint, roughness) (dielectric.h:18-20) with setAbbe (:74), setAbsorption(color, distance)
-    (:57) when absorption_color is given, setThinWalled (:43) and setThinFilm(thickness nm) (:46)."""
+    (:57) when absorption_color is given, setThinWalled (:43), setThinFilm(thickness nm) (:46) and
+    setScattering(density, color, anisotropy) (:61)."""
     m = disney(tint)
+    m.scatter_density = float(scatter_density)
+    m.scatter_color[:] = list(map(float, scatter_color))
+    m.scatter_anisotropy = float(scatter_anisotropy)
     m.type = capi.PE_MAT_DIELECTRIC
     m.ior, m.roughness, m.abbe = float(ior), float(roughness), float(abbe)
     if absorption_color is not None:

@@ -1,8 +1,9 @@
 """Transmission, volumes and subsurface on the integrator path: Dielectric (smooth, rough, thin-walled,
 dispersive; materials/dielectric.cpp), wavelength sampling and Cauchy dispersion (core/color/spectral.cpp),
-the medium stack with Beer-Lambert absorption, pass-through and shadow transmittance, and the subsurface random
-walk (integrators/path_tracer.cpp:63-292, :457-479, :602-607, :744-786), plus Disney specular/diffuse
-transmission (materials/disney.cpp:183-220).
+the medium stack with Beer-Lambert absorption and scattering media (free flights with spectral MIS,
+sampleFreeFlight :97-129), pass-through and shadow transmittance, and the subsurface random walk
+(integrators/path_tracer.cpp:63-292, :457-479, :602-607, :744-786), plus Disney specular/diffuse transmission
+(materials/disney.cpp:183-220).
 
 CPU: the oracle against the reference's own strict build, sample for sample, on a studio that holds each of
 these (and separately per feature, so a regression names its feature); the dielectric BSDF bit for bit;
@@ -47,6 +48,19 @@ FEATURES = {
                                          sc.add_material(dielectric(1.5, tint=(0.9, 0.95, 1.0), thin_walled=True))),
     "rough_glass": lambda sc: sc.add_shape(icosphere(2, 0.25, (1.15, 0.25, -0.55)),
                                            sc.add_material(dielectric(1.45, tint=(1.0, 0.97, 0.9), roughness=0.25))),
+    # A milky glass block (setScattering: density 12 per metre, so a mean free path of 8 cm in a 50 cm block,
+    # forward scattering) and a scattering, absorbing liquid: free flights with spectral MIS, Henyey-Greenstein
+    # directions, roulette past 8 events.
+    "scattering_block": lambda sc: sc.add_shape(box_mesh((0.3, 0.002, -0.25), (0.8, 0.5, 0.25)),
+                                                sc.add_material(dielectric(1.45, roughness=0.15, scatter_density=12.0,
+                                                                           scatter_color=(0.95, 0.8, 0.6),
+                                                                           scatter_anisotropy=0.4))),
+    "scattering_liquid": lambda sc: sc.add_shape(box_mesh((-0.45, 0.002, -0.2), (0.05, 0.6, 0.3)),
+                                                 sc.add_material(dielectric(1.33, absorption_color=(0.8, 0.9, 0.6),
+                                                                            absorption_distance=0.2,
+                                                                            scatter_density=30.0,
+                                                                            scatter_color=(0.9, 0.9, 0.95),
+                                                                            scatter_anisotropy=-0.3))),
     "disney_transmission": lambda sc: sc.add_shape(icosphere(2, 0.22, (-0.2, 0.85, 0.6)),
                                                    sc.add_material(disney((0.5, 0.8, 0.4), 0.0, 0.35, 0.5,
                                                                           specular_transmission=0.6,
@@ -204,7 +218,8 @@ def test_gpu_glass_studio_matches_oracle_per_sample(dev):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["glass_studio", "subsurface_block", "thin_pane", "absorbing_liquid"])
+@pytest.mark.parametrize("name", ["glass_studio", "subsurface_block", "thin_pane", "absorbing_liquid",
+                                  "scattering_block", "scattering_liquid"])
 def test_gpu_image_matches_oracle_relmse(dev, name):
     if name == "glass_studio":
         sc, view = configs.glass_studio(led_quads=100)
