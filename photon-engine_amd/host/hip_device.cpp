@@ -9,8 +9,8 @@
 // What the HIP path does not implement is refused, never approximated: setScene leaves the device without a
 // scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle / static
 // MeshInstance, visibility masks other than all rays or all but camera rays, shadow catchers, lights other
-// than AreaLight / TriangleLight, textured or flaked Disney materials and materials other than Disney /
-// Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO and
+// than AreaLight / TriangleLight, light linking, rounded edges, textured or flaked Disney materials and
+// materials other than Disney / Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO and
 // motion blur. Path guiding uses the library's learned guide in place of the reference's SD-tree.
 #include "hip_device.h"
 
@@ -96,6 +96,10 @@ struct PackedScene {
 
 bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
     o = pe_material_desc{};
+    if (m->roundEdgeRadius() > 0.0f) {  // roundedEdgeNormal (path_tracer.cpp:612-615)
+        err = "rounded edges (Material::roundEdgeRadius) are not implemented on the HIP device";
+        return false;
+    }
     SurfaceInteraction si;
     putc(o.emission, m->emitted(si));
     if (auto* d = dynamic_cast<const DisneyMaterial*>(m)) {
@@ -182,6 +186,10 @@ bool packScene(const Scene& scene, PackedScene& p, std::string& err) {
     // Lights in Scene::lights() order: the light index is part of the sampling sequence.
     for (const Light* L : scene.lights()) {
         pe_light_desc d{};
+        if (!L->lightLink().isDefault()) {  // Scene::isLightLinked / isShadowLinked (scene.cpp:242-256)
+            err = "light linking is not implemented on the HIP device";
+            return false;
+        }
         if (auto* a = dynamic_cast<const AreaLight*>(L)) {
             d.type = PE_LIGHT_AREA;
             put3(d.p0, a->position());
