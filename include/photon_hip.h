@@ -211,13 +211,36 @@ int pe_scene_release(pe_scene* scene);
 int pe_scene_stats(const pe_scene* scene, uint64_t* triangles, uint64_t* bvh_nodes, uint64_t* device_bytes);
 
 /* ---- camera / settings ------------------------------------------------------------------------------
- * pe_camera is CameraGpuDesc (camera/camera.h:19-31) of a pinhole PerspectiveCamera. */
+ * pe_camera is CameraGpuDesc (camera/camera.h:19-31) of a PerspectiveCamera, ThinLensCamera or
+ * OrthographicCamera, plus the shutter-end pose Camera::setShutterEnd records (camera.h:49-54). A camera ray
+ * (Camera::generateRay, camera.h:57-91) for screen point (s, t) and lens sample (lu, lv):
+ *   orthographic          origin lower_left + s horizontal + t vertical, direction forward
+ *   lens_radius = 0       origin position, towards lower_left + s horizontal + t vertical
+ *   lens_radius > 0       origin position + radius (x u + y v) with (x, y) uniform on the unit disk (blades < 3)
+ *                         or on the regular blades-gon turned by blade_rotation (thin_lens_camera.cpp:36-61)
+ * With pe_settings.motion_blur and has_shutter_end, a time t in [0, 1] drawn after the lens sample blends every
+ * field linearly towards the end_* pose (the aperture is then always a disk, as camera.h:75-84 samples it). */
 typedef struct pe_camera {
     float position[3];
     float lower_left[3];
     float horizontal[3];
     float vertical[3];
-    float lens_radius;          /* must be 0: only the pinhole camera is implemented */
+    float lens_radius;          /* 0: pinhole */
+    float u[3];                 /* lens axes (unit right and up) */
+    float v[3];
+    float forward[3];           /* view direction of an orthographic camera */
+    int32_t blades;             /* aperture polygon (>= 3) or 0 for a disk */
+    float blade_rotation;       /* radians */
+    int32_t orthographic;
+    int32_t has_shutter_end;
+    float end_position[3];      /* the pose at shutter close (CameraGpuDesc of the end camera) */
+    float end_lower_left[3];
+    float end_horizontal[3];
+    float end_vertical[3];
+    float end_u[3];
+    float end_v[3];
+    float end_forward[3];
+    float end_lens_radius;
 } pe_camera;
 
 #define PE_FILTER_BOX 0             /* core/image/pixel_filter.h PixelFilterType */
@@ -248,7 +271,10 @@ typedef struct pe_settings {
     int32_t min_adaptive_samples; /* at least 2 (reference default 16) */
     int32_t paths_in_flight;    /* path pool: paths traced together, a finished path's id taking the window's next
                                    sample at once; 0 = 2^26 (never more than the window's samples) */
-    int32_t reserved[2];
+    int32_t motion_blur;        /* RenderSettings::motionBlur: every camera ray draws a shutter time after its lens
+                                   sample (renderer.cpp:151-153); only the camera moves (moving instances are
+                                   refused at upload) */
+    int32_t reserved;
 } pe_settings;
 
 /* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray and traversal counters. */
@@ -286,6 +312,22 @@ int pe_film_download(pe_film* film, float* sum_rgb, uint32_t* count, float* sum_
  * (camera/perspective_camera.cpp:11-34, camera.h:19-31). */
 int pe_camera_perspective(const float position[3], const float target[3], const float up[3], float vfov_degrees,
                           float aspect, pe_camera* out);
+/* ThinLensCamera(position, target, up, vFovDegrees, aspect, apertureRadius, focusDistance) followed by
+ * setApertureShape(blades, rotationDegrees) -> CameraGpuDesc (camera/thin_lens_camera.cpp:14-33, :63-77,
+ * thin_lens_camera.h:31-34). */
+int pe_camera_thin_lens(const float position[3], const float target[3], const float up[3], float vfov_degrees,
+                        float aspect, float aperture_radius, float focus_distance, int blades, float rotation_degrees,
+                        pe_camera* out);
+/* OrthographicCamera(position, target, up, verticalExtent, aspect) -> CameraGpuDesc
+ * (camera/orthographic_camera.cpp:11-40). */
+int pe_camera_orthographic(const float position[3], const float target[3], const float up[3], float vertical_extent,
+                           float aspect, pe_camera* out);
+/* PerspectiveCamera::setShift / ThinLensCamera::setShift (perspective_camera.h:25-27, thin_lens_camera.h:25-27):
+ * a lens shift moves the image rectangle by shift_x image heights along u and shift_y along vertical. */
+int pe_camera_set_shift(pe_camera* camera, float shift_x, float shift_y);
+/* Camera::setShutterEnd(endCamera) (camera.h:49-54): `camera` keeps its pose at shutter open and takes `end`'s
+ * CameraGpuDesc as its pose at shutter close. */
+int pe_camera_set_shutter_end(pe_camera* camera, const pe_camera* end);
 
 /* RenderDevice::render: adds samples [first_sample, first_sample + sample_count) of every pixel into the
  * film. cancel (may be NULL) is polled between iterations of the wavefront (a few milliseconds apart); a set

@@ -68,6 +68,7 @@ _ORACLE_SIGS = {
                              _F]),
     "trace_passes": (C.c_int, [_VP, _P(capi.Camera), _P(capi.Settings), C.c_int, C.c_int, C.c_size_t, _I, _I, _I,
                                _F, _F]),
+    "camera_rays": (C.c_int, [_P(capi.Camera), C.c_size_t, _F, _F]),
 }
 
 _REF_SIGS = {
@@ -78,6 +79,11 @@ _REF_SIGS = {
     "intersect": (C.c_int, [_VP, C.c_size_t, _F, _I, _F]),
     "occluded": (C.c_int, [_VP, C.c_size_t, _F, _P(C.c_uint8)]),
     "camera_describe": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, _F]),
+    "camera_select": (C.c_int, [C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, _F,
+                                _F, C.c_float]),
+    "camera_describe_full": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, _F]),
+    "camera_shift": (C.c_int, [C.c_float, C.c_float]),
+    "camera_rays": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
     "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
     "bsdf_eval": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F, _F]),
     "bsdf_sample": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F]),
@@ -291,6 +297,66 @@ class OracleScene:
         if rc != 0:
             raise RuntimeError("oracle_render failed (feature outside the restated scope)")
         return film, secs.value, (rays[0], rays[1])
+
+
+CAMERA_KINDS = {"perspective": 0, "thin_lens": 1, "orthographic": 2}
+
+
+def camera_rays_oracle(camera, q):
+    """The oracle's camera ray for q (n, 5): u, v, lens x, lens y, time (< 0: no shutter time) -> (n, 6)."""
+    q = np.ascontiguousarray(q, np.float32)
+    out = np.zeros((q.shape[0], 6), np.float32)
+    oracle().oracle_camera_rays(C.byref(camera), q.shape[0], _fp(q), _fp(out))
+    return out
+
+
+class reference_camera:
+    """Context manager: the reference driver builds its cameras as ThinLensCamera (aperture, focus, blades,
+    rotation_deg) or OrthographicCamera (extent) instead of PerspectiveCamera, optionally with a shutter-end pose of
+    the same kind at end_position / end_target (end_aperture) and the motion-blur flag; back to the pinhole on exit."""
+
+    def __init__(self, strict: bool = False, kind="perspective", aperture=0.0, focus=1.0, blades=0, rotation_deg=0.0,
+                 extent=2.0, motion_blur=False, end_position=None, end_target=None, end_aperture=0.0,
+                 shift=(0.0, 0.0), harness: bool = False):
+        """harness: select the camera of the HipDevice harness (its own copy of the reference driver)."""
+        self.lib = hip_device_harness() if harness else reference(strict)
+        if self.lib is None:
+            raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
+        if harness:
+            for name in ("camera_select", "camera_shift"):
+                getattr(self.lib, "ref_" + name).argtypes = _REF_SIGS[name][1]
+                getattr(self.lib, "ref_" + name).restype = C.c_int
+        self.shift = (float(shift[0]), float(shift[1]))
+        self.args = (CAMERA_KINDS[kind], float(aperture), float(focus), int(blades), float(rotation_deg),
+                     float(extent), int(bool(motion_blur)), int(end_position is not None),
+                     (C.c_float * 3)(*map(float, end_position or (0, 0, 0))),
+                     (C.c_float * 3)(*map(float, end_target or (0, 0, -1))), float(end_aperture))
+
+    def __enter__(self):
+        self.lib.ref_camera_select(*self.args)
+        self.lib.ref_camera_shift(*self.shift)
+        return self
+
+    def __exit__(self, *exc):
+        self.lib.ref_camera_select(0, 0.0, 1.0, 0, 0.0, 2.0, 0, 0, None, None, 0.0)
+        return False
+
+    def describe(self, view):
+        """describeGpu() and describeGpuEnd() of the selected camera: (2, 22) floats, then blades, rotation, ortho."""
+        out = np.zeros(47, np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self.lib.ref_camera_describe_full(f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect,
+                                          _fp(out))
+        return out
+
+    def rays(self, view, q):
+        """Camera::generateRay for q (n, 5): u, v, lens x, lens y, time (< 0: the three-argument form) -> (n, 6)."""
+        q = np.ascontiguousarray(q, np.float32)
+        out = np.zeros((q.shape[0], 6), np.float32)
+        f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+        self.lib.ref_camera_rays(f3(view.position), f3(view.target), f3(view.up), view.vfov, view.aspect, q.shape[0],
+                                 _fp(q), _fp(out))
+        return out
 
 
 class RefScene:

@@ -23,6 +23,11 @@ void ref_copy_passes(const RenderPassFilms& pf, int width, int height, float* ou
 
 #define HIPDEV_API __attribute__((visibility("default")))
 
+// The camera the reference driver (ref_driver.cpp, linked into the same library) builds: ref_camera_select's
+// choice of PerspectiveCamera, ThinLensCamera or OrthographicCamera with its shutter end, and its motion-blur flag.
+std::unique_ptr<Camera> makeCamera(const float* pos, const float* target, const float* up, float fov, float aspect);
+bool selectedCameraMotionBlur();
+
 namespace {
 void copy_error(const std::string& msg, char* err, int errlen) {
     if (!err || errlen <= 0) return;
@@ -53,9 +58,7 @@ HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* 
         copy_error(dev->lastError(), err, errlen);
         return 1;
     }
-    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
-                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
-                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    const std::shared_ptr<Camera> cam = makeCamera(camPos, camTarget, camUp, fov, aspect);
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -64,6 +67,7 @@ HIPDEV_API int hipdev_session_render(void* h, const float* camPos, const float* 
     s.shadowQuality = shadowQuality;
     s.filter.type = static_cast<PixelFilterType>(filterType);
     s.filter.width = filterWidth;
+    s.motionBlur = selectedCameraMotionBlur();
     s.aoStrength = aoStrength;
     s.pathGuiding = pathGuiding != 0;
     SessionConfig cfg;
@@ -112,9 +116,7 @@ HIPDEV_API int hipdev_session_render_adaptive(void* h, const float* camPos, cons
         copy_error(dev->lastError(), err, errlen);
         return 1;
     }
-    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
-                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
-                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    const std::shared_ptr<Camera> cam = makeCamera(camPos, camTarget, camUp, fov, aspect);
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -122,6 +124,7 @@ HIPDEV_API int hipdev_session_render_adaptive(void* h, const float* camPos, cons
     s.maxBounces = maxBounces;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = selectedCameraMotionBlur();
     s.adaptiveSampling = true;
     s.noiseThreshold = threshold;
     s.minAdaptiveSamples = minSpp;
@@ -171,9 +174,7 @@ HIPDEV_API int hipdev_session_render_aovs(void* h, const float* camPos, const fl
         copy_error(dev->lastError(), err, errlen);
         return 1;
     }
-    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
-                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
-                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    const std::shared_ptr<Camera> cam = makeCamera(camPos, camTarget, camUp, fov, aspect);
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -181,6 +182,7 @@ HIPDEV_API int hipdev_session_render_aovs(void* h, const float* camPos, const fl
     s.maxBounces = maxBounces;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = selectedCameraMotionBlur();
     SessionConfig cfg;
     cfg.mode = RenderMode::Final;
     cfg.targetSpp = spp;
@@ -235,9 +237,7 @@ HIPDEV_API int hipdev_session_render_passes(void* h, const float* camPos, const 
         copy_error(dev->lastError(), err, errlen);
         return 1;
     }
-    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
-                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
-                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    const std::shared_ptr<Camera> cam = makeCamera(camPos, camTarget, camUp, fov, aspect);
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -245,6 +245,7 @@ HIPDEV_API int hipdev_session_render_passes(void* h, const float* camPos, const 
     s.maxBounces = maxBounces;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = selectedCameraMotionBlur();
     SessionConfig cfg;
     cfg.mode = RenderMode::Final;
     cfg.targetSpp = spp;
@@ -321,9 +322,7 @@ HIPDEV_API int hipdev_session_interactive(void* h, const float* camPos, const fl
         copy_error(dev->lastError(), err, errlen);
         return 1;
     }
-    auto cam = std::make_shared<PerspectiveCamera>(Vec3f(camPos[0], camPos[1], camPos[2]),
-                                                   Vec3f(camTarget[0], camTarget[1], camTarget[2]),
-                                                   Vec3f(camUp[0], camUp[1], camUp[2]), fov, aspect);
+    const std::shared_ptr<Camera> cam = makeCamera(camPos, camTarget, camUp, fov, aspect);
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -331,6 +330,7 @@ HIPDEV_API int hipdev_session_interactive(void* h, const float* camPos, const fl
     s.maxBounces = maxBounces;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = selectedCameraMotionBlur();
     {
         SessionConfig cfg;
         cfg.mode = RenderMode::Interactive;

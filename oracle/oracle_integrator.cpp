@@ -1016,6 +1016,7 @@ LightContext prepare(const Integrator& it, V3 p, V3 n) {  // light_sampler.cpp:1
 struct Settings {
     int maxDepth = 8, rrDepth = 3, shadowSamples = 1;
     float clampIndirect = 0.0f;
+    bool motionBlur = false;  // RenderSettings::motionBlur: tracePixel draws a shutter time (renderer.cpp:152)
 };
 
 struct RayCounters {
@@ -1586,6 +1587,70 @@ V3 idColor(uint64_t id) {
     }
 }
 
+// ThinLensCamera::samplePolygon (camera/thin_lens_camera.cpp:36-47): uniform on the regular n-gon inscribed in
+// the unit circle, turned by `rot`; the slice from u0, uniformSampleTriangle (sampling.h:97-100) within it.
+void samplePolygon(int blades, float rot, float u0, float u1, float& x, float& y) {
+    const float n = static_cast<float>(blades);
+    const float scaled = std::min(u0 * n, n - 1e-4f);
+    const float k = std::floor(scaled);
+    const float ux = scaled - k;
+    const float a0 = rot + 6.28318530717958647692f * k / n;
+    const float a1 = a0 + 6.28318530717958647692f / n;
+    const float su0 = std::sqrt(ux);
+    const float b0 = 1.0f - su0, b1 = u1 * su0;
+    x = std::cos(a0) * b0 + std::cos(a1) * b1;
+    y = std::sin(a0) * b0 + std::sin(a1) * b1;
+}
+
+// Camera::generateRay(u, v, lensSample, time) (camera/camera.h:57-91): at time 0, or without motion blur or a
+// shutter-end pose, the camera's own generateRay (PerspectiveCamera perspective_camera.cpp:36-40, ThinLensCamera
+// thin_lens_camera.cpp:49-61, OrthographicCamera orthographic_camera.cpp:25-28); otherwise the two poses blended
+// linearly, a disk aperture of the blended radius on the blended, renormalised lens axes.
+Ray generateCameraRay(const pe_camera& c, bool motionBlur, float sx, float sy, float lx, float ly, float time) {
+    Ray ray;
+    if (!motionBlur || !c.has_shutter_end || time <= 0.0f) {
+        const V3 pos = c3(c.position), ll = c3(c.lower_left), hor = c3(c.horizontal), ver = c3(c.vertical);
+        const V3 target = ll + sx * hor + sy * ver;
+        if (c.orthographic) {
+            ray.o = target;
+            ray.d = c3(c.forward);
+            return ray;
+        }
+        ray.o = pos;
+        if (c.lens_radius > 0.0f) {
+            float x, y;
+            if (c.blades >= 3) samplePolygon(c.blades, c.blade_rotation, lx, ly, x, y);
+            else uniformSampleDisk(lx, ly, x, y);
+            x *= c.lens_radius;
+            y *= c.lens_radius;
+            ray.o = pos + c3(c.u) * x + c3(c.v) * y;
+        }
+        ray.d = normalized(target - ray.o);
+        return ray;
+    }
+    const float t = std::min(time, 1.0f);
+    const auto mix3 = [t](V3 a, V3 b) { return a * (1.0f - t) + b * t; };
+    const V3 pos = mix3(c3(c.position), c3(c.end_position));
+    const V3 ll = mix3(c3(c.lower_left), c3(c.end_lower_left));
+    const V3 hor = mix3(c3(c.horizontal), c3(c.end_horizontal));
+    const V3 ver = mix3(c3(c.vertical), c3(c.end_vertical));
+    if (c.orthographic) {
+        ray.o = ll + hor * sx + ver * sy;
+        ray.d = normalized(mix3(c3(c.forward), c3(c.end_forward)));
+        return ray;
+    }
+    const float radius = c.lens_radius * (1.0f - t) + c.end_lens_radius * t;
+    ray.o = pos;
+    if (radius > 0.0f) {
+        const V3 cu = normalized(mix3(c3(c.u), c3(c.end_u))), cv = normalized(mix3(c3(c.v), c3(c.end_v)));
+        float x, y;
+        uniformSampleDisk(lx, ly, x, y);
+        ray.o = pos + cu * (x * radius) + cv * (y * radius);
+    }
+    ray.d = normalized(ll + hor * sx + ver * sy - ray.o);
+    return ray;
+}
+
 struct Tracer {
     const Scene& scene;
     const Integrator& it;
@@ -1595,12 +1660,9 @@ struct Tracer {
     int width, height;
     uint64_t seed;
 
-    // tracePixel (renderer.cpp:139-162): filter offset (dim 0), lens sample (dim 1), Li; with `aov`, the AOVs
-    // (:175-205): the primary hit's clamped albedo (primaryAlbedo, :47-58: Disney baseColor, Lambert albedo,
-    // Dielectric tint), its shading normal and coverage 1 (the thin walls' coverage where the camera ray passed
-    // them to a transparent background), or on a miss the clamped Color background or environment radiance
-    // along the camera ray, a zero normal and coverage 0.
-    V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* aov = nullptr) const {
+    // tracePixel's camera ray (renderer.cpp:143-153): filter offset (dim 0), lens sample (dim 1) and, with motion
+    // blur, the shutter time (dim 2), then Camera::generateRay (camera.h:57-91) of the camera pe_camera describes.
+    Ray cameraRay(int x, int y, int s, Sobol& sampler) const {
         sampler.startPixel(x, y);
         sampler.startSample(s);
         float fu, fv;
@@ -1610,11 +1672,17 @@ struct Tracer {
         const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(height);
         float lx, ly;
         sampler.get2D(lx, ly);
-        const V3 pos = c3(cam.position), ll = c3(cam.lower_left), hor = c3(cam.horizontal), ver = c3(cam.vertical);
-        const V3 target = ll + sx * hor + sy * ver;  // perspective_camera.cpp:36-40
-        Ray ray;
-        ray.o = pos;
-        ray.d = normalized(target - pos);
+        const float time = st.motionBlur ? sampler.get1D() : 0.0f;
+        return generateCameraRay(cam, st.motionBlur, sx, sy, lx, ly, time);
+    }
+
+    // tracePixel (renderer.cpp:139-162): the camera ray, then Li; with `aov`, the AOVs
+    // (:175-205): the primary hit's clamped albedo (primaryAlbedo, :47-58: Disney baseColor, Lambert albedo,
+    // Dielectric tint), its shading normal and coverage 1 (the thin walls' coverage where the camera ray passed
+    // them to a transparent background), or on a miss the clamped Color background or environment radiance
+    // along the camera ray, a zero normal and coverage 0.
+    V3 trace(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* aov = nullptr) const {
+        const Ray ray = cameraRay(x, y, s, sampler);
         if (!aov) return Li(scene, it, st, ray, sampler, rc);
         PrimaryHit primary;
         const V3 L = Li(scene, it, st, ray, sampler, rc, &primary);
@@ -1644,20 +1712,7 @@ struct Tracer {
     // id rgb, material id rgb, then the hit's shape and material index (-1 for a miss; the cryptomatte lookup
     // keys): 84 floats.
     V3 tracePasses(int x, int y, int s, Sobol& sampler, RayCounters& rc, float* out) const {
-        sampler.startPixel(x, y);
-        sampler.startSample(s);
-        float fu, fv;
-        sampler.get2D(fu, fv);
-        const float ox = filter.sample1D(fu), oy = filter.sample1D(fv);
-        const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(width);
-        const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(height);
-        float lx, ly;
-        sampler.get2D(lx, ly);
-        const V3 pos = c3(cam.position), ll = c3(cam.lower_left), hor = c3(cam.horizontal), ver = c3(cam.vertical);
-        const V3 target = ll + sx * hor + sy * ver;
-        Ray ray;
-        ray.o = pos;
-        ray.d = normalized(target - pos);
+        const Ray ray = cameraRay(x, y, s, sampler);
         PrimaryHit primary;
         LightPasses lp;
         const V3 L = Li(scene, it, st, ray, sampler, rc, &primary, &lp);
@@ -1768,6 +1823,18 @@ ORACLE_API int oracle_light_select(void* sp, size_t n, const float* p, const flo
     }
 }
 
+// generateCameraRay for n queries (in: u, v, lens x, lens y, time; time < 0 is the form without a time) ->
+// origin and direction (6 floats each).
+ORACLE_API int oracle_camera_rays(const pe_camera* cam, size_t n, const float* in, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const float* q = in + 5 * i;
+        const Ray r = generateCameraRay(*cam, q[4] >= 0.0f, q[0], q[1], q[2], q[3], q[4] >= 0.0f ? q[4] : 0.0f);
+        const float v[6] = {r.o.x, r.o.y, r.o.z, r.d.x, r.d.y, r.d.z};
+        std::copy(v, v + 6, out + 6 * i);
+    }
+    return 0;
+}
+
 // Per-sample radiance (tracePixel + Li) for n (px, py, sample) triples.
 ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_settings* ps, int width, int height,
                                     size_t n, const int32_t* px, const int32_t* py, const int32_t* s, float* L,
@@ -1780,6 +1847,7 @@ ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_set
         st.rrDepth = ps->rr_depth;
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        st.motionBlur = ps->motion_blur != 0;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1811,6 +1879,7 @@ ORACLE_API int oracle_trace_aovs(void* sp, const pe_camera* cam, const pe_settin
         st.rrDepth = ps->rr_depth;
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        st.motionBlur = ps->motion_blur != 0;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1838,6 +1907,7 @@ ORACLE_API int oracle_trace_passes(void* sp, const pe_camera* cam, const pe_sett
         st.rrDepth = ps->rr_depth;
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        st.motionBlur = ps->motion_blur != 0;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1867,6 +1937,7 @@ ORACLE_API int oracle_render(void* sp, const pe_camera* cam, const pe_settings* 
         st.rrDepth = ps->rr_depth;
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
+        st.motionBlur = ps->motion_blur != 0;
         const Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         if (row_end <= row_begin) {
             row_begin = 0;

@@ -9,7 +9,9 @@
 #include "engine/denoiser.h"
 #include "engine/renderer.h"
 #include "engine/scene.h"
+#include "camera/orthographic_camera.h"
 #include "camera/perspective_camera.h"
+#include "camera/thin_lens_camera.h"
 #include "core/image/pixel_filter.h"
 #include "core/image/cryptomatte.h"
 #include "core/image/imager.h"
@@ -107,6 +109,53 @@ SurfaceInteraction shadingPoint(const float* n, const float* t, int frontFace) {
 }  // namespace
 
 #define REF_API __attribute__((visibility("default")))
+
+// The camera every function below builds from (position, target, up, fov, aspect): a PerspectiveCamera unless
+// ref_camera_select chose a ThinLensCamera (with setApertureShape) or an OrthographicCamera (vertical extent =
+// `fov`'s argument is ignored for it), optionally with a shutter-end pose (Camera::setShutterEnd) of the same kind
+// at another position / target, and the motion-blur flag tracePixel reads.
+struct CameraSpec {
+    int kind = 0;  // 0 perspective, 1 thin lens, 2 orthographic
+    float aperture = 0.0f, focus = 1.0f, extent = 2.0f, rotationDeg = 0.0f;
+    int blades = 0;
+    bool hasEnd = false, motionBlur = false;
+    float endPos[3] = {0, 0, 0}, endTarget[3] = {0, 0, -1}, endAperture = 0.0f;
+    float shiftX = 0.0f, shiftY = 0.0f;  // setShift of a perspective or thin-lens camera (both poses)
+};
+CameraSpec g_camera;
+
+std::unique_ptr<Camera> makeCameraAt(const float* pos, const float* target, const float* up, float fov, float aspect,
+                                     float aperture) {
+    switch (g_camera.kind) {
+        case 1: {
+            auto c = std::make_unique<ThinLensCamera>(v3(pos), v3(target), v3(up), fov, aspect, aperture, g_camera.focus);
+            c->setApertureShape(g_camera.blades, g_camera.rotationDeg);
+            if (g_camera.shiftX != 0.0f || g_camera.shiftY != 0.0f) c->setShift(g_camera.shiftX, g_camera.shiftY);
+            return c;
+        }
+        case 2: return std::make_unique<OrthographicCamera>(v3(pos), v3(target), v3(up), g_camera.extent, aspect);
+        default: {
+            auto c = std::make_unique<PerspectiveCamera>(v3(pos), v3(target), v3(up), fov, aspect);
+            if (g_camera.shiftX != 0.0f || g_camera.shiftY != 0.0f) c->setShift(g_camera.shiftX, g_camera.shiftY);
+            return c;
+        }
+    }
+}
+
+std::unique_ptr<Camera> makeCamera(const float* pos, const float* target, const float* up, float fov, float aspect) {
+    auto cam = makeCameraAt(pos, target, up, fov, aspect, g_camera.aperture);
+    if (g_camera.hasEnd) cam->setShutterEnd(*makeCameraAt(g_camera.endPos, g_camera.endTarget, up, fov, aspect,
+                                                          g_camera.endAperture));
+    return cam;
+}
+
+bool selectedCameraMotionBlur() { return g_camera.motionBlur; }
+
+// tracePixel's camera ray (renderer.cpp:151-153) after the filter dimension.
+Ray cameraRay(const Camera& cam, Sampler& sampler, float sx, float sy) {
+    const Vec2f lens = sampler.get2D();
+    return g_camera.motionBlur ? cam.generateRay(sx, sy, lens, sampler.get1D()) : cam.generateRay(sx, sy, lens);
+}
 
 extern "C" {
 
@@ -369,6 +418,69 @@ REF_API int ref_occluded(void* h, size_t n, const float* rays, uint8_t* occ) {
     return 0;
 }
 
+REF_API int ref_camera_shift(float shiftX, float shiftY) {
+    g_camera.shiftX = shiftX;
+    g_camera.shiftY = shiftY;
+    return 0;
+}
+
+REF_API int ref_camera_select(int kind, float aperture, float focus, int blades, float rotationDeg, float extent,
+                              int motionBlur, int hasEnd, const float* endPos, const float* endTarget, float endAperture) {
+    g_camera = CameraSpec{};
+    g_camera.kind = kind;
+    g_camera.aperture = aperture;
+    g_camera.focus = focus;
+    g_camera.blades = blades;
+    g_camera.rotationDeg = rotationDeg;
+    g_camera.extent = extent;
+    g_camera.motionBlur = motionBlur != 0;
+    g_camera.hasEnd = hasEnd != 0;
+    if (hasEnd) {
+        std::copy(endPos, endPos + 3, g_camera.endPos);
+        std::copy(endTarget, endTarget + 3, g_camera.endTarget);
+        g_camera.endAperture = endAperture;
+    }
+    return 0;
+}
+
+// The selected camera's describeGpu() and describeGpuEnd(): 22 floats each (position, lowerLeft, horizontal,
+// vertical, u, v, forward, lensRadius, blades, bladeRotation, orthographic).
+REF_API int ref_camera_describe_full(const float* pos, const float* target, const float* up, float fov, float aspect,
+                                     float* out) {
+    const auto cam = makeCamera(pos, target, up, fov, aspect);
+    const CameraGpuDesc ds[2] = {cam->describeGpu(), cam->describeGpuEnd()};
+    for (int e = 0; e < 2; ++e) {
+        const CameraGpuDesc& d = ds[e];
+        float* o = out + 22 * e;
+        const Vec3f vs[7] = {d.position, d.lowerLeft, d.horizontal, d.vertical, d.u, d.v, d.forward};
+        for (int k = 0; k < 7; ++k) {
+            o[3 * k] = vs[k].x;
+            o[3 * k + 1] = vs[k].y;
+            o[3 * k + 2] = vs[k].z;
+        }
+        o[21] = d.lensRadius;
+    }
+    out[44] = static_cast<float>(ds[0].blades);
+    out[45] = ds[0].bladeRotation;
+    out[46] = ds[0].orthographic ? 1.0f : 0.0f;
+    return 0;
+}
+
+// Camera::generateRay(u, v, lensSample, time) of the selected camera for n samples (in: 5 floats each, u, v,
+// lens x, lens y, time; time < 0 calls the three-argument form): out 6 floats each, origin and direction.
+REF_API int ref_camera_rays(const float* pos, const float* target, const float* up, float fov, float aspect, size_t n,
+                            const float* in, float* out) {
+    const auto cam = makeCamera(pos, target, up, fov, aspect);
+    for (size_t i = 0; i < n; ++i) {
+        const float* q = in + 5 * i;
+        const Ray r = q[4] < 0.0f ? cam->generateRay(q[0], q[1], Vec2f(q[2], q[3]))
+                                  : cam->generateRay(q[0], q[1], Vec2f(q[2], q[3]), q[4]);
+        const float v[6] = {r.origin.x, r.origin.y, r.origin.z, r.direction.x, r.direction.y, r.direction.z};
+        std::copy(v, v + 6, out + 6 * i);
+    }
+    return 0;
+}
+
 // PerspectiveCamera(...).describeGpu(): position, lowerLeft, horizontal, vertical.
 REF_API int ref_camera_describe(const float* pos, const float* target, const float* up, float fov, float aspect, float* out) {
     PerspectiveCamera cam(v3(pos), v3(target), v3(up), fov, aspect);
@@ -442,7 +554,8 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
                       float filterWidth, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
                       const int32_t* sampleIndex, float* L) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     PixelFilter pf;
     pf.type = static_cast<PixelFilterType>(filterType);
     pf.width = filterWidth;
@@ -455,8 +568,7 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
         const Vec2f offset = filter.sample(sampler.get2D());
         const float sx = (static_cast<float>(px[i]) + 0.5f + offset.x) / static_cast<float>(width);
         const float sy = 1.0f - (static_cast<float>(py[i]) + 0.5f + offset.y) / static_cast<float>(height);
-        const Vec2f lens = sampler.get2D();
-        Ray ray = cam.generateRay(sx, sy, lens);
+        Ray ray = cameraRay(cam, sampler, sx, sy);
         ray.coneWidth = cam.pixelConeWidth(height);
         ray.coneSpread = cam.pixelConeSpread(height);
         // tracePixel hands Li a primary-hit record whenever the background is not the environment
@@ -480,7 +592,8 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
                              float filterWidth, uint64_t seed, size_t n, const int32_t* px, const int32_t* py,
                              const int32_t* sampleIndex, float* L, float* out) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     PixelFilter pf;
     pf.type = static_cast<PixelFilterType>(filterType);
     pf.width = filterWidth;
@@ -493,8 +606,7 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
         const Vec2f offset = filter.sample(sampler.get2D());
         const float sx = (static_cast<float>(px[i]) + 0.5f + offset.x) / static_cast<float>(width);
         const float sy = 1.0f - (static_cast<float>(py[i]) + 0.5f + offset.y) / static_cast<float>(height);
-        const Vec2f lens = sampler.get2D();
-        Ray ray = cam.generateRay(sx, sy, lens);
+        Ray ray = cameraRay(cam, sampler, sx, sy);
         ray.coneWidth = cam.pixelConeWidth(height);
         ray.coneSpread = cam.pixelConeSpread(height);
         PathTracer::PrimaryHit primary;
@@ -535,7 +647,8 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
                int width, int height, int maxBounces, int shadowQuality, int filterType, float filterWidth,
                int firstSample, int count, int threads, float* mean, uint32_t* counts, double* seconds) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -545,6 +658,7 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
     s.numThreads = threads;
     s.filter.type = static_cast<PixelFilterType>(filterType);
     s.filter.width = filterWidth;
+    s.motionBlur = g_camera.motionBlur;
     Film film(width, height);
     Renderer renderer;
     const auto t0 = std::chrono::steady_clock::now();
@@ -608,7 +722,8 @@ REF_API int ref_render_aovs(void* h, const float* camPos, const float* camTarget
                             float aspect, int width, int height, int maxBounces, int firstSample, int count,
                             float* beauty, float* albedo, float* normal, float* alpha) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -617,6 +732,7 @@ REF_API int ref_render_aovs(void* h, const float* camPos, const float* camTarget
     s.numThreads = 1;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = g_camera.motionBlur;
     Film film(width, height);
     AovFilms aovs;
     aovs.resize(width, height);
@@ -645,7 +761,8 @@ REF_API int ref_render_adaptive(void* h, const float* camPos, const float* camTa
                                 const int* counts, float threshold, int minSpp, float* mean, uint32_t* countOut,
                                 float* relErr, uint64_t* traced) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -653,6 +770,7 @@ REF_API int ref_render_adaptive(void* h, const float* camPos, const float* camTa
     s.numThreads = 1;
     s.filter.type = static_cast<PixelFilterType>(0);  // box, width 1: the filter the oracle's tests use
     s.filter.width = 1.0f;
+    s.motionBlur = g_camera.motionBlur;
     s.adaptiveSampling = true;
     s.noiseThreshold = threshold;
     s.minAdaptiveSamples = minSpp;
@@ -718,7 +836,8 @@ REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarg
                               float aspect, int width, int height, int maxBounces, int firstSample, int count,
                               int threads, float* beauty, float* passes, float* crypto) {
     auto* rs = static_cast<RefScene*>(h);
-    PerspectiveCamera cam(v3(camPos), v3(camTarget), v3(camUp), fov, aspect);
+    const auto camPtr = makeCamera(camPos, camTarget, camUp, fov, aspect);
+    const Camera& cam = *camPtr;
     RenderSettings s;
     s.width = width;
     s.height = height;
@@ -727,6 +846,7 @@ REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarg
     s.numThreads = threads;
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
+    s.motionBlur = g_camera.motionBlur;
     Film film(width, height);
     RenderPassFilms pf;
     pf.resize(width, height);

@@ -213,6 +213,12 @@ __device__ __forceinline__ void ext_media_leave(uint4& e, int mat) {  // MediumS
     }
 }
 
+// One pose of CameraGpuDesc (camera/camera.h:19-31).
+struct CameraPose {
+    V3 pos, ll, h, v, u, w, fwd;  // position, lowerLeft, horizontal, vertical, lens axes u / v, forward
+    float radius;
+};
+
 struct LaunchParams {
     int width, height;
     int first_sample;
@@ -243,7 +249,11 @@ struct LaunchParams {
     int guide_records;    // guiding: capture training records
     float filter_width, filter_radius;
     const float* filter_cdf;  // 513 entries when not a box
-    V3 cam_pos, cam_ll, cam_h, cam_v;
+    CameraPose cam, cam_end;  // CameraGpuDesc at shutter open and close
+    int cam_ortho, cam_blades;
+    float cam_blade_rotation;
+    int cam_moving;           // motion blur with a shutter-end pose: rays blend the poses
+    uint32_t camera_dims;     // Sobol dimensions tracePixel draws before Li: filter, lens (+ shutter time)
 };
 
 // clamp01 (renderer.cpp:41-43) of c, with w.
@@ -293,9 +303,70 @@ __device__ __forceinline__ float power_heuristic(int nf, float f_pdf, int ng, fl
 
 // ---------------------------------------------------------------------------------------------- kernels
 
-// tracePixel's camera ray (renderer.cpp:139-162) for window sample g: Sobol dimensions 0-1 for the pixel filter,
-// 2-3 drawn for the lens and unused by the pinhole camera.
-__device__ __forceinline__ V3 camera_dir(const LaunchParams& lp, uint32_t g) {
+// A point uniform on the regular n-gon inscribed in the unit circle, turned by `rot`: the slice from u.x, a
+// uniform point of that triangle from the stretched (u.x, u.y) (ThinLensCamera::samplePolygon,
+// camera/thin_lens_camera.cpp:36-47; uniformSampleTriangle, sampling.h:97-100).
+__device__ __forceinline__ void polygon_sample(int blades, float rot, float ux, float uy, float& x, float& y) {
+    const float n = static_cast<float>(blades);
+    const float scaled = fminf(ux * n, n - 1e-4f);
+    const float k = floorf(scaled);
+    const float a0 = rot + (2.0f * kPi) * k / n;
+    const float a1 = a0 + (2.0f * kPi) / n;
+    const float s0 = sqrtf(scaled - k);
+    const float b0 = 1.0f - s0, b1 = uy * s0;
+    x = cosf(a0) * b0 + cosf(a1) * b1;
+    y = sinf(a0) * b0 + sinf(a1) * b1;
+}
+
+// Camera::generateRay for screen point (sx, sy), lens sample (lu, lv) and shutter time t (camera/camera.h:57-91;
+// PerspectiveCamera, perspective_camera.cpp:36-40; ThinLensCamera, thin_lens_camera.cpp:49-61;
+// OrthographicCamera, orthographic_camera.cpp:25-28).
+__device__ __forceinline__ void camera_ray(const LaunchParams& lp, float sx, float sy, float lu, float lv, float t,
+                                           V3& o, V3& d) {
+    const CameraPose& a = lp.cam;
+    if (!lp.cam_moving || t <= 0.0f) {
+        const V3 target = a.ll + sx * a.h + sy * a.v;
+        if (lp.cam_ortho) {
+            o = target;
+            d = a.fwd;
+            return;
+        }
+        o = a.pos;
+        if (a.radius > 0.0f) {
+            float x, y;
+            if (lp.cam_blades >= 3) polygon_sample(lp.cam_blades, lp.cam_blade_rotation, lu, lv, x, y);
+            else concentric_disk(lu, lv, x, y);
+            x *= a.radius;
+            y *= a.radius;
+            o = a.pos + a.u * x + a.w * y;
+        }
+        d = normalized(target - o);
+        return;
+    }
+    const CameraPose& b = lp.cam_end;
+    t = fminf(t, 1.0f);
+    const float s = 1.0f - t;
+    const V3 pos = a.pos * s + b.pos * t, ll = a.ll * s + b.ll * t;
+    const V3 hor = a.h * s + b.h * t, ver = a.v * s + b.v * t;
+    if (lp.cam_ortho) {
+        o = ll + hor * sx + ver * sy;
+        d = normalized(a.fwd * s + b.fwd * t);
+        return;
+    }
+    const float radius = a.radius * s + b.radius * t;
+    o = pos;
+    if (radius > 0.0f) {
+        const V3 cu = normalized(a.u * s + b.u * t), cv = normalized(a.w * s + b.w * t);
+        float x, y;
+        concentric_disk(lu, lv, x, y);
+        o = pos + cu * (x * radius) + cv * (y * radius);
+    }
+    d = normalized(ll + hor * sx + ver * sy - o);
+}
+
+// tracePixel's camera ray (renderer.cpp:139-162) for window sample g: Sobol dimension 0 for the pixel filter,
+// 1 for the lens and, with motion blur, 2 for the shutter time.
+__device__ __forceinline__ void camera_sample_ray(const LaunchParams& lp, uint32_t g, V3& o, V3& d) {
     const uint32_t m = lp.num_pixels;
     const uint32_t q = g % m;
     const uint32_t pix = lp.pixel_list ? lp.pixel_list[q] : q;
@@ -303,16 +374,15 @@ __device__ __forceinline__ V3 camera_dir(const LaunchParams& lp, uint32_t g) {
     const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
     const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
     SobolState ss{sobol_pixel_hash(lp.seed, x, y), rev_bits(sample), 0u};
-    float fu, fv, lu, lv;
+    float fu, fv, lu, lv, t = 0.0f;
     sobol_get2d(ss, fu, fv);  // pixel filter (renderer.cpp:148)
-    sobol_get2d(ss, lu, lv);  // lens sample, unused by the pinhole camera but consumed (renderer.cpp:151)
+    sobol_get2d(ss, lu, lv);  // lens sample (renderer.cpp:151), drawn for every camera
+    if (lp.camera_dims > 2) t = sobol_get1d(ss);  // shutter time (renderer.cpp:152-153)
     const float ox = filter_sample(lp, fu), oy = filter_sample(lp, fv);
     const float sx = (static_cast<float>(x) + 0.5f + ox) / static_cast<float>(lp.width);
     const float sy = 1.0f - (static_cast<float>(y) + 0.5f + oy) / static_cast<float>(lp.height);
-    const V3 target = lp.cam_ll + sx * lp.cam_h + sy * lp.cam_v;
-    return normalized(target - lp.cam_pos);
+    camera_ray(lp, sx, sy, lu, lv, t, o, d);
 }
-constexpr uint32_t kCameraDims = 2;  // Sobol dimensions camera_dir consumes
 
 // The state a camera path starts with (tracePixel, renderer.cpp:139-162; PathTracer::Li, path_tracer.cpp:321-360).
 __device__ __forceinline__ uint4 camera_ids(const LaunchParams& lp, uint32_t g) {
@@ -322,7 +392,7 @@ __device__ __forceinline__ uint4 camera_ids(const LaunchParams& lp, uint32_t g) 
     const uint32_t sample = static_cast<uint32_t>(lp.first_sample) + g / m;
     const int x = static_cast<int>(pix % static_cast<uint32_t>(lp.width));
     const int y = static_cast<int>(pix / static_cast<uint32_t>(lp.width));
-    return make_uint4(g, sample, sobol_pixel_hash(lp.seed, x, y), pack_bits(kCameraDims, 0, 0, true, true));
+    return make_uint4(g, sample, sobol_pixel_hash(lp.seed, x, y), pack_bits(lp.camera_dims, 0, 0, true, true));
 }
 
 // Start of a window: every path id free, nothing handed out; held-back slots cleared (a cancelled window may
@@ -675,8 +745,9 @@ __global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers p
         }
         if (e < cam) {
             const uint32_t g = seg_sample_index(k, gen + e);
-            const V3 d = camera_dir(lp, g);
-            ro[e] = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, __uint_as_float(pid));
+            V3 o, d;
+            camera_sample_ray(lp, g, o, d);
+            ro[e] = make_float4(o.x, o.y, o.z, __uint_as_float(pid));
             rd[e] = make_float4(d.x, d.y, d.z, __uint_as_float(kCamEntry | g));
             if (pb.aov_res) {
                 pb.aov_res[2 * static_cast<size_t>(g)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -901,7 +972,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
             float last_pdf = 0.0f;
             uint4 ids;
             uint4 ext = make_uint4(0u, 0u, 0u, 0u);
-            float4 lp4 = make_float4(lp.cam_pos.x, lp.cam_pos.y, lp.cam_pos.z, 0.0f);
+            float4 lp4 = make_float4(o4.x, o4.y, o4.z, 0.0f);  // lastPoint = the camera ray's origin (path_tracer.cpp:330)
             float4 ln4 = make_float4(0.0f, 1.0f, 0.0f, 0.0f);
             uint32_t media_events = 0;  // scattering events in media so far (path_tracer.cpp:51, :463), in L.w
             if (cam_entry) {
@@ -1075,8 +1146,10 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     if (kPasses && camera) {  // the data passes of the primary hit (renderer.cpp:163-167)
                         float4* d1 = pb.pass_res + static_cast<size_t>(lp.pass_count - 1) * lp.num_paths + g;
                         if (d1->w < 0.0f) {
+                            V3 co, cd;  // the camera ray's origin (a lens point for a thin lens)
+                            camera_sample_ray(lp, g, co, cd);
                             pb.pass_res[static_cast<size_t>(lp.pass_count - 2) * lp.num_paths + g] =
-                                make_float4(si.point.x, si.point.y, si.point.z, length(si.point - lp.cam_pos));
+                                make_float4(si.point.x, si.point.y, si.point.z, length(si.point - co));
                             *d1 = make_float4(__int_as_float(sc.shade[tri].shape), __int_as_float(si.material), 0.0f, 0.0f);
                         }
                     }
@@ -2159,9 +2232,9 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         set_error("pe_render: negative sample range");
         return PE_ERR_INVALID;
     }
-    if (camera->lens_radius != 0.0f) {
-        set_error("pe_render: only the pinhole camera (lens_radius 0) is implemented");
-        return PE_ERR_UNSUPPORTED;
+    if (!(camera->lens_radius >= 0.0f) || (camera->has_shutter_end && !(camera->end_lens_radius >= 0.0f))) {
+        set_error("pe_render: the camera's lens radius must be >= 0");
+        return PE_ERR_INVALID;
     }
     if (settings->max_bounces < 0 || settings->max_bounces > 100) {
         set_error("pe_render: max_bounces must be in [0, 100]");
@@ -2247,10 +2320,17 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             }
             lp.filter_cdf = wf->filter.as<float>();
         }
-        lp.cam_pos = mk(camera->position[0], camera->position[1], camera->position[2]);
-        lp.cam_ll = mk(camera->lower_left[0], camera->lower_left[1], camera->lower_left[2]);
-        lp.cam_h = mk(camera->horizontal[0], camera->horizontal[1], camera->horizontal[2]);
-        lp.cam_v = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
+        const auto v3 = [](const float* f) { return mk(f[0], f[1], f[2]); };
+        lp.cam = CameraPose{v3(camera->position), v3(camera->lower_left), v3(camera->horizontal), v3(camera->vertical),
+                            v3(camera->u), v3(camera->v), v3(camera->forward), camera->lens_radius};
+        lp.cam_end = CameraPose{v3(camera->end_position), v3(camera->end_lower_left), v3(camera->end_horizontal),
+                                v3(camera->end_vertical), v3(camera->end_u), v3(camera->end_v), v3(camera->end_forward),
+                                camera->end_lens_radius};
+        lp.cam_ortho = camera->orthographic ? 1 : 0;
+        lp.cam_blades = camera->blades >= 3 ? camera->blades : 0;
+        lp.cam_blade_rotation = camera->blade_rotation;
+        lp.cam_moving = settings->motion_blur && camera->has_shutter_end ? 1 : 0;
+        lp.camera_dims = settings->motion_blur ? 3u : 2u;
 
         PathBuffers pb = wf->view();
         pb.aov_res = want_aov ? wf->aov_res.as<float4>() : nullptr;

@@ -133,9 +133,11 @@ __device__ __forceinline__ bool spawn_ray_to(const Surface& si, V3 target, V3& o
 }
 
 // ------------------------------------------------------------------------------------------ sampling.h
-__device__ __forceinline__ V3 cosine_sample_hemisphere(float u0, float u1) {
+// A point uniform on the unit disk by the concentric map (uniformSampleDisk, sampling.h:55-75).
+__device__ __forceinline__ void concentric_disk(float u0, float u1, float& x, float& y) {
     const float sx = 2.0f * u0 - 1.0f, sy = 2.0f * u1 - 1.0f;
-    float x = 0.0f, y = 0.0f;
+    x = 0.0f;
+    y = 0.0f;
     if (!(sx == 0.0f && sy == 0.0f)) {
         float r, theta;
         if (fabsf(sx) > fabsf(sy)) {
@@ -148,6 +150,11 @@ __device__ __forceinline__ V3 cosine_sample_hemisphere(float u0, float u1) {
         x = r * cosf(theta);
         y = r * sinf(theta);
     }
+}
+
+__device__ __forceinline__ V3 cosine_sample_hemisphere(float u0, float u1) {
+    float x, y;
+    concentric_disk(u0, u1, x, y);
     return mk(x, y, sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y)));
 }
 

@@ -452,8 +452,8 @@ int pe_job_submit(pe_server* s, const pe_job_desc* job, uint64_t* id) {
         set_error("pe_job_submit: a job needs a scene of the server's device, a positive film size and spp");
         return PE_ERR_INVALID;
     }
-    if (job->camera.lens_radius != 0.0f || job->settings.max_bounces < 0 || job->settings.max_bounces > 100) {
-        set_error("pe_job_submit: pinhole camera and max_bounces in [0, 100] only");
+    if (!(job->camera.lens_radius >= 0.0f) || job->settings.max_bounces < 0 || job->settings.max_bounces > 100) {
+        set_error("pe_job_submit: lens radius >= 0 and max_bounces in [0, 100] only");
         return PE_ERR_INVALID;
     }
     try {

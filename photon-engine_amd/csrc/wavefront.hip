@@ -59,14 +59,125 @@ int pe_camera_perspective(const float position[3], const float target[3], const 
     const V3 hor = vw * u;
     const V3 ver = vh * v;
     const V3 ll = pos - hor * 0.5f - ver * 0.5f - w;
-    const V3 vs[4] = {pos, ll, hor, ver};
-    float* dst[4] = {out->position, out->lower_left, out->horizontal, out->vertical};
-    for (int k = 0; k < 4; ++k) {
+    *out = pe_camera{};
+    const V3 vs[7] = {pos, ll, hor, ver, u, v, -w};
+    float* dst[7] = {out->position, out->lower_left, out->horizontal, out->vertical, out->u, out->v, out->forward};
+    for (int k = 0; k < 7; ++k) {
         dst[k][0] = vs[k].x;
         dst[k][1] = vs[k].y;
         dst[k][2] = vs[k].z;
     }
-    out->lens_radius = 0.0f;
+    return PE_OK;
+}
+
+// ThinLensCamera constructor (camera/thin_lens_camera.cpp:14-33), setApertureShape (thin_lens_camera.h:31-34)
+// and describeGpu (:63-77): the pinhole's image plane moved out to the focus distance.
+int pe_camera_thin_lens(const float position[3], const float target[3], const float up[3], float vfov_degrees,
+                        float aspect, float aperture_radius, float focus_distance, int blades, float rotation_degrees,
+                        pe_camera* out) {
+#pragma clang fp contract(off)
+    if (!position || !target || !up || !out) {
+        set_error("pe_camera_thin_lens: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    const V3 pos = mk(position[0], position[1], position[2]);
+    const V3 tgt = mk(target[0], target[1], target[2]);
+    const V3 upv = mk(up[0], up[1], up[2]);
+    const float theta = vfov_degrees * (kPi / 180.0f);
+    const float h = std::tan(theta * 0.5f);
+    const float vh = 2.0f * h * focus_distance;
+    const float vw = aspect * vh;
+    const V3 w = normalized(pos - tgt);
+    const V3 u = normalized(cross(upv, w));
+    const V3 v = cross(w, u);
+    const V3 hor = vw * u;
+    const V3 ver = vh * v;
+    const V3 ll = pos - hor * 0.5f - ver * 0.5f - focus_distance * w;
+    *out = pe_camera{};
+    const V3 vs[7] = {pos, ll, hor, ver, u, v, -w};
+    float* dst[7] = {out->position, out->lower_left, out->horizontal, out->vertical, out->u, out->v, out->forward};
+    for (int k = 0; k < 7; ++k) {
+        dst[k][0] = vs[k].x;
+        dst[k][1] = vs[k].y;
+        dst[k][2] = vs[k].z;
+    }
+    out->lens_radius = aperture_radius;
+    out->blades = blades >= 3 ? blades : 0;
+    out->blade_rotation = rotation_degrees * 0.017453292519943295f;
+    return PE_OK;
+}
+
+// OrthographicCamera constructor and describeGpu (camera/orthographic_camera.cpp:11-40): parallel rays along
+// the view direction from a verticalExtent-high image rectangle through the camera position.
+int pe_camera_orthographic(const float position[3], const float target[3], const float up[3], float vertical_extent,
+                           float aspect, pe_camera* out) {
+#pragma clang fp contract(off)
+    if (!position || !target || !up || !out) {
+        set_error("pe_camera_orthographic: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    const V3 pos = mk(position[0], position[1], position[2]);
+    const V3 tgt = mk(target[0], target[1], target[2]);
+    const V3 upv = mk(up[0], up[1], up[2]);
+    const V3 w = normalized(pos - tgt);
+    const V3 u = normalized(cross(upv, w));
+    const V3 v = cross(w, u);
+    const float height = vertical_extent > 1e-4f ? vertical_extent : 1.0f;
+    const float width = aspect * height;
+    const V3 hor = width * u;
+    const V3 ver = height * v;
+    const V3 ll = pos - hor * 0.5f - ver * 0.5f;
+    const V3 centre = ll + hor * 0.5f + ver * 0.5f;
+    const V3 du = dot(hor, hor) > 0.0f ? normalized(hor) : mk(1.0f, 0.0f, 0.0f);
+    const V3 dv = dot(ver, ver) > 0.0f ? normalized(ver) : mk(0.0f, 1.0f, 0.0f);
+    *out = pe_camera{};
+    const V3 vs[7] = {centre, ll, hor, ver, du, dv, -w};
+    float* dst[7] = {out->position, out->lower_left, out->horizontal, out->vertical, out->u, out->v, out->forward};
+    for (int k = 0; k < 7; ++k) {
+        dst[k][0] = vs[k].x;
+        dst[k][1] = vs[k].y;
+        dst[k][2] = vs[k].z;
+    }
+    out->orthographic = 1;
+    return PE_OK;
+}
+
+// setShift (camera/perspective_camera.h:25-27, thin_lens_camera.h:25-27): lowerLeft += u (shiftX |vertical|) +
+// vertical shiftY.
+int pe_camera_set_shift(pe_camera* camera, float shift_x, float shift_y) {
+#pragma clang fp contract(off)
+    if (!camera) {
+        set_error("pe_camera_set_shift: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    if (camera->orthographic) {
+        set_error("pe_camera_set_shift: an orthographic camera has no lens shift");
+        return PE_ERR_INVALID;
+    }
+    const V3 ver = mk(camera->vertical[0], camera->vertical[1], camera->vertical[2]);
+    const V3 u = mk(camera->u[0], camera->u[1], camera->u[2]);
+    const V3 ll = mk(camera->lower_left[0], camera->lower_left[1], camera->lower_left[2]) +
+                  u * (shift_x * length(ver)) + ver * shift_y;
+    camera->lower_left[0] = ll.x;
+    camera->lower_left[1] = ll.y;
+    camera->lower_left[2] = ll.z;
+    return PE_OK;
+}
+
+// Camera::setShutterEnd(const Camera&) (camera/camera.h:49-54): the end camera's description becomes the pose
+// at shutter close.
+int pe_camera_set_shutter_end(pe_camera* camera, const pe_camera* end) {
+    if (!camera || !end) {
+        set_error("pe_camera_set_shutter_end: NULL argument");
+        return PE_ERR_INVALID;
+    }
+    const float* src[7] = {end->position, end->lower_left, end->horizontal, end->vertical, end->u, end->v, end->forward};
+    float* dst[7] = {camera->end_position, camera->end_lower_left, camera->end_horizontal, camera->end_vertical,
+                     camera->end_u, camera->end_v, camera->end_forward};
+    for (int k = 0; k < 7; ++k)
+        for (int c = 0; c < 3; ++c) dst[k][c] = src[k][c];
+    camera->end_lens_radius = end->lens_radius;
+    camera->has_shutter_end = 1;
     return PE_OK;
 }
 

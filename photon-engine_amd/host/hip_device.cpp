@@ -10,8 +10,9 @@
 // scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle / static
 // MeshInstance, visibility masks other than all rays or all but camera rays, shadow catchers, lights other
 // than AreaLight / TriangleLight, light linking, rounded edges, textured or flaked Disney materials and
-// materials other than Disney / Lambertian / Dielectric; render() returns completed = false for a thin-lens or orthographic camera, AO and
-// motion blur. Path guiding uses the library's learned guide in place of the reference's SD-tree.
+// materials other than Disney / Lambertian / Dielectric; render() returns completed = false for AO. Pinhole,
+// thin-lens and orthographic cameras render, with motion blur of the camera (moving instances are refused).
+// Path guiding uses the library's learned guide in place of the reference's SD-tree.
 #include "hip_device.h"
 
 #include "camera/camera.h"
@@ -523,12 +524,8 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     if (!m_scene || !targets.beauty) return res;
     Film& film = *targets.beauty;
     const CameraGpuDesc cd = camera.describeGpu();
-    if (cd.orthographic || cd.lensRadius > 0.0f) {
-        m_error = "HipDevice::render: only the pinhole perspective camera is implemented";
-        return res;
-    }
-    if (settings.aoStrength > 0.0f || settings.motionBlur) {
-        m_error = "HipDevice::render: AO and motion blur are not implemented";
+    if (settings.aoStrength > 0.0f) {
+        m_error = "HipDevice::render: AO is not implemented";
         return res;
     }
     // Path guiding (renderer.cpp:413-429): a learned guide per scene, made afresh when a render starts at sample
@@ -552,12 +549,32 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
         m_guide = nullptr;
         m_guideScene = nullptr;
     }
+    // The camera as CameraGpuDesc (camera.h:19-31) of both shutter poses; the library draws the rays as
+    // Camera::generateRay does for a pinhole, thin-lens or orthographic camera (camera.h:57-91).
     pe_camera cam{};
     put3(cam.position, cd.position);
     put3(cam.lower_left, cd.lowerLeft);
     put3(cam.horizontal, cd.horizontal);
     put3(cam.vertical, cd.vertical);
-    cam.lens_radius = 0.0f;
+    put3(cam.u, cd.u);
+    put3(cam.v, cd.v);
+    put3(cam.forward, cd.forward);
+    cam.lens_radius = cd.lensRadius;
+    cam.blades = cd.blades;
+    cam.blade_rotation = cd.bladeRotation;
+    cam.orthographic = cd.orthographic ? 1 : 0;
+    if (camera.hasShutterEnd()) {
+        const CameraGpuDesc ce = camera.describeGpuEnd();
+        cam.has_shutter_end = 1;
+        put3(cam.end_position, ce.position);
+        put3(cam.end_lower_left, ce.lowerLeft);
+        put3(cam.end_horizontal, ce.horizontal);
+        put3(cam.end_vertical, ce.vertical);
+        put3(cam.end_u, ce.u);
+        put3(cam.end_v, ce.v);
+        put3(cam.end_forward, ce.forward);
+        cam.end_lens_radius = ce.lensRadius;
+    }
     pe_settings st{};
     st.max_bounces = settings.maxBounces;
     st.rr_depth = 3;                                   // Renderer::renderSamples (renderer.cpp:432)
@@ -568,6 +585,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     st.seed = 12345;                                   // createRenderSampler default (renderer.h:37)
     st.samples_per_launch = 0;                         // the library sizes launches for the device
     st.adaptive = settings.adaptiveSampling ? 1 : 0;   // renderer.cpp:438-505
+    st.motion_blur = settings.motionBlur ? 1 : 0;      // a shutter time per camera ray (renderer.cpp:152-153)
     st.noise_threshold = settings.noiseThreshold;
     st.min_adaptive_samples = settings.minAdaptiveSamples;
     const bool adaptive = settings.adaptiveSampling;

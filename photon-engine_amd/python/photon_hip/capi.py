@@ -116,7 +116,10 @@ PE_CRYPTO_SLOTS = 8
 
 class Camera(C.Structure):
     _fields_ = [("position", F3), ("lower_left", F3), ("horizontal", F3), ("vertical", F3),
-                ("lens_radius", C.c_float)]
+                ("lens_radius", C.c_float), ("u", F3), ("v", F3), ("forward", F3), ("blades", C.c_int32),
+                ("blade_rotation", C.c_float), ("orthographic", C.c_int32), ("has_shutter_end", C.c_int32),
+                ("end_position", F3), ("end_lower_left", F3), ("end_horizontal", F3), ("end_vertical", F3),
+                ("end_u", F3), ("end_v", F3), ("end_forward", F3), ("end_lens_radius", C.c_float)]
 
 
 class Settings(C.Structure):
@@ -124,7 +127,7 @@ class Settings(C.Structure):
                 ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
                 ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("count_visits", C.c_int32),
                 ("adaptive", C.c_int32), ("noise_threshold", C.c_float), ("min_adaptive_samples", C.c_int32),
-                ("paths_in_flight", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("paths_in_flight", C.c_int32), ("motion_blur", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BatchResult(C.Structure):
@@ -219,6 +222,12 @@ SIGNATURES = {
     "pe_scene_release": (C.c_int, [_VP]),
     "pe_scene_stats": (C.c_int, [_VP, _P(C.c_uint64), _P(C.c_uint64), _P(C.c_uint64)]),
     "pe_camera_perspective": (C.c_int, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, _P(Camera)]),
+    "pe_camera_thin_lens": (C.c_int, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float, C.c_float,
+                                      C.c_float, C.c_int, C.c_float, _P(Camera)]),
+    "pe_camera_orthographic": (C.c_int, [_P(C.c_float), _P(C.c_float), _P(C.c_float), C.c_float, C.c_float,
+                                         _P(Camera)]),
+    "pe_camera_set_shutter_end": (C.c_int, [_P(Camera), _P(Camera)]),
+    "pe_camera_set_shift": (C.c_int, [_P(Camera), C.c_float, C.c_float]),
     "pe_film_create": (C.c_int, [_VP, C.c_int, C.c_int, _P(_VP)]),
     "pe_film_destroy": (C.c_int, [_VP]),
     "pe_film_clear": (C.c_int, [_VP]),

@@ -23,11 +23,45 @@ def camera_perspective(position, target, up, vfov, aspect) -> capi.Camera:
     return cam
 
 
+def camera_thin_lens(position, target, up, vfov, aspect, aperture, focus, blades=0, rotation_deg=0.0) -> capi.Camera:
+    """ThinLensCamera(position, target, up, vfov, aspect, aperture, focus) with setApertureShape(blades, rotation)."""
+    cam = capi.Camera()
+    f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+    check("pe_camera_thin_lens", load().pe_camera_thin_lens(f3(position), f3(target), f3(up), float(vfov),
+                                                              float(aspect), float(aperture), float(focus), int(blades),
+                                                              float(rotation_deg), C.byref(cam)))
+    return cam
+
+
+def camera_orthographic(position, target, up, vertical_extent, aspect) -> capi.Camera:
+    """OrthographicCamera(position, target, up, verticalExtent, aspect)."""
+    cam = capi.Camera()
+    f3 = lambda v: (C.c_float * 3)(*map(float, v))  # noqa: E731
+    check("pe_camera_orthographic", load().pe_camera_orthographic(f3(position), f3(target), f3(up),
+                                                                    float(vertical_extent), float(aspect), C.byref(cam)))
+    return cam
+
+
+def camera_with_shift(camera: capi.Camera, shift_x: float, shift_y: float) -> capi.Camera:
+    """A copy of a perspective or thin-lens `camera` with a lens shift (setShift)."""
+    out = capi.Camera.from_buffer_copy(camera)
+    check("pe_camera_set_shift", load().pe_camera_set_shift(C.byref(out), float(shift_x), float(shift_y)))
+    return out
+
+
+def camera_with_shutter_end(camera: capi.Camera, end: capi.Camera) -> capi.Camera:
+    """A copy of `camera` with `end`'s pose at shutter close (Camera::setShutterEnd)."""
+    out = capi.Camera.from_buffer_copy(camera)
+    check("pe_camera_set_shutter_end", load().pe_camera_set_shutter_end(C.byref(out), C.byref(end)))
+    return out
+
+
 def settings(max_bounces=8, rr_depth=3, shadow_quality=1, filter_type=capi.PE_FILTER_BOX, filter_width=1.0,
              seed=12345, samples_per_launch=1, clamp_indirect=0.0, adaptive=False, noise_threshold=0.02,
-             min_adaptive_samples=16, paths_in_flight=0) -> capi.Settings:
+             min_adaptive_samples=16, paths_in_flight=0, motion_blur=False) -> capi.Settings:
     """RenderSettings as pe_settings (the reference's defaults for the adaptive fields, render_settings.h:30-32)."""
     s = capi.Settings()
+    s.motion_blur = int(bool(motion_blur))
     s.max_bounces, s.rr_depth, s.shadow_quality = max_bounces, rr_depth, shadow_quality
     s.filter_type, s.filter_width, s.clamp_indirect = filter_type, filter_width, clamp_indirect
     s.seed, s.samples_per_launch, s.paths_in_flight = seed, samples_per_launch, paths_in_flight

@@ -960,14 +960,42 @@ def build_render_scene(pr: Project, clay: bool = False, sphere_segments: tuple |
 
 def orbit_view(pr: Project, width: int, height: int, spp: int = 16, max_bounces: int | None = None) -> View:
     """The project's orbit camera (photon_render's cameraFromJson, the golden test's orbitCamera,
-    test_golden_reference.cpp:44-51): eye = target + r (sin th cos ph, cos th, sin th sin ph), up +y."""
+    test_golden_reference.cpp:44-51): eye = target + r (sin th cos ph, cos th, sin th sin ph), up +y; the
+    pinhole's view (orbit_camera gives the thin-lens or orthographic camera the block may ask for)."""
     c = pr.camera
     f32 = np.float32
     t = np.asarray(c.get("target", (0.0, 0.0, 0.0)), f32)
     r, th, ph = f32(c.get("radius", 5.0)), f32(c.get("theta", 1.15)), f32(c.get("phi", 0.45))
-    if c.get("orthographic") or float(c.get("aperture", 0.0)) > 0.0:
-        raise DocumentError("orthographic and thin-lens cameras are not implemented on the HIP path")
     eye = t + np.array([math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph)], f32) * r
     mb = int(pr.render.get("maxBounces", 12)) if max_bounces is None else max_bounces
     return View(tuple(map(float, eye)), tuple(map(float, t)), (0.0, 1.0, 0.0), float(c.get("fov", 35.0)), width,
                 height, spp, mb)
+
+
+def orbit_camera(pr: Project, width: int, height: int):
+    """The camera cameraFromJson builds from the project's camera block (main.cpp:89-111): orthographic with
+    vertical extent 2 tan(fov / 2) r, a ThinLensCamera when aperture > 0 (focus = focusDistance when
+    focusExplicit, else r; apertureBlades, bladeRotation, shiftX / shiftY) or a PerspectiveCamera with its
+    lens shift; -> pe_camera."""
+    from .device import camera_orthographic, camera_perspective, camera_thin_lens, camera_with_shift
+
+    c = pr.camera
+    view = orbit_view(pr, width, height)
+    f32 = np.float32
+    aspect = f32(width) / f32(height)
+    r = f32(c.get("radius", 5.0))
+    fov = f32(c.get("fov", 35.0))
+    if c.get("orthographic", False):
+        h = f32(2.0) * f32(math.tan(fov * f32(math.pi / 180.0) * f32(0.5))) * r
+        return camera_orthographic(view.position, view.target, view.up, float(h), float(aspect))
+    aperture = float(c.get("aperture", 0.0))
+    if aperture > 0.0:
+        focus = float(c.get("focusDistance", r)) if c.get("focusExplicit", False) else float(r)
+        cam = camera_thin_lens(view.position, view.target, view.up, float(fov), float(aspect), aperture, focus,
+                               int(c.get("apertureBlades", 0)), float(c.get("bladeRotation", 0.0)))
+    else:
+        cam = camera_perspective(view.position, view.target, view.up, float(fov), float(aspect))
+    sx, sy = float(c.get("shiftX", 0.0)), float(c.get("shiftY", 0.0))
+    return camera_with_shift(cam, sx, sy) if (sx or sy) else cam
+
+

@@ -318,6 +318,40 @@ def test_project_compiles_as_build_render_scene(tmp_path):
     np.testing.assert_allclose(np.linalg.norm(np.subtract(v.position, v.target)), 4.0, rtol=1e-6)
 
 
+def test_project_cameras_as_camera_from_json(tmp_path):
+    """cameraFromJson (main.cpp:89-111): the camera block's aperture makes a ThinLensCamera focused at r (or the
+    explicit focus distance) with its blades, rotation and lens shift; orthographic makes an OrthographicCamera
+    2 tan(fov / 2) r high; the rest a shifted pinhole. Each equals the reference's own camera of those
+    parameters, description for description."""
+    if bindings.reference(strict=True) is None:
+        pytest.skip("oracle/_ref not built here (reference tree absent)")
+    base = {"target": [0, 0.5, 0], "radius": 4.0, "theta": 1.2, "phi": 0.7, "fov": 40}
+    blocks = [
+        (dict(base, aperture=0.08, apertureBlades=7, bladeRotation=12.0, shiftX=0.05, shiftY=-0.02),
+         dict(kind="thin_lens", aperture=0.08, focus=4.0, blades=7, rotation_deg=12.0, shift=(0.05, -0.02))),
+        (dict(base, aperture=0.05, focusExplicit=True, focusDistance=2.5),
+         dict(kind="thin_lens", aperture=0.05, focus=2.5)),
+        (dict(base, shiftY=0.1), dict(kind="perspective", shift=(0.0, 0.1))),
+        (dict(base, orthographic=True), None),
+    ]
+    for block, spec in blocks:
+        path, _ = _project(tmp_path, camera=block)
+        pr = D.load_project(path)
+        cam = D.orbit_camera(pr, 64, 48)
+        view = D.orbit_view(pr, 64, 48)
+        if spec is None:
+            h = np.float32(2.0) * np.float32(np.tan(np.float32(40.0) * np.float32(np.pi / 180.0) * np.float32(0.5)))
+            spec = dict(kind="orthographic", extent=float(h * np.float32(4.0)))
+            assert cam.orthographic == 1
+        with bindings.reference_camera(strict=True, **spec) as rc:
+            want = rc.describe(view)
+        got = np.array(list(cam.position) + list(cam.lower_left) + list(cam.horizontal) + list(cam.vertical)
+                       + list(cam.u) + list(cam.v) + list(cam.forward) + [cam.lens_radius], np.float32)
+        np.testing.assert_allclose(got, want[:22], rtol=2e-6, atol=2e-7)
+        assert (cam.blades, cam.orthographic) == (int(want[44]), int(want[46]))
+        assert abs(cam.blade_rotation - want[45]) <= 1e-7
+
+
 def test_project_refuses_what_the_hip_path_does_not_render(tmp_path):
     cases = [
         ("materials", [{"type": "disney", "maps": {"albedo": "a.png"}}], "texture maps"),

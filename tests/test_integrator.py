@@ -172,8 +172,8 @@ def test_gpu_cancel_and_errors(dev):
     cam = _cam(view)
     st = settings()
     assert lib.pe_render(dev.h, gs.h, C.byref(cam), C.byref(st), film.h, -1, 1, None, C.byref(bad)) == capi.PE_ERR_INVALID
-    cam.lens_radius = 0.1
-    assert lib.pe_render(dev.h, gs.h, C.byref(cam), C.byref(st), film.h, 0, 1, None, C.byref(bad)) == capi.PE_ERR_UNSUPPORTED
+    cam.lens_radius = -0.1
+    assert lib.pe_render(dev.h, gs.h, C.byref(cam), C.byref(st), film.h, 0, 1, None, C.byref(bad)) == capi.PE_ERR_INVALID
 
 
 @pytest.mark.gpu
