@@ -119,6 +119,16 @@ typedef struct pe_material_desc {
     float scatter_density;
     float scatter_color[3];
     float scatter_anisotropy;
+    /* Disney only: DisneyMaterial::setFlakes(density, size, roughness, color) (disney.h:120-125; clamped to
+     * [0, 1], size >= 1e-6, [0, 1], [0, 1]). Space is cut into cells of the flake size, one seed per cell (3-D
+     * Voronoi); a point whose nearest seed hashes below the density lies on a metallic flake tilted by a GGX-like
+     * slope of the flake roughness, which replaces the base layer with a narrow GGX lobe of Schlick reflectance
+     * flake_color about the flake normal (resolveFlake, evalFlake, disney.cpp:104-126, :423-460). Density 0:
+     * no flakes. */
+    float flake_density;
+    float flake_size;
+    float flake_roughness;
+    float flake_color[3];
     float reserved[3];
 } pe_material_desc;
 

@@ -507,27 +507,29 @@ def ref_render_adaptive(scene_h, lib, view, batches, threshold, min_spp):
     return mean, cnt, rel, traced.value
 
 
-def bsdf_eval(lib_kind: str, material, normal, tangent, front, wo, wi):
-    """lib_kind 'oracle' or 'ref': Material::evalPdf on n tuples -> (f (n,3), pdf (n,))."""
+def bsdf_eval(lib_kind: str, material, normal, tangent, front, wo, wi, points=None):
+    """lib_kind 'oracle' or 'ref': Material::evalPdf on n tuples -> (f (n,3), pdf (n,)); points: the shading
+    points (n, 3), where flakes are resolved (default: the origin)."""
     n = normal.shape[0]
     args = [np.ascontiguousarray(a, np.float32) for a in (normal, tangent)]
     front = np.ascontiguousarray(front, np.int32)
     wo = np.ascontiguousarray(wo, np.float32)
     wi = np.ascontiguousarray(wi, np.float32)
+    pts = np.ascontiguousarray(points, np.float32) if points is not None else None
     f = np.zeros((n, 3), np.float32)
     pdf = np.zeros(n, np.float32)
-    if lib_kind == "oracle":
-        fn = oracle().oracle_bsdf_eval
-    else:
-        fn = reference(strict=(lib_kind == "ref_strict")).ref_bsdf_eval
-    rc = fn(C.byref(material), n, _fp(args[0]), _fp(args[1]), capi.ptr(front, C.c_int32), _fp(wo), _fp(wi), _fp(f),
-            _fp(pdf))
+    lib = oracle().lib if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).lib
+    fn = getattr(lib, "oracle_bsdf_eval_at" if lib_kind == "oracle" else "ref_bsdf_eval_at")
+    fn.restype = C.c_int
+    fn.argtypes = [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _F, _I, _F, _F, _F, _F]
+    rc = fn(C.byref(material), n, _fp(pts) if pts is not None else None, _fp(args[0]), _fp(args[1]),
+            capi.ptr(front, C.c_int32), _fp(wo), _fp(wi), _fp(f), _fp(pdf))
     if rc != 0:
         raise RuntimeError("bsdf_eval failed")
     return f, pdf
 
 
-def bsdf_sample(lib_kind: str, material, normal, tangent, front, wo, u3):
+def bsdf_sample(lib_kind: str, material, normal, tangent, front, wo, u3, points=None):
     """Material::sampleFull -> (n, 14): ok, wi, f, pdf, delta, subsurface, passThrough, weight."""
     n = normal.shape[0]
     normal = np.ascontiguousarray(normal, np.float32)
@@ -535,11 +537,32 @@ def bsdf_sample(lib_kind: str, material, normal, tangent, front, wo, u3):
     front = np.ascontiguousarray(front, np.int32)
     wo = np.ascontiguousarray(wo, np.float32)
     u3 = np.ascontiguousarray(u3, np.float32)
+    pts = np.ascontiguousarray(points, np.float32) if points is not None else None
     out = np.zeros((n, 14), np.float32)
-    fn = oracle().oracle_bsdf_sample if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).ref_bsdf_sample
-    rc = fn(C.byref(material), n, _fp(normal), _fp(tangent), capi.ptr(front, C.c_int32), _fp(wo), _fp(u3), _fp(out))
+    lib = oracle().lib if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).lib
+    fn = getattr(lib, "oracle_bsdf_sample_at" if lib_kind == "oracle" else "ref_bsdf_sample_at")
+    fn.restype = C.c_int
+    fn.argtypes = [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _F, _I, _F, _F, _F]
+    rc = fn(C.byref(material), n, _fp(pts) if pts is not None else None, _fp(normal), _fp(tangent),
+            capi.ptr(front, C.c_int32), _fp(wo), _fp(u3), _fp(out))
     if rc != 0:
         raise RuntimeError("bsdf_sample failed")
+    return out
+
+
+def flake_resolve(lib_kind: str, material, points, normal, tangent, front):
+    """DisneyMaterial::resolve's flake state -> (n, 4): inFlake, flakeNormal."""
+    n = normal.shape[0]
+    arrs = [np.ascontiguousarray(a, np.float32) for a in (points, normal, tangent)]
+    front = np.ascontiguousarray(front, np.int32)
+    out = np.zeros((n, 4), np.float32)
+    lib = oracle().lib if lib_kind == "oracle" else reference(strict=(lib_kind == "ref_strict")).lib
+    fn = getattr(lib, "oracle_flake_resolve" if lib_kind == "oracle" else "ref_flake_resolve")
+    fn.restype = C.c_int
+    fn.argtypes = [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _F, _I, _F]
+    rc = fn(C.byref(material), n, *[_fp(a) for a in arrs], capi.ptr(front, C.c_int32), _fp(out))
+    if rc != 0:
+        raise RuntimeError("flake_resolve failed")
     return out
 
 

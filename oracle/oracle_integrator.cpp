@@ -1156,7 +1156,7 @@ V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow,
         fillHit(scene, r, h, hit);
         const pe_material_desc& m = scene.materials[static_cast<size_t>(hit.material)];
         if (!materialTransmitsShadows(m)) return V3(0.0f);
-        ShadingPoint sp{hit.normal, hit.tangent, hit.frontFace, hit.ng, 0.0f};
+        ShadingPoint sp{hit.normal, hit.tangent, hit.frontFace, hit.ng, 0.0f, hit.point};
         T *= materialShadowTransmittance(m, sp, dir);
         if (isBlack(T)) return T;
         origin = offsetRayOrigin(hit.point, hit.pError, lengthSq(hit.ng) > 0.0f ? hit.ng : hit.normal, dir);
@@ -1389,7 +1389,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
             throughput *= w;
         }
         const bool isSpecular = materialIsDelta(mat);
-        const ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace, isect.ng, lambda};
+        const ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace, isect.ng, lambda, isect.point};
         if (lightCount > 0 && !isSpecular) {  // NEE, :631-698
             picker = prepare(it, isect.point, isect.normal);
             for (int s = 0; s < shadowSamples; ++s) {
@@ -1983,13 +1983,16 @@ ORACLE_API int oracle_render(void* sp, const pe_camera* cam, const pe_settings* 
     }
 }
 
-ORACLE_API int oracle_bsdf_eval(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
-                                const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+ORACLE_API int oracle_bsdf_eval_at(const pe_material_desc* m, size_t n, const float* points, const float* normal,
+                                   const float* tangent, const int32_t* frontFace, const float* wo, const float* wi,
+                                   float* f, float* pdf) {
     try {
         for (size_t i = 0; i < n; ++i) {
-            // As ref_driver's shadingPoint(): the geometric normal is the given normal.
+            // As ref_driver's shadingPoint(): the geometric normal is the given normal; the point is the origin
+            // unless given.
             const V3 nn(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
-            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f};
+            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f,
+                            points ? V3(points[3 * i], points[3 * i + 1], points[3 * i + 2]) : V3(0.0f)};
             float p = 0.0f;
             const V3 v = evalPdf(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), V3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]),
                                  sp, p);
@@ -2004,13 +2007,36 @@ ORACLE_API int oracle_bsdf_eval(const pe_material_desc* m, size_t n, const float
     }
 }
 
-ORACLE_API int oracle_bsdf_sample(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
-                                  const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+ORACLE_API int oracle_flake_resolve(const pe_material_desc* m, size_t n, const float* points, const float* normal,
+                                    const float* tangent, const int32_t* frontFace, float* out) {
+    for (size_t i = 0; i < n; ++i) {
+        const V3 nn(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
+        const ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn,
+                              0.0f, V3(points[3 * i], points[3 * i + 1], points[3 * i + 2])};
+        bool in = false;
+        V3 fn;
+        flakeState(*m, sp, in, fn);
+        out[4 * i] = in ? 1.0f : 0.0f;
+        out[4 * i + 1] = fn.x;
+        out[4 * i + 2] = fn.y;
+        out[4 * i + 3] = fn.z;
+    }
+    return 0;
+}
+
+ORACLE_API int oracle_bsdf_eval(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
+                                const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+    return oracle_bsdf_eval_at(m, n, nullptr, normal, tangent, frontFace, wo, wi, f, pdf);
+}
+
+ORACLE_API int oracle_bsdf_sample_at(const pe_material_desc* m, size_t n, const float* points, const float* normal,
+                                     const float* tangent, const int32_t* frontFace, const float* wo, const float* u3,
+                                     float* out) {
     try {
         for (size_t i = 0; i < n; ++i) {
-            // As ref_driver's shadingPoint(): the geometric normal is the given normal.
             const V3 nn(normal[3 * i], normal[3 * i + 1], normal[3 * i + 2]);
-            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f};
+            ShadingPoint sp{nn, V3(tangent[3 * i], tangent[3 * i + 1], tangent[3 * i + 2]), frontFace[i] != 0, nn, 0.0f,
+                            points ? V3(points[3 * i], points[3 * i + 1], points[3 * i + 2]) : V3(0.0f)};
             BsdfSample bs;
             const bool ok = sampleFull(*m, V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), sp, u3[3 * i], u3[3 * i + 1],
                                        u3[3 * i + 2], bs);
@@ -2028,6 +2054,11 @@ ORACLE_API int oracle_bsdf_sample(const pe_material_desc* m, size_t n, const flo
     } catch (const std::exception&) {
         return -1;
     }
+}
+
+ORACLE_API int oracle_bsdf_sample(const pe_material_desc* m, size_t n, const float* normal, const float* tangent,
+                                  const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+    return oracle_bsdf_sample_at(m, n, nullptr, normal, tangent, frontFace, wo, u3, out);
 }
 
 // which: 0 ggxDirectionalAlbedo, 1 ggxSchlickAlbedo, 2 ggxSchlickAverageAlbedo, 3 charlieSheenAlbedo

@@ -292,12 +292,69 @@ struct Params {
     float thinFilmThickness, thinFilmIor;
     V3 normal, tangent;
     bool frontFace;
+    bool inFlake = false;       // the point lies on a metallic flake (disney.h:201-203)
+    V3 flakeNormal{0.0f, 1.0f, 0.0f};
+    V3 flakeColor{0.9f, 0.9f, 0.9f};
 };
 
 V3 c3(const float* v) { return V3(v[0], v[1], v[2]); }
+
+constexpr float FLAKE_ALPHA = 0.06f;  // disney.cpp:75
+
+// lowbias32 integer hash, cell hash and unit float of the flake cells (disney.cpp:407-421).
+uint32_t flakeHash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+
+uint32_t flakeCellHash(int x, int y, int z) {
+    return flakeHash(static_cast<uint32_t>(x) * 0x8da6b343u ^
+                     flakeHash(static_cast<uint32_t>(y) * 0xd8163841u ^ flakeHash(static_cast<uint32_t>(z) * 0xcb1ab31fu)));
+}
+
+float flakeUnit(uint32_t h) { return static_cast<float>(h >> 8) * 0x1p-24f; }
+
+// resolveFlake (disney.cpp:430-460): the nearest of the 27 neighbouring cells' seeds (3-D Voronoi) decides
+// whether the point is on a flake (density) and the flake's tilt: tan^2 theta = r^2 u1 / (1 - u1), phi = 2 pi u2
+// about the shading normal.
+void resolveFlake(V3 point, float size, float density, float rough, Params& p) {
+    const V3 q = point / size;
+    const int cx = static_cast<int>(std::floor(q.x));
+    const int cy = static_cast<int>(std::floor(q.y));
+    const int cz = static_cast<int>(std::floor(q.z));
+    float best = 1e30f;
+    uint32_t bestHash = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const uint32_t h = flakeCellHash(cx + dx, cy + dy, cz + dz);
+                const V3 f(static_cast<float>(cx + dx) + flakeUnit(h),
+                           static_cast<float>(cy + dy) + flakeUnit(flakeHash(h ^ 0x68e31da4u)),
+                           static_cast<float>(cz + dz) + flakeUnit(flakeHash(h ^ 0xb5297a4du)));
+                const float d2 = lengthSq(q - f);
+                if (d2 < best) {
+                    best = d2;
+                    bestHash = h;
+                }
+            }
+    if (flakeUnit(flakeHash(bestHash ^ 0x1b56c4e9u)) >= density) return;
+    const float u1 = std::min(flakeUnit(flakeHash(bestHash ^ 0x2c1b3c6du)), 0.999f);
+    const float u2 = flakeUnit(flakeHash(bestHash ^ 0x297a2d39u));
+    const float tan2 = rough * rough * u1 / (1.0f - u1);
+    const float cosT = 1.0f / std::sqrt(1.0f + tan2);
+    const float sinT = std::sqrt(std::max(0.0f, 1.0f - cosT * cosT));
+    const float phi = TWO_PI * u2;
+    const Frame f(p.normal);
+    p.inFlake = true;
+    p.flakeNormal = normalized(f.toWorld(V3(sinT * std::cos(phi), sinT * std::sin(phi), cosT)));
+}
 V3 clamp01(V3 c) { return V3(clampf(c.x, 0.0f, 1.0f), clampf(c.y, 0.0f, 1.0f), clampf(c.z, 0.0f, 1.0f)); }
 
-Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.cpp:543-603 (no textures, no flakes)
+Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.cpp:543-603 (no textures)
     Params p;
     p.baseColor = c3(d.base_color);
     p.metallic = d.metallic;
@@ -323,6 +380,10 @@ Params resolve(const pe_material_desc& d, const ShadingPoint& sp) {  // disney.c
     p.normal = sp.normal;
     p.tangent = sp.tangent;
     p.frontFace = sp.frontFace;
+    p.flakeColor = clamp01(c3(d.flake_color));  // setFlakes clamps (disney.h:120-125)
+    if (d.flake_density > 0.0f)
+        resolveFlake(sp.point, std::max(d.flake_size, 1e-6f), clampf(d.flake_density, 0.0f, 1.0f),
+                     clampf(d.flake_roughness, 0.0f, 1.0f), p);
     return p;
 }
 
@@ -540,16 +601,40 @@ V3 eonEval(V3 rho, float r, V3 wi, V3 wo) {  // eon.h:37-51
     return fss + rhoMs * (INV_PI * ms);
 }
 
-V3 evalLobes(V3 wo, V3 wi, const Params& p, LobeSplit* split = nullptr) {  // :223-311 (no flakes)
+// evalFlake / pdfFlake (disney.cpp:104-126): a narrow GGX lobe about the flake normal with per-channel Schlick
+// reflectance of the flake colour, both directions in world space.
+V3 evalFlake(V3 wo, V3 wi, const Params& p) {
+    const Frame ff(p.flakeNormal);
+    const V3 o = ff.toLocal(wo), i = ff.toLocal(wi);
+    if (o.z <= 0.0f || i.z <= 0.0f) return V3(0.0f);
+    const V3 h = normalized(o + i);
+    const float D = ggxD(h, FLAKE_ALPHA, FLAKE_ALPHA);
+    const float G = ggxG(o, i, FLAKE_ALPHA, FLAKE_ALPHA);
+    const float c = dot(o, h);
+    const V3 F(schlickF(p.flakeColor.x, c), schlickF(p.flakeColor.y, c), schlickF(p.flakeColor.z, c));
+    return F * (D * G / (4.0f * o.z * i.z));
+}
+
+float pdfFlake(V3 wo, V3 wi, const Params& p) {
+    const Frame ff(p.flakeNormal);
+    const V3 o = ff.toLocal(wo), i = ff.toLocal(wi);
+    if (o.z <= 0.0f || i.z <= 0.0f) return 0.0f;
+    const V3 h = normalized(o + i);
+    const float c = dot(o, h);
+    if (c <= 0.0f) return 0.0f;
+    return ggxVisiblePdf(o, h, FLAKE_ALPHA, FLAKE_ALPHA) / (4.0f * c);
+}
+
+V3 evalLobes(V3 wo, V3 wi, const Params& p, LobeSplit* split = nullptr) {  // :223-311
     const Frame frame = shadingFrame(p.normal, p.tangent);
     const V3 woL = frame.toLocal(wo), wiL = frame.toLocal(wi);
     if (woL.z <= 0.0f) return V3(0.0f);
     const float cosThetaO = woL.z;
     const float coatO = coatTransmit(p, cosThetaO);
     const V3 cTint = coatTintFactor(p);
-    const float stW = clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
+    const float stW = p.inFlake ? 0.0f : clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
     const float opaqueDielectric = (1.0f - p.metallic) * (1.0f - clampf(p.specularTransmission, 0.0f, 1.0f));
-    const float trans = clampf(p.diffuseTransmission, 0.0f, 1.0f) * opaqueDielectric;
+    const float trans = p.inFlake ? 0.0f : clampf(p.diffuseTransmission, 0.0f, 1.0f) * opaqueDielectric;
     if (wiL.z <= 0.0f) {
         V3 ft(0.0f);
         if (trans > 0.0f) {
@@ -567,7 +652,9 @@ V3 evalLobes(V3 wo, V3 wi, const Params& p, LobeSplit* split = nullptr) {  // :2
     const float cosThetaI = wiL.z;
     const float cosThetaD = dot(woL, hL);
     V3 base, baseDiffuse;
-    {
+    if (p.inFlake) {
+        base = evalFlake(wo, wi, p);
+    } else {
         const DielectricLayer dl(p);
         const float diffW = opaqueDielectric * (1.0f - clampf(p.diffuseTransmission, 0.0f, 1.0f)) * (1.0f - p.subsurface);
         const V3 fDiffuse = diffW > 0.0f ? eonEval(p.baseColor, p.diffuseRoughness, wiL, woL) *
@@ -615,8 +702,13 @@ V3 evalLobes(V3 wo, V3 wi, const Params& p, LobeSplit* split = nullptr) {  // :2
 }
 
 void lobeWeights(const Params& p, float& pCoat, float& pSpec, float& pDiffRefl, float& pTrans, float& pSss,
-                 float& pSpecTrans) {  // :313-338 (no flakes)
+                 float& pSpecTrans) {  // :313-338
     pCoat = 0.25f * p.clearCoat;
+    if (p.inFlake) {
+        pSpec = 1.0f - pCoat;
+        pDiffRefl = pTrans = pSss = pSpecTrans = 0.0f;
+        return;
+    }
     const float rem = 1.0f - pCoat;
     const float st = clampf(p.specularTransmission, 0.0f, 1.0f) * (1.0f - p.metallic);
     pSpec = rem * lerpf(0.5f * (st > 0.0f ? 1.0f : dielectricWeight(p)), 1.0f, p.metallic);
@@ -631,7 +723,7 @@ void lobeWeights(const Params& p, float& pCoat, float& pSpec, float& pDiffRefl, 
     pSpecTrans = pSt;
 }
 
-float pdfLobes(V3 wo, V3 wi, const Params& p) {  // :340-378 (no flakes)
+float pdfLobes(V3 wo, V3 wi, const Params& p) {  // :340-378
     const Frame frame = shadingFrame(p.normal, p.tangent);
     const V3 woL = frame.toLocal(wo), wiL = frame.toLocal(wi);
     if (woL.z <= 0.0f) return 0.0f;
@@ -647,9 +739,14 @@ float pdfLobes(V3 wo, V3 wi, const Params& p) {  // :340-378 (no flakes)
     const float cosThetaD = dot(woL, hL);
     if (cosThetaD <= 0.0f) return 0.0f;
     const float pdfDiffuse = cosineHemispherePdf(wiL.z);
-    float ax, ay;
-    anisoAlpha(p.roughness, p.anisotropy, ax, ay);
-    const float pdfSpecular = ggxVisiblePdf(woL, hL, ax, ay) / (4.0f * cosThetaD);
+    float pdfSpecular;
+    if (p.inFlake) {
+        pdfSpecular = pdfFlake(wo, wi, p);
+    } else {
+        float ax, ay;
+        anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+        pdfSpecular = ggxVisiblePdf(woL, hL, ax, ay) / (4.0f * cosThetaD);
+    }
     float pdfCoat = 0.0f;
     if (pCoat > 0.0f) {
         const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
@@ -670,10 +767,19 @@ bool disneySampleFull(const Params& p, V3 wo, float uc, float u0, float u1, Bsdf
         const float alphaC = p.clearCoatRoughness * p.clearCoatRoughness;
         wiL = reflectVec(woL, ggxSampleVisibleNormal(woL, alphaC, alphaC, u0, u1));
         if (wiL.z <= 0.0f) return false;
-    } else if (uc < pCoat + pSpec) {
-        float ax, ay;  // sampleSpecularLobe, :382-396
-        anisoAlpha(p.roughness, p.anisotropy, ax, ay);
-        wiL = reflectVec(woL, ggxSampleVisibleNormal(woL, ax, ay, u0, u1));
+    } else if (uc < pCoat + pSpec) {  // sampleSpecularLobe, :380-396
+        if (p.inFlake) {
+            const Frame ff(p.flakeNormal);
+            const V3 o = ff.toLocal(wo);
+            if (o.z <= 0.0f) return false;
+            const V3 i = reflectVec(o, ggxSampleVisibleNormal(o, FLAKE_ALPHA, FLAKE_ALPHA, u0, u1));
+            if (i.z <= 0.0f) return false;
+            wiL = frame.toLocal(ff.toWorld(i));
+        } else {
+            float ax, ay;
+            anisoAlpha(p.roughness, p.anisotropy, ax, ay);
+            wiL = reflectVec(woL, ggxSampleVisibleNormal(woL, ax, ay, u0, u1));
+        }
         if (!(wiL.z > 0.0f)) return false;
     } else if (uc < pCoat + pSpec + pSpecTrans) {
         const float alpha = ggxClampAlpha(p.roughness * p.roughness);  // sampleSpecTransLobe, :398-405
@@ -1026,6 +1132,12 @@ bool subsurfaceCoeffs(const pe_material_desc& m, const ShadingPoint& sp, V3& sig
         albedo[c] = st > 0.0f ? ss / st : 0.0f;
     }
     return true;
+}
+
+void flakeState(const pe_material_desc& m, const ShadingPoint& sp, bool& inFlake, V3& flakeNormal) {
+    const Params p = resolve(m, sp);
+    inFlake = p.inFlake;
+    flakeNormal = p.flakeNormal;
 }
 
 bool materialIsDelta(const pe_material_desc& m) {

@@ -69,6 +69,7 @@ struct ShadingPoint {
     bool frontFace = true;
     V3 ng;                   // geometric normal, not flipped (SurfaceInteraction::ng)
     float wavelength = 0.0f; // > 0: the path's single wavelength (nm) after a dispersive hit
+    V3 point;                // SurfaceInteraction::point (where flakes are resolved)
 };
 
 struct BsdfSample {  // material.h:37-49
@@ -78,6 +79,8 @@ struct BsdfSample {  // material.h:37-49
     V3 weight;
 };
 
+// DisneyMaterial::resolve's flake state at a shading point (disney.cpp:600-601).
+void flakeState(const pe_material_desc& m, const ShadingPoint& sp, bool& inFlake, V3& flakeNormal);
 bool materialIsDelta(const pe_material_desc& m);
 // Material::isVolumeBoundary / transmitsShadows / isDispersive (material.h:109-122; dielectric.h:48-78;
 // disney.h:76).

@@ -81,6 +81,8 @@ std::shared_ptr<Material> makeMaterial(const pe_material_desc& d) {
     m->setThinFilm(d.thin_film_thickness, d.thin_film_ior);
     m->setEmission(c3(d.emission));
     if (d.has_opacity) m->setOpacity(d.opacity);
+    if (d.flake_density > 0.0f)
+        m->setFlakes(d.flake_density, d.flake_size, d.flake_roughness, c3(d.flake_color));
     return m;
 }
 
@@ -510,11 +512,13 @@ REF_API int ref_microfacet(int which, size_t n, const float* a, const float* b, 
 }
 
 // Material::evalPdf for n (normal, tangent, frontFace, wo, wi) tuples; f: 3 floats, pdf: 1 float each.
-REF_API int ref_bsdf_eval(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
-                  const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+REF_API int ref_bsdf_eval_at(const pe_material_desc* md, size_t n, const float* points, const float* normal,
+                             const float* tangent, const int32_t* frontFace, const float* wo, const float* wi, float* f,
+                             float* pdf) {
     auto mat = makeMaterial(*md);
     for (size_t i = 0; i < n; ++i) {
         SurfaceInteraction si = shadingPoint(normal + 3 * i, tangent + 3 * i, frontFace[i]);
+        if (points) si.point = v3(points + 3 * i);
         si.material = mat.get();
         float p = 0.0f;
         const Color3f v = mat->evalPdf(v3(wo + 3 * i), v3(wi + 3 * i), si, p);
@@ -525,11 +529,35 @@ REF_API int ref_bsdf_eval(const pe_material_desc* md, size_t n, const float* nor
 }
 
 // Material::sampleFull; out per sample: ok, wi xyz, f rgb, pdf, delta, subsurface, passThrough, weight rgb (14 floats).
-REF_API int ref_bsdf_sample(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
-                    const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+REF_API int ref_bsdf_eval(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
+                  const int32_t* frontFace, const float* wo, const float* wi, float* f, float* pdf) {
+    return ref_bsdf_eval_at(md, n, nullptr, normal, tangent, frontFace, wo, wi, f, pdf);
+}
+
+// DisneyMaterial::resolve's flake state at n shading points: out 4 floats each, inFlake, flakeNormal.
+REF_API int ref_flake_resolve(const pe_material_desc* md, size_t n, const float* points, const float* normal,
+                              const float* tangent, const int32_t* frontFace, float* out) {
+    auto mat = makeMaterial(*md);
+    const auto* dm = dynamic_cast<const DisneyMaterial*>(mat.get());
+    if (!dm) return -1;
+    for (size_t i = 0; i < n; ++i) {
+        SurfaceInteraction si = shadingPoint(normal + 3 * i, tangent + 3 * i, frontFace[i]);
+        si.point = v3(points + 3 * i);
+        const DisneyMaterial::ShadingParams p = dm->resolve(si);
+        float* o = out + 4 * i;
+        o[0] = p.inFlake ? 1.0f : 0.0f;
+        o[1] = p.flakeNormal.x; o[2] = p.flakeNormal.y; o[3] = p.flakeNormal.z;
+    }
+    return 0;
+}
+
+REF_API int ref_bsdf_sample_at(const pe_material_desc* md, size_t n, const float* points, const float* normal,
+                               const float* tangent, const int32_t* frontFace, const float* wo, const float* u3,
+                               float* out) {
     auto mat = makeMaterial(*md);
     for (size_t i = 0; i < n; ++i) {
         SurfaceInteraction si = shadingPoint(normal + 3 * i, tangent + 3 * i, frontFace[i]);
+        if (points) si.point = v3(points + 3 * i);
         si.material = mat.get();
         BSDFSample bs;
         const bool ok = mat->sampleFull(v3(wo + 3 * i), si, u3[3 * i], Vec2f(u3[3 * i + 1], u3[3 * i + 2]), bs);
@@ -544,6 +572,11 @@ REF_API int ref_bsdf_sample(const pe_material_desc* md, size_t n, const float* n
         o[11] = bs.weight.r; o[12] = bs.weight.g; o[13] = bs.weight.b;
     }
     return 0;
+}
+
+REF_API int ref_bsdf_sample(const pe_material_desc* md, size_t n, const float* normal, const float* tangent,
+                    const int32_t* frontFace, const float* wo, const float* u3, float* out) {
+    return ref_bsdf_sample_at(md, n, nullptr, normal, tangent, frontFace, wo, u3, out);
 }
 
 // Per-sample radiance along the path of tracePixel (renderer.cpp:139-162) with no AOVs: the Sobol sampler

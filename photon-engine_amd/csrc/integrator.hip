@@ -1202,6 +1202,10 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 if (static_cast<int>(depth) >= lp.max_depth) {
                     alive = false;
                 } else {
+                    // A point on a metallic flake shades with the flake lobe (DisneyMaterial::resolve, disney.cpp:601).
+                    Flake flake;
+                    const Flake* fl = kTransport && (m.flags & kMatFlagFlakes) &&
+                                              resolve_flake(m, si.point, si.normal, flake) ? &flake : nullptr;
                     // Dispersion: the first wavelength-dependent surface picks the path's wavelength (:602-607).
                     if (kTransport && lambda == 0.0f && (m.flags & kMatFlagDispersive)) {
                         V3 w;
@@ -1250,7 +1254,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                     break;
                                 }
                                 float bsdf_pdf = 0.0f;
-                                const V3 f = bsdf_eval_pdf<kTransport>(m, T, si, wo, ls.wi, lambda, bsdf_pdf);
+                                const V3 f = bsdf_eval_pdf<kTransport>(m, T, si, wo, ls.wi, lambda, bsdf_pdf, fl);
                                 if (guided) bsdf_pdf = (1.0f - alpha) * bsdf_pdf + alpha * guide_pdf(gmix, ls.wi);
                                 const float cos_t = fabsf(dot(ls.wi, si.normal));
                                 if (is_black(f) || !(cos_t > 0.0f)) break;
@@ -1272,7 +1276,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                     float4* fr = pb.pfrac + 3 * (static_cast<size_t>(pid) * S + s);
                                     if (depth == 0) {  // the first surface: this sample's lobe shares (:682-690)
                                         V3 sh[3];
-                                        lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, ls.wi, lambda), sh);
+                                        lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, ls.wi, lambda, fl), sh);
                                         fr[0] = make_float4(sh[0].x, sh[0].y, sh[0].z, __uint_as_float(nee_route(true, group)));
                                         fr[1] = make_float4(sh[1].x, sh[1].y, sh[1].z, 0.0f);
                                         fr[2] = make_float4(sh[2].x, sh[2].y, sh[2].z, 0.0f);
@@ -1303,13 +1307,13 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                         // From the mixture: the BSDF value and the mixture density of the direction.
                         bs.wi = guide_sample(gmix, uc / alpha, u0, u1);
                         float p_bsdf = 0.0f;
-                        bs.f = bsdf_eval_pdf<kTransport>(m, T, si, wo, bs.wi, lambda, p_bsdf);
+                        bs.f = bsdf_eval_pdf<kTransport>(m, T, si, wo, bs.wi, lambda, p_bsdf, fl);
                         bs.pdf = (1.0f - alpha) * p_bsdf + alpha * guide_pdf(gmix, bs.wi);
                         bs.delta = bs.pass_through = bs.subsurface = false;
                         sampled = bs.pdf > 0.0f;
                     } else {
                         const float ub = guided ? (uc - alpha) / (1.0f - alpha) : uc;
-                        sampled = bsdf_sample<kTransport>(m, T, si, wo, ub, u0, u1, lambda, bs);
+                        sampled = bsdf_sample<kTransport>(m, T, si, wo, ub, u0, u1, lambda, bs, fl);
                         // The BSDF branch is taken with probability 1 - alpha (path_tracer.cpp:721-730): a
                         // subsurface entry, which the mixture cannot produce, carries its weight / (1 - alpha);
                         // delta and pass-through samples their pdf x (1 - alpha); the rest the mixture density.
@@ -1351,7 +1355,7 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                 sh[1] = splat(refracted ? 0.0f : 1.0f);
                                 sh[2] = splat(refracted ? 1.0f : 0.0f);
                             } else {
-                                lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, bs.wi, lambda), sh);
+                                lobe_shares(bsdf_eval_split<kTransport>(m, T, si, wo, bs.wi, lambda, fl), sh);
                             }
                             for (int k = 0; k < 3; ++k)
                                 pb.lobe[3 * static_cast<size_t>(pid) + k] = make_float4(sh[k].x, sh[k].y, sh[k].z, 0.0f);

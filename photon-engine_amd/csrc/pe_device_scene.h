@@ -74,6 +74,10 @@ struct alignas(16) GpuMaterial {
     // Dielectric interior scattering (setScattering, dielectric.cpp:129-144): sigma_s and the Henyey-Greenstein g
     V3 sigma_s;
     float medium_g;
+    // Disney metallic flakes (setFlakes, disney.h:120-125; kMatFlagFlakes when the density is above 0)
+    V3 flake_color;
+    float flake_density;
+    float flake_size, flake_roughness, pad9, pad10;
 };
 
 // Shading code paths the shade kernel groups a round's hits by (when a scene has more than one).
@@ -92,6 +96,7 @@ enum : uint32_t {
     kMatFlagFilm = 256u,             // thin-film interference on the specular Fresnel / the thin wall
     kMatFlagCutout = 512u,           // Disney opacity below 1: stochastic cutout, shadow transmittance 1 - opacity
     kMatFlagScatter = 1024u,         // interior medium scatters (sigma_s != 0): free flights inside
+    kMatFlagFlakes = 2048u,          // Disney metallic flakes (resolved per shading point)
 };
 
 // AreaLight (rectangle/parallelogram) or TriangleLight with the constructor's derived values.

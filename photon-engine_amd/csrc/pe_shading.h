@@ -4,7 +4,7 @@
 // geometry/ray_spawn.h:25-87; the BSDF is the reference's Principled/OpenPBR layering (materials/disney.cpp)
 // evaluated from a GpuMaterial whose material-only terms the host already computed (scene_prep.cpp).
 // The Dielectric (smooth / rough / thin-walled, with Cauchy dispersion) follows materials/dielectric.cpp.
-// Not on this path (rejected at scene upload): texture maps, metallic flakes, thin-film interference layers.
+// Not on this path (rejected at scene upload): texture maps.
 #pragma once
 
 #include "pe_device_scene.h"
@@ -386,15 +386,124 @@ struct LobeSplit {
     V3 diffuse, glossy, transmission;
 };
 
+// ------------------------------------------------------------------------------------ metallic flakes
+// A shading point on a flake (DisneyMaterial::resolve, disney.cpp:600-601): the flake's normal (its frame is
+// rebuilt where it is used, so that only three floats stay live across the vertex).
+constexpr float kFlakeAlpha = 0.06f;  // FLAKE_ALPHA (disney.cpp:75)
+struct Flake {
+    V3 n;
+};
+
+// lowbias32 and the cell hash of the flake lattice (disney.cpp:407-421).
+__device__ __forceinline__ uint32_t flake_hash(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__device__ __forceinline__ uint32_t flake_cell_hash(int x, int y, int z) {
+    return flake_hash(static_cast<uint32_t>(x) * 0x8da6b343u ^
+                      flake_hash(static_cast<uint32_t>(y) * 0xd8163841u ^ flake_hash(static_cast<uint32_t>(z) * 0xcb1ab31fu)));
+}
+__device__ __forceinline__ float flake_unit(uint32_t h) { return static_cast<float>(h >> 8) * 0x1p-24f; }
+
+// resolveFlake (disney.cpp:430-460): the nearest seed of the 27 cells around the point (one seed per cell of
+// the flake size) decides whether the point is on a flake and how the flake is tilted about the shading normal.
+__device__ __forceinline__ bool resolve_flake(const GpuMaterial& m, V3 point, V3 normal, Flake& fl) {
+#pragma clang fp contract(off)
+    const V3 q = point / m.flake_size;
+    const int cx = static_cast<int>(floorf(q.x)), cy = static_cast<int>(floorf(q.y)), cz = static_cast<int>(floorf(q.z));
+    float best = 1e30f;
+    uint32_t best_hash = 0;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const uint32_t h = flake_cell_hash(cx + dx, cy + dy, cz + dz);
+                const V3 f = mk(static_cast<float>(cx + dx) + flake_unit(h),
+                                static_cast<float>(cy + dy) + flake_unit(flake_hash(h ^ 0x68e31da4u)),
+                                static_cast<float>(cz + dz) + flake_unit(flake_hash(h ^ 0xb5297a4du)));
+                const V3 e = q - f;
+                const float d2 = e.x * e.x + e.y * e.y + e.z * e.z;
+                if (d2 < best) {
+                    best = d2;
+                    best_hash = h;
+                }
+            }
+    if (flake_unit(flake_hash(best_hash ^ 0x1b56c4e9u)) >= m.flake_density) return false;
+    const float u1 = fminf(flake_unit(flake_hash(best_hash ^ 0x2c1b3c6du)), 0.999f);
+    const float u2 = flake_unit(flake_hash(best_hash ^ 0x297a2d39u));
+    const float r = m.flake_roughness;
+    const float tan2 = r * r * u1 / (1.0f - u1);
+    const float cos_t = 1.0f / sqrtf(1.0f + tan2);
+    const float sin_t = sqrtf(fmaxf(0.0f, 1.0f - cos_t * cos_t));
+    const float phi = kTwoPi * u2;
+    fl.n = normalized(to_world(frame_from_normal(normal), mk(sin_t * cosf(phi), sin_t * sinf(phi), cos_t)));
+    return true;
+}
+
+// evalFlake / pdfFlake (disney.cpp:104-126): a narrow GGX lobe about the flake normal, Schlick reflectance of the
+// flake colour per channel; world-space directions.
+__device__ __forceinline__ V3 flake_eval(const GpuMaterial& m, const Flake& fl, V3 wo, V3 wi) {
+    const Frame ff = frame_from_normal(fl.n);
+    const V3 o = to_local(ff, wo), i = to_local(ff, wi);
+    if (o.z <= 0.0f || i.z <= 0.0f) return splat(0.0f);
+    const V3 h = normalized(o + i);
+    const float D = ggx_d(h, kFlakeAlpha, kFlakeAlpha);
+    const float G = ggx_g(o, i, kFlakeAlpha, kFlakeAlpha);
+    const float c = dot(o, h);
+    const V3 col = m.flake_color;
+    return mk(schlick_f(col.x, c), schlick_f(col.y, c), schlick_f(col.z, c)) * (D * G / (4.0f * o.z * i.z));
+}
+__device__ __forceinline__ float flake_pdf(const Flake& fl, V3 wo, V3 wi) {
+    const Frame ff = frame_from_normal(fl.n);
+    const V3 o = to_local(ff, wo), i = to_local(ff, wi);
+    if (o.z <= 0.0f || i.z <= 0.0f) return 0.0f;
+    const V3 h = normalized(o + i);
+    const float c = dot(o, h);
+    if (c <= 0.0f) return 0.0f;
+    return ggx_visible_pdf(o, h, kFlakeAlpha, kFlakeAlpha) / (4.0f * c);
+}
+
 // evalLobes (disney.cpp:223-311) in the local frame: reflection above the surface; diffuse and specular
 // transmission below it. kTransport = false is the variant for scenes whose materials all have zero
 // transmission weights, where the lobes below the surface are exactly zero. With `split` (the light passes),
 // the value's diffuse, glossy (specular, sheen, coat) and transmission parts as well (disney.cpp:250, :290-309).
 template <bool kTransport>
 __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const EnergyTables& T, bool front, V3 wo, V3 wi,
-                                                LobeSplit* split = nullptr) {
+                                                LobeSplit* split = nullptr, const V3* flake_base = nullptr) {
     if (wo.z <= 0.0f) return splat(0.0f);
     const float cos_o = wo.z;
+    if (kTransport && flake_base) {
+        // On a flake (disney.cpp:234-237, :260-261): no transmission, the flake lobe in place of the base layer
+        // (no diffuse, no sheen), under the clear coat.
+        if (wi.z <= 0.0f) {
+            if (split) split->transmission = splat(0.0f);
+            return splat(0.0f);
+        }
+        const V3 base = *flake_base;
+        if (m.clearcoat <= 0.0f) {
+            if (split) {
+                split->diffuse = splat(0.0f);
+                split->glossy = base;
+            }
+            return base;
+        }
+        const V3 h = normalized(wo + wi);
+        const float cos_d = dot(wo, h), cos_i = wi.z;
+        const float dc = ggx_d(h, m.alpha_c, m.alpha_c);
+        const float gc = ggx_g(wo, wi, m.alpha_c, m.alpha_c);
+        const float fc = m.f0c + (1.0f - m.f0c) * pow5(1.0f - clampf(cos_d, 0.0f, 1.0f));
+        const float ms_c = 1.0f + m.f0c * (1.0f / ggx_directional_albedo(T, cos_o, m.alpha_c) - 1.0f);
+        const V3 f_coat = splat(m.clearcoat * fc * dc * gc / (4.0f * cos_o * cos_i) * ms_c);
+        const V3 coat_att = m.coat_tint_factor * (coat_transmit(m, T, cos_o) * coat_transmit(m, T, cos_i));
+        if (split) {
+            split->diffuse = splat(0.0f);
+            split->glossy = base * coat_att + f_coat;
+        }
+        return base * coat_att + f_coat;
+    }
     if (wi.z <= 0.0f) {
         V3 ft = splat(0.0f);
         if (!kTransport) return ft;
@@ -466,8 +575,18 @@ __device__ __forceinline__ V3 disney_eval_local(const GpuMaterial& m, const Ener
 
 // pdfLobes (disney.cpp:340-378).
 template <bool kTransport>
-__device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, bool front, V3 wo, V3 wi) {
+__device__ __forceinline__ float disney_pdf_local(const GpuMaterial& m, bool front, V3 wo, V3 wi,
+                                                  const float* flake_pdf_v = nullptr) {
     if (wo.z <= 0.0f) return 0.0f;
+    if (kTransport && flake_pdf_v) {  // on a flake: coat and the flake lobe only (lobeWeights, disney.cpp:315-322)
+        if (wi.z <= 0.0f) return 0.0f;
+        const V3 h = normalized(wo + wi);
+        const float cos_d = dot(wo, h);
+        if (cos_d <= 0.0f) return 0.0f;
+        float pdf_coat = 0.0f;
+        if (m.p_coat > 0.0f) pdf_coat = ggx_visible_pdf(wo, h, m.alpha_c, m.alpha_c) / (4.0f * cos_d);
+        return (1.0f - m.p_coat) * *flake_pdf_v + m.p_coat * pdf_coat;
+    }
     if (wi.z <= 0.0f) {
         float pdf = 0.0f;
         if (!kTransport) return pdf;
@@ -575,7 +694,7 @@ __device__ __forceinline__ V3 thin_wall_transmittance(const GpuMaterial& m, cons
 // kTransport = false: the scene has no Dielectric and no transmitting or subsurface Disney material.
 template <bool kTransport>
 __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
-                                            V3 wi, float lambda, float& pdf) {
+                                            V3 wi, float lambda, float& pdf, const Flake* fl = nullptr) {
     if (m.type == kMatLambert) {  // lambertian.cpp:38-60
         const Frame f = frame_from_normal(si.normal);
         const float ci = to_local(f, wi).z, co = to_local(f, wo).z;
@@ -603,6 +722,12 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
     }
     const Frame f = shading_frame(si.normal, si.tangent);
     const V3 wol = to_local(f, wo), wil = to_local(f, wi);
+    if (kTransport && fl) {
+        const float fp = flake_pdf(*fl, wo, wi);
+        const V3 fb = flake_eval(m, *fl, wo, wi);
+        pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil, &fp);
+        return disney_eval_local<kTransport>(m, T, si.front_face, wol, wil, nullptr, &fb);
+    }
     pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil);
     return disney_eval_local<kTransport>(m, T, si.front_face, wol, wil);
 }
@@ -612,11 +737,14 @@ __device__ __forceinline__ V3 bsdf_eval_pdf(const GpuMaterial& m, const EnergyTa
 // geometric normal's side of wo is diffuse (glossy for glass), across it transmission.
 template <bool kTransport>
 __device__ __forceinline__ LobeSplit bsdf_eval_split(const GpuMaterial& m, const EnergyTables& T, const Surface& si,
-                                                     V3 wo, V3 wi, float lambda) {
+                                                     V3 wo, V3 wi, float lambda, const Flake* fl = nullptr) {
     LobeSplit sp{splat(0.0f), splat(0.0f), splat(0.0f)};
     if (m.type == kMatDisney) {
         const Frame f = shading_frame(si.normal, si.tangent);
-        (void)disney_eval_local<kTransport>(m, T, si.front_face, to_local(f, wo), to_local(f, wi), &sp);
+        V3 fb;
+        if (kTransport && fl) fb = flake_eval(m, *fl, wo, wi);
+        (void)disney_eval_local<kTransport>(m, T, si.front_face, to_local(f, wo), to_local(f, wi), &sp,
+                                            kTransport && fl ? &fb : nullptr);
         return sp;
     }
     float pdf = 0.0f;
@@ -690,7 +818,8 @@ __device__ __forceinline__ bool glass_sample(const GpuMaterial& m, const Surface
 // the specular-transmission and subsurface lobes have zero probability and are left out.
 template <bool kTransport>
 __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTables& T, const Surface& si, V3 wo,
-                                            float uc, float u0, float u1, float lambda, BsdfSample& out) {
+                                            float uc, float u0, float u1, float lambda, BsdfSample& out,
+                                            const Flake* fl = nullptr) {
     out.delta = false;
     out.pass_through = false;
     out.subsurface = false;
@@ -744,6 +873,33 @@ __device__ __forceinline__ bool bsdf_sample(const GpuMaterial& m, const EnergyTa
     const V3 wol = to_local(f, wo);
     if (wol.z <= 0.0f) return false;
     V3 wil;
+    if (kTransport && fl) {
+        // On a flake (lobeWeights disney.cpp:315-322, sampleFull :689-729): the coat, else the flake lobe sampled
+        // in the flake's frame (sampleSpecularLobe :380-396); the remaining branch is the (empty) transmission.
+        const float p_spec = 1.0f - m.p_coat;
+        if (uc < m.p_coat) {
+            wil = reflect_vec(wol, ggx_sample_visible_normal(wol, m.alpha_c, m.alpha_c, u0, u1));
+            if (wil.z <= 0.0f) return false;
+        } else if (uc < m.p_coat + p_spec) {
+            const Frame ff = frame_from_normal(fl->n);
+            const V3 o = to_local(ff, wo);
+            if (o.z <= 0.0f) return false;
+            const V3 i = reflect_vec(o, ggx_sample_visible_normal(o, kFlakeAlpha, kFlakeAlpha, u0, u1));
+            if (i.z <= 0.0f) return false;
+            wil = to_local(f, to_world(ff, i));
+            if (!(wil.z > 0.0f)) return false;
+        } else {
+            wil = cosine_sample_hemisphere(u0, u1);
+            wil.z = -wil.z;
+        }
+        out.wi = normalized(to_world(f, wil));
+        const V3 wil2 = to_local(f, out.wi);
+        const float fp = flake_pdf(*fl, wo, out.wi);
+        const V3 fb = flake_eval(m, *fl, wo, out.wi);
+        out.f = disney_eval_local<kTransport>(m, T, si.front_face, wol, wil2, nullptr, &fb);
+        out.pdf = disney_pdf_local<kTransport>(m, si.front_face, wol, wil2, &fp);
+        return out.pdf > 0.0f;
+    }
     if (uc < m.p_coat) {
         wil = reflect_vec(wol, ggx_sample_visible_normal(wol, m.alpha_c, m.alpha_c, u0, u1));
         if (wil.z <= 0.0f) return false;

@@ -108,6 +108,13 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
         g.opacity = clampf(d.opacity, 0.0f, 1.0f);
         if (g.opacity < 1.0f) g.flags |= kMatFlagCutout;
     }
+    if (d.flake_density > 0.0f) {  // setFlakes clamps (disney.h:120-125); resolve() flakes when density > 0 (:601)
+        g.flake_density = clampf(d.flake_density, 0.0f, 1.0f);
+        g.flake_size = std::max(d.flake_size, 1e-6f);
+        g.flake_roughness = clampf(d.flake_roughness, 0.0f, 1.0f);
+        g.flake_color = clamp01v(c3(d.flake_color));
+        if (g.flake_density > 0.0f) g.flags |= kMatFlagFlakes;
+    }
 
     // resolvedSheenColor (disney.cpp:160-166)
     V3 sc = splat(0.0f);
@@ -138,7 +145,7 @@ int prepare_material(const pe_material_desc& d, const EnergyTables& T, GpuMateri
     g.alpha_c = g.clearcoat_roughness * g.clearcoat_roughness;
     g.coat_tint_factor = g.clearcoat > 0.0f ? lerp3(splat(1.0f), g.coat_tint, g.clearcoat) : splat(1.0f);
 
-    // lobeWeights (disney.cpp:313-338), no flakes
+    // lobeWeights (disney.cpp:313-338) off a flake (a flake point's weights are taken at the shading point)
     const float p_coat = 0.25f * g.clearcoat;
     const float rem = 1.0f - p_coat;
     const float st = clampf(g.specular_transmission, 0.0f, 1.0f) * (1.0f - g.metallic);
@@ -467,11 +474,12 @@ int prepare_materials(pe_device* dev, const pe_scene_desc& desc, pe_scene& sc) {
                                  g.p_sss > 0.0f;
         transport = transport || g.type == kMatDielectric || lobes_below ||
                     (g.flags & (kMatFlagVolume | kMatFlagThinWall | kMatFlagDispersive | kMatFlagMedium |
-                                kMatFlagFilm | kMatFlagCutout | kMatFlagScatter)) != 0;
+                                kMatFlagFilm | kMatFlagCutout | kMatFlagScatter | kMatFlagFlakes)) != 0;
         g.sort_class = g.type == kMatDielectric ? kSortClassDielectric
                        : g.type != kMatDisney   ? kSortClassSimple
-                       : (lobes_below || (g.flags & (kMatFlagFilm | kMatFlagCutout))) ? kSortClassDisneyTransport
-                                                                   : kSortClassDisney;
+                       : (lobes_below || (g.flags & (kMatFlagFilm | kMatFlagCutout | kMatFlagFlakes)))
+                           ? kSortClassDisneyTransport
+                           : kSortClassDisney;
         if (i < num_mats) classes |= 1u << g.sort_class;
     }
     hipStream_t s = dev->stream;

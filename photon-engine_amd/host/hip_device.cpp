@@ -9,7 +9,7 @@
 // What the HIP path does not implement is refused, never approximated: setScene leaves the device without a
 // scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle / static
 // MeshInstance, visibility masks other than all rays or all but camera rays, shadow catchers, lights other
-// than AreaLight / TriangleLight, light linking, rounded edges, textured or flaked Disney materials and
+// than AreaLight / TriangleLight, light linking, rounded edges, textured Disney materials and
 // materials other than Disney / Lambertian / Dielectric; render() returns completed = false for AO. Pinhole,
 // thin-lens and orthographic cameras render, with motion blur of the camera (moving instances are refused).
 // Path guiding uses the library's learned guide in place of the reference's SD-tree.
@@ -104,10 +104,15 @@ bool packMaterial(const Material* m, pe_material_desc& o, std::string& err) {
     SurfaceInteraction si;
     putc(o.emission, m->emitted(si));
     if (auto* d = dynamic_cast<const DisneyMaterial*>(m)) {
-        if (d->flakeDensity() > 0.0f || d->albedoImage() || d->normalImage() || d->roughnessImage() ||
-            d->metalnessImage() || d->opacityImage()) {
-            err = "DisneyMaterial with texture maps or flakes is not implemented on the HIP device";
+        if (d->albedoImage() || d->normalImage() || d->roughnessImage() || d->metalnessImage() || d->opacityImage()) {
+            err = "DisneyMaterial with texture maps is not implemented on the HIP device";
             return false;
+        }
+        if (d->flakeDensity() > 0.0f) {  // metallic flakes (setFlakes, disney.h:120-125)
+            o.flake_density = d->flakeDensity();
+            o.flake_size = d->flakeSize();
+            o.flake_roughness = d->flakeRoughness();
+            putc(o.flake_color, d->flakeColor());
         }
         if (d->baseOpacity() < 1.0f) {  // a cutout (setOpacity, disney.h:147-158)
             o.opacity = d->baseOpacity();

@@ -58,7 +58,8 @@ class MaterialDesc(C.Structure):
         ("abbe", C.c_float), ("absorption_color", F3), ("absorption_distance", C.c_float),
         ("thin_walled", C.c_int32), ("opacity", C.c_float), ("has_opacity", C.c_int32),
         ("scatter_density", C.c_float), ("scatter_color", C.c_float * 3), ("scatter_anisotropy", C.c_float),
-        ("reserved", C.c_float * 3),
+        ("flake_density", C.c_float), ("flake_size", C.c_float), ("flake_roughness", C.c_float),
+        ("flake_color", F3), ("reserved", C.c_float * 3),
     ]
 
 

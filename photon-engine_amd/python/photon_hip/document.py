@@ -17,7 +17,7 @@ What each part follows:
   * read_mesh_binary / write_mesh_binary: scene/mesh_binary.cpp (PHMESH01);
   * read_hdr: the Radiance RGBE format (flat and run-length scanlines) decoded as stb_image decodes it.
 
-What the HIP path does not render is refused with DocumentError, not approximated: texture maps, flakes and
+What the HIP path does not render is refused with DocumentError, not approximated: texture maps and
 rounded edges; point, spot and directional lights and light linking;
 the physical sky; shadow catchers; OpenEXR, glTF, FBX, Alembic and USD files. Analytic sphere nodes are refused
 unless the caller asks for them tessellated (sphere_segments), as the reference's USD arm of its golden test
@@ -489,8 +489,11 @@ def _material(j: dict, base: Path):
         m.thin_film_thickness = _f(film, "thickness", 0.0)
         m.thin_film_ior = _f(film, "ior", 1.4)
     fl = j.get("flakes")
-    if isinstance(fl, dict) and _f(fl, "density", 0.0) > 0.0:
-        why.append("metallic flakes")
+    if isinstance(fl, dict):  # material_library.cpp:254-259 (MaterialPreset defaults, material_library.h:73-76)
+        m.flake_density = _f(fl, "density", 0.0)
+        m.flake_size = _f(fl, "size", 0.004)
+        m.flake_roughness = _f(fl, "roughness", 0.3)
+        m.flake_color[:] = list(_vec3(fl, "color", (0.9, 0.9, 0.9)))
     return m, ", ".join(why) or None
 
 

@@ -39,6 +39,8 @@ def disney(base_color, metallic=0.0, roughness=0.5, specular=0.5, **kw) -> capi.
     m.absorption_color[:] = [1.0, 1.0, 1.0]
     m.absorption_distance = 0.0
     m.scatter_color[:] = [1.0, 1.0, 1.0]
+    m.flake_density, m.flake_size, m.flake_roughness = 0.0, 0.004, 0.3   # setFlakes defaults (disney.h:237-240)
+    m.flake_color[:] = [0.9, 0.9, 0.9]
     if "opacity" in kw:                      # setOpacity (disney.h:148)
         m.has_opacity = 1
     for k, v in kw.items():

@@ -318,6 +318,21 @@ def test_project_compiles_as_build_render_scene(tmp_path):
     np.testing.assert_allclose(np.linalg.norm(np.subtract(v.position, v.target)), 4.0, rtol=1e-6)
 
 
+def test_project_flakes_load_as_material_presets(tmp_path):
+    """A material's "flakes" block (material_library.cpp:254-259) becomes setFlakes(density, size, roughness,
+    color) with the preset's defaults for missing keys (material_library.h:73-76)."""
+    mats = [{"type": "disney", "baseColor": [0.5, 0.1, 0.1], "flakes": {"density": 0.4, "color": [1, 0.8, 0.5]}},
+            {"type": "disney", "flakes": {"density": 0.9, "size": 0.002, "roughness": 0.7}}]
+    path, _ = _project(tmp_path, materials=mats)
+    pr = D.load_project(path)
+    sc = D.build_render_scene(pr)
+    m0, m1 = sc.materials[0], sc.materials[1]
+    assert (m0.flake_density, m0.flake_size, m0.flake_roughness) == (np.float32(0.4), np.float32(0.004), np.float32(0.3))
+    assert tuple(m0.flake_color) == (1.0, np.float32(0.8), 0.5)
+    assert (m1.flake_density, m1.flake_size, m1.flake_roughness) == (np.float32(0.9), np.float32(0.002), np.float32(0.7))
+    assert tuple(m1.flake_color) == (np.float32(0.9),) * 3
+
+
 def test_project_cameras_as_camera_from_json(tmp_path):
     """cameraFromJson (main.cpp:89-111): the camera block's aperture makes a ThinLensCamera focused at r (or the
     explicit focus distance) with its blades, rotation and lens shift; orthographic makes an OrthographicCamera
@@ -355,7 +370,6 @@ def test_project_cameras_as_camera_from_json(tmp_path):
 def test_project_refuses_what_the_hip_path_does_not_render(tmp_path):
     cases = [
         ("materials", [{"type": "disney", "maps": {"albedo": "a.png"}}], "texture maps"),
-        ("materials", [{"type": "disney", "flakes": {"density": 0.5}}], "flakes"),
         ("lights", [{"type": "point", "intensity": 1.0}], "point lights"),
         ("environment", {"kind": "sky", "intensity": 1.0}, "physical sky"),
         ("environment", {"intensity": 0.0, "ground": {"enabled": True, "shadowCatcher": True}}, "shadow catcher"),
