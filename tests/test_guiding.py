@@ -229,6 +229,36 @@ def test_guided_render_converges_to_plain(dev, small_room):
 
 
 @pytest.mark.gpu
+def test_guided_mean_within_sigma_of_plain_over_independent_ranges(dev, small_room):
+    """Unbiasedness as a statistical test: a guide trained for 4 launches and then frozen renders 12 disjoint
+    64-spp sample ranges, plain path tracing 12 others; for the image mean and each quadrant's mean the guided and
+    plain estimates differ by less than 4 standard errors of their difference. The test's reach is printed: the
+    smallest relative bias it would flag (4 sigma of the difference over the mean)."""
+    sc, view, gs = small_room
+    g = _guide(dev, gs, seed=17, batch=16384, steps_per_launch=8, train_launches=4)
+    _render(dev, gs, view, 4 * 8, guide=g, first=1 << 16)  # the training launches
+    K, spp = 12, 64
+    guided = np.stack([_render(dev, gs, view, spp, guide=g, first=k * spp) for k in range(K)])
+    plain = np.stack([_render(dev, gs, view, spp, first=(K + k) * spp) for k in range(K)])
+    h, w = view.height // 2, view.width // 2
+    regions = {"image": (slice(None), slice(None))}
+    for qy in range(2):
+        for qx in range(2):
+            regions[f"q{qy}{qx}"] = (slice(qy * h, (qy + 1) * h), slice(qx * w, (qx + 1) * w))
+    reach = []
+    for name, (ys, xs) in regions.items():
+        a = guided[:, ys, xs].mean(axis=(1, 2, 3))
+        b = plain[:, ys, xs].mean(axis=(1, 2, 3))
+        se = np.sqrt(a.var(ddof=1) / K + b.var(ddof=1) / K)
+        diff = a.mean() - b.mean()
+        reach.append(4.0 * se / b.mean())
+        assert abs(diff) <= 4.0 * se, (name, diff, se)
+    print(f"guided vs plain over {K} independent ranges: would flag a relative bias above {max(reach):.4f} "
+          f"(image {reach[0]:.4f})", flush=True)
+    assert reach[0] < 0.05
+
+
+@pytest.mark.gpu
 def test_guiding_reduces_error_on_the_slit_room(dev):
     """At equal sample count, after 24 launches x 32 Adam steps of online training, the guided image of the
     door-gap room has less error than the plain one (measured: about 0.83 x at 240 x 135)."""
