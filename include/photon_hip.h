@@ -284,7 +284,9 @@ typedef struct pe_settings {
     int32_t motion_blur;        /* RenderSettings::motionBlur: every camera ray draws a shutter time after its lens
                                    sample (renderer.cpp:151-153); only the camera moves (moving instances are
                                    refused at upload) */
-    int32_t reserved;
+    float ao_strength;          /* RenderSettings::aoStrength: at the first vertex, shadow_quality cosine rays of
+                                   0.35 scene units scale the path by lerp(1, unoccluded fraction, strength)
+                                   (approxAo, path_tracer.cpp:27-44, :617-621); 0 = off */
 } pe_settings;
 
 /* SampleBatchResult (engine/renderer.h:114-117) plus the device's own ray and traversal counters. */

@@ -83,6 +83,7 @@ _REF_SIGS = {
                                 _F, C.c_float]),
     "camera_describe_full": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, _F]),
     "camera_shift": (C.c_int, [C.c_float, C.c_float]),
+    "ao_strength": (C.c_int, [C.c_float]),
     "camera_rays": (C.c_int, [_F, _F, _F, C.c_float, C.c_float, C.c_size_t, _F, _F]),
     "microfacet": (C.c_int, [C.c_int, C.c_size_t, _F, _F, _F, _F]),
     "bsdf_eval": (C.c_int, [_P(capi.MaterialDesc), C.c_size_t, _F, _F, _I, _F, _F, _F, _F]),
@@ -317,16 +318,17 @@ class reference_camera:
 
     def __init__(self, strict: bool = False, kind="perspective", aperture=0.0, focus=1.0, blades=0, rotation_deg=0.0,
                  extent=2.0, motion_blur=False, end_position=None, end_target=None, end_aperture=0.0,
-                 shift=(0.0, 0.0), harness: bool = False):
+                 shift=(0.0, 0.0), ao_strength=0.0, harness: bool = False):
         """harness: select the camera of the HipDevice harness (its own copy of the reference driver)."""
         self.lib = hip_device_harness() if harness else reference(strict)
         if self.lib is None:
             raise FileNotFoundError("oracle/_ref was not built (the reference tree is absent)")
         if harness:
-            for name in ("camera_select", "camera_shift"):
+            for name in ("camera_select", "camera_shift", "ao_strength"):
                 getattr(self.lib, "ref_" + name).argtypes = _REF_SIGS[name][1]
                 getattr(self.lib, "ref_" + name).restype = C.c_int
         self.shift = (float(shift[0]), float(shift[1]))
+        self.ao = float(ao_strength)
         self.args = (CAMERA_KINDS[kind], float(aperture), float(focus), int(blades), float(rotation_deg),
                      float(extent), int(bool(motion_blur)), int(end_position is not None),
                      (C.c_float * 3)(*map(float, end_position or (0, 0, 0))),
@@ -335,6 +337,7 @@ class reference_camera:
     def __enter__(self):
         self.lib.ref_camera_select(*self.args)
         self.lib.ref_camera_shift(*self.shift)
+        self.lib.ref_ao_strength(self.ao)
         return self
 
     def __exit__(self, *exc):

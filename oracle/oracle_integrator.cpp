@@ -1017,6 +1017,7 @@ struct Settings {
     int maxDepth = 8, rrDepth = 3, shadowSamples = 1;
     float clampIndirect = 0.0f;
     bool motionBlur = false;  // RenderSettings::motionBlur: tracePixel draws a shutter time (renderer.cpp:152)
+    float aoStrength = 0.0f;  // RenderSettings::aoStrength: contact AO at the first vertex (path_tracer.cpp:617-621)
 };
 
 struct RayCounters {
@@ -1124,6 +1125,27 @@ bool randomWalkSubsurface(const Scene& scene, const SI& entry, V3 dir, V3 sigmaT
         if (step >= 8 && !russianRoulette(throughput, sampler)) return false;
     }
     return false;
+}
+
+// approxAo (path_tracer.cpp:27-44): `samples` cosine-distributed any-hit rays of 0.35 m about the shading normal;
+// the throughput factor lerp(1, unoccluded fraction, strength).
+float approxAo(const Scene& scene, const SI& isect, Sobol& sampler, float strength, int samples) {
+    if (strength <= 0.0f || samples <= 0) return 1.0f;
+    const Frame frame(isect.normal);
+    float occ = 0.0f;
+    for (int i = 0; i < samples; ++i) {
+        float u0, u1;
+        sampler.get2D(u0, u1);
+        const V3 dir = normalized(frame.toWorld(cosineSampleHemisphere(u0, u1)));
+        Ray r;
+        r.o = spawnOrigin(isect, dir);
+        r.d = dir;
+        r.tMin = 0.0f;
+        r.tMax = 0.35f;
+        if (intersectAny(scene, r)) occ += 1.0f;
+    }
+    const float visibility = 1.0f - (occ / static_cast<float>(samples));
+    return lerpf(1.0f, visibility, strength);
 }
 
 // passesCutout (path_tracer.cpp:159-167): opacity 1 never passes, 0 always, in between a sampler draw (one
@@ -1388,6 +1410,8 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
             lambda = sampleWavelength(sampler.get1D(), w);
             throughput *= w;
         }
+        if (st.aoStrength > 0.0f && depth == 0)  // contact AO (:617-621)
+            throughput *= approxAo(scene, isect, sampler, st.aoStrength, shadowSamples);
         const bool isSpecular = materialIsDelta(mat);
         const ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace, isect.ng, lambda, isect.point};
         if (lightCount > 0 && !isSpecular) {  // NEE, :631-698
@@ -1848,6 +1872,7 @@ ORACLE_API int oracle_trace_samples(void* sp, const pe_camera* cam, const pe_set
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
         st.motionBlur = ps->motion_blur != 0;
+        st.aoStrength = ps->ao_strength;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1880,6 +1905,7 @@ ORACLE_API int oracle_trace_aovs(void* sp, const pe_camera* cam, const pe_settin
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
         st.motionBlur = ps->motion_blur != 0;
+        st.aoStrength = ps->ao_strength;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1908,6 +1934,7 @@ ORACLE_API int oracle_trace_passes(void* sp, const pe_camera* cam, const pe_sett
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
         st.motionBlur = ps->motion_blur != 0;
+        st.aoStrength = ps->ao_strength;
         Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         Sobol sampler(ps->seed);
         RayCounters rc;
@@ -1938,6 +1965,7 @@ ORACLE_API int oracle_render(void* sp, const pe_camera* cam, const pe_settings* 
         st.shadowSamples = std::max(1, ps->shadow_quality);
         st.clampIndirect = ps->clamp_indirect > 0.0f ? ps->clamp_indirect : 0.0f;
         st.motionBlur = ps->motion_blur != 0;
+        st.aoStrength = ps->ao_strength;
         const Tracer tr{scene, it, st, *cam, FilterSampler(ps->filter_type, ps->filter_width), width, height, ps->seed};
         if (row_end <= row_begin) {
             row_begin = 0;

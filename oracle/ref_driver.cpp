@@ -123,6 +123,7 @@ struct CameraSpec {
     bool hasEnd = false, motionBlur = false;
     float endPos[3] = {0, 0, 0}, endTarget[3] = {0, 0, -1}, endAperture = 0.0f;
     float shiftX = 0.0f, shiftY = 0.0f;  // setShift of a perspective or thin-lens camera (both poses)
+    float aoStrength = 0.0f;             // RenderSettings::aoStrength of the renders and traces below
 };
 CameraSpec g_camera;
 
@@ -420,6 +421,11 @@ REF_API int ref_occluded(void* h, size_t n, const float* rays, uint8_t* occ) {
     return 0;
 }
 
+REF_API int ref_ao_strength(float strength) {
+    g_camera.aoStrength = strength;
+    return 0;
+}
+
 REF_API int ref_camera_shift(float shiftX, float shiftY) {
     g_camera.shiftX = shiftX;
     g_camera.shiftY = shiftY;
@@ -593,7 +599,7 @@ REF_API int ref_trace_samples(void* h, const float* camPos, const float* camTarg
     pf.type = static_cast<PixelFilterType>(filterType);
     pf.width = filterWidth;
     const PixelFilterSampler filter(pf);
-    PathTracer integrator(maxBounces, 3, 0.0f, shadowQuality);
+    PathTracer integrator(maxBounces, 3, g_camera.aoStrength, shadowQuality);
     SobolSampler sampler(seed);
     for (size_t i = 0; i < n; ++i) {
         sampler.startPixel(px[i], py[i]);
@@ -631,7 +637,7 @@ REF_API int ref_trace_passes(void* h, const float* camPos, const float* camTarge
     pf.type = static_cast<PixelFilterType>(filterType);
     pf.width = filterWidth;
     const PixelFilterSampler filter(pf);
-    PathTracer integrator(maxBounces, 3, 0.0f, shadowQuality);
+    PathTracer integrator(maxBounces, 3, g_camera.aoStrength, shadowQuality);
     SobolSampler sampler(seed);
     for (size_t i = 0; i < n; ++i) {
         sampler.startPixel(px[i], py[i]);
@@ -692,6 +698,7 @@ REF_API int ref_render(void* h, const float* camPos, const float* camTarget, con
     s.filter.type = static_cast<PixelFilterType>(filterType);
     s.filter.width = filterWidth;
     s.motionBlur = g_camera.motionBlur;
+    s.aoStrength = g_camera.aoStrength;
     Film film(width, height);
     Renderer renderer;
     const auto t0 = std::chrono::steady_clock::now();
@@ -766,6 +773,7 @@ REF_API int ref_render_aovs(void* h, const float* camPos, const float* camTarget
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
     s.motionBlur = g_camera.motionBlur;
+    s.aoStrength = g_camera.aoStrength;
     Film film(width, height);
     AovFilms aovs;
     aovs.resize(width, height);
@@ -804,6 +812,7 @@ REF_API int ref_render_adaptive(void* h, const float* camPos, const float* camTa
     s.filter.type = static_cast<PixelFilterType>(0);  // box, width 1: the filter the oracle's tests use
     s.filter.width = 1.0f;
     s.motionBlur = g_camera.motionBlur;
+    s.aoStrength = g_camera.aoStrength;
     s.adaptiveSampling = true;
     s.noiseThreshold = threshold;
     s.minAdaptiveSamples = minSpp;
@@ -880,6 +889,7 @@ REF_API int ref_render_passes(void* h, const float* camPos, const float* camTarg
     s.filter.type = static_cast<PixelFilterType>(0);
     s.filter.width = 1.0f;
     s.motionBlur = g_camera.motionBlur;
+    s.aoStrength = g_camera.aoStrength;
     Film film(width, height);
     RenderPassFilms pf;
     pf.resize(width, height);

@@ -169,6 +169,8 @@ constexpr uint32_t kExtWalkStepShift = 10u, kExtWalkStepMask = 0x1FFu << kExtWal
 constexpr uint32_t kExtWalk = 1u << 19;
 constexpr uint32_t kExtRecShift = 20u, kExtRecMask = 3u << kExtRecShift;  // guiding records written (0..2)
 constexpr uint32_t kExtRecOpen = 1u << 22;  // the last record waits for the radiance baseline
+constexpr uint32_t kExtAo = 1u << 23;       // contact AO rays of the first vertex are out (approxAo)
+constexpr uint32_t kAoTag = 1u << 31;       // a shadow-queue entry that is an AO ray (its slot: an AO record)
 constexpr int kGuideRecords = 2;
 constexpr uint32_t kMaxWalkSteps = 256;   // randomWalkSubsurface's step limit (path_tracer.cpp:270)
 constexpr uint32_t kMaxPassThroughs = 64;  // path_tracer.cpp:783
@@ -254,6 +256,9 @@ struct LaunchParams {
     float cam_blade_rotation;
     int cam_moving;           // motion blur with a shutter-end pose: rays blend the poses
     uint32_t camera_dims;     // Sobol dimensions tracePixel draws before Li: filter, lens (+ shutter time)
+    float ao_strength;        // contact AO at the first vertex (0: off)
+    uint32_t ao_base;         // pend index of the AO records (pool ids x shadow samples)
+    int shadow_slots;         // shadow-queue entries per path id: the NEE samples, twice that with AO
 };
 
 // clamp01 (renderer.cpp:41-43) of c, with w.
@@ -466,9 +471,17 @@ struct ShadowJob {
         return TraceRay{mk(a.x, a.y, a.z), mk(b.x, b.y, b.z), 0.0f, a.w, __float_as_uint(b.w)};
     }
     __device__ __forceinline__ void any(uint32_t, const TraceRay& r, bool occluded) const {
-        if (occluded) pend[r.tag] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (occluded) pend[r.tag & ~kAoTag] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 };
+
+// approxAo's factor (path_tracer.cpp:42-43) from a path's AO records (1 each unoccluded ray, 0 each blocked one).
+__device__ __forceinline__ float ao_factor(const LaunchParams& lp, const PathBuffers& pb, uint32_t pid, int S) {
+    float occ = 0.0f;
+    for (int a = 0; a < S; ++a) occ += 1.0f - pb.pend[lp.ao_base + static_cast<size_t>(pid) * S + a].x;
+    const float visibility = 1.0f - (occ / static_cast<float>(S));
+    return 1.0f + lp.ao_strength * (visibility - 1.0f);
+}
 
 __device__ __forceinline__ V3 clamp_indirect(V3 c, float limit) {
     if (limit > 0.0f) {
@@ -575,6 +588,10 @@ struct TransmitShadowJob {
     }
     __device__ __forceinline__ bool closest(uint32_t, const TraceRay& r, float t, int32_t tri, float u, float v,
                                             float4& a, float4& b) const {
+        if (r.tag & kAoTag) {  // an AO ray: any surface within reach blocks it (intersectAny, path_tracer.cpp:40)
+            if (tri >= 0) pend[r.tag & ~kAoTag] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return false;
+        }
         float4 p = pend[r.tag];
         const uint32_t bits = __float_as_uint(p.w);
         V3 c = mk(p.x, p.y, p.z);
@@ -642,13 +659,15 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {  // splitmix64 finaliser
 
 // A path's result: its radiance plus the held-back NEE of its last vertex, into its sample's slot
 // (the sum Film::addSample receives, path_tracer.cpp:355-359 and the end of Li).
+// `ao`: the first vertex's contact-AO factor when its NEE records are folded (1 otherwise).
 __device__ __forceinline__ V3 finish_radiance(const LaunchParams& lp, const PathBuffers& pb, uint32_t pid, uint32_t g,
-                                              V3 L, int S) {
+                                              V3 L, int S, float ao = 1.0f) {
     for (int s = 0; s < S; ++s) {
         const size_t k = static_cast<size_t>(pid) * S + s;
         const float4 e = pb.pend[k];
-        L += mk(e.x, e.y, e.z);
-        if (lp.want_passes) credit_nee(pb, lp, pid, g, k, mk(e.x, e.y, e.z));
+        const V3 c = mk(e.x, e.y, e.z) * ao;
+        L += c;
+        if (lp.want_passes) credit_nee(pb, lp, pid, g, k, c);
         pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     return L;
@@ -739,7 +758,9 @@ __global__ void __launch_bounds__(kBlock) k_regen(LaunchParams lp, PathBuffers p
         if (entry & kDeferred) {
             const uint4 ids = pb.ids[pid];
             const float4 L4 = pb.L[pid];
-            const V3 L = finish_radiance(lp, pb, pid, ids.x, mk(L4.x, L4.y, L4.z), S);
+            // A path that ended at its first vertex with contact-AO rays out: their factor scales that vertex's NEE.
+            const float ao = lp.ao_strength > 0.0f && (pb.ext[pid].w & kExtAo) ? ao_factor(lp, pb, pid, S) : 1.0f;
+            const V3 L = finish_radiance(lp, pb, pid, ids.x, mk(L4.x, L4.y, L4.z), S, ao);
             pb.res[ids.x] = make_float4(L.x, L.y, L.z, 0.0f);
             if (kGuide && lp.guide_records) written += guide_collect(lp, pb, pid, ids.x, L, pb.ext[pid].w);
         }
@@ -870,8 +891,8 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
     float4* rd_next = pb.rd[cur ^ 1] + seg_base;
     const EnergyTables& T = *sc.tables;
     const int S = lp.shadow_samples;
-    float4* sq_o = pb.sq_o + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
-    float4* sq_d = pb.sq_d + static_cast<size_t>(sh.seg) * pb.seg_cap * S;
+    float4* sq_o = pb.sq_o + static_cast<size_t>(sh.seg) * pb.seg_cap * lp.shadow_slots;
+    float4* sq_d = pb.sq_d + static_cast<size_t>(sh.seg) * pb.seg_cap * lp.shadow_slots;
     const uint32_t stride = sh.workers * blockDim.x;
     // Each block takes 256 queue slots per round and reorders them in LDS by a sort key: in scenes that mix
     // shading code paths, the hit material's path (GpuMaterial::sort_class: opaque Disney, Disney with lobes
@@ -1002,12 +1023,21 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                 }
                 lp4 = kStage ? stage[6][src] : pb.last_p[pid];
                 ln4 = kStage ? stage[7][src] : pb.last_n[pid];
+                // The first vertex's contact AO (path_tracer.cpp:617-621) is known now that its rays are traced:
+                // it scales that vertex's held-back NEE and the path's throughput.
+                float ao = 1.0f;
+                if (kTransport && (ext.w & kExtAo)) {
+                    ao = ao_factor(lp, pb, pid, S);
+                    thr = thr * ao;
+                    ext.w &= ~kExtAo;
+                }
                 // Held-back NEE of the previous vertex enters the radiance before anything this vertex adds.
                 for (int s = 0; s < S; ++s) {
                     const size_t k = static_cast<size_t>(pid) * S + s;
                     const float4 p = kStage && kStagePend && s == 0 ? stage[8][src] : pb.pend[k];
-                    L += mk(p.x, p.y, p.z);
-                    if (kPasses) credit_nee(pb, lp, pid, ids.x, k, mk(p.x, p.y, p.z));
+                    const V3 c = kTransport ? mk(p.x, p.y, p.z) * ao : mk(p.x, p.y, p.z);
+                    L += c;
+                    if (kPasses) credit_nee(pb, lp, pid, ids.x, k, c);
                     pb.pend[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 }
             }
@@ -1212,6 +1242,24 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                         lambda = sample_wavelength(T, sobol_get1d(ss), w);
                         thr *= w;
                         ext.x = __float_as_uint(lambda);
+                    }
+                    // Contact AO at the first vertex (approxAo, path_tracer.cpp:27-44, :617-621): S cosine rays of
+                    // 0.35 about the shading normal go out as AO entries of the shadow queue; their factor is applied
+                    // when the path's next vertex (or k_regen) folds this vertex's NEE.
+                    if (kTransport && lp.ao_strength > 0.0f && depth == 0) {
+                        const Frame af = frame_from_normal(si.normal);
+                        for (int a = 0; a < S; ++a) {
+                            float a0, a1;
+                            sobol_get2d(ss, a0, a1);
+                            const V3 ad = normalized(to_world(af, cosine_sample_hemisphere(a0, a1)));
+                            const V3 ao_o = spawn_origin(si, ad);
+                            const uint32_t rec = lp.ao_base + pid * static_cast<uint32_t>(S) + static_cast<uint32_t>(a);
+                            pb.pend[rec] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+                            const uint32_t ao_slot = wave_append(shadow_count, true);
+                            sq_o[ao_slot] = make_float4(ao_o.x, ao_o.y, ao_o.z, 0.35f);
+                            sq_d[ao_slot] = make_float4(ad.x, ad.y, ad.z, __uint_as_float(kAoTag | rec));
+                        }
+                        ext.w |= kExtAo;
                     }
                     const bool is_delta = (m.flags & kMatFlagDelta) != 0;
                     // Guided: the slot's mixture was inferred (vertices below depth guided_bounces).
@@ -2281,10 +2329,13 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         const int spl_cap = adaptive ? std::min(8, static_cast<int>(std::max<uint64_t>(1, max_window / pixels)))
                                      : static_cast<int>(std::max<uint64_t>(1, max_window / pixels));
         const uint64_t window = uint64_t{pixels} * std::min(spl_cap, sample_count);
-        const uint64_t pool_limit = std::min<uint64_t>(kMaxPoolIds, 0xFFFFFFFFull / static_cast<uint64_t>(S) - kQueueSegs * 256u);
+        // Shadow-queue entries per id: the NEE samples, and as many AO rays at the first vertex with contact AO
+        // (their records follow the NEE records in pend). Record indices stay below the AO tag bit.
+        const int Sq = settings->ao_strength > 0.0f ? 2 * S : S;
+        const uint64_t pool_limit = std::min<uint64_t>(kMaxPoolIds, 0x7FFFFFFFull / static_cast<uint64_t>(Sq) - kQueueSegs * 256u);
         const uint64_t pool_req = settings->paths_in_flight > 0 ? static_cast<uint64_t>(settings->paths_in_flight) : kDefaultPoolIds;
         const uint32_t pool = static_cast<uint32_t>(std::min({pool_req, window, pool_limit}));
-        int rc = wf->ensure(pool, S);
+        int rc = wf->ensure(pool, Sq);
         if (rc == PE_OK) rc = wf->ensure_results(window, want_aov);
         if (rc == PE_OK && want_passes) rc = wf->ensure_passes(window, pass_count);
         if (rc != PE_OK) return rc;
@@ -2304,12 +2355,15 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
         lp.max_depth = settings->max_bounces;
         lp.rr_depth = settings->rr_depth;
         lp.shadow_samples = S;
+        lp.shadow_slots = Sq;
+        lp.ao_base = wf->capacity * static_cast<uint32_t>(S);
         lp.clamp_indirect = settings->clamp_indirect > 0.0f ? settings->clamp_indirect : 0.0f;
         lp.seed = settings->seed;
         lp.filter_width = std::clamp(settings->filter_width, 0.25f, 8.0f);
         lp.filter_radius = 0.5f * lp.filter_width;
         lp.filter_box = settings->filter_type == PE_FILTER_BOX ? 1 : 0;
-        lp.transport = scene->view.transport;
+        lp.ao_strength = settings->ao_strength > 0.0f ? settings->ao_strength : 0.0f;
+        lp.transport = scene->view.transport || lp.ao_strength > 0.0f;  // AO rides the ext column
         if (!lp.filter_box) {
             if (wf->filter_type != settings->filter_type || wf->filter_width != settings->filter_width) {
                 const std::vector<float> cdf = filter_cdf(settings->filter_type, settings->filter_width);
@@ -2398,15 +2452,15 @@ int render_samples(pe_device* dev, const RenderLane& lane, pe_scene* scene, cons
             wf->prof.begin(kKShadow, s);
             if (view.has_transmitters) {
                 if (count)
-                    hipLaunchKernelGGL(k_shadow_transmit<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S,
+                    hipLaunchKernelGGL(k_shadow_transmit<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, Sq,
                                        lp.clamp_indirect, spill);
                 else
-                    hipLaunchKernelGGL(k_shadow_transmit<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S,
+                    hipLaunchKernelGGL(k_shadow_transmit<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, Sq,
                                        lp.clamp_indirect, spill);
             } else if (count) {
-                hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S, spill);
+                hipLaunchKernelGGL(k_shadow<true>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, Sq, spill);
             } else {
-                hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, S, spill);
+                hipLaunchKernelGGL(k_shadow<false>, dim3(gblocks), dim3(kTraceBlock), 0, s, view, pb, Sq, spill);
             }
             wf->prof.end(s);
             hipLaunchKernelGGL(k_rotate, dim3(1), dim3(kQueueSegs), 0, s, pb, lp.num_paths);

@@ -10,8 +10,8 @@
 // scene (hasScene() false, lastError() says why) for shapes other than TriangleMesh / Triangle / static
 // MeshInstance, visibility masks other than all rays or all but camera rays, shadow catchers, lights other
 // than AreaLight / TriangleLight, light linking, rounded edges, textured Disney materials and
-// materials other than Disney / Lambertian / Dielectric; render() returns completed = false for AO. Pinhole,
-// thin-lens and orthographic cameras render, with motion blur of the camera (moving instances are refused).
+// materials other than Disney / Lambertian / Dielectric. Pinhole, thin-lens and orthographic cameras render, with
+// motion blur of the camera (moving instances are refused), and contact AO.
 // Path guiding uses the library's learned guide in place of the reference's SD-tree.
 #include "hip_device.h"
 
@@ -529,10 +529,6 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     if (!m_scene || !targets.beauty) return res;
     Film& film = *targets.beauty;
     const CameraGpuDesc cd = camera.describeGpu();
-    if (settings.aoStrength > 0.0f) {
-        m_error = "HipDevice::render: AO is not implemented";
-        return res;
-    }
     // Path guiding (renderer.cpp:413-429): a learned guide per scene, made afresh when a render starts at sample
     // 0 or the scene changed, trained during its first launches (the reference trains its SD-tree over the
     // first 32 samples, then freezes it), dropped when guiding is switched off.
@@ -591,6 +587,7 @@ SampleBatchResult HipDevice::render(const Camera& camera, const RenderSettings& 
     st.samples_per_launch = 0;                         // the library sizes launches for the device
     st.adaptive = settings.adaptiveSampling ? 1 : 0;   // renderer.cpp:438-505
     st.motion_blur = settings.motionBlur ? 1 : 0;      // a shutter time per camera ray (renderer.cpp:152-153)
+    st.ao_strength = settings.aoStrength;              // contact AO at the first vertex (renderer.cpp:432)
     st.noise_threshold = settings.noiseThreshold;
     st.min_adaptive_samples = settings.minAdaptiveSamples;
     const bool adaptive = settings.adaptiveSampling;

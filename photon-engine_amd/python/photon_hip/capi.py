@@ -128,7 +128,7 @@ class Settings(C.Structure):
                 ("filter_type", C.c_int32), ("filter_width", C.c_float), ("clamp_indirect", C.c_float),
                 ("seed", C.c_uint64), ("samples_per_launch", C.c_int32), ("count_visits", C.c_int32),
                 ("adaptive", C.c_int32), ("noise_threshold", C.c_float), ("min_adaptive_samples", C.c_int32),
-                ("paths_in_flight", C.c_int32), ("motion_blur", C.c_int32), ("reserved", C.c_int32)]
+                ("paths_in_flight", C.c_int32), ("motion_blur", C.c_int32), ("ao_strength", C.c_float)]
 
 
 class BatchResult(C.Structure):

@@ -58,10 +58,11 @@ def camera_with_shutter_end(camera: capi.Camera, end: capi.Camera) -> capi.Camer
 
 def settings(max_bounces=8, rr_depth=3, shadow_quality=1, filter_type=capi.PE_FILTER_BOX, filter_width=1.0,
              seed=12345, samples_per_launch=1, clamp_indirect=0.0, adaptive=False, noise_threshold=0.02,
-             min_adaptive_samples=16, paths_in_flight=0, motion_blur=False) -> capi.Settings:
+             min_adaptive_samples=16, paths_in_flight=0, motion_blur=False, ao_strength=0.0) -> capi.Settings:
     """RenderSettings as pe_settings (the reference's defaults for the adaptive fields, render_settings.h:30-32)."""
     s = capi.Settings()
     s.motion_blur = int(bool(motion_blur))
+    s.ao_strength = float(ao_strength)
     s.max_bounces, s.rr_depth, s.shadow_quality = max_bounces, rr_depth, shadow_quality
     s.filter_type, s.filter_width, s.clamp_indirect = filter_type, filter_width, clamp_indirect
     s.seed, s.samples_per_launch, s.paths_in_flight = seed, samples_per_launch, paths_in_flight

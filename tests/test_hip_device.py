@@ -74,14 +74,27 @@ def test_session_through_hip_device_matches_direct_render(dev, name):
 
 
 @pytest.mark.gpu
-def test_hip_device_refuses_what_it_does_not_implement(dev):
-    """Settings the HIP path does not implement (here ambient occlusion) make the batch fail with a message
-    instead of an approximation; the session reports it."""
+def test_contact_ao_through_hip_device_matches_direct_render(dev):
+    """RenderSettings::aoStrength through HipDevice: the session's batches give the image of a direct pe_render
+    with the same contact-AO strength, and the AO darkens the image (by about 1% here: only the box's corners and
+    the sphere's contact lie within the rays' 0.35)."""
     _harness_or_skip()
-    sc, view = configs.box_icosphere(level=1, width=8, height=8)
-    rc, _, _, _, err = bindings.HipDeviceSession(sc).render(view, 1, ao_strength=0.5)
-    assert rc == 1
-    assert "not implemented" in err, err
+    sc, view = configs.box_icosphere(level=2, width=24, height=24)
+    spp = 8
+    rc, mean, counts, _, err = bindings.HipDeviceSession(sc).render(view, spp, max_batch=4, ao_strength=0.7)
+    assert rc == 0, err
+    assert (counts == spp).all()
+    gs = dev.upload(sc)
+    cam = camera_perspective(view.position, view.target, view.up, view.vfov, view.aspect)
+    imgs = []
+    for ao in (0.7, 0.0):
+        film = dev.film(view.width, view.height)
+        render(dev, gs, cam, settings(max_bounces=view.max_bounces, seed=12345, samples_per_launch=4, ao_strength=ao),
+               film, 0, spp)
+        s_, c_, _, _ = film.download()
+        imgs.append(s_ / c_[..., None])
+    np.testing.assert_allclose(mean, imgs[0], rtol=2e-5, atol=1e-6)
+    assert imgs[0].mean() < 0.995 * imgs[1].mean()  # the box's corners and the sphere's contact darken
 
 
 @pytest.mark.gpu
