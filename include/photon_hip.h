@@ -195,7 +195,21 @@ typedef struct pe_scene_desc {
      * coverage, 1 - luminance of its throughput). A zeroed descriptor has the environment background. */
     int32_t background_mode;
     float background_color[3];
+    /* Light linking (Light::setLightLink, lights/light.h:23-32; Scene::isLightLinked / isShadowLinked,
+     * scene.cpp:120-141, :237-256). Per light of `lights`: link_modes[2 i] the receiver mode and [2 i + 1] the
+     * shadow mode (PE_LINK_*); link_offsets[4 i .. 4 i + 3] = first index and count into link_uids of its
+     * receiver node uids, then of its shadow-blocker node uids. An Include / Exclude receiver mode lights only /
+     * all but the listed nodes' surfaces (NEE and emission found by a bounce from them); an Include / Exclude
+     * shadow mode lets only / all but the listed nodes block the light's shadow rays. link_modes == NULL: no
+     * links. */
+    const int32_t* link_modes;
+    const uint32_t* link_offsets;
+    const uint64_t* link_uids;
 } pe_scene_desc;
+
+#define PE_LINK_ALL 0
+#define PE_LINK_INCLUDE 1
+#define PE_LINK_EXCLUDE 2
 
 #define PE_BACKGROUND_ENVIRONMENT 0
 #define PE_BACKGROUND_COLOR 1

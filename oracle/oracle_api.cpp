@@ -64,6 +64,22 @@ Scene* buildScene(const pe_scene_desc& d) {
     }
     s->nodeUid.assign(static_cast<size_t>(d.num_meshes), 0);
     if (d.node_uids) s->nodeUid.assign(d.node_uids, d.node_uids + d.num_meshes);
+    const size_t nl = static_cast<size_t>(d.num_lights);
+    s->linkMode.assign(nl, PE_LINK_ALL);
+    s->shadowLinkMode.assign(nl, PE_LINK_ALL);
+    s->linkUids.assign(nl, {});
+    s->shadowLinkUids.assign(nl, {});
+    if (d.link_modes) {
+        for (size_t i = 0; i < nl; ++i) {
+            const uint32_t* o = d.link_offsets + 4 * i;
+            s->linkMode[i] = d.link_modes[2 * i];
+            s->shadowLinkMode[i] = d.link_modes[2 * i + 1];
+            s->linkUids[i].assign(d.link_uids + o[0], d.link_uids + o[0] + o[1]);
+            s->shadowLinkUids[i].assign(d.link_uids + o[2], d.link_uids + o[2] + o[3]);
+            s->hasLinks = s->hasLinks || s->linkMode[i] != PE_LINK_ALL;
+            s->hasShadowLinks = s->hasShadowLinks || s->shadowLinkMode[i] != PE_LINK_ALL;
+        }
+    }
     s->bgMode = d.background_mode;
     s->bgColor = V3(d.background_color[0], d.background_color[1], d.background_color[2]);
     buildBvh(*s);

@@ -1159,8 +1159,33 @@ bool passesCutout(const pe_material_desc& m, Sobol& sampler) {
 
 // shadowTransmittance (path_tracer.cpp:172-202), without shadow linking: thin-walled surfaces and cutouts pass
 // the shadow ray on with their transmittance, anything else blocks it.
-V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow, RayCounters& rc) {
-    if (!transmitters) return intersectAny(scene, shadow) ? V3(0.0f) : V3(1.0f);
+// Scene::nodeUidOf (scene.h) of a hit, and the link tests isLightLinked / isShadowLinked (scene.cpp:237-256).
+uint64_t nodeUidOf(const Scene& scene, const SI& si) {
+    const int shape = si.tri >= 0 ? scene.tris[static_cast<size_t>(si.tri)].shape : -1;
+    return shape >= 0 ? scene.nodeUid[static_cast<size_t>(shape)] : 0;
+}
+
+bool linkedBy(int mode, const std::vector<uint64_t>& uids, uint64_t uid) {
+    if (mode == PE_LINK_ALL) return true;
+    const bool found = std::find(uids.begin(), uids.end(), uid) != uids.end();
+    return mode == PE_LINK_INCLUDE ? found : !found;
+}
+
+bool isLightLinked(const Scene& s, int li, uint64_t uid) {
+    if (!s.hasLinks || li < 0 || static_cast<size_t>(li) >= s.linkMode.size()) return true;
+    return linkedBy(s.linkMode[static_cast<size_t>(li)], s.linkUids[static_cast<size_t>(li)], uid);
+}
+
+bool hasShadowLinkFor(const Scene& s, int li) {
+    if (!s.hasShadowLinks || li < 0 || static_cast<size_t>(li) >= s.shadowLinkMode.size()) return false;
+    return s.shadowLinkMode[static_cast<size_t>(li)] != PE_LINK_ALL;
+}
+
+// shadowTransmittance (path_tracer.cpp:172-202) for the shadow ray of light `lightIndex` (-1: the environment):
+// blockers the light's shadow links leave out are passed as if absent.
+V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow, RayCounters& rc, int lightIndex = -1) {
+    const bool needShadowLink = hasShadowLinkFor(scene, lightIndex);
+    if (!transmitters && !needShadowLink) return intersectAny(scene, shadow) ? V3(0.0f) : V3(1.0f);
     V3 T(1.0f);
     V3 origin = shadow.o;
     const V3 dir = shadow.d;
@@ -1176,6 +1201,14 @@ V3 shadowTransmittance(const Scene& scene, bool transmitters, const Ray& shadow,
         if (!intersect(scene, r, h)) return T;
         SI hit;
         fillHit(scene, r, h, hit);
+        if (needShadowLink &&
+            !linkedBy(scene.shadowLinkMode[static_cast<size_t>(lightIndex)],
+                      scene.shadowLinkUids[static_cast<size_t>(lightIndex)], nodeUidOf(scene, hit))) {
+            origin = offsetRayOrigin(hit.point, hit.pError, lengthSq(hit.ng) > 0.0f ? hit.ng : hit.normal, dir);
+            remaining -= hit.t;
+            if (remaining <= 0.0f) return T;
+            continue;
+        }
         const pe_material_desc& m = scene.materials[static_cast<size_t>(hit.material)];
         if (!materialTransmitsShadows(m)) return V3(0.0f);
         ShadingPoint sp{hit.normal, hit.tangent, hit.frontFace, hit.ng, 0.0f, hit.point};
@@ -1231,6 +1264,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
     bool specularBounce = true;
     float lastBsdfPdf = 0.0f;
     V3 lastPoint = ray.o;
+    uint64_t prevReceiverUid = 0;  // :331
     MediumStack media;
     int mediumEvents = 0;  // :51, :463
     float lambda = 0.0f;
@@ -1387,7 +1421,8 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                 break;
             }
             const V3 emitted = materialEmitted(materialOf(isect.material));
-            if (!isBlack(emitted)) {  // :565-593
+            // :565-593; light linking: a bounce from a receiver the light is not linked to does not see it
+            if (!isBlack(emitted) && (depth == 0 || isect.emitter < 0 || isLightLinked(scene, isect.emitter, prevReceiverUid))) {
                 const int li = isect.emitter;
                 float weight = 1.0f;
                 if (!specularBounce) {
@@ -1412,6 +1447,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
         }
         if (st.aoStrength > 0.0f && depth == 0)  // contact AO (:617-621)
             throughput *= approxAo(scene, isect, sampler, st.aoStrength, shadowSamples);
+        const uint64_t receiverUid = nodeUidOf(scene, isect);  // :623
         const bool isSpecular = materialIsDelta(mat);
         const ShadingPoint sp{isect.normal, isect.tangent, isect.frontFace, isect.ng, lambda, isect.point};
         if (lightCount > 0 && !isSpecular) {  // NEE, :631-698
@@ -1425,6 +1461,10 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                     continue;
                 }
                 const bool fromEnv = lightIndex >= explicitLights;
+                if (!fromEnv && !isLightLinked(scene, lightIndex, receiverUid)) {  // :641-644
+                    sampler.get2D(ux, uy);
+                    continue;
+                }
                 sampler.get2D(ux, uy);
                 const LightSample ls = fromEnv ? it.env.sampleLi(ux, uy)
                                                : lightSampleLi(it.lights[static_cast<size_t>(lightIndex)], isect.point, ux, uy);
@@ -1432,7 +1472,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
                 Ray shadow;
                 if (!makeShadowRay(isect, ls, shadow)) continue;
                 ++rc.shadow;
-                const V3 vis = shadowTransmittance(scene, transmitters, shadow, rc);
+                const V3 vis = shadowTransmittance(scene, transmitters, shadow, rc, fromEnv ? -1 : lightIndex);
                 if (!isBlack(vis)) {
                     float bsdfPdf = 0.0f;
                     const V3 f = evalPdf(mat, wo, ls.wi, sp, bsdfPdf);
@@ -1515,6 +1555,7 @@ V3 Li(const Scene& scene, const Integrator& it, const Settings& st, const Ray& r
         if (!isFinite(throughput)) break;
         lastBsdfPdf = bs.pdf;
         lastPoint = isect.point;
+        prevReceiverUid = receiverUid;
         specularBounce = bs.delta;
         if (!bs.delta) ++diffuseBounces;
         cameraRay = cameraRay && bs.delta && dot(wi, isect.normal) < 0.0f;

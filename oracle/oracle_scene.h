@@ -40,6 +40,10 @@ struct Scene {
     int envLightGroup = -1;
     int numLightGroups = 0;
     std::vector<uint64_t> nodeUid;  // per shape (mesh of the descriptor)
+    // Light links (scene.cpp:120-141): per light the receiver / shadow modes (PE_LINK_*) and node uids.
+    bool hasLinks = false, hasShadowLinks = false;
+    std::vector<int> linkMode, shadowLinkMode;
+    std::vector<std::vector<uint64_t>> linkUids, shadowLinkUids;
     // Scene::background() (scene.h:24-28): PE_BACKGROUND_* and the Color mode's colour.
     int bgMode = PE_BACKGROUND_ENVIRONMENT;
     V3 bgColor{0.18f, 0.18f, 0.18f};

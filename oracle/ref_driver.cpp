@@ -203,6 +203,15 @@ void* buildRef(const pe_scene_desc* desc, const char* const* lightGroupNames, co
             auto area = std::make_shared<AreaLight>(v3(L.p0), v3(L.p1), v3(L.p2), c3(L.radiance));
             area->setVisibleToCamera(L.visible_to_camera != 0);
             if (lightGroupNames && lightGroupNames[li]) area->setLightGroup(lightGroupNames[li]);
+            if (desc->link_modes) {  // Light::setLightLink (light.h:81)
+                LightLinkSet ls;
+                const uint32_t* o = desc->link_offsets + 4 * li;
+                ls.mode = static_cast<LightLinkSet::Mode>(desc->link_modes[2 * li]);
+                ls.shadowMode = static_cast<LightLinkSet::Mode>(desc->link_modes[2 * li + 1]);
+                ls.nodeUids.assign(desc->link_uids + o[0], desc->link_uids + o[0] + o[1]);
+                ls.shadowNodeUids.assign(desc->link_uids + o[2], desc->link_uids + o[2] + o[3]);
+                area->setLightLink(ls);
+            }
             rs->scene->addLight(area);
             continue;
         }

@@ -223,6 +223,8 @@ static void rebind_view(pe_scene& sc) {
     v.light_nodes = sc.light_nodes.as<GpuLightNode>();
     v.light_paths = sc.light_paths.as<LightPath>();
     v.node_uids = sc.node_uids.as<uint64_t>();
+    v.link_info = sc.link_info.as<uint32_t>();
+    v.link_uids = sc.link_uids.as<uint64_t>();
     v.env.rgb = sc.env_rgb.as<float>();
     v.env.func = sc.env_func.as<float>();
     v.env.marginal = sc.env_marginal.as<float>();

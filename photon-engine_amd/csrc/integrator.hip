@@ -475,6 +475,20 @@ struct ShadowJob {
     }
 };
 
+// Scene::isLightLinked / isShadowLinked (scene.cpp:237-256): `shadow` picks the light's shadow set.
+__device__ __forceinline__ bool light_linked(const SceneView& sc, int li, uint64_t uid, bool shadow) {
+    const uint32_t* w = sc.link_info + 6 * li;
+    const uint32_t mode = shadow ? w[1] : w[0];
+    if (mode == PE_LINK_ALL) return true;
+    const uint32_t first = shadow ? w[4] : w[2], count = shadow ? w[5] : w[3];
+    bool found = false;
+    for (uint32_t k = 0; k < count; ++k) found = found || sc.link_uids[first + k] == uid;
+    return mode == PE_LINK_INCLUDE ? found : !found;
+}
+__device__ __forceinline__ uint64_t receiver_uid(const SceneView& sc, int32_t tri) {
+    return tri >= 0 ? sc.node_uids[sc.shade[tri].shape] : 0ull;
+}
+
 // approxAo's factor (path_tracer.cpp:42-43) from a path's AO records (1 each unoccluded ray, 0 each blocked one).
 __device__ __forceinline__ float ao_factor(const LaunchParams& lp, const PathBuffers& pb, uint32_t pid, int S) {
     float occ = 0.0f;
@@ -600,7 +614,24 @@ struct TransmitShadowJob {
             Surface si;
             fill_hit(sc, tri, r.d, u, v, si);
             const GpuMaterial& m = sc.materials[si.material];
-            const uint32_t queries = (bits >> 1) + 1u;
+            const uint32_t queries = ((bits >> 1) & 0x7Fu) + 1u;
+            // A blocker the light's shadow links leave out is passed as if absent (path_tracer.cpp:187-192); the
+            // record carries the light index + 1 above bit 8 (0: the environment, which has no links).
+            const int li = static_cast<int>(bits >> 8) - 1;
+            if (sc.has_shadow_links && li >= 0 && !light_linked(sc, li, receiver_uid(sc, tri), true)) {
+                const float remaining = r.tmax - t;
+                if (remaining > 0.0f && queries < kMaxShadowQueries) {
+                    const V3 o = offset_ray_origin(si.point, si.p_error, length_sq(si.ng) > 0.0f ? si.ng : si.normal, r.d);
+                    a = make_float4(o.x, o.y, o.z, remaining);
+                    b = make_float4(r.d.x, r.d.y, r.d.z, __uint_as_float(r.tag));
+                    pend[r.tag] = make_float4(c.x, c.y, c.z, __uint_as_float((bits & ~0xFEu) | (queries << 1)));
+                    return true;
+                }
+                if (remaining > 0.0f) c = splat(0.0f);
+                if (bits & kPendClamp) c = clamp_indirect(c, clamp_limit);
+                pend[r.tag] = make_float4(c.x, c.y, c.z, 0.0f);
+                return false;
+            }
             // Thin walls pass the shadow ray on with their transmittance, cutouts with 1 - opacity
             // (shadowTransmittance, dielectric.cpp / disney.cpp:511-513); anything else blocks it.
             if (!(m.flags & (kMatFlagThinWall | kMatFlagCutout))) {
@@ -620,7 +651,7 @@ struct TransmitShadowJob {
                 }
             }
             if (more) {
-                pend[r.tag] = make_float4(c.x, c.y, c.z, __uint_as_float((bits & kPendClamp) | (queries << 1)));
+                pend[r.tag] = make_float4(c.x, c.y, c.z, __uint_as_float((bits & ~0xFEu) | (queries << 1)));
                 return true;
             }
         }
@@ -1204,7 +1235,11 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                     }
                     if (alive && !scattered) {
                         const GpuMaterial& m = sc.materials[si.material];
-                        if (m.flags & kMatFlagEmissive) {  // path_tracer.cpp:565-593
+                        // path_tracer.cpp:565-593; with light links a bounce from a receiver the light is not linked
+                        // to does not see it (:571-572); the previous receiver's shape is in last_p.w.
+                        if ((m.flags & kMatFlagEmissive) &&
+                            !(kTransport && sc.has_links && depth > 0 && si.emitter >= 0 && si.emitter < sc.num_lights &&
+                              !light_linked(sc, si.emitter, sc.node_uids[__float_as_uint(lp4.w)], false))) {
                             const int li = si.emitter;
                             float weight = 1.0f;
                             if (!specular) {
@@ -1291,6 +1326,9 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             do {
                                 if (li < 0 || !(pmf > 0.0f)) break;
                                 const bool from_env = li >= sc.num_lights;
+                                if (kTransport && sc.has_links && !from_env &&
+                                    !light_linked(sc, li, receiver_uid(sc, tri), false))
+                                    break;  // an unlinked receiver: the sample is drawn, the light skipped (:641-644)
                                 const LightSample ls = from_env ? env_sample(sc.env, ux, uy)
                                                                 : light_sample(sc.lights[li], si.point, ux, uy);
                                 if (!(ls.pdf > 0.0f) || is_black(ls.li)) break;
@@ -1316,6 +1354,8 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                                     if (kTransport && sc.has_transmitters) pend_bits = kPendClamp;
                                     else c = clamp_indirect(c, lp.clamp_indirect);
                                 }
+                                if (kTransport && sc.has_shadow_links && !from_env)
+                                    pend_bits |= static_cast<uint32_t>(li + 1) << 8;  // for the shadow links
                                 if (!finite3(c)) break;
                                 pb.pend[static_cast<size_t>(pid) * S + s] =
                                     make_float4(c.x, c.y, c.z, __uint_as_float(pend_bits));
@@ -1423,7 +1463,9 @@ __device__ __forceinline__ void shade_body(SceneView sc, LaunchParams lp, PathBu
                             alive = false;
                         } else {
                             last_pdf = bs.pdf;
-                            lp4 = make_float4(si.point.x, si.point.y, si.point.z, 0.0f);
+                            // last_p.w: the receiver's shape, the prevReceiverUid of light linking (:832)
+                            lp4 = make_float4(si.point.x, si.point.y, si.point.z,
+                                              __uint_as_float(tri >= 0 ? static_cast<uint32_t>(sc.shade[tri].shape) : 0u));
                             ln4 = make_float4(si.normal.x, si.normal.y, si.normal.z, 0.0f);
                             state_moved = true;
                             specular = bs.delta;

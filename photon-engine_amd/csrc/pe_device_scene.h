@@ -177,6 +177,11 @@ struct SceneView {
     int32_t bg_mode;           // PE_BACKGROUND_*: what a camera path leaving before its first vertex sees
     float3 bg_color;           // the Color background
     const uint64_t* node_uids; // per shape: Scene::nodeUidOf (object-ID pass)
+    // Light links (scene.cpp:120-141): per light six words (receiver mode, shadow mode, receiver uids first /
+    // count, shadow uids first / count) into link_uids; has_links / has_shadow_links: any mode other than All.
+    const uint32_t* link_info;
+    const uint64_t* link_uids;
+    int32_t has_links, has_shadow_links;
     // Light BVH (kPickBvh): tree over the bounded lights, unbounded candidates on a top level with the tree.
     const GpuLightNode* light_nodes;
     const LightPath* light_paths;  // light_count entries

@@ -89,7 +89,7 @@ struct pe_scene {
     int bvh_depth = 0;
     pe::DevBuf nodes, tris, tri_shade, positions, normals, uvs;
     pe::DevBuf materials, lights, tables, env_rgb, env_func, env_marginal, env_cond, env_mguide, env_cguide,
-        light_nodes, light_paths, node_uids;
+        light_nodes, light_paths, node_uids, link_info, link_uids;
     pe::SceneView view{};
     uint64_t num_nodes = 0;
     int max_material_ref = -1;  // largest material index a mesh uses
@@ -98,13 +98,14 @@ struct pe_scene {
     // The device buffers, in a fixed order (parking and restoring walk them).
     std::vector<pe::DevBuf*> buffers() {
         return {&nodes, &tris, &tri_shade, &positions, &normals, &uvs, &materials, &lights, &tables, &env_rgb,
-                &env_func, &env_marginal, &env_cond, &env_mguide, &env_cguide, &light_nodes, &light_paths, &node_uids};
+                &env_func, &env_marginal, &env_cond, &env_mguide, &env_cguide, &light_nodes, &light_paths, &node_uids, &link_info, &link_uids};
     }
     uint64_t device_bytes() const {
         return nodes.bytes() + tris.bytes() + tri_shade.bytes() + positions.bytes() + normals.bytes() + uvs.bytes() +
                materials.bytes() + lights.bytes() + tables.bytes() + env_rgb.bytes() + env_func.bytes() +
                env_marginal.bytes() + env_cond.bytes() + env_mguide.bytes() +
-               env_cguide.bytes() + light_nodes.bytes() + light_paths.bytes() + node_uids.bytes();
+               env_cguide.bytes() + light_nodes.bytes() + light_paths.bytes() + node_uids.bytes() +
+               link_info.bytes() + link_uids.bytes();
     }
 };
 

@@ -15,6 +15,7 @@ PKG_DIR = Path(__file__).resolve().parents[2]
 LIB_PATH = Path(os.environ.get("PHOTON_HIP_LIB", PKG_DIR / "libphoton_hip.so"))
 
 PE_OK = 0
+PE_LINK_ALL, PE_LINK_INCLUDE, PE_LINK_EXCLUDE = 0, 1, 2  # light-link modes (pe_scene_desc.link_modes)
 PE_ERR_INVALID = -1
 PE_ERR_NO_DEVICE = -2
 PE_ERR_HIP = -3
@@ -88,7 +89,8 @@ class SceneDesc(C.Structure):
                 ("environment", C.POINTER(EnvDesc)), ("light_groups", C.POINTER(C.c_int32)),
                 ("env_light_group", C.c_int32), ("num_light_groups", C.c_int32),
                 ("node_uids", C.POINTER(C.c_uint64)), ("background_mode", C.c_int32),
-                ("background_color", C.c_float * 3)]
+                ("background_color", C.c_float * 3), ("link_modes", C.POINTER(C.c_int32)),
+                ("link_offsets", C.POINTER(C.c_uint32)), ("link_uids", C.POINTER(C.c_uint64))]
 
 
 # Background::Mode (engine/scene.h:24-28)

@@ -116,6 +116,11 @@ class AreaLight:
     radiance: tuple
     visible_to_camera: bool = True
     light_group: str = ""      # Light::setLightGroup (light.h:85): the EXR group LG_<name>; "" = none
+    # Light::setLightLink (light.h:23-32, :81): receiver and shadow-blocker node uids with capi.PE_LINK_* modes
+    link_mode: int = 0
+    link_uids: tuple = ()
+    shadow_mode: int = 0
+    shadow_uids: tuple = ()
 
 
 @dataclass
@@ -183,6 +188,7 @@ class Scene:
     lights: list = field(default_factory=list)     # LightDesc in Scene::lights() order
     environment: Environment | None = None
     light_group_of: list = field(default_factory=list)  # per light: its group name ("" = none)
+    link_of: dict = field(default_factory=dict)  # light index -> (mode, uids, shadow mode, shadow uids)
     material_names: dict = field(default_factory=dict)  # material index -> Scene::setMaterialName name
     # Scene::setBackground (engine/scene.h:24-28): capi.PE_BACKGROUND_* and the colour of the Color mode
     background_mode: int = 0
@@ -222,6 +228,9 @@ class Scene:
         L.visible_to_camera = 1 if light.visible_to_camera else 0
         self.lights.append(L)
         self.light_group_of.append(light.light_group)
+        if light.link_mode or light.shadow_mode:
+            self.link_of[index] = (int(light.link_mode), tuple(light.link_uids), int(light.shadow_mode),
+                                   tuple(light.shadow_uids))
         # The quad inserted into the BVH as two black emissive Lambert triangles (scene.cpp:77-98).
         mat = self.add_material(lambert((0.0, 0.0, 0.0), light.radiance))
         p0 = np.asarray(light.position, dtype=np.float32)
@@ -349,6 +358,21 @@ class Scene:
         desc.node_uids = capi.ptr(uids, C.c_uint64)
         desc.background_mode = int(self.background_mode)
         desc.background_color[:] = [float(x) for x in self.background_color]
+        if self.link_of:
+            modes, offs, luids = [], [], []
+            for i in range(len(self.lights)):
+                mode, rec, smode, sh = self.link_of.get(i, (0, (), 0, ()))
+                modes += [mode, smode]
+                offs += [len(luids), len(rec)]
+                luids += list(rec)
+                offs += [len(luids), len(sh)]
+                luids += list(sh)
+            arrs = (np.ascontiguousarray(modes + [0], np.int32), np.ascontiguousarray(offs + [0], np.uint32),
+                    np.ascontiguousarray(luids + [0], np.uint64))
+            keep.extend(arrs)
+            desc.link_modes = capi.ptr(arrs[0], C.c_int32)
+            desc.link_offsets = capi.ptr(arrs[1], C.c_uint32)
+            desc.link_uids = capi.ptr(arrs[2], C.c_uint64)
         if self.environment is not None:
             e = self.environment
             env = capi.EnvDesc()
